@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures from the reference's own compiled host code.
+
+Runs only where oracle/_ref/libzjref.so exists (built by ``make -C oracle``
+from the reference sources in place).  Every number written here comes out of
+the reference's code (FROperators, GaussLegendre, CorrectionFunction, Mesh,
+GmshReader, ElementGeometry, IdealGas, RiemannSolvers); inputs are the
+reference's test meshes and seeded synthetic ones.  Usage:
+
+    python tests/golden/make_golden.py [--ref-root /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import pyref  # noqa: E402
+from zhijian_amd import cases  # noqa: E402
+
+
+def gen_operators():
+    out = {}
+    for p in range(1, 6):
+        base = pyref.quad_operators(p, 0)
+        for k, v in base.items():
+            out[f"P{p}_{k}"] = v
+        # FluxType is ignored by the reference's operator build: record that
+        # DG/SD/HU/GA tables coincide bit for bit.
+        same = all(all(np.array_equal(pyref.quad_operators(p, ft)[k], base[k]) for k in base)
+                   for ft in (1, 2, 3))
+        out[f"P{p}_fluxtype_invariant"] = np.array(int(same))
+    for n in range(1, 9):
+        pts, wts = pyref.gauss(n)
+        out[f"gauss{n}_pts"] = pts
+        out[f"gauss{n}_wts"] = wts
+    xs = np.concatenate([[-1.0, 1.0], np.linspace(-0.95, 0.95, 9)])
+    out["corr_x"] = xs
+    for p in range(1, 6):
+        out[f"corr_P{p}_left"] = np.array([pyref.corr_deriv(0, p, x, False) for x in xs])
+        out[f"corr_P{p}_right"] = np.array([pyref.corr_deriv(0, p, x, True) for x in xs])
+    return out
+
+
+def synthetic_meshes():
+    m = {}
+    v = cases.vortex_mesh(10)
+    m["vortex10"] = (v.xy, v.elem_nodes)
+    u = cases.uniform_quad_mesh(4, 4, 0.0, 1.0, 0.0, 1.0)
+    m["unit4"] = (u.xy, u.elem_nodes)
+    s = cases.uniform_quad_mesh(3, 5, 0.0, 3.0, 0.0, 2.0)
+    xy = s.xy.copy()
+    xy[:, 0] = 3.0 * (xy[:, 0] / 3.0) ** 1.5       # stretched in x
+    xy[:, 1] = xy[:, 1] + 0.1 * np.sin(xy[:, 0])    # sheared rows
+    m["stretched3x5"] = (xy, s.elem_nodes)
+    j = cases.uniform_quad_mesh(6, 6, 0.0, 1.0, 0.0, 1.0, jitter=0.2, seed=7)
+    m["jitter6"] = (j.xy, j.elem_nodes)
+    # triangles: each quad split along (i,j)-(i+1,j+1)
+    t = cases.uniform_quad_mesh(5, 5, 0.0, 1.0, 0.0, 1.0, jitter=0.15, seed=3)
+    en = t.elem_nodes
+    tri = np.concatenate([en[:, [0, 1, 2]], en[:, [0, 2, 3]]], axis=0)
+    m["tri5"] = (t.xy, tri)
+    return m
+
+
+def gen_meshes(ref_root):
+    out = {}
+    names = []
+    for path in sorted(glob.glob(os.path.join(ref_root, "test_meshes", "*.msh"))):
+        name = os.path.splitext(os.path.basename(path))[0]
+        d = pyref.mesh_read_gmsh(path, p_geom=None)
+        all_quads = bool(np.all(d["elem_info"][:, 0] == 1))
+        if all_quads and np.all(np.isin(d["elem_info"][:, 2], (4, 9, 25))):
+            d = pyref.mesh_read_gmsh(path, p_geom=3)
+        names.append(name)
+        for k, v in d.items():
+            out[f"{name}__{k}"] = v
+    for name, (xy, en) in synthetic_meshes().items():
+        d = pyref.mesh_from_arrays(xy, en, p_geom=3 if en.shape[1] == 4 else None)
+        names.append(name)
+        for k, v in d.items():
+            out[f"{name}__{k}"] = v
+    out["mesh_names"] = np.array(names)
+    return out
+
+
+def gen_physics():
+    rng = np.random.default_rng(20261202)
+    n = 96
+    gamma = 1.4
+    rho = np.exp(rng.normal(0.0, 0.5, (n, 2)))
+    u = rng.normal(0.0, 1.0, (n, 2))
+    v = rng.normal(0.0, 1.0, (n, 2))
+    p = np.exp(rng.normal(0.0, 0.7, (n, 2)))
+    # a few hard cases: supersonic, near vacuum, strong jump
+    u[0] = [5.0, 4.0]; u[1] = [-5.0, -4.5]; rho[2] = [1e-6, 1.0]; p[3] = [1e-6, 2.0]
+    rho[4] = [10.0, 0.1]; p[4] = [100.0, 0.01]
+    UL = cases.prim_to_cons(rho[:, 0], u[:, 0], v[:, 0], p[:, 0], gamma)
+    UR = cases.prim_to_cons(rho[:, 1], u[:, 1], v[:, 1], p[:, 1], gamma)
+    th = rng.uniform(0, 2 * np.pi, n)
+    nrm = np.stack([np.cos(th), np.sin(th)], axis=1)
+    out = dict(UL=UL, UR=UR, normals=nrm, gamma=np.array(gamma))
+    out["fx"] = np.array([pyref.flux_xy(UL[i], gamma)[0] for i in range(n)])
+    out["fy"] = np.array([pyref.flux_xy(UL[i], gamma)[1] for i in range(n)])
+    out["ws"] = np.array([pyref.max_wave_speed(UL[i], gamma) for i in range(n)])
+    for w, name in enumerate(("rusanov", "roe", "hllc")):
+        out[name] = np.array([pyref.riemann(w, UL[i], UR[i], nrm[i, 0], nrm[i, 1], gamma)
+                              for i in range(n)])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref-root", default="/root/reference")
+    a = ap.parse_args()
+    if not pyref.available():
+        sys.exit("oracle/_ref/libzjref.so is missing; run `make -C oracle` with the reference present")
+    np.savez_compressed(os.path.join(HERE, "operators.npz"), **gen_operators())
+    np.savez_compressed(os.path.join(HERE, "meshes.npz"), **gen_meshes(a.ref_root))
+    np.savez_compressed(os.path.join(HERE, "physics.npz"), **gen_physics())
+    for f in ("operators.npz", "meshes.npz", "physics.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+
+
+if __name__ == "__main__":
+    main()
