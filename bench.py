@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Residual DOF-updates/s of the FR residual + SSP-RK3 step on one MI355X.
+
+Workload (BASELINE config "2D Taylor-Green vortex"): periodic [0,2pi]^2,
+256x256 uniform quads, P4 (25 solution points/element, 1,638,400 points,
+6,553,600 DOF), Mach 0.1, the reference's executed arithmetic (DG correction,
+inline Rusanov, inviscid), SSP-RK3 with the reference's positivity limiter, dt
+from the CFL rule at CFL 0.5.  Synthetic initial state (closed-form TGV).
+
+One "step" = one SSP-RK3 step = three fused stage kernels, i.e. three residual
+DOF-updates per DOF.  Timed with HIP events on the stream the kernels run on,
+inputs resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "residual DOF-updates/s, FP32-MFMA vs FP64 plain at stated L2 err; adjoint/fwd cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_stage(E, p, w_in, w_g, w_t, r, faces_per_elem=2):
+    """Algorithmic HBM bytes of one stage: state in (4 w_in), FP64 state out
+    (32), r*U^n reads (32 r), geometry (4 Jinv words; the path never reads J),
+    traces written once and read once (4*Fp*4*w_t*2), and per face a normal
+    plus 16 B of connectivity."""
+    S = (p + 1) ** 2
+    Fp = p + 1
+    return E * (S * (4 * w_in + 32 + 32 * r + 4 * w_g) + 4 * Fp * 4 * w_t * 2
+                + faces_per_elem * (2 * w_g + 16))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_precision(sol, U0, dt, steps, warmup, stream, hiprt):
+    """Graph-replayed steps and per-stage kernel times (ms)."""
+    sol.set_state(U0)
+    for _ in range(warmup):
+        sol.rk3_step(dt, stream.ptr)
+    stream.synchronize()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    hiprt.device_synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        sol.rk3_step(dt, stream.ptr)
+    e1.record(stream)
+    e1.synchronize()
+    hiprt.device_synchronize()
+    ms_step = e0.elapsed_ms(e1) / steps
+    # per-stage kernel durations: events between individually launched stages
+    nst = max(steps // 2, 5)
+    evs = [[hiprt.Event() for _ in range(4)] for _ in range(nst)]
+    for k in range(nst):
+        evs[k][0].record(stream)
+        for st in (1, 2, 3):
+            sol.rk3_stage(st, dt, stream.ptr)
+            evs[k][st].record(stream)
+    evs[-1][3].synchronize()
+    hiprt.device_synchronize()
+    st_ms = [float(np.mean([evs[k][st - 1].elapsed_ms(evs[k][st]) for k in range(nst)])) for st in (1, 2, 3)]
+    return ms_step, st_ms
+
+
+def cpu_baseline(mesh, p, U0, dt, budget_s):
+    """The oracle (a plain-C restatement of the reference's executed kernels)
+    on the same workload, OpenMP over elements and faces on this host."""
+    from oracle import pyoracle as O
+    from tests import fr_case
+    cores = len(os.sched_getaffinity(0))
+    case = fr_case.build(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+    o = case.solver()
+    O.set_threads(cores)
+    U = U0.copy()
+    t0 = time.perf_counter()
+    U = o.advance(U, dt, True)       # warm-up step
+    t_first = time.perf_counter() - t0
+    nsteps = max(1, min(200, int(budget_s / max(t_first, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        U = o.advance(U, dt, True)
+    el = time.perf_counter() - t0
+    O.set_threads(1)
+    dof = case.E * case.S * 4
+    return dict(value=3.0 * dof * nsteps / el, unit="DOF-updates/s", cores=cores, kind="port",
+                sample=f"{nsteps} SSP-RK3 steps of the full 256x256 P4 TGV mesh "
+                       f"({el:.1f} s, after 1 warm-up step), OpenMP over elements and faces",
+                rk_steps_per_s=nsteps / el)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256, help="elements per side")
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--err-steps", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU owning the whole mesh; --gpus {args.gpus} "
+              f"(WORLD_SIZE={world}) is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+
+    from zhijian_amd import cases, hiprt, solver
+
+    mesh = cases.tgv_mesh(args.n)
+    t_build = time.perf_counter()
+    tables = solver.build_tables(mesh.xy, mesh.elem_nodes, args.p, periodic=(mesh.Lx, mesh.Ly))
+    log(f"tables: {tables.E} elements, {tables.faces.shape[0]} faces, {time.perf_counter() - t_build:.2f} s")
+    U0 = cases.tgv_ic(tables.pos, mach=0.1).reshape(-1)
+    E, S = tables.E, tables.S
+    dof = E * S * 4
+
+    s64 = solver.Solver(tables, precision="fp64", cfl=0.5)
+    s32 = solver.Solver(tables, precision="fp32bulk", cfl=0.5)
+    s64.set_state(U0)
+    dt = s64.dt_cfl(0.5)   # A21: params.dt = 0 -> the CFL step of the initial state
+    stream = hiprt.Stream()
+
+    res = {}
+    for name, sol in (("fp64", s64), ("fp32bulk", s32)):
+        ms_step, st_ms = time_precision(sol, U0, dt, args.steps, args.warmup, stream, hiprt)
+        w = 8 if name == "fp64" else 4
+        by = [alg_bytes_per_stage(E, args.p, 8, w, w, r) for r in (0.0, 1.0, 1.0)]
+        achieved = sum(by) / (sum(st_ms) * 1e-3) / 1e9
+        res[name] = dict(ms_per_step=ms_step, stage_ms=st_ms,
+                         dof_updates_per_s=3.0 * dof / (ms_step * 1e-3),
+                         rk_steps_per_s=1e3 / ms_step,
+                         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                                       frac=achieved / HBM_PEAK_GBS, traffic=None,
+                                       kernel=f"stage_kernel<{'double' if w == 8 else 'float'},P{args.p}>",
+                                       bytes_per_launch=sum(by) / 3.0,
+                                       avg_launch_ms=sum(st_ms) / 3.0))
+        log(f"{name}: {ms_step:.4f} ms/step, stages {['%.4f' % x for x in st_ms]} ms, "
+            f"{res[name]['dof_updates_per_s']:.3e} DOF-upd/s, {achieved:.0f} GB/s")
+
+    # FP32-bulk error against FP64 after a fixed number of steps from the same state
+    err = None
+    if args.err_steps > 0:
+        s64.set_state(U0); s32.set_state(U0)
+        for _ in range(args.err_steps):
+            s64.rk3_step(dt); s32.rk3_step(dt)
+        a, b = s64.get_state(), s32.get_state()
+        err = dict(steps=args.err_steps, rel_l2=float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                   rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
+        log(f"fp32bulk vs fp64 after {args.err_steps} steps: {err}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, args.p, U0, dt, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    head = res["fp32bulk"]
+    out = {
+        "metric": METRIC,
+        "value": head["dof_updates_per_s"],
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "state_dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D Taylor-Green vortex, periodic [0,2pi]^2, {args.n}x{args.n} quads, "
+                               f"P{args.p}, Mach 0.1, reference arithmetic (DG correction, inline Rusanov, "
+                               f"inviscid), SSP-RK3 + positivity limiter, CFL 0.5",
+                   "elements": E, "solution_points": E * S, "dof": dof, "dt": dt,
+                   "precision": "fp32bulk (FP32 flux/interp/correction, FP64 RK state)"},
+        "roofline": head["roofline"],
+        "cpu_baseline": cpu,
+        "rk_steps_per_s": head["rk_steps_per_s"],
+        "fp64_plain": res["fp64"],
+        "fp32bulk_over_fp64": head["dof_updates_per_s"] / res["fp64"]["dof_updates_per_s"],
+        "fp32bulk_error_vs_fp64": err,
+        "adjoint_over_forward": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,164 @@
+/*
+ * zhijian_amd.h -- C ABI of the MI355X flux-reconstruction residual and SSP-RK3
+ * time step.
+ *
+ * This is the boundary under the C++ FRSolver / SSPRK3 API (include/zhijian/):
+ * plain pointers and sizes, int status codes (ZJ_OK = 0), never exit().  One
+ * handle owns one mesh's device state on one GPU.  Host arrays are copied at
+ * the call; the caller keeps ownership.  Streams are hipStream_t passed as
+ * void* (NULL = the handle's own stream).  Calls on one handle are not
+ * thread-safe.
+ *
+ * Reference interfaces replaced (Zhijian, file:line):
+ *   zj_create        FRSolver::initialize/allocateGPUMemory/initGeometry
+ *                    (src/solver/fr_solver.cpp:30-266) -- the device upload set
+ *   zj_set_state     FRSolver::copyToDevice  (fr_solver.cpp:854-871)
+ *   zj_get_state     FRSolver::copyToHost    (fr_solver.cpp:835-852)
+ *   zj_residual      FRSolver::computeResidual_GPU / computeInviscidFlux_GPU
+ *                    (fr_solver.cpp:335-608): kernels.cu K4,K1,K9,K8,K11,K13
+ *   zj_rk3_step      FRSolver::advance (fr_solver.cpp:305-333) with
+ *                    gpu::rkStage1/2/3 + enforcePositivity (kernels.cu:1236-1349)
+ *   zj_rk3_stage     SSPRK3::stage{1,2,3}_GPU (include/time/rk_integrator.hpp:33-58)
+ *   zj_dt_cfl        FRSolver::computeTimeStep (fr_solver.cpp:784-816)
+ *   zj_residual_l2   FRSolver::computeResidual (fr_solver.cpp:818-833)
+ *   zj_check_finite  the checkNaN scans (fr_solver.cpp:11-22), as a status
+ *   zj_quad_operators  FROperators::initQuad (src/basis/basis.cpp:464-613)
+ *   zj_build_faces     Mesh::buildFaces (src/mesh/mesh.cpp:18-83)
+ *   zj_face_normals    Mesh::faceNormal (mesh.cpp:159-183)
+ *   zj_elem_areas      Mesh::elementArea (mesh.cpp:122-146)
+ *   zj_geometry        FRSolver::initGeometry (fr_solver.cpp:85-129)
+ *   zj_pair_periodic   no counterpart (the reference has no periodic faces)
+ *
+ * Array layouts at this boundary are the reference's (include/gpu/kernels.hpp):
+ *   U, dUdt  [E][S][4] AoS (rho, rhou, rhov, rhoE), S = (p+1)^2
+ *   Jinv     [E][S][4] (dxi/dx, dxi/dy, deta/dx, deta/dy);  J [E][S]
+ *   diff_xi, diff_eta [S][S];  interp [4][p+1][S];  corr [4][S]
+ *   normals  [F][2];  face_* [F];  bc_data [n_bc][8]
+ */
+#ifndef ZHIJIAN_AMD_H
+#define ZHIJIAN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZJ_OK 0
+#define ZJ_ERR_ARG 1          /* invalid argument or inconsistent tables */
+#define ZJ_ERR_HIP 2          /* HIP runtime error (message in zj_last_error) */
+#define ZJ_ERR_NOMEM 3
+#define ZJ_ERR_UNSUPPORTED 4  /* valid input this build does not handle */
+#define ZJ_ERR_NONFINITE 5    /* state or residual holds NaN/Inf */
+#define ZJ_ERR_STATE 6        /* call out of order (e.g. no state set) */
+
+#define ZJ_PREC_FP64 0        /* plain: FP64 end to end */
+#define ZJ_PREC_FP32BULK 1    /* FP32 interpolation/flux/correction, FP64 RK state */
+
+#define ZJ_COMPAT_REFERENCE 0 /* the reference's executed arithmetic (quirks kept) */
+
+/* BCType values (include/common/types.hpp:47-56) */
+#define ZJ_BC_INTERIOR 0
+#define ZJ_BC_WALL 1
+#define ZJ_BC_SLIPWALL 2
+#define ZJ_BC_SYMMETRY 3
+#define ZJ_BC_FARFIELD 4
+#define ZJ_BC_INFLOW 5
+#define ZJ_BC_OUTFLOW 6
+#define ZJ_BC_PERIODIC 7
+
+typedef struct zj_solver* zj_handle;
+
+/* What FRSolver uploads: operators flattened as fr_solver.cpp:164-221, geometry
+ * as fr_solver.cpp:85-129, connectivity and normals as fr_solver.cpp:232-265,
+ * BC tables as fr_solver.cpp:432-494 (bc_type/bc_face_map per face, 8 doubles
+ * of bc_data per boundary entry). */
+typedef struct {
+    int poly_order;              /* 1..5 */
+    int n_elem;
+    int n_faces;
+    const double* diff_xi;       /* [S*S] */
+    const double* diff_eta;      /* [S*S] */
+    const double* interp;        /* [4*(p+1)*S] */
+    const double* corr;          /* [4*S] */
+    const double* J;             /* [E*S] or NULL (the reference residual never reads it) */
+    const double* Jinv;          /* [E*S*4] */
+    const double* normals;       /* [F*2] */
+    const int* face_left_elem;   /* [F] */
+    const int* face_left_local;  /* [F] */
+    const int* face_right_elem;  /* [F], -1 on boundary faces */
+    const int* face_right_local; /* [F] */
+    const int* bc_type;          /* [F] or NULL: no boundary faces expected */
+    const int* bc_face_map;      /* [F] or NULL */
+    const double* bc_data;       /* [n_bc_data*8] or NULL */
+    int n_bc_data;
+    const double* elem_area;     /* [E] or NULL (then the CFL dt is unavailable) */
+} zj_mesh_desc;
+
+typedef struct {
+    double gamma;                /* SimParams::gamma */
+    double cfl;                  /* SimParams::CFL, used by the fused CFL dt */
+    int precision;               /* ZJ_PREC_* */
+    int compat;                  /* ZJ_COMPAT_REFERENCE */
+    int positivity;              /* 1: the reference's limiter after each stage */
+    int device;                  /* HIP device ordinal */
+} zj_params;
+
+/* ---- solver --------------------------------------------------------- */
+int zj_create(zj_handle* out, const zj_mesh_desc* mesh, const zj_params* params);
+int zj_destroy(zj_handle h);
+/* Last error text for h, or the last zj_create failure when h is NULL. */
+const char* zj_last_error(zj_handle h);
+int zj_version(void);
+
+int zj_set_state(zj_handle h, const double* U_aos);          /* host [E][S][4] */
+int zj_get_state(zj_handle h, double* U_aos);                /* host, synchronises */
+int zj_set_state_device(zj_handle h, const double* d_U_aos, void* stream);
+int zj_get_state_device(zj_handle h, double* d_U_aos, void* stream);
+
+/* dUdt of the current state into dUdt_aos (host pointer; NULL keeps it on the
+ * device for zj_residual_l2).  Synchronises when dUdt_aos is not NULL. */
+int zj_residual(zj_handle h, double* dUdt_aos, void* stream);
+/* One SSP-RK3 step of size dt: three fused stage kernels (hipGraph replay). */
+int zj_rk3_step(zj_handle h, double dt, void* stream);
+/* One stage (1, 2 or 3) of the step, for callers that time or interleave them.
+ * Stage 1 must come first; the state reads as U^n again only after stage 3. */
+int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream);
+/* CFL time step of the current state (synchronises). */
+int zj_dt_cfl(zj_handle h, double cfl, double* dt);
+/* L2 norm of the last residual: that of zj_residual, or of stage 3 of the last
+ * step (synchronises). */
+int zj_residual_l2(zj_handle h, double* l2);
+/* Number of non-finite values in the current state (synchronises). */
+int zj_check_finite(zj_handle h, int64_t* n_bad);
+/* Launch configuration of the stage kernel (elements per workgroup). */
+int zj_set_tile(zj_handle h, int elems_per_block);
+int zj_get_info(zj_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp);
+
+/* ---- host mesh and operator tables (native, used by FRSolver) -------- */
+int zj_quad_operators(int poly_order, double* sol_xi, double* sol_eta, double* quad_w,
+                      double* diff_xi, double* diff_eta, double* interp, double* corr);
+/* Faces of n_elem elements with nv corners each (row stride `stride` in
+ * elem_nodes).  Arrays hold n_elem*nv entries; *n_faces receives the count. */
+int zj_build_faces(int n_elem, int nv, int stride, const int64_t* elem_nodes, int* n_faces,
+                   int* face_le, int* face_ll, int* face_re, int* face_rl);
+/* Pairs boundary faces one period apart (Lx or Ly; 0 disables a direction): the
+ * lower-indexed face is kept in place with the partner as its right side, the
+ * partner is removed, the order stays stable.  Updates *n_faces. */
+int zj_pair_periodic(int* n_faces, int* face_le, int* face_ll, int* face_re, int* face_rl,
+                     int nv, int stride, const int64_t* elem_nodes, const double* xy,
+                     double Lx, double Ly, double tol);
+int zj_face_normals(int n_faces, const int* face_le, const int* face_ll, int nv, int stride,
+                    const int64_t* elem_nodes, const double* xy, double* normals);
+int zj_elem_areas(int n_elem, int nv, int stride, const int64_t* elem_nodes,
+                  const double* xy, double* area);
+/* Quads with 4, 9 or 25 nodes (Gmsh order).  J [E*S], Jinv [E*S*4]; pos
+ * [E*S*2] may be NULL. */
+int zj_geometry(int n_elem, int node_count, int stride, const int64_t* elem_nodes,
+                const double* xy, int S, const double* sol_xi, const double* sol_eta,
+                double* J, double* Jinv, double* pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

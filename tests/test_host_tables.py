@@ -1,0 +1,136 @@
+"""The product's native host tables against the reference (golden) and the oracle,
+and the C ABI's exports and argument checks (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests.conftest import ROOT, mesh_from_golden
+from zhijian_amd import _lib, cases, solver
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "zhijian_amd.h")).read()
+    declared = set(re.findall(r"\b(zj_[a-z0-9_]+)\s*\(", hdr))
+    assert declared == set(_lib.EXPORTS)
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.zj_version() >= 1
+
+
+@pytest.mark.parametrize("p", range(1, 6))
+def test_operators_bit_exact(golden_ops, p):
+    ops = _lib.quad_operators(p)
+    for k in ("sol_xi", "sol_eta", "qw", "diff_xi", "diff_eta", "interp", "corr"):
+        assert np.array_equal(ops[k], golden_ops[f"P{p}_{k}"]), k
+
+
+def test_faces_normals_areas_geometry_bit_exact(golden_meshes):
+    for name in [str(n) for n in golden_meshes["mesh_names"]]:
+        m = mesh_from_golden(golden_meshes, name)
+        nv = 3 if int(m["elem_info"][0, 0]) == 0 else 4
+        f = _lib.build_faces(m["elem_nodes"], nv=nv)
+        assert np.array_equal(f, m["faces"]), name
+        assert np.array_equal(_lib.face_normals(f, m["elem_nodes"], m["xy"], nv=nv), m["normals"]), name
+        assert np.array_equal(_lib.elem_areas(m["elem_nodes"], m["xy"], nv=nv), m["area"]), name
+        if "J" in m:
+            nc = int(m["elem_info"][0, 2])
+            J, Jinv = _lib.geometry(m["elem_nodes"], m["xy"], _lib.quad_operators(3), node_count=nc)
+            if nc == 4:
+                assert np.array_equal(J, m["J"]) and np.array_equal(Jinv, m["Jinv"]), name
+            else:
+                assert np.allclose(J, m["J"], rtol=1e-13, atol=0), name
+
+
+@pytest.mark.parametrize("nx,ny,jit", [(10, 10, 0.0), (7, 3, 0.2), (1, 1, 0.0), (1, 4, 0.0), (16, 16, 0.1)])
+def test_periodic_pairing_matches_oracle(nx, ny, jit):
+    mesh = cases.uniform_quad_mesh(nx, ny, -1.0, 2.0, 0.0, 1.5, jitter=jit, seed=nx)
+    f = _lib.build_faces(mesh.elem_nodes)
+    fp = _lib.pair_periodic(f, mesh.elem_nodes, mesh.xy, mesh.Lx, mesh.Ly)
+    of = O.pair_periodic(O.build_faces(mesh.elem_nodes), mesh.elem_nodes, mesh.xy, mesh.Lx, mesh.Ly,
+                         1e-9 * max(mesh.Lx, mesh.Ly))
+    assert np.array_equal(fp, of.as_array())
+    assert fp.shape[0] == 2 * nx * ny
+
+
+def test_tables_match_oracle_case():
+    from tests import fr_case
+    mesh = cases.uniform_quad_mesh(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.2, seed=4)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=(mesh.Lx, mesh.Ly))
+    oc = fr_case.from_quadmesh(mesh, 3)
+    assert np.array_equal(t.faces, oc.faces.as_array())
+    assert np.array_equal(t.normals, oc.normals)
+    assert np.array_equal(t.J, oc.J) and np.array_equal(t.Jinv, oc.Jinv)
+    assert np.array_equal(t.area, oc.area)
+    assert np.array_equal(t.bc_type, oc.bc_type) and np.array_equal(t.bc_face_map, oc.bc_face_map)
+
+
+def _desc_for(t, corrupt=None):
+    keep = []
+
+    def c(a, dt):
+        a = np.ascontiguousarray(a, dt).copy()
+        keep.append(a)
+        return a
+    ops = {k: c(v, np.float64) for k, v in t.ops.items()}
+    if corrupt:
+        corrupt(ops)
+    fc = c(t.faces, np.int32)
+    cols = [c(fc[:, k], np.int32) for k in range(4)]
+    d = _lib.MeshDesc()
+    d.poly_order = t.p; d.n_elem = t.E; d.n_faces = fc.shape[0]
+    d.diff_xi = ops["diff_xi"].ctypes.data; d.diff_eta = ops["diff_eta"].ctypes.data
+    d.interp = ops["interp"].ctypes.data; d.corr = ops["corr"].ctypes.data
+    d.Jinv = c(t.Jinv, np.float64).ctypes.data
+    d.normals = c(t.normals, np.float64).ctypes.data
+    d.face_left_elem, d.face_left_local, d.face_right_elem, d.face_right_local = (a.ctypes.data for a in cols)
+    return d, keep, cols
+
+
+def _create(d, prec=0, compat=0, gamma=1.4):
+    h = C.c_void_p()
+    prm = _lib.Params(gamma, 0.5, prec, compat, 1, 0)
+    rc = _lib.lib().zj_create(C.byref(h), C.byref(d), C.byref(prm))
+    return rc, _lib.lib().zj_last_error(None).decode()
+
+
+def test_create_rejects_bad_input_before_touching_the_gpu():
+    mesh = cases.uniform_quad_mesh(3, 3, 0.0, 1.0, 0.0, 1.0)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 2, periodic=(1.0, 1.0))
+    # dense operators that are not Gauss tensor products
+    def bad_diff(ops):
+        ops["diff_xi"][1] += 1e-3
+    d, keep, _ = _desc_for(t, bad_diff)
+    rc, msg = _create(d)
+    assert rc == 4 and "tensor" in msg
+    def bad_interp(ops):
+        S = 9
+        I = ops["interp"].reshape(4, 3, S)
+        I[2] = I[2][::-1].copy()  # edge 2 in forward order: not the reference's layout
+        ops["interp"][:] = I.reshape(-1)
+    d, keep, _ = _desc_for(t, bad_interp)
+    rc, msg = _create(d)
+    assert rc == 4 and "reversed" in msg
+    # an element edge used twice
+    d, keep, cols = _desc_for(t)
+    cols[1][1] = cols[1][0]; cols[0][1] = cols[0][0]
+    rc, msg = _create(d)
+    assert rc == 1 and "two faces" in msg
+    # unsupported arithmetic mode, bad order, bad gamma
+    d, keep, _ = _desc_for(t)
+    assert _create(d, compat=1)[0] == 4
+    assert _create(d, gamma=1.0)[0] == 1
+    d.poly_order = 7
+    assert _create(d)[0] == 1
+    # far-field faces without bc_data
+    open_t = solver.build_tables(mesh.xy, mesh.elem_nodes, 2, periodic=None)
+    d, keep, _ = _desc_for(open_t)
+    bct = np.full(open_t.faces.shape[0], 4, np.int32); keep.append(bct)
+    bmap = np.arange(open_t.faces.shape[0], dtype=np.int32); keep.append(bmap)
+    d.bc_type = bct.ctypes.data; d.bc_face_map = bmap.ctypes.data
+    rc, msg = _create(d)
+    assert rc == 1 and "bc_data" in msg

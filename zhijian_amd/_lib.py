@@ -1,0 +1,166 @@
+"""ctypes binding of libzhijian_amd.so (include/zhijian_amd.h).
+
+The library is built in-tree by ``python -m zhijian_amd.build``.  There is no
+fallback: every entry point raises if the library is missing or a call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzhijian_amd.so")
+
+ZJ_OK = 0
+ZJ_PREC_FP64 = 0
+ZJ_PREC_FP32BULK = 1
+ZJ_COMPAT_REFERENCE = 0
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class ZJError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"zhijian_amd error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("poly_order", C.c_int), ("n_elem", C.c_int), ("n_faces", C.c_int),
+                ("diff_xi", C.c_void_p), ("diff_eta", C.c_void_p), ("interp", C.c_void_p),
+                ("corr", C.c_void_p), ("J", C.c_void_p), ("Jinv", C.c_void_p),
+                ("normals", C.c_void_p), ("face_left_elem", C.c_void_p),
+                ("face_left_local", C.c_void_p), ("face_right_elem", C.c_void_p),
+                ("face_right_local", C.c_void_p), ("bc_type", C.c_void_p),
+                ("bc_face_map", C.c_void_p), ("bc_data", C.c_void_p), ("n_bc_data", C.c_int),
+                ("elem_area", C.c_void_p)]
+
+
+class Params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("precision", C.c_int),
+                ("compat", C.c_int), ("positivity", C.c_int), ("device", C.c_int)]
+
+
+_lib = None
+
+# every exported symbol of include/zhijian_amd.h
+EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_state", "zj_get_state",
+           "zj_set_state_device", "zj_get_state_device", "zj_residual", "zj_rk3_step",
+           "zj_rk3_stage", "zj_dt_cfl", "zj_residual_l2", "zj_check_finite", "zj_set_tile",
+           "zj_get_info", "zj_quad_operators", "zj_build_faces", "zj_pair_periodic",
+           "zj_face_normals", "zj_elem_areas", "zj_geometry")
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m zhijian_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        H = C.c_void_p
+        L.zj_create.argtypes = [C.POINTER(H), C.POINTER(MeshDesc), C.POINTER(Params)]
+        L.zj_destroy.argtypes = [H]
+        L.zj_last_error.argtypes = [H]
+        L.zj_last_error.restype = C.c_char_p
+        L.zj_set_state.argtypes = [H, _dp]
+        L.zj_get_state.argtypes = [H, _dp]
+        L.zj_set_state_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_get_state_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_residual.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_rk3_step.argtypes = [H, C.c_double, C.c_void_p]
+        L.zj_rk3_stage.argtypes = [H, C.c_int, C.c_double, C.c_void_p]
+        L.zj_dt_cfl.argtypes = [H, C.c_double, C.POINTER(C.c_double)]
+        L.zj_residual_l2.argtypes = [H, C.POINTER(C.c_double)]
+        L.zj_check_finite.argtypes = [H, C.POINTER(C.c_int64)]
+        L.zj_set_tile.argtypes = [H, C.c_int]
+        L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
+        L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]
+        L.zj_pair_periodic.argtypes = [C.POINTER(C.c_int), _ip, _ip, _ip, _ip, C.c_int, C.c_int, _lp, _dp,
+                                       C.c_double, C.c_double, C.c_double]
+        L.zj_face_normals.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_int, _lp, _dp, _dp]
+        L.zj_elem_areas.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, _dp]
+        L.zj_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def check(rc, h=None):
+    if rc != ZJ_OK:
+        msg = lib().zj_last_error(h)
+        raise ZJError(rc, msg.decode() if msg else "")
+    return rc
+
+
+# ----------------------------------------------------------- host tables
+def quad_operators(p: int) -> dict:
+    n = p + 1; S = n * n
+    out = dict(sol_xi=np.zeros(S), sol_eta=np.zeros(S), qw=np.zeros(S),
+               diff_xi=np.zeros(S * S), diff_eta=np.zeros(S * S), interp=np.zeros(4 * n * S),
+               corr=np.zeros(4 * S))
+    check(lib().zj_quad_operators(p, out["sol_xi"], out["sol_eta"], out["qw"], out["diff_xi"],
+                                  out["diff_eta"], out["interp"], out["corr"]))
+    return out
+
+
+def build_faces(elem_nodes: np.ndarray, nv: int | None = None):
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    E, stride = en.shape
+    nv = stride if nv is None else nv
+    cap = max(E * nv, 1)
+    arrs = [np.zeros(cap, np.int32) for _ in range(4)]
+    nf = C.c_int(0)
+    check(lib().zj_build_faces(E, nv, stride, en.reshape(-1), C.byref(nf), *arrs))
+    return np.stack([a[: nf.value] for a in arrs], axis=1)
+
+
+def pair_periodic(faces: np.ndarray, elem_nodes, xy, Lx, Ly, tol=None, nv=None):
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    stride = en.shape[1]
+    nv = stride if nv is None else nv
+    arrs = [np.ascontiguousarray(faces[:, k], np.int32).copy() for k in range(4)]
+    nf = C.c_int(faces.shape[0])
+    tol = 1e-9 * max(Lx, Ly) if tol is None else tol
+    check(lib().zj_pair_periodic(C.byref(nf), *arrs, nv, stride, en.reshape(-1),
+                                 np.ascontiguousarray(xy, np.float64).reshape(-1), Lx, Ly, tol))
+    return np.stack([a[: nf.value] for a in arrs], axis=1)
+
+
+def face_normals(faces, elem_nodes, xy, nv=None):
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    stride = en.shape[1]
+    nv = stride if nv is None else nv
+    F = faces.shape[0]
+    out = np.zeros(max(2 * F, 1))
+    check(lib().zj_face_normals(F, np.ascontiguousarray(faces[:, 0], np.int32),
+                                np.ascontiguousarray(faces[:, 1], np.int32), nv, stride,
+                                en.reshape(-1), np.ascontiguousarray(xy, np.float64).reshape(-1), out))
+    return out[: 2 * F].reshape(F, 2)
+
+
+def elem_areas(elem_nodes, xy, nv=None):
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    E, stride = en.shape
+    nv = stride if nv is None else nv
+    out = np.zeros(max(E, 1))
+    check(lib().zj_elem_areas(E, nv, stride, en.reshape(-1), np.ascontiguousarray(xy, np.float64).reshape(-1), out))
+    return out[:E]
+
+
+def geometry(elem_nodes, xy, ops, node_count=None, want_pos=False):
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    E, stride = en.shape
+    nc = stride if node_count is None else node_count
+    S = ops["sol_xi"].shape[0]
+    J = np.zeros(max(E * S, 1)); Jinv = np.zeros(max(E * S * 4, 1))
+    pos = np.zeros(max(E * S * 2, 1)) if want_pos else None
+    check(lib().zj_geometry(E, nc, stride, en.reshape(-1), np.ascontiguousarray(xy, np.float64).reshape(-1),
+                            S, ops["sol_xi"], ops["sol_eta"], J, Jinv,
+                            None if pos is None else pos.ctypes.data))
+    out = (J[: E * S], Jinv[: E * S * 4])
+    if want_pos:
+        out = out + (pos[: E * S * 2].reshape(E, S, 2),)
+    return out

@@ -1,0 +1,372 @@
+// zj_host.cpp -- host-side tables the device path consumes: FR operators,
+// faces (with periodic pairing), normals, areas and Jacobians.
+//
+// These run once per mesh.  They follow the reference's definitions so that
+// the tables agree with what the reference uploads:
+//   operators   src/basis/basis.cpp:12-104 (Gauss), 169-224 (Lagrange),
+//               314-379 (Radau correction derivative), 464-613 (quad tables)
+//   faces       src/mesh/mesh.cpp:18-83 (std::map order of sorted vertex pairs)
+//   normals     mesh.cpp:159-183;  areas mesh.cpp:122-146
+//   geometry    src/solver/fr_solver.cpp:85-129, basis.cpp:717-863
+#include "zhijian_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+// 1D Gauss-Legendre rule with the reference's closed forms (n <= 5), its
+// 15-digit table (n = 6) and Newton iteration beyond.
+void gauss_rule(int n, double* x, double* w) {
+    switch (n) {
+    case 1: x[0] = 0.0; w[0] = 2.0; return;
+    case 2: {
+        double a = 1.0 / std::sqrt(3.0);
+        x[0] = -a; x[1] = a; w[0] = w[1] = 1.0;
+        return;
+    }
+    case 3: {
+        double a = std::sqrt(3.0 / 5.0);
+        x[0] = -a; x[1] = 0.0; x[2] = a;
+        w[0] = w[2] = 5.0 / 9.0; w[1] = 8.0 / 9.0;
+        return;
+    }
+    case 4: {
+        double r = std::sqrt(6.0 / 5.0);
+        double a = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * r), b = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * r);
+        x[0] = -a; x[1] = -b; x[2] = b; x[3] = a;
+        double s = std::sqrt(30.0);
+        w[0] = w[3] = (18.0 - s) / 36.0; w[1] = w[2] = (18.0 + s) / 36.0;
+        return;
+    }
+    case 5: {
+        double r = std::sqrt(10.0 / 7.0);
+        double a = std::sqrt(5.0 + 2.0 * r) / 3.0, b = std::sqrt(5.0 - 2.0 * r) / 3.0;
+        x[0] = -a; x[1] = -b; x[2] = 0.0; x[3] = b; x[4] = a;
+        double s = std::sqrt(70.0);
+        w[0] = w[4] = (322.0 - 13.0 * s) / 900.0;
+        w[1] = w[3] = (322.0 + 13.0 * s) / 900.0;
+        w[2] = 128.0 / 225.0;
+        return;
+    }
+    case 6: {
+        static const double X[3] = {0.932469514203152, 0.661209386466265, 0.238619186083197};
+        static const double W[3] = {0.171324492379170, 0.360761573048139, 0.467913934572691};
+        for (int i = 0; i < 3; ++i) {
+            x[i] = -X[i]; x[5 - i] = X[i]; w[i] = w[5 - i] = W[i];
+        }
+        return;
+    }
+    default:
+        for (int i = 0; i < n; ++i) {
+            double t = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            auto leg = [n](double s, double& Pn, double& Pm) {
+                double a = 1.0, b = s;
+                for (int j = 2; j <= n; ++j) {
+                    double c = ((2 * j - 1) * s * b - (j - 1) * a) / j;
+                    a = b; b = c;
+                }
+                Pn = b; Pm = a;
+            };
+            for (int it = 0; it < 20; ++it) {
+                double Pn, Pm;
+                leg(t, Pn, Pm);
+                double d = n * (t * Pn - Pm) / (t * t - 1.0);
+                double dx = Pn / d;
+                t -= dx;
+                if (std::fabs(dx) < 1e-15) break;
+            }
+            double Pn, Pm;
+            leg(t, Pn, Pm);
+            double d = n * (t * Pn - Pm) / (t * t - 1.0);
+            x[i] = t;
+            w[i] = 2.0 / ((1.0 - t * t) * d * d);
+        }
+    }
+}
+
+// Lagrange basis on `nodes` (product and sum-of-products forms).
+double lag(const double* nodes, int n, int j, double x) {
+    double r = 1.0;
+    for (int i = 0; i < n; ++i)
+        if (i != j) r *= (x - nodes[i]) / (nodes[j] - nodes[i]);
+    return r;
+}
+
+double lag_d(const double* nodes, int n, int j, double x) {
+    double r = 0.0;
+    for (int k = 0; k < n; ++k) {
+        if (k == j) continue;
+        double t = 1.0 / (nodes[j] - nodes[k]);
+        for (int i = 0; i < n; ++i)
+            if (i != j && i != k) t *= (x - nodes[i]) / (nodes[j] - nodes[i]);
+        r += t;
+    }
+    return r;
+}
+
+double legendre_p(int n, double x) {
+    if (n == 0) return 1.0;
+    if (n == 1) return x;
+    double a = 1.0, b = x;
+    for (int i = 2; i <= n; ++i) {
+        double c = ((2 * i - 1) * x * b - (i - 1) * a) / i;
+        a = b; b = c;
+    }
+    return b;
+}
+
+double legendre_dp(int n, double x) {
+    if (n == 0) return 0.0;
+    double Pn = legendre_p(n, x), Pm = legendre_p(n - 1, x);
+    if (std::fabs(x * x - 1.0) < 1e-14)  // the reference's endpoint branch (basis.cpp:336-339)
+        return n * (n + 1) / 2.0 * ((x > 0) ? 1.0 : ((n % 2 == 0) ? 1.0 : -1.0));
+    return n / (x * x - 1.0) * (x * Pn - Pm);
+}
+
+// Radau (DG) correction derivatives g_L', g_R' (Huynh 2007); the reference uses
+// these for every FluxType.
+double radau_left_d(int p, double x) {
+    double s = (p % 2 == 0) ? 1.0 : -1.0;
+    return s * 0.5 * (legendre_dp(p, x) - legendre_dp(p + 1, x));
+}
+double radau_right_d(int p, double x) { return 0.5 * (legendre_dp(p, x) + legendre_dp(p + 1, x)); }
+
+inline int64_t node_of(const int64_t* en, int stride, int e, int k) { return en[(int64_t)e * stride + k]; }
+
+}  // namespace
+
+extern "C" int zj_quad_operators(int p, double* sol_xi, double* sol_eta, double* qw,
+                                 double* diff_xi, double* diff_eta, double* interp, double* corr) {
+    if (p < 1 || p > 5) return ZJ_ERR_ARG;
+    const int n = p + 1, S = n * n;
+    double g[8], w[8];
+    gauss_rule(n, g, w);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            if (sol_xi) sol_xi[j * n + i] = g[i];
+            if (sol_eta) sol_eta[j * n + i] = g[j];
+            if (qw) qw[j * n + i] = w[i] * w[j];
+        }
+    double D[6][6];
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) D[i][k] = lag_d(g, n, k, g[i]);
+    if (diff_xi && diff_eta) {
+        std::fill(diff_xi, diff_xi + S * S, 0.0);
+        std::fill(diff_eta, diff_eta + S * S, 0.0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < n; ++k) {
+                    diff_xi[(j * n + i) * S + j * n + k] = D[i][k];
+                    diff_eta[(j * n + i) * S + k * n + i] = D[j][k];
+                }
+    }
+    if (interp) {
+        // edge 0: (s,-1), 1: (1,s), 2: (-s,1), 3: (-1,-s)
+        for (int e = 0; e < 4; ++e)
+            for (int f = 0; f < n; ++f) {
+                double s = g[f];
+                double xf = e == 0 ? s : e == 1 ? 1.0 : e == 2 ? -s : -1.0;
+                double ef = e == 0 ? -1.0 : e == 1 ? s : e == 2 ? 1.0 : -s;
+                double Lx[6], Le[6];
+                for (int i = 0; i < n; ++i) { Lx[i] = lag(g, n, i, xf); Le[i] = lag(g, n, i, ef); }
+                for (int j = 0; j < n; ++j)
+                    for (int i = 0; i < n; ++i) interp[(e * n + f) * S + j * n + i] = Lx[i] * Le[j];
+            }
+    }
+    if (corr) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int sp = j * n + i;
+                double li = lag(g, n, i, g[i]), lj = lag(g, n, j, g[j]);
+                corr[0 * S + sp] = radau_left_d(p, g[j]) * li;
+                corr[2 * S + sp] = radau_right_d(p, g[j]) * li;
+                corr[3 * S + sp] = radau_left_d(p, g[i]) * lj;
+                corr[1 * S + sp] = radau_right_d(p, g[i]) * lj;
+            }
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_build_faces(int n_elem, int nv, int stride, const int64_t* en, int* n_faces,
+                              int* le, int* ll, int* re, int* rl) {
+    if (n_elem < 0 || nv < 3 || stride < nv || !n_faces || (n_elem > 0 && !en)) return ZJ_ERR_ARG;
+    const int64_t ne = (int64_t)n_elem * nv;
+    struct Rec { int64_t lo, hi; int32_t slot; };  // slot = element*nv + local edge
+    std::vector<Rec> r((size_t)ne);
+    for (int e = 0; e < n_elem; ++e)
+        for (int k = 0; k < nv; ++k) {
+            int64_t a = node_of(en, stride, e, k), b = node_of(en, stride, e, (k + 1) % nv);
+            r[(size_t)e * nv + k] = {std::min(a, b), std::max(a, b), (int32_t)(e * nv + k)};
+        }
+    // std::map iteration order = ascending (lo, hi); ties keep insertion order.
+    std::stable_sort(r.begin(), r.end(), [](const Rec& x, const Rec& y) {
+        return x.lo != y.lo ? x.lo < y.lo : x.hi < y.hi;
+    });
+    int nf = 0;
+    for (size_t i = 0; i < r.size();) {
+        size_t j = i + 1;
+        while (j < r.size() && r[j].lo == r[i].lo && r[j].hi == r[i].hi) ++j;
+        if (j - i == 1) {
+            le[nf] = r[i].slot / nv; ll[nf] = r[i].slot % nv; re[nf] = -1; rl[nf] = -1; ++nf;
+        } else if (j - i == 2) {
+            le[nf] = r[i].slot / nv; ll[nf] = r[i].slot % nv;
+            re[nf] = r[i + 1].slot / nv; rl[nf] = r[i + 1].slot % nv; ++nf;
+        }  // edges shared by more than two elements are dropped, as in the reference
+        i = j;
+    }
+    *n_faces = nf;
+    return ZJ_OK;
+}
+
+extern "C" int zj_pair_periodic(int* n_faces, int* le, int* ll, int* re, int* rl, int nv,
+                                int stride, const int64_t* en, const double* xy, double Lx,
+                                double Ly, double tol) {
+    if (!n_faces || *n_faces < 0) return ZJ_ERR_ARG;
+    const int nf = *n_faces;
+    // boundary faces sorted by midpoint so each partner search is a binary search
+    struct B { double x, y; int f; };
+    std::vector<B> b;
+    for (int f = 0; f < nf; ++f) {
+        if (re[f] >= 0) continue;
+        int64_t v0 = node_of(en, stride, le[f], ll[f]), v1 = node_of(en, stride, le[f], (ll[f] + 1) % nv);
+        b.push_back({(xy[2 * v0] + xy[2 * v1]) * 0.5, (xy[2 * v0 + 1] + xy[2 * v1 + 1]) * 0.5, f});
+    }
+    std::vector<B> byx = b;
+    std::sort(byx.begin(), byx.end(), [](const B& p, const B& q) { return p.x < q.x; });
+    std::vector<char> dead(nf, 0), taken(nf, 0);
+    std::vector<int> partner(nf, -1);
+    auto candidates = [&](double x, double y, int self, std::vector<int>& out) {
+        out.clear();
+        auto lo = std::lower_bound(byx.begin(), byx.end(), x - tol, [](const B& p, double v) { return p.x < v; });
+        for (auto it = lo; it != byx.end() && it->x <= x + tol; ++it)
+            if (it->f != self && std::fabs(it->y - y) <= tol) out.push_back(it->f);
+    };
+    std::vector<int> cand, all;
+    for (const B& q : b) {  // ascending face order
+        int f = q.f;
+        if (taken[f]) continue;
+        all.clear();
+        const double shifts[4][2] = {{Lx, 0}, {-Lx, 0}, {0, Ly}, {0, -Ly}};
+        for (int s = 0; s < 4; ++s) {
+            if ((s < 2 && Lx <= 0) || (s >= 2 && Ly <= 0)) continue;
+            candidates(q.x + shifts[s][0], q.y + shifts[s][1], f, cand);
+            for (int g : cand)
+                if (g > f && !taken[g]) all.push_back(g);
+        }
+        std::sort(all.begin(), all.end());
+        all.erase(std::unique(all.begin(), all.end()), all.end());
+        if (all.size() > 1) return ZJ_ERR_ARG;  // ambiguous: not a periodic-conforming mesh
+        if (all.size() == 1) {
+            int g = all[0];
+            partner[f] = g; taken[f] = taken[g] = 1; dead[g] = 1;
+        }
+    }
+    int out = 0;
+    for (int f = 0; f < nf; ++f) {
+        if (dead[f]) continue;
+        int g = partner[f];
+        int nre = g >= 0 ? le[g] : re[f], nrl = g >= 0 ? ll[g] : rl[f];
+        le[out] = le[f]; ll[out] = ll[f]; re[out] = nre; rl[out] = nrl; ++out;
+    }
+    *n_faces = out;
+    return ZJ_OK;
+}
+
+extern "C" int zj_face_normals(int nf, const int* le, const int* ll, int nv, int stride,
+                               const int64_t* en, const double* xy, double* normals) {
+    for (int f = 0; f < nf; ++f) {
+        const int e = le[f];
+        int64_t v0 = node_of(en, stride, e, ll[f]), v1 = node_of(en, stride, e, (ll[f] + 1) % nv);
+        double tx = xy[2 * v1] - xy[2 * v0], ty = xy[2 * v1 + 1] - xy[2 * v0 + 1];
+        double len = std::sqrt(tx * tx + ty * ty);
+        double nx = ty / len, ny = -tx / len;
+        double cx = 0.0, cy = 0.0;
+        for (int k = 0; k < nv; ++k) {
+            int64_t v = node_of(en, stride, e, k);
+            cx = cx + xy[2 * v]; cy = cy + xy[2 * v + 1];
+        }
+        cx = cx / (double)nv; cy = cy / (double)nv;
+        double mx = (xy[2 * v0] + xy[2 * v1]) * 0.5, my = (xy[2 * v0 + 1] + xy[2 * v1 + 1]) * 0.5;
+        if (nx * (mx - cx) + ny * (my - cy) < 0) { nx = nx * (-1.0); ny = ny * (-1.0); }
+        normals[2 * f] = nx; normals[2 * f + 1] = ny;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_elem_areas(int n_elem, int nv, int stride, const int64_t* en, const double* xy,
+                             double* area) {
+    auto tri = [&](int64_t a, int64_t b, int64_t c) {
+        return 0.5 * std::fabs((xy[2 * b] - xy[2 * a]) * (xy[2 * c + 1] - xy[2 * a + 1]) -
+                               (xy[2 * c] - xy[2 * a]) * (xy[2 * b + 1] - xy[2 * a + 1]));
+    };
+    for (int e = 0; e < n_elem; ++e) {
+        int64_t v0 = node_of(en, stride, e, 0), v1 = node_of(en, stride, e, 1), v2 = node_of(en, stride, e, 2);
+        if (nv == 3) area[e] = tri(v0, v1, v2);
+        else area[e] = tri(v0, v1, v2) + tri(v0, v2, node_of(en, stride, e, 3));
+    }
+    return ZJ_OK;
+}
+
+namespace {
+// Quad shape functions in Gmsh node order: bilinear (4), biquadratic (9) and
+// biquartic (25) tensor-product Lagrange.
+const int Q9X[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, Q9Y[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+const int Q25X[25] = {0, 4, 4, 0, 1, 2, 3, 4, 4, 4, 3, 2, 1, 0, 0, 0, 1, 2, 3, 1, 2, 3, 1, 2, 3};
+const int Q25Y[25] = {0, 0, 4, 4, 0, 0, 0, 1, 2, 3, 4, 4, 4, 3, 2, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3};
+
+bool shape(int nc, double xi, double eta, double* N, double* Nx, double* Ne) {
+    if (nc == 4) {
+        const double a = 1 - xi, b = 1 + xi, c = 1 - eta, d = 1 + eta;
+        N[0] = 0.25 * a * c; N[1] = 0.25 * b * c; N[2] = 0.25 * b * d; N[3] = 0.25 * a * d;
+        Nx[0] = -0.25 * c; Nx[1] = 0.25 * c; Nx[2] = 0.25 * d; Nx[3] = -0.25 * d;
+        Ne[0] = -0.25 * a; Ne[1] = -0.25 * b; Ne[2] = 0.25 * b; Ne[3] = 0.25 * a;
+        return true;
+    }
+    int m = nc == 9 ? 2 : nc == 25 ? 4 : 0;
+    if (!m) return false;
+    double nodes[5], Lx[5], dLx[5], Le[5], dLe[5];
+    for (int i = 0; i <= m; ++i) nodes[i] = -1.0 + 2.0 * i / m;
+    for (int i = 0; i <= m; ++i) {
+        Lx[i] = lag(nodes, m + 1, i, xi); dLx[i] = lag_d(nodes, m + 1, i, xi);
+        Le[i] = lag(nodes, m + 1, i, eta); dLe[i] = lag_d(nodes, m + 1, i, eta);
+    }
+    const int* IX = m == 2 ? Q9X : Q25X;
+    const int* IY = m == 2 ? Q9Y : Q25Y;
+    for (int k = 0; k < nc; ++k) {
+        N[k] = Lx[IX[k]] * Le[IY[k]];
+        Nx[k] = dLx[IX[k]] * Le[IY[k]];
+        Ne[k] = Lx[IX[k]] * dLe[IY[k]];
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" int zj_geometry(int n_elem, int nc, int stride, const int64_t* en, const double* xy, int S,
+                           const double* sx, const double* se, double* J, double* Jinv, double* pos) {
+    double N[25], Nx[25], Ne[25];
+    for (int sp = 0; sp < S; ++sp)
+        if (!shape(nc, sx[sp], se[sp], N, Nx, Ne)) return ZJ_ERR_UNSUPPORTED;
+    for (int e = 0; e < n_elem; ++e)
+        for (int sp = 0; sp < S; ++sp) {
+            shape(nc, sx[sp], se[sp], N, Nx, Ne);
+            double xx = 0, xe = 0, yx = 0, ye = 0, px = 0, py = 0;
+            for (int i = 0; i < nc; ++i) {
+                int64_t v = node_of(en, stride, e, i);
+                const double X = xy[2 * v], Y = xy[2 * v + 1];
+                xx += Nx[i] * X; xe += Ne[i] * X; yx += Nx[i] * Y; ye += Ne[i] * Y;
+                px += N[i] * X; py += N[i] * Y;
+            }
+            const double det = xx * ye - xe * yx, inv = 1.0 / det;
+            const int64_t o = (int64_t)e * S + sp;
+            if (J) J[o] = det;
+            Jinv[4 * o + 0] = ye * inv; Jinv[4 * o + 1] = -xe * inv;
+            Jinv[4 * o + 2] = -yx * inv; Jinv[4 * o + 3] = xx * inv;
+            if (pos) { pos[2 * o] = px; pos[2 * o + 1] = py; }
+        }
+    return ZJ_OK;
+}

@@ -1,0 +1,594 @@
+// zj_kernels.hip -- fused flux-reconstruction stage kernels for gfx950.
+//
+// One launch per SSP-RK3 stage does the whole residual and the stage update
+// of a tile of elements (the reference needs 7 kernels, a memset and two D2D
+// copies per stage, kernels.cu:30-1349):
+//
+//   A  load U_in (SoA, coalesced), Euler flux at every solution point
+//      [K4 computeInviscidFluxSPKernel, kernels.cu:157-194]       -> LDS
+//   B1 the element's own flux-point states, sum-factorised 1D
+//      interpolation from LDS [K1 interpolateToFluxPointsKernel, :30-67]
+//   B2 per (edge, flux point): common flux against the neighbour's trace
+//      written by the previous stage, or a ghost state on a boundary, then the
+//      flux difference of this element's side
+//      [K9 computeRiemannFluxWithBCKernel :453-676, K8 computeFluxDiffForFRKernel
+//       :346-426]; each face's common flux is evaluated by both of its elements
+//      with identical inputs, so no face buffer goes through HBM
+//   C  divergence with the 1D derivative matrix from LDS [K11 :813-859],
+//      FR correction [K13 :948-1021], SSP-RK3 combination [K16-18 :1297-1349]
+//      and the positivity limiter [K15 :1236-1288] in FP64; U_out written
+//   D  trace of U_out for the next stage (written once, read once by neighbours)
+//
+// Every reference quirk on this path is kept: the clamped inline Rusanov that
+// ignores the configured solver, the NaN/Inf scrub, the right element pairing
+// F_common[fp] with its own slot fp, the correction indexed by the unmirrored
+// xi index on edges 0/2 and eta index on edges 1/3 without metric, and the
+// limiter's SI constants with gamma = 1.4.
+#include "zj_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+
+namespace zj {
+
+namespace {
+
+template <typename R> __device__ __forceinline__ R fmaR(R a, R b, R c);
+template <> __device__ __forceinline__ double fmaR<double>(double a, double b, double c) { return fma(a, b, c); }
+template <> __device__ __forceinline__ float fmaR<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename R> __device__ __forceinline__ R sqrtR(R x);
+template <> __device__ __forceinline__ double sqrtR<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float sqrtR<float>(float x) { return sqrtf(x); }
+
+template <typename R> __device__ __forceinline__ R fmaxR(R a, R b);
+template <> __device__ __forceinline__ double fmaxR<double>(double a, double b) { return fmax(a, b); }
+template <> __device__ __forceinline__ float fmaxR<float>(float a, float b) { return fmaxf(a, b); }
+template <typename R> __device__ __forceinline__ R fminR(R a, R b);
+template <> __device__ __forceinline__ double fminR<double>(double a, double b) { return fmin(a, b); }
+template <> __device__ __forceinline__ float fminR<float>(float a, float b) { return fminf(a, b); }
+
+// 4-wide loads and stores of one state (vector memory ops).
+__device__ __forceinline__ void load4(const double* p, double* v) {
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void load4(const float* p, float* v) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void store4(double* p, const double* v) {
+    reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void store4(float* p, const float* v) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// ---------------------------------------------------------------- physics
+// IdealGas::fluxX/fluxY (euler.hpp:101-129), no clamps.
+template <typename R>
+__device__ __forceinline__ void euler_flux(const R* U, R g, R* Fx, R* Fy) {
+    const R inv = R(1) / U[0];
+    const R u = U[1] * inv, v = U[2] * inv, E = U[3] * inv;
+    const R p = (g - R(1)) * U[0] * (E - R(0.5) * (u * u + v * v));
+    Fx[0] = U[1]; Fx[1] = U[1] * u + p; Fx[2] = U[1] * v; Fx[3] = (U[3] + p) * u;
+    Fy[0] = U[2]; Fy[1] = U[2] * u; Fy[2] = U[2] * v + p; Fy[3] = (U[3] + p) * v;
+}
+
+template <typename R>
+__device__ __forceinline__ void normal_flux(const R* U, R g, R nx, R ny, R* Fn) {
+    R fx[NV], fy[NV];
+    euler_flux(U, g, fx, fy);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) Fn[k] = fx[k] * nx + fy[k] * ny;
+}
+
+// Inline clamped Rusanov of computeRiemannFluxWithBCKernel (kernels.cu:601-668).
+template <typename R>
+__device__ __forceinline__ void rusanov(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
+    const R tiny = R(1e-10);
+    const R rhoL = fmaxR(UL[0], tiny), rhoR = fmaxR(UR[0], tiny);
+    const R iL = R(1) / rhoL, iR = R(1) / rhoR;
+    const R uL = UL[1] * iL, vL = UL[2] * iL, uR = UR[1] * iR, vR = UR[2] * iR;
+    const R EL = UL[3] * iL, ER = UR[3] * iR;
+    const R qL = uL * uL + vL * vL, qR = uR * uR + vR * vR;
+    const R pL = fmaxR((g - R(1)) * rhoL * (EL - R(0.5) * qL), tiny);
+    const R pR = fmaxR((g - R(1)) * rhoR * (ER - R(0.5) * qR), tiny);
+    const R cL = sqrtR(fmaxR(g * pL * iL, tiny)), cR = sqrtR(fmaxR(g * pR * iR, tiny));
+    const R vnL = uL * nx + vL * ny, vnR = uR * nx + vR * ny;
+    const R smax = fmaxR(sqrtR(qL) + cL, sqrtR(qR) + cR);
+    const R HL = EL + pL * iL, HR = ER + pR * iR;
+    const R FL[NV] = {rhoL * vnL, rhoL * uL * vnL + pL * nx, rhoL * vL * vnL + pL * ny, rhoL * HL * vnL};
+    const R FR[NV] = {rhoR * vnR, rhoR * uR * vnR + pR * nx, rhoR * vR * vnR + pR * ny, rhoR * HR * vnR};
+    const R hs = R(0.5) * smax;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        R f = R(0.5) * (FL[k] + FR[k]) - hs * (UR[k] - UL[k]);
+        F[k] = isfinite(f) ? f : R(0);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void prim2cons(R rho, R u, R v, R p, R g, R* U) {
+    U[0] = rho; U[1] = rho * u; U[2] = rho * v;
+    U[3] = p / (g - R(1)) + R(0.5) * rho * (u * u + v * v);
+}
+
+// computeGhostState (kernels.cu:453-525).
+template <typename R>
+__device__ void ghost_state(const R* Ui, R nx, R ny, int bct, const double* bc, R g, R* G) {
+    if (bct == 1 || bct == 2 || bct == 3) {  // Wall, SlipWall, Symmetry: reflect
+        const R rho = Ui[0], u = Ui[1] / rho, v = Ui[2] / rho;
+        const R p = (g - R(1)) * rho * (Ui[3] / rho - R(0.5) * (u * u + v * v));
+        const R vn = u * nx + v * ny;
+        prim2cons(rho, u - R(2) * vn * nx, v - R(2) * vn * ny, p, g, G);
+    } else if (bct == 4) {  // FarField: averaged pressure with clamps
+        const R rinf = R(bc[0]), uinf = R(bc[1]), vinf = R(bc[2]), pinf = R(bc[3]);
+        const R ri = fmaxR(Ui[0], R(1e-10));
+        const R ui = Ui[1] / ri, vi = Ui[2] / ri;
+        const R pi = fmaxR((g - R(1)) * (Ui[3] - R(0.5) * ri * (ui * ui + vi * vi)), R(1e-10));
+        const R vninf = uinf * nx + vinf * ny;
+        R rb, ub, vb;
+        if (vninf > R(0)) { rb = rinf; ub = uinf; vb = vinf; }
+        else { rb = ri; ub = ui; vb = vi; }
+        R pb = R(0.5) * (pinf + pi);
+        rb = fmaxR(fminR(rb, R(100)), R(0.01));
+        ub = fmaxR(fminR(ub, R(1000)), R(-1000));
+        vb = fmaxR(fminR(vb, R(1000)), R(-1000));
+        pb = fmaxR(fminR(pb, R(1e7)), R(1e3));
+        prim2cons(rb, ub, vb, pb, g, G);
+    } else if (bct == 6) {  // Outflow: prescribed static pressure
+        const R rho = Ui[0], u = Ui[1] / rho, v = Ui[2] / rho;
+        prim2cons(rho, u, v, R(bc[0]), g, G);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) G[k] = Ui[k];
+    }
+}
+
+// enforcePositivityKernel (kernels.cu:1236-1288), FP64, gamma fixed at 1.4.
+__device__ __forceinline__ void positivity(double* q) {
+    constexpr double GM1 = 1.4 - 1.0;
+    const double rn = fmax(0.01, fmin(100.0, q[0]));
+    const double ir = 1.0 / fmax(rn, 1e-10);
+    double u = q[1] * ir, v = q[2] * ir;
+    const double vm = sqrt(u * u + v * v);
+    if (vm > 2000.0) {
+        const double sc = 2000.0 / vm;
+        u *= sc; v *= sc;
+    }
+    const double ke = 0.5 * rn * (u * u + v * v);
+    const double p = fmax(100.0, fmin(1e8, GM1 * (q[3] - ke)));
+    q[0] = rn; q[1] = rn * u; q[2] = rn * v; q[3] = p / GM1 + ke;
+}
+
+// ------------------------------------------------------- wave reductions
+// Cross-lane moves through DPP on the two halves of a double.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double x, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+struct OpSum { __device__ static double f(double a, double b) { return a + b; } };
+struct OpMin { __device__ static double f(double a, double b) { return fmin(a, b); } };
+
+// Whole-wavefront reduction: butterfly within rows of 16 lanes by DPP
+// (quad_perm 1032, quad_perm 2301, row_half_mirror, row_mirror), then the four
+// row results by readlane.  Every lane of the wave must call it.
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v) {
+    v = Op::f(v, dpp_mov<0xB1>(v));
+    v = Op::f(v, dpp_mov<0x4E>(v));
+    v = Op::f(v, dpp_mov<0x141>(v));
+    v = Op::f(v, dpp_mov<0x140>(v));
+    return Op::f(Op::f(read_lane(v, 0), read_lane(v, 16)), Op::f(read_lane(v, 32), read_lane(v, 48)));
+}
+
+// Block reduction over BLOCK threads; the result is valid in thread 0.
+template <typename Op>
+__device__ __forceinline__ double block_reduce(double v, double* sred) {
+    v = wave_reduce<Op>(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sred[w] = v;
+    __syncthreads();
+    double r = sred[0];
+    if (threadIdx.x == 0)
+        for (int k = 1; k < BLOCK / 64; ++k) r = Op::f(r, sred[k]);
+    return r;
+}
+
+// ------------------------------------------------------------ operators
+// Value of one variable at flux point f of local edge k, from the solution
+// points of one element (u[S], xi index fastest).  Edge 0 (eta=-1) and 2
+// (eta=+1) run along xi, edges 1 (xi=+1) and 3 (xi=-1) along eta; edges 2 and
+// 3 are stored in reversed order (basis.cpp:540-552).  The explicit FMA chain
+// fixes the rounding, so both elements of a face see identical traces.
+template <typename R, int P>
+__device__ __forceinline__ R trace_point(const R* u, int k, int f, const R* Lm, const R* Lp) {
+    constexpr int N1 = P + 1;
+    const R* L = (k == 1 || k == 2) ? Lp : Lm;
+    int base, stride;
+    if (k == 0) { base = f; stride = N1; }
+    else if (k == 1) { base = f * N1; stride = 1; }
+    else if (k == 2) { base = P - f; stride = N1; }
+    else { base = (P - f) * N1; stride = 1; }
+    R acc = L[0] * u[base];
+#pragma unroll
+    for (int m = 1; m < N1; ++m) acc = fmaR(L[m], u[base + m * stride], acc);
+    return acc;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename R, int P, int EPB, int MODE>
+__global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
+    constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
+    constexpr int NT = EPB * S, NF = EPB * 4 * FP;
+    static_assert(NT <= BLOCK, "tile exceeds the workgroup");
+    __shared__ R sU[EPB][NV][S];
+    __shared__ R sFx[EPB][NV][S];
+    __shared__ R sFy[EPB][NV][S];
+    __shared__ R sTr[EPB][4][FP][NV];
+    __shared__ R sFd[EPB][4][FP][NV];
+    __shared__ R sD[N1 * N1], sLm[N1], sLp[N1], sC[4 * S];
+    __shared__ double sRed[BLOCK / 64];
+
+    const int tid = threadIdx.x;
+    const int e0 = blockIdx.x * EPB;
+    const int64_t PS = (int64_t)a.E * S;
+    const R g = R(a.gamma);
+
+    for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
+    for (int t = tid; t < 4 * S; t += BLOCK) sC[t] = ops.corr[t];
+    if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
+
+    const int el = tid / S, sp = tid - (tid / S) * S;
+    const int e = e0 + el;
+    const bool active = tid < NT && e < a.E;
+    const int64_t gi = (int64_t)e * S + sp;
+
+    // ---- A: state and volume flux
+    double q[NV];
+    if (active) {
+        R u[NV], fx[NV], fy[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { q[v] = a.U_in[v * PS + gi]; u[v] = R(q[v]); }
+        euler_flux(u, g, fx, fy);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { sU[el][v][sp] = u[v]; sFx[el][v][sp] = fx[v]; sFy[el][v][sp] = fy[v]; }
+    }
+    __syncthreads();
+
+    // ---- B1: own traces
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        if (e0 + el2 < a.E) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sTr[el2][k][f][v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+        }
+    }
+    __syncthreads();
+
+    // ---- B2: common flux and flux difference per (edge, flux point)
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        R Fd[NV] = {R(0), R(0), R(0), R(0)};
+        if (L.role != ROLE_NONE) {
+            const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+            R own[NV], other[NV], Fc[NV], Fn[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) own[v] = sTr[el2][k][f][v];
+            normal_flux(own, g, nx, ny, Fn);
+            if (L.role == ROLE_RIGHT) {
+                // common flux of the face as its left element computes it:
+                // U_L = left slot f, U_R = this element's slot Fp-1-f
+                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + f) * NV, other);
+                R mine[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) mine[v] = sTr[el2][k][P - f][v];
+                rusanov(other, mine, nx, ny, g, Fc);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fd[v] = -Fc[v] + Fn[v];
+            } else {
+                if (L.role == ROLE_LEFT)
+                    load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (P - f)) * NV, other);
+                else
+                    ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
+                rusanov(own, other, nx, ny, g, Fc);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fd[v] = Fc[v] - Fn[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sFd[el2][k][f][v] = Fd[v];
+    }
+    __syncthreads();
+
+    // ---- C: divergence, correction, stage update
+    double rsq = 0.0;
+    if (active) {
+        const int i = sp % N1, j = sp / N1;  // xi index, eta index
+        const R jx = a.Jinv[gi], jy = a.Jinv[PS + gi], kx = a.Jinv[2 * PS + gi], ky = a.Jinv[3 * PS + gi];
+        double res[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R* Fx = sFx[el][v];
+            const R* Fy = sFy[el][v];
+            R dFdxi = 0, dFdeta = 0, dGdxi = 0, dGdeta = 0;
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                const R di = sD[i * N1 + m], dj = sD[j * N1 + m];
+                dFdxi += di * Fx[j * N1 + m];
+                dGdxi += di * Fy[j * N1 + m];
+                dFdeta += dj * Fx[m * N1 + i];
+                dGdeta += dj * Fy[m * N1 + i];
+            }
+            const R div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
+            const R c = sFd[el][0][i][v] * sC[sp] + sFd[el][1][j][v] * sC[S + sp] +
+                        sFd[el][2][i][v] * sC[2 * S + sp] + sFd[el][3][j][v] * sC[3 * S + sp];
+            res[v] = double(-div - c);
+            rsq += res[v] * res[v];
+        }
+        if (MODE == MODE_RESIDUAL) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) a.R_out[v * PS + gi] = res[v];
+        } else {
+            const double dt = *a.dt_ptr;
+            double un[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if (MODE == MODE_STAGE1) {
+                    un[v] = q[v] + dt * res[v];
+                } else {
+                    const double u0 = a.U0[v * PS + gi];
+                    if (MODE == MODE_STAGE2) un[v] = 0.75 * u0 + 0.25 * q[v] + 0.25 * dt * res[v];
+                    else un[v] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * q[v] + (2.0 / 3.0) * dt * res[v];
+                }
+            }
+            if (a.positivity) positivity(un);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                a.U_out[v * PS + gi] = un[v];
+                sU[el][v][sp] = R(un[v]);  // safe: sU was last read before the B1->B2 barrier
+            }
+        }
+    }
+    if (a.l2_partial) {
+        const double s = block_reduce<OpSum>(rsq, sRed);
+        __shared__ unsigned int sLast;
+        if (tid == 0) {
+            a.l2_partial[blockIdx.x] = s;
+            __threadfence();
+            sLast = (atomicAdd(a.ticket, 1u) == gridDim.x - 1) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (sLast) {  // every other workgroup's partial is visible: reduce in a fixed order
+            __threadfence();
+            double acc = 0.0;
+            for (int b = tid; b < (int)gridDim.x; b += BLOCK) acc += a.l2_partial[b];
+            const double tot = block_reduce<OpSum>(acc, sRed);
+            if (tid == 0) {
+                *a.l2_out = sqrt(tot);
+                *a.ticket = 0u;  // ready for the next launch
+            }
+        }
+    }
+    if (MODE == MODE_RESIDUAL) return;
+    __syncthreads();
+
+    // ---- D: traces of the new state
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        R tr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+        store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
+    }
+}
+
+template <typename R, int P, int EPB>
+__global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, const OpTables<R> ops) {
+    constexpr int N1 = P + 1, S = N1 * N1, FP = N1, NT = EPB * S, NF = EPB * 4 * FP;
+    __shared__ R sU[EPB][NV][S];
+    __shared__ R sLm[N1], sLp[N1];
+    const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
+    const int64_t PS = (int64_t)a.E * S;
+    if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
+    const int el = tid / S, sp = tid - (tid / S) * S;
+    if (tid < NT && e0 + el < a.E) {
+        const int64_t gi = (int64_t)(e0 + el) * S + sp;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sU[el][v][sp] = R(a.U[v * PS + gi]);
+    }
+    __syncthreads();
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        R tr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+        store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
+    }
+}
+
+__global__ void aos_to_soa_kernel(const double* __restrict__ aos, double* __restrict__ soa, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[NV];
+    load4(aos + 4 * i, v);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) soa[k * n + i] = v[k];
+}
+
+__global__ void soa_to_aos_kernel(const double* __restrict__ soa, double* __restrict__ aos, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = soa[k * n + i];
+    store4(aos + 4 * i, v);
+}
+
+__global__ void set_scalar_kernel(double* p, double v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// FRSolver::computeTimeStep per point (fr_solver.cpp:799-811, euler.hpp:34-41,166-171).
+__global__ __launch_bounds__(BLOCK) void dt_partial_kernel(const double* __restrict__ U,
+                                                           const double* __restrict__ hcell,
+                                                           int E, int S, double gamma, double cfl,
+                                                           double cf, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double sRed[BLOCK / 64];
+    const int64_t n = (int64_t)E * S;
+    double m = 1e20;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const double rho = U[i], ru = U[n + i], rv = U[2 * n + i], rE = U[3 * n + i];
+        const double u = ru / rho, v = rv / rho;
+        const double p = (gamma - 1.0) * rho * (rE / rho - 0.5 * (u * u + v * v));
+        const double ws = sqrt(u * u + v * v) + sqrt(gamma * fmax(p, 1e-10) / fmax(rho, 1e-10));
+        const double h = hcell[i / S];
+        const double d = cfl * h / (ws * cf);
+        m = d < m ? d : m;
+    }
+    const double r = block_reduce<OpMin>(m, sRed);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(BLOCK) void nonfinite_kernel(const double* __restrict__ U, int64_t n,
+                                                          unsigned long long* __restrict__ count) {
+    __shared__ double sRed[BLOCK / 64];
+    double c = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        c += isfinite(U[i]) ? 0.0 : 1.0;
+    const double r = block_reduce<OpSum>(c, sRed);
+    if (threadIdx.x == 0 && r > 0) atomicAdd(count, (unsigned long long)r);
+}
+
+__global__ __launch_bounds__(BLOCK) void finalize_kernel(const double* __restrict__ partial, int n,
+                                                         int op_min, double* __restrict__ out) {
+    __shared__ double sRed[BLOCK / 64];
+    double acc = op_min ? 1e300 : 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK)
+        acc = op_min ? fmin(acc, partial[i]) : acc + partial[i];
+    const double r = op_min ? block_reduce<OpMin>(acc, sRed) : block_reduce<OpSum>(acc, sRed);
+    if (threadIdx.x == 0) *out = op_min ? r : sqrt(r);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launch
+int max_epb(int P) { return BLOCK / ((P + 1) * (P + 1)); }
+int default_epb(int P) { return max_epb(P); }
+int num_blocks(int E, int epb) { return (E + epb - 1) / epb; }
+
+#define ZJ_EPB(P) (BLOCK / (((P) + 1) * ((P) + 1)))
+
+template <typename R, int P, int EPB>
+static hipError_t stage_dispatch_mode(int mode, const StageArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
+    const dim3 grid(num_blocks(a.E, EPB)), block(BLOCK);
+    switch (mode) {
+    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3><<<grid, block, 0, s>>>(a, ops); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int P>
+static hipError_t stage_dispatch_epb(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, hipStream_t s) {
+    constexpr int M = ZJ_EPB(P);
+    if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M>(mode, a, ops, s);
+    if (M / 2 >= 1 && epb == M / 2) return stage_dispatch_mode<R, P, (M / 2 >= 1 ? M / 2 : 1)>(mode, a, ops, s);
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, hipStream_t s) {
+    if (a.E <= 0) return hipSuccess;
+    switch (P) {
+    case 1: return stage_dispatch_epb<R, 1>(mode, a, ops, epb, s);
+    case 2: return stage_dispatch_epb<R, 2>(mode, a, ops, epb, s);
+    case 3: return stage_dispatch_epb<R, 3>(mode, a, ops, epb, s);
+    case 4: return stage_dispatch_epb<R, 4>(mode, a, ops, epb, s);
+    case 5: return stage_dispatch_epb<R, 5>(mode, a, ops, epb, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename R, int P>
+static hipError_t trace_dispatch(const TraceArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
+    constexpr int M = ZJ_EPB(P);
+    trace_kernel<R, P, M><<<dim3(num_blocks(a.E, M)), dim3(BLOCK), 0, s>>>(a, ops);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, int, hipStream_t s) {
+    if (a.E <= 0) return hipSuccess;
+    switch (P) {
+    case 1: return trace_dispatch<R, 1>(a, ops, s);
+    case 2: return trace_dispatch<R, 2>(a, ops, s);
+    case 3: return trace_dispatch<R, 3>(a, ops, s);
+    case 4: return trace_dispatch<R, 4>(a, ops, s);
+    case 5: return trace_dispatch<R, 5>(a, ops, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, hipStream_t);
+template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&, int, hipStream_t);
+template hipError_t launch_trace<double>(int, const TraceArgs<double>&, const OpTables<double>&, int, hipStream_t);
+template hipError_t launch_trace<float>(int, const TraceArgs<float>&, const OpTables<float>&, int, hipStream_t);
+
+hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    aos_to_soa_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(aos, soa, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    soa_to_aos_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(soa, aos, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_scalar(double* p, double v, hipStream_t s) {
+    set_scalar_kernel<<<1, 64, 0, s>>>(p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma, double cfl,
+                             double cf, double* partial, int nblk, hipStream_t s) {
+    dt_partial_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, hcell, E, S, gamma, cfl, cf, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count, int nblk, hipStream_t s) {
+    nonfinite_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, n, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* partial, int n, int op_min, double* out, hipStream_t s) {
+    finalize_kernel<<<1, BLOCK, 0, s>>>(partial, n, op_min, out);
+    return hipGetLastError();
+}
+
+}  // namespace zj

@@ -1,0 +1,89 @@
+// zj_kernels.hpp -- device-side data layout and launch interface of the fused
+// FR stage kernels (zj_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zj {
+
+constexpr int NV = 4;           // conservative variables
+constexpr int BLOCK = 256;      // threads per workgroup of the stage kernels
+
+// Role of one (element, local edge) slot on its face.
+enum EdgeRole : int { ROLE_LEFT = 0, ROLE_RIGHT = 1, ROLE_BOUNDARY = 2, ROLE_NONE = 3 };
+
+// Per (element, local edge): neighbour element, neighbour edge (or BC type on a
+// boundary), role, BC data index.
+struct alignas(16) EdgeLink {
+    int nbr;
+    int nbr_edge;
+    int role;
+    int bc_idx;
+};
+
+// Sum-factorised operators of one order, extracted from the reference's dense
+// tables: D[i][k] = l_k'(x_i), Lm[j] = l_j(-1), Lp[j] = l_j(+1), corr[4][S].
+template <typename R>
+struct OpTables {
+    R D[36];
+    R Lm[6];
+    R Lp[6];
+    R corr[4 * 36];
+};
+
+enum StageMode : int { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3 };
+
+// Arguments of one stage launch.  State planes are SoA: plane v holds E*S
+// doubles, element-major then solution point.
+template <typename R>
+struct StageArgs {
+    int E;
+    const double* U_in;      // stage input state
+    const double* U0;        // U^n (stages 2, 3)
+    double* U_out;           // stage output state (may alias U_in or U0 element-locally)
+    double* R_out;           // residual planes (MODE_RESIDUAL)
+    const R* Jinv;           // 4 planes of E*S
+    const R* T_in;           // traces of U_in [E][4][Fp][4]
+    R* T_out;                // traces of U_out
+    const EdgeLink* links;   // [E][4]
+    const R* normals;        // [E][4][2]
+    const double* bc_data;   // [n][8]
+    const double* dt_ptr;    // device scalar
+    double* l2_partial;      // [grid] sum of R^2 per workgroup, or nullptr
+    unsigned int* ticket;    // workgroups done; the last one reduces the partials
+    double* l2_out;          // sqrt of the summed partials (written by the last workgroup)
+    double gamma;
+    int positivity;
+};
+
+template <typename R>
+struct TraceArgs {
+    int E;
+    const double* U;         // SoA planes
+    R* T_out;
+};
+
+// Launch wrappers (host).  `epb` = elements per workgroup (0 = default).
+template <typename R>
+hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
+                        int epb, hipStream_t s);
+template <typename R>
+hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, int epb,
+                        hipStream_t s);
+int default_epb(int P);
+int max_epb(int P);
+int num_blocks(int E, int epb);
+
+hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n_points, hipStream_t s);
+hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n_points, hipStream_t s);
+hipError_t launch_set_scalar(double* p, double v, hipStream_t s);
+// dt = min over points of cfl*h_e/(ws*cf), h_e = sqrt(area_e); partial per block.
+hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma,
+                             double cfl, double cf, double* partial, int nblk, hipStream_t s);
+hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count,
+                                  int nblk, hipStream_t s);
+// Deterministic reduction of block partials: out_sum = sqrt(sum) (l2) or min.
+hipError_t launch_finalize(const double* partial, int n, int op_min, double* out, hipStream_t s);
+
+}  // namespace zj

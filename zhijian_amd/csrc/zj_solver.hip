@@ -1,0 +1,513 @@
+// zj_solver.hip -- the C ABI (include/zhijian_amd.h) over the fused stage
+// kernels: device state ownership, operator extraction, launch sequencing and
+// hipGraph replay of the SSP-RK3 step.
+//
+// Device layout (one GPU owns the whole mesh):
+//   U, W        state planes [4][E*S] FP64 (U = U^n; W = U1, then U2 in place)
+//   T[3]        flux-point traces [E][4][Fp][4] in the compute precision;
+//               stage 1 reads T[c] and writes T[c+1], stage 2 T[c+1]->T[c+2],
+//               stage 3 T[c+2]->T[c], so every step is the same launch sequence
+//   Jinv        4 planes [E*S] in the compute precision
+//   links       [E][4] EdgeLink; normals [E][4][2] (face normal of the left side)
+#include "zhijian_amd.h"
+#include "zj_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace zj;
+
+struct zj_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int P = 0, N1 = 0, S = 0, E = 0, F = 0;
+    int prec = ZJ_PREC_FP64;
+    int positivity = 1;
+    double gamma = 1.4, cfl = 0.5;
+    int epb = 0;
+    int nblk = 0;        // workgroups of the stage kernel
+    int nred = 0;        // workgroups of the standalone reductions
+    double* U = nullptr;
+    double* W = nullptr;
+    double* Rres = nullptr;
+    double* aos = nullptr;
+    void* Jinv = nullptr;
+    void* T[3] = {nullptr, nullptr, nullptr};
+    EdgeLink* links = nullptr;
+    void* normals = nullptr;
+    double* bc_data = nullptr;
+    double* hcell = nullptr;
+    double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl
+    double* partial = nullptr;
+    unsigned long long* d_count = nullptr;
+    unsigned int* d_ticket = nullptr;
+    double* h_pinned = nullptr;
+    double dt_on_device = NAN;   // last dt written to d_scalars[0]
+    OpTables<double> ops64{};
+    OpTables<float> ops32{};
+    bool has_state = false;
+    bool has_area = false;
+    int stage_next = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+};
+
+static std::string g_create_error;
+
+#define ZJ_HIP(h, call)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return ZJ_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+
+static int fail(zj_solver* h, int code, const std::string& msg) {
+    h->err = msg;
+    return code;
+}
+
+static hipStream_t pick(zj_solver* h, void* s) { return s ? (hipStream_t)s : h->stream; }
+
+// ---------------------------------------------------------------- tables
+// The reference's dense tables (basis.cpp:508-569) are tensor products of 1D
+// Gauss-point operators; extract the 1D factors and check every entry.
+static bool extract_ops(const zj_mesh_desc* m, int P, OpTables<double>& o, std::string& why) {
+    const int n = P + 1, S = n * n;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) o.D[i * n + k] = m->diff_xi[(size_t)i * S + k];
+    for (int j = 0; j < n; ++j) o.Lm[j] = m->interp[(size_t)(0 * n + 0) * S + j * n + 0];
+    for (int i = 0; i < n; ++i) o.Lp[i] = m->interp[(size_t)(1 * n + 0) * S + 0 * n + i];
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const int row = j * n + i;
+            for (int c = 0; c < S; ++c) {
+                const int cj = c / n, ci = c % n;
+                const double ex = (cj == j) ? o.D[i * n + ci] : 0.0;
+                const double ee = (ci == i) ? o.D[j * n + cj] : 0.0;
+                if (m->diff_xi[(size_t)row * S + c] != ex || m->diff_eta[(size_t)row * S + c] != ee) {
+                    why = "differentiation tables are not tensor products of one 1D matrix";
+                    return false;
+                }
+            }
+        }
+    for (int k = 0; k < 4; ++k)
+        for (int f = 0; f < n; ++f)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    double ex;
+                    if (k == 0) ex = (i == f) ? o.Lm[j] : 0.0;
+                    else if (k == 1) ex = (j == f) ? o.Lp[i] : 0.0;
+                    else if (k == 2) ex = (i == P - f) ? o.Lp[j] : 0.0;
+                    else ex = (j == P - f) ? o.Lm[i] : 0.0;
+                    if (m->interp[(size_t)(k * n + f) * S + j * n + i] != ex) {
+                        why = "interpolation tables do not place flux points at the solution-point "
+                              "abscissae (edges 0/1 forward, 2/3 reversed)";
+                        return false;
+                    }
+                }
+    for (int t = 0; t < 4 * S; ++t) o.corr[t] = m->corr[t];
+    return true;
+}
+
+static void ops_to_f32(const OpTables<double>& a, OpTables<float>& b) {
+    for (int t = 0; t < 36; ++t) b.D[t] = (float)a.D[t];
+    for (int t = 0; t < 6; ++t) { b.Lm[t] = (float)a.Lm[t]; b.Lp[t] = (float)a.Lp[t]; }
+    for (int t = 0; t < 4 * 36; ++t) b.corr[t] = (float)a.corr[t];
+}
+
+template <typename T>
+static int upload(zj_solver* h, void** dst, const std::vector<T>& v) {
+    ZJ_HIP(h, hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T)));
+    if (!v.empty()) ZJ_HIP(h, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return ZJ_OK;
+}
+
+static void release(zj_solver* h) {
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->T[0], h->T[1], h->T[2], h->links,
+                    h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->d_count, h->d_ticket};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (h->h_pinned) hipHostFree(h->h_pinned);
+    if (h->stream) hipStreamDestroy(h->stream);
+}
+
+static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
+    const int P = m->poly_order;
+    if (P < 1 || P > 5) return fail(h, ZJ_ERR_ARG, "poly_order must be 1..5");
+    if (m->n_elem < 0 || m->n_faces < 0) return fail(h, ZJ_ERR_ARG, "negative mesh sizes");
+    if (!m->diff_xi || !m->diff_eta || !m->interp || !m->corr || (m->n_elem > 0 && !m->Jinv))
+        return fail(h, ZJ_ERR_ARG, "operator or geometry tables missing");
+    if (m->n_faces > 0 && (!m->normals || !m->face_left_elem || !m->face_left_local ||
+                           !m->face_right_elem || !m->face_right_local))
+        return fail(h, ZJ_ERR_ARG, "face tables missing");
+    if (p->precision != ZJ_PREC_FP64 && p->precision != ZJ_PREC_FP32BULK)
+        return fail(h, ZJ_ERR_ARG, "unknown precision");
+    if (p->compat != ZJ_COMPAT_REFERENCE)
+        return fail(h, ZJ_ERR_UNSUPPORTED, "only the reference-compatible arithmetic is built");
+    if (!(p->gamma > 1.0)) return fail(h, ZJ_ERR_ARG, "gamma must exceed 1");
+    if ((int64_t)m->n_elem * (P + 1) * (P + 1) * 4 > (int64_t)INT32_MAX * 4)
+        return fail(h, ZJ_ERR_UNSUPPORTED, "mesh too large for 32-bit element indices");
+
+    h->P = P; h->N1 = P + 1; h->S = h->N1 * h->N1; h->E = m->n_elem; h->F = m->n_faces;
+    h->prec = p->precision; h->positivity = p->positivity ? 1 : 0;
+    h->gamma = p->gamma; h->cfl = p->cfl; h->device = p->device;
+    h->epb = default_epb(P);
+    h->nblk = num_blocks(std::max(h->E, 1), h->epb);
+
+    std::string why;
+    if (!extract_ops(m, P, h->ops64, why)) return fail(h, ZJ_ERR_UNSUPPORTED, why);
+    ops_to_f32(h->ops64, h->ops32);
+
+    // element-edge links from the face list (fr_solver.cpp:240-250, kernels.cu:549-595)
+    const int E = h->E, F = h->F, Fp = h->N1;
+    std::vector<EdgeLink> links((size_t)E * 4, EdgeLink{-1, 0, ROLE_NONE, 0});
+    std::vector<double> nrm((size_t)E * 8, 0.0);
+    int max_bc = -1;
+    for (int f = 0; f < F; ++f) {
+        const int le = m->face_left_elem[f], ll = m->face_left_local[f];
+        const int re = m->face_right_elem[f], rl = m->face_right_local[f];
+        if (le < 0 || le >= E || ll < 0 || ll > 3) return fail(h, ZJ_ERR_ARG, "face left side out of range");
+        EdgeLink& L = links[(size_t)le * 4 + ll];
+        if (L.role != ROLE_NONE) return fail(h, ZJ_ERR_ARG, "an element edge belongs to two faces");
+        const double nx = m->normals[2 * f], ny = m->normals[2 * f + 1];
+        nrm[((size_t)le * 4 + ll) * 2] = nx; nrm[((size_t)le * 4 + ll) * 2 + 1] = ny;
+        if (re >= 0) {
+            if (re >= E || rl < 0 || rl > 3) return fail(h, ZJ_ERR_ARG, "face right side out of range");
+            EdgeLink& Rr = links[(size_t)re * 4 + rl];
+            if (Rr.role != ROLE_NONE || (re == le && rl == ll))
+                return fail(h, ZJ_ERR_ARG, "an element edge belongs to two faces");
+            L = EdgeLink{re, rl, ROLE_LEFT, 0};
+            Rr = EdgeLink{le, ll, ROLE_RIGHT, 0};
+            nrm[((size_t)re * 4 + rl) * 2] = nx; nrm[((size_t)re * 4 + rl) * 2 + 1] = ny;
+        } else {
+            int bct = m->bc_type ? m->bc_type[f] : ZJ_BC_SLIPWALL;
+            int bci = m->bc_face_map ? m->bc_face_map[f] : -1;
+            if (bci < 0) { bct = ZJ_BC_SLIPWALL; bci = 0; }  // kernels.cu:589-593
+            const bool needs_data = (bct == ZJ_BC_FARFIELD || bct == ZJ_BC_OUTFLOW);
+            if (needs_data && (!m->bc_data || bci >= m->n_bc_data))
+                return fail(h, ZJ_ERR_ARG, "boundary face refers to missing bc_data");
+            if (needs_data) max_bc = std::max(max_bc, bci);
+            L = EdgeLink{-1, bct, ROLE_BOUNDARY, needs_data ? bci : 0};
+        }
+    }
+
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+    const size_t npts = (size_t)E * h->S;
+    const size_t rsz = h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float);
+    ZJ_HIP(h, hipMalloc((void**)&h->U, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->W, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->Rres, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->aos, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    for (int k = 0; k < 3; ++k) ZJ_HIP(h, hipMalloc(&h->T[k], std::max<size_t>((size_t)E * 4 * Fp * 4, 1) * rsz));
+
+    // Jinv: [E][S][4] -> 4 planes, converted to the compute precision
+    std::vector<double> jp(npts * 4);
+    for (size_t g = 0; g < npts; ++g)
+        for (int c = 0; c < 4; ++c) jp[c * npts + g] = m->Jinv[g * 4 + c];
+    int rc;
+    if (h->prec == ZJ_PREC_FP64) {
+        if ((rc = upload(h, &h->Jinv, jp))) return rc;
+        if ((rc = upload(h, &h->normals, nrm))) return rc;
+    } else {
+        std::vector<float> jf(jp.begin(), jp.end()), nf(nrm.begin(), nrm.end());
+        if ((rc = upload(h, &h->Jinv, jf))) return rc;
+        if ((rc = upload(h, &h->normals, nf))) return rc;
+    }
+    if ((rc = upload(h, (void**)&h->links, links))) return rc;
+    std::vector<double> bcd;
+    if (m->bc_data && m->n_bc_data > 0) bcd.assign(m->bc_data, m->bc_data + (size_t)m->n_bc_data * 8);
+    if ((rc = upload(h, (void**)&h->bc_data, bcd))) return rc;
+    std::vector<double> hc;
+    if (m->elem_area) {
+        hc.resize(E);
+        for (int e = 0; e < E; ++e) hc[e] = std::sqrt(m->elem_area[e]);  // fr_solver.cpp:799
+        h->has_area = true;
+    }
+    if ((rc = upload(h, (void**)&h->hcell, hc))) return rc;
+
+    h->nred = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)npts + BLOCK - 1) / BLOCK));
+    const int npart = std::max(std::max(h->nblk, num_blocks(std::max(E, 1), 1)), h->nred);
+    ZJ_HIP(h, hipMalloc((void**)&h->partial, (size_t)npart * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->d_scalars, 8 * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->d_count, sizeof(unsigned long long)));
+    ZJ_HIP(h, hipMalloc((void**)&h->d_ticket, sizeof(unsigned int)));
+    ZJ_HIP(h, hipMemset(h->d_ticket, 0, sizeof(unsigned int)));
+    ZJ_HIP(h, hipHostMalloc((void**)&h->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
+    ZJ_HIP(h, hipMemset(h->d_scalars, 0, 8 * sizeof(double)));
+    ZJ_HIP(h, hipDeviceSynchronize());
+    return ZJ_OK;
+}
+
+extern "C" int zj_version(void) { return 1; }
+
+extern "C" int zj_create(zj_handle* out, const zj_mesh_desc* m, const zj_params* p) {
+    if (!out || !m || !p) {
+        g_create_error = "null argument";
+        return ZJ_ERR_ARG;
+    }
+    *out = nullptr;
+    zj_solver* h = new (std::nothrow) zj_solver();
+    if (!h) {
+        g_create_error = "out of host memory";
+        return ZJ_ERR_NOMEM;
+    }
+    int rc = build(h, m, p);
+    if (rc != ZJ_OK) {
+        g_create_error = h->err;
+        release(h);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return ZJ_OK;
+}
+
+extern "C" int zj_destroy(zj_handle h) {
+    if (!h) return ZJ_ERR_ARG;
+    hipSetDevice(h->device);
+    hipStreamSynchronize(h->stream);
+    release(h);
+    delete h;
+    return ZJ_OK;
+}
+
+extern "C" const char* zj_last_error(zj_handle h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+// ---------------------------------------------------------------- launches
+template <typename R>
+static StageArgs<R> stage_args(zj_solver* h, int mode) {
+    StageArgs<R> a{};
+    a.E = h->E;
+    a.Jinv = (const R*)h->Jinv;
+    a.links = h->links;
+    a.normals = (const R*)h->normals;
+    a.bc_data = h->bc_data;
+    a.dt_ptr = h->d_scalars;
+    a.ticket = h->d_ticket;
+    a.l2_out = h->d_scalars + 1;
+    a.gamma = h->gamma;
+    a.positivity = h->positivity;
+    R** T = (R**)h->T;
+    switch (mode) {
+    case MODE_RESIDUAL:
+        a.U_in = h->U; a.R_out = h->Rres; a.T_in = T[0]; a.l2_partial = h->partial;
+        break;
+    case MODE_STAGE1:
+        a.U_in = h->U; a.U_out = h->W; a.T_in = T[0]; a.T_out = T[1];
+        break;
+    case MODE_STAGE2:
+        a.U0 = h->U; a.U_in = h->W; a.U_out = h->W; a.T_in = T[1]; a.T_out = T[2];
+        break;
+    default:
+        a.U0 = h->U; a.U_in = h->W; a.U_out = h->U; a.T_in = T[2]; a.T_out = T[0];
+        a.l2_partial = h->partial;
+        break;
+    }
+    return a;
+}
+
+static hipError_t enqueue_stage(zj_solver* h, int mode, hipStream_t s) {
+    hipError_t e;
+    if (h->prec == ZJ_PREC_FP64)
+        e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, s);
+    else
+        e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, s);
+    return e;
+}
+
+// The step size lives in device memory so that the captured step graph serves
+// every dt; it is rewritten only when it changes.
+static hipError_t set_dt(zj_solver* h, double dt, hipStream_t s) {
+    if (dt == h->dt_on_device) return hipSuccess;
+    h->dt_on_device = dt;
+    return launch_set_scalar(h->d_scalars, dt, s);
+}
+
+static int refresh_trace(zj_solver* h, hipStream_t s) {
+    hipError_t e;
+    if (h->prec == ZJ_PREC_FP64)
+        e = launch_trace<double>(h->P, TraceArgs<double>{h->E, h->U, (double*)h->T[0]}, h->ops64, 0, s);
+    else
+        e = launch_trace<float>(h->P, TraceArgs<float>{h->E, h->U, (float*)h->T[0]}, h->ops32, 0, s);
+    ZJ_HIP(h, e);
+    return ZJ_OK;
+}
+
+static void drop_graph(zj_solver* h) {
+    if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+}
+
+extern "C" int zj_set_state_device(zj_handle h, const double* d_aos, void* stream) {
+    if (!h || !d_aos) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, launch_aos_to_soa(d_aos, h->U, (int64_t)h->E * h->S, s));
+    int rc = refresh_trace(h, s);
+    if (rc) return rc;
+    h->has_state = true;
+    h->stage_next = 1;
+    return ZJ_OK;
+}
+
+extern "C" int zj_set_state(zj_handle h, const double* U_aos) {
+    if (!h || !U_aos) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->aos, U_aos, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int rc = zj_set_state_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+extern "C" int zj_get_state_device(zj_handle h, double* d_aos, void* stream) {
+    if (!h || !d_aos) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "state requested in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, launch_soa_to_aos(h->U, d_aos, (int64_t)h->E * h->S, pick(h, stream)));
+    return ZJ_OK;
+}
+
+extern "C" int zj_get_state(zj_handle h, double* U_aos) {
+    if (!h || !U_aos) return ZJ_ERR_ARG;
+    int rc = zj_get_state_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(U_aos, h->aos, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+extern "C" int zj_residual(zj_handle h, double* dUdt_aos, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual requested in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, enqueue_stage(h, MODE_RESIDUAL, s));
+    if (dUdt_aos) {
+        ZJ_HIP(h, launch_soa_to_aos(h->Rres, h->aos, (int64_t)h->E * h->S, s));
+        ZJ_HIP(h, hipMemcpyAsync(dUdt_aos, h->aos, (size_t)h->E * h->S * 4 * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+        ZJ_HIP(h, hipStreamSynchronize(s));
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (stage != h->stage_next) return fail(h, ZJ_ERR_STATE, "stages must run in the order 1, 2, 3");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, set_dt(h, dt, s));
+    ZJ_HIP(h, enqueue_stage(h, stage, s));
+    h->stage_next = stage == 3 ? 1 : stage + 1;
+    return ZJ_OK;
+}
+
+static int capture_step(zj_solver* h) {
+    drop_graph(h);
+    ZJ_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int st = 1; st <= 3 && e == hipSuccess; ++st) e = enqueue_stage(h, st, h->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        ZJ_HIP(h, e != hipSuccess ? e : e2);
+    }
+    h->graph = g;
+    ZJ_HIP(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    return ZJ_OK;
+}
+
+extern "C" int zj_rk3_step(zj_handle h, double dt, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    if (!h->gexec) {
+        int rc = capture_step(h);
+        if (rc) return rc;
+    }
+    ZJ_HIP(h, set_dt(h, dt, s));
+    ZJ_HIP(h, hipGraphLaunch(h->gexec, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
+    if (!h || !dt) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (!h->has_area) return fail(h, ZJ_ERR_STATE, "element areas were not provided");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "dt requested in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial,
+                                h->nred, h->stream));
+    ZJ_HIP(h, launch_finalize(h->partial, h->nred, 1, h->d_scalars + 2, h->stream));
+    ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    *dt = h->h_pinned[0];
+    return ZJ_OK;
+}
+
+extern "C" int zj_residual_l2(zj_handle h, double* l2) {
+    if (!h || !l2) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipDeviceSynchronize());
+    ZJ_HIP(h, hipMemcpy(h->h_pinned, h->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost));
+    *l2 = h->h_pinned[0];
+    return ZJ_OK;
+}
+
+extern "C" int zj_check_finite(zj_handle h, int64_t* n_bad) {
+    if (!h || !n_bad) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipMemsetAsync(h->d_count, 0, sizeof(unsigned long long), h->stream));
+    ZJ_HIP(h, launch_count_nonfinite(h->U, (int64_t)h->E * h->S * 4, h->d_count, h->nred, h->stream));
+    unsigned long long c = 0;
+    ZJ_HIP(h, hipMemcpyAsync(&c, h->d_count, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    *n_bad = (int64_t)c;
+    return ZJ_OK;
+}
+
+extern "C" int zj_set_tile(zj_handle h, int epb) {
+    if (!h) return ZJ_ERR_ARG;
+    const int M = max_epb(h->P);
+    if (epb != M && epb != M / 2) return fail(h, ZJ_ERR_UNSUPPORTED, "tile must be the maximum or half of it");
+    if (epb == h->epb) return ZJ_OK;
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "tile change in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    // the partial-sum buffer holds one entry per element, enough for any tile
+    h->epb = epb;
+    h->nblk = num_blocks(std::max(h->E, 1), epb);
+    drop_graph(h);
+    return ZJ_OK;
+}
+
+extern "C" int zj_get_info(zj_handle h, int64_t* n_dof, int* epb, int* n_sp) {
+    if (!h) return ZJ_ERR_ARG;
+    if (n_dof) *n_dof = (int64_t)h->E * h->S * 4;
+    if (epb) *epb = h->epb;
+    if (n_sp) *n_sp = h->S;
+    return ZJ_OK;
+}

@@ -1,0 +1,192 @@
+"""Host-side handle of one device solver (Python face of include/zhijian_amd.h).
+
+``Solver`` owns a zj_handle.  ``Solver.from_mesh`` builds every table the
+reference's FRSolver::initialize builds (operators, faces with periodic
+pairing, normals, Jacobians, areas, BC tables) with the library's native host
+code, then creates the device solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+PRECISIONS = {"fp64": _lib.ZJ_PREC_FP64, "fp32bulk": _lib.ZJ_PREC_FP32BULK}
+BC_FARFIELD = 4
+BC_OUTFLOW = 6
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class Tables:
+    """Everything uploaded at zj_create, in the reference's layouts."""
+    p: int
+    ops: dict
+    faces: np.ndarray        # [F,4] (left elem, left edge, right elem, right edge)
+    normals: np.ndarray      # [F,2]
+    J: np.ndarray
+    Jinv: np.ndarray
+    area: np.ndarray
+    bc_type: np.ndarray
+    bc_face_map: np.ndarray
+    bc_data: np.ndarray
+    pos: Optional[np.ndarray] = None
+
+    @property
+    def E(self):
+        return self.area.shape[0]
+
+    @property
+    def S(self):
+        return (self.p + 1) ** 2
+
+
+def freestream(gamma=1.4, p_inf=101325.0, rho_inf=1.225, mach=0.5, aoa_deg=0.0):
+    """Far-field state as FRSolver builds it (fr_solver.cpp:438-443)."""
+    c = math.sqrt(gamma * p_inf / rho_inf)
+    vel = mach * c
+    a = aoa_deg * math.pi / 180.0
+    return rho_inf, vel * math.cos(a), vel * math.sin(a), p_inf
+
+
+def bc_tables(faces: np.ndarray, bc_type_per_face=None, fs=None, outflow_p=None):
+    """fr_solver.cpp:432-482: one 8-double entry per boundary face, face order."""
+    F = faces.shape[0]
+    bct = np.zeros(F, np.int32)
+    bmap = -np.ones(F, np.int32)
+    bnd = np.nonzero(faces[:, 2] < 0)[0]
+    fs = freestream() if fs is None else fs
+    data = np.zeros((max(len(bnd), 1), 8))
+    for n, f in enumerate(bnd):
+        t = BC_FARFIELD if bc_type_per_face is None else int(bc_type_per_face[f])
+        bct[f] = t
+        bmap[f] = n
+        if t == BC_FARFIELD:
+            data[n, :4] = fs
+        elif t == BC_OUTFLOW:
+            data[n, 0] = fs[3] if outflow_p is None else outflow_p
+    return bct, bmap, data.reshape(-1)
+
+
+def build_tables(xy, elem_nodes, p, periodic=None, bc_type_per_face=None, fs=None,
+                 node_count=None, want_pos=True) -> Tables:
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    ops = _lib.quad_operators(p)
+    faces = _lib.build_faces(en[:, :4], nv=4)
+    if periodic is not None:
+        faces = _lib.pair_periodic(faces, en[:, :4], xy, periodic[0], periodic[1])
+    normals = _lib.face_normals(faces, en[:, :4], xy, nv=4)
+    geo = _lib.geometry(en, xy, ops, node_count=node_count, want_pos=want_pos)
+    area = _lib.elem_areas(en[:, :4], xy, nv=4)
+    bct, bmap, bd = bc_tables(faces, bc_type_per_face, fs)
+    return Tables(p, ops, faces, normals, geo[0], geo[1], area, bct, bmap, bd,
+                  geo[2] if want_pos else None)
+
+
+class Solver:
+    def __init__(self, t: Tables, gamma=1.4, cfl=0.5, precision="fp64", positivity=True, device=0):
+        self.tables = t
+        self.p, self.E, self.S = t.p, t.E, t.S
+        self.n = self.E * self.S * 4
+        keep = []
+
+        def c(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+        fc = c(t.faces, np.int32)
+        le, ll, re, rl = (c(fc[:, k], np.int32) for k in range(4))
+        d = _lib.MeshDesc()
+        d.poly_order = t.p; d.n_elem = t.E; d.n_faces = fc.shape[0]
+        d.diff_xi = _ptr(c(t.ops["diff_xi"], np.float64)); d.diff_eta = _ptr(c(t.ops["diff_eta"], np.float64))
+        d.interp = _ptr(c(t.ops["interp"], np.float64)); d.corr = _ptr(c(t.ops["corr"], np.float64))
+        d.J = _ptr(c(t.J, np.float64)); d.Jinv = _ptr(c(t.Jinv, np.float64))
+        d.normals = _ptr(c(t.normals, np.float64))
+        d.face_left_elem = _ptr(le); d.face_left_local = _ptr(ll)
+        d.face_right_elem = _ptr(re); d.face_right_local = _ptr(rl)
+        d.bc_type = _ptr(c(t.bc_type, np.int32)); d.bc_face_map = _ptr(c(t.bc_face_map, np.int32))
+        bd = c(t.bc_data, np.float64)
+        d.bc_data = _ptr(bd); d.n_bc_data = bd.size // 8
+        d.elem_area = _ptr(c(t.area, np.float64))
+        prm = _lib.Params(gamma, cfl, PRECISIONS[precision], _lib.ZJ_COMPAT_REFERENCE, int(positivity), device)
+        h = C.c_void_p()
+        check(lib().zj_create(C.byref(h), C.byref(d), C.byref(prm)))
+        self._h = h
+        self.precision = precision
+
+    @classmethod
+    def from_mesh(cls, mesh, p, periodic=True, **kw):
+        t = build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly) if periodic else None)
+        return cls(t, **kw)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().zj_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _chk(self, rc):
+        return check(rc, self._h)
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, np.float64).reshape(-1)
+        assert U.size == self.n
+        self._chk(lib().zj_set_state(self._h, U))
+
+    def get_state(self):
+        out = np.zeros(self.n)
+        self._chk(lib().zj_get_state(self._h, out))
+        return out
+
+    def residual(self, fetch=True, stream=None):
+        out = np.zeros(self.n) if fetch else None
+        self._chk(lib().zj_residual(self._h, None if out is None else out.ctypes.data, stream))
+        return out
+
+    def rk3_step(self, dt, stream=None):
+        self._chk(lib().zj_rk3_step(self._h, float(dt), stream))
+
+    def rk3_stage(self, stage, dt, stream=None):
+        self._chk(lib().zj_rk3_stage(self._h, int(stage), float(dt), stream))
+
+    def dt_cfl(self, cfl):
+        v = C.c_double()
+        self._chk(lib().zj_dt_cfl(self._h, float(cfl), C.byref(v)))
+        return v.value
+
+    def residual_l2(self):
+        v = C.c_double()
+        self._chk(lib().zj_residual_l2(self._h, C.byref(v)))
+        return v.value
+
+    def count_nonfinite(self):
+        v = C.c_int64()
+        self._chk(lib().zj_check_finite(self._h, C.byref(v)))
+        return v.value
+
+    def set_tile(self, epb):
+        self._chk(lib().zj_set_tile(self._h, int(epb)))
+
+    def info(self):
+        n = C.c_int64(); e = C.c_int(); s = C.c_int()
+        self._chk(lib().zj_get_info(self._h, C.byref(n), C.byref(e), C.byref(s)))
+        return dict(n_dof=n.value, elems_per_block=e.value, n_sp=s.value)
