@@ -135,6 +135,13 @@ int zj_check_finite(zj_handle h, int64_t* n_bad);
 int zj_set_tile(zj_handle h, int elems_per_block);
 int zj_get_info(zj_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp);
 
+/* ---- measurement (no reference counterpart) -------------------------- */
+/* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times
+ * with a plain grid-stride copy kernel and reports the mean GB/s (read + write
+ * bytes over kernel time, hipEvents).  Used to calibrate HBM counters and the
+ * achievable bandwidth next to the stage kernels' roofline. */
+int zj_copy_probe(int device, int64_t bytes, int reps, double* gbps);
+
 /* ---- host mesh and operator tables (native, used by FRSolver) -------- */
 int zj_quad_operators(int poly_order, double* sol_xi, double* sol_eta, double* quad_w,
                       double* diff_xi, double* diff_eta, double* interp, double* corr);

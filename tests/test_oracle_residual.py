@@ -157,6 +157,31 @@ def test_freestream_farfield_reference_mesh(golden_meshes):
     assert np.abs(R).max() < 1e-8 * pinf / 0.1
 
 
+def _jacobian(s, U0, eps=1e-7):
+    n = U0.size
+    Jm = np.zeros((n, n))
+    for k in range(n):
+        d = np.zeros(n); d[k] = eps
+        Jm[:, k] = (s.residual(U0 + d) - s.residual(U0 - d)) / (2 * eps)
+    return Jm
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_reference_scheme_is_unstable(p):
+    """A property of the reference's executed arithmetic, recorded because it
+    bounds every multi-step comparison: linearised about a uniform flow on a
+    periodic mesh, the residual has eigenvalues with Re(lambda) of the order of
+    |lambda| (the g_L terms carry the wrong sign, kernels.cu:1009-1020).  Any
+    round-off difference therefore grows like exp(Re(lambda) t)."""
+    mesh = cases.uniform_quad_mesh(3, 3, 0.0, 1.0, 0.0, 1.0)
+    case = fr_case.from_quadmesh(mesh, p)
+    s = case.solver()
+    U0 = np.tile(cases.prim_to_cons(np.array(1.0), np.array(0.5), np.array(0.2), np.array(1 / 1.4), 1.4),
+                 (case.E * case.S, 1)).reshape(-1)
+    lam = np.linalg.eigvals(_jacobian(s, U0))
+    assert lam.real.max() > 0.5 * np.abs(lam).max()
+
+
 def test_residual_l2_is_sequential_sum():
     mesh = cases.vortex_mesh(4)
     case = fr_case.from_quadmesh(mesh, 2)

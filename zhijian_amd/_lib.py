@@ -51,7 +51,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_set_state_device", "zj_get_state_device", "zj_residual", "zj_rk3_step",
            "zj_rk3_stage", "zj_dt_cfl", "zj_residual_l2", "zj_check_finite", "zj_set_tile",
            "zj_get_info", "zj_quad_operators", "zj_build_faces", "zj_pair_periodic",
-           "zj_face_normals", "zj_elem_areas", "zj_geometry")
+           "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe")
 
 
 def lib() -> C.CDLL:
@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
                                        C.c_double, C.c_double, C.c_double]
         L.zj_face_normals.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_int, _lp, _dp, _dp]
         L.zj_elem_areas.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, _dp]
+        L.zj_copy_probe.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.zj_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
         _lib = L
     return _lib
@@ -164,3 +165,10 @@ def geometry(elem_nodes, xy, ops, node_count=None, want_pos=False):
     if want_pos:
         out = out + (pos[: E * S * 2].reshape(E, S, 2),)
     return out
+
+
+def copy_probe(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Achievable HBM bandwidth (GB/s) of a plain copy kernel."""
+    v = C.c_double()
+    check(lib().zj_copy_probe(device, int(nbytes), int(reps), C.byref(v)))
+    return v.value

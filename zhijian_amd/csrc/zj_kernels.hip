@@ -38,9 +38,15 @@ template <typename R> __device__ __forceinline__ R fmaR(R a, R b, R c);
 template <> __device__ __forceinline__ double fmaR<double>(double a, double b, double c) { return fma(a, b, c); }
 template <> __device__ __forceinline__ float fmaR<float>(float a, float b, float c) { return fmaf(a, b, c); }
 
+// FP64 keeps correctly rounded square root and division (the plain form must
+// track the reference to 1e-12); the FP32 bulk uses the 1-ulp hardware
+// instructions, whose error sits far below the FP32 rounding of the state.
 template <typename R> __device__ __forceinline__ R sqrtR(R x);
 template <> __device__ __forceinline__ double sqrtR<double>(double x) { return sqrt(x); }
-template <> __device__ __forceinline__ float sqrtR<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ float sqrtR<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename R> __device__ __forceinline__ R rcpR(R x);
+template <> __device__ __forceinline__ double rcpR<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float rcpR<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename R> __device__ __forceinline__ R fmaxR(R a, R b);
 template <> __device__ __forceinline__ double fmaxR<double>(double a, double b) { return fmax(a, b); }
@@ -71,7 +77,7 @@ __device__ __forceinline__ void store4(float* p, const float* v) {
 // IdealGas::fluxX/fluxY (euler.hpp:101-129), no clamps.
 template <typename R>
 __device__ __forceinline__ void euler_flux(const R* U, R g, R* Fx, R* Fy) {
-    const R inv = R(1) / U[0];
+    const R inv = rcpR(U[0]);
     const R u = U[1] * inv, v = U[2] * inv, E = U[3] * inv;
     const R p = (g - R(1)) * U[0] * (E - R(0.5) * (u * u + v * v));
     Fx[0] = U[1]; Fx[1] = U[1] * u + p; Fx[2] = U[1] * v; Fx[3] = (U[3] + p) * u;
@@ -91,7 +97,7 @@ template <typename R>
 __device__ __forceinline__ void rusanov(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
     const R tiny = R(1e-10);
     const R rhoL = fmaxR(UL[0], tiny), rhoR = fmaxR(UR[0], tiny);
-    const R iL = R(1) / rhoL, iR = R(1) / rhoR;
+    const R iL = rcpR(rhoL), iR = rcpR(rhoR);
     const R uL = UL[1] * iL, vL = UL[2] * iL, uR = UR[1] * iR, vR = UR[2] * iR;
     const R EL = UL[3] * iL, ER = UR[3] * iR;
     const R qL = uL * uL + vL * vL, qR = uR * uR + vR * vR;
@@ -213,18 +219,26 @@ __device__ __forceinline__ double block_reduce(double v, double* sred) {
 // (eta=+1) run along xi, edges 1 (xi=+1) and 3 (xi=-1) along eta; edges 2 and
 // 3 are stored in reversed order (basis.cpp:540-552).  The explicit FMA chain
 // fixes the rounding, so both elements of a face see identical traces.
-template <typename R, int P>
-__device__ __forceinline__ R trace_point(const R* u, int k, int f, const R* Lm, const R* Lp) {
+// Where flux point f of edge k reads its line of solution points: first point
+// `base`, step `stride`, weights Lm (edges 0, 3) or Lp (edges 1, 2).  Computed
+// once per thread without branches.
+struct TraceLine {
+    int base, stride, plus;
+};
+template <int P>
+__device__ __forceinline__ TraceLine trace_line(int k, int f) {
     constexpr int N1 = P + 1;
-    const R* L = (k == 1 || k == 2) ? Lp : Lm;
-    int base, stride;
-    if (k == 0) { base = f; stride = N1; }
-    else if (k == 1) { base = f * N1; stride = 1; }
-    else if (k == 2) { base = P - f; stride = N1; }
-    else { base = (P - f) * N1; stride = 1; }
-    R acc = L[0] * u[base];
+    const int ff = k >= 2 ? P - f : f;
+    const bool along_xi = (k & 1) != 0;
+    return TraceLine{along_xi ? ff * N1 : ff, along_xi ? 1 : N1, (k == 1 || k == 2) ? 1 : 0};
+}
+template <typename R, int P>
+__device__ __forceinline__ R trace_point(const R* u, const TraceLine& t, const R* Lm, const R* Lp) {
+    constexpr int N1 = P + 1;
+    const R* L = t.plus ? Lp : Lm;
+    R acc = L[0] * u[t.base];
 #pragma unroll
-    for (int m = 1; m < N1; ++m) acc = fmaR(L[m], u[base + m * stride], acc);
+    for (int m = 1; m < N1; ++m) acc = fmaR(L[m], u[t.base + m * t.stride], acc);
     return acc;
 }
 
@@ -272,8 +286,9 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     for (int t = tid; t < NF; t += BLOCK) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         if (e0 + el2 < a.E) {
+            const TraceLine tl = trace_line<P>(k, f);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sTr[el2][k][f][v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+            for (int v = 0; v < NV; ++v) sTr[el2][k][f][v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
         }
     }
     __syncthreads();
@@ -287,29 +302,24 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         R Fd[NV] = {R(0), R(0), R(0), R(0)};
         if (L.role != ROLE_NONE) {
             const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
-            R own[NV], other[NV], Fc[NV], Fn[NV];
+            const bool right = L.role == ROLE_RIGHT;
+            R own[NV], mirr[NV], other[NV], UL[NV], UR[NV], Fc[NV], Fn[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) own[v] = sTr[el2][k][f][v];
+            for (int v = 0; v < NV; ++v) { own[v] = sTr[el2][k][f][v]; mirr[v] = sTr[el2][k][P - f][v]; }
+            // The face's common flux as its left element computes it: U_L = left
+            // slot f, U_R = right slot Fp-1-f (kernels.cu:577).  One call site
+            // serves both roles so a wave never runs the flux twice.
+            if (L.role == ROLE_BOUNDARY)
+                ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
+            else
+                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (right ? f : P - f)) * NV, other);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
+            rusanov(UL, UR, nx, ny, g, Fc);
             normal_flux(own, g, nx, ny, Fn);
-            if (L.role == ROLE_RIGHT) {
-                // common flux of the face as its left element computes it:
-                // U_L = left slot f, U_R = this element's slot Fp-1-f
-                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + f) * NV, other);
-                R mine[NV];
+            // left: F_com - F(U).n; right: -F_com + F(U).n at its own slot f (kernels.cu:397,423)
 #pragma unroll
-                for (int v = 0; v < NV; ++v) mine[v] = sTr[el2][k][P - f][v];
-                rusanov(other, mine, nx, ny, g, Fc);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Fd[v] = -Fc[v] + Fn[v];
-            } else {
-                if (L.role == ROLE_LEFT)
-                    load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (P - f)) * NV, other);
-                else
-                    ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
-                rusanov(own, other, nx, ny, g, Fc);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) Fd[v] = Fc[v] - Fn[v];
-            }
+            for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) sFd[el2][k][f][v] = Fd[v];
@@ -365,25 +375,9 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
             }
         }
     }
-    if (a.l2_partial) {
+    if (a.l2_partial) {  // one partial sum of R^2 per workgroup; reduced on request
         const double s = block_reduce<OpSum>(rsq, sRed);
-        __shared__ unsigned int sLast;
-        if (tid == 0) {
-            a.l2_partial[blockIdx.x] = s;
-            __threadfence();
-            sLast = (atomicAdd(a.ticket, 1u) == gridDim.x - 1) ? 1u : 0u;
-        }
-        __syncthreads();
-        if (sLast) {  // every other workgroup's partial is visible: reduce in a fixed order
-            __threadfence();
-            double acc = 0.0;
-            for (int b = tid; b < (int)gridDim.x; b += BLOCK) acc += a.l2_partial[b];
-            const double tot = block_reduce<OpSum>(acc, sRed);
-            if (tid == 0) {
-                *a.l2_out = sqrt(tot);
-                *a.ticket = 0u;  // ready for the next launch
-            }
-        }
+        if (tid == 0) a.l2_partial[blockIdx.x] = s;
     }
     if (MODE == MODE_RESIDUAL) return;
     __syncthreads();
@@ -393,9 +387,10 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
+        const TraceLine tl = trace_line<P>(k, f);
         R tr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
         store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
     }
 }
@@ -419,9 +414,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, cons
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
+        const TraceLine tl = trace_line<P>(k, f);
         R tr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], k, f, sLm, sLp);
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
         store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
     }
 }

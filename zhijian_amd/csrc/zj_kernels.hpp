@@ -51,8 +51,6 @@ struct StageArgs {
     const double* bc_data;   // [n][8]
     const double* dt_ptr;    // device scalar
     double* l2_partial;      // [grid] sum of R^2 per workgroup, or nullptr
-    unsigned int* ticket;    // workgroups done; the last one reduces the partials
-    double* l2_out;          // sqrt of the summed partials (written by the last workgroup)
     double gamma;
     int positivity;
 };

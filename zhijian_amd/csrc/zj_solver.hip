@@ -44,9 +44,10 @@ struct zj_solver {
     double* bc_data = nullptr;
     double* hcell = nullptr;
     double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl
-    double* partial = nullptr;
+    double* partial = nullptr;     // per-workgroup sums of R^2 of the last residual / stage 3
+    double* partial_dt = nullptr;  // per-workgroup minima of the CFL dt
+    int l2_blocks = 0;             // valid entries of `partial` (0: no residual yet)
     unsigned long long* d_count = nullptr;
-    unsigned int* d_ticket = nullptr;
     double* h_pinned = nullptr;
     double dt_on_device = NAN;   // last dt written to d_scalars[0]
     OpTables<double> ops64{};
@@ -135,7 +136,7 @@ static void release(zj_solver* h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);
     void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->T[0], h->T[1], h->T[2], h->links,
-                    h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->d_count, h->d_ticket};
+                    h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -241,10 +242,9 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
     h->nred = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)npts + BLOCK - 1) / BLOCK));
     const int npart = std::max(std::max(h->nblk, num_blocks(std::max(E, 1), 1)), h->nred);
     ZJ_HIP(h, hipMalloc((void**)&h->partial, (size_t)npart * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->partial_dt, (size_t)h->nred * sizeof(double)));
     ZJ_HIP(h, hipMalloc((void**)&h->d_scalars, 8 * sizeof(double)));
     ZJ_HIP(h, hipMalloc((void**)&h->d_count, sizeof(unsigned long long)));
-    ZJ_HIP(h, hipMalloc((void**)&h->d_ticket, sizeof(unsigned int)));
-    ZJ_HIP(h, hipMemset(h->d_ticket, 0, sizeof(unsigned int)));
     ZJ_HIP(h, hipHostMalloc((void**)&h->h_pinned, 8 * sizeof(double), hipHostMallocDefault));
     ZJ_HIP(h, hipMemset(h->d_scalars, 0, 8 * sizeof(double)));
     ZJ_HIP(h, hipDeviceSynchronize());
@@ -296,8 +296,6 @@ static StageArgs<R> stage_args(zj_solver* h, int mode) {
     a.normals = (const R*)h->normals;
     a.bc_data = h->bc_data;
     a.dt_ptr = h->d_scalars;
-    a.ticket = h->d_ticket;
-    a.l2_out = h->d_scalars + 1;
     a.gamma = h->gamma;
     a.positivity = h->positivity;
     R** T = (R**)h->T;
@@ -325,6 +323,7 @@ static hipError_t enqueue_stage(zj_solver* h, int mode, hipStream_t s) {
         e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, s);
     else
         e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, s);
+    if (e == hipSuccess && (mode == MODE_RESIDUAL || mode == MODE_STAGE3)) h->l2_blocks = h->nblk;
     return e;
 }
 
@@ -458,9 +457,9 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
     if (!h->has_area) return fail(h, ZJ_ERR_STATE, "element areas were not provided");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "dt requested in the middle of a step");
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial,
+    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial_dt,
                                 h->nred, h->stream));
-    ZJ_HIP(h, launch_finalize(h->partial, h->nred, 1, h->d_scalars + 2, h->stream));
+    ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, 1, h->d_scalars + 2, h->stream));
     ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     *dt = h->h_pinned[0];
@@ -469,9 +468,12 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
 
 extern "C" int zj_residual_l2(zj_handle h, double* l2) {
     if (!h || !l2) return ZJ_ERR_ARG;
+    if (h->l2_blocks == 0) return fail(h, ZJ_ERR_STATE, "no residual has been evaluated");
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, hipDeviceSynchronize());
-    ZJ_HIP(h, hipMemcpy(h->h_pinned, h->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost));
+    ZJ_HIP(h, hipDeviceSynchronize());  // the last residual may have run on a caller's stream
+    ZJ_HIP(h, launch_finalize(h->partial, h->l2_blocks, 0, h->d_scalars + 1, h->stream));
+    ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
     *l2 = h->h_pinned[0];
     return ZJ_OK;
 }
@@ -510,4 +512,50 @@ extern "C" int zj_get_info(zj_handle h, int64_t* n_dof, int* epb, int* n_sp) {
     if (epb) *epb = h->epb;
     if (n_sp) *n_sp = h->S;
     return ZJ_OK;
+}
+
+// ------------------------------------------------------------ measurement
+namespace {
+__global__ __launch_bounds__(256) void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                   int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int zj_copy_probe(int device, int64_t bytes, int reps, double* gbps) {
+    if (bytes < 32 || reps < 1 || !gbps) return ZJ_ERR_ARG;
+    zj_solver tmp;  // only for error text plumbing
+    zj_solver* h = &tmp;
+    const int64_t n = bytes / 32;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ZJ_OK;
+    auto fin = [&](int code) {
+        if (a) hipFree(a);
+        if (b) hipFree(b);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (code) g_create_error = h->err;
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess || hipMalloc(&a, n * 32) != hipSuccess ||
+        hipMalloc(&b, n * 32) != hipSuccess || hipMemset(a, 0, n * 32) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        h->err = "copy probe: allocation failed";
+        return fin(ZJ_ERR_HIP);
+    }
+    const int grid = 256 * 8 * 4;
+    copy_kernel<<<grid, 256>>>((const double4*)a, (double4*)b, n);  // warm-up
+    hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) copy_kernel<<<grid, 256>>>((const double4*)a, (double4*)b, n);
+    hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+        h->err = "copy probe: kernel failed";
+        return fin(ZJ_ERR_HIP);
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    *gbps = 2.0 * (double)n * 32.0 * reps / (ms * 1e-3) / 1e9;
+    return fin(rc);
 }
