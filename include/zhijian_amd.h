@@ -124,6 +124,8 @@ int zj_rk3_step(zj_handle h, double dt, void* stream);
 /* One stage (1, 2 or 3) of the step, for callers that time or interleave them.
  * Stage 1 must come first; the state reads as U^n again only after stage 3. */
 int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream);
+/* Waits for the work queued on the handle's own stream. */
+int zj_synchronize(zj_handle h);
 /* CFL time step of the current state (synchronises). */
 int zj_dt_cfl(zj_handle h, double cfl, double* dt);
 /* L2 norm of the last residual: that of zj_residual, or of stage 3 of the last
@@ -131,6 +133,20 @@ int zj_dt_cfl(zj_handle h, double cfl, double* dt);
 int zj_residual_l2(zj_handle h, double* l2);
 /* Number of non-finite values in the current state (synchronises). */
 int zj_check_finite(zj_handle h, int64_t* n_bad);
+/* Replaces the boundary tables given at zj_create (same layout: bc_type and
+ * bc_face_map per face of the creation face list, 8 doubles of bc_data per
+ * entry).  FRSolver::computeInviscidFlux_GPU rebuilds these tables at every
+ * residual (fr_solver.cpp:432-494); here they change only when re-set. */
+int zj_set_boundary(zj_handle h, const int* bc_type, const int* bc_face_map, const double* bc_data,
+                    int n_bc_data);
+/* Elementwise SSP-RK3 stage combination on device arrays of n doubles, as
+ * gpu::rkStage1/2/3 (kernels.cu:1297-1349) without the limiter:
+ *   stage 1: out = U0 + dt R;  stage 2: out = 3/4 U0 + 1/4 Uk + 1/4 dt R;
+ *   stage 3: out = 1/3 U0 + 2/3 Uk + 2/3 dt R   (Uk unused for stage 1). */
+int zj_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
+                   int64_t n, double dt, void* stream);
+/* U += dt R on device arrays (ForwardEuler::advance_GPU). */
+int zj_euler_update(double* U, const double* R, int64_t n, double dt, void* stream);
 /* Launch configuration of the stage kernel (elements per workgroup). */
 int zj_set_tile(zj_handle h, int elems_per_block);
 int zj_get_info(zj_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp);

@@ -1,0 +1,139 @@
+// frsolver_driver.cpp -- exercises the C++ drop-in API the way the reference's
+// main() drives its solver (src/main.cpp:291-416): build a mesh, initialize,
+// set an initial condition, then per iteration dt = computeTimeStep(),
+// dt = min(dt, params.dt), advance(dt), residual = computeResidual().
+//
+//   frsolver_driver mesh   <n> <out.bin>
+//       (no GPU) faces and normals of the periodic n x n unit mesh
+//   frsolver_driver run    <vortex|tgv> <n> <p> <steps> <dt_max> <fp64|fp32bulk> <prefix>
+//       writes <prefix>_U0.bin, <prefix>_U.bin and one JSON line on stdout
+#include "zhijian/fr_solver.hpp"
+#include "zhijian/rk_integrator.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace zhijian;
+
+static Mesh uniform_mesh(int n, Real x0, Real x1) {
+    Mesh m;
+    const Real h = (x1 - x0) / n;
+    for (int j = 0; j <= n; ++j)
+        for (int i = 0; i <= n; ++i) m.nodes().push_back(Vec2(x0 + i * h, x0 + j * h));
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            Element e;
+            e.type = ElementType::Quadrilateral;
+            e.order = 1;
+            const Index n0 = (Index)j * (n + 1) + i;
+            e.node_ids = {n0, n0 + 1, n0 + n + 2, n0 + n + 1};
+            m.elements().push_back(e);
+        }
+    m.buildFaces();
+    m.pairPeriodic(x1 - x0, x1 - x0);
+    m.computeGeometry();
+    return m;
+}
+
+static void dump(const std::string& path, const std::vector<double>& v) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f || std::fwrite(v.data(), sizeof(double), v.size(), f) != v.size()) {
+        std::fprintf(stderr, "cannot write %s\n", path.c_str());
+        std::exit(2);
+    }
+    std::fclose(f);
+}
+
+static std::vector<double> state(const FRSolver& s) {
+    std::vector<double> U;
+    for (const ElementSolution& e : s.solution())
+        for (const State& q : e.sol_pts)
+            for (int v = 0; v < N_VARS; ++v) U.push_back(q[v]);
+    return U;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 4 && std::strcmp(argv[1], "mesh") == 0) {
+        Mesh m = uniform_mesh(std::atoi(argv[2]), 0.0, 1.0);
+        std::vector<double> out;
+        for (Index f = 0; f < m.numFaces(); ++f) {
+            const Face& fc = m.face(f);
+            const Vec2 n = m.faceNormal(f);
+            out.insert(out.end(), {(double)fc.left_elem, (double)fc.left_face, (double)fc.right_elem,
+                                   (double)fc.right_face, n.x, n.y});
+        }
+        dump(argv[3], out);
+        std::printf("{\"faces\": %lld}\n", (long long)m.numFaces());
+        return 0;
+    }
+    if (argc < 9 || std::strcmp(argv[1], "run") != 0) {
+        std::fprintf(stderr, "usage: %s mesh <n> <out> | run <vortex|tgv> <n> <p> <steps> <dt_max> <prec> <prefix>\n",
+                     argv[0]);
+        return 2;
+    }
+    const std::string kind = argv[2];
+    const int n = std::atoi(argv[3]), p = std::atoi(argv[4]), steps = std::atoi(argv[5]);
+    const double dt_max = std::atof(argv[6]);
+    const std::string prec = argv[7], prefix = argv[8];
+    const bool vortex = kind == "vortex";
+    Mesh mesh = vortex ? uniform_mesh(n, -5.0, 5.0) : uniform_mesh(n, 0.0, 2.0 * M_PI);
+
+    SimParams params;
+    params.poly_order = p;
+    params.gamma = 1.4;
+    params.CFL = 0.5;
+    params.dt = dt_max;
+    params.viscous = false;
+    params.flux_type = FluxType::DG;
+    params.riemann = RiemannSolver::Rusanov;
+    params.precision = prec == "fp32bulk" ? Precision::FP32Bulk : Precision::FP64;
+
+    FRSolver solver;
+    solver.initialize(mesh, params);
+    const Real g = params.gamma;
+    IdealGas gas(g);
+    if (vortex) {
+        const Real beta = 5.0;
+        solver.setInitialCondition([&](Vec2 x) {
+            const Real r2 = x.x * x.x + x.y * x.y, f = std::exp(0.5 * (1.0 - r2));
+            const Real T = 1.0 - (g - 1.0) * beta * beta / (8.0 * g * M_PI * M_PI) * std::exp(1.0 - r2);
+            const Real rho = std::pow(T, 1.0 / (g - 1.0));
+            return gas.primToConserv(rho, 1.0 - beta / (2 * M_PI) * x.y * f, beta / (2 * M_PI) * x.x * f,
+                                     std::pow(rho, g));
+        });
+    } else {
+        const Real mach = 0.1;
+        solver.setInitialCondition([&](Vec2 x) {
+            return gas.primToConserv(1.0, std::sin(x.x) * std::cos(x.y), -std::cos(x.x) * std::sin(x.y),
+                                     1.0 / (g * mach * mach) + 0.25 * (std::cos(2 * x.x) + std::cos(2 * x.y)));
+        });
+    }
+    solver.copyToHost();
+    dump(prefix + "_U0.bin", state(solver));
+
+    std::vector<double> dts;
+    Real residual = 0.0;
+    for (int it = 0; it < steps; ++it) {
+        Real dt = solver.computeTimeStep();
+        if (params.dt > 0) dt = std::min(dt, params.dt);
+        solver.advance(dt);
+        residual = solver.computeResidual();
+        dts.push_back(dt);
+    }
+    solver.copyToHost();
+    dump(prefix + "_U.bin", state(solver));
+    std::vector<std::vector<Real>> pr, ma;
+    solver.computeDerivedQuantities(pr, ma);
+
+    std::printf("{\"steps\": %d, \"iteration\": %d, \"time\": %.17g, \"residual\": %.17g, \"E\": %lld, \"S\": %d, "
+                "\"p00\": %.17g, \"dt\": [",
+                steps, solver.iteration(), solver.currentTime(), residual, (long long)mesh.numElements(),
+                (int)solver.solution()[0].sol_pts.size(), pr[0][0]);
+    for (size_t k = 0; k < dts.size(); ++k) std::printf("%s%.17g", k ? ", " : "", dts[k]);
+    std::printf("]}\n");
+    return 0;
+}

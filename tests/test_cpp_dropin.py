@@ -1,0 +1,72 @@
+"""The C++ drop-in API (include/zhijian/fr_solver.hpp) through a driver program
+that uses it the way the reference's main() does (tests/cpp/frsolver_driver.cpp).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests import fr_case
+from tests.conftest import ROOT
+from zhijian_amd import cases
+
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "frsolver_driver")
+
+
+def _need_driver():
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} missing: run `python -m zhijian_amd.build`")
+
+
+def test_mesh_api_matches_oracle(tmp_path):
+    """Mesh::buildFaces + pairPeriodic + faceNormal (C++ API, no GPU) against the oracle."""
+    _need_driver()
+    out = tmp_path / "m.bin"
+    subprocess.run([DRIVER, "mesh", "7", str(out)], check=True, capture_output=True)
+    got = np.fromfile(out, dtype=np.float64).reshape(-1, 6)
+    mesh = cases.uniform_quad_mesh(7, 7, 0.0, 1.0, 0.0, 1.0)
+    f = O.pair_periodic(O.build_faces(mesh.elem_nodes), mesh.elem_nodes, mesh.xy, 1.0, 1.0, 1e-9)
+    assert np.array_equal(got[:, :4].astype(np.int64), f.as_array())
+    assert np.allclose(got[:, 4:], O.face_normals(f, mesh.elem_nodes, mesh.xy), rtol=0, atol=1e-15)
+
+
+def _run(tmp_path, kind, n, p, steps, dt_max, prec):
+    prefix = str(tmp_path / f"{kind}{n}p{p}")
+    print(f"driver: {kind} n={n} P{p} steps={steps} {prec}", flush=True)
+    r = subprocess.run([DRIVER, "run", kind, str(n), str(p), str(steps), repr(dt_max), prec, prefix],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    U0 = np.fromfile(prefix + "_U0.bin", dtype=np.float64)
+    U = np.fromfile(prefix + "_U.bin", dtype=np.float64)
+    return info, U0, U
+
+
+@pytest.mark.gpu
+def test_frsolver_vortex_matches_oracle(tmp_path):
+    """C1 geometry (10x10 on [-5,5]^2, P3), dt = min(CFL dt, 1e-3), 10 steps
+    (inside the reference scheme's pre-blow-up window)."""
+    _need_driver()
+    info, U0, U = _run(tmp_path, "vortex", 10, 3, 10, 1e-3, "fp64")
+    assert info["iteration"] == 10 and len(info["dt"]) == 10
+    mesh = cases.vortex_mesh(10)
+    o = fr_case.from_quadmesh(mesh, 3).solver()
+    V = U0.copy()
+    for dt in info["dt"]:
+        assert dt <= 1e-3
+        V = o.advance(V, dt, True)
+    assert np.linalg.norm(U - V) / np.linalg.norm(V) < 1e-12
+    assert abs(info["residual"] - o.residual_l2()) <= 1e-12 * o.residual_l2()
+    assert abs(info["time"] - sum(info["dt"])) < 1e-15
+
+
+@pytest.mark.gpu
+def test_frsolver_tgv_fp32bulk_runs(tmp_path):
+    _need_driver()
+    info, U0, U = _run(tmp_path, "tgv", 32, 4, 3, 0.0, "fp32bulk")
+    _, _, U64 = _run(tmp_path, "tgv", 32, 4, 3, 0.0, "fp64")
+    assert np.all(np.isfinite(U))
+    assert np.linalg.norm(U - U64) / np.linalg.norm(U64) < 1e-5

@@ -51,7 +51,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_set_state_device", "zj_get_state_device", "zj_residual", "zj_rk3_step",
            "zj_rk3_stage", "zj_dt_cfl", "zj_residual_l2", "zj_check_finite", "zj_set_tile",
            "zj_get_info", "zj_quad_operators", "zj_build_faces", "zj_pair_periodic",
-           "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe")
+           "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe", "zj_set_boundary",
+           "zj_rk3_combine", "zj_euler_update", "zj_synchronize")
 
 
 def lib() -> C.CDLL:
@@ -83,6 +84,11 @@ def lib() -> C.CDLL:
                                        C.c_double, C.c_double, C.c_double]
         L.zj_face_normals.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_int, _lp, _dp, _dp]
         L.zj_elem_areas.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, _dp]
+        L.zj_set_boundary.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.zj_rk3_combine.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.c_double, C.c_void_p]
+        L.zj_euler_update.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_void_p]
+        L.zj_synchronize.argtypes = [H]
         L.zj_copy_probe.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.zj_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
         _lib = L

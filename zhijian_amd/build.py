@@ -23,10 +23,13 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-function"]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", INC, "-I", CSRC, "-Wall"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", INC, "-I", CSRC, "-Wall",
+             "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include"]
 
 HIP_SRCS = ["zj_kernels.hip", "zj_solver.hip"]
-CXX_SRCS = ["zj_host.cpp"]
+CXX_SRCS = ["zj_host.cpp", "zj_frsolver.cpp"]
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "frsolver_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "frsolver_driver")
 
 
 def _run(cmd, verbose):
@@ -72,6 +75,11 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    # the C++ drop-in driver (links the library by rpath)
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", INC,
+          DRIVER_SRC, "-o", DRIVER, "-L", PKG, "-lzhijian_amd",
+          "-Wl,-rpath," + os.path.relpath(PKG, os.path.dirname(DRIVER)).join(["$ORIGIN/", ""])], verbose)
     return LIB
 
 

@@ -9,6 +9,7 @@
 //   normals     mesh.cpp:159-183;  areas mesh.cpp:122-146
 //   geometry    src/solver/fr_solver.cpp:85-129, basis.cpp:717-863
 #include "zhijian_amd.h"
+#include "zj_host_math.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -17,128 +18,7 @@
 #include <numeric>
 #include <vector>
 
-namespace {
-
-// 1D Gauss-Legendre rule with the reference's closed forms (n <= 5), its
-// 15-digit table (n = 6) and Newton iteration beyond.
-void gauss_rule(int n, double* x, double* w) {
-    switch (n) {
-    case 1: x[0] = 0.0; w[0] = 2.0; return;
-    case 2: {
-        double a = 1.0 / std::sqrt(3.0);
-        x[0] = -a; x[1] = a; w[0] = w[1] = 1.0;
-        return;
-    }
-    case 3: {
-        double a = std::sqrt(3.0 / 5.0);
-        x[0] = -a; x[1] = 0.0; x[2] = a;
-        w[0] = w[2] = 5.0 / 9.0; w[1] = 8.0 / 9.0;
-        return;
-    }
-    case 4: {
-        double r = std::sqrt(6.0 / 5.0);
-        double a = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * r), b = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * r);
-        x[0] = -a; x[1] = -b; x[2] = b; x[3] = a;
-        double s = std::sqrt(30.0);
-        w[0] = w[3] = (18.0 - s) / 36.0; w[1] = w[2] = (18.0 + s) / 36.0;
-        return;
-    }
-    case 5: {
-        double r = std::sqrt(10.0 / 7.0);
-        double a = std::sqrt(5.0 + 2.0 * r) / 3.0, b = std::sqrt(5.0 - 2.0 * r) / 3.0;
-        x[0] = -a; x[1] = -b; x[2] = 0.0; x[3] = b; x[4] = a;
-        double s = std::sqrt(70.0);
-        w[0] = w[4] = (322.0 - 13.0 * s) / 900.0;
-        w[1] = w[3] = (322.0 + 13.0 * s) / 900.0;
-        w[2] = 128.0 / 225.0;
-        return;
-    }
-    case 6: {
-        static const double X[3] = {0.932469514203152, 0.661209386466265, 0.238619186083197};
-        static const double W[3] = {0.171324492379170, 0.360761573048139, 0.467913934572691};
-        for (int i = 0; i < 3; ++i) {
-            x[i] = -X[i]; x[5 - i] = X[i]; w[i] = w[5 - i] = W[i];
-        }
-        return;
-    }
-    default:
-        for (int i = 0; i < n; ++i) {
-            double t = std::cos(M_PI * (i + 0.75) / (n + 0.5));
-            auto leg = [n](double s, double& Pn, double& Pm) {
-                double a = 1.0, b = s;
-                for (int j = 2; j <= n; ++j) {
-                    double c = ((2 * j - 1) * s * b - (j - 1) * a) / j;
-                    a = b; b = c;
-                }
-                Pn = b; Pm = a;
-            };
-            for (int it = 0; it < 20; ++it) {
-                double Pn, Pm;
-                leg(t, Pn, Pm);
-                double d = n * (t * Pn - Pm) / (t * t - 1.0);
-                double dx = Pn / d;
-                t -= dx;
-                if (std::fabs(dx) < 1e-15) break;
-            }
-            double Pn, Pm;
-            leg(t, Pn, Pm);
-            double d = n * (t * Pn - Pm) / (t * t - 1.0);
-            x[i] = t;
-            w[i] = 2.0 / ((1.0 - t * t) * d * d);
-        }
-    }
-}
-
-// Lagrange basis on `nodes` (product and sum-of-products forms).
-double lag(const double* nodes, int n, int j, double x) {
-    double r = 1.0;
-    for (int i = 0; i < n; ++i)
-        if (i != j) r *= (x - nodes[i]) / (nodes[j] - nodes[i]);
-    return r;
-}
-
-double lag_d(const double* nodes, int n, int j, double x) {
-    double r = 0.0;
-    for (int k = 0; k < n; ++k) {
-        if (k == j) continue;
-        double t = 1.0 / (nodes[j] - nodes[k]);
-        for (int i = 0; i < n; ++i)
-            if (i != j && i != k) t *= (x - nodes[i]) / (nodes[j] - nodes[i]);
-        r += t;
-    }
-    return r;
-}
-
-double legendre_p(int n, double x) {
-    if (n == 0) return 1.0;
-    if (n == 1) return x;
-    double a = 1.0, b = x;
-    for (int i = 2; i <= n; ++i) {
-        double c = ((2 * i - 1) * x * b - (i - 1) * a) / i;
-        a = b; b = c;
-    }
-    return b;
-}
-
-double legendre_dp(int n, double x) {
-    if (n == 0) return 0.0;
-    double Pn = legendre_p(n, x), Pm = legendre_p(n - 1, x);
-    if (std::fabs(x * x - 1.0) < 1e-14)  // the reference's endpoint branch (basis.cpp:336-339)
-        return n * (n + 1) / 2.0 * ((x > 0) ? 1.0 : ((n % 2 == 0) ? 1.0 : -1.0));
-    return n / (x * x - 1.0) * (x * Pn - Pm);
-}
-
-// Radau (DG) correction derivatives g_L', g_R' (Huynh 2007); the reference uses
-// these for every FluxType.
-double radau_left_d(int p, double x) {
-    double s = (p % 2 == 0) ? 1.0 : -1.0;
-    return s * 0.5 * (legendre_dp(p, x) - legendre_dp(p + 1, x));
-}
-double radau_right_d(int p, double x) { return 0.5 * (legendre_dp(p, x) + legendre_dp(p + 1, x)); }
-
-inline int64_t node_of(const int64_t* en, int stride, int e, int k) { return en[(int64_t)e * stride + k]; }
-
-}  // namespace
+using namespace zj::host;
 
 extern "C" int zj_quad_operators(int p, double* sol_xi, double* sol_eta, double* qw,
                                  double* diff_xi, double* diff_eta, double* interp, double* corr) {
@@ -311,40 +191,6 @@ extern "C" int zj_elem_areas(int n_elem, int nv, int stride, const int64_t* en, 
     }
     return ZJ_OK;
 }
-
-namespace {
-// Quad shape functions in Gmsh node order: bilinear (4), biquadratic (9) and
-// biquartic (25) tensor-product Lagrange.
-const int Q9X[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, Q9Y[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
-const int Q25X[25] = {0, 4, 4, 0, 1, 2, 3, 4, 4, 4, 3, 2, 1, 0, 0, 0, 1, 2, 3, 1, 2, 3, 1, 2, 3};
-const int Q25Y[25] = {0, 0, 4, 4, 0, 0, 0, 1, 2, 3, 4, 4, 4, 3, 2, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3};
-
-bool shape(int nc, double xi, double eta, double* N, double* Nx, double* Ne) {
-    if (nc == 4) {
-        const double a = 1 - xi, b = 1 + xi, c = 1 - eta, d = 1 + eta;
-        N[0] = 0.25 * a * c; N[1] = 0.25 * b * c; N[2] = 0.25 * b * d; N[3] = 0.25 * a * d;
-        Nx[0] = -0.25 * c; Nx[1] = 0.25 * c; Nx[2] = 0.25 * d; Nx[3] = -0.25 * d;
-        Ne[0] = -0.25 * a; Ne[1] = -0.25 * b; Ne[2] = 0.25 * b; Ne[3] = 0.25 * a;
-        return true;
-    }
-    int m = nc == 9 ? 2 : nc == 25 ? 4 : 0;
-    if (!m) return false;
-    double nodes[5], Lx[5], dLx[5], Le[5], dLe[5];
-    for (int i = 0; i <= m; ++i) nodes[i] = -1.0 + 2.0 * i / m;
-    for (int i = 0; i <= m; ++i) {
-        Lx[i] = lag(nodes, m + 1, i, xi); dLx[i] = lag_d(nodes, m + 1, i, xi);
-        Le[i] = lag(nodes, m + 1, i, eta); dLe[i] = lag_d(nodes, m + 1, i, eta);
-    }
-    const int* IX = m == 2 ? Q9X : Q25X;
-    const int* IY = m == 2 ? Q9Y : Q25Y;
-    for (int k = 0; k < nc; ++k) {
-        N[k] = Lx[IX[k]] * Le[IY[k]];
-        Nx[k] = dLx[IX[k]] * Le[IY[k]];
-        Ne[k] = Lx[IX[k]] * dLe[IY[k]];
-    }
-    return true;
-}
-}  // namespace
 
 extern "C" int zj_geometry(int n_elem, int nc, int stride, const int64_t* en, const double* xy, int S,
                            const double* sx, const double* se, double* J, double* Jinv, double* pos) {

@@ -243,16 +243,46 @@ __device__ __forceinline__ R trace_point(const R* u, const TraceLine& t, const R
 }
 
 // ---------------------------------------------------------------- kernels
+// LDS layout of one element tile, chosen per (precision, order) to minimise
+// bank conflicts over the kernel's access patterns (32 banks x 4 B):
+// solution-point arrays are [element][variable][point] with padded strides;
+// trace and flux-difference arrays are either variable-major
+// ([element][variable][edge*Fp+point]) or point-major with a padded point
+// stride ([element][edge*Fp+point][variable]).
+template <typename R, int P> struct Lay;
+#define ZJ_LAYOUT(R_, P_, PAD_V, PAD_E, VAR_MAJOR, PAD_T)                                     \
+    template <> struct Lay<R_, P_> {                                                           \
+        static constexpr int N1 = P_ + 1, S = N1 * N1, FP = N1;                                \
+        static constexpr int UV = S + PAD_V, UE = NV * UV + PAD_E;                             \
+        static constexpr bool VM = VAR_MAJOR;                                                  \
+        static constexpr int TE = (VAR_MAJOR ? 4 * FP * NV : 4 * FP * (NV + 1)) + PAD_T;       \
+        __device__ static constexpr int tr(int el, int k, int f, int v) {                      \
+            return VAR_MAJOR ? el * TE + v * (4 * FP) + k * FP + f : el * TE + (k * FP + f) * (NV + 1) + v; \
+        }                                                                                      \
+    };
+ZJ_LAYOUT(double, 1, 0, 4, false, 0)
+ZJ_LAYOUT(double, 2, 0, 5, false, 0)
+ZJ_LAYOUT(double, 3, 0, 0, true, 0)
+ZJ_LAYOUT(double, 4, 0, 5, true, 5)
+ZJ_LAYOUT(double, 5, 0, 7, false, 0)
+ZJ_LAYOUT(float, 1, 0, 4, false, 0)
+ZJ_LAYOUT(float, 2, 0, 5, false, 0)
+ZJ_LAYOUT(float, 3, 3, 4, false, 0)
+ZJ_LAYOUT(float, 4, 1, 5, true, 4)
+ZJ_LAYOUT(float, 5, 0, 0, false, 0)
+#undef ZJ_LAYOUT
+
 template <typename R, int P, int EPB, int MODE>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
+    using Y = Lay<R, P>;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
     static_assert(NT <= BLOCK, "tile exceeds the workgroup");
-    __shared__ R sU[EPB][NV][S];
-    __shared__ R sFx[EPB][NV][S];
-    __shared__ R sFy[EPB][NV][S];
-    __shared__ R sTr[EPB][4][FP][NV];
-    __shared__ R sFd[EPB][4][FP][NV];
+    __shared__ R sU[EPB * Y::UE];
+    __shared__ R sFx[EPB * Y::UE];
+    __shared__ R sFy[EPB * Y::UE];
+    __shared__ R sTr[EPB * Y::TE];
+    __shared__ R sFd[EPB * Y::TE];
     __shared__ R sD[N1 * N1], sLm[N1], sLp[N1], sC[4 * S];
     __shared__ double sRed[BLOCK / 64];
 
@@ -269,6 +299,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     const int e = e0 + el;
     const bool active = tid < NT && e < a.E;
     const int64_t gi = (int64_t)e * S + sp;
+    const int ub = el * Y::UE + sp;  // this thread's point in the solution-point arrays
 
     // ---- A: state and volume flux
     double q[NV];
@@ -278,7 +309,9 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         for (int v = 0; v < NV; ++v) { q[v] = a.U_in[v * PS + gi]; u[v] = R(q[v]); }
         euler_flux(u, g, fx, fy);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { sU[el][v][sp] = u[v]; sFx[el][v][sp] = fx[v]; sFy[el][v][sp] = fy[v]; }
+        for (int v = 0; v < NV; ++v) {
+            sU[ub + v * Y::UV] = u[v]; sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
+        }
     }
     __syncthreads();
 
@@ -288,7 +321,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         if (e0 + el2 < a.E) {
             const TraceLine tl = trace_line<P>(k, f);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) sTr[el2][k][f][v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
+            for (int v = 0; v < NV; ++v)
+                sTr[Y::tr(el2, k, f, v)] = trace_point<R, P>(sU + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
         }
     }
     __syncthreads();
@@ -305,7 +339,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
             const bool right = L.role == ROLE_RIGHT;
             R own[NV], mirr[NV], other[NV], UL[NV], UR[NV], Fc[NV], Fn[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { own[v] = sTr[el2][k][f][v]; mirr[v] = sTr[el2][k][P - f][v]; }
+            for (int v = 0; v < NV; ++v) { own[v] = sTr[Y::tr(el2, k, f, v)]; mirr[v] = sTr[Y::tr(el2, k, P - f, v)]; }
             // The face's common flux as its left element computes it: U_L = left
             // slot f, U_R = right slot Fp-1-f (kernels.cu:577).  One call site
             // serves both roles so a wave never runs the flux twice.
@@ -322,7 +356,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
             for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sFd[el2][k][f][v] = Fd[v];
+        for (int v = 0; v < NV; ++v) sFd[Y::tr(el2, k, f, v)] = Fd[v];
     }
     __syncthreads();
 
@@ -334,8 +368,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         double res[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const R* Fx = sFx[el][v];
-            const R* Fy = sFy[el][v];
+            const R* Fx = sFx + el * Y::UE + v * Y::UV;
+            const R* Fy = sFy + el * Y::UE + v * Y::UV;
             R dFdxi = 0, dFdeta = 0, dGdxi = 0, dGdeta = 0;
 #pragma unroll
             for (int m = 0; m < N1; ++m) {
@@ -346,8 +380,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
                 dGdeta += dj * Fy[m * N1 + i];
             }
             const R div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
-            const R c = sFd[el][0][i][v] * sC[sp] + sFd[el][1][j][v] * sC[S + sp] +
-                        sFd[el][2][i][v] * sC[2 * S + sp] + sFd[el][3][j][v] * sC[3 * S + sp];
+            const R c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
+                        sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
             res[v] = double(-div - c);
             rsq += res[v] * res[v];
         }
@@ -371,7 +405,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 a.U_out[v * PS + gi] = un[v];
-                sU[el][v][sp] = R(un[v]);  // safe: sU was last read before the B1->B2 barrier
+                sU[ub + v * Y::UV] = R(un[v]);  // safe: sU was last read before the B1->B2 barrier
             }
         }
     }
@@ -390,15 +424,16 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         const TraceLine tl = trace_line<P>(k, f);
         R tr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
         store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
     }
 }
 
 template <typename R, int P, int EPB>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, const OpTables<R> ops) {
+    using Y = Lay<R, P>;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1, NT = EPB * S, NF = EPB * 4 * FP;
-    __shared__ R sU[EPB][NV][S];
+    __shared__ R sU[EPB * Y::UE];
     __shared__ R sLm[N1], sLp[N1];
     const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
     const int64_t PS = (int64_t)a.E * S;
@@ -407,7 +442,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, cons
     if (tid < NT && e0 + el < a.E) {
         const int64_t gi = (int64_t)(e0 + el) * S + sp;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sU[el][v][sp] = R(a.U[v * PS + gi]);
+        for (int v = 0; v < NV; ++v) sU[el * Y::UE + v * Y::UV + sp] = R(a.U[v * PS + gi]);
     }
     __syncthreads();
     for (int t = tid; t < NF; t += BLOCK) {
@@ -417,7 +452,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, cons
         const TraceLine tl = trace_line<P>(k, f);
         R tr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU[el2][v], tl, sLm, sLp);
+        for (int v = 0; v < NV; ++v) tr[v] = trace_point<R, P>(sU + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
         store4(a.T_out + (((int64_t)e2 * 4 + k) * FP + f) * NV, tr);
     }
 }
@@ -438,6 +473,21 @@ __global__ void soa_to_aos_kernel(const double* __restrict__ soa, double* __rest
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = soa[k * n + i];
     store4(aos + 4 * i, v);
+}
+
+// gpu::rkStage1/2/3 arithmetic (kernels.cu:1297-1349) on arbitrary arrays.
+__global__ void rk3_combine_kernel(int stage, double* out, const double* U0, const double* Uk,
+                                   const double* R, int64_t n, double dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (stage == 1) out[i] = U0[i] + dt * R[i];
+    else if (stage == 2) out[i] = 0.75 * U0[i] + 0.25 * Uk[i] + 0.25 * dt * R[i];
+    else out[i] = (1.0 / 3.0) * U0[i] + (2.0 / 3.0) * Uk[i] + (2.0 / 3.0) * dt * R[i];
+}
+
+__global__ void euler_update_kernel(double* U, const double* R, int64_t n, double dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) U[i] = U[i] + dt * R[i];
 }
 
 __global__ void set_scalar_kernel(double* p, double v) {
@@ -568,6 +618,19 @@ hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n, hipStrea
 
 hipError_t launch_set_scalar(double* p, double v, hipStream_t s) {
     set_scalar_kernel<<<1, 64, 0, s>>>(p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
+                              int64_t n, double dt, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    rk3_combine_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(stage, out, U0, Uk, R, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    euler_update_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(U, R, n, dt);
     return hipGetLastError();
 }
 

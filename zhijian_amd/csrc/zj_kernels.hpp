@@ -76,6 +76,9 @@ int num_blocks(int E, int epb);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n_points, hipStream_t s);
 hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n_points, hipStream_t s);
 hipError_t launch_set_scalar(double* p, double v, hipStream_t s);
+hipError_t launch_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
+                              int64_t n, double dt, hipStream_t s);
+hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt, hipStream_t s);
 // dt = min over points of cfl*h_e/(ws*cf), h_e = sqrt(area_e); partial per block.
 hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma,
                              double cfl, double cf, double* partial, int nblk, hipStream_t s);

@@ -58,7 +58,28 @@ struct zj_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::string err;
+    // host copies kept for zj_set_boundary
+    std::vector<EdgeLink> links_host;
+    std::vector<int> bface;      // creation-order index of each boundary face
+    std::vector<int> bface_slot; // element*4 + local edge of that face
+    int n_bc_alloc = 0;          // entries of bc_data allocated on the device
 };
+
+// Boundary link of one face (kernels.cu:584-595): a missing entry falls back
+// to a slip wall; far field and outflow read their 8-double entry.
+static int boundary_link(zj_solver* h, int f, const int* bc_type, const int* bc_face_map, int n_bc_data,
+                         bool have_data, EdgeLink& L) {
+    int bct = bc_type ? bc_type[f] : ZJ_BC_SLIPWALL;
+    int bci = bc_face_map ? bc_face_map[f] : -1;
+    if (bci < 0) { bct = ZJ_BC_SLIPWALL; bci = 0; }
+    const bool needs_data = (bct == ZJ_BC_FARFIELD || bct == ZJ_BC_OUTFLOW);
+    if (needs_data && (!have_data || bci >= n_bc_data)) {
+        h->err = "boundary face refers to missing bc_data";
+        return ZJ_ERR_ARG;
+    }
+    L = EdgeLink{-1, bct, ROLE_BOUNDARY, needs_data ? bci : 0};
+    return ZJ_OK;
+}
 
 static std::string g_create_error;
 
@@ -174,7 +195,6 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
     const int E = h->E, F = h->F, Fp = h->N1;
     std::vector<EdgeLink> links((size_t)E * 4, EdgeLink{-1, 0, ROLE_NONE, 0});
     std::vector<double> nrm((size_t)E * 8, 0.0);
-    int max_bc = -1;
     for (int f = 0; f < F; ++f) {
         const int le = m->face_left_elem[f], ll = m->face_left_local[f];
         const int re = m->face_right_elem[f], rl = m->face_right_local[f];
@@ -192,14 +212,10 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
             Rr = EdgeLink{le, ll, ROLE_RIGHT, 0};
             nrm[((size_t)re * 4 + rl) * 2] = nx; nrm[((size_t)re * 4 + rl) * 2 + 1] = ny;
         } else {
-            int bct = m->bc_type ? m->bc_type[f] : ZJ_BC_SLIPWALL;
-            int bci = m->bc_face_map ? m->bc_face_map[f] : -1;
-            if (bci < 0) { bct = ZJ_BC_SLIPWALL; bci = 0; }  // kernels.cu:589-593
-            const bool needs_data = (bct == ZJ_BC_FARFIELD || bct == ZJ_BC_OUTFLOW);
-            if (needs_data && (!m->bc_data || bci >= m->n_bc_data))
-                return fail(h, ZJ_ERR_ARG, "boundary face refers to missing bc_data");
-            if (needs_data) max_bc = std::max(max_bc, bci);
-            L = EdgeLink{-1, bct, ROLE_BOUNDARY, needs_data ? bci : 0};
+            int rc = boundary_link(h, f, m->bc_type, m->bc_face_map, m->n_bc_data, m->bc_data != nullptr, L);
+            if (rc) return rc;
+            h->bface.push_back(f);
+            h->bface_slot.push_back(le * 4 + ll);
         }
     }
 
@@ -228,9 +244,11 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
         if ((rc = upload(h, &h->normals, nf))) return rc;
     }
     if ((rc = upload(h, (void**)&h->links, links))) return rc;
+    h->links_host = std::move(links);
     std::vector<double> bcd;
     if (m->bc_data && m->n_bc_data > 0) bcd.assign(m->bc_data, m->bc_data + (size_t)m->n_bc_data * 8);
     if ((rc = upload(h, (void**)&h->bc_data, bcd))) return rc;
+    h->n_bc_alloc = (int)(bcd.size() / 8);
     std::vector<double> hc;
     if (m->elem_area) {
         hc.resize(E);
@@ -558,4 +576,57 @@ extern "C" int zj_copy_probe(int device, int64_t bytes, int reps, double* gbps) 
     hipEventElapsedTime(&ms, e0, e1);
     *gbps = 2.0 * (double)n * 32.0 * reps / (ms * 1e-3) / 1e9;
     return fin(rc);
+}
+
+extern "C" int zj_set_boundary(zj_handle h, const int* bc_type, const int* bc_face_map, const double* bc_data,
+                               int n_bc_data) {
+    if (!h || n_bc_data < 0) return ZJ_ERR_ARG;
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "boundary change in the middle of a step");
+    std::vector<EdgeLink> links = h->links_host;
+    for (size_t b = 0; b < h->bface.size(); ++b) {
+        int rc = boundary_link(h, h->bface[b], bc_type, bc_face_map, n_bc_data, bc_data != nullptr,
+                               links[h->bface_slot[b]]);
+        if (rc) return rc;
+    }
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    if (n_bc_data > h->n_bc_alloc) {
+        drop_graph(h);  // the data buffer moves
+        ZJ_HIP(h, hipFree(h->bc_data));
+        h->bc_data = nullptr;
+        ZJ_HIP(h, hipMalloc((void**)&h->bc_data, (size_t)n_bc_data * 8 * sizeof(double)));
+        h->n_bc_alloc = n_bc_data;
+    }
+    if (n_bc_data > 0 && bc_data)
+        ZJ_HIP(h, hipMemcpy(h->bc_data, bc_data, (size_t)n_bc_data * 8 * sizeof(double), hipMemcpyHostToDevice));
+    ZJ_HIP(h, hipMemcpy(h->links, links.data(), links.size() * sizeof(EdgeLink), hipMemcpyHostToDevice));
+    h->links_host = std::move(links);
+    return ZJ_OK;
+}
+
+extern "C" int zj_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
+                              int64_t n, double dt, void* stream) {
+    if (stage < 1 || stage > 3 || n < 0 || (n > 0 && (!out || !U0 || !R || (stage > 1 && !Uk))))
+        return ZJ_ERR_ARG;
+    if (launch_rk3_combine(stage, out, U0, Uk, R, n, dt, (hipStream_t)stream) != hipSuccess) {
+        g_create_error = "zj_rk3_combine: launch failed";
+        return ZJ_ERR_HIP;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_euler_update(double* U, const double* R, int64_t n, double dt, void* stream) {
+    if (n < 0 || (n > 0 && (!U || !R))) return ZJ_ERR_ARG;
+    if (launch_euler_update(U, R, n, dt, (hipStream_t)stream) != hipSuccess) {
+        g_create_error = "zj_euler_update: launch failed";
+        return ZJ_ERR_HIP;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_synchronize(zj_handle h) {
+    if (!h) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
 }
