@@ -31,15 +31,33 @@ METRIC = "residual DOF-updates/s, FP32-MFMA vs FP64 plain at stated L2 err; adjo
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes_per_stage(E, p, w_in, w_g, w_t, r, faces_per_elem=2):
-    """Algorithmic HBM bytes of one stage: state in (4 w_in), FP64 state out
-    (32), r*U^n reads (32 r), geometry (4 Jinv words; the path never reads J),
-    traces written once and read once (4*Fp*4*w_t*2), and per face a normal
-    plus 16 B of connectivity."""
+def alg_bytes_per_stage(E, p, stage, precision, affine, cross_edges=None, faces_per_elem=2):
+    """Algorithmic HBM bytes of one SSP-RK3 stage launch (stage 1, 2, 3).
+
+    Per solution point: the FP64 state U^n (32 B) is read by every stage; stages
+    2 and 3 also read the intermediate state (FP64: 32 B; FP32-bulk: an FP32
+    increment over U^n, 16 B); stages 1 and 2 write the intermediate state
+    (32 B or 16 B), stage 3 writes U^{n+1} (32 B). Geometry is 4 Jinv words per
+    point, or per element on affine meshes (J is never read). The Fp x 4 trace
+    of an element edge whose neighbour lies in another tile is written once and
+    read once in the compute precision (edges inside a tile stay in LDS;
+    `cross_edges` = None counts all 4E); per face a normal (2 words) plus 16 B
+    of connectivity."""
     S = (p + 1) ** 2
     Fp = p + 1
-    return E * (S * (4 * w_in + 32 + 32 * r + 4 * w_g) + 4 * Fp * 4 * w_t * 2
-                + faces_per_elem * (2 * w_g + 16))
+    w = 8 if precision == "fp64" else 4
+    inter = 32 if precision == "fp64" else 16
+    per_pt = 32 + (inter if stage > 1 else 0) + (inter if stage < 3 else 32)
+    geo = 4 * w * (1 if affine else S)
+    ce = 4 * E if cross_edges is None else cross_edges
+    return E * (S * per_pt + geo + faces_per_elem * (2 * w + 16)) + ce * Fp * 4 * w * 2
+
+
+def cross_tile_edges(faces, epb):
+    """Element edges whose trace crosses tiles of `epb` consecutive elements."""
+    f = np.asarray(faces)
+    inner = f[:, 2] >= 0
+    return int(2 * np.count_nonzero(f[inner, 0] // epb != f[inner, 2] // epb))
 
 
 def log(*a):
@@ -152,7 +170,10 @@ def main():
             continue
         ms_step, st_ms = time_precision(sol, U0, dt, args.steps, args.warmup, stream, hiprt)
         w = 8 if name == "fp64" else 4
-        by = [alg_bytes_per_stage(E, args.p, 8, w, w, r) for r in (0.0, 1.0, 1.0)]
+        info = sol.info()
+        affine = info["affine"]
+        ce = cross_tile_edges(tables.faces, info["elems_per_block"])
+        by = [alg_bytes_per_stage(E, args.p, st, name, affine, ce) for st in (1, 2, 3)]
         achieved = sum(by) / (sum(st_ms) * 1e-3) / 1e9
         res[name] = dict(ms_per_step=ms_step, stage_ms=st_ms,
                          dof_updates_per_s=3.0 * dof / (ms_step * 1e-3),
@@ -160,7 +181,8 @@ def main():
                          roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                                        frac=achieved / HBM_PEAK_GBS, traffic=None,
                                        kernel=f"stage_kernel<{'double' if w == 8 else 'float'},P{args.p}>",
-                                       bytes_per_launch=sum(by) / 3.0,
+                                       bytes_per_launch=sum(by) / 3.0, affine_metric=affine,
+                                       cross_tile_edge_fraction=ce / (4.0 * E),
                                        avg_launch_ms=sum(st_ms) / 3.0))
         log(f"{name}: {ms_step:.4f} ms/step, stages {['%.4f' % x for x in st_ms]} ms, "
             f"{res[name]['dof_updates_per_s']:.3e} DOF-upd/s, {achieved:.0f} GB/s")

@@ -150,6 +150,17 @@ int zj_euler_update(double* U, const double* R, int64_t n, double dt, void* stre
 /* Launch configuration of the stage kernel (elements per workgroup). */
 int zj_set_tile(zj_handle h, int elems_per_block);
 int zj_get_info(zj_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp);
+/* Metric storage of the stage kernel.  The reference stores Jinv per solution
+ * point (fr_solver.cpp:85-129); when every element is affine (its per-point
+ * Jinv agree to 1e-13 relative) the library reads one Jinv per element and
+ * differentiates the contravariant fluxes instead, which is the default for
+ * such meshes.  mode: -1 query, 0 per-point, 1 per-element (ZJ_ERR_UNSUPPORTED
+ * on a non-affine mesh).  *active (may be NULL) receives the mode in use. */
+int zj_set_geometry_mode(zj_handle h, int mode, int* active);
+/* Workgroup-to-tile order of the stage kernel: `groups` contiguous runs of
+ * tiles, one per XCD (default: the device's XCD count; 1 = launch order).
+ * Results do not depend on it. */
+int zj_set_xcd_groups(zj_handle h, int groups);
 
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times

@@ -199,19 +199,32 @@ def test_graph_replay_equals_stage_launches():
         assert np.array_equal(ga, gb), k
 
 
-def test_half_tile_is_identical():
-    say("case: elements-per-workgroup max vs half")
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_launch_shape_does_not_change_results(precision):
+    """Tile size (max vs half, also changed mid-run), and workgroup-to-tile
+    order (per-XCD runs vs launch order): traces of edges inside a tile are
+    exchanged in LDS, the rest through HBM, with identical arithmetic."""
+    say(f"case: tiles max vs half, XCD order vs launch order, {precision}")
     mesh = cases.uniform_quad_mesh(9, 7, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=8)
     t = solver.build_tables(mesh.xy, mesh.elem_nodes, 4, periodic=(1.0, 1.0))
-    a = solver.Solver(t)
-    b = solver.Solver(t)
-    b.set_tile(b.info()["elems_per_block"] // 2)
+    a = solver.Solver(t, precision=precision)
+    b = solver.Solver(t, precision=precision)
+    c = solver.Solver(t, precision=precision)
+    half = b.info()["elems_per_block"] // 2
+    b.set_tile(half)
+    c.set_xcd_groups(1)
     U0 = tgv_state(t, amp=0.02, seed=6)
-    a.set_state(U0); b.set_state(U0)
-    assert np.array_equal(a.residual(), b.residual())
-    for _ in range(3):
-        a.rk3_step(1e-3); b.rk3_step(1e-3)
+    a.set_state(U0); b.set_state(U0); c.set_state(U0)
+    Ra = a.residual()
+    assert np.array_equal(Ra, b.residual()) and np.array_equal(Ra, c.residual())
+    assert a.residual_l2() == c.residual_l2()
+    for k in range(4):
+        a.rk3_step(1e-3); b.rk3_step(1e-3); c.rk3_step(1e-3)
+        if k == 1:  # retile with a state in place: the traces are rebuilt
+            b.set_tile(2 * half)
+            c.set_tile(half)
     assert np.array_equal(a.get_state(), b.get_state())
+    assert np.array_equal(a.get_state(), c.get_state())
 
 
 def test_nonfinite_detection_and_scrub():
@@ -248,3 +261,48 @@ def test_fp32bulk_residual_vs_fp64(p):
     say(f"  fp32bulk P{p}: residual rel L2 {r_res:.3e}; state after 3 steps rel L2 {r_l2:.3e}")
     assert r_res < 1e-2
     assert r_l2 < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_affine_metric_matches_per_point(precision):
+    """Per-element metric with contravariant fluxes (affine meshes) against the
+    per-point metric the reference stores, every order; a jittered mesh has no
+    per-element form."""
+    tol = 1e-13 if precision == "fp64" else 2e-6
+    for p in range(1, 6):
+        say(f"case: affine vs per-point metric, vortex 6x6 P{p} {precision}")
+        mesh = cases.vortex_mesh(6)
+        t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+        a = solver.Solver(t, precision=precision)
+        b = solver.Solver(t, precision=precision)
+        assert a.set_geometry_mode(-1) == 1
+        assert b.set_geometry_mode(0) == 0
+        U0 = vortex_state(t, seed=p)
+        a.set_state(U0); b.set_state(U0)
+        assert rel(a.residual(), b.residual()) < tol
+        for _ in range(2):
+            a.rk3_step(1e-3); b.rk3_step(1e-3)
+        assert rel(a.get_state(), b.get_state()) < tol
+    mesh = cases.uniform_quad_mesh(5, 5, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=3)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=(1.0, 1.0))
+    s = solver.Solver(t)
+    assert s.set_geometry_mode(-1) == 0
+    with pytest.raises(RuntimeError):
+        s.set_geometry_mode(1)
+
+
+def test_fp32bulk_stage_launches_equal_graph():
+    """FP32-bulk keeps the intermediate stages as FP32 increments over U^n; the
+    stage-by-stage entry point and the graph step must agree bit for bit."""
+    say("case: FP32-bulk hipGraph step vs three stage launches")
+    mesh = cases.tgv_mesh(16)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 4, periodic=(mesh.Lx, mesh.Ly))
+    a = solver.Solver(t, precision="fp32bulk")
+    b = solver.Solver(t, precision="fp32bulk")
+    U0 = tgv_state(t, amp=0.02, seed=9)
+    a.set_state(U0); b.set_state(U0)
+    for k in range(3):
+        a.rk3_step(2e-3)
+        for st in (1, 2, 3):
+            b.rk3_stage(st, 2e-3)
+        assert np.array_equal(a.get_state(), b.get_state()), k

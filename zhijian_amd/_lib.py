@@ -52,7 +52,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_rk3_stage", "zj_dt_cfl", "zj_residual_l2", "zj_check_finite", "zj_set_tile",
            "zj_get_info", "zj_quad_operators", "zj_build_faces", "zj_pair_periodic",
            "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe", "zj_set_boundary",
-           "zj_rk3_combine", "zj_euler_update", "zj_synchronize")
+           "zj_rk3_combine", "zj_euler_update", "zj_synchronize", "zj_set_geometry_mode",
+           "zj_set_xcd_groups")
 
 
 def lib() -> C.CDLL:
@@ -77,6 +78,8 @@ def lib() -> C.CDLL:
         L.zj_residual_l2.argtypes = [H, C.POINTER(C.c_double)]
         L.zj_check_finite.argtypes = [H, C.POINTER(C.c_int64)]
         L.zj_set_tile.argtypes = [H, C.c_int]
+        L.zj_set_geometry_mode.argtypes = [H, C.c_int, C.POINTER(C.c_int)]
+        L.zj_set_xcd_groups.argtypes = [H, C.c_int]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
         L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]

@@ -272,9 +272,23 @@ ZJ_LAYOUT(float, 4, 1, 5, true, 4)
 ZJ_LAYOUT(float, 5, 0, 0, false, 0)
 #undef ZJ_LAYOUT
 
-template <typename R, int P, int EPB, int MODE>
+// Workgroups are handed to the XCDs round-robin (workgroup b runs on XCD
+// b mod 8), and each XCD has its own L2.  Giving each XCD a contiguous run of
+// tiles keeps a tile's mesh neighbours, whose traces and state lines it
+// shares, in the same L2.  `groups` = 1 leaves the order unchanged.
+__device__ __forceinline__ int tile_of_block(int groups) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    if (groups <= 1) return b;
+    const int q = nb / groups, r = nb - q * groups, x = b % groups, loc = b / groups;
+    return x * q + min(x, r) + loc;
+}
+
+template <typename R, int P, int EPB, int MODE, bool AFF>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
     using Y = Lay<R, P>;
+    // FP32-bulk keeps the intermediate stage state as FP32 increments over U^n
+    constexpr bool INC = std::is_same<R, float>::value;
+    constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
     static_assert(NT <= BLOCK, "tile exceeds the workgroup");
@@ -285,9 +299,10 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     __shared__ R sFd[EPB * Y::TE];
     __shared__ R sD[N1 * N1], sLm[N1], sLp[N1], sC[4 * S];
     __shared__ double sRed[BLOCK / 64];
+    __shared__ unsigned char sOut[EPB * 4];  // edge trace needed outside this tile
 
     const int tid = threadIdx.x;
-    const int e0 = blockIdx.x * EPB;
+    const int e0 = tile_of_block(a.xcd_groups) * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma);
 
@@ -302,15 +317,34 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     const int ub = el * Y::UE + sp;  // this thread's point in the solution-point arrays
 
     // ---- A: state and volume flux
-    double q[NV];
+    // Affine elements (constant metric): the divergence regroups into the
+    // derivatives of the contravariant fluxes xi_x F + xi_y G and eta_x F + eta_y G,
+    // with the metric read once per element.
+    double q[NV], u0[NV];
+    R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
     if (active) {
+        if (AFF) {
+            jx = a.Jinv[(int64_t)e * 4]; jy = a.Jinv[(int64_t)e * 4 + 1];
+            kx = a.Jinv[(int64_t)e * 4 + 2]; ky = a.Jinv[(int64_t)e * 4 + 3];
+        }
         R u[NV], fx[NV], fy[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { q[v] = a.U_in[v * PS + gi]; u[v] = R(q[v]); }
+        for (int v = 0; v < NV; ++v) {
+            if (INC && HAS_U0) u0[v] = a.U0[v * PS + gi];
+            if (INC && HAS_U0) q[v] = u0[v] + double(a.dW_in[v * PS + gi]);
+            else q[v] = a.U_in[v * PS + gi];
+            u[v] = R(q[v]);
+        }
         euler_flux(u, g, fx, fy);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            sU[ub + v * Y::UV] = u[v]; sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
+            sU[ub + v * Y::UV] = u[v];
+            if (AFF) {
+                sFx[ub + v * Y::UV] = jx * fx[v] + jy * fy[v];
+                sFy[ub + v * Y::UV] = kx * fx[v] + ky * fy[v];
+            } else {
+                sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
+            }
         }
     }
     __syncthreads();
@@ -334,6 +368,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
         if (e2 >= a.E) continue;
         const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
         R Fd[NV] = {R(0), R(0), R(0), R(0)};
+        if (f == 0) sOut[el2 * 4 + k] = 0;
         if (L.role != ROLE_NONE) {
             const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
             const bool right = L.role == ROLE_RIGHT;
@@ -343,10 +378,20 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
             // The face's common flux as its left element computes it: U_L = left
             // slot f, U_R = right slot Fp-1-f (kernels.cu:577).  One call site
             // serves both roles so a wave never runs the flux twice.
-            if (L.role == ROLE_BOUNDARY)
+            // A neighbour inside this tile has its traces in sTr already (the
+            // same arithmetic on the same state as the copy in T_in), so only
+            // edges that cross tiles travel through T_in/T_out.
+            const int nl = L.nbr - e0;
+            const bool inside = L.role != ROLE_BOUNDARY && nl >= 0 && nl < EPB;
+            if (f == 0) sOut[el2 * 4 + k] = L.role != ROLE_BOUNDARY && !inside;
+            if (L.role == ROLE_BOUNDARY) {
                 ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
-            else
+            } else if (inside) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, right ? f : P - f, v)];
+            } else {
                 load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (right ? f : P - f)) * NV, other);
+            }
 #pragma unroll
             for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
             rusanov(UL, UR, nx, ny, g, Fc);
@@ -364,22 +409,33 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     double rsq = 0.0;
     if (active) {
         const int i = sp % N1, j = sp / N1;  // xi index, eta index
-        const R jx = a.Jinv[gi], jy = a.Jinv[PS + gi], kx = a.Jinv[2 * PS + gi], ky = a.Jinv[3 * PS + gi];
+        if (!AFF) { jx = a.Jinv[gi]; jy = a.Jinv[PS + gi]; kx = a.Jinv[2 * PS + gi]; ky = a.Jinv[3 * PS + gi]; }
         double res[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const R* Fx = sFx + el * Y::UE + v * Y::UV;
             const R* Fy = sFy + el * Y::UE + v * Y::UV;
-            R dFdxi = 0, dFdeta = 0, dGdxi = 0, dGdeta = 0;
+            R div;
+            if (AFF) {  // Fx, Fy hold the contravariant fluxes
+                R dxi = 0, deta = 0;
 #pragma unroll
-            for (int m = 0; m < N1; ++m) {
-                const R di = sD[i * N1 + m], dj = sD[j * N1 + m];
-                dFdxi += di * Fx[j * N1 + m];
-                dGdxi += di * Fy[j * N1 + m];
-                dFdeta += dj * Fx[m * N1 + i];
-                dGdeta += dj * Fy[m * N1 + i];
+                for (int m = 0; m < N1; ++m) {
+                    dxi += sD[i * N1 + m] * Fx[j * N1 + m];
+                    deta += sD[j * N1 + m] * Fy[m * N1 + i];
+                }
+                div = dxi + deta;
+            } else {
+                R dFdxi = 0, dFdeta = 0, dGdxi = 0, dGdeta = 0;
+#pragma unroll
+                for (int m = 0; m < N1; ++m) {
+                    const R di = sD[i * N1 + m], dj = sD[j * N1 + m];
+                    dFdxi += di * Fx[j * N1 + m];
+                    dGdxi += di * Fy[j * N1 + m];
+                    dFdeta += dj * Fx[m * N1 + i];
+                    dGdeta += dj * Fy[m * N1 + i];
+                }
+                div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
             }
-            const R div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
             const R c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
                         sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
             res[v] = double(-div - c);
@@ -393,34 +449,39 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
             double un[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                if (MODE == MODE_STAGE1) {
-                    un[v] = q[v] + dt * res[v];
-                } else {
-                    const double u0 = a.U0[v * PS + gi];
-                    if (MODE == MODE_STAGE2) un[v] = 0.75 * u0 + 0.25 * q[v] + 0.25 * dt * res[v];
-                    else un[v] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * q[v] + (2.0 / 3.0) * dt * res[v];
-                }
+                if (!INC && HAS_U0) u0[v] = a.U0[v * PS + gi];
+                if (MODE == MODE_STAGE1) un[v] = q[v] + dt * res[v];
+                else if (MODE == MODE_STAGE2) un[v] = 0.75 * u0[v] + 0.25 * q[v] + 0.25 * dt * res[v];
+                else un[v] = (1.0 / 3.0) * u0[v] + (2.0 / 3.0) * q[v] + (2.0 / 3.0) * dt * res[v];
             }
             if (a.positivity) positivity(un);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                a.U_out[v * PS + gi] = un[v];
-                sU[ub + v * Y::UV] = R(un[v]);  // safe: sU was last read before the B1->B2 barrier
+                double next = un[v];  // the state the next stage will read back
+                if (INC && MODE != MODE_STAGE3) {
+                    const double base = MODE == MODE_STAGE1 ? q[v] : u0[v];
+                    const float dw = float(un[v] - base);
+                    a.dW_out[v * PS + gi] = dw;
+                    next = base + double(dw);
+                } else {
+                    a.U_out[v * PS + gi] = un[v];
+                }
+                sU[ub + v * Y::UV] = R(next);  // safe: sU was last read before the B1->B2 barrier
             }
         }
     }
     if (a.l2_partial) {  // one partial sum of R^2 per workgroup; reduced on request
         const double s = block_reduce<OpSum>(rsq, sRed);
-        if (tid == 0) a.l2_partial[blockIdx.x] = s;
+        if (tid == 0) a.l2_partial[e0 / EPB] = s;  // tile order: the sum does not depend on the remap
     }
     if (MODE == MODE_RESIDUAL) return;
     __syncthreads();
 
-    // ---- D: traces of the new state
+    // ---- D: traces of the new state, for edges read by another tile
     for (int t = tid; t < NF; t += BLOCK) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
-        if (e2 >= a.E) continue;
+        if (e2 >= a.E || !sOut[el2 * 4 + k]) continue;
         const TraceLine tl = trace_line<P>(k, f);
         R tr[NV];
 #pragma unroll
@@ -435,7 +496,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, cons
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1, NT = EPB * S, NF = EPB * 4 * FP;
     __shared__ R sU[EPB * Y::UE];
     __shared__ R sLm[N1], sLp[N1];
-    const int tid = threadIdx.x, e0 = blockIdx.x * EPB;
+    const int tid = threadIdx.x, e0 = tile_of_block(a.xcd_groups) * EPB;
     const int64_t PS = (int64_t)a.E * S;
     if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
     const int el = tid / S, sp = tid - (tid / S) * S;
@@ -545,36 +606,44 @@ int num_blocks(int E, int epb) { return (E + epb - 1) / epb; }
 
 #define ZJ_EPB(P) (BLOCK / (((P) + 1) * ((P) + 1)))
 
-template <typename R, int P, int EPB>
+template <typename R, int P, int EPB, bool AFF>
 static hipError_t stage_dispatch_mode(int mode, const StageArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
     const dim3 grid(num_blocks(a.E, EPB)), block(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL, AFF><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1, AFF><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2, AFF><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3, AFF><<<grid, block, 0, s>>>(a, ops); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <typename R, int P>
+template <typename R, int P, bool AFF>
 static hipError_t stage_dispatch_epb(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, hipStream_t s) {
     constexpr int M = ZJ_EPB(P);
-    if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M>(mode, a, ops, s);
-    if (M / 2 >= 1 && epb == M / 2) return stage_dispatch_mode<R, P, (M / 2 >= 1 ? M / 2 : 1)>(mode, a, ops, s);
+    if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF>(mode, a, ops, s);
+    if (M / 2 >= 1 && epb == M / 2) return stage_dispatch_mode<R, P, (M / 2 >= 1 ? M / 2 : 1), AFF>(mode, a, ops, s);
     return hipErrorInvalidValue;
 }
 
+template <typename R, int P>
+static hipError_t stage_dispatch_aff(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool aff,
+                                     hipStream_t s) {
+    return aff ? stage_dispatch_epb<R, P, true>(mode, a, ops, epb, s)
+               : stage_dispatch_epb<R, P, false>(mode, a, ops, epb, s);
+}
+
 template <typename R>
-hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, hipStream_t s) {
+hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool affine,
+                        hipStream_t s) {
     if (a.E <= 0) return hipSuccess;
     switch (P) {
-    case 1: return stage_dispatch_epb<R, 1>(mode, a, ops, epb, s);
-    case 2: return stage_dispatch_epb<R, 2>(mode, a, ops, epb, s);
-    case 3: return stage_dispatch_epb<R, 3>(mode, a, ops, epb, s);
-    case 4: return stage_dispatch_epb<R, 4>(mode, a, ops, epb, s);
-    case 5: return stage_dispatch_epb<R, 5>(mode, a, ops, epb, s);
+    case 1: return stage_dispatch_aff<R, 1>(mode, a, ops, epb, affine, s);
+    case 2: return stage_dispatch_aff<R, 2>(mode, a, ops, epb, affine, s);
+    case 3: return stage_dispatch_aff<R, 3>(mode, a, ops, epb, affine, s);
+    case 4: return stage_dispatch_aff<R, 4>(mode, a, ops, epb, affine, s);
+    case 5: return stage_dispatch_aff<R, 5>(mode, a, ops, epb, affine, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -599,8 +668,10 @@ hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, in
     }
 }
 
-template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, hipStream_t);
-template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&, int, hipStream_t);
+template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, bool,
+                                        hipStream_t);
+template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&, int, bool,
+                                       hipStream_t);
 template hipError_t launch_trace<double>(int, const TraceArgs<double>&, const OpTables<double>&, int, hipStream_t);
 template hipError_t launch_trace<float>(int, const TraceArgs<float>&, const OpTables<float>&, int, hipStream_t);
 

@@ -36,14 +36,21 @@ enum StageMode : int { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE
 
 // Arguments of one stage launch.  State planes are SoA: plane v holds E*S
 // doubles, element-major then solution point.
+//
+// FP64 form: the stage state lives in U_in/U_out (U1, U2 in the W buffer).
+// FP32-bulk form: the intermediate stage state is stored as an FP32 increment
+// over U^n (dW_in/dW_out): U1 = U^n + dW1, U2 = U^n + dW2; the increments are
+// of order dt*R, so their FP32 rounding sits far below the FP32 residual's.
 template <typename R>
 struct StageArgs {
     int E;
-    const double* U_in;      // stage input state
+    const double* U_in;      // stage input state (FP64 form; U^n in stage 1 of both forms)
     const double* U0;        // U^n (stages 2, 3)
     double* U_out;           // stage output state (may alias U_in or U0 element-locally)
+    const float* dW_in;      // FP32-bulk: increment of the stage input over U^n (stages 2, 3)
+    float* dW_out;           // FP32-bulk: increment of the stage output (stages 1, 2)
     double* R_out;           // residual planes (MODE_RESIDUAL)
-    const R* Jinv;           // 4 planes of E*S
+    const R* Jinv;           // 4 planes of E*S, or [E][4] for affine meshes
     const R* T_in;           // traces of U_in [E][4][Fp][4]
     R* T_out;                // traces of U_out
     const EdgeLink* links;   // [E][4]
@@ -53,6 +60,7 @@ struct StageArgs {
     double* l2_partial;      // [grid] sum of R^2 per workgroup, or nullptr
     double gamma;
     int positivity;
+    int xcd_groups;          // workgroup -> tile remap groups (1: identity)
 };
 
 template <typename R>
@@ -60,12 +68,13 @@ struct TraceArgs {
     int E;
     const double* U;         // SoA planes
     R* T_out;
+    int xcd_groups;
 };
 
 // Launch wrappers (host).  `epb` = elements per workgroup (0 = default).
 template <typename R>
 hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
-                        int epb, hipStream_t s);
+                        int epb, bool affine, hipStream_t s);
 template <typename R>
 hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, int epb,
                         hipStream_t s);

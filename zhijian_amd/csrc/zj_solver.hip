@@ -3,11 +3,13 @@
 // hipGraph replay of the SSP-RK3 step.
 //
 // Device layout (one GPU owns the whole mesh):
-//   U, W        state planes [4][E*S] FP64 (U = U^n; W = U1, then U2 in place)
+//   U, W        state planes [4][E*S] FP64 (U = U^n; W = U1, then U2 in place);
+//               in FP32-bulk W holds FP32 increments U1 - U^n, then U2 - U^n
 //   T[3]        flux-point traces [E][4][Fp][4] in the compute precision;
 //               stage 1 reads T[c] and writes T[c+1], stage 2 T[c+1]->T[c+2],
 //               stage 3 T[c+2]->T[c], so every step is the same launch sequence
-//   Jinv        4 planes [E*S] in the compute precision
+//   Jinv        4 planes [E*S] in the compute precision; Jinv_e [E][4] as well
+//               when every element is affine (constant metric)
 //   links       [E][4] EdgeLink; normals [E][4][2] (face normal of the left side)
 #include "zhijian_amd.h"
 #include "zj_kernels.hpp"
@@ -19,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace zj;
@@ -31,6 +34,7 @@ struct zj_solver {
     int positivity = 1;
     double gamma = 1.4, cfl = 0.5;
     int epb = 0;
+    int xcd_groups = 1;  // workgroup -> tile remap (number of XCDs)
     int nblk = 0;        // workgroups of the stage kernel
     int nred = 0;        // workgroups of the standalone reductions
     double* U = nullptr;
@@ -38,6 +42,9 @@ struct zj_solver {
     double* Rres = nullptr;
     double* aos = nullptr;
     void* Jinv = nullptr;
+    void* Jinv_e = nullptr;  // per-element metric (affine meshes only)
+    bool affine_ok = false;  // every element's metric is constant
+    bool affine = false;     // kernels read Jinv_e
     void* T[3] = {nullptr, nullptr, nullptr};
     EdgeLink* links = nullptr;
     void* normals = nullptr;
@@ -156,7 +163,7 @@ static int upload(zj_solver* h, void** dst, const std::vector<T>& v) {
 static void release(zj_solver* h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);
-    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->T[0], h->T[1], h->T[2], h->links,
+    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -221,6 +228,9 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
 
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int nxcc = 1;
+    if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || nxcc < 1) nxcc = 1;
+    h->xcd_groups = nxcc;
 
     const size_t npts = (size_t)E * h->S;
     const size_t rsz = h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float);
@@ -234,13 +244,31 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
     std::vector<double> jp(npts * 4);
     for (size_t g = 0; g < npts; ++g)
         for (int c = 0; c < 4; ++c) jp[c * npts + g] = m->Jinv[g * 4 + c];
+    // Affine detection: an element whose per-point metric agrees with its first
+    // point's to 1e-13 of its largest entry is stored once (the bilinear map of a
+    // parallelogram has a constant Jacobian; the tables differ only by rounding).
+    std::vector<double> je((size_t)E * 4);
+    h->affine_ok = E > 0;
+    for (int e = 0; e < E && h->affine_ok; ++e) {
+        const double* J0 = m->Jinv + (size_t)e * h->S * 4;
+        double mx = 0.0;
+        for (int c = 0; c < 4; ++c) mx = std::max(mx, std::fabs(J0[c]));
+        for (int sp = 1; sp < h->S && h->affine_ok; ++sp)
+            for (int c = 0; c < 4; ++c)
+                if (!(std::fabs(J0[sp * 4 + c] - J0[c]) <= 1e-13 * mx)) { h->affine_ok = false; break; }
+        for (int c = 0; c < 4; ++c) je[(size_t)e * 4 + c] = J0[c];
+    }
+    if (!h->affine_ok) je.clear();
+    h->affine = h->affine_ok;
     int rc;
     if (h->prec == ZJ_PREC_FP64) {
         if ((rc = upload(h, &h->Jinv, jp))) return rc;
+        if ((rc = upload(h, &h->Jinv_e, je))) return rc;
         if ((rc = upload(h, &h->normals, nrm))) return rc;
     } else {
-        std::vector<float> jf(jp.begin(), jp.end()), nf(nrm.begin(), nrm.end());
+        std::vector<float> jf(jp.begin(), jp.end()), jef(je.begin(), je.end()), nf(nrm.begin(), nrm.end());
         if ((rc = upload(h, &h->Jinv, jf))) return rc;
+        if ((rc = upload(h, &h->Jinv_e, jef))) return rc;
         if ((rc = upload(h, &h->normals, nf))) return rc;
     }
     if ((rc = upload(h, (void**)&h->links, links))) return rc;
@@ -309,13 +337,14 @@ template <typename R>
 static StageArgs<R> stage_args(zj_solver* h, int mode) {
     StageArgs<R> a{};
     a.E = h->E;
-    a.Jinv = (const R*)h->Jinv;
+    a.Jinv = (const R*)(h->affine ? h->Jinv_e : h->Jinv);
     a.links = h->links;
     a.normals = (const R*)h->normals;
     a.bc_data = h->bc_data;
     a.dt_ptr = h->d_scalars;
     a.gamma = h->gamma;
     a.positivity = h->positivity;
+    a.xcd_groups = h->xcd_groups;
     R** T = (R**)h->T;
     switch (mode) {
     case MODE_RESIDUAL:
@@ -332,15 +361,21 @@ static StageArgs<R> stage_args(zj_solver* h, int mode) {
         a.l2_partial = h->partial;
         break;
     }
+    if (std::is_same<R, float>::value) {  // FP32-bulk: W holds FP32 increments over U^n
+        float* dw = (float*)h->W;
+        if (mode == MODE_STAGE1) { a.dW_out = dw; a.U_out = nullptr; }
+        if (mode == MODE_STAGE2) { a.dW_in = dw; a.dW_out = dw; a.U_in = nullptr; a.U_out = nullptr; }
+        if (mode == MODE_STAGE3) { a.dW_in = dw; a.U_in = nullptr; }
+    }
     return a;
 }
 
 static hipError_t enqueue_stage(zj_solver* h, int mode, hipStream_t s) {
     hipError_t e;
     if (h->prec == ZJ_PREC_FP64)
-        e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, s);
+        e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, h->affine, s);
     else
-        e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, s);
+        e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, h->affine, s);
     if (e == hipSuccess && (mode == MODE_RESIDUAL || mode == MODE_STAGE3)) h->l2_blocks = h->nblk;
     return e;
 }
@@ -356,9 +391,9 @@ static hipError_t set_dt(zj_solver* h, double dt, hipStream_t s) {
 static int refresh_trace(zj_solver* h, hipStream_t s) {
     hipError_t e;
     if (h->prec == ZJ_PREC_FP64)
-        e = launch_trace<double>(h->P, TraceArgs<double>{h->E, h->U, (double*)h->T[0]}, h->ops64, 0, s);
+        e = launch_trace<double>(h->P, TraceArgs<double>{h->E, h->U, (double*)h->T[0], h->xcd_groups}, h->ops64, 0, s);
     else
-        e = launch_trace<float>(h->P, TraceArgs<float>{h->E, h->U, (float*)h->T[0]}, h->ops32, 0, s);
+        e = launch_trace<float>(h->P, TraceArgs<float>{h->E, h->U, (float*)h->T[0], h->xcd_groups}, h->ops32, 0, s);
     ZJ_HIP(h, e);
     return ZJ_OK;
 }
@@ -521,6 +556,37 @@ extern "C" int zj_set_tile(zj_handle h, int epb) {
     h->epb = epb;
     h->nblk = num_blocks(std::max(h->E, 1), epb);
     drop_graph(h);
+    // traces of edges inside a tile never reach T; a new tiling needs all of them
+    if (h->has_state) {
+        int rc = refresh_trace(h, h->stream);
+        if (rc) return rc;
+        ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_set_xcd_groups(zj_handle h, int groups) {
+    if (!h || groups < 1 || groups > 64) return ZJ_ERR_ARG;
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "launch change in the middle of a step");
+    if (groups == h->xcd_groups) return ZJ_OK;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    h->xcd_groups = groups;
+    drop_graph(h);
+    return ZJ_OK;
+}
+
+extern "C" int zj_set_geometry_mode(zj_handle h, int mode, int* active) {
+    if (!h || mode < -1 || mode > 1) return ZJ_ERR_ARG;
+    if (mode == 1 && !h->affine_ok) return fail(h, ZJ_ERR_UNSUPPORTED, "the mesh has non-affine elements");
+    if (mode >= 0 && (mode == 1) != h->affine) {
+        if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "geometry change in the middle of a step");
+        ZJ_HIP(h, hipSetDevice(h->device));
+        ZJ_HIP(h, hipStreamSynchronize(h->stream));
+        h->affine = mode == 1;
+        drop_graph(h);
+    }
+    if (active) *active = h->affine ? 1 : 0;
     return ZJ_OK;
 }
 

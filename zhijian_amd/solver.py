@@ -186,7 +186,18 @@ class Solver:
     def set_tile(self, epb):
         self._chk(lib().zj_set_tile(self._h, int(epb)))
 
+    def set_xcd_groups(self, groups):
+        self._chk(lib().zj_set_xcd_groups(self._h, int(groups)))
+
+    def set_geometry_mode(self, mode):
+        """-1 query, 0 per-point metric, 1 per-element metric (affine meshes);
+        returns the mode in use."""
+        a = C.c_int()
+        self._chk(lib().zj_set_geometry_mode(self._h, int(mode), C.byref(a)))
+        return a.value
+
     def info(self):
         n = C.c_int64(); e = C.c_int(); s = C.c_int()
         self._chk(lib().zj_get_info(self._h, C.byref(n), C.byref(e), C.byref(s)))
-        return dict(n_dof=n.value, elems_per_block=e.value, n_sp=s.value)
+        return dict(n_dof=n.value, elems_per_block=e.value, n_sp=s.value,
+                    affine=bool(self.set_geometry_mode(-1)))
