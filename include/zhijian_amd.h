@@ -119,6 +119,8 @@ int zj_get_state_device(zj_handle h, double* d_U_aos, void* stream);
 /* dUdt of the current state into dUdt_aos (host pointer; NULL keeps it on the
  * device for zj_residual_l2).  Synchronises when dUdt_aos is not NULL. */
 int zj_residual(zj_handle h, double* dUdt_aos, void* stream);
+/* The same into a device buffer [E][S][4]; asynchronous on `stream`. */
+int zj_residual_device(zj_handle h, double* d_dUdt_aos, void* stream);
 /* One SSP-RK3 step of size dt: three fused stage kernels (hipGraph replay). */
 int zj_rk3_step(zj_handle h, double dt, void* stream);
 /* One stage (1, 2 or 3) of the step, for callers that time or interleave them.
@@ -161,6 +163,28 @@ int zj_set_geometry_mode(zj_handle h, int mode, int* active);
  * tiles, one per XCD (default: the device's XCD count; 1 = launch order).
  * Results do not depend on it. */
 int zj_set_xcd_groups(zj_handle h, int groups);
+
+/* ---- discrete adjoint (no reference counterpart) ----------------------
+ * Vector-Jacobian products of the reference-arithmetic path: the derivative
+ * of every executed operation (fluxes kernels.cu:157-194, Rusanov and ghost
+ * states :453-701, flux difference :346-450, divergence and correction
+ * :813-1034, SSP-RK3 and the limiter :1236-1349), taking the branch of each
+ * clamp the forward took.
+ *
+ * The handle keeps the cotangent of its state on the device.  Reverse time
+ * stepping: zj_adjoint_set(dJ/dU^N); then for n = N-1 .. 0:
+ * zj_set_state(U^n), zj_rk3_step_vjp(dt_n) -> the cotangent of U^n (the step
+ * is recomputed from U^n); finally zj_adjoint_get.  dt_bar (may be NULL)
+ * receives dJ/d(dt_n) for that step. */
+int zj_adjoint_set(zj_handle h, const double* ubar_aos);
+int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void* stream);
+int zj_adjoint_get(zj_handle h, double* ubar_aos);
+int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stream);
+int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream);
+/* (dR/dU)^T w at the handle's state, R = FRSolver::computeResidual_GPU's dU/dt
+ * (fr_solver.cpp:335-608); w and ubar are [E][S][4]. */
+int zj_residual_vjp(zj_handle h, const double* w_aos, double* ubar_aos);
+int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream);
 
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times

@@ -53,7 +53,9 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_get_info", "zj_quad_operators", "zj_build_faces", "zj_pair_periodic",
            "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe", "zj_set_boundary",
            "zj_rk3_combine", "zj_euler_update", "zj_synchronize", "zj_set_geometry_mode",
-           "zj_set_xcd_groups")
+           "zj_set_xcd_groups", "zj_adjoint_set", "zj_adjoint_set_device", "zj_adjoint_get",
+           "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
+           "zj_residual_device")
 
 
 def lib() -> C.CDLL:
@@ -80,6 +82,14 @@ def lib() -> C.CDLL:
         L.zj_set_tile.argtypes = [H, C.c_int]
         L.zj_set_geometry_mode.argtypes = [H, C.c_int, C.POINTER(C.c_int)]
         L.zj_set_xcd_groups.argtypes = [H, C.c_int]
+        L.zj_adjoint_set.argtypes = [H, _dp]
+        L.zj_adjoint_set_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_adjoint_get.argtypes = [H, _dp]
+        L.zj_adjoint_get_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_rk3_step_vjp.argtypes = [H, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+        L.zj_residual_vjp.argtypes = [H, _dp, _dp]
+        L.zj_residual_vjp_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.zj_residual_device.argtypes = [H, C.c_void_p, C.c_void_p]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
         L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]

@@ -1,0 +1,52 @@
+// zj_adjoint.hpp -- argument blocks and launch wrappers of the stage VJP
+// kernels (zj_adjoint.hip).
+#pragma once
+
+#include "zj_kernels.hpp"
+
+namespace zj {
+
+// One adjoint launch for a stage u_n = a U^n + b q + c dt R(q), u_out = limiter(u_n)
+// (MODE_STAGE1..3), or for the residual alone (MODE_RESIDUAL: Ub_in = cotangent of R).
+template <typename R>
+struct VjpArgs {
+    int E;
+    // forward inputs of the stage, as in StageArgs
+    const double* U_in;      // q (FP64 form)
+    const double* U0;        // U^n (stages 2, 3)
+    const float* dW_in;      // FP32-bulk: q = U^n + dW_in (stages 2, 3)
+    const R* Jinv;
+    const R* T_in;           // traces of q on edges read across tiles
+    const EdgeLink* links;
+    const R* normals;
+    const double* bc_data;
+    const double* dt_ptr;
+    // cotangents
+    const double* Ub_in;     // partial cotangent of u_out (planes), or of R
+    const R* Tb_in;          // cross-tile edge cotangents completing Ub_in (nullptr: none)
+    double* Ub_out;          // partial cotangent of q (may alias Ub_in)
+    R* Tb_out;               // cross-tile edge cotangents of q, written at the neighbour's slots
+    double* U0b;             // cotangent of U^n: stage 3 writes, stage 2 adds, stage 1 reads
+    double* dtb_partial;     // per-tile sum of c * R . du_n (nullptr: skip)
+    double gamma;
+    int positivity;
+    int xcd_groups;
+};
+
+template <typename R>
+struct CompleteArgs {
+    int E;
+    const double* Ub;
+    const R* Tb;
+    double* out;
+    const EdgeLink* links;
+    int xcd_groups;
+};
+
+int adjoint_tile(int P);
+template <typename R>
+hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, bool affine, hipStream_t s);
+template <typename R>
+hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& ops, hipStream_t s);
+
+}  // namespace zj

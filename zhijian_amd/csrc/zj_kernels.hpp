@@ -93,7 +93,9 @@ hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S,
                              double cfl, double cf, double* partial, int nblk, hipStream_t s);
 hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count,
                                   int nblk, hipStream_t s);
-// Deterministic reduction of block partials: out_sum = sqrt(sum) (l2) or min.
-hipError_t launch_finalize(const double* partial, int n, int op_min, double* out, hipStream_t s);
+// Deterministic reduction of block partials (fixed order): sqrt of the sum
+// (an L2 norm), the minimum, or the plain sum.
+enum FinalizeOp : int { FIN_SQRT_SUM = 0, FIN_MIN = 1, FIN_SUM = 2 };
+hipError_t launch_finalize(const double* partial, int n, int op, double* out, hipStream_t s);
 
 }  // namespace zj

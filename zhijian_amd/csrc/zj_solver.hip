@@ -13,6 +13,7 @@
 //   links       [E][4] EdgeLink; normals [E][4][2] (face normal of the left side)
 #include "zhijian_amd.h"
 #include "zj_kernels.hpp"
+#include "zj_adjoint.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -70,6 +71,19 @@ struct zj_solver {
     std::vector<int> bface;      // creation-order index of each boundary face
     std::vector<int> bface_slot; // element*4 + local edge of that face
     int n_bc_alloc = 0;          // entries of bc_data allocated on the device
+    // adjoint workspace, allocated on first use
+    bool adj_ready = false;
+    int adj_epb = 0;               // tile of the adjoint launches and their forward recompute
+    int adj_nt = 0;                // tiles at adj_epb
+    double* A_W1 = nullptr;        // U1 (FP64) or FP32 increment U1 - U^n
+    double* A_W2 = nullptr;        // U2, likewise
+    double* A_Ub = nullptr;        // partial cotangent of the current state (planes)
+    double* A_U0b = nullptr;       // cotangent of U^n within a step
+    void* A_Tb[2] = {nullptr, nullptr};  // cross-tile edge cotangents, ping-pong
+    double* A_dtb = nullptr;       // per-tile dt cotangents of the three stages
+    int A_cur = 0;
+    bool A_has_tb = false;
+    bool A_seeded = false;
 };
 
 // Boundary link of one face (kernels.cu:584-595): a missing entry falls back
@@ -164,7 +178,8 @@ static void release(zj_solver* h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);
     void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
-                    h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count};
+                    h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
+                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -461,6 +476,17 @@ extern "C" int zj_residual(zj_handle h, double* dUdt_aos, void* stream) {
     return ZJ_OK;
 }
 
+extern "C" int zj_residual_device(zj_handle h, double* d_dUdt_aos, void* stream) {
+    if (!h || !d_dUdt_aos) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual requested in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, enqueue_stage(h, MODE_RESIDUAL, s));
+    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_dUdt_aos, (int64_t)h->E * h->S, s));
+    return ZJ_OK;
+}
+
 extern "C" int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream) {
     if (!h) return ZJ_ERR_ARG;
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
@@ -512,7 +538,7 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial_dt,
                                 h->nred, h->stream));
-    ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, 1, h->d_scalars + 2, h->stream));
+    ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, h->stream));
     ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     *dt = h->h_pinned[0];
@@ -524,7 +550,7 @@ extern "C" int zj_residual_l2(zj_handle h, double* l2) {
     if (h->l2_blocks == 0) return fail(h, ZJ_ERR_STATE, "no residual has been evaluated");
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipDeviceSynchronize());  // the last residual may have run on a caller's stream
-    ZJ_HIP(h, launch_finalize(h->partial, h->l2_blocks, 0, h->d_scalars + 1, h->stream));
+    ZJ_HIP(h, launch_finalize(h->partial, h->l2_blocks, FIN_SQRT_SUM, h->d_scalars + 1, h->stream));
     ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     *l2 = h->h_pinned[0];
@@ -693,6 +719,214 @@ extern "C" int zj_euler_update(double* U, const double* R, int64_t n, double dt,
 extern "C" int zj_synchronize(zj_handle h) {
     if (!h) return ZJ_ERR_ARG;
     ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+// Discrete adjoint of the SSP-RK3 step (no reference counterpart): the
+// cotangent of the handle's state is kept on the device as a partial value
+// plus the cross-tile edge cotangents of the last adjoint launch, which the
+// next launch (or zj_adjoint_get) folds in.
+static int ensure_adjoint(zj_solver* h) {
+    if (h->adj_ready) return ZJ_OK;
+    const size_t npts = (size_t)h->E * h->S;
+    const size_t rsz = h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const size_t ntr = std::max<size_t>((size_t)h->E * 4 * h->N1 * 4, 1);
+    h->adj_epb = adjoint_tile(h->P);
+    h->adj_nt = num_blocks(std::max(h->E, 1), h->adj_epb);
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_W1, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_W2, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_Ub, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_U0b, std::max<size_t>(npts * 4, 1) * sizeof(double)));
+    for (int k = 0; k < 2; ++k) ZJ_HIP(h, hipMalloc(&h->A_Tb[k], ntr * rsz));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * h->adj_nt * sizeof(double)));
+    h->adj_ready = true;
+    return ZJ_OK;
+}
+
+template <typename R>
+static VjpArgs<R> vjp_args(zj_solver* h) {
+    VjpArgs<R> a{};
+    a.E = h->E;
+    a.Jinv = (const R*)(h->affine ? h->Jinv_e : h->Jinv);
+    a.links = h->links;
+    a.normals = (const R*)h->normals;
+    a.bc_data = h->bc_data;
+    a.dt_ptr = h->d_scalars;
+    a.gamma = h->gamma;
+    a.positivity = h->positivity;
+    a.xcd_groups = h->xcd_groups;
+    return a;
+}
+
+// The forward stages 1 and 2 at the adjoint tile, keeping U1 and U2 apart.
+template <typename R>
+static hipError_t recompute_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s) {
+    R** T = (R**)h->T;
+    const bool inc = std::is_same<R, float>::value;
+    StageArgs<R> a = stage_args<R>(h, MODE_STAGE1);
+    a.T_in = T[0]; a.T_out = T[1]; a.l2_partial = nullptr;
+    if (inc) { a.dW_out = (float*)h->A_W1; a.U_out = nullptr; } else { a.U_out = h->A_W1; }
+    hipError_t e = launch_stage<R>(h->P, MODE_STAGE1, a, ops, h->adj_epb, h->affine, s);
+    if (e != hipSuccess) return e;
+    StageArgs<R> b = stage_args<R>(h, MODE_STAGE2);
+    b.T_in = T[1]; b.T_out = T[2]; b.l2_partial = nullptr;
+    if (inc) { b.dW_in = (const float*)h->A_W1; b.dW_out = (float*)h->A_W2; b.U_in = nullptr; b.U_out = nullptr; }
+    else { b.U_in = h->A_W1; b.U_out = h->A_W2; }
+    return launch_stage<R>(h->P, MODE_STAGE2, b, ops, h->adj_epb, h->affine, s);
+}
+
+template <typename R>
+static hipError_t adjoint_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s) {
+    R** T = (R**)h->T;
+    R** Tb = (R**)h->A_Tb;
+    const bool inc = std::is_same<R, float>::value;
+    for (int st = 3; st >= 1; --st) {
+        VjpArgs<R> a = vjp_args<R>(h);
+        double* W = st == 3 ? h->A_W2 : h->A_W1;
+        if (st > 1) {
+            a.U0 = h->U;
+            if (inc) a.dW_in = (const float*)W; else a.U_in = W;
+        } else {
+            a.U_in = h->U;
+        }
+        a.T_in = T[st - 1];
+        a.Ub_in = h->A_Ub;
+        a.Ub_out = h->A_Ub;
+        a.Tb_in = h->A_has_tb ? Tb[h->A_cur] : nullptr;
+        a.Tb_out = Tb[1 - h->A_cur];
+        a.U0b = h->A_U0b;
+        a.dtb_partial = h->A_dtb + (size_t)(3 - st) * h->adj_nt;
+        hipError_t e = launch_vjp<R>(h->P, st, a, ops, h->affine, s);
+        if (e != hipSuccess) return e;
+        h->A_cur = 1 - h->A_cur;
+        h->A_has_tb = true;
+    }
+    return hipSuccess;
+}
+
+extern "C" int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void* stream) {
+    if (!h || !d_ubar_aos) return ZJ_ERR_ARG;
+    int rc = ensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, launch_aos_to_soa(d_ubar_aos, h->A_Ub, (int64_t)h->E * h->S, pick(h, stream)));
+    h->A_has_tb = false;
+    h->A_seeded = true;
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_set(zj_handle h, const double* ubar_aos) {
+    if (!h || !ubar_aos) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->aos, ubar_aos, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int rc = zj_adjoint_set_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+template <typename R>
+static hipError_t complete_into(zj_solver* h, const double* Ub, const void* Tb, double* out, const OpTables<R>& ops,
+                                hipStream_t s) {
+    CompleteArgs<R> c{h->E, Ub, (const R*)Tb, out, h->links, h->xcd_groups};
+    return launch_complete<R>(h->P, c, ops, s);
+}
+
+extern "C" int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stream) {
+    if (!h || !d_ubar_aos) return ZJ_ERR_ARG;
+    if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    const void* tb = h->A_has_tb ? h->A_Tb[h->A_cur] : nullptr;
+    hipError_t e = h->prec == ZJ_PREC_FP64 ? complete_into<double>(h, h->A_Ub, tb, h->Rres, h->ops64, s)
+                                           : complete_into<float>(h, h->A_Ub, tb, h->Rres, h->ops32, s);
+    ZJ_HIP(h, e);
+    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_ubar_aos, (int64_t)h->E * h->S, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_get(zj_handle h, double* ubar_aos) {
+    if (!h || !ubar_aos) return ZJ_ERR_ARG;
+    int rc = zj_adjoint_get_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(ubar_aos, h->aos, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, set_dt(h, dt, s));
+    int rc = refresh_trace(h, s);  // traces of U^n on every edge, valid for the adjoint tile
+    if (rc) return rc;
+    hipError_t e;
+    if (h->prec == ZJ_PREC_FP64) {
+        e = recompute_stages<double>(h, h->ops64, s);
+        if (e == hipSuccess) e = adjoint_stages<double>(h, h->ops64, s);
+    } else {
+        e = recompute_stages<float>(h, h->ops32, s);
+        if (e == hipSuccess) e = adjoint_stages<float>(h, h->ops32, s);
+    }
+    ZJ_HIP(h, e);
+    if (dt_bar) {
+        ZJ_HIP(h, launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_scalars + 3, s));
+        ZJ_HIP(h, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+        ZJ_HIP(h, hipStreamSynchronize(s));
+        *dt_bar = h->h_pinned[3];
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream) {
+    if (!h || !d_w_aos || !d_ubar_aos) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual VJP requested in the middle of a step");
+    int rc = ensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, launch_aos_to_soa(d_w_aos, h->Rres, (int64_t)h->E * h->S, s));
+    rc = refresh_trace(h, s);
+    if (rc) return rc;
+    // the adjoint accumulator's pending edge cotangents sit in A_Tb[A_cur]; use the other buffer
+    void* tb = h->A_Tb[1 - h->A_cur];
+    hipError_t e;
+    if (h->prec == ZJ_PREC_FP64) {
+        VjpArgs<double> a = vjp_args<double>(h);
+        a.U_in = h->U; a.T_in = (const double*)h->T[0];
+        a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (double*)tb;
+        e = launch_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->affine, s);
+        if (e == hipSuccess) e = complete_into<double>(h, h->Rres, tb, h->W, h->ops64, s);
+    } else {
+        VjpArgs<float> a = vjp_args<float>(h);
+        a.U_in = h->U; a.T_in = (const float*)h->T[0];
+        a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (float*)tb;
+        e = launch_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->affine, s);
+        if (e == hipSuccess) e = complete_into<float>(h, h->Rres, tb, h->W, h->ops32, s);
+    }
+    ZJ_HIP(h, e);
+    ZJ_HIP(h, launch_soa_to_aos(h->W, d_ubar_aos, (int64_t)h->E * h->S, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_residual_vjp(zj_handle h, const double* w_aos, double* ubar_aos) {
+    if (!h || !w_aos || !ubar_aos) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->aos, w_aos, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int rc = zj_residual_vjp_device(h, h->aos, h->aos, nullptr);  // w is consumed before aos is rewritten
+    if (rc) return rc;
+    ZJ_HIP(h, hipMemcpyAsync(ubar_aos, h->aos, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     return ZJ_OK;
 }

@@ -157,6 +157,26 @@ class Solver:
         self._chk(lib().zj_get_state(self._h, out))
         return out
 
+    # device-pointer forms (ints from e.g. torch's data_ptr(); stream = hipStream_t or None)
+    def set_state_device(self, ptr, stream=None):
+        self._chk(lib().zj_set_state_device(self._h, ptr, stream))
+
+    def get_state_device(self, ptr, stream=None):
+        self._chk(lib().zj_get_state_device(self._h, ptr, stream))
+
+    def adjoint_set_device(self, ptr, stream=None):
+        self._chk(lib().zj_adjoint_set_device(self._h, ptr, stream))
+
+    def adjoint_get_device(self, ptr, stream=None):
+        self._chk(lib().zj_adjoint_get_device(self._h, ptr, stream))
+
+    def residual_vjp_device(self, w_ptr, out_ptr, stream=None):
+        self._chk(lib().zj_residual_vjp_device(self._h, w_ptr, out_ptr, stream))
+
+    def residual_device(self, out_ptr, stream=None):
+        """dU/dt into a device AoS buffer [E][S][4]."""
+        self._chk(lib().zj_residual_device(self._h, out_ptr, stream))
+
     def residual(self, fetch=True, stream=None):
         out = np.zeros(self.n) if fetch else None
         self._chk(lib().zj_residual(self._h, None if out is None else out.ctypes.data, stream))
@@ -185,6 +205,32 @@ class Solver:
 
     def set_tile(self, epb):
         self._chk(lib().zj_set_tile(self._h, int(epb)))
+
+    # ---- discrete adjoint
+    def residual_vjp(self, w):
+        """(dR/dU)^T w at the current state."""
+        w = np.ascontiguousarray(w, np.float64).reshape(-1)
+        assert w.size == self.n
+        out = np.empty(self.n)
+        self._chk(lib().zj_residual_vjp(self._h, w, out))
+        return out
+
+    def adjoint_set(self, ubar):
+        ubar = np.ascontiguousarray(ubar, np.float64).reshape(-1)
+        assert ubar.size == self.n
+        self._chk(lib().zj_adjoint_set(self._h, ubar))
+
+    def adjoint_get(self):
+        out = np.empty(self.n)
+        self._chk(lib().zj_adjoint_get(self._h, out))
+        return out
+
+    def rk3_step_vjp(self, dt, stream=None):
+        """Maps the held cotangent of U^{n+1} to that of U^n (the state set on
+        the handle); returns dJ/d(dt)."""
+        dtb = C.c_double()
+        self._chk(lib().zj_rk3_step_vjp(self._h, float(dt), C.byref(dtb), stream))
+        return dtb.value
 
     def set_xcd_groups(self, groups):
         self._chk(lib().zj_set_xcd_groups(self._h, int(groups)))
