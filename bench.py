@@ -60,6 +60,34 @@ def cross_tile_edges(faces, epb):
     return int(2 * np.count_nonzero(f[inner, 0] // epb != f[inner, 2] // epb))
 
 
+def time_adjoint(n, p, precision, steps, warmup, stream, hiprt):
+    """Forward SSP-RK3 step vs its discrete adjoint (stage recompute + three
+    VJP launches) on the TGV n x n, order p, FP64 state; ms per step each."""
+    from zhijian_amd import adjoint, cases, solver
+    mesh = cases.tgv_mesh(n)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+    sol = solver.Solver(t, precision=precision, cfl=0.5)
+    U0 = cases.tgv_ic(t.pos, mach=0.1).reshape(-1)
+    sol.set_state(U0)
+    dt = sol.dt_cfl(0.5)
+    ms_fwd, _ = time_precision(sol, U0, dt, steps, warmup, stream, hiprt)
+    sol.set_state(U0)
+    sol.adjoint_set(adjoint.kinetic_energy_grad(U0, t))
+    for _ in range(warmup):
+        sol.rk3_step_vjp(dt, stream.ptr, want_dt=False)
+    stream.synchronize()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        sol.rk3_step_vjp(dt, stream.ptr, want_dt=False)
+    e1.record(stream)
+    e1.synchronize()
+    ms_adj = e0.elapsed_ms(e1) / steps
+    return dict(workload=f"TGV {n}x{n} P{p}", precision=precision, fwd_ms_per_step=ms_fwd,
+                adj_ms_per_step=ms_adj, adjoint_over_forward=ms_adj / ms_fwd,
+                note="adjoint step = traces of U^n + stages 1-2 recomputed + 3 VJP launches")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -131,6 +159,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the adjoint/forward cost ratio (TGV 128^2 P3)")
     ap.add_argument("--only", choices=("fp64", "fp32bulk"), default=None,
                     help="time one precision only (profiling runs)")
     ap.add_argument("--probe-bytes", type=float, default=1 << 30,
@@ -202,6 +232,12 @@ def main():
                    rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
         log(f"fp32bulk vs fp64 after {args.err_steps} steps: {err}")
 
+    adj = {}
+    if args.adjoint:
+        for prec in ("fp64", "fp32bulk"):
+            adj[prec] = time_adjoint(128, 3, prec, args.steps, args.warmup, stream, hiprt)
+            log(f"adjoint {prec}: {adj[prec]}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.p, U0, dt, args.cpu_seconds)
@@ -233,7 +269,8 @@ def main():
         "fp64_plain": res["fp64"],
         "fp32bulk_over_fp64": head["dof_updates_per_s"] / res["fp64"]["dof_updates_per_s"],
         "fp32bulk_error_vs_fp64": err,
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": adj.get("fp64", {}).get("adjoint_over_forward"),
+        "adjoint": adj or None,
         "copy_kernel_gbps": copy_gbps,
     }
     print(json.dumps(out), flush=True)

@@ -186,6 +186,26 @@ int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream);
 int zj_residual_vjp(zj_handle h, const double* w_aos, double* ubar_aos);
 int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream);
 
+/* ---- batches and tuning (no reference counterpart) -------------------
+ * A batch steps many independent meshes (each its own zj_handle, own order,
+ * state and dt) with one launch per stage and order/metric group, the groups
+ * running side by side in one captured graph.  Members keep working on their
+ * own; a member must not be stepped on another stream while its batch runs.
+ * Members must use the full tile of their order (the default). */
+typedef struct zj_batch* zj_batch_handle;
+int zj_batch_create(zj_batch_handle* out, const zj_handle* members, int n_members);
+int zj_batch_destroy(zj_batch_handle b);
+const char* zj_batch_last_error(zj_batch_handle b);
+/* dts[i] is member i's step (FRSolver::advance, fr_solver.cpp:305-333, for all members). */
+int zj_batch_rk3_step(zj_batch_handle b, const double* dts, void* stream);
+int zj_batch_synchronize(zj_batch_handle b);
+/* Workgroups of each group's launch: 0 = one per work item (element tile);
+ * k > 0 = k per CU, each looping over its share of the items (persistent). */
+int zj_batch_set_grid(zj_batch_handle b, int policy);
+/* Times the residual launch for every tile (full, half) and XCD order and
+ * keeps the fastest; results do not depend on the choice. */
+int zj_autotune(zj_handle h, int reps, int* elems_per_block, int* xcd_groups, double* ms);
+
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times
  * with a plain grid-stride copy kernel and reports the mean GB/s (read + write

@@ -55,7 +55,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_rk3_combine", "zj_euler_update", "zj_synchronize", "zj_set_geometry_mode",
            "zj_set_xcd_groups", "zj_adjoint_set", "zj_adjoint_set_device", "zj_adjoint_get",
            "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
-           "zj_residual_device")
+           "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
+           "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune")
 
 
 def lib() -> C.CDLL:
@@ -90,6 +91,14 @@ def lib() -> C.CDLL:
         L.zj_residual_vjp.argtypes = [H, _dp, _dp]
         L.zj_residual_vjp_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
         L.zj_residual_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_batch_create.argtypes = [C.POINTER(H), C.POINTER(H), C.c_int]
+        L.zj_batch_destroy.argtypes = [H]
+        L.zj_batch_last_error.argtypes = [H]
+        L.zj_batch_last_error.restype = C.c_char_p
+        L.zj_batch_rk3_step.argtypes = [H, _dp, C.c_void_p]
+        L.zj_batch_synchronize.argtypes = [H]
+        L.zj_batch_set_grid.argtypes = [H, C.c_int]
+        L.zj_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
         L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]

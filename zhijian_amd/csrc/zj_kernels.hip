@@ -28,6 +28,7 @@
 
 #include "zj_device.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <type_traits>
@@ -36,8 +37,32 @@ namespace zj {
 
 namespace {
 
-template <typename R, int P, int EPB, int MODE, bool AFF>
-__global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
+// LDS of one element tile of the stage kernel, as a policy of accessors.
+// SepSmem gives every array its own LDS object (distinct objects never alias,
+// which the scheduler exploits); the batched kernel carves the same arrays
+// out of one buffer (BatchSmem, below).
+template <typename R, int P, int EPB>
+struct SepSmem {
+    using Y = Lay<R, P>;
+    static constexpr int N1 = P + 1, S = N1 * N1;
+    __device__ static R* U() { __shared__ R a[EPB * Y::UE]; return a; }
+    __device__ static R* Fx() { __shared__ R a[EPB * Y::UE]; return a; }
+    __device__ static R* Fy() { __shared__ R a[EPB * Y::UE]; return a; }
+    __device__ static R* Tr() { __shared__ R a[EPB * Y::TE]; return a; }
+    __device__ static R* Fd() { __shared__ R a[EPB * Y::TE]; return a; }
+    __device__ static R* D() { __shared__ R a[N1 * N1]; return a; }
+    __device__ static R* Lm() { __shared__ R a[N1]; return a; }
+    __device__ static R* Lp() { __shared__ R a[N1]; return a; }
+    __device__ static R* C() { __shared__ R a[4 * S]; return a; }
+    __device__ static double* Red() { __shared__ double a[BLOCK / 64]; return a; }
+    __device__ static unsigned char* Out() { __shared__ unsigned char a[EPB * 4]; return a; }
+};
+
+// One stage over tile `tile` (elements tile*EPB ...) as thread `tid`, shared
+// by the single-mesh kernel and the batched persistent kernel.
+template <typename R, int P, int EPB, int MODE, bool AFF, class SM>
+__device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables<R>& ops, int tile, const SM& sm,
+                                           int tid) {
     using Y = Lay<R, P>;
     // FP32-bulk keeps the intermediate stage state as FP32 increments over U^n
     constexpr bool INC = std::is_same<R, float>::value;
@@ -45,17 +70,19 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
     static_assert(NT <= BLOCK, "tile exceeds the workgroup");
-    __shared__ R sU[EPB * Y::UE];
-    __shared__ R sFx[EPB * Y::UE];
-    __shared__ R sFy[EPB * Y::UE];
-    __shared__ R sTr[EPB * Y::TE];
-    __shared__ R sFd[EPB * Y::TE];
-    __shared__ R sD[N1 * N1], sLm[N1], sLp[N1], sC[4 * S];
-    __shared__ double sRed[BLOCK / 64];
-    __shared__ unsigned char sOut[EPB * 4];  // edge trace needed outside this tile
+    R* const sU = sm.U();
+    R* const sFx = sm.Fx();
+    R* const sFy = sm.Fy();
+    R* const sTr = sm.Tr();
+    R* const sFd = sm.Fd();
+    R* const sD = sm.D();
+    R* const sLm = sm.Lm();
+    R* const sLp = sm.Lp();
+    R* const sC = sm.C();
+    double* const sRed = sm.Red();
+    unsigned char* const sOut = sm.Out();  // edge trace needed outside this tile
 
-    const int tid = threadIdx.x;
-    const int e0 = tile_of_block(a.xcd_groups) * EPB;
+    const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma);
 
@@ -243,6 +270,57 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, cons
     }
 }
 
+template <typename R, int P, int EPB, int MODE, bool AFF>
+__global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
+    stage_body<R, P, EPB, MODE, AFF>(a, ops, tile_of_block(a.xcd_groups), SepSmem<R, P, EPB>{}, threadIdx.x);
+}
+
+// ------------------------------------------------------------ batched stage
+// The same arrays carved out of one LDS buffer.
+template <typename R, int P, int EPB>
+struct BatchSmem {
+    using Y = Lay<R, P>;
+    static constexpr int N1 = P + 1, S = N1 * N1;
+    static constexpr int oU = 0, oFx = oU + EPB * Y::UE, oFy = oFx + EPB * Y::UE, oTr = oFy + EPB * Y::UE;
+    static constexpr int oFd = oTr + EPB * Y::TE, oD = oFd + EPB * Y::TE, oLm = oD + N1 * N1, oLp = oLm + N1;
+    static constexpr int oC = oLp + N1, nR = oC + 4 * S;
+    static constexpr int red_byte = ((nR * (int)sizeof(R) + 15) / 16) * 16;
+    static constexpr int bytes = red_byte + (BLOCK / 64) * 8 + EPB * 4;
+    unsigned char* base;
+    __device__ R* U() const { return reinterpret_cast<R*>(base) + oU; }
+    __device__ R* Fx() const { return reinterpret_cast<R*>(base) + oFx; }
+    __device__ R* Fy() const { return reinterpret_cast<R*>(base) + oFy; }
+    __device__ R* Tr() const { return reinterpret_cast<R*>(base) + oTr; }
+    __device__ R* Fd() const { return reinterpret_cast<R*>(base) + oFd; }
+    __device__ R* D() const { return reinterpret_cast<R*>(base) + oD; }
+    __device__ R* Lm() const { return reinterpret_cast<R*>(base) + oLm; }
+    __device__ R* Lp() const { return reinterpret_cast<R*>(base) + oLp; }
+    __device__ R* C() const { return reinterpret_cast<R*>(base) + oC; }
+    __device__ double* Red() const { return reinterpret_cast<double*>(base + red_byte); }
+    __device__ unsigned char* Out() const { return base + red_byte + (BLOCK / 64) * 8; }
+};
+
+// One launch per (stage, order, metric form) for many meshes: workgroup b
+// takes work items b, b + grid, ... (instance, tile), items ordered by
+// instance size so that the static round-robin stays balanced.  Every
+// instance uses its order's full tile, the tiling its trace buffers were
+// written with.  The launches of one stage are independent branches of the
+// step graph, so they share the GPU.
+template <typename R, int P, int MODE, bool AFF>
+__global__ __launch_bounds__(BLOCK) void batched_stage_kernel(const BatchInst<R>* __restrict__ insts,
+                                                              const int2* __restrict__ items, int n_items) {
+    constexpr int T = BLOCK / ((P + 1) * (P + 1));
+    using SM = BatchSmem<R, P, T>;
+    __shared__ __align__(16) unsigned char raw[SM::bytes];
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const int2 it = items[w];
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // keeps the per-thread index math in the loop body (registers)
+        stage_body<R, P, T, MODE, AFF>(insts[it.x].args, insts[it.x].ops, it.y, SM{raw}, tid);
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
 template <typename R, int P, int EPB>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, const OpTables<R> ops) {
     using Y = Lay<R, P>;
@@ -306,6 +384,10 @@ __global__ void euler_update_kernel(double* U, const double* R, int64_t n, doubl
 
 __global__ void set_scalar_kernel(double* p, double v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void set_many_kernel(double* dst, ScalarChunk c) {
+    if (threadIdx.x < c.n) dst[threadIdx.x] = c.v[threadIdx.x];
 }
 
 // FRSolver::computeTimeStep per point (fr_solver.cpp:799-811, euler.hpp:34-41,166-171).
@@ -422,6 +504,40 @@ hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, in
     }
 }
 
+template <typename R, int P, bool AFF>
+static hipError_t batched_mode(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                               hipStream_t s) {
+    const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
+    switch (mode) {
+    case MODE_RESIDUAL: batched_stage_kernel<R, P, MODE_RESIDUAL, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
+    case MODE_STAGE1: batched_stage_kernel<R, P, MODE_STAGE1, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
+    case MODE_STAGE2: batched_stage_kernel<R, P, MODE_STAGE2, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
+    case MODE_STAGE3: batched_stage_kernel<R, P, MODE_STAGE3, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>* insts, const int2* items,
+                                int n_items, int grid, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+#define ZJ_BP(PP)                                                                                    \
+    case PP:                                                                                         \
+        return affine ? batched_mode<R, PP, true>(mode, insts, items, n_items, grid, s)              \
+                      : batched_mode<R, PP, false>(mode, insts, items, n_items, grid, s);
+    switch (P) {
+        ZJ_BP(1) ZJ_BP(2) ZJ_BP(3) ZJ_BP(4) ZJ_BP(5)
+    default: return hipErrorInvalidValue;
+    }
+#undef ZJ_BP
+}
+
+template hipError_t launch_batched_stage<double>(int, bool, int, const BatchInst<double>*, const int2*, int, int,
+                                                 hipStream_t);
+template hipError_t launch_batched_stage<float>(int, bool, int, const BatchInst<float>*, const int2*, int, int,
+                                                hipStream_t);
+
 template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, bool,
                                         hipStream_t);
 template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&, int, bool,
@@ -438,6 +554,16 @@ hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n, hipStrea
 hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     soa_to_aos_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(soa, aos, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_many(double* dst, const double* v, int n, hipStream_t s) {
+    for (int k = 0; k < n; k += 64) {
+        ScalarChunk c{};
+        c.n = std::min(64, n - k);
+        for (int t = 0; t < c.n; ++t) c.v[t] = v[k + t];
+        set_many_kernel<<<1, 64, 0, s>>>(dst + k, c);
+    }
     return hipGetLastError();
 }
 

@@ -71,6 +71,16 @@ struct TraceArgs {
     int xcd_groups;
 };
 
+// One mesh of a batched launch: its stage arguments for one mode, its
+// operator tables and order.
+template <typename R>
+struct BatchInst {
+    StageArgs<R> args;
+    OpTables<R> ops;
+    int P;
+    int affine;
+};
+
 // Launch wrappers (host).  `epb` = elements per workgroup (0 = default).
 template <typename R>
 hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
@@ -78,6 +88,11 @@ hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R
 template <typename R>
 hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, int epb,
                         hipStream_t s);
+// items[k] = (instance, tile) over instances of order P; insts holds the
+// instances' arguments for `mode`.
+template <typename R>
+hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>* insts, const int2* items,
+                                int n_items, int grid, hipStream_t s);
 int default_epb(int P);
 int max_epb(int P);
 int num_blocks(int E, int epb);
@@ -85,6 +100,12 @@ int num_blocks(int E, int epb);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n_points, hipStream_t s);
 hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n_points, hipStream_t s);
 hipError_t launch_set_scalar(double* p, double v, hipStream_t s);
+// dst[0..n) = v[0..n) by value through kernel arguments (no host staging)
+struct ScalarChunk {
+    double v[64];
+    int n;
+};
+hipError_t launch_set_many(double* dst, const double* v, int n, hipStream_t s);
 hipError_t launch_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
                               int64_t n, double dt, hipStream_t s);
 hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt, hipStream_t s);

@@ -930,3 +930,289 @@ extern "C" int zj_residual_vjp(zj_handle h, const double* w_aos, double* ubar_ao
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     return ZJ_OK;
 }
+
+// ---------------------------------------------------------------- batches
+// Many meshes stepped together (no reference counterpart): per stage, one
+// launch per (order, metric form) group over all member tiles, the groups as
+// parallel branches of one captured graph.
+struct zj_batch {
+    int device = 0;
+    int prec = ZJ_PREC_FP64;
+    hipStream_t stream = nullptr;
+    std::vector<zj_solver*> m;
+    struct Group {
+        int P = 0;
+        bool affine = false;
+        int n_items = 0;
+        int2* items = nullptr;
+        void* insts[4] = {nullptr, nullptr, nullptr, nullptr};  // BatchInst<R>[members of the group] per mode
+        hipStream_t branch = nullptr;
+        hipEvent_t join = nullptr;
+    };
+    std::vector<Group> groups;
+    double* d_dt = nullptr;
+    std::vector<double> dt_on_device;
+    hipEvent_t fork = nullptr;
+    int policy = 0;  // 0: one work item per workgroup; k > 0: k workgroups per CU, persistent
+    int n_cu = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+};
+
+static std::string g_batch_error;
+
+static void batch_release(zj_batch* b) {
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto& g : b->groups) {
+        if (g.items) hipFree(g.items);
+        for (void* p : g.insts)
+            if (p) hipFree(p);
+        if (g.join) hipEventDestroy(g.join);
+        if (g.branch) hipStreamDestroy(g.branch);
+    }
+    if (b->d_dt) hipFree(b->d_dt);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->stream) hipStreamDestroy(b->stream);
+}
+
+#define ZJ_BHIP(b, call)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return ZJ_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+
+template <typename R>
+static int batch_upload_group(zj_batch* b, zj_batch::Group& g, const std::vector<int>& members) {
+    for (int mode = MODE_STAGE1; mode <= MODE_STAGE3; ++mode) {
+        std::vector<BatchInst<R>> v;
+        for (int idx : members) {
+            zj_solver* h = b->m[idx];
+            BatchInst<R> in{};
+            in.args = stage_args<R>(h, mode);
+            in.args.dt_ptr = b->d_dt + idx;
+            in.ops = *reinterpret_cast<const OpTables<R>*>(std::is_same<R, double>::value ? (const void*)&h->ops64
+                                                                                             : (const void*)&h->ops32);
+            in.P = h->P;
+            in.affine = h->affine ? 1 : 0;
+            v.push_back(in);
+        }
+        ZJ_BHIP(b, hipMalloc(&g.insts[mode], v.size() * sizeof(BatchInst<R>)));
+        ZJ_BHIP(b, hipMemcpy(g.insts[mode], v.data(), v.size() * sizeof(BatchInst<R>), hipMemcpyHostToDevice));
+    }
+    return ZJ_OK;
+}
+
+static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
+    b->m.assign(hs, hs + n);
+    b->device = hs[0]->device;
+    b->prec = hs[0]->prec;
+    for (int i = 0; i < n; ++i) {
+        zj_solver* h = hs[i];
+        if (!h) { b->err = "null member"; return ZJ_ERR_ARG; }
+        if (h->device != b->device || h->prec != b->prec) {
+            b->err = "members must share the device and the precision";
+            return ZJ_ERR_ARG;
+        }
+        if (h->epb != max_epb(h->P)) {
+            b->err = "members must use the full tile of their order";
+            return ZJ_ERR_UNSUPPORTED;
+        }
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == h) { b->err = "a handle appears twice"; return ZJ_ERR_ARG; }
+    }
+    ZJ_BHIP(b, hipSetDevice(b->device));
+    ZJ_BHIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    ZJ_BHIP(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    hipDeviceProp_t prop;
+    ZJ_BHIP(b, hipGetDeviceProperties(&prop, b->device));
+    b->n_cu = std::max(1, prop.multiProcessorCount);
+    ZJ_BHIP(b, hipMalloc((void**)&b->d_dt, (size_t)n * sizeof(double)));
+    b->dt_on_device.assign(n, NAN);
+    for (int P = 1; P <= 5; ++P)
+        for (int af = 0; af < 2; ++af) {
+            std::vector<int> members;
+            for (int i = 0; i < n; ++i)
+                if (hs[i]->P == P && (hs[i]->affine ? 1 : 0) == af) members.push_back(i);
+            if (members.empty()) continue;
+            // larger meshes first; tiles of one mesh stay consecutive
+            std::stable_sort(members.begin(), members.end(), [&](int x, int y) { return hs[x]->E > hs[y]->E; });
+            zj_batch::Group g;
+            g.P = P;
+            g.affine = af != 0;
+            std::vector<int2> items;
+            for (size_t k = 0; k < members.size(); ++k)
+                for (int t = 0; t < hs[members[k]]->nblk; ++t) items.push_back(make_int2((int)k, t));
+            g.n_items = (int)items.size();
+            b->groups.push_back(g);
+            zj_batch::Group& gg = b->groups.back();
+            ZJ_BHIP(b, hipMalloc((void**)&gg.items, std::max<size_t>(items.size(), 1) * sizeof(int2)));
+            if (!items.empty())
+                ZJ_BHIP(b, hipMemcpy(gg.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+            ZJ_BHIP(b, hipStreamCreateWithFlags(&gg.branch, hipStreamNonBlocking));
+            ZJ_BHIP(b, hipEventCreateWithFlags(&gg.join, hipEventDisableTiming));
+            int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_group<double>(b, gg, members)
+                                             : batch_upload_group<float>(b, gg, members);
+            if (rc) return rc;
+        }
+    return ZJ_OK;
+}
+
+static int batch_grid(const zj_batch* b, const zj_batch::Group& g) {
+    return b->policy <= 0 ? g.n_items : std::min(g.n_items, b->policy * b->n_cu);
+}
+
+static int batch_capture(zj_batch* b) {
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    ZJ_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int mode = MODE_STAGE1; mode <= MODE_STAGE3 && e == hipSuccess; ++mode) {
+        e = hipEventRecord(b->fork, b->stream);
+        for (auto& g : b->groups) {
+            if (e == hipSuccess) e = hipStreamWaitEvent(g.branch, b->fork, 0);
+            if (e == hipSuccess) {
+                if (b->prec == ZJ_PREC_FP64)
+                    e = launch_batched_stage<double>(g.P, g.affine, mode, (const BatchInst<double>*)g.insts[mode],
+                                                     g.items, g.n_items, batch_grid(b, g), g.branch);
+                else
+                    e = launch_batched_stage<float>(g.P, g.affine, mode, (const BatchInst<float>*)g.insts[mode],
+                                                    g.items, g.n_items, batch_grid(b, g), g.branch);
+            }
+            if (e == hipSuccess) e = hipEventRecord(g.join, g.branch);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, g.join, 0);
+        }
+    }
+    hipGraph_t gr = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (gr) hipGraphDestroy(gr);
+        ZJ_BHIP(b, e != hipSuccess ? e : e2);
+    }
+    b->graph = gr;
+    ZJ_BHIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    return ZJ_OK;
+}
+
+extern "C" int zj_batch_create(zj_batch** out, const zj_handle* members, int n) {
+    if (!out || !members || n < 1) {
+        g_batch_error = "bad arguments";
+        return ZJ_ERR_ARG;
+    }
+    *out = nullptr;
+    zj_batch* b = new (std::nothrow) zj_batch();
+    if (!b) return ZJ_ERR_NOMEM;
+    int rc = batch_build(b, members, n);
+    if (rc) {
+        g_batch_error = b->err;
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return ZJ_OK;
+}
+
+extern "C" int zj_batch_destroy(zj_batch* b) {
+    if (!b) return ZJ_ERR_ARG;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    batch_release(b);
+    delete b;
+    return ZJ_OK;
+}
+
+extern "C" const char* zj_batch_last_error(zj_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+extern "C" int zj_batch_set_grid(zj_batch* b, int policy) {
+    if (!b || policy < 0 || policy > 64) return ZJ_ERR_ARG;
+    if (policy != b->policy) {
+        b->policy = policy;
+        if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
+    if (!b || !dts) return ZJ_ERR_ARG;
+    for (zj_solver* h : b->m) {
+        if (!h->has_state || h->stage_next != 1) {
+            b->err = "every member needs a state and no partial step";
+            return ZJ_ERR_STATE;
+        }
+    }
+    ZJ_BHIP(b, hipSetDevice(b->device));
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    if (!b->gexec) {
+        int rc = batch_capture(b);
+        if (rc) return rc;
+    }
+    const int n = (int)b->m.size();
+    bool same = true;
+    for (int i = 0; i < n && same; ++i) same = dts[i] == b->dt_on_device[i];
+    if (!same) {
+        ZJ_BHIP(b, launch_set_many(b->d_dt, dts, n, s));
+        b->dt_on_device.assign(dts, dts + n);
+    }
+    ZJ_BHIP(b, hipGraphLaunch(b->gexec, s));
+    for (zj_solver* h : b->m) h->l2_blocks = h->nblk;
+    return ZJ_OK;
+}
+
+extern "C" int zj_batch_synchronize(zj_batch* b) {
+    if (!b) return ZJ_ERR_ARG;
+    ZJ_BHIP(b, hipSetDevice(b->device));
+    ZJ_BHIP(b, hipStreamSynchronize(b->stream));
+    return ZJ_OK;
+}
+
+// ------------------------------------------------------------- autotuning
+// Times the residual launch (which leaves the state alone) for each tile and
+// XCD order and keeps the fastest.
+extern "C" int zj_autotune(zj_handle h, int reps, int* epb_out, int* xcd_out, double* ms_out) {
+    if (!h || reps < 1) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "autotuning needs a state");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "autotune in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    int nxcc = 1;
+    if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || nxcc < 1) nxcc = 1;
+    const int M = max_epb(h->P);
+    const int epbs[2] = {M, M / 2 >= 1 ? M / 2 : M};
+    const int xcds[2] = {nxcc, 1};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ZJ_HIP(h, hipEventCreate(&e0));
+    ZJ_HIP(h, hipEventCreate(&e1));
+    double best = 1e300;
+    int best_epb = h->epb, best_x = h->xcd_groups;
+    int rc = ZJ_OK;
+    for (int a = 0; a < 2 && rc == ZJ_OK; ++a)
+        for (int x = 0; x < 2 && rc == ZJ_OK; ++x) {
+            if ((rc = zj_set_tile(h, epbs[a])) || (rc = zj_set_xcd_groups(h, xcds[x]))) break;
+            hipError_t e = enqueue_stage(h, MODE_RESIDUAL, h->stream);  // warm-up
+            if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = enqueue_stage(h, MODE_RESIDUAL, h->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                h->err = std::string("autotune: ") + hipGetErrorString(e);
+                rc = ZJ_ERR_HIP;
+                break;
+            }
+            if (ms / reps < best) { best = ms / reps; best_epb = epbs[a]; best_x = xcds[x]; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = zj_set_tile(h, best_epb)) || (rc = zj_set_xcd_groups(h, best_x))) return rc;
+    if (epb_out) *epb_out = best_epb;
+    if (xcd_out) *xcd_out = best_x;
+    if (ms_out) *ms_out = best;
+    return ZJ_OK;
+}

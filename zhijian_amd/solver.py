@@ -15,7 +15,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import ZJError, check, lib
 
 PRECISIONS = {"fp64": _lib.ZJ_PREC_FP64, "fp32bulk": _lib.ZJ_PREC_FP32BULK}
 BC_FARFIELD = 4
@@ -225,12 +225,22 @@ class Solver:
         self._chk(lib().zj_adjoint_get(self._h, out))
         return out
 
-    def rk3_step_vjp(self, dt, stream=None):
+    def rk3_step_vjp(self, dt, stream=None, want_dt=True):
         """Maps the held cotangent of U^{n+1} to that of U^n (the state set on
-        the handle); returns dJ/d(dt)."""
+        the handle); returns dJ/d(dt) (synchronising), or None with
+        want_dt=False (asynchronous)."""
+        if not want_dt:
+            self._chk(lib().zj_rk3_step_vjp(self._h, float(dt), None, stream))
+            return None
         dtb = C.c_double()
         self._chk(lib().zj_rk3_step_vjp(self._h, float(dt), C.byref(dtb), stream))
         return dtb.value
+
+    def autotune(self, reps=20):
+        """Picks the fastest tile and XCD order; returns (epb, xcd_groups, ms)."""
+        e, x, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(lib().zj_autotune(self._h, int(reps), C.byref(e), C.byref(x), C.byref(ms)))
+        return e.value, x.value, ms.value
 
     def set_xcd_groups(self, groups):
         self._chk(lib().zj_set_xcd_groups(self._h, int(groups)))
@@ -247,3 +257,44 @@ class Solver:
         self._chk(lib().zj_get_info(self._h, C.byref(n), C.byref(e), C.byref(s)))
         return dict(n_dof=n.value, elems_per_block=e.value, n_sp=s.value,
                     affine=bool(self.set_geometry_mode(-1)))
+
+
+class Batch:
+    """Many Solvers stepped together (zj_batch_*).  Keeps its members alive."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = C.c_void_p()
+        rc = lib().zj_batch_create(C.byref(h), arr, len(self.members))
+        if rc != 0:
+            msg = lib().zj_batch_last_error(None)
+            raise ZJError(rc, msg.decode() if msg else "")
+        self._h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = lib().zj_batch_last_error(self._h)
+            raise ZJError(rc, msg.decode() if msg else "")
+
+    def rk3_step(self, dts, stream=None):
+        d = np.ascontiguousarray(dts, np.float64)
+        assert d.size == len(self.members)
+        self._chk(lib().zj_batch_rk3_step(self._h, d, stream))
+
+    def synchronize(self):
+        self._chk(lib().zj_batch_synchronize(self._h))
+
+    def set_grid(self, policy):
+        self._chk(lib().zj_batch_set_grid(self._h, int(policy)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().zj_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
