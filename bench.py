@@ -88,6 +88,70 @@ def time_adjoint(n, p, precision, steps, warmup, stream, hiprt):
                 note="adjoint step = traces of U^n + stages 1-2 recomputed + 3 VJP launches")
 
 
+def batch_members(n_inst, seed, precision):
+    """Seeded batch: N ~ U{16..128} elements per side, P ~ U{1..5},
+    Mach ~ U[0.1, 0.8], alternating Taylor-Green / isentropic vortex."""
+    from zhijian_amd import cases, solver
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_inst):
+        n, p, mach = int(rng.integers(16, 129)), int(rng.integers(1, 6)), float(rng.uniform(0.1, 0.8))
+        if k % 2 == 0:
+            mesh = cases.tgv_mesh(n)
+            t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+            U = cases.tgv_ic(t.pos, mach=mach).reshape(-1)
+        else:
+            mesh = cases.vortex_mesh(n)
+            t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+            U = cases.vortex_ic(t.pos, u_inf=mach * math.sqrt(1.4)).reshape(-1)
+        s = solver.Solver(t, precision=precision, cfl=0.5)
+        s.set_state(U)
+        out.append((s, U, dict(n=n, p=p, mach=mach, kind="tgv" if k % 2 == 0 else "vortex")))
+    return out
+
+
+def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt):
+    """One batched SSP-RK3 step of all members vs each member's own graph step."""
+    from zhijian_amd import solver
+    mem = batch_members(n_inst, seed, precision)
+    sols = [m[0] for m in mem]
+    dts = np.array([s.dt_cfl(0.5) for s in sols])
+    dof = sum(s.n for s in sols)
+    batch = solver.Batch(sols)
+    res = {}
+    for policy in (0, 2, 4):
+        batch.set_grid(policy)
+        for _ in range(warmup):
+            batch.rk3_step(dts, stream.ptr)
+        stream.synchronize()
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        for _ in range(steps):
+            batch.rk3_step(dts, stream.ptr)
+        e1.record(stream)
+        e1.synchronize()
+        res[policy] = e0.elapsed_ms(e1) / steps
+    best = min(res, key=res.get)
+    for _ in range(warmup):
+        for s, dt in zip(sols, dts):
+            s.rk3_step(dt, stream.ptr)
+    stream.synchronize()
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        for s, dt in zip(sols, dts):
+            s.rk3_step(dt, stream.ptr)
+    e1.record(stream)
+    e1.synchronize()
+    seq = e0.elapsed_ms(e1) / steps
+    batch.close()
+    return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], "
+                         f"alternating TGV/vortex", precision=precision, dof=dof,
+                ms_per_step_by_grid_policy={str(k): v for k, v in res.items()}, grid_policy=best,
+                batched_ms_per_step=res[best], sequential_ms_per_step=seq, speedup=seq / res[best],
+                dof_updates_per_s=3.0 * dof / (res[best] * 1e-3))
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -159,6 +223,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="members of the batched-step measurement (0: skip)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint/forward cost ratio (TGV 128^2 P3)")
     ap.add_argument("--only", choices=("fp64", "fp32bulk"), default=None,
@@ -232,6 +298,11 @@ def main():
                    rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
         log(f"fp32bulk vs fp64 after {args.err_steps} steps: {err}")
 
+    batched = None
+    if args.batch > 0:
+        batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt)
+        log(f"batched: {batched}")
+
     adj = {}
     if args.adjoint:
         for prec in ("fp64", "fp32bulk"):
@@ -271,6 +342,7 @@ def main():
         "fp32bulk_error_vs_fp64": err,
         "adjoint_over_forward": adj.get("fp64", {}).get("adjoint_over_forward"),
         "adjoint": adj or None,
+        "batched": batched,
         "copy_kernel_gbps": copy_gbps,
     }
     print(json.dumps(out), flush=True)

@@ -76,6 +76,16 @@ def lib() -> C.CDLL:
         L.oz_dt_cfl.restype = C.c_double
         L.oz_residual_l2.argtypes = [C.c_void_p]
         L.oz_residual_l2.restype = C.c_double
+        L.oz_flux_normal.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp]
+        L.oz_max_wave_speed.argtypes = [_dp, C.c_double]
+        L.oz_max_wave_speed.restype = C.c_double
+        L.oz_riemann.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
+        L.oz_vcjh_deriv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.oz_tb_create.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
+        L.oz_tb_create.restype = C.c_void_p
+        L.oz_tb_destroy.argtypes = [C.c_void_p]
+        L.oz_tb_residual.argtypes = [C.c_void_p, _dp, _dp]
+        L.oz_tb_advance.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int]
         for name in ("oz_buf_fx", "oz_buf_fy", "oz_buf_ufp", "oz_buf_fface", "oz_buf_fdiff",
                      "oz_buf_dudt", "oz_buf_ufp_mut", "oz_buf_fdiff_mut"):
             getattr(L, name).argtypes = [C.c_void_p]
@@ -274,6 +284,10 @@ class OracleSolver:
             raise MemoryError("oz_create failed")
         self._h = h
         self.n = self.E * self.S * 4
+        self.desc = (d, keep)
+
+    def textbook(self, edge_half_len, riemann="roe", corr="dg") -> "TextbookSolver":
+        return TextbookSolver(self, self.desc, edge_half_len, riemann, corr)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -355,3 +369,67 @@ def positivity(U):
     U = np.ascontiguousarray(U, np.float64).reshape(-1).copy()
     lib().oz_positivity(U, U.size // 4)
     return U
+
+
+# ------------------------------------------------------------ textbook scheme
+RIEMANN = {"rusanov": 0, "roe": 1, "hllc": 2}
+CORR = {"dg": 0, "sd": 1, "hu": 2, "ga": 3}
+
+
+def flux_normal(U, nx, ny, gamma=1.4):
+    out = np.zeros(4)
+    lib().oz_flux_normal(np.ascontiguousarray(U, np.float64), nx, ny, gamma, out)
+    return out
+
+
+def riemann(kind, UL, UR, nx, ny, gamma=1.4):
+    out = np.zeros(4)
+    lib().oz_riemann(RIEMANN[kind], np.ascontiguousarray(UL, np.float64),
+                     np.ascontiguousarray(UR, np.float64), nx, ny, gamma, out)
+    return out
+
+
+def vcjh_deriv(p, kind, x):
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    gl, gr = np.zeros(x.size), np.zeros(x.size)
+    if lib().oz_vcjh_deriv(p, CORR[kind], x, x.size, gl, gr) != 0:
+        raise ValueError("bad order or kind")
+    return gl, gr
+
+
+def edge_half_lengths(elem_nodes, xy):
+    """[E][4]: half the length of edge k = (node k, node k+1) of each element."""
+    en = np.asarray(elem_nodes)[:, :4]
+    a = xy[en]
+    b = xy[np.roll(en, -1, axis=1)]
+    return 0.5 * np.linalg.norm(b - a, axis=2)
+
+
+class TextbookSolver:
+    """Textbook FR residual / SSP-RK3 (zj_textbook.c) on the CPU."""
+
+    def __init__(self, base: OracleSolver, keep_desc, edge_half_len, riemann="roe", corr="dg"):
+        self.n = base.n
+        self._ehl = np.ascontiguousarray(edge_half_len, np.float64).reshape(-1)
+        self._desc, self._keep = keep_desc
+        h = lib().oz_tb_create(C.byref(self._desc), self._ehl, RIEMANN[riemann], CORR[corr])
+        if not h:
+            raise ValueError("oz_tb_create failed")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().oz_tb_destroy(h)
+            self._h = None
+
+    def residual(self, U):
+        U = np.ascontiguousarray(U, np.float64).reshape(-1)
+        out = np.zeros(self.n)
+        lib().oz_tb_residual(self._h, U, out)
+        return out
+
+    def advance(self, U, dt, positivity=True):
+        U = np.ascontiguousarray(U, np.float64).reshape(-1).copy()
+        lib().oz_tb_advance(self._h, U, dt, int(positivity))
+        return U

@@ -187,6 +187,27 @@ void oz_rusanov_inline(const double* UL, const double* UR, double nx, double ny,
 void oz_ghost_state(const double* Uint, double nx, double ny, int bc_type,
                     const double* bc_data, int bc_idx, double gamma, double* ghost);
 
+/* ---- textbook scheme (zj_textbook.c): what the reference's FluxType and
+ * RiemannSolver settings name but its executed path does not run. */
+enum { OZ_RIEMANN_RUSANOV = 0, OZ_RIEMANN_ROE = 1, OZ_RIEMANN_HLLC = 2 };
+enum { OZ_CORR_DG = 0, OZ_CORR_SD = 1, OZ_CORR_HU = 2, OZ_CORR_GA = 3 };
+/* IdealGas::fluxNormal and maxWaveSpeed (euler.hpp:133-147, 166-171). */
+void oz_flux_normal(const double* U, double nx, double ny, double gamma, double* Fn);
+double oz_max_wave_speed(const double* U, double gamma);
+/* RiemannSolvers::rusanov / roe / hllc (euler.hpp:184-375).  Pinned: the
+ * compiled reference's golden vectors (tests/golden/physics.npz). */
+void oz_riemann(int kind, const double* UL, const double* UR, double nx, double ny, double gamma,
+                double* F);
+/* VCJH correction-function derivatives gL'(x), gR'(x) at n points.  Pinned:
+ * DG = the reference's Radau tables; HU = Huynh's g2; SD/GA by their zeros. */
+int oz_vcjh_deriv(int p, int kind, const double* x, int n, double* gLd, double* gRd);
+typedef struct oz_tb oz_tb;
+/* edge_half_len: [E][4] half lengths of the straight element edges. */
+oz_tb* oz_tb_create(const oz_desc* d, const double* edge_half_len, int riemann, int corr);
+void oz_tb_destroy(oz_tb* t);
+void oz_tb_residual(oz_tb* t, const double* U, double* dUdt);
+void oz_tb_advance(oz_tb* t, double* U, double dt, int positivity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,364 @@
+/* zj_textbook.c -- CPU oracle of the textbook flux-reconstruction residual.
+ *
+ * TEST INFRASTRUCTURE ONLY (see zj_oracle.h).
+ *
+ * The reference configures DG/SD/HU/GA correction functions and
+ * Rusanov/Roe/HLLC Riemann solvers (types.hpp:59-70) but executes neither:
+ * its correction tables are Radau for every flux type (basis.cpp:314-379)
+ * and its residual kernel calls an inline clamped Rusanov
+ * (kernels.cu:601-668).  This file restates the scheme those settings name:
+ *
+ *   dU/dt = -(1/J) [ d/dxi Ft^c + d/deta Gt^c ]                 Huynh (2007)
+ *   Ft = J (xi_x F + xi_y G),  Gt = J (eta_x F + eta_y G)
+ *   d/dxi Ft^c (xi_i) = D Ft + (Ft^I_L - Ft^D(-1)) gL'(xi_i) + (Ft^I_R - Ft^D(+1)) gR'(xi_i)
+ *
+ * with Ft^D(+-1) the interpolated transformed flux, Ft^I = +-|dx/ds| (F^I . n)
+ * the transformed common flux (sign + on the xi = +1 and eta = +1 edges),
+ * F^I = RiemannSolvers::{rusanov,roe,hllc} (euler.hpp:184-375) evaluated once
+ * per face point with the geometrically matching states (left slot f, right
+ * slot p-f), and gL, gR the VCJH correction functions of Vincent, Castonguay
+ * and Jameson (2011):
+ *
+ *   gL = (-1)^p/2 [P_p - (eta_p P_{p-1} + P_{p+1})/(1+eta_p)],  gR(x) = gL(-x)
+ *   eta_p: DG 0, SD p/(p+1), HU (p+1)/p;  GA = Huynh's g_Ga = (-1)^p/2 (1-x) P_p,
+ *
+ * which by the Bonnet recursion is the SD member (Jameson 2010).
+ *
+ * Pins (tests/test_oracle_textbook.py): the three Riemann solvers against the
+ * compiled reference's golden vectors; DG against the reference's Radau
+ * tables; HU against Huynh's closed form g2; SD/GA against the
+ * characterisation (zeros at the Gauss points and +1, value 1 at -1);
+ * freestream preservation on jittered meshes; discrete conservation of
+ * sum w J U; order of accuracy on the isentropic vortex; linear stability of
+ * the Jacobian spectrum.
+ */
+#include "zj_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NV 4
+
+/* ---------------------------------------------------------------- physics */
+static double pressure(const double* U, double g) {
+    const double rho = U[0], u = U[1] / rho, v = U[2] / rho, E = U[3] / rho;
+    return (g - 1.0) * rho * (E - 0.5 * (u * u + v * v));
+}
+
+/* IdealGas::fluxNormal (euler.hpp:133-147) */
+void oz_flux_normal(const double* U, double nx, double ny, double g, double* Fn) {
+    const double rho = fmax(U[0], 1e-10);
+    const double u = U[1] / rho, v = U[2] / rho;
+    const double p = fmax(pressure(U, g), 1e-10);
+    const double H = U[3] + pressure(U, g);
+    const double vn = u * nx + v * ny;
+    Fn[0] = rho * vn;
+    Fn[1] = rho * u * vn + p * nx;
+    Fn[2] = rho * v * vn + p * ny;
+    Fn[3] = H * vn;
+}
+
+/* IdealGas::soundSpeed / maxWaveSpeed (euler.hpp:34-41, 166-171) */
+static double sound_speed(const double* U, double g) {
+    const double p = fmax(pressure(U, g), 1e-10), rho = fmax(U[0], 1e-10);
+    return sqrt(g * p / rho);
+}
+double oz_max_wave_speed(const double* U, double g) {
+    const double u = U[1] / U[0], v = U[2] / U[0];
+    return sqrt(u * u + v * v) + sound_speed(U, g);
+}
+
+/* RiemannSolvers::rusanov (euler.hpp:184-200) */
+static void riemann_rusanov(const double* UL, const double* UR, double nx, double ny, double g, double* F) {
+    double FL[NV], FR[NV];
+    oz_flux_normal(UL, nx, ny, g, FL);
+    oz_flux_normal(UR, nx, ny, g, FR);
+    const double smax = fmax(oz_max_wave_speed(UL, g), oz_max_wave_speed(UR, g));
+    for (int i = 0; i < NV; ++i) F[i] = 0.5 * (FL[i] + FR[i]) - 0.5 * smax * (UR[i] - UL[i]);
+}
+
+/* RiemannSolvers::roe with Harten's fix (euler.hpp:204-294) */
+static void riemann_roe(const double* UL, const double* UR, double nx, double ny, double g, double* F) {
+    const double rhoL = fmax(UL[0], 1e-10), rhoR = fmax(UR[0], 1e-10);
+    const double uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
+    const double HL = (UL[3] + pressure(UL, g)) / rhoL, HR = (UR[3] + pressure(UR, g)) / rhoR;
+    const double sL = sqrt(rhoL), sR = sqrt(rhoR);
+    const double den = fmax(sL + sR, 1e-10);
+    const double rr = sL * sR;
+    const double ur = (sL * uL + sR * uR) / den, vr = (sL * vL + sR * vR) / den;
+    const double Hr = (sL * HL + sR * HR) / den;
+    const double vnr = ur * nx + vr * ny, q2 = ur * ur + vr * vr;
+    const double c = sqrt(fmax((g - 1.0) * (Hr - 0.5 * q2), 1e-10));
+    double l1 = fabs(vnr - c), l2 = fabs(vnr), l3 = fabs(vnr), l4 = fabs(vnr + c);
+    const double eps = 0.1 * c;
+    if (l1 < eps) l1 = (l1 * l1 + eps * eps) / (2.0 * eps);
+    if (l4 < eps) l4 = (l4 * l4 + eps * eps) / (2.0 * eps);
+    const double drho = rhoR - rhoL, du = uR - uL, dv = vR - vL;
+    const double dp = pressure(UR, g) - pressure(UL, g), dvn = du * nx + dv * ny;
+    const double c2 = fmax(c * c, 1e-10);
+    const double a1 = (dp - rr * c * dvn) / (2.0 * c2);
+    const double a2 = drho - dp / c2;
+    const double a3 = rr * (dv * nx - du * ny);
+    const double a4 = (dp + rr * c * dvn) / (2.0 * c2);
+    double FL[NV], FR[NV], dF[NV];
+    oz_flux_normal(UL, nx, ny, g, FL);
+    oz_flux_normal(UR, nx, ny, g, FR);
+    const double tx = -ny, ty = nx;
+    dF[0] = l1 * a1 * 1.0;
+    dF[1] = l1 * a1 * (ur - c * nx);
+    dF[2] = l1 * a1 * (vr - c * ny);
+    dF[3] = l1 * a1 * (Hr - c * vnr);
+    dF[0] += l2 * a2 * 1.0;
+    dF[1] += l2 * a2 * ur;
+    dF[2] += l2 * a2 * vr;
+    dF[3] += l2 * a2 * 0.5 * q2;
+    dF[0] += l3 * a3 * 0.0;
+    dF[1] += l3 * a3 * tx;
+    dF[2] += l3 * a3 * ty;
+    dF[3] += l3 * a3 * (ur * tx + vr * ty);
+    dF[0] += l4 * a4 * 1.0;
+    dF[1] += l4 * a4 * (ur + c * nx);
+    dF[2] += l4 * a4 * (vr + c * ny);
+    dF[3] += l4 * a4 * (Hr + c * vnr);
+    for (int i = 0; i < NV; ++i) F[i] = 0.5 * (FL[i] + FR[i]) - 0.5 * dF[i];
+}
+
+static double guard(double d) { return fabs(d) < 1e-10 ? (d >= 0 ? 1e-10 : -1e-10) : d; }
+
+/* RiemannSolvers::hllc (euler.hpp:298-375) */
+static void riemann_hllc(const double* UL, const double* UR, double nx, double ny, double g, double* F) {
+    const double rhoL = fmax(UL[0], 1e-10), rhoR = fmax(UR[0], 1e-10);
+    const double uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
+    const double pL = fmax(pressure(UL, g), 1e-10), pR = fmax(pressure(UR, g), 1e-10);
+    const double cL = sound_speed(UL, g), cR = sound_speed(UR, g);
+    const double vnL = uL * nx + vL * ny, vnR = uR * nx + vR * ny;
+    double ps = 0.5 * (pL + pR) - 0.5 * (vnR - vnL) * 0.5 * (rhoL + rhoR) * 0.5 * (cL + cR);
+    ps = fmax(ps, 1e-10);
+    const double qL = (ps <= pL) ? 1.0 : sqrt(1.0 + (g + 1.0) / (2.0 * g) * (ps / pL - 1.0));
+    const double qR = (ps <= pR) ? 1.0 : sqrt(1.0 + (g + 1.0) / (2.0 * g) * (ps / pR - 1.0));
+    const double SL = vnL - cL * qL, SR = vnR + cR * qR;
+    const double dSM = guard(rhoL * (SL - vnL) - rhoR * (SR - vnR));
+    const double SM = (pR - pL + rhoL * vnL * (SL - vnL) - rhoR * vnR * (SR - vnR)) / dSM;
+    if (SL >= 0.0) {
+        oz_flux_normal(UL, nx, ny, g, F);
+    } else if (SR <= 0.0) {
+        oz_flux_normal(UR, nx, ny, g, F);
+    } else {
+        const int left = SM >= 0.0;
+        const double* U = left ? UL : UR;
+        const double rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR;
+        const double vn = left ? vnL : vnR, S = left ? SL : SR, p = left ? pL : pR;
+        const double dK = guard(S - SM), dvn = guard(S - vn);
+        const double fac = rho * dvn / dK;
+        double Us[NV], Fk[NV];
+        Us[0] = fac;
+        Us[1] = fac * (u + (SM - vn) * nx);
+        Us[2] = fac * (v + (SM - vn) * ny);
+        Us[3] = fac * (U[3] / rho + (SM - vn) * (SM + p / (rho * dvn)));
+        oz_flux_normal(U, nx, ny, g, Fk);
+        for (int i = 0; i < NV; ++i) F[i] = Fk[i] + S * (Us[i] - U[i]);
+    }
+}
+
+void oz_riemann(int kind, const double* UL, const double* UR, double nx, double ny, double g, double* F) {
+    if (kind == OZ_RIEMANN_ROE) riemann_roe(UL, UR, nx, ny, g, F);
+    else if (kind == OZ_RIEMANN_HLLC) riemann_hllc(UL, UR, nx, ny, g, F);
+    else riemann_rusanov(UL, UR, nx, ny, g, F);
+}
+
+/* ------------------------------------------------------- VCJH correction */
+static void legendre3(int n, double x, double* P, double* dP) {
+    /* P_k and P_k' for k = 0..n by the three-term recursion */
+    P[0] = 1.0; dP[0] = 0.0;
+    if (n >= 1) { P[1] = x; dP[1] = 1.0; }
+    for (int k = 2; k <= n; ++k) {
+        P[k] = ((2 * k - 1) * x * P[k - 1] - (k - 1) * P[k - 2]) / k;
+        dP[k] = dP[k - 2] + (2 * k - 1) * P[k - 1];
+    }
+}
+
+int oz_vcjh_deriv(int p, int kind, const double* x, int n, double* gLd, double* gRd) {
+    if (p < 1 || p > 9 || kind < 0 || kind > 3) return -1;
+    double eta = 0.0;
+    if (kind == OZ_CORR_SD || kind == OZ_CORR_GA) eta = (double)p / (p + 1);
+    else if (kind == OZ_CORR_HU) eta = (double)(p + 1) / p;
+    const double sgn = (p % 2) ? -1.0 : 1.0;
+    for (int k = 0; k < n; ++k) {
+        double P[12], dP[12], Q[12], dQ[12];
+        legendre3(p + 1, x[k], P, dP);
+        legendre3(p + 1, -x[k], Q, dQ);
+        /* d/dx gL(x) and d/dx gR(x) = -gL'(-x) */
+        const double gl = sgn * 0.5 * (dP[p] - (eta * dP[p - 1] + dP[p + 1]) / (1.0 + eta));
+        const double glm = sgn * 0.5 * (dQ[p] - (eta * dQ[p - 1] + dQ[p + 1]) / (1.0 + eta));
+        gLd[k] = gl;
+        gRd[k] = -glm;
+    }
+    return 0;
+}
+
+/* -------------------------------------------------------------- residual */
+struct oz_tb {
+    int p, n1, S, E, F;
+    double g;
+    int riemann;
+    double D[36], Lm[6], Lp[6], gLd[6], gRd[6];
+    double *J, *Jinv, *normals, *ehl, *bcd;
+    int *le, *ll, *re, *rl, *bct, *bcmap;
+    int nbcd;
+    double *Ft, *Gt, *Ufp, *Fout, *U0, *U1, *R;
+};
+
+static double* cp_d(const double* s, size_t n) {
+    double* d = (double*)calloc(n ? n : 1, sizeof(double));
+    if (d && s) memcpy(d, s, n * sizeof(double));
+    return d;
+}
+static int* cp_i(const int* s, size_t n, int fill) {
+    int* d = (int*)malloc((n ? n : 1) * sizeof(int));
+    if (!d) return d;
+    for (size_t i = 0; i < n; ++i) d[i] = s ? s[i] : fill;
+    return d;
+}
+
+oz_tb* oz_tb_create(const oz_desc* d, const double* edge_half_len, int riemann, int corr) {
+    if (d->p < 1 || d->p > 5 || !edge_half_len || riemann < 0 || riemann > 2) return NULL;
+    oz_tb* t = (oz_tb*)calloc(1, sizeof(oz_tb));
+    if (!t) return NULL;
+    const int n1 = d->p + 1, S = n1 * n1;
+    t->p = d->p; t->n1 = n1; t->S = S; t->E = d->n_elem; t->F = d->n_faces; t->g = d->gamma;
+    t->riemann = riemann;
+    for (int i = 0; i < n1; ++i)
+        for (int k = 0; k < n1; ++k) t->D[i * n1 + k] = d->diff_xi[(size_t)i * S + k];
+    for (int j = 0; j < n1; ++j) t->Lm[j] = d->interp[(size_t)0 * S + j * n1];
+    for (int i = 0; i < n1; ++i) t->Lp[i] = d->interp[(size_t)(1 * n1) * S + i];
+    double xs[6], ws[6];
+    oz_gauss(n1, xs, ws);
+    if (oz_vcjh_deriv(d->p, corr, xs, n1, t->gLd, t->gRd)) { free(t); return NULL; }
+    const size_t E = t->E, F = t->F;
+    t->J = cp_d(d->J, E * S); t->Jinv = cp_d(d->Jinv, E * S * 4);
+    t->normals = cp_d(d->normals, F * 2); t->ehl = cp_d(edge_half_len, E * 4);
+    t->le = cp_i(d->face_le, F, 0); t->ll = cp_i(d->face_ll, F, 0);
+    t->re = cp_i(d->face_re, F, -1); t->rl = cp_i(d->face_rl, F, -1);
+    t->bct = cp_i(d->bc_type, F, OZ_BC_INTERIOR); t->bcmap = cp_i(d->bc_face_map, F, -1);
+    t->nbcd = d->n_bc_data;
+    t->bcd = cp_d(d->bc_data, (size_t)(d->n_bc_data > 0 ? d->n_bc_data : 1) * 8);
+    const size_t sol = E * S * NV, fp = E * 4 * n1 * NV;
+    t->Ft = cp_d(NULL, sol); t->Gt = cp_d(NULL, sol); t->U0 = cp_d(NULL, sol);
+    t->U1 = cp_d(NULL, sol); t->R = cp_d(NULL, sol);
+    t->Ufp = cp_d(NULL, fp); t->Fout = cp_d(NULL, fp);
+    return t;
+}
+
+void oz_tb_destroy(oz_tb* t) {
+    if (!t) return;
+    void* ps[] = {t->J, t->Jinv, t->normals, t->ehl, t->bcd, t->le, t->ll, t->re, t->rl, t->bct,
+                  t->bcmap, t->Ft, t->Gt, t->Ufp, t->Fout, t->U0, t->U1, t->R};
+    for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
+    free(t);
+}
+
+/* value at flux point f of edge k from the element's solution points u[S]
+ * (edges 0, 1 forward, 2, 3 reversed: basis.cpp:540-552) */
+static double edge_value(const oz_tb* t, const double* u, int stride, int k, int f) {
+    const int n1 = t->n1, P = t->p;
+    double s = 0.0;
+    for (int m = 0; m < n1; ++m) {
+        int sp;
+        double w;
+        if (k == 0) { sp = m * n1 + f; w = t->Lm[m]; }
+        else if (k == 1) { sp = f * n1 + m; w = t->Lp[m]; }
+        else if (k == 2) { sp = m * n1 + (P - f); w = t->Lp[m]; }
+        else { sp = (P - f) * n1 + m; w = t->Lm[m]; }
+        s += w * u[(size_t)sp * stride];
+    }
+    return s;
+}
+
+void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
+    const int n1 = t->n1, S = t->S, E = t->E, P = t->p;
+    const double g = t->g;
+    /* transformed fluxes and traces */
+    for (int e = 0; e < E; ++e) {
+        for (int sp = 0; sp < S; ++sp) {
+            const size_t gi = (size_t)e * S + sp;
+            double Fx[NV], Fy[NV];
+            oz_flux_xy(U + gi * NV, g, Fx, Fy);
+            const double* Ji = t->Jinv + gi * 4;
+            const double J = t->J[gi];
+            for (int v = 0; v < NV; ++v) {
+                t->Ft[gi * NV + v] = J * (Ji[0] * Fx[v] + Ji[1] * Fy[v]);
+                t->Gt[gi * NV + v] = J * (Ji[2] * Fx[v] + Ji[3] * Fy[v]);
+            }
+        }
+        for (int k = 0; k < 4; ++k)
+            for (int f = 0; f < n1; ++f)
+                for (int v = 0; v < NV; ++v)
+                    t->Ufp[(((size_t)e * 4 + k) * n1 + f) * NV + v] =
+                        edge_value(t, U + (size_t)e * S * NV + v, NV, k, f);
+    }
+    /* common fluxes: one evaluation per face point, outward on each side */
+    for (int fc = 0; fc < t->F; ++fc) {
+        const int le = t->le[fc], ll = t->ll[fc], re = t->re[fc], rl = t->rl[fc];
+        const double nx = t->normals[2 * fc], ny = t->normals[2 * fc + 1];
+        for (int f = 0; f < n1; ++f) {
+            const double* UL = t->Ufp + (((size_t)le * 4 + ll) * n1 + f) * NV;
+            double Fc[NV], G[NV];
+            if (re >= 0) {
+                const double* UR = t->Ufp + (((size_t)re * 4 + rl) * n1 + (P - f)) * NV;
+                oz_riemann(t->riemann, UL, UR, nx, ny, g, Fc);
+                double* Fr = t->Fout + (((size_t)re * 4 + rl) * n1 + (P - f)) * NV;
+                for (int v = 0; v < NV; ++v) Fr[v] = -Fc[v];
+            } else {
+                int bct = t->bct[fc], bci = t->bcmap[fc];
+                if (bci < 0) { bct = OZ_BC_SLIPWALL; bci = 0; }  /* kernels.cu:589-593 */
+                oz_ghost_state(UL, nx, ny, bct, t->bcd, bci, g, G);
+                oz_riemann(t->riemann, UL, G, nx, ny, g, Fc);
+            }
+            double* Fl = t->Fout + (((size_t)le * 4 + ll) * n1 + f) * NV;
+            for (int v = 0; v < NV; ++v) Fl[v] = Fc[v];
+        }
+    }
+    /* divergence and correction */
+    for (int e = 0; e < E; ++e) {
+        double delta[4][6][NV];
+        const double sgn[4] = {-1.0, 1.0, 1.0, -1.0};
+        for (int k = 0; k < 4; ++k)
+            for (int f = 0; f < n1; ++f)
+                for (int v = 0; v < NV; ++v) {
+                    const double* base = (k == 1 || k == 3) ? t->Ft : t->Gt;
+                    const double fd = edge_value(t, base + (size_t)e * S * NV + v, NV, k, f);
+                    const double fi = sgn[k] * t->ehl[(size_t)e * 4 + k] *
+                                      t->Fout[(((size_t)e * 4 + k) * n1 + f) * NV + v];
+                    delta[k][f][v] = fi - fd;
+                }
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                const size_t gi = (size_t)e * S + j * n1 + i;
+                for (int v = 0; v < NV; ++v) {
+                    double div = 0.0;
+                    for (int m = 0; m < n1; ++m) {
+                        div += t->D[i * n1 + m] * t->Ft[((size_t)e * S + j * n1 + m) * NV + v];
+                        div += t->D[j * n1 + m] * t->Gt[((size_t)e * S + m * n1 + i) * NV + v];
+                    }
+                    div += t->gLd[i] * delta[3][P - j][v] + t->gRd[i] * delta[1][j][v] +
+                           t->gLd[j] * delta[0][i][v] + t->gRd[j] * delta[2][P - i][v];
+                    dUdt[gi * NV + v] = -div / t->J[gi];
+                }
+            }
+    }
+}
+
+void oz_tb_advance(oz_tb* t, double* U, double dt, int positivity) {
+    const int n = t->E * t->S * NV, ns = t->E * t->S;
+    memcpy(t->U0, U, sizeof(double) * n);
+    oz_tb_residual(t, U, t->R);
+    oz_rk_stage1(t->U1, t->U0, t->R, n, dt);
+    if (positivity) oz_positivity(t->U1, ns);
+    oz_tb_residual(t, t->U1, t->R);
+    oz_rk_stage2(t->U1, t->U0, t->U1, t->R, n, dt);
+    if (positivity) oz_positivity(t->U1, ns);
+    oz_tb_residual(t, t->U1, t->R);
+    oz_rk_stage3(U, t->U0, t->U1, t->R, n, dt);
+    if (positivity) oz_positivity(U, ns);
+}

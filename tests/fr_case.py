@@ -43,6 +43,9 @@ class OracleCase:
                               bc_face_map=self.bc_face_map, bc_data=self.bc_data,
                               elem_area=self.area)
 
+    def textbook(self, riemann="roe", corr="dg") -> O.TextbookSolver:
+        return self.solver().textbook(O.edge_half_lengths(self.elem_nodes, self.xy), riemann, corr)
+
 
 def freestream(gamma=1.4, p_inf=101325.0, rho_inf=1.225, mach=0.5, aoa_deg=0.0):
     c = math.sqrt(gamma * p_inf / rho_inf)
