@@ -36,7 +36,7 @@ enum class RiemannSolver : int { Rusanov = 0, Roe = 1, HLLC = 2 };
 
 // Arithmetic of the device path.
 enum class Precision : int { FP64 = 0, FP32Bulk = 1 };
-enum class Compat : int { Reference = 0 };
+enum class Compat : int { Reference = 0, Textbook = 1 };
 
 struct Vec2 {
     Real x, y;

@@ -56,6 +56,15 @@ extern "C" {
 #define ZJ_PREC_FP32BULK 1    /* FP32 interpolation/flux/correction, FP64 RK state */
 
 #define ZJ_COMPAT_REFERENCE 0 /* the reference's executed arithmetic (quirks kept) */
+#define ZJ_COMPAT_TEXTBOOK 1  /* the scheme its settings name: VCJH FR with metrics (Huynh 2007,
+                                 Vincent et al. 2011) and the configured Riemann solver */
+#define ZJ_RIEMANN_RUSANOV 0  /* RiemannSolver (types.hpp:66-70), textbook mode only */
+#define ZJ_RIEMANN_ROE 1
+#define ZJ_RIEMANN_HLLC 2
+#define ZJ_CORR_DG 0          /* FluxType (types.hpp:59-64), textbook mode only */
+#define ZJ_CORR_SD 1
+#define ZJ_CORR_HU 2
+#define ZJ_CORR_GA 3
 
 /* BCType values (include/common/types.hpp:47-56) */
 #define ZJ_BC_INTERIOR 0
@@ -93,6 +102,8 @@ typedef struct {
     const double* bc_data;       /* [n_bc_data*8] or NULL */
     int n_bc_data;
     const double* elem_area;     /* [E] or NULL (then the CFL dt is unavailable) */
+    const double* edge_half_len; /* [E][4] half lengths of the straight element edges
+                                    (edge k = node k -> node k+1); textbook mode only */
 } zj_mesh_desc;
 
 typedef struct {
@@ -102,6 +113,8 @@ typedef struct {
     int compat;                  /* ZJ_COMPAT_REFERENCE */
     int positivity;              /* 1: the reference's limiter after each stage */
     int device;                  /* HIP device ordinal */
+    int riemann;                 /* ZJ_RIEMANN_*, textbook mode */
+    int corr;                    /* ZJ_CORR_*, textbook mode */
 } zj_params;
 
 /* ---- solver --------------------------------------------------------- */

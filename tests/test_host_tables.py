@@ -122,7 +122,9 @@ def test_create_rejects_bad_input_before_touching_the_gpu():
     assert rc == 1 and "two faces" in msg
     # unsupported arithmetic mode, bad order, bad gamma
     d, keep, _ = _desc_for(t)
-    assert _create(d, compat=1)[0] == 4
+    assert _create(d, compat=7)[0] == 1
+    rc, msg = _create(d, compat=1)  # textbook mode without the edge lengths it needs
+    assert rc == 1 and "edge_half_len" in msg
     assert _create(d, gamma=1.0)[0] == 1
     d.poly_order = 7
     assert _create(d)[0] == 1

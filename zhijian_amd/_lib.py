@@ -16,6 +16,7 @@ ZJ_OK = 0
 ZJ_PREC_FP64 = 0
 ZJ_PREC_FP32BULK = 1
 ZJ_COMPAT_REFERENCE = 0
+ZJ_COMPAT_TEXTBOOK = 1
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -36,12 +37,13 @@ class MeshDesc(C.Structure):
                 ("face_left_local", C.c_void_p), ("face_right_elem", C.c_void_p),
                 ("face_right_local", C.c_void_p), ("bc_type", C.c_void_p),
                 ("bc_face_map", C.c_void_p), ("bc_data", C.c_void_p), ("n_bc_data", C.c_int),
-                ("elem_area", C.c_void_p)]
+                ("elem_area", C.c_void_p), ("edge_half_len", C.c_void_p)]
 
 
 class Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("precision", C.c_int),
-                ("compat", C.c_int), ("positivity", C.c_int), ("device", C.c_int)]
+                ("compat", C.c_int), ("positivity", C.c_int), ("device", C.c_int),
+                ("riemann", C.c_int), ("corr", C.c_int)]
 
 
 _lib = None

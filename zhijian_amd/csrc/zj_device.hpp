@@ -136,6 +136,121 @@ __device__ void ghost_state(const R* Ui, R nx, R ny, int bct, const double* bc, 
     }
 }
 
+// ------------------------------------------- textbook-mode Riemann solvers
+// IdealGas::fluxNormal with its floors (euler.hpp:133-147).
+template <typename R>
+__device__ __forceinline__ void flux_normal_floor(const R* U, R nx, R ny, R g, R* Fn) {
+    const R rho = fmaxR(U[0], R(1e-10));
+    const R u = U[1] / rho, v = U[2] / rho;
+    const R pr = (g - R(1)) * U[0] * (U[3] / U[0] - R(0.5) * ((U[1] / U[0]) * (U[1] / U[0]) +
+                                                                (U[2] / U[0]) * (U[2] / U[0])));
+    const R p = fmaxR(pr, R(1e-10));
+    const R vn = u * nx + v * ny;
+    Fn[0] = rho * vn;
+    Fn[1] = rho * u * vn + p * nx;
+    Fn[2] = rho * v * vn + p * ny;
+    Fn[3] = (U[3] + pr) * vn;
+}
+template <typename R>
+__device__ __forceinline__ R pressure_of(const R* U, R g) {
+    const R rho = U[0], u = U[1] / rho, v = U[2] / rho;
+    return (g - R(1)) * rho * (U[3] / rho - R(0.5) * (u * u + v * v));
+}
+template <typename R>
+__device__ __forceinline__ R sound_speed_of(const R* U, R g) {
+    return sqrtR(g * fmaxR(pressure_of(U, g), R(1e-10)) / fmaxR(U[0], R(1e-10)));
+}
+// RiemannSolvers::rusanov (euler.hpp:184-200)
+template <typename R>
+__device__ __forceinline__ void riemann_rusanov(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
+    R FL[NV], FR[NV];
+    flux_normal_floor(UL, nx, ny, g, FL);
+    flux_normal_floor(UR, nx, ny, g, FR);
+    const R uL = UL[1] / UL[0], vL = UL[2] / UL[0], uR = UR[1] / UR[0], vR = UR[2] / UR[0];
+    const R sL = sqrtR(uL * uL + vL * vL) + sound_speed_of(UL, g);
+    const R sR = sqrtR(uR * uR + vR * vR) + sound_speed_of(UR, g);
+    const R smax = fmaxR(sL, sR);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (FL[i] + FR[i]) - R(0.5) * smax * (UR[i] - UL[i]);
+}
+// RiemannSolvers::roe with Harten's entropy fix (euler.hpp:204-294)
+template <typename R>
+__device__ __forceinline__ void riemann_roe(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
+    const R rhoL = fmaxR(UL[0], R(1e-10)), rhoR = fmaxR(UR[0], R(1e-10));
+    const R uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
+    const R pL = pressure_of(UL, g), pR = pressure_of(UR, g);
+    const R HL = (UL[3] + pL) / rhoL, HR = (UR[3] + pR) / rhoR;
+    const R sL = sqrtR(rhoL), sR = sqrtR(rhoR);
+    const R den = fmaxR(sL + sR, R(1e-10));
+    const R rr = sL * sR;
+    const R ur = (sL * uL + sR * uR) / den, vr = (sL * vL + sR * vR) / den, Hr = (sL * HL + sR * HR) / den;
+    const R vnr = ur * nx + vr * ny, q2 = ur * ur + vr * vr;
+    const R c = sqrtR(fmaxR((g - R(1)) * (Hr - R(0.5) * q2), R(1e-10)));
+    R l1 = fabs(vnr - c), l4 = fabs(vnr + c);
+    const R l2 = fabs(vnr);
+    const R eps = R(0.1) * c;
+    if (l1 < eps) l1 = (l1 * l1 + eps * eps) / (R(2) * eps);
+    if (l4 < eps) l4 = (l4 * l4 + eps * eps) / (R(2) * eps);
+    const R du = uR - uL, dv = vR - vL, dp = pR - pL, dvn = du * nx + dv * ny;
+    const R c2 = fmaxR(c * c, R(1e-10));
+    const R a1 = (dp - rr * c * dvn) / (R(2) * c2);
+    const R a2 = (rhoR - rhoL) - dp / c2;
+    const R a3 = rr * (dv * nx - du * ny);
+    const R a4 = (dp + rr * c * dvn) / (R(2) * c2);
+    R FL[NV], FR[NV];
+    flux_normal_floor(UL, nx, ny, g, FL);
+    flux_normal_floor(UR, nx, ny, g, FR);
+    const R tx = -ny, ty = nx;
+    const R w1 = l1 * a1, w2 = l2 * a2, w3 = l2 * a3, w4 = l4 * a4;
+    const R dF[NV] = {w1 + w2 + w4,
+                      w1 * (ur - c * nx) + w2 * ur + w3 * tx + w4 * (ur + c * nx),
+                      w1 * (vr - c * ny) + w2 * vr + w3 * ty + w4 * (vr + c * ny),
+                      w1 * (Hr - c * vnr) + w2 * R(0.5) * q2 + w3 * (ur * tx + vr * ty) + w4 * (Hr + c * vnr)};
+#pragma unroll
+    for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (FL[i] + FR[i]) - R(0.5) * dF[i];
+}
+template <typename R>
+__device__ __forceinline__ R guard_den(R d) {
+    return fabs(d) < R(1e-10) ? (d >= R(0) ? R(1e-10) : R(-1e-10)) : d;
+}
+// RiemannSolvers::hllc (euler.hpp:298-375)
+template <typename R>
+__device__ __forceinline__ void riemann_hllc(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
+    const R rhoL = fmaxR(UL[0], R(1e-10)), rhoR = fmaxR(UR[0], R(1e-10));
+    const R uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
+    const R pL = fmaxR(pressure_of(UL, g), R(1e-10)), pR = fmaxR(pressure_of(UR, g), R(1e-10));
+    const R cL = sound_speed_of(UL, g), cR = sound_speed_of(UR, g);
+    const R vnL = uL * nx + vL * ny, vnR = uR * nx + vR * ny;
+    R ps = R(0.5) * (pL + pR) - R(0.5) * (vnR - vnL) * R(0.5) * (rhoL + rhoR) * R(0.5) * (cL + cR);
+    ps = fmaxR(ps, R(1e-10));
+    const R gp = (g + R(1)) / (R(2) * g);
+    const R qL = ps <= pL ? R(1) : sqrtR(R(1) + gp * (ps / pL - R(1)));
+    const R qR = ps <= pR ? R(1) : sqrtR(R(1) + gp * (ps / pR - R(1)));
+    const R SL = vnL - cL * qL, SR = vnR + cR * qR;
+    const R dSM = guard_den(rhoL * (SL - vnL) - rhoR * (SR - vnR));
+    const R SM = (pR - pL + rhoL * vnL * (SL - vnL) - rhoR * vnR * (SR - vnR)) / dSM;
+    if (SL >= R(0)) { flux_normal_floor(UL, nx, ny, g, F); return; }
+    if (SR <= R(0)) { flux_normal_floor(UR, nx, ny, g, F); return; }
+    const bool left = SM >= R(0);
+    const R* U = left ? UL : UR;
+    const R rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR;
+    const R vn = left ? vnL : vnR, S = left ? SL : SR, p = left ? pL : pR;
+    const R dK = guard_den(S - SM), dvn = guard_den(S - vn);
+    const R fac = rho * dvn / dK;
+    const R Us[NV] = {fac, fac * (u + (SM - vn) * nx), fac * (v + (SM - vn) * ny),
+                      fac * (U[3] / rho + (SM - vn) * (SM + p / (rho * dvn)))};
+    R Fk[NV];
+    flux_normal_floor(U, nx, ny, g, Fk);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) F[i] = Fk[i] + S * (Us[i] - U[i]);
+}
+template <typename R>
+__device__ __forceinline__ void riemann_by_kind(int kind, const R* UL, const R* UR, R nx, R ny, R g, R* F) {
+    if (kind == 1) riemann_roe(UL, UR, nx, ny, g, F);
+    else if (kind == 2) riemann_hllc(UL, UR, nx, ny, g, F);
+    else riemann_rusanov(UL, UR, nx, ny, g, F);
+}
+
 // enforcePositivityKernel (kernels.cu:1236-1288), FP64, gamma fixed at 1.4.
 __device__ __forceinline__ void positivity(double* q) {
     constexpr double GM1 = 1.4 - 1.0;

@@ -523,10 +523,24 @@ void FRSolver::initialize(const Mesh& mesh, const SimParams& params) {
     md.bc_data = bc_data_cache_.empty() ? nullptr : bc_data_cache_.data();
     md.n_bc_data = (int)(bc_data_cache_.size() / 8);
     md.elem_area = area.data();
+    // half lengths of the straight element edges (textbook correction metrics)
+    std::vector<double> ehl((size_t)std::max<Index>(E, 1) * 4);
+    for (Index e = 0; e < E; ++e) {
+        const Element& el = mesh.element(e);
+        for (int k = 0; k < 4; ++k) {
+            const Vec2 a = mesh.node(el.node_ids[k]), b = mesh.node(el.node_ids[(k + 1) % 4]);
+            ehl[(size_t)e * 4 + k] = 0.5 * std::sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y));
+        }
+    }
+    md.edge_half_len = ehl.data();
     zj_params zp{};
     zp.gamma = params.gamma; zp.cfl = params.CFL;
     zp.precision = params.precision == Precision::FP32Bulk ? ZJ_PREC_FP32BULK : ZJ_PREC_FP64;
-    zp.compat = ZJ_COMPAT_REFERENCE; zp.positivity = params.positivity ? 1 : 0; zp.device = params.device;
+    zp.compat = params.compat == Compat::Textbook ? ZJ_COMPAT_TEXTBOOK : ZJ_COMPAT_REFERENCE;
+    zp.positivity = params.positivity ? 1 : 0; zp.device = params.device;
+    zp.riemann = params.riemann == RiemannSolver::Roe ? ZJ_RIEMANN_ROE
+                 : params.riemann == RiemannSolver::HLLC ? ZJ_RIEMANN_HLLC : ZJ_RIEMANN_RUSANOV;
+    zp.corr = static_cast<int>(params.flux_type);  // DG, SD, HU, GA share ZJ_CORR_* values
     zj_handle h = nullptr;
     const int rc = zj_create(&h, &md, &zp);
     if (rc != ZJ_OK) throw std::runtime_error(std::string("FRSolver::initialize: ") + zj_last_error(nullptr));

@@ -128,6 +128,21 @@ inline double radau_left_d(int p, double x) {
 }
 inline double radau_right_d(int p, double x) { return 0.5 * (legendre_dp(p, x) + legendre_dp(p + 1, x)); }
 
+// VCJH correction-function derivatives (Vincent, Castonguay, Jameson 2011):
+// g_L = (-1)^p/2 [P_p - (eta P_{p-1} + P_{p+1})/(1+eta)], g_R(x) = g_L(-x), with
+// eta = 0 (DG), p/(p+1) (SD, equal to Huynh's g_Ga), (p+1)/p (HU, Huynh's g2).
+// kind: 0 DG, 1 SD, 2 HU, 3 GA.
+inline double vcjh_eta(int p, int kind) {
+    if (kind == 1 || kind == 3) return (double)p / (p + 1);
+    if (kind == 2) return (double)(p + 1) / p;
+    return 0.0;
+}
+inline double vcjh_left_d(int p, int kind, double x) {
+    const double eta = vcjh_eta(p, kind), sgn = (p % 2) ? -1.0 : 1.0;
+    return sgn * 0.5 * (legendre_dp(p, x) - (eta * legendre_dp(p - 1, x) + legendre_dp(p + 1, x)) / (1.0 + eta));
+}
+inline double vcjh_right_d(int p, int kind, double x) { return -vcjh_left_d(p, kind, -x); }
+
 inline int64_t node_of(const int64_t* en, int stride, int e, int k) { return en[(int64_t)e * stride + k]; }
 
 // Quad shape functions in Gmsh node order: bilinear (4), biquadratic (9) and

@@ -60,7 +60,7 @@ struct SepSmem {
 
 // One stage over tile `tile` (elements tile*EPB ...) as thread `tid`, shared
 // by the single-mesh kernel and the batched persistent kernel.
-template <typename R, int P, int EPB, int MODE, bool AFF, class SM>
+template <typename R, int P, int EPB, int MODE, bool AFF, bool TB = false, class SM>
 __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables<R>& ops, int tile, const SM& sm,
                                            int tid) {
     using Y = Lay<R, P>;
@@ -87,7 +87,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
     const R g = R(a.gamma);
 
     for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
-    for (int t = tid; t < 4 * S; t += BLOCK) sC[t] = ops.corr[t];
+    if (TB) {  // textbook: g_L', g_R' at the 1D points in place of the reference's corr table
+        if (tid < N1) { sC[tid] = ops.gL[tid]; sC[N1 + tid] = ops.gR[tid]; }
+    } else {
+        for (int t = tid; t < 4 * S; t += BLOCK) sC[t] = ops.corr[t];
+    }
     if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
 
     const int el = tid / S, sp = tid - (tid / S) * S;
@@ -103,9 +107,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
     double q[NV], u0[NV];
     R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
     if (active) {
+        R jd = R(1);  // textbook, non-affine: transformed fluxes carry J
         if (AFF) {
             jx = a.Jinv[(int64_t)e * 4]; jy = a.Jinv[(int64_t)e * 4 + 1];
             kx = a.Jinv[(int64_t)e * 4 + 2]; ky = a.Jinv[(int64_t)e * 4 + 3];
+        } else if (TB) {
+            jx = a.Jinv[gi]; jy = a.Jinv[PS + gi]; kx = a.Jinv[2 * PS + gi]; ky = a.Jinv[3 * PS + gi];
+            jd = a.Jpt[gi];
         }
         R u[NV], fx[NV], fy[NV];
 #pragma unroll
@@ -122,6 +130,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
             if (AFF) {
                 sFx[ub + v * Y::UV] = jx * fx[v] + jy * fy[v];
                 sFy[ub + v * Y::UV] = kx * fx[v] + ky * fy[v];
+            } else if (TB) {
+                sFx[ub + v * Y::UV] = jd * (jx * fx[v] + jy * fy[v]);
+                sFy[ub + v * Y::UV] = jd * (kx * fx[v] + ky * fy[v]);
             } else {
                 sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
             }
@@ -164,21 +175,40 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
             const int nl = L.nbr - e0;
             const bool inside = L.role != ROLE_BOUNDARY && nl >= 0 && nl < EPB;
             if (f == 0) sOut[el2 * 4 + k] = L.role != ROLE_BOUNDARY && !inside;
+            // reference: the neighbour's slot f on the right role (kernels.cu:577 quirk);
+            // textbook: the geometrically matching slot p-f on both sides
+            const int oslot = (TB || !right) ? P - f : f;
             if (L.role == ROLE_BOUNDARY) {
                 ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
             } else if (inside) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, right ? f : P - f, v)];
+                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
             } else {
-                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (right ? f : P - f)) * NV, other);
+                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * NV, other);
             }
+            if (TB) {
+                // one common flux per face point with the left element's normal,
+                // outward on each side; Delta = sign |e|/2 (F.n) [/J] - interpolated transformed flux
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
-            rusanov(UL, UR, nx, ny, g, Fc);
-            normal_flux(own, g, nx, ny, Fn);
-            // left: F_com - F(U).n; right: -F_com + F(U).n at its own slot f (kernels.cu:397,423)
+                for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? own[v] : other[v]; }
+                riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
+                const R ef = a.edge_f[(int64_t)e2 * 4 + k];
+                const TraceLine tl = trace_line<P>(k, f);
+                const R* Fbase = ((k & 1) ? sFx : sFy) + el2 * Y::UE;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
+                for (int v = 0; v < NV; ++v) {
+                    const R fdisc = trace_point<R, P>(Fbase + v * Y::UV, tl, sLm, sLp);
+                    Fd[v] = ef * (right ? -Fc[v] : Fc[v]) - fdisc;
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
+                rusanov(UL, UR, nx, ny, g, Fc);
+                normal_flux(own, g, nx, ny, Fn);
+                // left: F_com - F(U).n; right: -F_com + F(U).n at its own slot f (kernels.cu:397,423)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
+            }
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) sFd[Y::tr(el2, k, f, v)] = Fd[v];
@@ -189,14 +219,15 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
     double rsq = 0.0;
     if (active) {
         const int i = sp % N1, j = sp / N1;  // xi index, eta index
-        if (!AFF) { jx = a.Jinv[gi]; jy = a.Jinv[PS + gi]; kx = a.Jinv[2 * PS + gi]; ky = a.Jinv[3 * PS + gi]; }
+        if (!AFF && !TB) { jx = a.Jinv[gi]; jy = a.Jinv[PS + gi]; kx = a.Jinv[2 * PS + gi]; ky = a.Jinv[3 * PS + gi]; }
+        const R inv_j = (TB && !AFF) ? rcpR(a.Jpt[gi]) : R(1);
         double res[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const R* Fx = sFx + el * Y::UE + v * Y::UV;
             const R* Fy = sFy + el * Y::UE + v * Y::UV;
             R div;
-            if (AFF) {  // Fx, Fy hold the contravariant fluxes
+            if (AFF || TB) {  // Fx, Fy hold the contravariant (transformed) fluxes
                 R dxi = 0, deta = 0;
 #pragma unroll
                 for (int m = 0; m < N1; ++m) {
@@ -216,9 +247,15 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
                 }
                 div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
             }
-            const R c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
-                        sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
-            res[v] = double(-div - c);
+            R c;
+            if (TB) {  // xi line j: edges 3 (slot p-j) and 1 (slot j); eta line i: edges 0 (i), 2 (p-i)
+                c = sFd[Y::tr(el, 3, P - j, v)] * sC[i] + sFd[Y::tr(el, 1, j, v)] * sC[N1 + i] +
+                    sFd[Y::tr(el, 0, i, v)] * sC[j] + sFd[Y::tr(el, 2, P - i, v)] * sC[N1 + j];
+            } else {
+                c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
+                    sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
+            }
+            res[v] = TB ? double(-(div + c) * inv_j) : double(-div - c);
             rsq += res[v] * res[v];
         }
         if (MODE == MODE_RESIDUAL) {
@@ -270,9 +307,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
     }
 }
 
-template <typename R, int P, int EPB, int MODE, bool AFF>
+template <typename R, int P, int EPB, int MODE, bool AFF, bool TB = false>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(const StageArgs<R> a, const OpTables<R> ops) {
-    stage_body<R, P, EPB, MODE, AFF>(a, ops, tile_of_block(a.xcd_groups), SepSmem<R, P, EPB>{}, threadIdx.x);
+    stage_body<R, P, EPB, MODE, AFF, TB>(a, ops, tile_of_block(a.xcd_groups), SepSmem<R, P, EPB>{}, threadIdx.x);
 }
 
 // ------------------------------------------------------------ batched stage
@@ -442,44 +479,50 @@ int num_blocks(int E, int epb) { return (E + epb - 1) / epb; }
 
 #define ZJ_EPB(P) (BLOCK / (((P) + 1) * ((P) + 1)))
 
-template <typename R, int P, int EPB, bool AFF>
+template <typename R, int P, int EPB, bool AFF, bool TB>
 static hipError_t stage_dispatch_mode(int mode, const StageArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
     const dim3 grid(num_blocks(a.E, EPB)), block(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3, AFF><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <typename R, int P, bool AFF>
-static hipError_t stage_dispatch_epb(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, hipStream_t s) {
+static hipError_t stage_dispatch_epb(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool tb,
+                                     hipStream_t s) {
     constexpr int M = ZJ_EPB(P);
-    if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF>(mode, a, ops, s);
-    if (M / 2 >= 1 && epb == M / 2) return stage_dispatch_mode<R, P, (M / 2 >= 1 ? M / 2 : 1), AFF>(mode, a, ops, s);
+    if (tb) {  // the textbook scheme is built at the full tile only
+        if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF, true>(mode, a, ops, s);
+        return hipErrorInvalidValue;
+    }
+    if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF, false>(mode, a, ops, s);
+    if (M / 2 >= 1 && epb == M / 2)
+        return stage_dispatch_mode<R, P, (M / 2 >= 1 ? M / 2 : 1), AFF, false>(mode, a, ops, s);
     return hipErrorInvalidValue;
 }
 
 template <typename R, int P>
 static hipError_t stage_dispatch_aff(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool aff,
-                                     hipStream_t s) {
-    return aff ? stage_dispatch_epb<R, P, true>(mode, a, ops, epb, s)
-               : stage_dispatch_epb<R, P, false>(mode, a, ops, epb, s);
+                                     bool tb, hipStream_t s) {
+    return aff ? stage_dispatch_epb<R, P, true>(mode, a, ops, epb, tb, s)
+               : stage_dispatch_epb<R, P, false>(mode, a, ops, epb, tb, s);
 }
 
 template <typename R>
 hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool affine,
-                        hipStream_t s) {
+                        hipStream_t s, bool textbook) {
     if (a.E <= 0) return hipSuccess;
     switch (P) {
-    case 1: return stage_dispatch_aff<R, 1>(mode, a, ops, epb, affine, s);
-    case 2: return stage_dispatch_aff<R, 2>(mode, a, ops, epb, affine, s);
-    case 3: return stage_dispatch_aff<R, 3>(mode, a, ops, epb, affine, s);
-    case 4: return stage_dispatch_aff<R, 4>(mode, a, ops, epb, affine, s);
-    case 5: return stage_dispatch_aff<R, 5>(mode, a, ops, epb, affine, s);
+    case 1: return stage_dispatch_aff<R, 1>(mode, a, ops, epb, affine, textbook, s);
+    case 2: return stage_dispatch_aff<R, 2>(mode, a, ops, epb, affine, textbook, s);
+    case 3: return stage_dispatch_aff<R, 3>(mode, a, ops, epb, affine, textbook, s);
+    case 4: return stage_dispatch_aff<R, 4>(mode, a, ops, epb, affine, textbook, s);
+    case 5: return stage_dispatch_aff<R, 5>(mode, a, ops, epb, affine, textbook, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -539,9 +582,9 @@ template hipError_t launch_batched_stage<float>(int, bool, int, const BatchInst<
                                                 hipStream_t);
 
 template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, bool,
-                                        hipStream_t);
+                                        hipStream_t, bool);
 template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&, int, bool,
-                                       hipStream_t);
+                                       hipStream_t, bool);
 template hipError_t launch_trace<double>(int, const TraceArgs<double>&, const OpTables<double>&, int, hipStream_t);
 template hipError_t launch_trace<float>(int, const TraceArgs<float>&, const OpTables<float>&, int, hipStream_t);
 

@@ -30,6 +30,8 @@ struct OpTables {
     R Lm[6];
     R Lp[6];
     R corr[4 * 36];
+    R gL[6];   // textbook mode: VCJH g_L', g_R' at the 1D solution points
+    R gR[6];
 };
 
 enum StageMode : int { MODE_RESIDUAL = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3 };
@@ -61,6 +63,10 @@ struct StageArgs {
     double gamma;
     int positivity;
     int xcd_groups;          // workgroup -> tile remap groups (1: identity)
+    // textbook mode
+    const R* edge_f;         // [E][4] sign * edge half length (/ J on affine meshes)
+    const R* Jpt;            // [E*S] Jacobian determinant (non-affine meshes)
+    int riemann;             // 0 Rusanov, 1 Roe, 2 HLLC
 };
 
 template <typename R>
@@ -84,7 +90,7 @@ struct BatchInst {
 // Launch wrappers (host).  `epb` = elements per workgroup (0 = default).
 template <typename R>
 hipError_t launch_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
-                        int epb, bool affine, hipStream_t s);
+                        int epb, bool affine, hipStream_t s, bool textbook = false);
 template <typename R>
 hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, int epb,
                         hipStream_t s);

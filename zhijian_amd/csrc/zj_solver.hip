@@ -14,6 +14,7 @@
 #include "zhijian_amd.h"
 #include "zj_kernels.hpp"
 #include "zj_adjoint.hpp"
+#include "zj_host_math.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -46,6 +47,11 @@ struct zj_solver {
     void* Jinv_e = nullptr;  // per-element metric (affine meshes only)
     bool affine_ok = false;  // every element's metric is constant
     bool affine = false;     // kernels read Jinv_e
+    bool textbook = false;   // ZJ_COMPAT_TEXTBOOK
+    int riemann = 0;
+    void* edge_f_aff = nullptr;  // textbook: [E][4] sign * |edge|/2 / J (affine elements)
+    void* edge_f_pt = nullptr;   // textbook: [E][4] sign * |edge|/2 (per-point metric)
+    void* Jpt = nullptr;         // textbook: [E*S] J in the compute precision
     void* T[3] = {nullptr, nullptr, nullptr};
     EdgeLink* links = nullptr;
     void* normals = nullptr;
@@ -177,7 +183,7 @@ static int upload(zj_solver* h, void** dst, const std::vector<T>& v) {
 static void release(zj_solver* h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);
-    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
+    void* bufs[] = {h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
                     h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb};
     for (void* b : bufs)
@@ -197,8 +203,14 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
         return fail(h, ZJ_ERR_ARG, "face tables missing");
     if (p->precision != ZJ_PREC_FP64 && p->precision != ZJ_PREC_FP32BULK)
         return fail(h, ZJ_ERR_ARG, "unknown precision");
-    if (p->compat != ZJ_COMPAT_REFERENCE)
-        return fail(h, ZJ_ERR_UNSUPPORTED, "only the reference-compatible arithmetic is built");
+    if (p->compat != ZJ_COMPAT_REFERENCE && p->compat != ZJ_COMPAT_TEXTBOOK)
+        return fail(h, ZJ_ERR_ARG, "unknown compat mode");
+    if (p->compat == ZJ_COMPAT_TEXTBOOK) {
+        if (m->n_elem > 0 && (!m->J || !m->edge_half_len))
+            return fail(h, ZJ_ERR_ARG, "textbook mode needs J and edge_half_len");
+        if (p->riemann < 0 || p->riemann > 2 || p->corr < 0 || p->corr > 3)
+            return fail(h, ZJ_ERR_ARG, "unknown Riemann solver or correction function");
+    }
     if (!(p->gamma > 1.0)) return fail(h, ZJ_ERR_ARG, "gamma must exceed 1");
     if ((int64_t)m->n_elem * (P + 1) * (P + 1) * 4 > (int64_t)INT32_MAX * 4)
         return fail(h, ZJ_ERR_UNSUPPORTED, "mesh too large for 32-bit element indices");
@@ -211,6 +223,16 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
 
     std::string why;
     if (!extract_ops(m, P, h->ops64, why)) return fail(h, ZJ_ERR_UNSUPPORTED, why);
+    h->textbook = p->compat == ZJ_COMPAT_TEXTBOOK;
+    h->riemann = p->riemann;
+    if (h->textbook) {  // VCJH g_L', g_R' at the 1D Gauss points
+        double x[6], w[6];
+        zj::host::gauss_rule(P + 1, x, w);
+        for (int i = 0; i <= P; ++i) {
+            h->ops64.gL[i] = zj::host::vcjh_left_d(P, p->corr, x[i]);
+            h->ops64.gR[i] = zj::host::vcjh_right_d(P, p->corr, x[i]);
+        }
+    }
     ops_to_f32(h->ops64, h->ops32);
 
     // element-edge links from the face list (fr_solver.cpp:240-250, kernels.cu:549-595)
@@ -275,12 +297,31 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
     }
     if (!h->affine_ok) je.clear();
     h->affine = h->affine_ok;
+    // textbook correction factors: sign (-1 on edges 0, 3) * half edge length [/ J]
+    std::vector<double> efa, efp, jpt;
+    if (h->textbook) {
+        const double sgn[4] = {-1.0, 1.0, 1.0, -1.0};
+        efa.resize((size_t)E * 4); efp.resize((size_t)E * 4);
+        for (int e = 0; e < E; ++e)
+            for (int k = 0; k < 4; ++k) {
+                efp[(size_t)e * 4 + k] = sgn[k] * m->edge_half_len[(size_t)e * 4 + k];
+                efa[(size_t)e * 4 + k] = efp[(size_t)e * 4 + k] / m->J[(size_t)e * h->S];
+            }
+        jpt.assign(m->J, m->J + npts);
+    }
     int rc;
     if (h->prec == ZJ_PREC_FP64) {
         if ((rc = upload(h, &h->Jinv, jp))) return rc;
         if ((rc = upload(h, &h->Jinv_e, je))) return rc;
         if ((rc = upload(h, &h->normals, nrm))) return rc;
+        if ((rc = upload(h, &h->edge_f_aff, efa))) return rc;
+        if ((rc = upload(h, &h->edge_f_pt, efp))) return rc;
+        if ((rc = upload(h, &h->Jpt, jpt))) return rc;
     } else {
+        std::vector<float> efaf(efa.begin(), efa.end()), efpf(efp.begin(), efp.end()), jptf(jpt.begin(), jpt.end());
+        if ((rc = upload(h, &h->edge_f_aff, efaf))) return rc;
+        if ((rc = upload(h, &h->edge_f_pt, efpf))) return rc;
+        if ((rc = upload(h, &h->Jpt, jptf))) return rc;
         std::vector<float> jf(jp.begin(), jp.end()), jef(je.begin(), je.end()), nf(nrm.begin(), nrm.end());
         if ((rc = upload(h, &h->Jinv, jf))) return rc;
         if ((rc = upload(h, &h->Jinv_e, jef))) return rc;
@@ -360,6 +401,9 @@ static StageArgs<R> stage_args(zj_solver* h, int mode) {
     a.gamma = h->gamma;
     a.positivity = h->positivity;
     a.xcd_groups = h->xcd_groups;
+    a.edge_f = (const R*)(h->affine ? h->edge_f_aff : h->edge_f_pt);
+    a.Jpt = (const R*)h->Jpt;
+    a.riemann = h->riemann;
     R** T = (R**)h->T;
     switch (mode) {
     case MODE_RESIDUAL:
@@ -388,9 +432,9 @@ static StageArgs<R> stage_args(zj_solver* h, int mode) {
 static hipError_t enqueue_stage(zj_solver* h, int mode, hipStream_t s) {
     hipError_t e;
     if (h->prec == ZJ_PREC_FP64)
-        e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, h->affine, s);
+        e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, h->affine, s, h->textbook);
     else
-        e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, h->affine, s);
+        e = launch_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->epb, h->affine, s, h->textbook);
     if (e == hipSuccess && (mode == MODE_RESIDUAL || mode == MODE_STAGE3)) h->l2_blocks = h->nblk;
     return e;
 }
@@ -574,6 +618,7 @@ extern "C" int zj_set_tile(zj_handle h, int epb) {
     if (!h) return ZJ_ERR_ARG;
     const int M = max_epb(h->P);
     if (epb != M && epb != M / 2) return fail(h, ZJ_ERR_UNSUPPORTED, "tile must be the maximum or half of it");
+    if (h->textbook && epb != M) return fail(h, ZJ_ERR_UNSUPPORTED, "the textbook scheme runs at the full tile");
     if (epb == h->epb) return ZJ_OK;
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "tile change in the middle of a step");
     ZJ_HIP(h, hipSetDevice(h->device));
@@ -861,6 +906,7 @@ extern "C" int zj_adjoint_get(zj_handle h, double* ubar_aos) {
 
 extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream) {
     if (!h) return ZJ_ERR_ARG;
+    if (h->textbook) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint is built for the reference arithmetic");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
     if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
@@ -889,6 +935,7 @@ extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* str
 
 extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream) {
     if (!h || !d_w_aos || !d_ubar_aos) return ZJ_ERR_ARG;
+    if (h->textbook) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint is built for the reference arithmetic");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual VJP requested in the middle of a step");
     int rc = ensure_adjoint(h);
@@ -1020,6 +1067,10 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
         }
         if (h->epb != max_epb(h->P)) {
             b->err = "members must use the full tile of their order";
+            return ZJ_ERR_UNSUPPORTED;
+        }
+        if (h->textbook) {
+            b->err = "batches run the reference arithmetic";
             return ZJ_ERR_UNSUPPORTED;
         }
         for (int j = 0; j < i; ++j)
@@ -1182,7 +1233,7 @@ extern "C" int zj_autotune(zj_handle h, int reps, int* epb_out, int* xcd_out, do
     int nxcc = 1;
     if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || nxcc < 1) nxcc = 1;
     const int M = max_epb(h->P);
-    const int epbs[2] = {M, M / 2 >= 1 ? M / 2 : M};
+    const int epbs[2] = {M, (M / 2 >= 1 && !h->textbook) ? M / 2 : M};
     const int xcds[2] = {nxcc, 1};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ZJ_HIP(h, hipEventCreate(&e0));

@@ -40,6 +40,7 @@ class Tables:
     bc_face_map: np.ndarray
     bc_data: np.ndarray
     pos: Optional[np.ndarray] = None
+    edge_half_len: Optional[np.ndarray] = None   # [E][4], textbook mode
 
     @property
     def E(self):
@@ -88,12 +89,20 @@ def build_tables(xy, elem_nodes, p, periodic=None, bc_type_per_face=None, fs=Non
     geo = _lib.geometry(en, xy, ops, node_count=node_count, want_pos=want_pos)
     area = _lib.elem_areas(en[:, :4], xy, nv=4)
     bct, bmap, bd = bc_tables(faces, bc_type_per_face, fs)
+    corners = np.asarray(xy, np.float64)[en[:, :4]]
+    ehl = 0.5 * np.linalg.norm(np.roll(corners, -1, axis=1) - corners, axis=2)
     return Tables(p, ops, faces, normals, geo[0], geo[1], area, bct, bmap, bd,
-                  geo[2] if want_pos else None)
+                  geo[2] if want_pos else None, ehl)
+
+
+COMPATS = {"reference": 0, "textbook": 1}
+RIEMANNS = {"rusanov": 0, "roe": 1, "hllc": 2}
+CORRS = {"dg": 0, "sd": 1, "hu": 2, "ga": 3}
 
 
 class Solver:
-    def __init__(self, t: Tables, gamma=1.4, cfl=0.5, precision="fp64", positivity=True, device=0):
+    def __init__(self, t: Tables, gamma=1.4, cfl=0.5, precision="fp64", positivity=True, device=0,
+                 compat="reference", riemann="rusanov", corr="dg"):
         self.tables = t
         self.p, self.E, self.S = t.p, t.E, t.S
         self.n = self.E * self.S * 4
@@ -117,7 +126,10 @@ class Solver:
         bd = c(t.bc_data, np.float64)
         d.bc_data = _ptr(bd); d.n_bc_data = bd.size // 8
         d.elem_area = _ptr(c(t.area, np.float64))
-        prm = _lib.Params(gamma, cfl, PRECISIONS[precision], _lib.ZJ_COMPAT_REFERENCE, int(positivity), device)
+        if t.edge_half_len is not None:
+            d.edge_half_len = _ptr(c(t.edge_half_len, np.float64))
+        prm = _lib.Params(gamma, cfl, PRECISIONS[precision], COMPATS[compat], int(positivity), device,
+                          RIEMANNS[riemann], CORRS[corr])
         h = C.c_void_p()
         check(lib().zj_create(C.byref(h), C.byref(d), C.byref(prm)))
         self._h = h
