@@ -152,6 +152,32 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt):
                 dof_updates_per_s=3.0 * dof / (res[best] * 1e-3))
 
 
+def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
+    """The textbook scheme the reference's settings name (VCJH HU correction,
+    Roe flux, metric terms; stable), FP64 vs FP32-bulk: time per step and the
+    FP32 error after err_steps steps from the same state (no limiter)."""
+    from zhijian_amd import solver
+    kw = dict(compat="textbook", riemann="roe", corr="hu", positivity=False, cfl=0.5)
+    s64 = solver.Solver(tables, precision="fp64", **kw)
+    s32 = solver.Solver(tables, precision="fp32bulk", **kw)
+    s64.set_state(U0)
+    dt = s64.dt_cfl(0.5)
+    out = {"workload": "same mesh and state, textbook FR (VCJH HU, Roe, inviscid), no limiter", "dt": dt}
+    for name, sol in (("fp64", s64), ("fp32bulk", s32)):
+        ms, st = time_precision(sol, U0, dt, steps, warmup, stream, hiprt)
+        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3))
+    s64.set_state(U0)
+    s32.set_state(U0)
+    for _ in range(err_steps):
+        s64.rk3_step(dt)
+        s32.rk3_step(dt)
+    a, b = s64.get_state(), s32.get_state()
+    out["fp32bulk_error_vs_fp64"] = dict(steps=err_steps, rel_l2=float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                                         rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
+    out["fp32bulk_over_fp64"] = out["fp32bulk"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    return out
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -223,6 +249,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--textbook-steps", type=int, default=100,
+                    help="steps of the textbook-scheme FP32 error measurement (0: skip the textbook leg)")
     ap.add_argument("--batch", type=int, default=64,
                     help="members of the batched-step measurement (0: skip)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -298,6 +326,11 @@ def main():
                    rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
         log(f"fp32bulk vs fp64 after {args.err_steps} steps: {err}")
 
+    textbook = None
+    if args.textbook_steps > 0:
+        textbook = time_textbook(mesh, tables, U0, args.steps, args.warmup, args.textbook_steps, stream, hiprt)
+        log(f"textbook: {textbook}")
+
     batched = None
     if args.batch > 0:
         batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt)
@@ -343,6 +376,7 @@ def main():
         "adjoint_over_forward": adj.get("fp64", {}).get("adjoint_over_forward"),
         "adjoint": adj or None,
         "batched": batched,
+        "textbook": textbook,
         "copy_kernel_gbps": copy_gbps,
     }
     print(json.dumps(out), flush=True)

@@ -109,4 +109,18 @@ private:
     std::vector<Vec2> face_centroids_;
 };
 
+// Gmsh MSH 2.2 / 4.1 ASCII reader (src/mesh/gmsh_reader.cpp:161-760): fills
+// the nodes, elements and boundary groups, then buildFaces(), the boundary
+// tags of the faces, and computeGeometry().  Returns false with
+// errorMessage() set on failure.
+class GmshReader {
+public:
+    GmshReader() = default;
+    bool read(const std::string& filename, Mesh& mesh);
+    const std::string& errorMessage() const { return error_msg_; }
+
+private:
+    std::string error_msg_;
+};
+
 }  // namespace zhijian

@@ -219,6 +219,24 @@ int zj_batch_set_grid(zj_batch_handle b, int policy);
  * keeps the fastest; results do not depend on the choice. */
 int zj_autotune(zj_handle h, int reps, int* elems_per_block, int* xcd_groups, double* ms);
 
+/* ---- mesh ingestion: Gmsh MSH 2.2 / 4.1 ASCII -------------------------
+ * GmshReader::read (src/mesh/gmsh_reader.cpp:161-760): nodes in file order,
+ * kept 2D elements (3/6-node triangles, 4/9/25-node quads, 8-node
+ * serendipity quads completed with a centre node), boundary line edges with
+ * their physical tags, and boundary groups with names and inferred BC types.
+ * Faces are built afterwards with zj_build_faces; zj_gmsh_edge_tag gives the
+ * physical tag of a boundary face's vertex pair (0: none). */
+typedef struct zj_gmsh* zj_gmsh_handle;
+int zj_gmsh_read(const char* path, zj_gmsh_handle* out);
+const char* zj_gmsh_error(void);
+int zj_gmsh_free(zj_gmsh_handle g);
+int zj_gmsh_counts(zj_gmsh_handle g, int64_t* n_nodes, int* n_elem, int* max_nodes_per_elem, int* n_groups);
+int zj_gmsh_nodes(zj_gmsh_handle g, double* xy);
+/* kind: ElementType (0 triangle, 1 quadrilateral); nodes [E][stride], padded with -1 */
+int zj_gmsh_elements(zj_gmsh_handle g, int* kind, int* order, int* n_nodes, int64_t* nodes, int stride);
+int zj_gmsh_edge_tag(zj_gmsh_handle g, int64_t a, int64_t b, int* tag);
+int zj_gmsh_group(zj_gmsh_handle g, int i, int* tag, int* bc_type, char* name, int name_cap);
+
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times
  * with a plain grid-stride copy kernel and reports the mean GB/s (read + write

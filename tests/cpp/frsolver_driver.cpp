@@ -5,6 +5,8 @@
 //
 //   frsolver_driver mesh   <n> <out.bin>
 //       (no GPU) faces and normals of the periodic n x n unit mesh
+//   frsolver_driver gmsh   <file.msh> <out.bin>
+//       (no GPU) faces and boundary tags of a Gmsh mesh read by GmshReader
 //   frsolver_driver run    <vortex|tgv> <n> <p> <steps> <dt_max> <fp64|fp32bulk> <prefix>
 //       writes <prefix>_U0.bin, <prefix>_U.bin and one JSON line on stdout
 #include "zhijian/fr_solver.hpp"
@@ -68,6 +70,25 @@ int main(int argc, char** argv) {
         }
         dump(argv[3], out);
         std::printf("{\"faces\": %lld}\n", (long long)m.numFaces());
+        return 0;
+    }
+    if (argc >= 4 && std::strcmp(argv[1], "gmsh") == 0) {
+        // faces of a Gmsh mesh read through GmshReader: le ll re rl bc_type bc_tag per face
+        Mesh m;
+        GmshReader reader;
+        if (!reader.read(argv[2], m)) {
+            std::fprintf(stderr, "%s\n", reader.errorMessage().c_str());
+            return 3;
+        }
+        std::vector<double> out;
+        for (Index f = 0; f < m.numFaces(); ++f) {
+            const Face& fc = m.face(f);
+            out.insert(out.end(), {(double)fc.left_elem, (double)fc.left_face, (double)fc.right_elem,
+                                   (double)fc.right_face, (double)static_cast<int>(fc.bc_type), (double)fc.bc_tag});
+        }
+        dump(argv[3], out);
+        std::printf("{\"nodes\": %lld, \"elements\": %lld, \"faces\": %lld}\n", (long long)m.numNodes(),
+                    (long long)m.numElements(), (long long)m.numFaces());
         return 0;
     }
     if (argc < 9 || std::strcmp(argv[1], "run") != 0) {

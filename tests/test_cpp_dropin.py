@@ -33,6 +33,22 @@ def test_mesh_api_matches_oracle(tmp_path):
     assert np.allclose(got[:, 4:], O.face_normals(f, mesh.elem_nodes, mesh.xy), rtol=0, atol=1e-15)
 
 
+@pytest.mark.parametrize("name", ["quad_v41", "tri_v22", "serendipity_quad_v41", "uniform_quad_10x10"])
+def test_gmsh_reader_api_matches_reference(tmp_path, golden_meshes, name):
+    """GmshReader::read + Mesh::buildFaces + boundary tags (C++ API, no GPU)
+    against the compiled reference on its own test meshes."""
+    _need_driver()
+    from tests.conftest import mesh_from_golden
+    out = tmp_path / "f.bin"
+    subprocess.run([DRIVER, "gmsh", os.path.join(ROOT, "tests", "golden", "msh", name + ".msh"), str(out)],
+                   check=True, capture_output=True)
+    got = np.fromfile(out, dtype=np.float64).reshape(-1, 6).astype(np.int64)
+    g = mesh_from_golden(golden_meshes, name)
+    assert np.array_equal(got[:, :4], g["faces"])
+    assert np.array_equal(got[:, 4], g["face_bc_type"])
+    assert np.array_equal(got[:, 5], g["face_bc_tag"])
+
+
 def _run(tmp_path, kind, n, p, steps, dt_max, prec):
     prefix = str(tmp_path / f"{kind}{n}p{p}")
     print(f"driver: {kind} n={n} P{p} steps={steps} {prec}", flush=True)

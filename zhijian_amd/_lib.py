@@ -58,7 +58,9 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_set_xcd_groups", "zj_adjoint_set", "zj_adjoint_set_device", "zj_adjoint_get",
            "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
-           "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune")
+           "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
+           "zj_gmsh_read", "zj_gmsh_error", "zj_gmsh_free", "zj_gmsh_counts", "zj_gmsh_nodes",
+           "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group")
 
 
 def lib() -> C.CDLL:
@@ -101,6 +103,15 @@ def lib() -> C.CDLL:
         L.zj_batch_synchronize.argtypes = [H]
         L.zj_batch_set_grid.argtypes = [H, C.c_int]
         L.zj_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.zj_gmsh_read.argtypes = [C.c_char_p, C.POINTER(H)]
+        L.zj_gmsh_error.restype = C.c_char_p
+        L.zj_gmsh_free.argtypes = [H]
+        L.zj_gmsh_counts.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
+        L.zj_gmsh_nodes.argtypes = [H, C.c_void_p]
+        L.zj_gmsh_elements.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.zj_gmsh_edge_tag.argtypes = [H, C.c_int64, C.c_int64, C.POINTER(C.c_int)]
+        L.zj_gmsh_group.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
         L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]

@@ -453,6 +453,61 @@ FRSolver::~FRSolver() {
     if (handle_) zj_destroy(handle_);
 }
 
+// ------------------------------------------------------------- GmshReader
+bool GmshReader::read(const std::string& filename, Mesh& mesh) {
+    zj_gmsh_handle g = nullptr;
+    if (zj_gmsh_read(filename.c_str(), &g) != ZJ_OK) {
+        error_msg_ = zj_gmsh_error();
+        return false;
+    }
+    int64_t nn = 0;
+    int ne = 0, mx = 0, ng = 0;
+    zj_gmsh_counts(g, &nn, &ne, &mx, &ng);
+    std::vector<double> xy((size_t)nn * 2);
+    zj_gmsh_nodes(g, xy.data());
+    std::vector<int> kind(ne), order(ne), cnt(ne);
+    std::vector<int64_t> nodes((size_t)ne * mx);
+    zj_gmsh_elements(g, kind.data(), order.data(), cnt.data(), nodes.data(), mx);
+    mesh = Mesh();
+    for (int64_t i = 0; i < nn; ++i) mesh.nodes().push_back(Vec2(xy[2 * i], xy[2 * i + 1]));
+    for (int e = 0; e < ne; ++e) {
+        Element el;
+        el.type = static_cast<ElementType>(kind[e]);
+        el.order = order[e];
+        el.node_ids.assign(nodes.begin() + (size_t)e * mx, nodes.begin() + (size_t)e * mx + cnt[e]);
+        el.partition = 0;
+        mesh.elements().push_back(el);
+    }
+    for (int i = 0; i < ng; ++i) {
+        int tag = 0, type = 0;
+        char name[256];
+        zj_gmsh_group(g, i, &tag, &type, name, (int)sizeof(name));
+        BCInfo info{};
+        info.tag = tag;
+        info.name = name;
+        info.type = static_cast<BCType>(type);
+        info.flow_direction = Vec2(1, 0);
+        mesh.addBCInfo(tag, info);
+    }
+    mesh.buildFaces();
+    // GmshReader::applyBoundaryTags (gmsh_reader.cpp:736-758)
+    for (Index f = 0; f < mesh.numFaces(); ++f) {
+        Face& fc = mesh.face(f);
+        if (!fc.is_boundary) continue;
+        const Element& el = mesh.element(fc.left_elem);
+        const int nv = el.numVertices();
+        int tag = 0;
+        zj_gmsh_edge_tag(g, el.node_ids[fc.left_face % nv], el.node_ids[(fc.left_face + 1) % nv], &tag);
+        if (tag) {
+            fc.bc_tag = tag;
+            if (mesh.hasBCInfo(tag)) fc.bc_type = mesh.getBCInfo(tag).type;
+        }
+    }
+    zj_gmsh_free(g);
+    mesh.computeGeometry();
+    return true;
+}
+
 void FRSolver::check(int rc, const char* what) const {
     if (rc != ZJ_OK) throw std::runtime_error(std::string("FRSolver::") + what + ": " + zj_last_error(handle_));
 }

@@ -1,0 +1,426 @@
+// zj_gmsh.cpp -- native Gmsh MSH reader (ASCII 2.2 and 4.1) for the mesh
+// ingestion in front of the solver, with the reference's conventions
+// (src/mesh/gmsh_reader.cpp:161-760):
+//   - nodes numbered 0.. in file order; element node lists remapped;
+//   - kept 2D elements: 3-node triangle, 6-node triangle, 4-node quad,
+//     9-node quad, 25-node quad, and the 8-node serendipity quad, which gets a
+//     centre node 0.5*(mid-sides) - 0.25*(corners) appended to the node list
+//     (exact for bilinear geometry) and becomes a 9-node quad;
+//   - line elements of any order are boundary edges, keyed by their two end
+//     vertices, carrying the physical tag (v2: the first element tag; v4: the
+//     first physical tag of the curve entity);
+//   - boundary groups get the $PhysicalNames name (else "BC_<tag>") and a BC
+//     type inferred from it (gmsh_reader.cpp:117-158).
+// Exposed through a small C ABI (include/zhijian_amd.h) and, in C++, through
+// zhijian::GmshReader (include/zhijian/mesh.hpp).
+#include "zhijian_amd.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace zj {
+namespace gmsh {
+
+struct Elem {
+    int kind;   // ElementType: 0 triangle, 1 quadrilateral (types.hpp:41-44)
+    int order;  // geometric order
+    std::vector<int64_t> nodes;
+};
+
+struct Group {
+    std::string name;
+    int bc_type;
+};
+
+struct Parsed {
+    std::vector<double> xy;  // [n][2]
+    std::vector<Elem> elems;
+    std::map<std::pair<int64_t, int64_t>, int> edge_tag;  // sorted vertex pair -> physical tag
+    std::map<int, Group> groups;                           // physical tag -> group
+    std::string err;
+};
+
+// Nodes per Gmsh element type (types listed by the Gmsh format; -1 unknown).
+static int nodes_of(int t) {
+    switch (t) {
+    case 1: return 2;   case 2: return 3;   case 3: return 4;   case 4: return 4;
+    case 5: return 8;   case 6: return 6;   case 7: return 5;   case 8: return 3;
+    case 9: return 6;   case 10: return 9;  case 11: return 10; case 12: return 27;
+    case 13: return 18; case 14: return 14; case 15: return 1;  case 16: return 8;
+    case 20: return 9;  case 21: return 10; case 26: return 4;  case 27: return 5;
+    case 28: return 6;  case 36: return 16; case 37: return 25;
+    default: return -1;
+    }
+}
+
+static bool is_line(int t) { return t == 1 || t == 8 || t == 26 || t == 27 || t == 28; }
+
+static bool has(const std::string& s, const char* w) { return s.find(w) != std::string::npos; }
+
+// BC type from a physical-group name (gmsh_reader.cpp:117-158), as ZJ_BC_*.
+int infer_bc(const std::string& name) {
+    std::string l = name;
+    for (char& c : l) c = (char)std::tolower((unsigned char)c);
+    if (has(l, "wall")) return (has(l, "slip") || has(l, "inviscid")) ? ZJ_BC_SLIPWALL : ZJ_BC_WALL;
+    if (has(l, "airfoil") || has(l, "wing") || has(l, "body") || has(l, "surface")) return ZJ_BC_WALL;
+    if (has(l, "farfield") || has(l, "far_field") || has(l, "far-field") || has(l, "freestream"))
+        return ZJ_BC_FARFIELD;
+    if (has(l, "inflow") || has(l, "inlet")) return ZJ_BC_INFLOW;
+    if (has(l, "outflow") || has(l, "outlet") || has(l, "exit") || l == "out") return ZJ_BC_OUTFLOW;
+    if (has(l, "symmetry") || has(l, "sym")) return ZJ_BC_SYMMETRY;
+    if (has(l, "periodic")) return ZJ_BC_PERIODIC;
+    return ZJ_BC_WALL;
+}
+
+namespace {
+
+class Reader {
+public:
+    explicit Reader(std::istream& in) : in_(in) {}
+    bool line(std::string& s) { return (bool)std::getline(in_, s); }
+    // skip to the line containing `marker`
+    void skip_to(const std::string& marker) {
+        std::string s;
+        while (line(s))
+            if (has(s, marker.c_str())) return;
+    }
+
+private:
+    std::istream& in_;
+};
+
+struct State {
+    Parsed& out;
+    std::map<int64_t, int64_t> node_index;     // Gmsh node tag -> 0-based index
+    std::map<int, int> curve_phys;             // v4 curve entity -> physical tag
+    std::map<std::pair<int, int>, std::string> names;
+    std::vector<std::pair<std::vector<int64_t>, int>> bedges;  // (vertex nodes, physical tag)
+};
+
+bool add_element(State& st, int type, int phys, std::istringstream& ls) {
+    const int nn = nodes_of(type);
+    std::vector<int64_t> ids((size_t)nn);
+    for (int j = 0; j < nn; ++j) {
+        int64_t tag = 0;
+        ls >> tag;
+        auto it = st.node_index.find(tag);
+        if (it == st.node_index.end()) {
+            st.out.err = "element references unknown node tag " + std::to_string(tag);
+            return false;
+        }
+        ids[(size_t)j] = it->second;
+    }
+    if (is_line(type)) {
+        st.bedges.emplace_back(std::move(ids), phys);
+        return true;
+    }
+    Elem e;
+    switch (type) {
+    case 2: e = Elem{0, 1, ids}; break;
+    case 9: e = Elem{0, 2, ids}; break;
+    case 3: e = Elem{1, 1, ids}; break;
+    case 10: e = Elem{1, 2, ids}; break;
+    case 37: e = Elem{1, 4, ids}; break;
+    case 16: {  // serendipity: synthesise the centre node, keep as a 9-node quad
+        const std::vector<double>& xy = st.out.xy;
+        double c[2];
+        for (int d = 0; d < 2; ++d) {
+            const double mid = xy[2 * ids[4] + d] + xy[2 * ids[5] + d] + xy[2 * ids[6] + d] + xy[2 * ids[7] + d];
+            const double cor = xy[2 * ids[0] + d] + xy[2 * ids[1] + d] + xy[2 * ids[2] + d] + xy[2 * ids[3] + d];
+            c[d] = 0.5 * mid - 0.25 * cor;
+        }
+        const int64_t idx = (int64_t)(st.out.xy.size() / 2);
+        st.out.xy.push_back(c[0]);
+        st.out.xy.push_back(c[1]);
+        ids.push_back(idx);
+        e = Elem{1, 2, ids};
+        break;
+    }
+    default: return true;  // points and 3D elements are not part of a 2D mesh
+    }
+    st.out.elems.push_back(std::move(e));
+    return true;
+}
+
+bool read_nodes(Reader& r, State& st, bool v4) {
+    std::string s;
+    r.line(s);
+    std::istringstream h(s);
+    if (!v4) {
+        int64_t n = 0;
+        h >> n;
+        st.out.xy.assign((size_t)(2 * n), 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+            r.line(s);
+            std::istringstream ls(s);
+            int64_t tag;
+            double x, y, z;
+            ls >> tag >> x >> y >> z;
+            st.node_index[tag] = i;
+            st.out.xy[(size_t)(2 * i)] = x;
+            st.out.xy[(size_t)(2 * i + 1)] = y;
+        }
+    } else {
+        int64_t blocks = 0, n = 0, tmin, tmax;
+        h >> blocks >> n >> tmin >> tmax;
+        st.out.xy.assign((size_t)(2 * n), 0.0);
+        int64_t base = 0;
+        for (int64_t b = 0; b < blocks; ++b) {
+            r.line(s);
+            std::istringstream bl(s);
+            int dim, ent, param;
+            int64_t cnt = 0;
+            bl >> dim >> ent >> param >> cnt;
+            for (int64_t i = 0; i < cnt; ++i) {
+                r.line(s);
+                st.node_index[std::stoll(s)] = base + i;
+            }
+            for (int64_t i = 0; i < cnt; ++i) {
+                r.line(s);
+                std::istringstream ls(s);
+                double x, y, z;
+                ls >> x >> y >> z;
+                st.out.xy[(size_t)(2 * (base + i))] = x;
+                st.out.xy[(size_t)(2 * (base + i) + 1)] = y;
+            }
+            base += cnt;
+        }
+    }
+    r.skip_to("$EndNodes");
+    return true;
+}
+
+bool read_elements(Reader& r, State& st, bool v4) {
+    std::string s;
+    r.line(s);
+    std::istringstream h(s);
+    if (!v4) {
+        int64_t n = 0;
+        h >> n;
+        for (int64_t i = 0; i < n; ++i) {
+            r.line(s);
+            std::istringstream ls(s);
+            int id, type, ntags;
+            ls >> id >> type >> ntags;
+            int phys = 0;
+            for (int t = 0; t < ntags; ++t) {
+                int tag;
+                ls >> tag;
+                if (t == 0) phys = tag;
+            }
+            if (nodes_of(type) < 0) continue;
+            if (!add_element(st, type, phys, ls)) return false;
+        }
+    } else {
+        int64_t blocks = 0, n, tmin, tmax;
+        h >> blocks >> n >> tmin >> tmax;
+        for (int64_t b = 0; b < blocks; ++b) {
+            r.line(s);
+            std::istringstream bl(s);
+            int dim, ent, type;
+            int64_t cnt = 0;
+            bl >> dim >> ent >> type >> cnt;
+            int phys = 0;
+            if (dim == 1) {
+                auto it = st.curve_phys.find(ent);
+                if (it != st.curve_phys.end()) phys = it->second;
+            }
+            for (int64_t e = 0; e < cnt; ++e) {
+                r.line(s);
+                if (nodes_of(type) < 0 || type == 15) continue;
+                std::istringstream ls(s);
+                int64_t id;
+                ls >> id;
+                if (!add_element(st, type, phys, ls)) return false;
+            }
+        }
+    }
+    r.skip_to("$EndElements");
+    return true;
+}
+
+}  // namespace
+
+bool parse(const std::string& path, Parsed& out) {
+    std::ifstream f(path);
+    if (!f) {
+        out.err = "cannot open " + path;
+        return false;
+    }
+    Reader r(f);
+    State st{out, {}, {}, {}, {}};
+    std::string s;
+    double version = 0.0;
+    int ftype = -1;
+    bool have_nodes = false, have_elems = false;
+    while (r.line(s)) {
+        if (s.empty() || s[0] != '$') continue;
+        if (has(s, "$MeshFormat")) {
+            r.line(s);
+            std::istringstream ls(s);
+            int dsize;
+            ls >> version >> ftype >> dsize;
+            if (ftype != 0) {
+                out.err = "only ASCII Gmsh files are supported";
+                return false;
+            }
+            r.skip_to("$EndMeshFormat");
+        } else if (has(s, "$PhysicalNames")) {
+            r.line(s);
+            const int n = std::stoi(s);
+            for (int i = 0; i < n; ++i) {
+                r.line(s);
+                std::istringstream ls(s);
+                int dim, tag;
+                ls >> dim >> tag;
+                std::string name;
+                std::getline(ls, name);
+                const size_t a = name.find_first_not_of(" \t");
+                name = a == std::string::npos ? std::string() : name.substr(a);
+                if (!name.empty() && name.front() == '"') name.erase(0, 1);
+                if (!name.empty() && name.back() == '"') name.pop_back();
+                st.names[{dim, tag}] = name;
+            }
+            r.skip_to("$EndPhysicalNames");
+        } else if (has(s, "$Entities")) {
+            r.line(s);
+            std::istringstream hs(s);
+            int np, nc, ns, nv;
+            hs >> np >> nc >> ns >> nv;
+            for (int i = 0; i < np; ++i) r.line(s);
+            for (int i = 0; i < nc; ++i) {
+                r.line(s);
+                std::istringstream ls(s);
+                int tag, nphys;
+                double bb[6];
+                ls >> tag >> bb[0] >> bb[1] >> bb[2] >> bb[3] >> bb[4] >> bb[5] >> nphys;
+                if (nphys > 0) {
+                    int ph;
+                    ls >> ph;
+                    st.curve_phys[tag] = ph;
+                }
+            }
+            r.skip_to("$EndEntities");
+        } else if (has(s, "$Nodes")) {
+            if (version == 0.0) { out.err = "$Nodes before $MeshFormat"; return false; }
+            if (!read_nodes(r, st, version >= 4.0)) return false;
+            have_nodes = true;
+        } else if (has(s, "$Elements")) {
+            if (!have_nodes) { out.err = "$Elements before $Nodes"; return false; }
+            if (!read_elements(r, st, version >= 4.0)) return false;
+            have_elems = true;
+        } else if (!has(s, "$End")) {
+            r.skip_to("$End" + s.substr(1));
+        }
+    }
+    if (version == 0.0) { out.err = "no $MeshFormat section"; return false; }
+    if (!have_nodes) { out.err = "no $Nodes section"; return false; }
+    if (!have_elems) { out.err = "no $Elements section"; return false; }
+    if (out.elems.empty()) { out.err = "no 2D elements (triangles or quadrilaterals) in the file"; return false; }
+    std::set<int> tags;
+    for (const auto& be : st.bedges)
+        if (be.second > 0) tags.insert(be.second);
+    for (int t : tags) {
+        auto it = st.names.find({1, t});
+        const std::string name = it != st.names.end() ? it->second : "BC_" + std::to_string(t);
+        out.groups[t] = Group{name, infer_bc(name)};
+    }
+    for (const auto& be : st.bedges) {
+        if (be.first.size() < 2 || be.second <= 0) continue;
+        const int64_t a = std::min(be.first[0], be.first[1]), b = std::max(be.first[0], be.first[1]);
+        out.edge_tag[{a, b}] = be.second;
+    }
+    return true;
+}
+
+}  // namespace gmsh
+}  // namespace zj
+
+// ------------------------------------------------------------------ C ABI
+struct zj_gmsh {
+    zj::gmsh::Parsed p;
+    std::vector<int> group_tags;
+};
+
+static std::string g_gmsh_error;
+
+extern "C" int zj_gmsh_read(const char* path, zj_gmsh** out) {
+    if (!path || !out) return ZJ_ERR_ARG;
+    *out = nullptr;
+    zj_gmsh* g = new (std::nothrow) zj_gmsh();
+    if (!g) return ZJ_ERR_NOMEM;
+    if (!zj::gmsh::parse(path, g->p)) {
+        g_gmsh_error = g->p.err;
+        delete g;
+        return ZJ_ERR_ARG;
+    }
+    for (const auto& kv : g->p.groups) g->group_tags.push_back(kv.first);
+    *out = g;
+    return ZJ_OK;
+}
+
+extern "C" const char* zj_gmsh_error(void) { return g_gmsh_error.c_str(); }
+
+extern "C" int zj_gmsh_free(zj_gmsh* g) {
+    delete g;
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_counts(zj_gmsh* g, int64_t* n_nodes, int* n_elem, int* max_nodes, int* n_groups) {
+    if (!g) return ZJ_ERR_ARG;
+    size_t mx = 0;
+    for (const auto& e : g->p.elems) mx = std::max(mx, e.nodes.size());
+    if (n_nodes) *n_nodes = (int64_t)(g->p.xy.size() / 2);
+    if (n_elem) *n_elem = (int)g->p.elems.size();
+    if (max_nodes) *max_nodes = (int)mx;
+    if (n_groups) *n_groups = (int)g->group_tags.size();
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_nodes(zj_gmsh* g, double* xy) {
+    if (!g || !xy) return ZJ_ERR_ARG;
+    std::memcpy(xy, g->p.xy.data(), g->p.xy.size() * sizeof(double));
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_elements(zj_gmsh* g, int* kind, int* order, int* n_nodes, int64_t* nodes, int stride) {
+    if (!g || !kind || !order || !n_nodes || !nodes) return ZJ_ERR_ARG;
+    for (size_t e = 0; e < g->p.elems.size(); ++e) {
+        const auto& el = g->p.elems[e];
+        if ((int)el.nodes.size() > stride) return ZJ_ERR_ARG;
+        kind[e] = el.kind;
+        order[e] = el.order;
+        n_nodes[e] = (int)el.nodes.size();
+        for (int k = 0; k < stride; ++k) nodes[e * stride + k] = k < (int)el.nodes.size() ? el.nodes[k] : -1;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_edge_tag(zj_gmsh* g, int64_t a, int64_t b, int* tag) {
+    if (!g || !tag) return ZJ_ERR_ARG;
+    auto it = g->p.edge_tag.find({std::min(a, b), std::max(a, b)});
+    *tag = it == g->p.edge_tag.end() ? 0 : it->second;
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_group(zj_gmsh* g, int i, int* tag, int* bc_type, char* name, int name_cap) {
+    if (!g || i < 0 || i >= (int)g->group_tags.size()) return ZJ_ERR_ARG;
+    const int t = g->group_tags[(size_t)i];
+    const auto& grp = g->p.groups.at(t);
+    if (tag) *tag = t;
+    if (bc_type) *bc_type = grp.bc_type;
+    if (name && name_cap > 0) {
+        std::strncpy(name, grp.name.c_str(), (size_t)name_cap - 1);
+        name[name_cap - 1] = '\0';
+    }
+    return ZJ_OK;
+}
