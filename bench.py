@@ -178,6 +178,19 @@ def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
     return out
 
 
+def measured_traffic(n, p):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"stage_kernel_tgv{n}_p{p}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic:"):
+            f = line.split()
+            return {"fp32bulk": float(f[2]), "fp64": float(f[5])}
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -289,6 +302,7 @@ def main():
         log(f"copy kernel: {copy_gbps:.0f} GB/s over 2 x {args.probe_bytes / 2**20:.0f} MiB")
 
     res = {}
+    traffic = measured_traffic(args.n, args.p)
     for name, sol in (("fp64", s64), ("fp32bulk", s32)):
         if args.only and name != args.only:
             continue
@@ -303,7 +317,8 @@ def main():
                          dof_updates_per_s=3.0 * dof / (ms_step * 1e-3),
                          rk_steps_per_s=1e3 / ms_step,
                          roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                                       frac=achieved / HBM_PEAK_GBS, traffic=None,
+                                       frac=achieved / HBM_PEAK_GBS,
+                                       traffic=None if traffic is None else traffic[name],
                                        kernel=f"stage_kernel<{'double' if w == 8 else 'float'},P{args.p}>",
                                        bytes_per_launch=sum(by) / 3.0, affine_metric=affine,
                                        cross_tile_edge_fraction=ce / (4.0 * E),

@@ -38,6 +38,17 @@ def test_textbook_residual_solvers_and_corrections(riemann, corr):
     assert rel(s.residual(), tb.residual(U)) < TOL
 
 
+@pytest.mark.parametrize("p", [2, 4])
+def test_textbook_fp32bulk_residual(p):
+    say(f"case: textbook FP32-bulk residual vs oracle (FP64), vortex 6x6 P{p}, roe/hu")
+    s, tb, t = _pair(cases.vortex_mesh(6), p, "roe", "hu", precision="fp32bulk")
+    U = vortex_state(t, amp=0.05, seed=9)
+    s.set_state(U)
+    r = rel(s.residual(), tb.residual(U))
+    say(f"  rel L2 {r:.3e}")
+    assert r < 1e-5
+
+
 @pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
 def test_textbook_residual_orders_jittered(p):
     for jit in (0.0, 0.12):

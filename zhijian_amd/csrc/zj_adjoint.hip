@@ -175,10 +175,11 @@ __device__ void ghost_state_vjp(const R* Ui, R nx, R ny, int bct, const double* 
         if (!(ub0 >= R(-1000) && ub0 <= R(1000))) ubb = R(0);
         if (!(vb0 >= R(-1000) && vb0 <= R(1000))) vbb = R(0);
         if (!(pb0 >= R(1e3) && pb0 <= R(1e7))) pbb = R(0);
+        // p_i = (g-1)(U3 - ri (ui^2 + vi^2)/2) in the variables (ri, ui, vi, U3)
         const R pib = pok ? R(0.5) * pbb : R(0);
-        R rib = pib * gm * R(0.5) * (ui * ui + vi * vi);
+        R rib = -pib * gm * R(0.5) * (ui * ui + vi * vi);
         Ub[3] += pib * gm;
-        R uib = -pib * gm * ui, vib = -pib * gm * vi;
+        R uib = -pib * gm * ri * ui, vib = -pib * gm * ri * vi;
         if (!inflow) { rib += rbb; uib += ubb; vib += vbb; }
         Ub[1] += uib / ri; Ub[2] += vib / ri;
         rib -= (uib * ui + vib * vi) / ri;

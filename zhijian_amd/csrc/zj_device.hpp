@@ -257,10 +257,15 @@ __device__ __forceinline__ void positivity(double* q) {
     const double rn = fmax(0.01, fmin(100.0, q[0]));
     const double ir = 1.0 / fmax(rn, 1e-10);
     double u = q[1] * ir, v = q[2] * ir;
-    const double vm = sqrt(u * u + v * v);
-    if (vm > 2000.0) {
-        const double sc = 2000.0 / vm;
-        u *= sc; v *= sc;
+    // |v| > 2000 needs the square root only when |v|^2 > 4e6 (sqrt is
+    // correctly rounded and monotone, so below that it cannot exceed 2000)
+    const double s2 = u * u + v * v;
+    if (s2 > 4.0e6) {
+        const double vm = sqrt(s2);
+        if (vm > 2000.0) {
+            const double sc = 2000.0 / vm;
+            u *= sc; v *= sc;
+        }
     }
     const double ke = 0.5 * rn * (u * u + v * v);
     const double p = fmax(100.0, fmin(1e8, GM1 * (q[3] - ke)));

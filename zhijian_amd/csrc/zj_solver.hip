@@ -171,6 +171,7 @@ static void ops_to_f32(const OpTables<double>& a, OpTables<float>& b) {
     for (int t = 0; t < 36; ++t) b.D[t] = (float)a.D[t];
     for (int t = 0; t < 6; ++t) { b.Lm[t] = (float)a.Lm[t]; b.Lp[t] = (float)a.Lp[t]; }
     for (int t = 0; t < 4 * 36; ++t) b.corr[t] = (float)a.corr[t];
+    for (int t = 0; t < 6; ++t) { b.gL[t] = (float)a.gL[t]; b.gR[t] = (float)a.gR[t]; }
 }
 
 template <typename T>
