@@ -310,7 +310,7 @@ def main():
         w = 8 if name == "fp64" else 4
         info = sol.info()
         affine = info["affine"]
-        ce = cross_tile_edges(tables.faces, info["elems_per_block"])
+        ce = info["cross_tile_edges"]
         by = [alg_bytes_per_stage(E, args.p, st, name, affine, ce) for st in (1, 2, 3)]
         achieved = sum(by) / (sum(st_ms) * 1e-3) / 1e9
         res[name] = dict(ms_per_step=ms_step, stage_ms=st_ms,

@@ -104,6 +104,10 @@ typedef struct {
     const double* elem_area;     /* [E] or NULL (then the CFL dt is unavailable) */
     const double* edge_half_len; /* [E][4] half lengths of the straight element edges
                                     (edge k = node k -> node k+1); textbook mode only */
+    const double* elem_centroid; /* [E][2] or NULL.  When given, the library numbers
+                                    the elements along a Hilbert curve internally
+                                    (compact element tiles); every array the caller
+                                    passes or receives stays in the caller's order. */
 } zj_mesh_desc;
 
 typedef struct {
@@ -165,6 +169,9 @@ int zj_euler_update(double* U, const double* R, int64_t n, double dt, void* stre
 /* Launch configuration of the stage kernel (elements per workgroup). */
 int zj_set_tile(zj_handle h, int elems_per_block);
 int zj_get_info(zj_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp);
+/* Element edges whose neighbour lies in another tile at the current tile size
+ * (their traces go through HBM), and whether the elements were renumbered. */
+int zj_get_tiling(zj_handle h, int64_t* cross_tile_edges, int* reordered);
 /* Metric storage of the stage kernel.  The reference stores Jinv per solution
  * point (fr_solver.cpp:85-129); when every element is affine (its per-point
  * Jinv agree to 1e-13 relative) the library reads one Jinv per element and

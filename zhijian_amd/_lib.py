@@ -37,7 +37,7 @@ class MeshDesc(C.Structure):
                 ("face_left_local", C.c_void_p), ("face_right_elem", C.c_void_p),
                 ("face_right_local", C.c_void_p), ("bc_type", C.c_void_p),
                 ("bc_face_map", C.c_void_p), ("bc_data", C.c_void_p), ("n_bc_data", C.c_int),
-                ("elem_area", C.c_void_p), ("edge_half_len", C.c_void_p)]
+                ("elem_area", C.c_void_p), ("edge_half_len", C.c_void_p), ("elem_centroid", C.c_void_p)]
 
 
 class Params(C.Structure):
@@ -60,7 +60,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
            "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
            "zj_gmsh_read", "zj_gmsh_error", "zj_gmsh_free", "zj_gmsh_counts", "zj_gmsh_nodes",
-           "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group")
+           "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group", "zj_get_tiling")
 
 
 def lib() -> C.CDLL:
@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
         L.zj_gmsh_edge_tag.argtypes = [H, C.c_int64, C.c_int64, C.POINTER(C.c_int)]
         L.zj_gmsh_group.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.zj_get_tiling.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
         L.zj_build_faces.argtypes = [C.c_int, C.c_int, C.c_int, _lp, C.POINTER(C.c_int), _ip, _ip, _ip, _ip]
         L.zj_pair_periodic.argtypes = [C.POINTER(C.c_int), _ip, _ip, _ip, _ip, C.c_int, C.c_int, _lp, _dp,

@@ -252,10 +252,21 @@ __device__ __forceinline__ void riemann_by_kind(int kind, const R* UL, const R* 
 }
 
 // enforcePositivityKernel (kernels.cu:1236-1288), FP64, gamma fixed at 1.4.
+// FAST (the FP32-bulk form): the two divisions become a Newton-refined
+// hardware reciprocal and a multiplication by 1/(gamma-1), within 2 ulp of
+// the divisions, far below the form's FP32 rounding.
+__device__ __forceinline__ double rcp_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <bool FAST = false>
 __device__ __forceinline__ void positivity(double* q) {
     constexpr double GM1 = 1.4 - 1.0;
     const double rn = fmax(0.01, fmin(100.0, q[0]));
-    const double ir = 1.0 / fmax(rn, 1e-10);
+    const double ir = FAST ? rcp_refined(fmax(rn, 1e-10)) : 1.0 / fmax(rn, 1e-10);
     double u = q[1] * ir, v = q[2] * ir;
     // |v| > 2000 needs the square root only when |v|^2 > 4e6 (sqrt is
     // correctly rounded and monotone, so below that it cannot exceed 2000)
@@ -269,7 +280,8 @@ __device__ __forceinline__ void positivity(double* q) {
     }
     const double ke = 0.5 * rn * (u * u + v * v);
     const double p = fmax(100.0, fmin(1e8, GM1 * (q[3] - ke)));
-    q[0] = rn; q[1] = rn * u; q[2] = rn * v; q[3] = p / GM1 + ke;
+    q[0] = rn; q[1] = rn * u; q[2] = rn * v;
+    q[3] = (FAST ? p * (1.0 / GM1) : p / GM1) + ke;
 }
 
 // ------------------------------------------------------- wave reductions

@@ -588,6 +588,13 @@ void FRSolver::initialize(const Mesh& mesh, const SimParams& params) {
         }
     }
     md.edge_half_len = ehl.data();
+    // element centroids let the library tile elements along a Hilbert curve
+    std::vector<double> cen((size_t)std::max<Index>(E, 1) * 2);
+    for (Index e = 0; e < E; ++e) {
+        const Vec2 c = mesh.elementCentroid(e);
+        cen[(size_t)e * 2] = c.x; cen[(size_t)e * 2 + 1] = c.y;
+    }
+    md.elem_centroid = cen.data();
     zj_params zp{};
     zp.gamma = params.gamma; zp.cfl = params.CFL;
     zp.precision = params.precision == Precision::FP32Bulk ? ZJ_PREC_FP32BULK : ZJ_PREC_FP64;

@@ -271,7 +271,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<R>& a, const OpTables
                 else if (MODE == MODE_STAGE2) un[v] = 0.75 * u0[v] + 0.25 * q[v] + 0.25 * dt * res[v];
                 else un[v] = (1.0 / 3.0) * u0[v] + (2.0 / 3.0) * q[v] + (2.0 / 3.0) * dt * res[v];
             }
-            if (a.positivity) positivity(un);
+            if (a.positivity) positivity<INC>(un);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 double next = un[v];  // the state the next stage will read back
@@ -386,22 +386,28 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, cons
     }
 }
 
-__global__ void aos_to_soa_kernel(const double* __restrict__ aos, double* __restrict__ soa, int64_t n) {
+// AoS [E][S][4] in the caller's element order <-> SoA planes in the device
+// order; perm[e] = caller index of device element e (nullptr: identity).
+__global__ void aos_to_soa_kernel(const double* __restrict__ aos, double* __restrict__ soa, int64_t n,
+                                  const int* __restrict__ perm, int S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t src = perm ? (int64_t)perm[i / S] * S + i % S : i;
     double v[NV];
-    load4(aos + 4 * i, v);
+    load4(aos + 4 * src, v);
 #pragma unroll
     for (int k = 0; k < NV; ++k) soa[k * n + i] = v[k];
 }
 
-__global__ void soa_to_aos_kernel(const double* __restrict__ soa, double* __restrict__ aos, int64_t n) {
+__global__ void soa_to_aos_kernel(const double* __restrict__ soa, double* __restrict__ aos, int64_t n,
+                                  const int* __restrict__ perm, int S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const int64_t dst = perm ? (int64_t)perm[i / S] * S + i % S : i;
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = soa[k * n + i];
-    store4(aos + 4 * i, v);
+    store4(aos + 4 * dst, v);
 }
 
 // gpu::rkStage1/2/3 arithmetic (kernels.cu:1297-1349) on arbitrary arrays.
@@ -588,15 +594,15 @@ template hipError_t launch_stage<float>(int, int, const StageArgs<float>&, const
 template hipError_t launch_trace<double>(int, const TraceArgs<double>&, const OpTables<double>&, int, hipStream_t);
 template hipError_t launch_trace<float>(int, const TraceArgs<float>&, const OpTables<float>&, int, hipStream_t);
 
-hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n, hipStream_t s) {
+hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n, hipStream_t s, const int* perm, int S) {
     if (n <= 0) return hipSuccess;
-    aos_to_soa_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(aos, soa, n);
+    aos_to_soa_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(aos, soa, n, perm, S);
     return hipGetLastError();
 }
 
-hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n, hipStream_t s) {
+hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n, hipStream_t s, const int* perm, int S) {
     if (n <= 0) return hipSuccess;
-    soa_to_aos_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(soa, aos, n);
+    soa_to_aos_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(soa, aos, n, perm, S);
     return hipGetLastError();
 }
 

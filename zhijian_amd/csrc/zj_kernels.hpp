@@ -103,8 +103,11 @@ int default_epb(int P);
 int max_epb(int P);
 int num_blocks(int E, int epb);
 
-hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n_points, hipStream_t s);
-hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n_points, hipStream_t s);
+// perm[e] = caller index of device element e (nullptr: same order), S points per element
+hipError_t launch_aos_to_soa(const double* aos, double* soa, int64_t n_points, hipStream_t s,
+                             const int* perm = nullptr, int S = 1);
+hipError_t launch_soa_to_aos(const double* soa, double* aos, int64_t n_points, hipStream_t s,
+                             const int* perm = nullptr, int S = 1);
 hipError_t launch_set_scalar(double* p, double v, hipStream_t s);
 // dst[0..n) = v[0..n) by value through kernel arguments (no host staging)
 struct ScalarChunk {

@@ -52,6 +52,7 @@ struct zj_solver {
     void* edge_f_aff = nullptr;  // textbook: [E][4] sign * |edge|/2 / J (affine elements)
     void* edge_f_pt = nullptr;   // textbook: [E][4] sign * |edge|/2 (per-point metric)
     void* Jpt = nullptr;         // textbook: [E*S] J in the compute precision
+    int* perm = nullptr;         // device element e is the caller's element perm[e] (nullptr: same order)
     void* T[3] = {nullptr, nullptr, nullptr};
     EdgeLink* links = nullptr;
     void* normals = nullptr;
@@ -184,7 +185,7 @@ static int upload(zj_solver* h, void** dst, const std::vector<T>& v) {
 static void release(zj_solver* h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->graph) hipGraphDestroy(h->graph);
-    void* bufs[] = {h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
+    void* bufs[] = {h->perm, h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
                     h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb};
     for (void* b : bufs)
@@ -193,7 +194,87 @@ static void release(zj_solver* h) {
     if (h->stream) hipStreamDestroy(h->stream);
 }
 
+// Index of (x, y) along the Hilbert curve of an n x n grid (n a power of two).
+static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// Device element order: the caller's elements sorted along a Hilbert curve
+// through their centroids, so that a tile of consecutive elements is a
+// compact patch and fewer of its edges face another tile (55% -> 37% on a
+// 256^2 grid at 10 elements per tile).  perm[new] = old.
+static std::vector<int> hilbert_order(int E, const double* c) {
+    double x0 = c[0], x1 = c[0], y0 = c[1], y1 = c[1];
+    for (int e = 0; e < E; ++e) {
+        x0 = std::min(x0, c[2 * e]); x1 = std::max(x1, c[2 * e]);
+        y0 = std::min(y0, c[2 * e + 1]); y1 = std::max(y1, c[2 * e + 1]);
+    }
+    const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+    const uint32_t n = 1u << 16;
+    std::vector<std::pair<uint64_t, int>> key((size_t)E);
+    for (int e = 0; e < E; ++e) {
+        const uint32_t qx = (uint32_t)std::min<double>(n - 1, std::floor((c[2 * e] - x0) / span * n));
+        const uint32_t qy = (uint32_t)std::min<double>(n - 1, std::floor((c[2 * e + 1] - y0) / span * n));
+        key[(size_t)e] = {hilbert_index(n, qx, qy), e};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm((size_t)E);
+    for (int e = 0; e < E; ++e) perm[(size_t)e] = key[(size_t)e].second;
+    return perm;
+}
+
+static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p);
+
 static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
+    if (!m->elem_centroid || m->n_elem <= 1 || m->poly_order < 1 || m->poly_order > 5)
+        return build_ordered(h, m, p);
+    // the same mesh with its elements renumbered; faces keep their order
+    const int E = m->n_elem, F = m->n_faces, S = (m->poly_order + 1) * (m->poly_order + 1);
+    const std::vector<int> perm = hilbert_order(E, m->elem_centroid);
+    std::vector<int> inv((size_t)E);
+    for (int e = 0; e < E; ++e) inv[(size_t)perm[(size_t)e]] = e;
+    auto per_elem = [&](const double* src, int w) {
+        std::vector<double> out;
+        if (!src) return out;
+        out.resize((size_t)E * w);
+        for (int e = 0; e < E; ++e)
+            std::copy(src + (size_t)perm[(size_t)e] * w, src + (size_t)perm[(size_t)e] * w + w, out.begin() + (size_t)e * w);
+        return out;
+    };
+    const std::vector<double> J = per_elem(m->J, S), Jinv = per_elem(m->Jinv, 4 * S);
+    const std::vector<double> area = per_elem(m->elem_area, 1), ehl = per_elem(m->edge_half_len, 4);
+    std::vector<int> le((size_t)F), re((size_t)F);
+    for (int f = 0; f < F; ++f) {
+        const int a = m->face_left_elem ? m->face_left_elem[f] : -1;
+        const int b = m->face_right_elem ? m->face_right_elem[f] : -1;
+        if (a < 0 || a >= E || b >= E) return fail(h, ZJ_ERR_ARG, "face element index out of range");
+        le[(size_t)f] = inv[(size_t)a];
+        re[(size_t)f] = b >= 0 ? inv[(size_t)b] : b;
+    }
+    zj_mesh_desc pm = *m;
+    pm.J = m->J ? J.data() : nullptr;
+    pm.Jinv = m->Jinv ? Jinv.data() : nullptr;
+    pm.elem_area = m->elem_area ? area.data() : nullptr;
+    pm.edge_half_len = m->edge_half_len ? ehl.data() : nullptr;
+    pm.face_left_elem = F ? le.data() : m->face_left_elem;
+    pm.face_right_elem = F ? re.data() : m->face_right_elem;
+    pm.elem_centroid = nullptr;
+    int rc = build_ordered(h, &pm, p);
+    if (rc) return rc;
+    std::vector<int> pv = perm;
+    return upload(h, (void**)&h->perm, pv);
+}
+
+static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
     const int P = m->poly_order;
     if (P < 1 || P > 5) return fail(h, ZJ_ERR_ARG, "poly_order must be 1..5");
     if (m->n_elem < 0 || m->n_faces < 0) return fail(h, ZJ_ERR_ARG, "negative mesh sizes");
@@ -467,7 +548,7 @@ extern "C" int zj_set_state_device(zj_handle h, const double* d_aos, void* strea
     if (!h || !d_aos) return ZJ_ERR_ARG;
     ZJ_HIP(h, hipSetDevice(h->device));
     hipStream_t s = pick(h, stream);
-    ZJ_HIP(h, launch_aos_to_soa(d_aos, h->U, (int64_t)h->E * h->S, s));
+    ZJ_HIP(h, launch_aos_to_soa(d_aos, h->U, (int64_t)h->E * h->S, s, h->perm, h->S));
     int rc = refresh_trace(h, s);
     if (rc) return rc;
     h->has_state = true;
@@ -491,7 +572,7 @@ extern "C" int zj_get_state_device(zj_handle h, double* d_aos, void* stream) {
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "state requested in the middle of a step");
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, launch_soa_to_aos(h->U, d_aos, (int64_t)h->E * h->S, pick(h, stream)));
+    ZJ_HIP(h, launch_soa_to_aos(h->U, d_aos, (int64_t)h->E * h->S, pick(h, stream), h->perm, h->S));
     return ZJ_OK;
 }
 
@@ -513,7 +594,7 @@ extern "C" int zj_residual(zj_handle h, double* dUdt_aos, void* stream) {
     hipStream_t s = pick(h, stream);
     ZJ_HIP(h, enqueue_stage(h, MODE_RESIDUAL, s));
     if (dUdt_aos) {
-        ZJ_HIP(h, launch_soa_to_aos(h->Rres, h->aos, (int64_t)h->E * h->S, s));
+        ZJ_HIP(h, launch_soa_to_aos(h->Rres, h->aos, (int64_t)h->E * h->S, s, h->perm, h->S));
         ZJ_HIP(h, hipMemcpyAsync(dUdt_aos, h->aos, (size_t)h->E * h->S * 4 * sizeof(double),
                                  hipMemcpyDeviceToHost, s));
         ZJ_HIP(h, hipStreamSynchronize(s));
@@ -528,7 +609,7 @@ extern "C" int zj_residual_device(zj_handle h, double* d_dUdt_aos, void* stream)
     ZJ_HIP(h, hipSetDevice(h->device));
     hipStream_t s = pick(h, stream);
     ZJ_HIP(h, enqueue_stage(h, MODE_RESIDUAL, s));
-    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_dUdt_aos, (int64_t)h->E * h->S, s));
+    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_dUdt_aos, (int64_t)h->E * h->S, s, h->perm, h->S));
     return ZJ_OK;
 }
 
@@ -659,6 +740,18 @@ extern "C" int zj_set_geometry_mode(zj_handle h, int mode, int* active) {
         drop_graph(h);
     }
     if (active) *active = h->affine ? 1 : 0;
+    return ZJ_OK;
+}
+
+extern "C" int zj_get_tiling(zj_handle h, int64_t* cross_tile_edges, int* reordered) {
+    if (!h) return ZJ_ERR_ARG;
+    int64_t c = 0;
+    for (size_t s = 0; s < h->links_host.size(); ++s) {
+        const EdgeLink& L = h->links_host[s];
+        if ((L.role == ROLE_LEFT || L.role == ROLE_RIGHT) && L.nbr / h->epb != (int)(s / 4) / h->epb) ++c;
+    }
+    if (cross_tile_edges) *cross_tile_edges = c;
+    if (reordered) *reordered = h->perm ? 1 : 0;
     return ZJ_OK;
 }
 
@@ -858,7 +951,7 @@ extern "C" int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void
     int rc = ensure_adjoint(h);
     if (rc) return rc;
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, launch_aos_to_soa(d_ubar_aos, h->A_Ub, (int64_t)h->E * h->S, pick(h, stream)));
+    ZJ_HIP(h, launch_aos_to_soa(d_ubar_aos, h->A_Ub, (int64_t)h->E * h->S, pick(h, stream), h->perm, h->S));
     h->A_has_tb = false;
     h->A_seeded = true;
     return ZJ_OK;
@@ -891,7 +984,7 @@ extern "C" int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stre
     hipError_t e = h->prec == ZJ_PREC_FP64 ? complete_into<double>(h, h->A_Ub, tb, h->Rres, h->ops64, s)
                                            : complete_into<float>(h, h->A_Ub, tb, h->Rres, h->ops32, s);
     ZJ_HIP(h, e);
-    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_ubar_aos, (int64_t)h->E * h->S, s));
+    ZJ_HIP(h, launch_soa_to_aos(h->Rres, d_ubar_aos, (int64_t)h->E * h->S, s, h->perm, h->S));
     return ZJ_OK;
 }
 
@@ -943,7 +1036,7 @@ extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double
     if (rc) return rc;
     ZJ_HIP(h, hipSetDevice(h->device));
     hipStream_t s = pick(h, stream);
-    ZJ_HIP(h, launch_aos_to_soa(d_w_aos, h->Rres, (int64_t)h->E * h->S, s));
+    ZJ_HIP(h, launch_aos_to_soa(d_w_aos, h->Rres, (int64_t)h->E * h->S, s, h->perm, h->S));
     rc = refresh_trace(h, s);
     if (rc) return rc;
     // the adjoint accumulator's pending edge cotangents sit in A_Tb[A_cur]; use the other buffer
@@ -963,7 +1056,7 @@ extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double
         if (e == hipSuccess) e = complete_into<float>(h, h->Rres, tb, h->W, h->ops32, s);
     }
     ZJ_HIP(h, e);
-    ZJ_HIP(h, launch_soa_to_aos(h->W, d_ubar_aos, (int64_t)h->E * h->S, s));
+    ZJ_HIP(h, launch_soa_to_aos(h->W, d_ubar_aos, (int64_t)h->E * h->S, s, h->perm, h->S));
     return ZJ_OK;
 }
 

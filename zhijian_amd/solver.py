@@ -41,6 +41,7 @@ class Tables:
     bc_data: np.ndarray
     pos: Optional[np.ndarray] = None
     edge_half_len: Optional[np.ndarray] = None   # [E][4], textbook mode
+    centroid: Optional[np.ndarray] = None        # [E][2], device element ordering
 
     @property
     def E(self):
@@ -92,7 +93,7 @@ def build_tables(xy, elem_nodes, p, periodic=None, bc_type_per_face=None, fs=Non
     corners = np.asarray(xy, np.float64)[en[:, :4]]
     ehl = 0.5 * np.linalg.norm(np.roll(corners, -1, axis=1) - corners, axis=2)
     return Tables(p, ops, faces, normals, geo[0], geo[1], area, bct, bmap, bd,
-                  geo[2] if want_pos else None, ehl)
+                  geo[2] if want_pos else None, ehl, corners.mean(axis=1))
 
 
 COMPATS = {"reference": 0, "textbook": 1}
@@ -102,7 +103,7 @@ CORRS = {"dg": 0, "sd": 1, "hu": 2, "ga": 3}
 
 class Solver:
     def __init__(self, t: Tables, gamma=1.4, cfl=0.5, precision="fp64", positivity=True, device=0,
-                 compat="reference", riemann="rusanov", corr="dg"):
+                 compat="reference", riemann="rusanov", corr="dg", order="hilbert"):
         self.tables = t
         self.p, self.E, self.S = t.p, t.E, t.S
         self.n = self.E * self.S * 4
@@ -128,6 +129,8 @@ class Solver:
         d.elem_area = _ptr(c(t.area, np.float64))
         if t.edge_half_len is not None:
             d.edge_half_len = _ptr(c(t.edge_half_len, np.float64))
+        if order == "hilbert" and t.centroid is not None:
+            d.elem_centroid = _ptr(c(t.centroid, np.float64))
         prm = _lib.Params(gamma, cfl, PRECISIONS[precision], COMPATS[compat], int(positivity), device,
                           RIEMANNS[riemann], CORRS[corr])
         h = C.c_void_p()
@@ -267,8 +270,11 @@ class Solver:
     def info(self):
         n = C.c_int64(); e = C.c_int(); s = C.c_int()
         self._chk(lib().zj_get_info(self._h, C.byref(n), C.byref(e), C.byref(s)))
+        ce = C.c_int64(); ro = C.c_int()
+        self._chk(lib().zj_get_tiling(self._h, C.byref(ce), C.byref(ro)))
         return dict(n_dof=n.value, elems_per_block=e.value, n_sp=s.value,
-                    affine=bool(self.set_geometry_mode(-1)))
+                    affine=bool(self.set_geometry_mode(-1)), cross_tile_edges=ce.value,
+                    reordered=bool(ro.value))
 
 
 class Batch:
