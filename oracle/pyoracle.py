@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
         L.oz_max_wave_speed.restype = C.c_double
         L.oz_riemann.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
         L.oz_vcjh_deriv.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.oz_riemann_batch.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, _dp]
         L.oz_tb_create.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
         L.oz_tb_create.restype = C.c_void_p
         L.oz_tb_destroy.argtypes = [C.c_void_p]
@@ -433,3 +434,12 @@ class TextbookSolver:
         U = np.ascontiguousarray(U, np.float64).reshape(-1).copy()
         lib().oz_tb_advance(self._h, U, dt, int(positivity))
         return U
+
+
+def riemann_batch(kind, UL, UR, nrm, gamma=1.4):
+    UL = np.ascontiguousarray(UL, np.float64).reshape(-1, 4)
+    UR = np.ascontiguousarray(UR, np.float64).reshape(-1, 4)
+    nrm = np.ascontiguousarray(nrm, np.float64).reshape(-1, 2)
+    out = np.zeros_like(UL)
+    lib().oz_riemann_batch(RIEMANN[kind], UL.shape[0], UL, UR, nrm, gamma, out)
+    return out

@@ -201,6 +201,9 @@ void oz_riemann(int kind, const double* UL, const double* UR, double nx, double 
 /* VCJH correction-function derivatives gL'(x), gR'(x) at n points.  Pinned:
  * DG = the reference's Radau tables; HU = Huynh's g2; SD/GA by their zeros. */
 int oz_vcjh_deriv(int p, int kind, const double* x, int n, double* gLd, double* gRd);
+/* oz_riemann over n point pairs ([n][4] states, [n][2] normals). */
+void oz_riemann_batch(int kind, int n, const double* UL, const double* UR, const double* nrm, double gamma,
+                      double* F);
 typedef struct oz_tb oz_tb;
 /* edge_half_len: [E][4] half lengths of the straight element edges. */
 oz_tb* oz_tb_create(const oz_desc* d, const double* edge_half_len, int riemann, int corr);

@@ -167,6 +167,13 @@ void oz_riemann(int kind, const double* UL, const double* UR, double nx, double 
     else riemann_rusanov(UL, UR, nx, ny, g, F);
 }
 
+void oz_riemann_batch(int kind, int n, const double* UL, const double* UR, const double* nrm, double g,
+                      double* F) {
+    for (int i = 0; i < n; ++i)
+        oz_riemann(kind, UL + 4 * (size_t)i, UR + 4 * (size_t)i, nrm[2 * (size_t)i], nrm[2 * (size_t)i + 1], g,
+                   F + 4 * (size_t)i);
+}
+
 /* ------------------------------------------------------- VCJH correction */
 static void legendre3(int n, double x, double* P, double* dP) {
     /* P_k and P_k' for k = 0..n by the three-term recursion */

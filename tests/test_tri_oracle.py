@@ -1,0 +1,132 @@
+"""Pins of the triangle oracle (oracle/tri_oracle.py), CPU only.
+
+The reference's triangle operators are zero placeholders (basis.cpp:615-669),
+so everything here is fixed by mathematics: polynomial exactness, the
+orthonormality of the basis and the lifting identity under an exact
+triangle quadrature, freestream, conservation, linear stability and the
+order of accuracy on the isentropic vortex.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from oracle import tri_oracle as T
+from zhijian_amd import cases
+
+
+def _tri_quadrature(n):
+    """Collapsed (Duffy) Gauss rule on the reference triangle, exact for
+    degree 2n-2 in each collapsed variable."""
+    xg, wg = np.polynomial.legendre.leggauss(n)
+    pts, wts = [], []
+    for i in range(n):
+        for j in range(n):
+            a, b = xg[i], xg[j]
+            r = 0.5 * (1 + a) * (1 - b) - 1
+            pts.append((r, b))
+            wts.append(wg[i] * wg[j] * 0.5 * (1 - b))
+    return np.array(pts), np.array(wts)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_basis_orthonormal(p):
+    q, w = _tri_quadrature(p + 3)
+    V, _, _ = T.dubiner(p, q)
+    G = V.T @ (V * w[:, None])
+    assert np.allclose(G, np.eye(G.shape[0]), atol=1e-12)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_operators_exact_on_polynomials(p):
+    ops = T.operators(p)
+    sp, fp = ops["sp"], ops["fp"].reshape(-1, 2)
+    assert sp.shape[0] == (p + 1) * (p + 2) // 2
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            f = sp[:, 0] ** a * sp[:, 1] ** b
+            dr = a * sp[:, 0] ** max(a - 1, 0) * sp[:, 1] ** b
+            ds = b * sp[:, 0] ** a * sp[:, 1] ** max(b - 1, 0)
+            assert np.allclose(ops["Dr"] @ f, dr, atol=1e-10)
+            assert np.allclose(ops["Ds"] @ f, ds, atol=1e-10)
+            assert np.allclose(ops["interp"] @ f, fp[:, 0] ** a * fp[:, 1] ** b, atol=1e-12)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_lifting_identity(p):
+    """int_T phi_j q = w_j q(x_j) for every q of degree <= p."""
+    ops = T.operators(p)
+    q, w = _tri_quadrature(p + 3)
+    V, _, _ = T.dubiner(p, q)
+    Vsp, _, _ = T.dubiner(p, ops["sp"])
+    ell = V @ np.linalg.inv(Vsp)           # Lagrange basis of the solution points at q
+    fp = ops["fp"].reshape(-1, 2)
+    wj = np.tile(ops["wg"], 3)
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            qq = q[:, 0] ** a * q[:, 1] ** b
+            lhs = (ell @ ops["lift"]).T @ (w * qq)
+            rhs = wj * fp[:, 0] ** a * fp[:, 1] ** b
+            assert np.allclose(lhs, rhs, atol=1e-11)
+
+
+def _case(n, p, jitter=0.0, seed=0, L=1.0, x0=0.0, riemann="roe"):
+    xy, tris = T.split_quad_mesh(n, jitter=jitter, seed=seed, L=L)
+    xy = xy + x0
+    faces = O.build_faces(tris, nv=3)
+    faces = O.pair_periodic(faces, tris, xy, L, L, 1e-9 * L, nv=3)
+    nrm = O.face_normals(faces, tris, xy, nv=3)
+    return T.TriTextbook(p, xy, tris, faces.as_array(), nrm, riemann=riemann), xy, tris
+
+
+@pytest.mark.parametrize("riemann", ["rusanov", "roe", "hllc"])
+def test_freestream_jittered(riemann):
+    s, xy, tris = _case(4, 3, jitter=0.2, seed=2, riemann=riemann)
+    U = np.tile(cases.prim_to_cons(np.array(1.2), np.array(0.4), np.array(-0.3), np.array(1.1), 1.4),
+                (s.E * s.Np, 1)).reshape(-1)
+    assert np.abs(s.residual(U)).max() < 1e-11
+
+
+@pytest.mark.parametrize("p", [1, 3])
+def test_conservation(p):
+    s, xy, tris = _case(4, p, jitter=0.2, seed=3)
+    pos = T.sp_coords(xy, tris, s.ops)
+    U = cases.perturb(cases.tgv_ic(pos * 2 * math.pi, mach=0.3), 0.05, 4).reshape(-1)
+    R = s.residual(U).reshape(s.E, s.Np, 4)
+    M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+    tot = np.einsum("e,i,ij,ejv->v", s.met[:, 4], np.ones(s.Np), M, R)
+    scale = np.einsum("e,i,ij,ejv->v", s.met[:, 4], np.ones(s.Np), np.abs(M), np.abs(R))
+    assert np.all(np.abs(tot) <= 1e-12 * scale)
+
+
+def test_stable():
+    s, xy, tris = _case(2, 2)
+    U0 = np.tile(cases.prim_to_cons(np.array(1.0), np.array(0.5), np.array(0.2), np.array(1 / 1.4), 1.4),
+                 (s.E * s.Np, 1)).reshape(-1)
+    n = U0.size
+    Jm = np.zeros((n, n))
+    for k in range(n):
+        d = np.zeros(n)
+        d[k] = 1e-7
+        Jm[:, k] = (s.residual(U0 + d) - s.residual(U0 - d)) / 2e-7
+    lam = np.linalg.eigvals(Jm)
+    assert lam.real.max() < 1e-6 * np.abs(lam).max()
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_order_of_accuracy_vortex(p):
+    errs = []
+    for n in (16, 32):
+        s, xy, tris = _case(n, p, L=10.0, x0=-5.0)
+        pos = T.sp_coords(xy, tris, s.ops)
+        U = cases.vortex_ic(pos).reshape(-1)
+        t_end, steps = 0.5, 10 * n // 4
+        for _ in range(steps):
+            U = s.advance(U, t_end / steps)
+        ex = cases.vortex_ic(pos - np.array([t_end, 0.0])).reshape(-1)
+        M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+        d = (U - ex).reshape(s.E, s.Np, 4)[..., 0]
+        errs.append(math.sqrt(float(np.einsum("e,ei,ij,ej->", s.met[:, 4], d, M, d))))
+    order = math.log2(errs[0] / errs[1])
+    assert order > p + 0.5, (errs, order)
