@@ -244,6 +244,61 @@ int zj_gmsh_elements(zj_gmsh_handle g, int* kind, int* order, int* n_nodes, int6
 int zj_gmsh_edge_tag(zj_gmsh_handle g, int64_t a, int64_t b, int* tag);
 int zj_gmsh_group(zj_gmsh_handle g, int i, int* tag, int* bc_type, char* name, int name_cap);
 
+/* ---- triangles (textbook mode only) ---------------------------------
+ * The reference declares triangle elements (ElementType::Triangle,
+ * include/common/types.hpp) and builds their operators as zero placeholders
+ * (FROperators::initTriangle, src/basis/basis.cpp:615-669); FRSolver assumes
+ * quadrilaterals (fr_solver.cpp:138-140).  These entry points run the scheme
+ * the reference's settings name on straight-sided triangles: nodal FR with the
+ * DG-equivalent correction field (L2 lifting), warp-and-blend solution
+ * points, p+1 Gauss flux points per edge, the configured Riemann solver
+ * (euler.hpp:204-375), SSP-RK3 (include/time/rk_integrator.hpp:33-58).
+ * Params: compat = ZJ_COMPAT_TEXTBOOK, corr = ZJ_CORR_DG, positivity = 0.
+ * State layout [E][Np][4], Np = (p+1)(p+2)/2, solution points in the order
+ * zj_tri_operators returns.  Faces come from zj_build_faces (nv = 3) and
+ * zj_pair_periodic; normals from zj_face_normals. */
+typedef struct zj_tri* zj_tri_handle;
+typedef struct {
+    int poly_order;              /* 1..5 */
+    int64_t n_nodes;
+    const double* xy;            /* [n_nodes][2] */
+    int n_elem;
+    const int64_t* tri;          /* [E][3] corner nodes, counter-clockwise */
+    int n_faces;
+    const int* face_left_elem;   /* [F] */
+    const int* face_left_local;  /* [F] edge k runs from corner k to corner k+1 */
+    const int* face_right_elem;  /* [F], -1 on boundary faces */
+    const int* face_right_local; /* [F] */
+    const double* normals;       /* [F][2] unit normal, outward from the left element */
+    const int* bc_type;          /* [F] BCType per face, or NULL when there are no boundary faces */
+    const int* bc_face_map;      /* [F] bc_data row of far-field / outflow faces, or NULL */
+    const double* bc_data;       /* [n_bc_data][8] */
+    int n_bc_data;
+    int reorder;                 /* 1: number elements along a Hilbert curve internally */
+} zj_tri_desc;
+int zj_tri_create(zj_tri_handle* out, const zj_tri_desc* mesh, const zj_params* params);
+int zj_tri_destroy(zj_tri_handle h);
+const char* zj_tri_last_error(zj_tri_handle h);
+int zj_tri_set_state(zj_tri_handle h, const double* U_aos);
+int zj_tri_get_state(zj_tri_handle h, double* U_aos);
+int zj_tri_set_state_device(zj_tri_handle h, const double* d_U_aos, void* stream);
+int zj_tri_get_state_device(zj_tri_handle h, double* d_U_aos, void* stream);
+int zj_tri_residual(zj_tri_handle h, double* dUdt_aos, void* stream);  /* host array or NULL */
+int zj_tri_residual_device(zj_tri_handle h, double* d_dUdt_aos, void* stream);
+int zj_tri_rk3_step(zj_tri_handle h, double dt, void* stream);
+/* dt = cfl * d_e / ((2p+1) max(|u|+c)), d_e the inscribed-circle diameter */
+int zj_tri_dt_cfl(zj_tri_handle h, double cfl, double* dt);
+int zj_tri_residual_l2(zj_tri_handle h, double* l2);
+int zj_tri_synchronize(zj_tri_handle h);
+int zj_tri_set_xcd_groups(zj_tri_handle h, int groups);
+int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp,
+                    int64_t* cross_tile_edges);
+/* Operators of order p: solution points [Np][2] (r, s), flux points
+ * [3(p+1)][2], Gauss weights [p+1], Dr, Ds [Np][Np], interp [3(p+1)][Np],
+ * lift [Np][3(p+1)] (before the |e|/(2J) edge factor).  Any pointer may be NULL. */
+int zj_tri_operators(int p, double* sp, double* fp, double* wg, double* Dr, double* Ds,
+                     double* interp, double* lift);
+
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times
  * with a plain grid-stride copy kernel and reports the mean GB/s (read + write

@@ -23,10 +23,14 @@ follow.  This restates the textbook scheme for straight-sided triangles:
   LIFT[sp, j] = w_j sum_m psi_m(x_sp) psi_m(x_j), the L2 lifting of a unit
   value at flux point j.
 
+Boundary faces take the ghost state of the quadrilateral path
+(kernels.cu:453-525 via the C oracle) at each flux point.
+
 Pins (tests/test_tri_oracle.py): polynomial exactness of interpolation and
 differentiation; the lifting identity against an exact triangle quadrature;
-freestream on jittered meshes; conservation; linear stability; order of
-accuracy on the isentropic vortex.
+freestream on jittered meshes (periodic, and open with every BC type whose
+ghost state preserves a uniform flow); conservation; linear stability; order
+of accuracy on the isentropic vortex.
 """
 from __future__ import annotations
 
@@ -225,7 +229,8 @@ def sp_coords(xy, tris, ops):
 class TriTextbook:
     """Textbook FR residual / SSP-RK3 on straight-sided triangles."""
 
-    def __init__(self, p, xy, tris, faces, normals, riemann="roe", gamma=1.4):
+    def __init__(self, p, xy, tris, faces, normals, riemann="roe", gamma=1.4, bc_type=None, bc_face_map=None,
+                 bc_data=None):
         self.p, self.g = p, gamma
         self.ops = operators(p)
         self.Np = self.ops["sp"].shape[0]
@@ -235,6 +240,9 @@ class TriTextbook:
         self.faces = np.asarray(faces)
         self.normals = np.asarray(normals)
         self.kind = O.RIEMANN[riemann]
+        self.bc_type = None if bc_type is None else np.asarray(bc_type)
+        self.bc_face_map = None if bc_face_map is None else np.asarray(bc_face_map)
+        self.bc_data = None if bc_data is None else np.asarray(bc_data, np.float64).reshape(-1)
 
     def _flux(self, U):
         rho = U[..., 0]
@@ -256,20 +264,25 @@ class TriTextbook:
         Fxf = np.einsum("ij,ejv->eiv", I, Fx).reshape(E, 3, Fp, 4)
         Fyf = np.einsum("ij,ejv->eiv", I, Fy).reshape(E, 3, Fp, 4)
         f = self.faces
-        if np.any(f[:, 2] < 0):
-            raise ValueError("the triangle oracle covers periodic meshes")
+        inner = f[:, 2] >= 0
+        bnd = np.nonzero(~inner)[0]
         le, ll, re, rl = f[:, 0], f[:, 1], f[:, 2], f[:, 3]
         fr = np.arange(Fp)
         UL = Uf[le[:, None], ll[:, None], fr[None, :]]                 # [F][Fp][4]
-        UR = Uf[re[:, None], rl[:, None], (p - fr)[None, :]]
+        UR = Uf[np.maximum(re, 0)[:, None], rl[:, None], (p - fr)[None, :]]
+        for fi in bnd:  # boundary faces: the ghost state of each point (kernels.cu:453-525)
+            bci = int(self.bc_face_map[fi]) if self.bc_face_map is not None else 0
+            for q in range(Fp):
+                UR[fi, q] = O.ghost_state(UL[fi, q], self.normals[fi, 0], self.normals[fi, 1],
+                                          int(self.bc_type[fi]), self.bc_data, max(bci, 0), self.g)
         nrm = np.repeat(self.normals[:, None, :], Fp, axis=1)
         Fc = O.riemann_batch(list(O.RIEMANN)[self.kind], UL, UR, nrm, self.g).reshape(-1, Fp, 4)
         Fout = np.zeros((E, 3, Fp, 4))
         Fout[le[:, None], ll[:, None], fr[None, :]] = Fc
-        Fout[re[:, None], rl[:, None], (p - fr)[None, :]] = -Fc
+        Fout[re[inner][:, None], rl[inner][:, None], (p - fr)[None, :]] = -Fc[inner]
         nrm_out = np.zeros((E, 3, 2))
         nrm_out[le, ll] = self.normals
-        nrm_out[re, rl] = -self.normals
+        nrm_out[re[inner], rl[inner]] = -self.normals[inner]
         Fdn = Fxf * nrm_out[:, :, None, 0:1] + Fyf * nrm_out[:, :, None, 1:2]
         delta = (Fout - Fdn) * (self.elen[:, :, None, None] / (2 * J[:, None, None, None]))
         res = -div - np.einsum("ij,ejv->eiv", LIFT, delta.reshape(E, 3 * Fp, 4))

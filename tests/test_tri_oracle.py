@@ -130,3 +130,45 @@ def test_order_of_accuracy_vortex(p):
         errs.append(math.sqrt(float(np.einsum("e,ei,ij,ej->", s.met[:, 4], d, M, d))))
     order = math.log2(errs[0] / errs[1])
     assert order > p + 0.5, (errs, order)
+
+
+def _open_box(n, p, jitter, seed, riemann, kinds, fs):
+    """Open unit square: boundary faces typed by side (bottom, right, top,
+    left) from `kinds`; far-field/outflow rows of bc_data from `fs`."""
+    xy, tris = T.split_quad_mesh(n, jitter=jitter, seed=seed)
+    faces = O.build_faces(tris, nv=3).as_array()
+    nrm = O.face_normals(O.build_faces(tris, nv=3), tris, xy, nv=3)
+    bt = np.zeros(faces.shape[0], np.int32)
+    for f in np.nonzero(faces[:, 2] < 0)[0]:
+        nx, ny = nrm[f]
+        side = 0 if ny < -0.5 else 1 if nx > 0.5 else 2 if ny > 0.5 else 3
+        bt[f] = kinds[side]
+    rho, u, v, pr = fs
+    bc_data = np.array([[rho, u, v, pr, 0, 0, 0, 0], [pr, 0, 0, 0, 0, 0, 0, 0]], np.float64)
+    bmap = np.where(bt == 6, 1, 0).astype(np.int32)
+    s = T.TriTextbook(p, xy, tris, faces, nrm, riemann=riemann, bc_type=bt, bc_face_map=bmap, bc_data=bc_data)
+    return s, xy, tris
+
+
+@pytest.mark.parametrize("riemann", ["rusanov", "roe", "hllc"])
+def test_freestream_open_boundaries(riemann):
+    """Walls along the flow (vn = 0), far field in, outflow at the free-stream
+    pressure out: every ghost state equals the interior state."""
+    fs = (1.2, 0.7, 0.0, 2e3)
+    for kinds in ([1, 4, 2, 6], [3, 6, 1, 4], [2, 5, 3, 7]):
+        s, xy, tris = _open_box(4, 3, 0.2, 5, riemann, kinds, fs)
+        U = np.tile(cases.prim_to_cons(*(np.array(q) for q in fs), 1.4), (s.E * s.Np, 1)).reshape(-1)
+        assert np.abs(s.residual(U)).max() < 1e-9 * np.abs(U).max(), kinds
+
+
+@pytest.mark.parametrize("riemann", ["rusanov", "roe", "hllc"])
+def test_closed_box_conserves_mass(riemann):
+    """Reflecting walls on every side carry no mass flux."""
+    s, xy, tris = _open_box(4, 2, 0.2, 6, riemann, [1, 2, 3, 1], (1.0, 0.0, 0.0, 1.0))
+    pos = T.sp_coords(xy, tris, s.ops)
+    U = cases.perturb(cases.tgv_ic(pos * 2 * math.pi, mach=0.3), 0.05, 7).reshape(-1)
+    R = s.residual(U).reshape(s.E, s.Np, 4)
+    M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+    tot = np.einsum("e,ij,ej->", s.met[:, 4], M, R[..., 0])
+    scale = np.einsum("e,ij,ej->", s.met[:, 4], np.abs(M), np.abs(R[..., 0]))
+    assert abs(tot) <= 1e-12 * scale

@@ -40,6 +40,15 @@ class MeshDesc(C.Structure):
                 ("elem_area", C.c_void_p), ("edge_half_len", C.c_void_p), ("elem_centroid", C.c_void_p)]
 
 
+class TriDesc(C.Structure):
+    _fields_ = [("poly_order", C.c_int), ("n_nodes", C.c_int64), ("xy", C.c_void_p), ("n_elem", C.c_int),
+                ("tri", C.c_void_p), ("n_faces", C.c_int), ("face_left_elem", C.c_void_p),
+                ("face_left_local", C.c_void_p), ("face_right_elem", C.c_void_p),
+                ("face_right_local", C.c_void_p), ("normals", C.c_void_p), ("bc_type", C.c_void_p),
+                ("bc_face_map", C.c_void_p), ("bc_data", C.c_void_p), ("n_bc_data", C.c_int),
+                ("reorder", C.c_int)]
+
+
 class Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("precision", C.c_int),
                 ("compat", C.c_int), ("positivity", C.c_int), ("device", C.c_int),
@@ -60,7 +69,11 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
            "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
            "zj_gmsh_read", "zj_gmsh_error", "zj_gmsh_free", "zj_gmsh_counts", "zj_gmsh_nodes",
-           "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group", "zj_get_tiling")
+           "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group", "zj_get_tiling",
+           "zj_tri_create", "zj_tri_destroy", "zj_tri_last_error", "zj_tri_set_state", "zj_tri_get_state",
+           "zj_tri_set_state_device", "zj_tri_get_state_device", "zj_tri_residual", "zj_tri_residual_device",
+           "zj_tri_rk3_step", "zj_tri_dt_cfl", "zj_tri_residual_l2", "zj_tri_synchronize",
+           "zj_tri_set_xcd_groups", "zj_tri_get_info", "zj_tri_operators")
 
 
 def lib() -> C.CDLL:
@@ -127,6 +140,24 @@ def lib() -> C.CDLL:
         L.zj_synchronize.argtypes = [H]
         L.zj_copy_probe.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.zj_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _lp, _dp, C.c_int, _dp, _dp, _dp, _dp, C.c_void_p]
+        L.zj_tri_create.argtypes = [C.POINTER(H), C.POINTER(TriDesc), C.POINTER(Params)]
+        L.zj_tri_destroy.argtypes = [H]
+        L.zj_tri_last_error.argtypes = [H]
+        L.zj_tri_last_error.restype = C.c_char_p
+        L.zj_tri_set_state.argtypes = [H, _dp]
+        L.zj_tri_get_state.argtypes = [H, _dp]
+        L.zj_tri_set_state_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_get_state_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_residual.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_residual_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_rk3_step.argtypes = [H, C.c_double, C.c_void_p]
+        L.zj_tri_dt_cfl.argtypes = [H, C.c_double, C.POINTER(C.c_double)]
+        L.zj_tri_residual_l2.argtypes = [H, C.POINTER(C.c_double)]
+        L.zj_tri_synchronize.argtypes = [H]
+        L.zj_tri_set_xcd_groups.argtypes = [H, C.c_int]
+        L.zj_tri_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int64)]
+        L.zj_tri_operators.argtypes = [C.c_int] + [C.c_void_p] * 7
         _lib = L
     return _lib
 
@@ -206,6 +237,15 @@ def geometry(elem_nodes, xy, ops, node_count=None, want_pos=False):
     out = (J[: E * S], Jinv[: E * S * 4])
     if want_pos:
         out = out + (pos[: E * S * 2].reshape(E, S, 2),)
+    return out
+
+
+def tri_operators(p: int) -> dict:
+    """Triangle operators of order p as the library builds them."""
+    np_, nf = (p + 1) * (p + 2) // 2, 3 * (p + 1)
+    out = dict(sp=np.zeros((np_, 2)), fp=np.zeros((nf, 2)), wg=np.zeros(p + 1), Dr=np.zeros((np_, np_)),
+               Ds=np.zeros((np_, np_)), interp=np.zeros((nf, np_)), lift=np.zeros((np_, nf)))
+    check(lib().zj_tri_operators(p, *(out[k].ctypes.data for k in ("sp", "fp", "wg", "Dr", "Ds", "interp", "lift"))))
     return out
 
 

@@ -7,7 +7,10 @@ workloads the solver is measured and checked on:
 * isentropic vortex, periodic [-5,5]^2, Shu's vortex (strength 5, freestream
   rho=1, u=1, v=0, p=1);
 * 2D Taylor-Green vortex, periodic [0,2pi]^2, rho=1, p = 1/(gamma M^2) +
-  (cos 2x + cos 2y)/4.
+  (cos 2x + cos 2y)/4;
+* triangles: a periodic square of n x n cells, each split along its
+  (i,j)->(i+1,j+1) diagonal, interior nodes jittered; a uniform flow at Mach M
+  and angle of attack alpha carrying seeded Fourier velocity modes.
 
 Meshes follow the node/element numbering of the reference's uniform test mesh
 (test_meshes/uniform_quad_10x10.msh:258-357): nodes row-major with x fastest,
@@ -130,3 +133,63 @@ def tgv_mesh(n: int) -> QuadMesh:
 
 def vortex_mesh(n: int = 10) -> QuadMesh:
     return uniform_quad_mesh(n, n, -5.0, 5.0, -5.0, 5.0)
+
+
+@dataclass
+class TriMesh:
+    xy: np.ndarray          # [n_nodes, 2]
+    tris: np.ndarray        # [n_elem, 3] int64, counter-clockwise corners
+    n: int
+    L: float                # period in x and y
+
+    @property
+    def n_elem(self) -> int:
+        return int(self.tris.shape[0])
+
+
+def split_tri_mesh(n: int, jitter: float = 0.0, seed: int = 0, L: float = 1.0, x0: float = 0.0) -> TriMesh:
+    """n x n periodic square [x0, x0+L]^2; cell (i, j) splits into (n00, n10,
+    n11) and (n00, n11, n01), cells row-major.  Interior nodes move by
+    U(-jitter h, jitter h) per coordinate (PCG64 with `seed`, one draw pair per
+    interior node in node order), boundary nodes stay so that periodic
+    partners match."""
+    h = L / n
+    xs = np.linspace(0.0, L, n + 1)
+    X, Y = np.meshgrid(xs, xs)
+    xy = np.stack([X.reshape(-1), Y.reshape(-1)], axis=1)
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        x, y = X.reshape(-1), Y.reshape(-1)
+        inner = (x > 0) & (x < L) & (y > 0) & (y < L)
+        xy[inner] += rng.uniform(-jitter * h, jitter * h, size=(int(inner.sum()), 2))
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    n00 = (j * (n + 1) + i).reshape(-1)
+    n10, n01 = n00 + 1, n00 + n + 1
+    n11 = n01 + 1
+    tris = np.stack([np.stack([n00, n10, n11], 1), np.stack([n00, n11, n01], 1)], 1).reshape(-1, 3)
+    return TriMesh(xy + x0, tris.astype(np.int64), n, L)
+
+
+def fourier_freestream_ic(pos: np.ndarray, mach: float = 0.5, aoa_deg: float = 2.0, modes: int = 8,
+                          amp: float = 0.05, seed: int = 0, L: float = 1.0, gamma: float = 1.4) -> np.ndarray:
+    """rho = 1, |u_inf| = 1 at angle aoa, p = 1/(gamma M^2), plus `modes`
+    velocity modes amp*(a, b) sin(2 pi (kx x + ky y)/L + phi) with integer
+    wavenumbers kx, ky in [-4, 4] (not both 0), unit directions (a, b) and
+    phases phi drawn from PCG64(seed) in that order per mode."""
+    rng = np.random.default_rng(seed)
+    x, y = pos[..., 0], pos[..., 1]
+    a = math.radians(aoa_deg)
+    u = np.full_like(x, math.cos(a))
+    v = np.full_like(x, math.sin(a))
+    for _ in range(modes):
+        kx, ky = 0, 0
+        while kx == 0 and ky == 0:
+            kx, ky = (int(k) for k in rng.integers(-4, 5, size=2))
+        th = rng.uniform(0.0, 2 * math.pi)
+        phi = rng.uniform(0.0, 2 * math.pi)
+        w = np.sin(2 * math.pi * (kx * x + ky * y) / L + phi)
+        u = u + amp * math.cos(th) * w
+        v = v + amp * math.sin(th) * w
+    rho = np.ones_like(x)
+    p = np.full_like(x, 1.0 / (gamma * mach * mach))
+    return prim_to_cons(rho, u, v, p, gamma)

@@ -194,44 +194,6 @@ static void release(zj_solver* h) {
     if (h->stream) hipStreamDestroy(h->stream);
 }
 
-// Index of (x, y) along the Hilbert curve of an n x n grid (n a power of two).
-static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
-    uint64_t d = 0;
-    for (uint32_t s = n / 2; s > 0; s /= 2) {
-        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
-        d += (uint64_t)s * s * ((3u * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
-            std::swap(x, y);
-        }
-    }
-    return d;
-}
-
-// Device element order: the caller's elements sorted along a Hilbert curve
-// through their centroids, so that a tile of consecutive elements is a
-// compact patch and fewer of its edges face another tile (55% -> 37% on a
-// 256^2 grid at 10 elements per tile).  perm[new] = old.
-static std::vector<int> hilbert_order(int E, const double* c) {
-    double x0 = c[0], x1 = c[0], y0 = c[1], y1 = c[1];
-    for (int e = 0; e < E; ++e) {
-        x0 = std::min(x0, c[2 * e]); x1 = std::max(x1, c[2 * e]);
-        y0 = std::min(y0, c[2 * e + 1]); y1 = std::max(y1, c[2 * e + 1]);
-    }
-    const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
-    const uint32_t n = 1u << 16;
-    std::vector<std::pair<uint64_t, int>> key((size_t)E);
-    for (int e = 0; e < E; ++e) {
-        const uint32_t qx = (uint32_t)std::min<double>(n - 1, std::floor((c[2 * e] - x0) / span * n));
-        const uint32_t qy = (uint32_t)std::min<double>(n - 1, std::floor((c[2 * e + 1] - y0) / span * n));
-        key[(size_t)e] = {hilbert_index(n, qx, qy), e};
-    }
-    std::sort(key.begin(), key.end());
-    std::vector<int> perm((size_t)E);
-    for (int e = 0; e < E; ++e) perm[(size_t)e] = key[(size_t)e].second;
-    return perm;
-}
-
 static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p);
 
 static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
@@ -239,7 +201,7 @@ static int build(zj_solver* h, const zj_mesh_desc* m, const zj_params* p) {
         return build_ordered(h, m, p);
     // the same mesh with its elements renumbered; faces keep their order
     const int E = m->n_elem, F = m->n_faces, S = (m->poly_order + 1) * (m->poly_order + 1);
-    const std::vector<int> perm = hilbert_order(E, m->elem_centroid);
+    const std::vector<int> perm = zj::host::hilbert_order(E, m->elem_centroid);
     std::vector<int> inv((size_t)E);
     for (int e = 0; e < E; ++e) inv[(size_t)perm[(size_t)e]] = e;
     auto per_elem = [&](const double* src, int w) {
