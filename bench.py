@@ -178,6 +178,92 @@ def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
     return out
 
 
+def tri_alg_bytes_per_stage(E, p, stage, precision, cross_edges):
+    """Algorithmic HBM bytes of one triangle stage launch: the FP64 stage state
+    per solution point (stage 1 reads U^n and writes U1; stages 2 and 3 read
+    U^n and the intermediate state and write one), 10 geometry words per
+    element (affine metric, three face normals), 3 int2 links, and the
+    (p+1) x 4 trace of every element edge whose neighbour lies in another tile,
+    written once and read once in the compute precision."""
+    Np = (p + 1) * (p + 2) // 2
+    w = 8 if precision == "fp64" else 4
+    per_pt = 64 if stage == 1 else 96
+    return E * (Np * per_pt + 10 * w + 24) + cross_edges * (p + 1) * 4 * w * 2
+
+
+def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
+    """Triangles (textbook FR, DG-equivalent correction, HLLC): n x n periodic
+    unit square split along the cell diagonals, interior nodes jittered
+    U(-0.2h, 0.2h) by PCG64(2024); Mach 0.5 at 2 degrees with 8 Fourier
+    velocity modes of amplitude 0.05 (PCG64(2024)).  FP64 vs FP32-bulk time
+    per step, the FP32 error after err_steps steps, and the triangle oracle
+    (numpy, one core) on a bounded sample of the same workload."""
+    from zhijian_amd import cases
+    from zhijian_amd.tri import TriSolver, sp_coords
+    m = cases.split_tri_mesh(n, jitter=0.2, seed=2024)
+    U0 = cases.fourier_freestream_ic(sp_coords(m.xy, m.tris, p), seed=2024).reshape(-1)
+    Np = (p + 1) * (p + 2) // 2
+    dof = m.n_elem * Np * 4
+    out = {"workload": f"{n}x{n} split-quad periodic triangles ({m.n_elem} elements), P{p} ({Np} points), "
+                       f"jitter 0.2h seed 2024, Mach 0.5 at 2 deg + 8 Fourier modes (amp 0.05, seed 2024), "
+                       f"textbook FR (DG correction), HLLC, inviscid, no limiter, CFL 0.5",
+           "elements": m.n_elem, "dof": dof}
+    sols = {}
+    for name in ("fp64", "fp32bulk"):
+        s = TriSolver(m.xy, m.tris, p, periodic=(1.0, 1.0), riemann="hllc", precision=name)
+        s.set_state(U0)
+        if name == "fp64":
+            out["dt"] = dt = s.dt_cfl(0.5)
+        for _ in range(warmup):
+            s.rk3_step(dt, stream.ptr)
+        stream.synchronize()
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        for _ in range(steps):
+            s.rk3_step(dt, stream.ptr)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        ce = s.info()["cross_tile_edges"]
+        by = sum(tri_alg_bytes_per_stage(m.n_elem, p, st, name, ce) for st in (1, 2, 3))
+        achieved = by / (ms * 1e-3) / 1e9
+        out[name] = dict(ms_per_step=ms, dof_updates_per_s=3.0 * dof / (ms * 1e-3),
+                         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                                       frac=achieved / HBM_PEAK_GBS, bytes_per_launch=by / 3.0,
+                                       kernel=f"tri_stage_kernel<{'double' if name == 'fp64' else 'float'},P{p}>",
+                                       cross_tile_edge_fraction=ce / (3.0 * m.n_elem)))
+        sols[name] = s
+    for s in sols.values():
+        s.set_state(U0)
+    for _ in range(err_steps):
+        for s in sols.values():
+            s.rk3_step(dt)
+    a, b = sols["fp64"].get_state(), sols["fp32bulk"].get_state()
+    out["fp32bulk_error_vs_fp64"] = dict(steps=err_steps, rel_l2=float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                                         rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
+    out["fp32bulk_over_fp64"] = out["fp32bulk"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    for s in sols.values():
+        s.close()
+    if cpu_seconds > 0:
+        from oracle import tri_oracle as T
+        from zhijian_amd.tri import tri_faces
+        ns = min(n, 64)
+        ms_ = cases.split_tri_mesh(ns, jitter=0.2, seed=2024)
+        faces, nrm = tri_faces(ms_.xy, ms_.tris, periodic=(1.0, 1.0))
+        o = T.TriTextbook(p, ms_.xy, ms_.tris, faces, nrm, riemann="hllc")
+        U = cases.fourier_freestream_ic(sp_coords(ms_.xy, ms_.tris, p), seed=2024).reshape(-1)
+        U = o.advance(U, dt)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            U = o.advance(U, dt)
+            k += 1
+        el = time.perf_counter() - t0
+        out["cpu_port"] = dict(value=3.0 * ms_.n_elem * Np * 4 * k / el, unit="DOF-updates/s", cores=1, kind="port",
+                               sample=f"{k} SSP-RK3 steps of the {ns}x{ns} mesh of the same family "
+                                      f"({el:.1f} s, numpy oracle)")
+    return out
+
+
 def measured_traffic(n, p):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (profiles/), or None."""
@@ -266,6 +352,8 @@ def main():
                     help="steps of the textbook-scheme FP32 error measurement (0: skip the textbook leg)")
     ap.add_argument("--batch", type=int, default=64,
                     help="members of the batched-step measurement (0: skip)")
+    ap.add_argument("--tri-n", type=int, default=500,
+                    help="cells per side of the triangle leg (each split in two; 0: skip)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint/forward cost ratio (TGV 128^2 P3)")
     ap.add_argument("--only", choices=("fp64", "fp32bulk"), default=None,
@@ -357,6 +445,12 @@ def main():
             adj[prec] = time_adjoint(128, 3, prec, args.steps, args.warmup, stream, hiprt)
             log(f"adjoint {prec}: {adj[prec]}")
 
+    tri = None
+    if args.tri_n > 0:
+        tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.textbook_steps, stream, hiprt,
+                       0.0 if args.no_cpu else 5.0)
+        log(f"triangles: {tri}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.p, U0, dt, args.cpu_seconds)
@@ -392,6 +486,7 @@ def main():
         "adjoint": adj or None,
         "batched": batched,
         "textbook": textbook,
+        "triangles": tri,
         "copy_kernel_gbps": copy_gbps,
     }
     print(json.dumps(out), flush=True)

@@ -79,7 +79,11 @@ def test_tri_order_independent():
         for _ in range(3):
             s.rk3_step(1e-4)
     assert np.array_equal(a.get_state(), b.get_state())
-    assert a.info()["cross_tile_edges"] < b.info()["cross_tile_edges"]
+    m = cases.split_tri_mesh(40)
+    ce = [TriSolver(m.xy, m.tris, 3, periodic=(1.0, 1.0), order=o).info()["cross_tile_edges"]
+          for o in ("hilbert", "none")]
+    say(f"  cross-tile edge slots on 40x40: Hilbert {ce[0]}, caller order {ce[1]}")
+    assert ce[0] < ce[1]
 
 
 def test_tri_20_steps_vortex():

@@ -1,6 +1,6 @@
 // zj_tri.hip -- flux reconstruction on straight-sided triangles (textbook
 // scheme, DG-equivalent correction) and its SSP-RK3 step: one fused kernel per
-// stage, operators in LDS, C ABI zj_tri_* (include/zhijian_amd.h).
+// stage, C ABI zj_tri_* (include/zhijian_amd.h).
 //
 // The reference has no triangle semantics to follow: FROperators::initTriangle
 // leaves every table zero (src/basis/basis.cpp:615-669) and FRSolver assumes
@@ -16,18 +16,29 @@
 //   Fn_j = interpolated transformed normal flux: -Fs, Fr + Fs, -Fr on edges 0, 1, 2
 //   F^I_j = Riemann flux along the face normal, outward on each side
 //
+// Work mapping: one lane per (element, conservative variable); the four lanes
+// of an element form a DPP quad.  A lane keeps its variable at every solution
+// point in registers and applies the dense operators as sums of outer
+// products whose entries are wave-uniform, read from constant memory through
+// scalar loads: no operator or state traffic through LDS.  The Euler flux of
+// point m gathers the point's four variables across the quad by DPP.  LDS
+// holds only the element tile's flux-point traces (for the Riemann solves of
+// in-tile faces) and the common fluxes on their way back to their variables.
+//
 // Device layout (one GPU holds the whole mesh):
-//   U, W     state planes [4][E*Np] FP64 (W = U1, then U2 in place)
-//   T[3]     flux-point traces [E][3][p+1][4] in the compute precision; only
-//            edges whose neighbour lies in another element tile are written
-//            (a neighbour in the same tile reads them from LDS), rotating as
-//            the quadrilateral path's: stage 1 T0->T1, 2 T1->T2, 3 T2->T0
+//   U, W, R  state in blocks of 16 elements: [E/16][Np][16][4] FP64, so the 64
+//            lanes of a wave (16 elements x 4 variables) touch 512 contiguous
+//            bytes per solution point (W = U1, then U2 in place)
+//   T[3]     flux-point traces [E][3(p+1)][4] in the compute precision;
+//            only edges whose neighbour lies in another tile of 64 elements are
+//            written (a neighbour in the same tile is read from LDS), rotating
+//            as the quadrilateral path's: stage 1 T0->T1, 2 T1->T2, 3 T2->T0
 //   geo      [E][10]: r_x, r_y, s_x, s_y, then the face normal (as the face's
 //            left element sees it) of edges 0, 1, 2
 //   links    [E][3] int2: x = neighbour element (-1 on a boundary), y = packed
 //            neighbour edge (or BC type) | role << 3 | BC data index << 6
-//   ops      transposed operators in the compute precision:
-//            Dr^T [Np][Np], Ds^T [Np][Np], interp^T [Np][3(p+1)], lift^T [3(p+1)][Np]
+//   ops      constant memory, per order: Dr, Ds [Np][Np], interp [3(p+1)][Np],
+//            lift [Np][3(p+1)]
 #include "zhijian_amd.h"
 #include "zj_device.hpp"
 #include "zj_host_math.hpp"
@@ -39,6 +50,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,29 +63,55 @@ struct TriShape {
     static constexpr int NP = (P + 1) * (P + 2) / 2;        // solution points
     static constexpr int FP = P + 1;                        // flux points per edge
     static constexpr int NFP = 3 * FP;
-    static constexpr int EPB = BLOCK / NP;                  // elements per workgroup
-    static constexpr int NT = EPB * NP;                     // threads holding a solution point
-    static constexpr int NI = EPB * NFP;                    // flux-point items per tile
-    static constexpr int NIT = (NI + BLOCK - 1) / BLOCK;
-    static constexpr int OPS = 2 * NP * NP + 2 * NFP * NP;  // operator words
+    static constexpr int MINE = (NFP + NV - 1) / NV;        // flux points per lane in the Riemann phase
 };
 
-inline int tri_epb(int P) { return BLOCK / ((P + 1) * (P + 2) / 2); }
+constexpr int TILE = BLOCK / NV;  // elements per workgroup (one lane per element and variable)
+constexpr int EB = 16;            // elements per block of the state and trace layouts
+constexpr int GEO = 10;           // geometry words per element
 
-constexpr int GEO = 10;  // geometry words per element
+constexpr int ops_words(int P) {
+    return 2 * ((P + 1) * (P + 2) / 2) * ((P + 1) * (P + 2) / 2) + 2 * 3 * (P + 1) * ((P + 1) * (P + 2) / 2);
+}
+constexpr int ops_offset(int P) {
+    int o = 0;
+    for (int q = 1; q < P; ++q) o += ops_words(q);
+    return o;
+}
+constexpr int OPS_TOTAL = ops_offset(6);
+
+__constant__ double c_ops64[OPS_TOTAL];
+__constant__ float c_ops32[OPS_TOTAL];
+
+template <typename R> __device__ __forceinline__ R cop(int k);
+template <> __device__ __forceinline__ double cop<double>(int k) { return c_ops64[k]; }
+template <> __device__ __forceinline__ float cop<float>(int k) { return c_ops32[k]; }
+
+// value of lane W of this lane's quad
+template <int W> __device__ __forceinline__ double quad_get(double x) { return dpp_mov<W * 0x55>(x); }
+template <int W> __device__ __forceinline__ float quad_get(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), W * 0x55, 0xF, 0xF, false));
+}
+
+// offset of variable 0 of (element e, point j) in a [E/16][n][16][4] array
+__device__ __forceinline__ int64_t blk(int e, int j, int n) {
+    return ((int64_t)(e >> 4) * n + j) * (EB * NV) + (e & (EB - 1)) * NV;
+}
+// offset of variable 0 of (element e, flux point j) in the [E][n][4] trace
+// array: the p+1 points of an edge share one 128-byte line (FP64, p = 3)
+__device__ __forceinline__ int64_t trix(int e, int j, int n) { return ((int64_t)e * n + j) * NV; }
 
 template <typename R>
 struct TriArgs {
     int E;
-    const double* U_in;   // stage input planes
+    const double* U_in;   // stage input
     const double* U0;     // U^n (stages 2, 3)
-    double* U_out;        // stage output planes (may alias U_in or U0 element-locally)
-    double* R_out;        // residual planes (MODE_RESIDUAL)
+    double* U_out;        // stage output (may alias U_in or U0 lane-locally)
+    double* R_out;        // residual (MODE_RESIDUAL)
     const R* geo;
     const int2* links;
     const R* T_in;
     R* T_out;
-    const R* ops;
     const double* bc_data;
     const double* dt_ptr;
     double* l2_partial;   // per tile sum of R^2, or nullptr
@@ -86,183 +124,177 @@ __device__ __forceinline__ int link_edge(int info) { return info & 7; }
 __device__ __forceinline__ int link_role(int info) { return (info >> 3) & 3; }
 __device__ __forceinline__ int link_bc(int info) { return info >> 6; }
 
-// |e|/(2J) of local edge k from the affine metric.
-template <typename R>
-__device__ __forceinline__ R edge_scale(int k, R rx, R ry, R sx, R sy) {
-    const R ax = k == 0 ? sx : (k == 1 ? rx + sx : rx);
-    const R ay = k == 0 ? sy : (k == 1 ? ry + sy : ry);
-    return sqrtR(ax * ax + ay * ay);
-}
+// 1/x: a Newton-refined hardware reciprocal in FP64 (within 2 ulp), the
+// hardware reciprocal in FP32.
+template <typename R> __device__ __forceinline__ R fast_rcp(R x);
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return rcp_refined(x); }
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename R, int P, int MODE>
 __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
     using S = TriShape<P>;
-    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP, EPB = S::EPB, NIT = S::NIT;
-    __shared__ R sOp[S::OPS];
-    __shared__ R sU[EPB * NP * NV];
-    __shared__ R sF[EPB * NP * 2 * NV];
-    __shared__ R sT[EPB * NFP * NV];  // own traces, then the flux differences
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int NW = NP > NFP ? NP : NFP;
+    constexpr int ODR = ops_offset(P), ODS = ODR + NP * NP, OI = ODS + NP * NP, OL = OI + NFP * NP;
+    __shared__ R sT[TILE * NFP * NV];  // own traces
+    __shared__ R sW[TILE * NW * NV];   // the stage input state; neighbour traces from other tiles,
+                                       // then the scaled common fluxes
     __shared__ double sred[BLOCK / 64];
-    R* const sD = sT;
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
     const int tile = tile_of_block(a.xcd_groups);
-    const int e0 = tile * EPB;
-    for (int k = tid; k < S::OPS; k += BLOCK) sOp[k] = a.ops[k];
-    const R* DrT = sOp;
-    const R* DsT = DrT + NP * NP;
-    const R* IT = DsT + NP * NP;
-    const R* LT = IT + NP * NFP;
+    const int e0 = tile * TILE, e = e0 + el;
+    const bool live = e < a.E;
     const R g = R(a.gamma);
-    const int64_t PS = (int64_t)a.E * NP;
+    const int64_t base = blk(e, 0, NP) + v;  // this lane's variable at point 0; point i at + i*EB*NV
 
-    // ---- 1: state and contravariant fluxes at the solution points
-    const int el = tid / NP, i = tid - el * NP;
-    const int e = e0 + el;
-    const bool live = tid < S::NT && e < a.E;
-    const int64_t gi = (int64_t)e * NP + i;
-    double uin[NV] = {0.0, 0.0, 0.0, 0.0};
+    double u[NP];
+    R div[NP], fn[NFP];
+    R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
+    int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
+
+    // ---- 0: this lane's variable at every solution point (one coalesced
+    // 512-byte row per point and wave), kept for the stage update; the
+    // neighbour traces this lane's Riemann solves will need from other tiles
+    // are requested now so that their latency hides behind phase 1
+    R nrm[6];
+    R pre[S::MINE][NV];
     if (live) {
-        R u[NV], fx[NV], fy[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            uin[v] = a.U_in[v * PS + gi];
-            u[v] = R(uin[v]);
-            sU[(el * NP + i) * NV + v] = u[v];
-        }
-        euler_flux(u, g, fx, fy);
+        for (int m = 0; m < NP; ++m) u[m] = a.U_in[base + (int64_t)m * EB * NV];
         const R* G = a.geo + (int64_t)e * GEO;
-        const R rx = G[0], ry = G[1], sx = G[2], sy = G[3];
+        rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            sF[(el * NP + i) * 2 * NV + v] = rx * fx[v] + ry * fy[v];
-            sF[(el * NP + i) * 2 * NV + NV + v] = sx * fx[v] + sy * fy[v];
+        for (int k = 0; k < 6; ++k) nrm[k] = G[4 + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
+#pragma unroll
+        for (int k = 0; k < S::MINE; ++k) {
+            const int j = v + NV * k;
+            const int kk = j / FP, f = j - kk * FP;
+            const int2 Lk = kk == 0 ? L[0] : (kk == 1 ? L[1] : L[2]);
+            const int nl = Lk.x - e0;
+            if (j < NFP && link_role(Lk.y) != ROLE_BOUNDARY && (nl < 0 || nl >= TILE))
+                load4(a.T_in + trix(Lk.x, link_edge(Lk.y) * FP + (P - f), NFP), pre[k]);
+        }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) sW[(el * NP + m) * NV + v] = R(u[m]);
+    }
+    __syncthreads();
+
+    // ---- 1: fluxes, divergence, traces and interpolated normal fluxes, as
+    // sums over the solution points m of outer products with wave-uniform
+    // operator columns
+    if (live) {
+        R ut[NFP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) div[i] = R(0);
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) { ut[j] = R(0); fn[j] = R(0); }
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R um = sW[(el * NP + m) * NV + v];
+            const R q0 = quad_get<0>(um), q1 = quad_get<1>(um), q2 = quad_get<2>(um), q3 = quad_get<3>(um);
+            const R inv = fast_rcp(q0);
+            const R ux = q1 * inv, uy = q2 * inv;
+            const R p = (g - R(1)) * (q3 - R(0.5) * (q1 * ux + q2 * uy));
+            const R fx = v == 0 ? q1 : (v == 1 ? q1 * ux + p : (v == 2 ? q1 * uy : (q3 + p) * ux));
+            const R fy = v == 0 ? q2 : (v == 1 ? q2 * ux : (v == 2 ? q2 * uy + p : (q3 + p) * uy));
+            const R fr = rx * fx + ry * fy, fs = sx * fx + sy * fy, frs = fr + fs;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                div[i] = fmaR(cop<R>(ODS + i * NP + m), fs, fmaR(cop<R>(ODR + i * NP + m), fr, div[i]));
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) {
+                const R w = cop<R>(OI + j * NP + m);
+                ut[j] = fmaR(w, um, ut[j]);
+                fn[j] = fmaR(w, j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) sT[(el * NFP + j) * NV + v] = ut[j];
+    }
+    __syncthreads();  // traces in place, state reads done
+    // the prefetched neighbour traces move to their flux points' slots in sW
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < S::MINE; ++k) {
+            const int j = v + NV * k;
+            if (j < NFP)
+#pragma unroll
+                for (int w = 0; w < NV; ++w) sW[(el * NFP + j) * NV + w] = pre[k][w];
         }
     }
     __syncthreads();
 
-    // ---- 2: own traces and the interpolated transformed normal flux
-    R fn[NIT][NV];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int t = tid + it * BLOCK;
-        const int el2 = t / NFP, j = t - el2 * NFP;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) fn[it][v] = R(0);
-        if (t < S::NI && e0 + el2 < a.E) {
-            R ut[NV], ar[NV], as[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { ut[v] = R(0); ar[v] = R(0); as[v] = R(0); }
-#pragma unroll 2
-            for (int m = 0; m < NP; ++m) {
-                const R w = IT[m * NFP + j];
-                const R* u = sU + (el2 * NP + m) * NV;
-                const R* f = sF + (el2 * NP + m) * 2 * NV;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    ut[v] = fmaR(w, u[v], ut[v]);
-                    ar[v] = fmaR(w, f[v], ar[v]);
-                    as[v] = fmaR(w, f[NV + v], as[v]);
-                }
-            }
-            const int k = j / FP;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                fn[it][v] = k == 0 ? -as[v] : (k == 1 ? ar[v] + as[v] : -ar[v]);
-                sT[(el2 * NFP + j) * NV + v] = ut[v];
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- 3: common flux and flux difference per flux point
-    bool cross[NIT];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int t = tid + it * BLOCK;
-        const int el2 = t / NFP, j = t - el2 * NFP;
-        const int e2 = e0 + el2;
-        cross[it] = false;
-        if (t < S::NI && e2 < a.E) {
-            const int k = j / FP, f = j - k * FP;
-            const int2 L = a.links[(int64_t)e2 * 3 + k];
-            const int role = link_role(L.y);
-            const R* G = a.geo + (int64_t)e2 * GEO;
-            const R nx = G[4 + 2 * k], ny = G[5 + 2 * k];
+    // ---- 2: common fluxes; lane v of the quad takes flux points v, v+4, ...
+    if (live) {
+#pragma unroll 1
+        for (int j = v; j < NFP; j += NV) {
+            const int kk = j / FP, f = j - kk * FP;
+            const int2 Lk = kk == 0 ? L[0] : (kk == 1 ? L[1] : L[2]);
+            const int role = link_role(Lk.y);
+            const R nx = kk == 0 ? nrm[0] : (kk == 1 ? nrm[2] : nrm[4]);
+            const R ny = kk == 0 ? nrm[1] : (kk == 1 ? nrm[3] : nrm[5]);
             R own[NV], oth[NV], UL[NV], UR[NV], Fc[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) own[v] = sT[(el2 * NFP + j) * NV + v];
+            for (int w = 0; w < NV; ++w) own[w] = sT[(el * NFP + j) * NV + w];
             if (role == ROLE_BOUNDARY) {
-                ghost_state(own, nx, ny, link_edge(L.y), a.bc_data + (int64_t)link_bc(L.y) * 8, g, oth);
+                ghost_state(own, nx, ny, link_edge(Lk.y), a.bc_data + (int64_t)link_bc(Lk.y) * 8, g, oth);
             } else {
-                const int nl = L.x - e0;
-                const int oj = link_edge(L.y) * FP + (P - f);
-                if (nl >= 0 && nl < EPB) {
+                const int nl = Lk.x - e0;
+                const int oj = link_edge(Lk.y) * FP + (P - f);
+                if (nl >= 0 && nl < TILE) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) oth[v] = sT[(nl * NFP + oj) * NV + v];
+                    for (int w = 0; w < NV; ++w) oth[w] = sT[(nl * NFP + oj) * NV + w];
                 } else {
-                    cross[it] = true;
-                    load4(a.T_in + ((int64_t)L.x * NFP + oj) * NV, oth);
+#pragma unroll
+                    for (int w = 0; w < NV; ++w) oth[w] = sW[(el * NFP + j) * NV + w];
                 }
             }
             const bool right = role == ROLE_RIGHT;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { UL[v] = right ? oth[v] : own[v]; UR[v] = right ? own[v] : oth[v]; }
+            for (int w = 0; w < NV; ++w) { UL[w] = right ? oth[w] : own[w]; UR[w] = right ? own[w] : oth[w]; }
             riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
-            const R sc = edge_scale(k, G[0], G[1], G[2], G[3]);
+            const R ax = kk == 0 ? sx : (kk == 1 ? rx + sx : rx);
+            const R ay = kk == 0 ? sy : (kk == 1 ? ry + sy : ry);
+            const R sc = right ? -sqrtR(ax * ax + ay * ay) : sqrtR(ax * ax + ay * ay);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) fn[it][v] = sc * (right ? -Fc[v] : Fc[v]) - fn[it][v];
+            for (int w = 0; w < NV; ++w) sW[(el * NFP + j) * NV + w] = sc * Fc[w];
         }
-    }
-    __syncthreads();  // every trace read; the flux differences take their place
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int t = tid + it * BLOCK;
-        if (t < S::NI && e0 + t / NFP < a.E)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sD[t * NV + v] = fn[it][v];
     }
     __syncthreads();
 
-    // ---- 4: divergence + lifted correction, RK stage update
+    // ---- 3: lifted flux differences, RK stage update
     double l2 = 0.0;
+    double un[NP];
     if (live) {
-        R acc[NV] = {R(0), R(0), R(0), R(0)};
-#pragma unroll 2
-        for (int m = 0; m < NP; ++m) {
-            const R dr = DrT[m * NP + i], ds = DsT[m * NP + i];
-            const R* f = sF + (el * NP + m) * 2 * NV;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = fmaR(ds, f[NV + v], fmaR(dr, f[v], acc[v]));
-        }
-#pragma unroll 2
         for (int j = 0; j < NFP; ++j) {
-            const R w = LT[j * NP + i];
-            const R* d = sD + (el * NFP + j) * NV;
+            // Delta = F^I s - Fn; Fn = -fn on edges 0 and 2, +fn on edge 1
+            const R Fi = sW[(el * NFP + j) * NV + v];
+            const R d = (j >= FP && j < 2 * FP) ? Fi - fn[j] : Fi + fn[j];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = fmaR(w, d[v], acc[v]);
+            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OL + i * NFP + j), d, div[i]);
         }
-        double res[NV];
+        const double dt = MODE == MODE_RESIDUAL ? 0.0 : *a.dt_ptr;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) res[v] = -double(acc[v]);
-        if (MODE == MODE_RESIDUAL) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { a.R_out[v * PS + gi] = res[v]; l2 += res[v] * res[v]; }
-        } else {
-            const double dt = *a.dt_ptr;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double un;
+        for (int i = 0; i < NP; ++i) {
+            const double res = -double(div[i]);
+            const int64_t gi = base + (int64_t)i * EB * NV;
+            if (MODE == MODE_RESIDUAL) {
+                a.R_out[gi] = res;
+                l2 += res * res;
+            } else {
                 if (MODE == MODE_STAGE1) {
-                    un = uin[v] + dt * res[v];
+                    un[i] = u[i] + dt * res;
                 } else {
-                    const double u0 = a.U0[v * PS + gi];
-                    un = MODE == MODE_STAGE2 ? 0.75 * u0 + 0.25 * (uin[v] + dt * res[v])
-                                             : (1.0 / 3.0) * u0 + (2.0 / 3.0) * (uin[v] + dt * res[v]);
-                    if (MODE == MODE_STAGE3) l2 += res[v] * res[v];
+                    const double u0 = a.U0[gi];
+                    un[i] = MODE == MODE_STAGE2 ? 0.75 * u0 + 0.25 * (u[i] + dt * res)
+                                                : (1.0 / 3.0) * u0 + (2.0 / 3.0) * (u[i] + dt * res);
+                    if (MODE == MODE_STAGE3) l2 += res * res;
                 }
-                a.U_out[v * PS + gi] = un;
-                sU[(el * NP + i) * NV + v] = R(un);
+                a.U_out[gi] = un[i];
             }
         }
     }
@@ -270,61 +302,89 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
         const double s = block_reduce<OpSum>(l2, sred);
         if (tid == 0) a.l2_partial[tile] = s;
     }
-    if (MODE == MODE_RESIDUAL) return;
-    __syncthreads();
+    if (MODE == MODE_RESIDUAL || !live) return;
 
-    // ---- 5: traces of the new stage state on edges another tile reads
+    // ---- 4: traces of the new stage state on edges another tile reads
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        if (!cross[it]) continue;
-        const int t = tid + it * BLOCK;
-        const int el2 = t / NFP, j = t - el2 * NFP;
-        R ut[NV] = {R(0), R(0), R(0), R(0)};
-#pragma unroll 2
-        for (int m = 0; m < NP; ++m) {
-            const R w = IT[m * NFP + j];
-            const R* u = sU + (el2 * NP + m) * NV;
+    for (int kk = 0; kk < 3; ++kk) {
+        const int nl = L[kk].x - e0;
+        if (link_role(L[kk].y) == ROLE_BOUNDARY || (nl >= 0 && nl < TILE)) continue;
+#pragma unroll 1
+        for (int j = kk * FP; j < kk * FP + FP; ++j) {
+            R t = R(0);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) ut[v] = fmaR(w, u[v], ut[v]);
+            for (int m = 0; m < NP; ++m) t = fmaR(cop<R>(OI + j * NP + m), R(un[m]), t);
+            a.T_out[trix(e, j, NFP) + v] = t;
         }
-        store4(a.T_out + ((int64_t)(e0 + el2) * NFP + j) * NV, ut);
     }
 }
 
 // Traces of U on every edge (after a new state is set).
 template <typename R, int P>
-__global__ __launch_bounds__(BLOCK) void tri_trace_kernel(int E, const double* U, const R* ops, R* T_out,
-                                                          int xcd_groups) {
+__global__ __launch_bounds__(BLOCK) void tri_trace_kernel(int E, const double* U, R* T_out) {
     using S = TriShape<P>;
-    constexpr int NP = S::NP, NFP = S::NFP, EPB = S::EPB;
-    __shared__ R sI[NP * NFP];
-    __shared__ R sU[EPB * NP * NV];
-    const int tid = threadIdx.x;
-    const int e0 = tile_of_block(xcd_groups) * EPB;
-    for (int k = tid; k < NP * NFP; k += BLOCK) sI[k] = ops[2 * NP * NP + k];
-    const int64_t PS = (int64_t)E * NP;
-    const int el = tid / NP, i = tid - el * NP;
-    if (tid < S::NT && e0 + el < E)
+    constexpr int NP = S::NP, NFP = S::NFP, OI = ops_offset(P) + 2 * NP * NP;
+    const int e = blockIdx.x * TILE + (threadIdx.x >> 2), v = threadIdx.x & 3;
+    if (e >= E) return;
+    R u[NP];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sU[(el * NP + i) * NV + v] = R(U[v * PS + (int64_t)(e0 + el) * NP + i]);
-    __syncthreads();
-    for (int t = tid; t < S::NI; t += BLOCK) {
-        const int el2 = t / NFP, j = t - el2 * NFP;
-        if (e0 + el2 >= E) continue;
-        R ut[NV] = {R(0), R(0), R(0), R(0)};
+    for (int m = 0; m < NP; ++m) u[m] = R(U[blk(e, m, NP) + v]);
+#pragma unroll 1
+    for (int j = 0; j < NFP; ++j) {
+        R t = R(0);
 #pragma unroll
-        for (int m = 0; m < NP; ++m) {
-            const R w = sI[m * NFP + j];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ut[v] = fmaR(w, sU[(el2 * NP + m) * NV + v], ut[v]);
-        }
-        store4(T_out + ((int64_t)(e0 + el2) * NFP + j) * NV, ut);
+        for (int m = 0; m < NP; ++m) t = fmaR(cop<R>(OI + j * NP + m), u[m], t);
+        T_out[trix(e, j, NFP) + v] = t;
     }
+}
+
+// Caller AoS [E][Np][4] (caller numbering) <-> blocked device layout.
+__global__ void tri_to_blk_kernel(const double* __restrict__ aos, double* __restrict__ out, int E, int NP,
+                                  const int* __restrict__ perm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)E * NP) return;
+    const int e = (int)(t / NP), i = (int)(t - (int64_t)e * NP);
+    const int src = perm ? perm[e] : e;
+    double q[NV];
+    load4(aos + ((int64_t)src * NP + i) * NV, q);
+    store4(out + blk(e, i, NP), q);
+}
+
+__global__ void tri_from_blk_kernel(const double* __restrict__ in, double* __restrict__ aos, int E, int NP,
+                                    const int* __restrict__ perm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)E * NP) return;
+    const int e = (int)(t / NP), i = (int)(t - (int64_t)e * NP);
+    const int dst = perm ? perm[e] : e;
+    double q[NV];
+    load4(in + blk(e, i, NP), q);
+    store4(aos + ((int64_t)dst * NP + i) * NV, q);
+}
+
+// CFL step per point of the blocked state, minimum per workgroup.
+__global__ __launch_bounds__(BLOCK) void tri_dt_kernel(const double* __restrict__ U, const double* __restrict__ hcell,
+                                                       int E, int NP, double gamma, double cfl, double cf,
+                                                       double* __restrict__ partial) {
+    __shared__ double sRed[BLOCK / 64];
+    const int64_t n = (int64_t)E * NP;
+    double mn = 1e20;
+    for (int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x; t < n; t += (int64_t)gridDim.x * BLOCK) {
+        const int e = (int)(t / NP), i = (int)(t - (int64_t)e * NP);
+        double q[NV];
+        load4(U + blk(e, i, NP), q);
+        const double u = q[1] / q[0], w = q[2] / q[0];
+        const double p = (gamma - 1.0) * q[0] * (q[3] / q[0] - 0.5 * (u * u + w * w));
+        const double ws = sqrt(u * u + w * w) + sqrt(gamma * fmax(p, 1e-10) / fmax(q[0], 1e-10));
+        const double d = cfl * hcell[e] / (ws * cf);
+        mn = d < mn ? d : mn;
+    }
+    const double r = block_reduce<OpMin>(mn, sRed);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
 template <typename R, int P>
 hipError_t tri_launch_p(int mode, const TriArgs<R>& a, hipStream_t s) {
-    const int grid = (std::max(a.E, 1) + TriShape<P>::EPB - 1) / TriShape<P>::EPB;
+    const int grid = (std::max(a.E, 1) + TILE - 1) / TILE;
     switch (mode) {
     case MODE_RESIDUAL: tri_stage_kernel<R, P, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a); break;
     case MODE_STAGE1: tri_stage_kernel<R, P, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a); break;
@@ -345,22 +405,55 @@ hipError_t tri_launch(int P, int mode, const TriArgs<R>& a, hipStream_t s) {
     }
 }
 
-template <typename R, int P>
-hipError_t tri_trace_p(int E, const double* U, const R* ops, R* T, int groups, hipStream_t s) {
-    const int grid = (std::max(E, 1) + TriShape<P>::EPB - 1) / TriShape<P>::EPB;
-    tri_trace_kernel<R, P><<<grid, BLOCK, 0, s>>>(E, U, ops, T, groups);
+template <typename R>
+hipError_t tri_trace(int P, int E, const double* U, R* T, hipStream_t s) {
+    const int grid = (std::max(E, 1) + TILE - 1) / TILE;
+    switch (P) {
+    case 1: tri_trace_kernel<R, 1><<<grid, BLOCK, 0, s>>>(E, U, T); break;
+    case 2: tri_trace_kernel<R, 2><<<grid, BLOCK, 0, s>>>(E, U, T); break;
+    case 3: tri_trace_kernel<R, 3><<<grid, BLOCK, 0, s>>>(E, U, T); break;
+    case 4: tri_trace_kernel<R, 4><<<grid, BLOCK, 0, s>>>(E, U, T); break;
+    default: tri_trace_kernel<R, 5><<<grid, BLOCK, 0, s>>>(E, U, T); break;
+    }
     return hipGetLastError();
 }
 
-template <typename R>
-hipError_t tri_trace(int P, int E, const double* U, const R* ops, R* T, int groups, hipStream_t s) {
-    switch (P) {
-    case 1: return tri_trace_p<R, 1>(E, U, ops, T, groups, s);
-    case 2: return tri_trace_p<R, 2>(E, U, ops, T, groups, s);
-    case 3: return tri_trace_p<R, 3>(E, U, ops, T, groups, s);
-    case 4: return tri_trace_p<R, 4>(E, U, ops, T, groups, s);
-    default: return tri_trace_p<R, 5>(E, U, ops, T, groups, s);
+hipError_t tri_convert(bool to_blk, const double* src, double* dst, int E, int NP, const int* perm, hipStream_t s) {
+    const int64_t n = (int64_t)E * NP;
+    const int grid = (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK);
+    if (to_blk)
+        tri_to_blk_kernel<<<grid, BLOCK, 0, s>>>(src, dst, E, NP, perm);
+    else
+        tri_from_blk_kernel<<<grid, BLOCK, 0, s>>>(src, dst, E, NP, perm);
+    return hipGetLastError();
+}
+
+// Operator tables of every order into this device's constant memory, once.
+std::mutex g_ops_mutex;
+uint64_t g_ops_loaded = 0;  // bit d: device d holds the tables
+
+hipError_t load_tri_ops(int device) {
+    std::lock_guard<std::mutex> lock(g_ops_mutex);
+    if (device >= 0 && device < 64 && (g_ops_loaded >> device) & 1) return hipSuccess;
+    std::vector<double> all((size_t)OPS_TOTAL);
+    for (int P = 1; P <= 5; ++P) {
+        tri::Operators O;
+        if (!tri::operators(P, O)) return hipErrorInvalidValue;
+        double* d = all.data() + ops_offset(P);
+        const int np = O.np, nf = O.nf;
+        std::copy(O.Dr.begin(), O.Dr.end(), d);
+        std::copy(O.Ds.begin(), O.Ds.end(), d + np * np);
+        std::copy(O.interp.begin(), O.interp.end(), d + 2 * np * np);
+        std::copy(O.lift.begin(), O.lift.end(), d + 2 * np * np + nf * np);
     }
+    std::vector<float> allf(all.begin(), all.end());
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops64), all.data(), all.size() * sizeof(double), 0,
+                                     hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops32), allf.data(), allf.size() * sizeof(float), 0,
+                              hipMemcpyHostToDevice);
+    if (e == hipSuccess && device >= 0 && device < 64) g_ops_loaded |= 1ull << device;
+    return e;
 }
 
 }  // namespace
@@ -383,7 +476,6 @@ struct zj_tri {
     void* geo = nullptr;
     int2* links = nullptr;
     void* T[3] = {nullptr, nullptr, nullptr};
-    void* ops = nullptr;
     double* bc_data = nullptr;
     double* hcell = nullptr;
     double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl
@@ -432,7 +524,7 @@ static int tupload_as(zj_tri* h, void** dst, const std::vector<double>& v) {
 static void trelease(zj_tri* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
-    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->geo, h->links, h->T[0], h->T[1], h->T[2], h->ops,
+    void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->geo, h->links, h->T[0], h->T[1], h->T[2],
                     h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->perm};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -463,7 +555,7 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
 
     h->P = P; h->NP = NP; h->NFP = NFP; h->E = m->n_elem; h->F = m->n_faces;
     h->prec = p->precision; h->gamma = p->gamma; h->riemann = p->riemann; h->device = p->device;
-    h->epb = tri_epb(P);
+    h->epb = TILE;
     h->nblk = (h->E + h->epb - 1) / h->epb;
     const int E = h->E;
 
@@ -540,45 +632,25 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
     for (size_t k = 0; k < links.size(); ++k)
         if (links[k].y == -1) return tfail(h, ZJ_ERR_ARG, "an element edge has no face");
 
-    tri::Operators O;
-    if (!tri::operators(P, O)) return tfail(h, ZJ_ERR_UNSUPPORTED, "triangle operators failed");
-    std::vector<double> ops((size_t)(2 * NP * NP + 2 * NFP * NP));
-    {
-        double* DrT = ops.data();
-        double* DsT = DrT + NP * NP;
-        double* IT = DsT + NP * NP;
-        double* LT = IT + NP * NFP;
-        for (int i = 0; i < NP; ++i)
-            for (int j = 0; j < NP; ++j) {
-                DrT[j * NP + i] = O.Dr[(size_t)i * NP + j];
-                DsT[j * NP + i] = O.Ds[(size_t)i * NP + j];
-            }
-        for (int j = 0; j < NFP; ++j)
-            for (int i = 0; i < NP; ++i) {
-                IT[i * NFP + j] = O.interp[(size_t)j * NP + i];
-                LT[j * NP + i] = O.lift[(size_t)i * NFP + j];
-            }
-    }
-
     ZJ_THIP(h, hipSetDevice(h->device));
+    ZJ_THIP(h, load_tri_ops(h->device));
     ZJ_THIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int nxcc = 1;
     if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, h->device) != hipSuccess || nxcc < 1) nxcc = 1;
     h->xcd_groups = nxcc;
     const size_t npts = (size_t)E * NP;
+    const size_t eb = ((size_t)E + EB - 1) / EB * EB;  // elements rounded up to whole layout blocks
     const size_t rsz = h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float);
-    ZJ_THIP(h, hipMalloc((void**)&h->U, npts * 4 * sizeof(double)));
-    ZJ_THIP(h, hipMalloc((void**)&h->W, npts * 4 * sizeof(double)));
-    ZJ_THIP(h, hipMalloc((void**)&h->Rres, npts * 4 * sizeof(double)));
+    ZJ_THIP(h, hipMalloc((void**)&h->U, eb * NP * 4 * sizeof(double)));
+    ZJ_THIP(h, hipMalloc((void**)&h->W, eb * NP * 4 * sizeof(double)));
+    ZJ_THIP(h, hipMalloc((void**)&h->Rres, eb * NP * 4 * sizeof(double)));
     ZJ_THIP(h, hipMalloc((void**)&h->aos, npts * 4 * sizeof(double)));
-    for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->T[k], (size_t)E * NFP * 4 * rsz));
+    for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->T[k], eb * NFP * 4 * rsz));
     int rc;
     if (h->prec == ZJ_PREC_FP64) {
         if ((rc = tupload_as<double>(h, &h->geo, geo))) return rc;
-        if ((rc = tupload_as<double>(h, &h->ops, ops))) return rc;
     } else {
         if ((rc = tupload_as<float>(h, &h->geo, geo))) return rc;
-        if ((rc = tupload_as<float>(h, &h->ops, ops))) return rc;
     }
     if ((rc = tupload(h, (void**)&h->links, links))) return rc;
     std::vector<double> bcd;
@@ -602,7 +674,6 @@ static TriArgs<R> tri_args(zj_tri* h, int mode) {
     a.E = h->E;
     a.geo = (const R*)h->geo;
     a.links = h->links;
-    a.ops = (const R*)h->ops;
     a.bc_data = h->bc_data;
     a.dt_ptr = h->d_scalars;
     a.gamma = h->gamma;
@@ -634,9 +705,8 @@ static hipError_t tenqueue(zj_tri* h, int mode, hipStream_t s) {
 }
 
 static int trefresh(zj_tri* h, hipStream_t s) {
-    hipError_t e = h->prec == ZJ_PREC_FP64
-                       ? tri_trace<double>(h->P, h->E, h->U, (const double*)h->ops, (double*)h->T[0], h->xcd_groups, s)
-                       : tri_trace<float>(h->P, h->E, h->U, (const float*)h->ops, (float*)h->T[0], h->xcd_groups, s);
+    hipError_t e = h->prec == ZJ_PREC_FP64 ? tri_trace<double>(h->P, h->E, h->U, (double*)h->T[0], s)
+                                           : tri_trace<float>(h->P, h->E, h->U, (float*)h->T[0], s);
     ZJ_THIP(h, e);
     return ZJ_OK;
 }
@@ -683,7 +753,7 @@ extern "C" int zj_tri_set_state_device(zj_tri_handle h, const double* d_aos, voi
     if (!h || !d_aos) return ZJ_ERR_ARG;
     ZJ_THIP(h, hipSetDevice(h->device));
     hipStream_t s = tpick(h, stream);
-    ZJ_THIP(h, launch_aos_to_soa(d_aos, h->U, (int64_t)h->E * h->NP, s, h->perm, h->NP));
+    ZJ_THIP(h, tri_convert(true, d_aos, h->U, h->E, h->NP, h->perm, s));
     const int rc = trefresh(h, s);
     if (rc) return rc;
     h->has_state = true;
@@ -705,7 +775,7 @@ extern "C" int zj_tri_get_state_device(zj_tri_handle h, double* d_aos, void* str
     if (!h || !d_aos) return ZJ_ERR_ARG;
     if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
     ZJ_THIP(h, hipSetDevice(h->device));
-    ZJ_THIP(h, launch_soa_to_aos(h->U, d_aos, (int64_t)h->E * h->NP, tpick(h, stream), h->perm, h->NP));
+    ZJ_THIP(h, tri_convert(false, h->U, d_aos, h->E, h->NP, h->perm, tpick(h, stream)));
     return ZJ_OK;
 }
 
@@ -726,7 +796,7 @@ extern "C" int zj_tri_residual(zj_tri_handle h, double* dUdt_aos, void* stream) 
     hipStream_t s = tpick(h, stream);
     ZJ_THIP(h, tenqueue(h, MODE_RESIDUAL, s));
     if (dUdt_aos) {
-        ZJ_THIP(h, launch_soa_to_aos(h->Rres, h->aos, (int64_t)h->E * h->NP, s, h->perm, h->NP));
+        ZJ_THIP(h, tri_convert(false, h->Rres, h->aos, h->E, h->NP, h->perm, s));
         ZJ_THIP(h, hipMemcpyAsync(dUdt_aos, h->aos, (size_t)h->E * h->NP * 4 * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
         ZJ_THIP(h, hipStreamSynchronize(s));
@@ -740,7 +810,7 @@ extern "C" int zj_tri_residual_device(zj_tri_handle h, double* d_dUdt_aos, void*
     ZJ_THIP(h, hipSetDevice(h->device));
     hipStream_t s = tpick(h, stream);
     ZJ_THIP(h, tenqueue(h, MODE_RESIDUAL, s));
-    ZJ_THIP(h, launch_soa_to_aos(h->Rres, d_dUdt_aos, (int64_t)h->E * h->NP, s, h->perm, h->NP));
+    ZJ_THIP(h, tri_convert(false, h->Rres, d_dUdt_aos, h->E, h->NP, h->perm, s));
     return ZJ_OK;
 }
 
@@ -774,8 +844,9 @@ extern "C" int zj_tri_dt_cfl(zj_tri_handle h, double cfl, double* dt) {
     if (!h || !dt) return ZJ_ERR_ARG;
     if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
     ZJ_THIP(h, hipSetDevice(h->device));
-    ZJ_THIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->NP, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial_dt,
-                                 h->nred, h->stream));
+    tri_dt_kernel<<<h->nred, BLOCK, 0, h->stream>>>(h->U, h->hcell, h->E, h->NP, h->gamma, cfl, 2.0 * h->P + 1.0,
+                                                    h->partial_dt);
+    ZJ_THIP(h, hipGetLastError());
     ZJ_THIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, h->stream));
     ZJ_THIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ZJ_THIP(h, hipStreamSynchronize(h->stream));
