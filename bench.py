@@ -354,6 +354,8 @@ def main():
                     help="members of the batched-step measurement (0: skip)")
     ap.add_argument("--tri-n", type=int, default=500,
                     help="cells per side of the triangle leg (each split in two; 0: skip)")
+    ap.add_argument("--tri-only", action="store_true",
+                    help="run only the triangle leg and print its JSON (profiling runs)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint/forward cost ratio (TGV 128^2 P3)")
     ap.add_argument("--only", choices=("fp64", "fp32bulk"), default=None,
@@ -369,6 +371,12 @@ def main():
         sys.exit(2)
 
     from zhijian_amd import _lib, cases, hiprt, solver
+
+    if args.tri_only:
+        tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.err_steps, hiprt.Stream(), hiprt,
+                       0.0 if args.no_cpu else 5.0)
+        print(json.dumps({"triangles": tri}), flush=True)
+        return
 
     mesh = cases.tgv_mesh(args.n)
     t_build = time.perf_counter()

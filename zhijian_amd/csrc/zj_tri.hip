@@ -63,7 +63,6 @@ struct TriShape {
     static constexpr int NP = (P + 1) * (P + 2) / 2;        // solution points
     static constexpr int FP = P + 1;                        // flux points per edge
     static constexpr int NFP = 3 * FP;
-    static constexpr int MINE = (NFP + NV - 1) / NV;        // flux points per lane in the Riemann phase
 };
 
 constexpr int TILE = BLOCK / NV;  // elements per workgroup (one lane per element and variable)
@@ -137,8 +136,8 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
     constexpr int NW = NP > NFP ? NP : NFP;
     constexpr int ODR = ops_offset(P), ODS = ODR + NP * NP, OI = ODS + NP * NP, OL = OI + NFP * NP;
     __shared__ R sT[TILE * NFP * NV];  // own traces
-    __shared__ R sW[TILE * NW * NV];   // the stage input state; neighbour traces from other tiles,
-                                       // then the scaled common fluxes
+    __shared__ R sW[TILE * NW * NV];   // the stage input state, then the transformed normal fluxes,
+                                       // then the flux differences
     __shared__ double sred[BLOCK / 64];
 
     const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
@@ -149,16 +148,13 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
     const int64_t base = blk(e, 0, NP) + v;  // this lane's variable at point 0; point i at + i*EB*NV
 
     double u[NP];
-    R div[NP], fn[NFP];
+    R div[NP];
     R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
     int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
 
     // ---- 0: this lane's variable at every solution point (one coalesced
-    // 512-byte row per point and wave), kept for the stage update; the
-    // neighbour traces this lane's Riemann solves will need from other tiles
-    // are requested now so that their latency hides behind phase 1
+    // 512-byte row per point and wave), kept for the stage update
     R nrm[6];
-    R pre[S::MINE][NV];
     if (live) {
 #pragma unroll
         for (int m = 0; m < NP; ++m) u[m] = a.U_in[base + (int64_t)m * EB * NV];
@@ -169,15 +165,6 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
 #pragma unroll
-        for (int k = 0; k < S::MINE; ++k) {
-            const int j = v + NV * k;
-            const int kk = j / FP, f = j - kk * FP;
-            const int2 Lk = kk == 0 ? L[0] : (kk == 1 ? L[1] : L[2]);
-            const int nl = Lk.x - e0;
-            if (j < NFP && link_role(Lk.y) != ROLE_BOUNDARY && (nl < 0 || nl >= TILE))
-                load4(a.T_in + trix(Lk.x, link_edge(Lk.y) * FP + (P - f), NFP), pre[k]);
-        }
-#pragma unroll
         for (int m = 0; m < NP; ++m) sW[(el * NP + m) * NV + v] = R(u[m]);
     }
     __syncthreads();
@@ -185,12 +172,12 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
     // ---- 1: fluxes, divergence, traces and interpolated normal fluxes, as
     // sums over the solution points m of outer products with wave-uniform
     // operator columns
+    R fn[NFP];
     if (live) {
-        R ut[NFP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) div[i] = R(0);
 #pragma unroll
-        for (int j = 0; j < NFP; ++j) { ut[j] = R(0); fn[j] = R(0); }
+        for (int j = 0; j < NFP; ++j) fn[j] = R(0);
 #pragma unroll 1
         for (int m = 0; m < NP; ++m) {
             const R um = sW[(el * NP + m) * NV + v];
@@ -205,25 +192,28 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
             for (int i = 0; i < NP; ++i)
                 div[i] = fmaR(cop<R>(ODS + i * NP + m), fs, fmaR(cop<R>(ODR + i * NP + m), fr, div[i]));
 #pragma unroll
-            for (int j = 0; j < NFP; ++j) {
-                const R w = cop<R>(OI + j * NP + m);
-                ut[j] = fmaR(w, um, ut[j]);
-                fn[j] = fmaR(w, j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
-            }
+            for (int j = 0; j < NFP; ++j)
+                fn[j] = fmaR(cop<R>(OI + j * NP + m), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+        }
+        // traces in a second pass, so that they and the fluxes' sums are not live together
+        R ut[NFP];
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) ut[j] = R(0);
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R um = sW[(el * NP + m) * NV + v];
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) ut[j] = fmaR(cop<R>(OI + j * NP + m), um, ut[j]);
         }
 #pragma unroll
         for (int j = 0; j < NFP; ++j) sT[(el * NFP + j) * NV + v] = ut[j];
     }
     __syncthreads();  // traces in place, state reads done
-    // the prefetched neighbour traces move to their flux points' slots in sW
+    // transformed normal fluxes to their flux points' slots in sW:
+    // -Fs on edge 0, Fr + Fs on edge 1, -Fr on edge 2
     if (live) {
 #pragma unroll
-        for (int k = 0; k < S::MINE; ++k) {
-            const int j = v + NV * k;
-            if (j < NFP)
-#pragma unroll
-                for (int w = 0; w < NV; ++w) sW[(el * NFP + j) * NV + w] = pre[k][w];
-        }
+        for (int j = 0; j < NFP; ++j) sW[(el * NFP + j) * NV + v] = (j >= FP && j < 2 * FP) ? fn[j] : -fn[j];
     }
     __syncthreads();
 
@@ -248,8 +238,7 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
 #pragma unroll
                     for (int w = 0; w < NV; ++w) oth[w] = sT[(nl * NFP + oj) * NV + w];
                 } else {
-#pragma unroll
-                    for (int w = 0; w < NV; ++w) oth[w] = sW[(el * NFP + j) * NV + w];
+                    load4(a.T_in + trix(Lk.x, oj, NFP), oth);
                 }
             }
             const bool right = role == ROLE_RIGHT;
@@ -259,8 +248,9 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
             const R ax = kk == 0 ? sx : (kk == 1 ? rx + sx : rx);
             const R ay = kk == 0 ? sy : (kk == 1 ? ry + sy : ry);
             const R sc = right ? -sqrtR(ax * ax + ay * ay) : sqrtR(ax * ax + ay * ay);
+            R* d = sW + (el * NFP + j) * NV;  // flux difference F^I s - Fn in place
 #pragma unroll
-            for (int w = 0; w < NV; ++w) sW[(el * NFP + j) * NV + w] = sc * Fc[w];
+            for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
         }
     }
     __syncthreads();
@@ -269,11 +259,9 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
     double l2 = 0.0;
     double un[NP];
     if (live) {
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < NFP; ++j) {
-            // Delta = F^I s - Fn; Fn = -fn on edges 0 and 2, +fn on edge 1
-            const R Fi = sW[(el * NFP + j) * NV + v];
-            const R d = (j >= FP && j < 2 * FP) ? Fi - fn[j] : Fi + fn[j];
+            const R d = sW[(el * NFP + j) * NV + v];
 #pragma unroll
             for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OL + i * NFP + j), d, div[i]);
         }
