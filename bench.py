@@ -373,6 +373,8 @@ def main():
     from zhijian_amd import _lib, cases, hiprt, solver
 
     if args.tri_only:
+        if args.probe_bytes > 0:
+            log(f"copy kernel: {_lib.copy_probe(int(args.probe_bytes), 10):.0f} GB/s")
         tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.err_steps, hiprt.Stream(), hiprt,
                        0.0 if args.no_cpu else 5.0)
         print(json.dumps({"triangles": tri}), flush=True)
