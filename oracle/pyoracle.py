@@ -87,6 +87,9 @@ def lib() -> C.CDLL:
         L.oz_tb_destroy.argtypes = [C.c_void_p]
         L.oz_tb_residual.argtypes = [C.c_void_p, _dp, _dp]
         L.oz_tb_advance.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int]
+        L.oz_tb_set_viscous.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.oz_tb_gradient.argtypes = [C.c_void_p, _dp, _dp]
+        L.oz_visc_flux.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]
         for name in ("oz_buf_fx", "oz_buf_fy", "oz_buf_ufp", "oz_buf_fface", "oz_buf_fdiff",
                      "oz_buf_dudt", "oz_buf_ufp_mut", "oz_buf_fdiff_mut"):
             getattr(L, name).argtypes = [C.c_void_p]
@@ -434,6 +437,25 @@ class TextbookSolver:
         U = np.ascontiguousarray(U, np.float64).reshape(-1).copy()
         lib().oz_tb_advance(self._h, U, dt, int(positivity))
         return U
+
+    def set_viscous(self, Re, Pr=0.72, eta=1.0, gamma=1.4):
+        """BR2 viscous terms with mu = 1/Re, kappa = mu gamma / ((gamma-1) Pr)."""
+        mu = 1.0 / Re
+        lib().oz_tb_set_viscous(self._h, mu, mu * gamma / ((gamma - 1.0) * Pr), eta)
+        return self
+
+    def gradient(self, U):
+        """Corrected gradient [E*S][8] (d/dx of the 4 variables, then d/dy)."""
+        U = np.ascontiguousarray(U, np.float64).reshape(-1)
+        out = np.zeros(2 * self.n)
+        lib().oz_tb_gradient(self._h, U, out)
+        return out.reshape(-1, 8)
+
+
+def visc_flux(U, Qx, Qy, mu, kappa, gamma=1.4):
+    Fv, Gv = np.zeros(4), np.zeros(4)
+    lib().oz_visc_flux(*(np.ascontiguousarray(a, np.float64) for a in (U, Qx, Qy)), gamma, mu, kappa, Fv, Gv)
+    return Fv, Gv
 
 
 def riemann_batch(kind, UL, UR, nrm, gamma=1.4):

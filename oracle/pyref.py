@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
         L.ref_max_wave_speed.argtypes = [_dp, C.c_double]
         L.ref_max_wave_speed.restype = C.c_double
         L.ref_riemann.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
+        L.ref_visc_flux.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -129,3 +130,11 @@ def riemann(which, UL, UR, nx, ny, gamma=1.4):
     lib().ref_riemann(which, np.ascontiguousarray(UL, np.float64),
                       np.ascontiguousarray(UR, np.float64), nx, ny, gamma, F)
     return F
+
+
+def visc_flux(U, dUdx, dUdy, gamma=1.4, Re=1000.0, Pr=0.72):
+    """ViscousFlux::fluxX/fluxY and the (mu, k) they used."""
+    Fx, Fy, mk = np.zeros(4), np.zeros(4), np.zeros(2)
+    lib().ref_visc_flux(*(np.ascontiguousarray(a, np.float64) for a in (U, dUdx, dUdy)), gamma, Re, Pr,
+                        Fx, Fy, mk)
+    return Fx, Fy, mk

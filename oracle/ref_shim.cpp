@@ -9,6 +9,7 @@
 #include "basis/basis.hpp"
 #include "mesh/mesh.hpp"
 #include "physics/euler.hpp"
+#include "physics/navierstokes.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -178,6 +179,26 @@ void ref_riemann(int which, const double* UL, const double* UR, double nx, doubl
     else if (which == 1) out = RiemannSolvers::roe(l, r, nx, ny, gamma);
     else out = RiemannSolvers::hllc(l, r, nx, ny, gamma);
     for (int v = 0; v < N_VARS; ++v) F[v] = out[v];
+}
+
+// ViscousFlux at (U, dU/dx, dU/dy): velocityGradients, temperatureGradient,
+// fluxX, fluxY, and the viscosity and conductivity the two fluxes used.
+void ref_visc_flux(const double* U, const double* dUdx, const double* dUdy, double gamma, double Re,
+                   double Pr, double* Fx, double* Fy, double* mu_k) {
+    ViscousFlux vf(gamma, Re, Pr);
+    IdealGas gas(gamma);
+    State s, gx, gy;
+    for (int v = 0; v < N_VARS; ++v) { s[v] = U[v]; gx[v] = dUdx[v]; gy[v] = dUdy[v]; }
+    Real dudx, dudy, dvdx, dvdy, dTdx, dTdy;
+    ViscousFlux::velocityGradients(s, gx, gy, dudx, dudy, dvdx, dvdy);
+    vf.temperatureGradient(s, gx, gy, dTdx, dTdy);
+    State fx = vf.fluxX(s, dudx, dudy, dvdx, dvdy, dTdx, dTdy);
+    State fy = vf.fluxY(s, dudx, dudy, dvdx, dvdy, dTdx, dTdy);
+    for (int v = 0; v < N_VARS; ++v) { Fx[v] = fx[v]; Fy[v] = fy[v]; }
+    const Real T = gas.pressure(s) / s.rho();
+    const Real mu = vf.dynamicViscosity(T) / Re;
+    mu_k[0] = mu;
+    mu_k[1] = vf.thermalConductivity(mu) / Re;
 }
 
 }  // extern "C"

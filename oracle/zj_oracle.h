@@ -210,6 +210,15 @@ oz_tb* oz_tb_create(const oz_desc* d, const double* edge_half_len, int riemann, 
 void oz_tb_destroy(oz_tb* t);
 void oz_tb_residual(oz_tb* t, const double* U, double* dUdt);
 void oz_tb_advance(oz_tb* t, double* U, double dt, int positivity);
+/* Viscous terms (BR2) of the textbook residual: constant mu, kappa, BR2
+ * penalty eta (affine quadrilaterals; boundary faces carry the interior
+ * viscous flux).  oz_visc_flux: the NS viscous flux pair at (U, dU/dx, dU/dy).
+ * oz_tb_gradient: the corrected gradient [E*S][8] (d/dx of the 4 variables,
+ * then d/dy). */
+void oz_visc_flux(const double* U, const double* Qx, const double* Qy, double gamma, double mu,
+                  double kappa, double* Fv, double* Gv);
+void oz_tb_set_viscous(oz_tb* t, double mu, double kappa, double eta);
+void oz_tb_gradient(oz_tb* t, const double* U, double* Q);
 
 #ifdef __cplusplus
 }

@@ -214,6 +214,10 @@ struct oz_tb {
     int *le, *ll, *re, *rl, *bct, *bcmap;
     int nbcd;
     double *Ft, *Gt, *Ufp, *Fout, *U0, *U1, *R;
+    /* viscous terms (BR2), off unless oz_tb_set_viscous was called */
+    int viscous;
+    double mu, kappa, eta, gLe, gRe;   /* gLe = gL'(-1), gRe = gR'(+1) */
+    double *Gx, *Gy, *Gfp, *jump;      /* [E*S][4] x2, [E][4][n1][8], [E][4][n1][4] */
 };
 
 static double* cp_d(const double* s, size_t n) {
@@ -254,13 +258,20 @@ oz_tb* oz_tb_create(const oz_desc* d, const double* edge_half_len, int riemann, 
     t->Ft = cp_d(NULL, sol); t->Gt = cp_d(NULL, sol); t->U0 = cp_d(NULL, sol);
     t->U1 = cp_d(NULL, sol); t->R = cp_d(NULL, sol);
     t->Ufp = cp_d(NULL, fp); t->Fout = cp_d(NULL, fp);
+    t->Gx = cp_d(NULL, sol); t->Gy = cp_d(NULL, sol); t->Gfp = cp_d(NULL, 2 * fp); t->jump = cp_d(NULL, fp);
+    {
+        const double xe[2] = {-1.0, 1.0};
+        double gl[2], gr[2];
+        oz_vcjh_deriv(d->p, corr, xe, 2, gl, gr);
+        t->gLe = gl[0]; t->gRe = gr[1];
+    }
     return t;
 }
 
 void oz_tb_destroy(oz_tb* t) {
     if (!t) return;
     void* ps[] = {t->J, t->Jinv, t->normals, t->ehl, t->bcd, t->le, t->ll, t->re, t->rl, t->bct,
-                  t->bcmap, t->Ft, t->Gt, t->Ufp, t->Fout, t->U0, t->U1, t->R};
+                  t->bcmap, t->Ft, t->Gt, t->Ufp, t->Fout, t->U0, t->U1, t->R, t->Gx, t->Gy, t->Gfp, t->jump};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i) free(ps[i]);
     free(t);
 }
@@ -282,15 +293,178 @@ static double edge_value(const oz_tb* t, const double* u, int stride, int k, int
     return s;
 }
 
+
+/* ---------------------------------------------------- viscous terms (BR2)
+ *
+ * Compressible Navier-Stokes viscous flux from the conservative state U and
+ * its gradient (Qx, Qy), for a given dynamic viscosity mu and conductivity
+ * kappa, in the steps of the reference's ViscousFlux (navierstokes.hpp:
+ * velocityGradients 122-136, temperatureGradient 140-164, stressTensor 38-48,
+ * fluxX/fluxY 53-99).  The reference evaluates mu by Sutherland's law at
+ * T_ref = 1 and divides kappa by Re twice; the textbook terms take mu = 1/Re
+ * and kappa = mu gamma / ((gamma - 1) Pr) (R = 1), so both are parameters here.
+ */
+void oz_visc_flux(const double* U, const double* Qx, const double* Qy, double g, double mu, double kappa,
+                  double* Fv, double* Gv) {
+    const double rho = U[0], u = U[1] / rho, v = U[2] / rho;
+    const double dudx = (Qx[1] - u * Qx[0]) / rho, dudy = (Qy[1] - u * Qy[0]) / rho;
+    const double dvdx = (Qx[2] - v * Qx[0]) / rho, dvdy = (Qy[2] - v * Qy[0]) / rho;
+    const double p = (g - 1.0) * rho * (U[3] / rho - 0.5 * (u * u + v * v));
+    const double dpdx = (g - 1.0) * (Qx[3] - 0.5 * Qx[0] * (u * u + v * v) - rho * (u * dudx + v * dvdx));
+    const double dpdy = (g - 1.0) * (Qy[3] - 0.5 * Qy[0] * (u * u + v * v) - rho * (u * dudy + v * dvdy));
+    const double T = p / rho;
+    const double dTdx = (dpdx - T * Qx[0]) / rho, dTdy = (dpdy - T * Qy[0]) / rho;
+    const double div = dudx + dvdy, lam = -2.0 / 3.0 * mu;
+    const double txx = 2.0 * mu * dudx + lam * div, tyy = 2.0 * mu * dvdy + lam * div, txy = mu * (dudy + dvdx);
+    Fv[0] = 0.0; Fv[1] = txx; Fv[2] = txy; Fv[3] = u * txx + v * txy + kappa * dTdx;
+    Gv[0] = 0.0; Gv[1] = txy; Gv[2] = tyy; Gv[3] = u * txy + v * tyy + kappa * dTdy;
+}
+
+void oz_tb_set_viscous(oz_tb* t, double mu, double kappa, double eta) {
+    t->viscous = 1; t->mu = mu; t->kappa = kappa; t->eta = eta;
+}
+
+/* Gradients of one element state (per point, uncorrected) and of every
+ * element's traces; the common solution U^C = (U_L + U_R)/2 on interior faces
+ * (boundary faces: U^C = the interior trace) and the jumps U^C - U_own. */
+static void visc_gradients(oz_tb* t, const double* U) {
+    const int n1 = t->n1, S = t->S, E = t->E, P = t->p;
+    for (int e = 0; e < E; ++e) {
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                const size_t gi = (size_t)e * S + j * n1 + i;
+                const double* Ji = t->Jinv + gi * 4;
+                for (int v = 0; v < NV; ++v) {
+                    double uxi = 0.0, ueta = 0.0;
+                    for (int m = 0; m < n1; ++m) {
+                        uxi += t->D[i * n1 + m] * U[((size_t)e * S + j * n1 + m) * NV + v];
+                        ueta += t->D[j * n1 + m] * U[((size_t)e * S + m * n1 + i) * NV + v];
+                    }
+                    t->Gx[gi * NV + v] = Ji[0] * uxi + Ji[2] * ueta;
+                    t->Gy[gi * NV + v] = Ji[1] * uxi + Ji[3] * ueta;
+                }
+            }
+        for (int k = 0; k < 4; ++k)
+            for (int f = 0; f < n1; ++f)
+                for (int v = 0; v < NV; ++v) {
+                    double* gf = t->Gfp + (((size_t)e * 4 + k) * n1 + f) * 8;
+                    gf[v] = edge_value(t, t->Gx + (size_t)e * S * NV + v, NV, k, f);
+                    gf[4 + v] = edge_value(t, t->Gy + (size_t)e * S * NV + v, NV, k, f);
+                }
+    }
+    for (int fc = 0; fc < t->F; ++fc) {
+        const int le = t->le[fc], ll = t->ll[fc], re = t->re[fc], rl = t->rl[fc];
+        for (int f = 0; f < n1; ++f) {
+            const size_t li = ((size_t)le * 4 + ll) * n1 + f;
+            for (int v = 0; v < NV; ++v) {
+                if (re >= 0) {
+                    const size_t ri = ((size_t)re * 4 + rl) * n1 + (P - f);
+                    const double uc = 0.5 * (t->Ufp[li * NV + v] + t->Ufp[ri * NV + v]);
+                    t->jump[li * NV + v] = uc - t->Ufp[li * NV + v];
+                    t->jump[ri * NV + v] = uc - t->Ufp[ri * NV + v];
+                } else {
+                    t->jump[li * NV + v] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+/* Corrected gradient at solution point (i, j) of element e (xi line j: edges
+ * 3 at slot p-j and 1 at slot j; eta line i: edges 0 at slot i and 2 at slot
+ * p-i), mapped by the point's metric. */
+static void visc_corrected(const oz_tb* t, const double* U, int e, int i, int j, double* Qx, double* Qy) {
+    const int n1 = t->n1, S = t->S, P = t->p;
+    const size_t gi = (size_t)e * S + j * n1 + i;
+    const double* Ji = t->Jinv + gi * 4;
+    const double* jp = t->jump + (size_t)e * 4 * n1 * NV;
+    for (int v = 0; v < NV; ++v) {
+        double uxi = 0.0, ueta = 0.0;
+        for (int m = 0; m < n1; ++m) {
+            uxi += t->D[i * n1 + m] * U[((size_t)e * S + j * n1 + m) * NV + v];
+            ueta += t->D[j * n1 + m] * U[((size_t)e * S + m * n1 + i) * NV + v];
+        }
+        uxi += t->gLd[i] * jp[((size_t)3 * n1 + (P - j)) * NV + v] + t->gRd[i] * jp[((size_t)1 * n1 + j) * NV + v];
+        ueta += t->gLd[j] * jp[((size_t)0 * n1 + i) * NV + v] + t->gRd[j] * jp[((size_t)2 * n1 + (P - i)) * NV + v];
+        Qx[v] = Ji[0] * uxi + Ji[2] * ueta;
+        Qy[v] = Ji[1] * uxi + Ji[3] * ueta;
+    }
+}
+
+void oz_tb_gradient(oz_tb* t, const double* U, double* Q) {
+    const int n1 = t->n1, S = t->S, E = t->E;
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < 4; ++k)
+            for (int f = 0; f < n1; ++f)
+                for (int v = 0; v < NV; ++v)
+                    t->Ufp[(((size_t)e * 4 + k) * n1 + f) * NV + v] =
+                        edge_value(t, U + (size_t)e * S * NV + v, NV, k, f);
+    visc_gradients(t, U);
+    for (int e = 0; e < E; ++e)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                const size_t gi = (size_t)e * S + j * n1 + i;
+                visc_corrected(t, U, e, i, j, Q + gi * 8, Q + gi * 8 + 4);
+            }
+}
+
+/* BR2 gradient on one side of a face point: the element's own gradient there
+ * plus eta times the lifting of this face's jump alone, which at the face's
+ * own points is g'(+-1) times the jump along the edge's reference direction. */
+static void br2_side(const oz_tb* t, int e, int k, int f, double* qx, double* qy) {
+    const int n1 = t->n1, S = t->S;
+    const double* gf = t->Gfp + (((size_t)e * 4 + k) * n1 + f) * 8;
+    const double* jp = t->jump + (((size_t)e * 4 + k) * n1 + f) * NV;
+    const double* Ji = t->Jinv + (size_t)e * S * 4;   /* affine: one metric per element */
+    const double gd = (k == 0 || k == 3) ? t->gLe : t->gRe;
+    const int along_xi = (k == 1 || k == 3);
+    for (int v = 0; v < NV; ++v) {
+        const double r = gd * jp[v];
+        const double rx = along_xi ? Ji[0] * r : Ji[2] * r;
+        const double ry = along_xi ? Ji[1] * r : Ji[3] * r;
+        qx[v] = gf[v] + t->eta * rx;
+        qy[v] = gf[4 + v] + t->eta * ry;
+    }
+}
+
+/* Common viscous normal flux of one face point: the viscous flux at the mean
+ * state and mean BR2 gradient (BR2Scheme::commonFlux, navierstokes.hpp:177-204,
+ * with the BR2 lifting applied to each side's gradient). */
+static void visc_common(const oz_tb* t, const double* UL, const double* qxL, const double* qyL,
+                        const double* UR, const double* qxR, const double* qyR, double nx, double ny, double* Fn) {
+    double Um[NV], qx[NV], qy[NV], Fv[NV], Gv[NV];
+    for (int v = 0; v < NV; ++v) {
+        Um[v] = 0.5 * (UL[v] + UR[v]);
+        qx[v] = 0.5 * (qxL[v] + qxR[v]);
+        qy[v] = 0.5 * (qyL[v] + qyR[v]);
+    }
+    oz_visc_flux(Um, qx, qy, t->g, t->mu, t->kappa, Fv, Gv);
+    for (int v = 0; v < NV; ++v) Fn[v] = Fv[v] * nx + Gv[v] * ny;
+}
+
 void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
     const int n1 = t->n1, S = t->S, E = t->E, P = t->p;
     const double g = t->g;
-    /* transformed fluxes and traces */
+    /* traces, then (viscous) gradients and jumps */
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < 4; ++k)
+            for (int f = 0; f < n1; ++f)
+                for (int v = 0; v < NV; ++v)
+                    t->Ufp[(((size_t)e * 4 + k) * n1 + f) * NV + v] =
+                        edge_value(t, U + (size_t)e * S * NV + v, NV, k, f);
+    if (t->viscous) visc_gradients(t, U);
+    /* transformed total fluxes: inviscid minus viscous (with the corrected gradient) */
     for (int e = 0; e < E; ++e) {
         for (int sp = 0; sp < S; ++sp) {
             const size_t gi = (size_t)e * S + sp;
             double Fx[NV], Fy[NV];
             oz_flux_xy(U + gi * NV, g, Fx, Fy);
+            if (t->viscous) {
+                double Qx[NV], Qy[NV], Fv[NV], Gv[NV];
+                visc_corrected(t, U, e, sp % n1, sp / n1, Qx, Qy);
+                oz_visc_flux(U + gi * NV, Qx, Qy, g, t->mu, t->kappa, Fv, Gv);
+                for (int v = 0; v < NV; ++v) { Fx[v] -= Fv[v]; Fy[v] -= Gv[v]; }
+            }
             const double* Ji = t->Jinv + gi * 4;
             const double J = t->J[gi];
             for (int v = 0; v < NV; ++v) {
@@ -298,11 +472,6 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
                 t->Gt[gi * NV + v] = J * (Ji[2] * Fx[v] + Ji[3] * Fy[v]);
             }
         }
-        for (int k = 0; k < 4; ++k)
-            for (int f = 0; f < n1; ++f)
-                for (int v = 0; v < NV; ++v)
-                    t->Ufp[(((size_t)e * 4 + k) * n1 + f) * NV + v] =
-                        edge_value(t, U + (size_t)e * S * NV + v, NV, k, f);
     }
     /* common fluxes: one evaluation per face point, outward on each side */
     for (int fc = 0; fc < t->F; ++fc) {
@@ -314,6 +483,13 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
             if (re >= 0) {
                 const double* UR = t->Ufp + (((size_t)re * 4 + rl) * n1 + (P - f)) * NV;
                 oz_riemann(t->riemann, UL, UR, nx, ny, g, Fc);
+                if (t->viscous) {
+                    double qxL[NV], qyL[NV], qxR[NV], qyR[NV], Fvn[NV];
+                    br2_side(t, le, ll, f, qxL, qyL);
+                    br2_side(t, re, rl, P - f, qxR, qyR);
+                    visc_common(t, UL, qxL, qyL, UR, qxR, qyR, nx, ny, Fvn);
+                    for (int v = 0; v < NV; ++v) Fc[v] -= Fvn[v];
+                }
                 double* Fr = t->Fout + (((size_t)re * 4 + rl) * n1 + (P - f)) * NV;
                 for (int v = 0; v < NV; ++v) Fr[v] = -Fc[v];
             } else {
@@ -321,6 +497,12 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
                 if (bci < 0) { bct = OZ_BC_SLIPWALL; bci = 0; }  /* kernels.cu:589-593 */
                 oz_ghost_state(UL, nx, ny, bct, t->bcd, bci, g, G);
                 oz_riemann(t->riemann, UL, G, nx, ny, g, Fc);
+                if (t->viscous) {  /* no viscous boundary model: the interior viscous flux */
+                    double qxL[NV], qyL[NV], Fvn[NV];
+                    br2_side(t, le, ll, f, qxL, qyL);
+                    visc_common(t, UL, qxL, qyL, UL, qxL, qyL, nx, ny, Fvn);
+                    for (int v = 0; v < NV; ++v) Fc[v] -= Fvn[v];
+                }
             }
             double* Fl = t->Fout + (((size_t)le * 4 + ll) * n1 + f) * NV;
             for (int v = 0; v < NV; ++v) Fl[v] = Fc[v];

@@ -4,7 +4,7 @@
 Runs only where oracle/_ref/libzjref.so exists (built by ``make -C oracle``
 from the reference sources in place).  Every number written here comes out of
 the reference's code (FROperators, GaussLegendre, CorrectionFunction, Mesh,
-GmshReader, ElementGeometry, IdealGas, RiemannSolvers); inputs are the
+GmshReader, ElementGeometry, IdealGas, RiemannSolvers, ViscousFlux); inputs are the
 reference's test meshes and seeded synthetic ones.  Usage:
 
     python tests/golden/make_golden.py [--ref-root /root/reference]
@@ -116,17 +116,36 @@ def gen_physics():
     return out
 
 
+def gen_viscous():
+    """ViscousFlux (navierstokes.hpp) on seeded states and gradients, with the
+    viscosity and conductivity its Sutherland law and Re scaling produced."""
+    rng = np.random.default_rng(20261203)
+    n = 64
+    gamma, Re, Pr = 1.4, 1600.0, 0.72
+    U = cases.prim_to_cons(np.exp(rng.normal(0.0, 0.5, n)), rng.normal(0.0, 1.0, n), rng.normal(0.0, 1.0, n),
+                           np.exp(rng.normal(0.0, 0.7, n)), gamma)
+    dUdx = rng.normal(0.0, 1.0, (n, 4))
+    dUdy = rng.normal(0.0, 1.0, (n, 4))
+    fx, fy, mk = zip(*(pyref.visc_flux(U[i], dUdx[i], dUdy[i], gamma, Re, Pr) for i in range(n)))
+    return dict(U=U, dUdx=dUdx, dUdy=dUdy, gamma=np.array(gamma), Re=np.array(Re), Pr=np.array(Pr),
+                fx=np.array(fx), fy=np.array(fy), mu_k=np.array(mk))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref-root", default="/root/reference")
+    ap.add_argument("--only", default=None, help="write one fixture (operators, meshes, physics, viscous)")
     a = ap.parse_args()
     if not pyref.available():
         sys.exit("oracle/_ref/libzjref.so is missing; run `make -C oracle` with the reference present")
-    np.savez_compressed(os.path.join(HERE, "operators.npz"), **gen_operators())
-    np.savez_compressed(os.path.join(HERE, "meshes.npz"), **gen_meshes(a.ref_root))
-    np.savez_compressed(os.path.join(HERE, "physics.npz"), **gen_physics())
-    for f in ("operators.npz", "meshes.npz", "physics.npz"):
-        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+    gens = {"operators": gen_operators, "meshes": lambda: gen_meshes(a.ref_root), "physics": gen_physics,
+            "viscous": gen_viscous}
+    for name, gen in gens.items():
+        if a.only and name != a.only:
+            continue
+        f = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(f, **gen())
+        print(name + ".npz", os.path.getsize(f), "bytes")
 
 
 if __name__ == "__main__":
