@@ -178,6 +178,33 @@ def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
     return out
 
 
+def time_viscous(tables, U0, steps, warmup, err_steps, stream, hiprt, Re=1600.0):
+    """The configuration's full physics: textbook FR (VCJH HU, Roe) with the
+    compressible Navier-Stokes terms (BR2, Re 1600, Pr 0.72), FP64 vs
+    FP32-bulk: time per step and the FP32 error after err_steps steps."""
+    from zhijian_amd import solver
+    kw = dict(compat="textbook", riemann="roe", corr="hu", positivity=False, cfl=0.5, viscous=True, Re=Re, Pr=0.72)
+    s64 = solver.Solver(tables, precision="fp64", **kw)
+    s32 = solver.Solver(tables, precision="fp32bulk", **kw)
+    s64.set_state(U0)
+    dt = s64.dt_cfl(0.5)
+    out = {"workload": f"same mesh and state, textbook FR (VCJH HU, Roe) + Navier-Stokes BR2 (Re {Re:g}, Pr 0.72, "
+                       f"penalty 2), no limiter", "dt": dt}
+    for name, sol in (("fp64", s64), ("fp32bulk", s32)):
+        ms, st = time_precision(sol, U0, dt, steps, warmup, stream, hiprt)
+        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3))
+    s64.set_state(U0)
+    s32.set_state(U0)
+    for _ in range(err_steps):
+        s64.rk3_step(dt)
+        s32.rk3_step(dt)
+    a, b = s64.get_state(), s32.get_state()
+    out["fp32bulk_error_vs_fp64"] = dict(steps=err_steps, rel_l2=float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+                                         rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
+    out["fp32bulk_over_fp64"] = out["fp32bulk"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    return out
+
+
 def tri_alg_bytes_per_stage(E, p, stage, precision, cross_edges):
     """Algorithmic HBM bytes of one triangle stage launch: the FP64 stage state
     per solution point (stage 1 reads U^n and writes U1; stages 2 and 3 read
@@ -444,6 +471,11 @@ def main():
         textbook = time_textbook(mesh, tables, U0, args.steps, args.warmup, args.textbook_steps, stream, hiprt)
         log(f"textbook: {textbook}")
 
+    viscous = None
+    if args.textbook_steps > 0:
+        viscous = time_viscous(tables, U0, args.steps, args.warmup, args.textbook_steps, stream, hiprt)
+        log(f"viscous: {viscous}")
+
     batched = None
     if args.batch > 0:
         batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt)
@@ -496,6 +528,7 @@ def main():
         "adjoint": adj or None,
         "batched": batched,
         "textbook": textbook,
+        "navier_stokes": viscous,
         "triangles": tri,
         "copy_kernel_gbps": copy_gbps,
     }

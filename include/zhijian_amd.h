@@ -119,6 +119,13 @@ typedef struct {
     int device;                  /* HIP device ordinal */
     int riemann;                 /* ZJ_RIEMANN_*, textbook mode */
     int corr;                    /* ZJ_CORR_*, textbook mode */
+    int viscous;                 /* SimParams::viscous.  Reference mode: the reference computes the
+                                    viscous terms and discards them (fr_solver.cpp:645-646), so they
+                                    add nothing.  Textbook mode: compressible Navier-Stokes terms with
+                                    BR2 (affine meshes without boundary faces) */
+    double Re;                   /* SimParams::Re: mu = 1/Re */
+    double Pr;                   /* SimParams::Pr: kappa = mu gamma / ((gamma - 1) Pr) */
+    double br2_eta;              /* BR2 penalty (<= 0: 2) */
 } zj_params;
 
 /* ---- solver --------------------------------------------------------- */

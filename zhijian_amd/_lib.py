@@ -52,7 +52,8 @@ class TriDesc(C.Structure):
 class Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("precision", C.c_int),
                 ("compat", C.c_int), ("positivity", C.c_int), ("device", C.c_int),
-                ("riemann", C.c_int), ("corr", C.c_int)]
+                ("riemann", C.c_int), ("corr", C.c_int), ("viscous", C.c_int), ("Re", C.c_double),
+                ("Pr", C.c_double), ("br2_eta", C.c_double)]
 
 
 _lib = None

@@ -140,9 +140,22 @@ inline double vcjh_eta(int p, int kind) {
     if (kind == 2) return (double)(p + 1) / p;
     return 0.0;
 }
+// P_k'(x) for k = 0..n by P_k' = P_{k-2}' + (2k-1) P_{k-1}, exact at x = +-1
+// as well (the reference's legendre_dp endpoint branch is not: basis.cpp:336-339).
+inline void legendre_dp_all(int n, double x, double* dP) {
+    double P[12];
+    P[0] = 1.0; dP[0] = 0.0;
+    if (n >= 1) { P[1] = x; dP[1] = 1.0; }
+    for (int k = 2; k <= n; ++k) {
+        P[k] = ((2 * k - 1) * x * P[k - 1] - (k - 1) * P[k - 2]) / k;
+        dP[k] = dP[k - 2] + (2 * k - 1) * P[k - 1];
+    }
+}
 inline double vcjh_left_d(int p, int kind, double x) {
     const double eta = vcjh_eta(p, kind), sgn = (p % 2) ? -1.0 : 1.0;
-    return sgn * 0.5 * (legendre_dp(p, x) - (eta * legendre_dp(p - 1, x) + legendre_dp(p + 1, x)) / (1.0 + eta));
+    double dP[12];
+    legendre_dp_all(p + 1, x, dP);
+    return sgn * 0.5 * (dP[p] - (eta * dP[p - 1] + dP[p + 1]) / (1.0 + eta));
 }
 inline double vcjh_right_d(int p, int kind, double x) { return -vcjh_left_d(p, kind, -x); }
 

@@ -101,6 +101,20 @@ hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>
                                 int n_items, int grid, hipStream_t s);
 int default_epb(int P);
 int max_epb(int P);
+
+// Viscous terms of the textbook scheme (zj_visc.hip): constant viscosity and
+// conductivity, BR2 penalty, and the correction derivatives g_L'(-1), g_R'(+1).
+struct ViscParams {
+    double mu, kappa, eta, gLe, gRe;
+};
+// Stage launch with viscous terms (affine quadrilaterals, no boundary faces,
+// traces of 12 words per flux point: U, dU/dx, dU/dy); tile = max_epb(P).
+template <typename R>
+hipError_t launch_visc_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                             hipStream_t s);
+template <typename R>
+hipError_t launch_visc_trace(int P, int E, const double* U, const R* Jinv_e, R* T, const OpTables<R>& ops,
+                             int xcd_groups, hipStream_t s);
 int num_blocks(int E, int epb);
 
 // perm[e] = caller index of device element e (nullptr: same order), S points per element

@@ -48,6 +48,8 @@ struct zj_solver {
     bool affine_ok = false;  // every element's metric is constant
     bool affine = false;     // kernels read Jinv_e
     bool textbook = false;   // ZJ_COMPAT_TEXTBOOK
+    bool viscous = false;    // textbook Navier-Stokes terms (BR2); traces carry U and grad U
+    ViscParams vp{};
     int riemann = 0;
     void* edge_f_aff = nullptr;  // textbook: [E][4] sign * |edge|/2 / J (affine elements)
     void* edge_f_pt = nullptr;   // textbook: [E][4] sign * |edge|/2 (per-point metric)
@@ -269,6 +271,13 @@ static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p
     if (!extract_ops(m, P, h->ops64, why)) return fail(h, ZJ_ERR_UNSUPPORTED, why);
     h->textbook = p->compat == ZJ_COMPAT_TEXTBOOK;
     h->riemann = p->riemann;
+    if (h->textbook && p->viscous) {
+        if (!(p->Re > 0.0) || !(p->Pr > 0.0)) return fail(h, ZJ_ERR_ARG, "viscous terms need Re > 0 and Pr > 0");
+        const double mu = 1.0 / p->Re;
+        h->viscous = true;
+        h->vp = ViscParams{mu, mu * p->gamma / ((p->gamma - 1.0) * p->Pr), p->br2_eta > 0.0 ? p->br2_eta : 2.0,
+                           zj::host::vcjh_left_d(P, p->corr, -1.0), zj::host::vcjh_right_d(P, p->corr, 1.0)};
+    }
     if (h->textbook) {  // VCJH g_L', g_R' at the 1D Gauss points
         double x[6], w[6];
         zj::host::gauss_rule(P + 1, x, w);
@@ -319,7 +328,12 @@ static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p
     ZJ_HIP(h, hipMalloc((void**)&h->W, std::max<size_t>(npts * 4, 1) * sizeof(double)));
     ZJ_HIP(h, hipMalloc((void**)&h->Rres, std::max<size_t>(npts * 4, 1) * sizeof(double)));
     ZJ_HIP(h, hipMalloc((void**)&h->aos, std::max<size_t>(npts * 4, 1) * sizeof(double)));
-    for (int k = 0; k < 3; ++k) ZJ_HIP(h, hipMalloc(&h->T[k], std::max<size_t>((size_t)E * 4 * Fp * 4, 1) * rsz));
+    if (h->viscous)
+        for (const EdgeLink& L : links)
+            if (L.role != ROLE_LEFT && L.role != ROLE_RIGHT)
+                return fail(h, ZJ_ERR_UNSUPPORTED, "viscous terms are built for meshes without boundary faces");
+    const size_t tw = h->viscous ? 12 : 4;  // trace words per flux point
+    for (int k = 0; k < 3; ++k) ZJ_HIP(h, hipMalloc(&h->T[k], std::max<size_t>((size_t)E * 4 * Fp * tw, 1) * rsz));
 
     // Jinv: [E][S][4] -> 4 planes, converted to the compute precision
     std::vector<double> jp(npts * 4);
@@ -341,6 +355,8 @@ static int build_ordered(zj_solver* h, const zj_mesh_desc* m, const zj_params* p
     }
     if (!h->affine_ok) je.clear();
     h->affine = h->affine_ok;
+    if (h->viscous && !h->affine_ok)
+        return fail(h, ZJ_ERR_UNSUPPORTED, "viscous terms are built for affine elements (constant metric)");
     // textbook correction factors: sign (-1 on edges 0, 3) * half edge length [/ J]
     std::vector<double> efa, efp, jpt;
     if (h->textbook) {
@@ -475,6 +491,12 @@ static StageArgs<R> stage_args(zj_solver* h, int mode) {
 
 static hipError_t enqueue_stage(zj_solver* h, int mode, hipStream_t s) {
     hipError_t e;
+    if (h->viscous) {
+        e = h->prec == ZJ_PREC_FP64 ? launch_visc_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->vp, s)
+                                    : launch_visc_stage<float>(h->P, mode, stage_args<float>(h, mode), h->ops32, h->vp, s);
+        if (e == hipSuccess && (mode == MODE_RESIDUAL || mode == MODE_STAGE3)) h->l2_blocks = h->nblk;
+        return e;
+    }
     if (h->prec == ZJ_PREC_FP64)
         e = launch_stage<double>(h->P, mode, stage_args<double>(h, mode), h->ops64, h->epb, h->affine, s, h->textbook);
     else
@@ -493,7 +515,13 @@ static hipError_t set_dt(zj_solver* h, double dt, hipStream_t s) {
 
 static int refresh_trace(zj_solver* h, hipStream_t s) {
     hipError_t e;
-    if (h->prec == ZJ_PREC_FP64)
+    if (h->viscous)
+        e = h->prec == ZJ_PREC_FP64
+                ? launch_visc_trace<double>(h->P, h->E, h->U, (const double*)h->Jinv_e, (double*)h->T[0], h->ops64,
+                                            h->xcd_groups, s)
+                : launch_visc_trace<float>(h->P, h->E, h->U, (const float*)h->Jinv_e, (float*)h->T[0], h->ops32,
+                                           h->xcd_groups, s);
+    else if (h->prec == ZJ_PREC_FP64)
         e = launch_trace<double>(h->P, TraceArgs<double>{h->E, h->U, (double*)h->T[0], h->xcd_groups}, h->ops64, 0, s);
     else
         e = launch_trace<float>(h->P, TraceArgs<float>{h->E, h->U, (float*)h->T[0], h->xcd_groups}, h->ops32, 0, s);
@@ -694,6 +722,7 @@ extern "C" int zj_set_xcd_groups(zj_handle h, int groups) {
 extern "C" int zj_set_geometry_mode(zj_handle h, int mode, int* active) {
     if (!h || mode < -1 || mode > 1) return ZJ_ERR_ARG;
     if (mode == 1 && !h->affine_ok) return fail(h, ZJ_ERR_UNSUPPORTED, "the mesh has non-affine elements");
+    if (mode == 0 && h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the viscous terms run with the per-element metric");
     if (mode >= 0 && (mode == 1) != h->affine) {
         if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "geometry change in the middle of a step");
         ZJ_HIP(h, hipSetDevice(h->device));

@@ -103,7 +103,8 @@ CORRS = {"dg": 0, "sd": 1, "hu": 2, "ga": 3}
 
 class Solver:
     def __init__(self, t: Tables, gamma=1.4, cfl=0.5, precision="fp64", positivity=True, device=0,
-                 compat="reference", riemann="rusanov", corr="dg", order="hilbert"):
+                 compat="reference", riemann="rusanov", corr="dg", order="hilbert", viscous=False, Re=1000.0,
+                 Pr=0.72, br2_eta=0.0):
         self.tables = t
         self.p, self.E, self.S = t.p, t.E, t.S
         self.n = self.E * self.S * 4
@@ -132,7 +133,7 @@ class Solver:
         if order == "hilbert" and t.centroid is not None:
             d.elem_centroid = _ptr(c(t.centroid, np.float64))
         prm = _lib.Params(gamma, cfl, PRECISIONS[precision], COMPATS[compat], int(positivity), device,
-                          RIEMANNS[riemann], CORRS[corr])
+                          RIEMANNS[riemann], CORRS[corr], int(viscous), float(Re), float(Pr), float(br2_eta))
         h = C.c_void_p()
         check(lib().zj_create(C.byref(h), C.byref(d), C.byref(prm)))
         self._h = h
