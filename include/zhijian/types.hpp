@@ -100,6 +100,9 @@ struct SimParams {
     Real t_final = 1.0;
     int max_iter = 100000;
 
+    // Reference arithmetic: computed and discarded, as the reference does.
+    // Textbook scheme: compressible Navier-Stokes terms with BR2 (mu = 1/Re,
+    // affine elements, periodic meshes; initialize() throws otherwise).
     bool viscous = true;
 
     int output_freq = 100;

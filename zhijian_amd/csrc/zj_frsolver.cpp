@@ -603,6 +603,10 @@ void FRSolver::initialize(const Mesh& mesh, const SimParams& params) {
     zp.riemann = params.riemann == RiemannSolver::Roe ? ZJ_RIEMANN_ROE
                  : params.riemann == RiemannSolver::HLLC ? ZJ_RIEMANN_HLLC : ZJ_RIEMANN_RUSANOV;
     zp.corr = static_cast<int>(params.flux_type);  // DG, SD, HU, GA share ZJ_CORR_* values
+    // SimParams::viscous: the reference's executed arithmetic discards the
+    // viscous terms (fr_solver.cpp:645-646); the textbook scheme runs them (BR2)
+    zp.viscous = (params.compat == Compat::Textbook && params.viscous) ? 1 : 0;
+    zp.Re = params.Re; zp.Pr = params.Pr;
     zj_handle h = nullptr;
     const int rc = zj_create(&h, &md, &zp);
     if (rc != ZJ_OK) throw std::runtime_error(std::string("FRSolver::initialize: ") + zj_last_error(nullptr));
