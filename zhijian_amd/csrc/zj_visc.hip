@@ -90,7 +90,7 @@ __device__ __forceinline__ void point_gradient(const R* u, const R* D, int i, in
 }
 
 template <typename R, int P, int EPB, int MODE>
-__global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
+__global__ __launch_bounds__(BLOCK, 3) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
                                                            const ViscParams vp) {
     using Y = Lay<R, P>;
     using SM = ViscSmem<R, P, EPB>;
@@ -129,19 +129,17 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
     const int ub = el * Y::UE + sp;
     const int i = sp % N1, j = sp / N1;
 
-    // ---- A: state
-    double q[NV], u0[NV];
+    // ---- A: state (read again for the update in phase E, from cache, rather
+    // than held in registers through the face phases)
+    auto stage_input = [&](int v) -> double {
+        return (INC && HAS_U0) ? a.U0[v * PS + gi] + double(a.dW_in[v * PS + gi]) : a.U_in[v * PS + gi];
+    };
     R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
     if (active) {
         jx = a.Jinv[(int64_t)e * 4]; jy = a.Jinv[(int64_t)e * 4 + 1];
         kx = a.Jinv[(int64_t)e * 4 + 2]; ky = a.Jinv[(int64_t)e * 4 + 3];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            if (INC && HAS_U0) u0[v] = a.U0[v * PS + gi];
-            if (INC && HAS_U0) q[v] = u0[v] + double(a.dW_in[v * PS + gi]);
-            else q[v] = a.U_in[v * PS + gi];
-            sU[ub + v * Y::UV] = R(q[v]);
-        }
+        for (int v = 0; v < NV; ++v) sU[ub + v * Y::UV] = R(stage_input(v));
     }
     __syncthreads();
 
@@ -163,7 +161,9 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
     }
     __syncthreads();
 
-    // ---- B2: jumps and common total normal fluxes per (edge, flux point)
+    // ---- B2a: jumps and the common viscous normal flux per (edge, flux point)
+    // (kept apart from the Riemann solve so that the two never hold their
+    // registers at once)
     for (int t = tid; t < NF; t += BLOCK) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
@@ -175,56 +175,72 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
         if (f == 0) sOut[el2 * 4 + k] = !inside;
         const int oslot = P - f;
         const TraceLine tl = trace_line<P>(k, f);
-        R own[NV], oth[NV], gxo[NV], gyo[NV], gxn[NV], gyn[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            own[v] = sTr[Y::tr(el2, k, f, v)];
-            gxo[v] = trace_point<R, P>(sGx + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
-            gyo[v] = trace_point<R, P>(sGy + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
-        }
-        if (inside) {
-            const TraceLine tn = trace_line<P>(L.nbr_edge, oslot);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                oth[v] = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
-                gxn[v] = trace_point<R, P>(sGx + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
-                gyn[v] = trace_point<R, P>(sGy + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
-            }
-        } else {
-            const R* T = a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
-            load4(T, oth);
-            load4(T + NV, gxn);
-            load4(T + 2 * NV, gyn);
-        }
+        const TraceLine tn = trace_line<P>(L.nbr_edge, oslot);
+        const R* T = a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
         // BR2 gradients: own and neighbour, each with its own face's lifting
         // (this item's element e2 is not the thread's own element)
         const R* Jo = a.Jinv + (int64_t)e2 * 4;
         const R* Jn = a.Jinv + (int64_t)L.nbr * 4;
         const bool oxi = (k & 1) != 0, nxi = (L.nbr_edge & 1) != 0;
-        const R go = (k == 0 || k == 3) ? gLe : gRe;
-        const R gn = (L.nbr_edge == 0 || L.nbr_edge == 3) ? gLe : gRe;
-        const R ax = oxi ? Jo[0] : Jo[2], ay = oxi ? Jo[1] : Jo[3];
-        const R bx = nxi ? Jn[0] : Jn[2], by = nxi ? Jn[1] : Jn[3];
-        R UL[NV], UR[NV], qx[NV], qy[NV], Um[NV], Fc[NV], Fv[NV], Gv[NV];
-        const bool right = L.role == ROLE_RIGHT;
+        const R go = eta * ((k == 0 || k == 3) ? gLe : gRe);
+        const R gn = eta * ((L.nbr_edge == 0 || L.nbr_edge == 3) ? gLe : gRe);
+        const R ax = go * (oxi ? Jo[0] : Jo[2]), ay = go * (oxi ? Jo[1] : Jo[3]);
+        const R bx = gn * (nxi ? Jn[0] : Jn[2]), by = gn * (nxi ? Jn[1] : Jn[3]);
+        R qx[NV], qy[NV], Um[NV], Fv[NV], Gv[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const R uc = R(0.5) * (own[v] + oth[v]);
-            const R jo = uc - own[v], jn = uc - oth[v];
+            const R own = sTr[Y::tr(el2, k, f, v)];
+            R oth, gxn, gyn;
+            if (inside) {
+                oth = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
+                gxn = trace_point<R, P>(sGx + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+                gyn = trace_point<R, P>(sGy + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+            } else {
+                oth = T[v];
+                gxn = T[NV + v];
+                gyn = T[2 * NV + v];
+            }
+            const R gxo = trace_point<R, P>(sGx + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+            const R gyo = trace_point<R, P>(sGy + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+            const R uc = R(0.5) * (own + oth);
+            const R jo = uc - own, jn = uc - oth;
             sJ[Y::tr(el2, k, f, v)] = jo;
-            // mean of the two sides' BR2 gradients
-            qx[v] = R(0.5) * ((gxo[v] + eta * ax * go * jo) + (gxn[v] + eta * bx * gn * jn));
-            qy[v] = R(0.5) * ((gyo[v] + eta * ay * go * jo) + (gyn[v] + eta * by * gn * jn));
-            Um[v] = R(0.5) * (own[v] + oth[v]);
-            UL[v] = right ? oth[v] : own[v];
-            UR[v] = right ? own[v] : oth[v];
+            qx[v] = R(0.5) * ((gxo + ax * jo) + (gxn + bx * jn));
+            qy[v] = R(0.5) * ((gyo + ay * jo) + (gyn + by * jn));
+            Um[v] = uc;
         }
-        riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
         visc_flux(Um, qx, qy, g, mu, kappa, Fv, Gv);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sFI[Y::tr(el2, k, f, v)] = Fv[v] * nx + Gv[v] * ny;  // as the left side sees it
+    }
+    __syncthreads();
+
+    // ---- B2b: Riemann flux; total common normal flux, outward and scaled
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+        const int nl = L.nbr - e0;
+        const int oslot = P - f;
+        R own[NV], oth[NV], UL[NV], UR[NV], Fc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) own[v] = sTr[Y::tr(el2, k, f, v)];
+        if (nl >= 0 && nl < EPB) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) oth[v] = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
+        } else {
+            load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW, oth);
+        }
+        const bool right = L.role == ROLE_RIGHT;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { UL[v] = right ? oth[v] : own[v]; UR[v] = right ? own[v] : oth[v]; }
+        riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
         const R ef = a.edge_f[(int64_t)e2 * 4 + k];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const R fc = Fc[v] - (Fv[v] * nx + Gv[v] * ny);  // as the face's left side sees it
+            const R fc = Fc[v] - sFI[Y::tr(el2, k, f, v)];  // as the face's left side sees it
             sFI[Y::tr(el2, k, f, v)] = ef * (right ? -fc : fc);
         }
     }
@@ -290,14 +306,15 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
             const double dt = *a.dt_ptr;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                if (!INC && HAS_U0) u0[v] = a.U0[v * PS + gi];
+                const double q = stage_input(v);
+                const double u0 = HAS_U0 ? a.U0[v * PS + gi] : 0.0;
                 double un;
-                if (MODE == MODE_STAGE1) un = q[v] + dt * res[v];
-                else if (MODE == MODE_STAGE2) un = 0.75 * u0[v] + 0.25 * q[v] + 0.25 * dt * res[v];
-                else un = (1.0 / 3.0) * u0[v] + (2.0 / 3.0) * q[v] + (2.0 / 3.0) * dt * res[v];
+                if (MODE == MODE_STAGE1) un = q + dt * res[v];
+                else if (MODE == MODE_STAGE2) un = 0.75 * u0 + 0.25 * q + 0.25 * dt * res[v];
+                else un = (1.0 / 3.0) * u0 + (2.0 / 3.0) * q + (2.0 / 3.0) * dt * res[v];
                 double next = un;
                 if (INC && MODE != MODE_STAGE3) {
-                    const double base = MODE == MODE_STAGE1 ? q[v] : u0[v];
+                    const double base = MODE == MODE_STAGE1 ? q : u0;
                     const float dw = float(un - base);
                     a.dW_out[v * PS + gi] = dw;
                     next = base + double(dw);
