@@ -383,6 +383,8 @@ def main():
                     help="cells per side of the triangle leg (each split in two; 0: skip)")
     ap.add_argument("--tri-only", action="store_true",
                     help="run only the triangle leg and print its JSON (profiling runs)")
+    ap.add_argument("--ns-only", action="store_true",
+                    help="run only the Navier-Stokes (BR2) leg on the TGV mesh and print its JSON (profiling runs)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint/forward cost ratio (TGV 128^2 P3)")
     ap.add_argument("--only", choices=("fp64", "fp32bulk"), default=None,
@@ -405,6 +407,16 @@ def main():
         tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.err_steps, hiprt.Stream(), hiprt,
                        0.0 if args.no_cpu else 5.0)
         print(json.dumps({"triangles": tri}), flush=True)
+        return
+
+    if args.ns_only:
+        mesh = cases.tgv_mesh(args.n)
+        tables = solver.build_tables(mesh.xy, mesh.elem_nodes, args.p, periodic=(mesh.Lx, mesh.Ly))
+        if args.probe_bytes > 0:
+            log(f"copy kernel: {_lib.copy_probe(int(args.probe_bytes), 10):.0f} GB/s")
+        ns = time_viscous(tables, cases.tgv_ic(tables.pos, mach=0.1).reshape(-1), args.steps, args.warmup,
+                          args.err_steps, hiprt.Stream(), hiprt)
+        print(json.dumps({"navier_stokes": ns}), flush=True)
         return
 
     mesh = cases.tgv_mesh(args.n)

@@ -90,7 +90,7 @@ __device__ __forceinline__ void point_gradient(const R* u, const R* D, int i, in
 }
 
 template <typename R, int P, int EPB, int MODE>
-__global__ __launch_bounds__(BLOCK, 3) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
+__global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
                                                            const ViscParams vp) {
     using Y = Lay<R, P>;
     using SM = ViscSmem<R, P, EPB>;
