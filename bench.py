@@ -304,6 +304,79 @@ def measured_traffic(n, p):
     return None
 
 
+LIBHDF5_PY = "/opt/conda/bin/python3.9"
+_H5PY_WRITE = """
+import sys, time, numpy as np, h5py
+sol = np.load(sys.argv[2]); nsp = np.load(sys.argv[3])
+best_w = best_r = 1e30
+for _ in range(3):
+    t0 = time.perf_counter()
+    with h5py.File(sys.argv[1], "w") as f:
+        f.attrs.create("time", 0.5, dtype="f8"); f.attrs.create("iteration", 10, dtype="i4")
+        f.attrs.create("poly_order", 4, dtype="i4"); f.attrs.create("gamma", 1.4, dtype="f8")
+        f.create_dataset("solution", data=sol); f.create_dataset("elem_n_sp", data=nsp)
+    best_w = min(best_w, time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    with h5py.File(sys.argv[1], "r") as f:
+        a = f["solution"][:]; b = f["elem_n_sp"][:]; t = float(f.attrs["time"])
+    best_r = min(best_r, time.perf_counter() - t0)
+print(best_w * 1e3, best_r * 1e3)
+"""
+
+
+def time_io(sol, mesh, p, reps=3):
+    """Restart save / load of the device state (zj_restart_save / _load) and
+    VTK output of it, on the bench workload.  Files go to a temporary
+    directory that is removed afterwards.  With an installed libhdf5 (h5py),
+    the same restart layout written and read by libhdf5 is timed beside."""
+    import shutil
+    import subprocess
+    import tempfile
+    from zhijian_amd import io
+    d = tempfile.mkdtemp(prefix="zj_io_")
+    out = {}
+    try:
+        path = os.path.join(d, "restart.h5")
+        U = sol.get_state()
+        nbytes = U.nbytes
+        t_get = min(_timed(sol.get_state) for _ in range(reps))
+        t_save = min(_timed(lambda: sol.save_restart(path, 0.5, 10)) for _ in range(reps))
+        t_load = min(_timed(lambda: sol.load_restart(path)) for _ in range(reps))
+        fsize = os.path.getsize(path)
+        out["restart"] = dict(file_bytes=fsize, state_bytes=nbytes, get_state_ms=t_get * 1e3,
+                              save_ms=t_save * 1e3, load_ms=t_load * 1e3, save_gbps=fsize / t_save / 1e9,
+                              load_gbps=fsize / t_load / 1e9)
+        if os.path.exists(LIBHDF5_PY):
+            np.save(os.path.join(d, "sol.npy"), U)
+            np.save(os.path.join(d, "nsp.npy"), np.full(mesh.elem_nodes.shape[0], (p + 1) ** 2, np.int32))
+            r = subprocess.run([LIBHDF5_PY, "-c", _H5PY_WRITE, os.path.join(d, "h5py.h5"), os.path.join(d, "sol.npy"),
+                                os.path.join(d, "nsp.npy")], capture_output=True, text=True, timeout=300)
+            if r.returncode == 0:
+                w, rd = (float(x) for x in r.stdout.split())
+                out["restart"]["libhdf5_h5py"] = dict(write_ms=w, read_ms=rd,
+                                                      note="same layout, host arrays, h5py 3.3 / HDF5 1.10.6")
+        m = io.VisMesh(mesh.xy, mesh.elem_nodes, 1, 1)
+        nsp = (p + 1) ** 2
+        for name, flags in (("vtu_binary", io.BINARY), ("vtu_binary_interpolated", io.BINARY | io.INTERPOLATE),
+                            ("vtu_ascii", io.VTU)):
+            f = os.path.join(d, name + ".vtu")
+            n_rep = 1 if name == "vtu_ascii" else reps
+            t = min(_timed(lambda: io.write_vtk(f, m, nsp, U, p, 1.4, 4, flags)) for _ in range(n_rep))
+            out[name] = dict(ms=t * 1e3, file_bytes=os.path.getsize(f), subdivisions=4,
+                             points=mesh.elem_nodes.shape[0] * 25, mb_per_s=os.path.getsize(f) / t / 1e6)
+            os.remove(f)
+        out["host_threads"] = min(os.cpu_count() or 1, 16)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -383,6 +456,9 @@ def main():
                     help="cells per side of the triangle leg (each split in two; 0: skip)")
     ap.add_argument("--tri-only", action="store_true",
                     help="run only the triangle leg and print its JSON (profiling runs)")
+    ap.add_argument("--io-only", action="store_true",
+                    help="only the restart / VTK output timings on the bench workload")
+    ap.add_argument("--no-io", dest="io", action="store_false", help="skip the restart / VTK output timings")
     ap.add_argument("--ns-only", action="store_true",
                     help="run only the Navier-Stokes (BR2) leg on the TGV mesh and print its JSON (profiling runs)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
@@ -417,6 +493,14 @@ def main():
         ns = time_viscous(tables, cases.tgv_ic(tables.pos, mach=0.1).reshape(-1), args.steps, args.warmup,
                           args.err_steps, hiprt.Stream(), hiprt)
         print(json.dumps({"navier_stokes": ns}), flush=True)
+        return
+
+    if args.io_only:
+        mesh = cases.tgv_mesh(args.n)
+        tables = solver.build_tables(mesh.xy, mesh.elem_nodes, args.p, periodic=(mesh.Lx, mesh.Ly))
+        s = solver.Solver(tables, precision="fp64")
+        s.set_state(cases.tgv_ic(tables.pos, mach=0.1).reshape(-1))
+        print(json.dumps({"io": time_io(s, mesh, args.p)}), flush=True)
         return
 
     mesh = cases.tgv_mesh(args.n)
@@ -505,6 +589,11 @@ def main():
                        0.0 if args.no_cpu else 5.0)
         log(f"triangles: {tri}")
 
+    io_res = None
+    if args.io:
+        io_res = time_io(s64, mesh, args.p)
+        log(f"io: {io_res}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.p, U0, dt, args.cpu_seconds)
@@ -542,6 +631,7 @@ def main():
         "textbook": textbook,
         "navier_stokes": viscous,
         "triangles": tri,
+        "io": io_res,
         "copy_kernel_gbps": copy_gbps,
     }
     print(json.dumps(out), flush=True)

@@ -66,6 +66,10 @@ public:
     // ---- this build
     // Residual dU/dt of the current state, AoS [E][S][4] on the host.
     std::vector<Real> residualField();
+    // Resume from a restart file's solution (RestartIO::read): sets the state,
+    // the time and the iteration count.  The reference's main() reads the file
+    // but never hands it to the solver (src/main.cpp:349-360).
+    void setSolution(const std::vector<ElementSolution>& solution, Real time, int iteration);
     // The underlying C ABI handle (for callers that drive stages directly).
     zj_solver* handle() const { return handle_; }
 

@@ -306,6 +306,74 @@ int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_block, int* 
 int zj_tri_operators(int p, double* sp, double* fp, double* wg, double* Dr, double* Ds,
                      double* interp, double* lift);
 
+/* Polynomial order and gamma a handle was created with. */
+int zj_get_scheme(zj_handle h, int* poly_order, double* gamma);
+int zj_tri_get_scheme(zj_tri_handle h, int* poly_order, double* gamma);
+
+/* ---- restart files ----------------------------------------------------
+ * RestartIO (src/io/restart.cpp:8-165): HDF5 file format 0, as libhdf5
+ * writes it with default property lists, with root attributes time (f64),
+ * iteration (i32), poly_order (i32), gamma (f64) and datasets "solution"
+ * (f64, sum(elem_n_sp) * 4 values: element by element, point by point,
+ * variable fastest) and "elem_n_sp" (i32, one per element).  The library's
+ * own HDF5 codec writes and reads them (no HDF5 library is needed); files
+ * libhdf5 wrote in that layout read back the same.  On failure these return
+ * ZJ_ERR_ARG (or the handle call's code) with the message in
+ * zj_io_last_error(), which is per thread. */
+typedef struct {
+    int64_t n_elem;              /* length of elem_n_sp */
+    int64_t n_values;            /* length of solution */
+    double time;
+    int iteration;
+    int poly_order;
+    double gamma;
+} zj_restart_info;
+const char* zj_io_last_error(void);
+/* RestartIO::write (restart.cpp:8-91). */
+int zj_restart_write(const char* path, int64_t n_elem, const int* elem_n_sp, const double* solution,
+                     double time, int iteration, int poly_order, double gamma);
+/* RestartIO::read, attributes and sizes (restart.cpp:93-131). */
+int zj_restart_query(const char* path, zj_restart_info* info);
+/* RestartIO::read, the arrays (restart.cpp:133-165); n_elem and n_values
+ * must match the file; solution may be NULL. */
+int zj_restart_read(const char* path, int64_t n_elem, int* elem_n_sp, int64_t n_values, double* solution);
+/* A handle's device state to a restart file (FRSolver::copyToHost followed by
+ * RestartIO::write), and back: load checks the element count, points per
+ * element, poly_order and gamma against the handle, then sets the state (the
+ * copy into the solver that the reference's main() leaves out,
+ * src/main.cpp:349-360).  time / iteration may be NULL. */
+int zj_restart_save(zj_handle h, const char* path, double time, int iteration);
+int zj_restart_load(zj_handle h, const char* path, double* time, int* iteration);
+int zj_tri_restart_save(zj_tri_handle h, const char* path, double time, int iteration);
+int zj_tri_restart_load(zj_tri_handle h, const char* path, double* time, int* iteration);
+
+/* ---- visualisation output ---------------------------------------------
+ * VTKWriter (src/io/visualization.cpp:8-386): each element is subdivided
+ * into a lattice of `subdivisions` cells per edge (quadrilaterals: (n+1)^2
+ * points, 2n^2 triangles; triangles: (n+1)(n+2)/2 points, n^2 triangles),
+ * point coordinates from the element's shape functions, point data Density,
+ * Pressure, Mach, Velocity.  Default: the reference's text byte for byte,
+ * with the state of the element's first solution point at every lattice
+ * point (visualization.cpp:327, 360). */
+typedef struct {
+    int64_t n_nodes;
+    const double* xy;                /* [n_nodes][2] */
+    int64_t n_elem;
+    const int* elem_type;            /* ElementType: 0 triangle, 1 quadrilateral */
+    const int* elem_order;           /* geometric order: 1, 2 (6 / 9 nodes), 4 (25-node quads) */
+    const int64_t* elem_node_start;  /* [n_elem + 1] offsets into elem_nodes */
+    const int64_t* elem_nodes;       /* node ids, Gmsh order */
+} zj_vis_mesh;
+#define ZJ_VTK_VTU 0                 /* XML .vtu, ascii (VTKWriter::writeVTU) */
+#define ZJ_VTK_LEGACY 1              /* legacy .vtk, ascii (VTKWriter::writeLegacy) */
+#define ZJ_VTK_BINARY 2              /* .vtu data arrays appended raw (header_type UInt64) */
+#define ZJ_VTK_INTERPOLATE 4         /* the solution polynomial at the lattice points (textbook);
+                                        elem_n_sp must be (p+1)^2 / (p+1)(p+2)/2 */
+int zj_vtk_write(const char* path, const zj_vis_mesh* mesh, const int* elem_n_sp, const double* solution,
+                 int poly_order, double gamma, int subdivisions, int flags);
+/* VTKWriter::writePVTU (visualization.cpp:247-283). */
+int zj_pvtu_write(const char* path, int n_pieces, const char* const* piece_files);
+
 /* ---- measurement (no reference counterpart) -------------------------- */
 /* Copies `bytes` bytes (rounded to 16) between two device buffers `reps` times
  * with a plain grid-stride copy kernel and reports the mean GB/s (read + write

@@ -9,7 +9,15 @@
 //       (no GPU) faces and boundary tags of a Gmsh mesh read by GmshReader
 //   frsolver_driver run    <vortex|tgv> <n> <p> <steps> <dt_max> <fp64|fp32bulk> <prefix>
 //       writes <prefix>_U0.bin, <prefix>_U.bin and one JSON line on stdout
+//   frsolver_driver io     <dir>
+//       (no GPU) RestartIO, VTKWriter and ResidualWriter on a 3 x 3 mesh
+//       with a synthetic P2 solution; files under <dir>, one JSON line
+//   frsolver_driver resume <dir> <n> <p> <steps_a> <steps_b>
+//       vortex steps_a steps, restart file, a new solver resumed from it
+//       (RestartIO::read + setSolution), steps_b more; and one solver run
+//       straight through; writes <dir>/resumed.bin, <dir>/straight.bin
 #include "zhijian/fr_solver.hpp"
+#include "zhijian/io.hpp"
 #include "zhijian/rk_integrator.hpp"
 
 #include <cmath>
@@ -58,6 +66,144 @@ static std::vector<double> state(const FRSolver& s) {
     return U;
 }
 
+static std::vector<ElementSolution> synthetic_solution(Index E, int nsp) {
+    std::vector<ElementSolution> sol(E);
+    for (Index e = 0; e < E; ++e) {
+        sol[e].sol_pts.resize(nsp);
+        for (int k = 0; k < nsp; ++k) {
+            const Real rho = 1.0 + 0.01 * ((e * 7 + k * 3) % 11), u = 0.1 * ((e + 2 * k) % 5) - 0.2,
+                       v = 0.05 * ((3 * e + k) % 7) - 0.15, pr = 1.0 + 0.02 * ((e + k) % 13);
+            State& q = sol[e].sol_pts[k];
+            q[0] = rho;
+            q[1] = rho * u;
+            q[2] = rho * v;
+            q[3] = pr / 0.4 + 0.5 * rho * (u * u + v * v);
+        }
+    }
+    return sol;
+}
+
+static bool same_solution(const std::vector<ElementSolution>& a, const std::vector<ElementSolution>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t e = 0; e < a.size(); ++e) {
+        if (a[e].sol_pts.size() != b[e].sol_pts.size()) return false;
+        for (size_t k = 0; k < a[e].sol_pts.size(); ++k)
+            for (int v = 0; v < N_VARS; ++v)
+                if (std::memcmp(&a[e].sol_pts[k][v], &b[e].sol_pts[k][v], sizeof(Real))) return false;
+    }
+    return true;
+}
+
+static int io_case(const std::string& dir) {
+    Mesh mesh = uniform_mesh(3, 0.0, 1.0);
+    SimParams params;
+    params.poly_order = 2;
+    params.gamma = 1.4;
+    const auto sol = synthetic_solution(mesh.numElements(), 9);
+    std::vector<double> U;
+    for (const auto& e : sol)
+        for (const auto& q : e.sol_pts)
+            for (int v = 0; v < N_VARS; ++v) U.push_back(q[v]);
+    dump(dir + "/U.bin", U);
+
+    RestartIO rio;
+    bool ok = rio.write(dir + "/r.h5", mesh, sol, params, 0.75, 120);
+    std::vector<ElementSolution> back;
+    SimParams rp;
+    Real t = 0;
+    int it = 0;
+    ok = ok && rio.read(dir + "/r.h5", mesh, back, rp, t, it);
+    const bool restart_ok = ok && same_solution(sol, back) && t == 0.75 && it == 120 && rp.poly_order == 2 &&
+                            rp.gamma == 1.4;
+    ok = rio.writeParallel(dir + "/p.h5", 0, 1, mesh, sol, params, 0.5, 60);
+    std::vector<ElementSolution> pback;
+    ok = ok && rio.readParallel(dir + "/p.h5", 0, 1, mesh, pback, rp, t, it);
+    const bool parallel_ok = ok && same_solution(sol, pback) && t == 0.5 && it == 60;
+    std::vector<ElementSolution> none;
+    const bool missing_fails = !rio.read(dir + "/missing.h5", mesh, none, rp, t, it) && !rio.errorMessage().empty();
+
+    VTKWriter vtk;
+    bool vtk_ok = vtk.writeVTU(dir + "/a.vtu", mesh, sol, params) && vtk.writeLegacy(dir + "/a.vtk", mesh, sol, params);
+    vtk.setSubdivisions(2);
+    vtk_ok = vtk_ok && vtk.writeParallelVTU(dir + "/par", 0, 2, mesh, sol, params);
+    vtk.setSubdivisions(4);
+    vtk.setBinary(true);
+    vtk_ok = vtk_ok && vtk.writeVTU(dir + "/b.vtu", mesh, sol, params);
+    params.compat = Compat::Textbook;
+    vtk_ok = vtk_ok && vtk.writeVTU(dir + "/h.vtu", mesh, sol, params);
+
+    ResidualWriter res;
+    const bool res_ok = res.open(dir + "/res.csv");
+    res.write(10, 0.5, 1.25e-3, {1.0, 2.5e-7, -3.0, 4.125});
+    res.write(20, 1.0, 6.5e-4, {0.0, 0.0, 0.0, 0.0});
+    res.close();
+    std::printf("{\"restart\": %s, \"parallel\": %s, \"missing_fails\": %s, \"vtk\": %s, \"residual\": %s}\n",
+                restart_ok ? "true" : "false", parallel_ok ? "true" : "false", missing_fails ? "true" : "false",
+                vtk_ok ? "true" : "false", res_ok ? "true" : "false");
+    return 0;
+}
+
+static int resume_case(const std::string& dir, int n, int p, int steps_a, int steps_b) {
+    Mesh mesh = uniform_mesh(n, -5.0, 5.0);
+    SimParams params;
+    params.poly_order = p;
+    params.gamma = 1.4;
+    params.CFL = 0.5;
+    params.dt = 1e-3;
+    params.viscous = false;
+    const Real g = params.gamma, beta = 5.0;
+    IdealGas gas(g);
+    auto ic = [&](Vec2 x) {
+        const Real r2 = x.x * x.x + x.y * x.y, f = std::exp(0.5 * (1.0 - r2));
+        const Real T = 1.0 - (g - 1.0) * beta * beta / (8.0 * g * M_PI * M_PI) * std::exp(1.0 - r2);
+        const Real rho = std::pow(T, 1.0 / (g - 1.0));
+        return gas.primToConserv(rho, 1.0 - beta / (2 * M_PI) * x.y * f, beta / (2 * M_PI) * x.x * f,
+                                 std::pow(rho, g));
+    };
+    auto run = [&](FRSolver& s, int steps) {
+        for (int k = 0; k < steps; ++k) {
+            Real dt = std::min(s.computeTimeStep(), params.dt);
+            s.advance(dt);
+        }
+    };
+    FRSolver straight;
+    straight.initialize(mesh, params);
+    straight.setInitialCondition(ic);
+    run(straight, steps_a + steps_b);
+    straight.copyToHost();
+    dump(dir + "/straight.bin", state(straight));
+
+    RestartIO rio;
+    {
+        FRSolver first;
+        first.initialize(mesh, params);
+        first.setInitialCondition(ic);
+        run(first, steps_a);
+        first.copyToHost();
+        if (!rio.write(dir + "/resume.h5", mesh, first.solution(), params, first.currentTime(), first.iteration())) {
+            std::fprintf(stderr, "%s\n", rio.errorMessage().c_str());
+            return 1;
+        }
+    }
+    std::vector<ElementSolution> sol;
+    SimParams rp = params;
+    Real t = 0;
+    int it = 0;
+    if (!rio.read(dir + "/resume.h5", mesh, sol, rp, t, it)) {
+        std::fprintf(stderr, "%s\n", rio.errorMessage().c_str());
+        return 1;
+    }
+    FRSolver second;
+    second.initialize(mesh, rp);
+    second.setSolution(sol, t, it);
+    run(second, steps_b);
+    second.copyToHost();
+    dump(dir + "/resumed.bin", state(second));
+    std::printf("{\"time\": %.17g, \"time_straight\": %.17g, \"iteration\": %d, \"iteration_straight\": %d}\n",
+                second.currentTime(), straight.currentTime(), second.iteration(), straight.iteration());
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 4 && std::strcmp(argv[1], "mesh") == 0) {
         Mesh m = uniform_mesh(std::atoi(argv[2]), 0.0, 1.0);
@@ -91,6 +237,9 @@ int main(int argc, char** argv) {
                     (long long)m.numElements(), (long long)m.numFaces());
         return 0;
     }
+    if (argc >= 3 && std::strcmp(argv[1], "io") == 0) return io_case(argv[2]);
+    if (argc >= 7 && std::strcmp(argv[1], "resume") == 0)
+        return resume_case(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6]));
     if (argc < 9 || std::strcmp(argv[1], "run") != 0) {
         std::fprintf(stderr, "usage: %s mesh <n> <out> | run <vortex|tgv> <n> <p> <steps> <dt_max> <prec> <prefix>\n",
                      argv[0]);

@@ -86,3 +86,70 @@ def test_frsolver_tgv_fp32bulk_runs(tmp_path):
     _, _, U64 = _run(tmp_path, "tgv", 32, 4, 3, 0.0, "fp64")
     assert np.all(np.isfinite(U))
     assert np.linalg.norm(U - U64) / np.linalg.norm(U64) < 1e-5
+
+
+def _driver_mesh(n, x0, x1):
+    """The driver's uniform_mesh: nodes x0 + i h (i fastest), CCW quads."""
+    h = (x1 - x0) / n
+    xy = [(x0 + i * h, x0 + j * h) for j in range(n + 1) for i in range(n + 1)]
+    els = []
+    for j in range(n):
+        for i in range(n):
+            n0 = j * (n + 1) + i
+            els.append([n0, n0 + 1, n0 + n + 2, n0 + n + 1])
+    return xy, els
+
+
+def test_io_api(tmp_path):
+    """RestartIO, VTKWriter, ResidualWriter (C++ API, no GPU): restart round
+    trips, the reference's VTU / legacy / PVTU text byte for byte, binary
+    and interpolated VTU equal to the C ABI's output, residual CSV text."""
+    _need_driver()
+    from oracle import vtk_oracle as V
+    from zhijian_amd import io
+    r = subprocess.run([DRIVER, "io", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert all(info.values()), info
+    U = np.fromfile(tmp_path / "U.bin", dtype=np.float64)
+    xy, els = _driver_mesh(3, 0.0, 1.0)
+    E, S = len(els), 9
+    first = [tuple(U[e * S * 4:e * S * 4 + 4]) for e in range(E)]
+
+    rs = io.read_restart(tmp_path / "r.h5")
+    assert rs["solution"].tobytes() == U.tobytes() and np.all(rs["elem_n_sp"] == S)
+    assert rs["time"] == 0.75 and rs["iteration"] == 120 and rs["poly_order"] == 2 and rs["gamma"] == 1.4
+
+    for name, n, fn in (("a.vtu", 4, V.vtu_text), ("a.vtk", 4, V.legacy_text), ("par_0.vtu", 2, V.vtu_text)):
+        want = fn(*V.lattice(xy, els, [1] * E, [1] * E, first, n, 1.4))
+        assert (tmp_path / name).read_text() == want, name
+    pieces = [str(tmp_path / "par_0.vtu"), str(tmp_path / "par_1.vtu")]
+    assert (tmp_path / "par.pvtu").read_text() == V.pvtu_text(pieces)
+
+    m = io.VisMesh(np.array(xy), np.array(els), 1, 1)
+    for name, flags in (("b.vtu", io.BINARY), ("h.vtu", io.BINARY | io.INTERPOLATE)):
+        io.write_vtk(tmp_path / ("c_" + name), m, S, U, poly_order=2, subdivisions=4, flags=flags)
+        assert (tmp_path / name).read_bytes() == (tmp_path / ("c_" + name)).read_bytes(), name
+    assert (tmp_path / "b.vtu").read_bytes() != (tmp_path / "h.vtu").read_bytes()
+
+    want = ("# Iteration, Time, Residual, Rho_res, RhoU_res, RhoV_res, RhoE_res\n"
+            "10, %.8e, %.8e, %.8e, %.8e, %.8e, %.8e\n" % (0.5, 1.25e-3, 1.0, 2.5e-7, -3.0, 4.125) +
+            "20, %.8e, %.8e, %.8e, %.8e, %.8e, %.8e\n" % (1.0, 6.5e-4, 0, 0, 0, 0))
+    assert (tmp_path / "res.csv").read_text() == want
+
+
+@pytest.mark.gpu
+def test_frsolver_resume_from_restart_bit_exact(tmp_path):
+    """main()'s restart path completed: 6 steps, RestartIO::write, a new
+    solver from RestartIO::read + setSolution, 4 more steps == 10 steps
+    straight through, bit for bit (vortex 10 x 10, P3)."""
+    _need_driver()
+    print("driver: resume vortex n=10 P3 6+4 steps", flush=True)
+    r = subprocess.run([DRIVER, "resume", str(tmp_path), "10", "3", "6", "4"], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    a = np.fromfile(tmp_path / "resumed.bin", dtype=np.float64)
+    b = np.fromfile(tmp_path / "straight.bin", dtype=np.float64)
+    assert a.tobytes() == b.tobytes()
+    assert info["time"] == info["time_straight"] and info["iteration"] == info["iteration_straight"] == 10

@@ -56,6 +56,16 @@ class Params(C.Structure):
                 ("Pr", C.c_double), ("br2_eta", C.c_double)]
 
 
+class RestartInfo(C.Structure):
+    _fields_ = [("n_elem", C.c_int64), ("n_values", C.c_int64), ("time", C.c_double), ("iteration", C.c_int),
+                ("poly_order", C.c_int), ("gamma", C.c_double)]
+
+
+class VisMesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("xy", C.c_void_p), ("n_elem", C.c_int64), ("elem_type", C.c_void_p),
+                ("elem_order", C.c_void_p), ("elem_node_start", C.c_void_p), ("elem_nodes", C.c_void_p)]
+
+
 _lib = None
 
 # every exported symbol of include/zhijian_amd.h
@@ -74,7 +84,10 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_tri_create", "zj_tri_destroy", "zj_tri_last_error", "zj_tri_set_state", "zj_tri_get_state",
            "zj_tri_set_state_device", "zj_tri_get_state_device", "zj_tri_residual", "zj_tri_residual_device",
            "zj_tri_rk3_step", "zj_tri_dt_cfl", "zj_tri_residual_l2", "zj_tri_synchronize",
-           "zj_tri_set_xcd_groups", "zj_tri_get_info", "zj_tri_operators")
+           "zj_tri_set_xcd_groups", "zj_tri_get_info", "zj_tri_operators", "zj_get_scheme",
+           "zj_tri_get_scheme", "zj_io_last_error", "zj_restart_write", "zj_restart_query", "zj_restart_read",
+           "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
+           "zj_pvtu_write")
 
 
 def lib() -> C.CDLL:
@@ -159,6 +172,21 @@ def lib() -> C.CDLL:
         L.zj_tri_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]
         L.zj_tri_operators.argtypes = [C.c_int] + [C.c_void_p] * 7
+        L.zj_get_scheme.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.zj_tri_get_scheme.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.zj_io_last_error.argtypes = []
+        L.zj_io_last_error.restype = C.c_char_p
+        L.zj_restart_write.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                       C.c_int, C.c_double]
+        L.zj_restart_query.argtypes = [C.c_char_p, C.POINTER(RestartInfo)]
+        L.zj_restart_read.argtypes = [C.c_char_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+        for f in ("zj_restart_save", "zj_tri_restart_save"):
+            getattr(L, f).argtypes = [H, C.c_char_p, C.c_double, C.c_int]
+        for f in ("zj_restart_load", "zj_tri_restart_load"):
+            getattr(L, f).argtypes = [H, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.zj_vtk_write.argtypes = [C.c_char_p, C.POINTER(VisMesh), C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                   C.c_int, C.c_int]
+        L.zj_pvtu_write.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p)]
         _lib = L
     return _lib
 

@@ -707,6 +707,18 @@ void FRSolver::copyToDevice() {
     check(zj_set_state(handle_, U.data()), "copyToDevice");
 }
 
+void FRSolver::setSolution(const std::vector<ElementSolution>& solution, Real time, int iteration) {
+    const Index E = mesh_->numElements();
+    if ((Index)solution.size() != E) throw std::runtime_error("setSolution: element count differs from the mesh");
+    for (Index e = 0; e < E; ++e)
+        if ((int)solution[e].sol_pts.size() != n_sp_quad_)
+            throw std::runtime_error("setSolution: points per element differ from the solver's");
+    for (Index e = 0; e < E; ++e) solution_[e].sol_pts = solution[e].sol_pts;
+    copyToDevice();
+    time_ = time;
+    iter_ = iteration;
+}
+
 std::vector<Real> FRSolver::residualField() {
     std::vector<Real> R(mesh_->numElements() * n_sp_quad_ * N_VARS);
     check(zj_residual(handle_, R.data(), nullptr), "residualField");

@@ -746,6 +746,13 @@ extern "C" int zj_get_tiling(zj_handle h, int64_t* cross_tile_edges, int* reorde
     return ZJ_OK;
 }
 
+extern "C" int zj_get_scheme(zj_handle h, int* poly_order, double* gamma) {
+    if (!h) return ZJ_ERR_ARG;
+    if (poly_order) *poly_order = h->P;
+    if (gamma) *gamma = h->gamma;
+    return ZJ_OK;
+}
+
 extern "C" int zj_get_info(zj_handle h, int64_t* n_dof, int* epb, int* n_sp) {
     if (!h) return ZJ_ERR_ARG;
     if (n_dof) *n_dof = (int64_t)h->E * h->S * 4;

@@ -868,6 +868,13 @@ extern "C" int zj_tri_set_xcd_groups(zj_tri_handle h, int groups) {
     return ZJ_OK;
 }
 
+extern "C" int zj_tri_get_scheme(zj_tri_handle h, int* poly_order, double* gamma) {
+    if (!h) return ZJ_ERR_ARG;
+    if (poly_order) *poly_order = h->P;
+    if (gamma) *gamma = h->gamma;
+    return ZJ_OK;
+}
+
 extern "C" int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_block, int* n_sp,
                                int64_t* cross_tile_edges) {
     if (!h) return ZJ_ERR_ARG;

@@ -173,6 +173,16 @@ class Solver:
         self._chk(lib().zj_get_state(self._h, out))
         return out
 
+    def save_restart(self, path, time=0.0, iteration=0):
+        """Device state to a restart file (zj_restart_save)."""
+        from . import io
+        io.save_handle("quad", self._h, path, time, iteration)
+
+    def load_restart(self, path):
+        """Restart file to the device state (zj_restart_load); returns (time, iteration)."""
+        from . import io
+        return io.load_handle("quad", self._h, path)
+
     # device-pointer forms (ints from e.g. torch's data_ptr(); stream = hipStream_t or None)
     def set_state_device(self, ptr, stream=None):
         self._chk(lib().zj_set_state_device(self._h, ptr, stream))

@@ -104,6 +104,16 @@ class TriSolver:
         self._chk(lib().zj_tri_get_state(self._h, out))
         return out
 
+    def save_restart(self, path, time=0.0, iteration=0):
+        """Device state to a restart file (zj_tri_restart_save)."""
+        from . import io
+        io.save_handle("tri", self._h, path, time, iteration)
+
+    def load_restart(self, path):
+        """Restart file to the device state (zj_tri_restart_load); returns (time, iteration)."""
+        from . import io
+        return io.load_handle("tri", self._h, path)
+
     def set_state_device(self, ptr, stream=None):
         self._chk(lib().zj_tri_set_state_device(self._h, ptr, stream))
 
