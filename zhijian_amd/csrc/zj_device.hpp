@@ -137,69 +137,120 @@ __device__ void ghost_state(const R* Ui, R nx, R ny, int bct, const double* bc, 
 }
 
 // ------------------------------------------- textbook-mode Riemann solvers
+// The solvers of the reference's euler.hpp with every quotient a / b taken as
+// a * (1/b), each reciprocal computed once: FP64 a Newton-refined hardware
+// reciprocal (within 2 ulp of 1/b), FP32 the hardware reciprocal.  An IEEE
+// division costs about ten VALU instructions in either precision.
+__device__ __forceinline__ double rcp_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <typename R> __device__ __forceinline__ R recipR(R x);
+template <> __device__ __forceinline__ double recipR<double>(double x) { return rcp_refined(x); }
+template <> __device__ __forceinline__ float recipR<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// One side of a face: floored density rho = max(U0, 1e-10) and 1/rho, 1/U0
+// (the reference divides by the unfloored density in pressure and fluxNormal),
+// velocity from the floored density, pressure from the unfloored one
+// (IdealGas::pressure, euler.hpp:16-23).
+template <typename R>
+struct Side {
+    R rho, ir, i0, u, v, p;
+};
+template <typename R>
+__device__ __forceinline__ Side<R> side_of(const R* U, R g) {
+    Side<R> s;
+    s.rho = fmaxR(U[0], R(1e-10));
+    s.ir = recipR(s.rho);
+    s.i0 = U[0] == s.rho ? s.ir : recipR(U[0]);
+    s.u = U[1] * s.ir;
+    s.v = U[2] * s.ir;
+    const R u0 = U[1] * s.i0, v0 = U[2] * s.i0;
+    s.p = (g - R(1)) * U[0] * (U[3] * s.i0 - R(0.5) * (u0 * u0 + v0 * v0));
+    return s;
+}
 // IdealGas::fluxNormal with its floors (euler.hpp:133-147).
 template <typename R>
+__device__ __forceinline__ void flux_normal_floor(const R* U, const Side<R>& s, R nx, R ny, R* Fn) {
+    const R p = fmaxR(s.p, R(1e-10));
+    const R vn = s.u * nx + s.v * ny;
+    Fn[0] = s.rho * vn;
+    Fn[1] = s.rho * s.u * vn + p * nx;
+    Fn[2] = s.rho * s.v * vn + p * ny;
+    Fn[3] = (U[3] + s.p) * vn;
+}
+template <typename R>
 __device__ __forceinline__ void flux_normal_floor(const R* U, R nx, R ny, R g, R* Fn) {
-    const R rho = fmaxR(U[0], R(1e-10));
-    const R u = U[1] / rho, v = U[2] / rho;
-    const R pr = (g - R(1)) * U[0] * (U[3] / U[0] - R(0.5) * ((U[1] / U[0]) * (U[1] / U[0]) +
-                                                                (U[2] / U[0]) * (U[2] / U[0])));
-    const R p = fmaxR(pr, R(1e-10));
-    const R vn = u * nx + v * ny;
-    Fn[0] = rho * vn;
-    Fn[1] = rho * u * vn + p * nx;
-    Fn[2] = rho * v * vn + p * ny;
-    Fn[3] = (U[3] + pr) * vn;
+    flux_normal_floor(U, side_of(U, g), nx, ny, Fn);
 }
+// IdealGas::soundSpeed (euler.hpp:32-41)
 template <typename R>
-__device__ __forceinline__ R pressure_of(const R* U, R g) {
-    const R rho = U[0], u = U[1] / rho, v = U[2] / rho;
-    return (g - R(1)) * rho * (U[3] / rho - R(0.5) * (u * u + v * v));
+__device__ __forceinline__ R sound_speed_of(const Side<R>& s, R g) {
+    return sqrtR(g * fmaxR(s.p, R(1e-10)) * s.ir);
 }
+// RiemannSolvers::rusanov (euler.hpp:184-200); its velocities use the unfloored
+// density.  One side at a time: F holds F_L until F_R is known.
 template <typename R>
-__device__ __forceinline__ R sound_speed_of(const R* U, R g) {
-    return sqrtR(g * fmaxR(pressure_of(U, g), R(1e-10)) / fmaxR(U[0], R(1e-10)));
+__device__ __forceinline__ R rusanov_side(const R* U, R nx, R ny, R g, R* Fn) {
+    const Side<R> s = side_of(U, g);
+    flux_normal_floor(U, s, nx, ny, Fn);
+    const R u = U[1] * s.i0, v = U[2] * s.i0;
+    return sqrtR(u * u + v * v) + sound_speed_of(s, g);
 }
-// RiemannSolvers::rusanov (euler.hpp:184-200)
 template <typename R>
 __device__ __forceinline__ void riemann_rusanov(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
-    R FL[NV], FR[NV];
-    flux_normal_floor(UL, nx, ny, g, FL);
-    flux_normal_floor(UR, nx, ny, g, FR);
-    const R uL = UL[1] / UL[0], vL = UL[2] / UL[0], uR = UR[1] / UR[0], vR = UR[2] / UR[0];
-    const R sL = sqrtR(uL * uL + vL * vL) + sound_speed_of(UL, g);
-    const R sR = sqrtR(uR * uR + vR * vR) + sound_speed_of(UR, g);
+    const R sL = rusanov_side(UL, nx, ny, g, F);
+    R FR[NV];
+    const R sR = rusanov_side(UR, nx, ny, g, FR);
     const R smax = fmaxR(sL, sR);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (FL[i] + FR[i]) - R(0.5) * smax * (UR[i] - UL[i]);
+    for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (F[i] + FR[i]) - R(0.5) * smax * (UR[i] - UL[i]);
 }
-// RiemannSolvers::roe with Harten's entropy fix (euler.hpp:204-294)
+// RiemannSolvers::roe with Harten's entropy fix (euler.hpp:204-294).  The
+// average of the two sides' normal fluxes goes into F first, so that neither
+// side's state stays live through the wave decomposition.
 template <typename R>
 __device__ __forceinline__ void riemann_roe(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
-    const R rhoL = fmaxR(UL[0], R(1e-10)), rhoR = fmaxR(UR[0], R(1e-10));
-    const R uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
-    const R pL = pressure_of(UL, g), pR = pressure_of(UR, g);
-    const R HL = (UL[3] + pL) / rhoL, HR = (UR[3] + pR) / rhoR;
+    R rhoL, rhoR, uL, vL, uR, vR, pL, pR, HL, HR;
+    {
+        const Side<R> L = side_of(UL, g);
+        flux_normal_floor(UL, L, nx, ny, F);
+        rhoL = L.rho; uL = L.u; vL = L.v; pL = L.p;
+        HL = (UL[3] + pL) * L.ir;
+    }
+    {
+        const Side<R> Rt = side_of(UR, g);
+        R FR[NV];
+        flux_normal_floor(UR, Rt, nx, ny, FR);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (F[i] + FR[i]);
+        rhoR = Rt.rho; uR = Rt.u; vR = Rt.v; pR = Rt.p;
+        HR = (UR[3] + pR) * Rt.ir;
+    }
     const R sL = sqrtR(rhoL), sR = sqrtR(rhoR);
-    const R den = fmaxR(sL + sR, R(1e-10));
+    const R iden = recipR(fmaxR(sL + sR, R(1e-10)));
     const R rr = sL * sR;
-    const R ur = (sL * uL + sR * uR) / den, vr = (sL * vL + sR * vR) / den, Hr = (sL * HL + sR * HR) / den;
+    const R ur = (sL * uL + sR * uR) * iden, vr = (sL * vL + sR * vR) * iden, Hr = (sL * HL + sR * HR) * iden;
     const R vnr = ur * nx + vr * ny, q2 = ur * ur + vr * vr;
     const R c = sqrtR(fmaxR((g - R(1)) * (Hr - R(0.5) * q2), R(1e-10)));
     R l1 = fabs(vnr - c), l4 = fabs(vnr + c);
     const R l2 = fabs(vnr);
     const R eps = R(0.1) * c;
-    if (l1 < eps) l1 = (l1 * l1 + eps * eps) / (R(2) * eps);
-    if (l4 < eps) l4 = (l4 * l4 + eps * eps) / (R(2) * eps);
+    if (l1 < eps || l4 < eps) {
+        const R i2e = recipR(R(2) * eps);
+        if (l1 < eps) l1 = (l1 * l1 + eps * eps) * i2e;
+        if (l4 < eps) l4 = (l4 * l4 + eps * eps) * i2e;
+    }
     const R du = uR - uL, dv = vR - vL, dp = pR - pL, dvn = du * nx + dv * ny;
-    const R c2 = fmaxR(c * c, R(1e-10));
-    const R a1 = (dp - rr * c * dvn) / (R(2) * c2);
-    const R a2 = (rhoR - rhoL) - dp / c2;
+    const R ic2 = recipR(fmaxR(c * c, R(1e-10)));
+    const R i2c2 = R(0.5) * ic2;
+    const R a1 = (dp - rr * c * dvn) * i2c2;
+    const R a2 = (rhoR - rhoL) - dp * ic2;
     const R a3 = rr * (dv * nx - du * ny);
-    const R a4 = (dp + rr * c * dvn) / (R(2) * c2);
-    R FL[NV], FR[NV];
-    flux_normal_floor(UL, nx, ny, g, FL);
-    flux_normal_floor(UR, nx, ny, g, FR);
+    const R a4 = (dp + rr * c * dvn) * i2c2;
     const R tx = -ny, ty = nx;
     const R w1 = l1 * a1, w2 = l2 * a2, w3 = l2 * a3, w4 = l4 * a4;
     const R dF[NV] = {w1 + w2 + w4,
@@ -207,40 +258,52 @@ __device__ __forceinline__ void riemann_roe(const R* UL, const R* UR, R nx, R ny
                       w1 * (vr - c * ny) + w2 * vr + w3 * ty + w4 * (vr + c * ny),
                       w1 * (Hr - c * vnr) + w2 * R(0.5) * q2 + w3 * (ur * tx + vr * ty) + w4 * (Hr + c * vnr)};
 #pragma unroll
-    for (int i = 0; i < NV; ++i) F[i] = R(0.5) * (FL[i] + FR[i]) - R(0.5) * dF[i];
+    for (int i = 0; i < NV; ++i) F[i] = F[i] - R(0.5) * dF[i];
 }
 template <typename R>
 __device__ __forceinline__ R guard_den(R d) {
     return fabs(d) < R(1e-10) ? (d >= R(0) ? R(1e-10) : R(-1e-10)) : d;
 }
-// RiemannSolvers::hllc (euler.hpp:298-375)
+// RiemannSolvers::hllc (euler.hpp:298-375).  The wave estimates need a few
+// scalars of each side; the side whose flux is taken is rebuilt from its state
+// afterwards instead of keeping both sides' states live.
 template <typename R>
 __device__ __forceinline__ void riemann_hllc(const R* UL, const R* UR, R nx, R ny, R g, R* F) {
-    const R rhoL = fmaxR(UL[0], R(1e-10)), rhoR = fmaxR(UR[0], R(1e-10));
-    const R uL = UL[1] / rhoL, vL = UL[2] / rhoL, uR = UR[1] / rhoR, vR = UR[2] / rhoR;
-    const R pL = fmaxR(pressure_of(UL, g), R(1e-10)), pR = fmaxR(pressure_of(UR, g), R(1e-10));
-    const R cL = sound_speed_of(UL, g), cR = sound_speed_of(UR, g);
-    const R vnL = uL * nx + vL * ny, vnR = uR * nx + vR * ny;
+    R rhoL, rhoR, vnL, vnR, pL, pR, cL, cR;
+    {
+        const Side<R> L = side_of(UL, g);
+        rhoL = L.rho; pL = fmaxR(L.p, R(1e-10)); cL = sound_speed_of(L, g); vnL = L.u * nx + L.v * ny;
+    }
+    {
+        const Side<R> Rt = side_of(UR, g);
+        rhoR = Rt.rho; pR = fmaxR(Rt.p, R(1e-10)); cR = sound_speed_of(Rt, g); vnR = Rt.u * nx + Rt.v * ny;
+    }
     R ps = R(0.5) * (pL + pR) - R(0.5) * (vnR - vnL) * R(0.5) * (rhoL + rhoR) * R(0.5) * (cL + cR);
     ps = fmaxR(ps, R(1e-10));
-    const R gp = (g + R(1)) / (R(2) * g);
-    const R qL = ps <= pL ? R(1) : sqrtR(R(1) + gp * (ps / pL - R(1)));
-    const R qR = ps <= pR ? R(1) : sqrtR(R(1) + gp * (ps / pR - R(1)));
+    const R gp = (g + R(1)) * recipR(R(2) * g);
+    const R qL = ps <= pL ? R(1) : sqrtR(R(1) + gp * (ps * recipR(pL) - R(1)));
+    const R qR = ps <= pR ? R(1) : sqrtR(R(1) + gp * (ps * recipR(pR) - R(1)));
     const R SL = vnL - cL * qL, SR = vnR + cR * qR;
     const R dSM = guard_den(rhoL * (SL - vnL) - rhoR * (SR - vnR));
-    const R SM = (pR - pL + rhoL * vnL * (SL - vnL) - rhoR * vnR * (SR - vnR)) / dSM;
-    if (SL >= R(0)) { flux_normal_floor(UL, nx, ny, g, F); return; }
-    if (SR <= R(0)) { flux_normal_floor(UR, nx, ny, g, F); return; }
-    const bool left = SM >= R(0);
+    const R SM = (pR - pL + rhoL * vnL * (SL - vnL) - rhoR * vnR * (SR - vnR)) * recipR(dSM);
+    // the upwind side: left if SL >= 0, right if SR <= 0, else the side of the contact
+    const bool plain = SL >= R(0) || SR <= R(0);
+    const bool left = plain ? SL >= R(0) : SM >= R(0);
     const R* U = left ? UL : UR;
-    const R rho = left ? rhoL : rhoR, u = left ? uL : uR, v = left ? vL : vR;
-    const R vn = left ? vnL : vnR, S = left ? SL : SR, p = left ? pL : pR;
-    const R dK = guard_den(S - SM), dvn = guard_den(S - vn);
-    const R fac = rho * dvn / dK;
-    const R Us[NV] = {fac, fac * (u + (SM - vn) * nx), fac * (v + (SM - vn) * ny),
-                      fac * (U[3] / rho + (SM - vn) * (SM + p / (rho * dvn)))};
+    const Side<R> K = side_of(U, g);
     R Fk[NV];
-    flux_normal_floor(U, nx, ny, g, Fk);
+    flux_normal_floor(U, K, nx, ny, Fk);
+    if (plain) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) F[i] = Fk[i];
+        return;
+    }
+    const R vn = left ? vnL : vnR, S = left ? SL : SR, p = left ? pL : pR;
+    const R rho = K.rho, u = K.u, v = K.v;
+    const R dK = guard_den(S - SM), dvn = guard_den(S - vn);
+    const R fac = rho * dvn * recipR(dK);
+    const R Us[NV] = {fac, fac * (u + (SM - vn) * nx), fac * (v + (SM - vn) * ny),
+                      fac * (U[3] * K.ir + (SM - vn) * (SM + p * recipR(rho * dvn)))};
 #pragma unroll
     for (int i = 0; i < NV; ++i) F[i] = Fk[i] + S * (Us[i] - U[i]);
 }
@@ -255,13 +318,6 @@ __device__ __forceinline__ void riemann_by_kind(int kind, const R* UL, const R* 
 // FAST (the FP32-bulk form): the two divisions become a Newton-refined
 // hardware reciprocal and a multiplication by 1/(gamma-1), within 2 ulp of
 // the divisions, far below the form's FP32 rounding.
-__device__ __forceinline__ double rcp_refined(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
 template <bool FAST = false>
 __device__ __forceinline__ void positivity(double* q) {
     constexpr double GM1 = 1.4 - 1.0;

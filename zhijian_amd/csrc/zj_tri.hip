@@ -129,8 +129,13 @@ template <typename R> __device__ __forceinline__ R fast_rcp(R x);
 template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return rcp_refined(x); }
 template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Waves per SIMD the stage kernel is compiled for: the FP64 P3 kernel fits
+// three (at most 168 VGPRs) without scratch; the rest is left to the compiler.
+template <typename R, int P> struct TriWaves { static constexpr int value = 1; };
+template <> struct TriWaves<double, 3> { static constexpr int value = 3; };
+
 template <typename R, int P, int MODE>
-__global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
+__global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kernel(TriArgs<R> a) {
     using S = TriShape<P>;
     constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
     constexpr int NW = NP > NFP ? NP : NFP;
@@ -154,14 +159,11 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
 
     // ---- 0: this lane's variable at every solution point (one coalesced
     // 512-byte row per point and wave), kept for the stage update
-    R nrm[6];
     if (live) {
 #pragma unroll
         for (int m = 0; m < NP; ++m) u[m] = a.U_in[base + (int64_t)m * EB * NV];
         const R* G = a.geo + (int64_t)e * GEO;
         rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) nrm[k] = G[4 + k];
 #pragma unroll
         for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
 #pragma unroll
@@ -224,8 +226,8 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(TriArgs<R> a) {
             const int kk = j / FP, f = j - kk * FP;
             const int2 Lk = kk == 0 ? L[0] : (kk == 1 ? L[1] : L[2]);
             const int role = link_role(Lk.y);
-            const R nx = kk == 0 ? nrm[0] : (kk == 1 ? nrm[2] : nrm[4]);
-            const R ny = kk == 0 ? nrm[1] : (kk == 1 ? nrm[3] : nrm[5]);
+            // the edge normal is read here rather than kept from phase 0 (registers)
+            const R nx = a.geo[(int64_t)e * GEO + 4 + 2 * kk], ny = a.geo[(int64_t)e * GEO + 5 + 2 * kk];
             R own[NV], oth[NV], UL[NV], UR[NV], Fc[NV];
 #pragma unroll
             for (int w = 0; w < NV; ++w) own[w] = sT[(el * NFP + j) * NV + w];
