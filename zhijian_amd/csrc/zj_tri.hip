@@ -144,6 +144,9 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
     __shared__ R sW[TILE * NW * NV];   // the stage input state, then the transformed normal fluxes,
                                        // then the flux differences
     __shared__ double sred[BLOCK / 64];
+    __shared__ R sEdge[TILE * 3];                // |a| of every element edge of the tile
+    __shared__ unsigned char sList[TILE * 3];    // edges whose face points this tile solves (el * 3 + edge)
+    __shared__ int sCnt[BLOCK / 64];             // such edges per wave
 
     const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
     const int tile = tile_of_block(a.xcd_groups);
@@ -168,7 +171,23 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
         for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
 #pragma unroll
         for (int m = 0; m < NP; ++m) sW[(el * NP + m) * NV + v] = R(u[m]);
+        if (v < 3) {  // a: the metric gradient normal to edge v (-Fs on edge 0, Fr + Fs on 1, -Fr on 2)
+            const R ax = v == 0 ? sx : (v == 1 ? rx + sx : rx), ay = v == 0 ? sy : (v == 1 ? ry + sy : ry);
+            sEdge[el * 3 + v] = sqrtR(ax * ax + ay * ay);
+        }
     }
+    // Lane v < 3 of an element stands for its edge v.  An in-tile face is
+    // solved by its left element only; an edge on a boundary or shared with
+    // another tile by its own element.
+    bool owned = false;
+    if (live && v < 3) {
+        const int2 Lv = v == 0 ? L[0] : (v == 1 ? L[1] : L[2]);
+        const int nl = Lv.x - e0;
+        owned = !(link_role(Lv.y) == ROLE_RIGHT && nl >= 0 && nl < TILE);
+    }
+    const unsigned long long own_mask = __ballot(owned);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) sCnt[wave] = __popcll(own_mask);
     __syncthreads();
 
     // ---- 1: fluxes, divergence, traces and interpolated normal fluxes, as
@@ -210,6 +229,11 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
 #pragma unroll
         for (int j = 0; j < NFP; ++j) sT[(el * NFP + j) * NV + v] = ut[j];
     }
+    if (owned) {  // compact the solved edges, in element order
+        int pos = __popcll(own_mask & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) pos += sCnt[w];
+        sList[pos] = (unsigned char)(el * 3 + v);
+    }
     __syncthreads();  // traces in place, state reads done
     // transformed normal fluxes to their flux points' slots in sW:
     // -Fs on edge 0, Fr + Fs on edge 1, -Fr on edge 2
@@ -219,40 +243,56 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
     }
     __syncthreads();
 
-    // ---- 2: common fluxes; lane v of the quad takes flux points v, v+4, ...
-    if (live) {
+    // ---- 2: common fluxes over the tile's solved edges, spread evenly over
+    // the workgroup: the flux difference sc F^I - Fn of this side, and of the
+    // in-tile neighbour's side with the same common flux (both elements store
+    // the face normal of the left element, so the neighbour would have solved
+    // the same problem).  sc = +-|a|, negative on the right.
+    {
+        int nsolved = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) nsolved += sCnt[w];
 #pragma unroll 1
-        for (int j = v; j < NFP; j += NV) {
-            const int kk = j / FP, f = j - kk * FP;
-            const int2 Lk = kk == 0 ? L[0] : (kk == 1 ? L[1] : L[2]);
+        for (int t = tid; t < nsolved * FP; t += BLOCK) {
+            const int c = sList[t / FP], f = t - (t / FP) * FP;
+            const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
+            const int2 Lk = a.links[(int64_t)(e0 + elt) * 3 + kk];
             const int role = link_role(Lk.y);
-            // the edge normal is read here rather than kept from phase 0 (registers)
-            const R nx = a.geo[(int64_t)e * GEO + 4 + 2 * kk], ny = a.geo[(int64_t)e * GEO + 5 + 2 * kk];
+            const R nx = a.geo[(int64_t)(e0 + elt) * GEO + 4 + 2 * kk];
+            const R ny = a.geo[(int64_t)(e0 + elt) * GEO + 5 + 2 * kk];
             R own[NV], oth[NV], UL[NV], UR[NV], Fc[NV];
 #pragma unroll
-            for (int w = 0; w < NV; ++w) own[w] = sT[(el * NFP + j) * NV + w];
+            for (int w = 0; w < NV; ++w) own[w] = sT[(elt * NFP + j) * NV + w];
+            int nl = -1, oj = 0;
             if (role == ROLE_BOUNDARY) {
                 ghost_state(own, nx, ny, link_edge(Lk.y), a.bc_data + (int64_t)link_bc(Lk.y) * 8, g, oth);
             } else {
-                const int nl = Lk.x - e0;
-                const int oj = link_edge(Lk.y) * FP + (P - f);
+                nl = Lk.x - e0;
+                oj = link_edge(Lk.y) * FP + (P - f);
                 if (nl >= 0 && nl < TILE) {
 #pragma unroll
                     for (int w = 0; w < NV; ++w) oth[w] = sT[(nl * NFP + oj) * NV + w];
                 } else {
                     load4(a.T_in + trix(Lk.x, oj, NFP), oth);
+                    nl = -1;
                 }
             }
             const bool right = role == ROLE_RIGHT;
 #pragma unroll
             for (int w = 0; w < NV; ++w) { UL[w] = right ? oth[w] : own[w]; UR[w] = right ? own[w] : oth[w]; }
             riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
-            const R ax = kk == 0 ? sx : (kk == 1 ? rx + sx : rx);
-            const R ay = kk == 0 ? sy : (kk == 1 ? ry + sy : ry);
-            const R sc = right ? -sqrtR(ax * ax + ay * ay) : sqrtR(ax * ax + ay * ay);
-            R* d = sW + (el * NFP + j) * NV;  // flux difference F^I s - Fn in place
+            {
+                const R sc = right ? -sEdge[c] : sEdge[c];
+                R* d = sW + (elt * NFP + j) * NV;
 #pragma unroll
-            for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
+                for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
+            }
+            if (nl >= 0) {  // the in-tile neighbour, whose side is the right one
+                const R sc = -sEdge[nl * 3 + link_edge(Lk.y)];
+                R* d = sW + (nl * NFP + oj) * NV;
+#pragma unroll
+                for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
+            }
         }
     }
     __syncthreads();
