@@ -69,8 +69,20 @@ constexpr int TILE = BLOCK / NV;  // elements per workgroup (one lane per elemen
 constexpr int EB = 16;            // elements per block of the state and trace layouts
 constexpr int GEO = 10;           // geometry words per element
 
+// Operator tables of order P in constant memory, laid out for wide scalar
+// loads in the kernels' loops:
+//   A  for each solution point m: column m of Dr, of Ds (NP each) and of the
+//      interpolation to the flux points (NFP), contiguous -- the point loop of
+//      the volume phase reads one such block per iteration;
+//   B  the lifting matrix by columns: column j (NP values) contiguous;
+//   C  the interpolation matrix by rows (row j: NP values), for the traces.
+template <int P> struct TriOps {
+    static constexpr int NP = (P + 1) * (P + 2) / 2, NFP = 3 * (P + 1);
+    static constexpr int CA = 2 * NP + NFP;         // words per block of A
+    static constexpr int B = NP * CA, C = B + NFP * NP;
+};
 constexpr int ops_words(int P) {
-    return 2 * ((P + 1) * (P + 2) / 2) * ((P + 1) * (P + 2) / 2) + 2 * 3 * (P + 1) * ((P + 1) * (P + 2) / 2);
+    return 2 * ((P + 1) * (P + 2) / 2) * ((P + 1) * (P + 2) / 2) + 3 * 3 * (P + 1) * ((P + 1) * (P + 2) / 2);
 }
 constexpr int ops_offset(int P) {
     int o = 0;
@@ -139,7 +151,8 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
     using S = TriShape<P>;
     constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
     constexpr int NW = NP > NFP ? NP : NFP;
-    constexpr int ODR = ops_offset(P), ODS = ODR + NP * NP, OI = ODS + NP * NP, OL = OI + NFP * NP;
+    using OP = TriOps<P>;
+    constexpr int O0 = ops_offset(P), CA = OP::CA, OLT = O0 + OP::B, OI = O0 + OP::C;
     __shared__ R sT[TILE * NFP * NV];  // own traces
     __shared__ R sW[TILE * NW * NV];   // the stage input state, then the transformed normal fluxes,
                                        // then the flux differences
@@ -211,10 +224,10 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
             const R fr = rx * fx + ry * fy, fs = sx * fx + sy * fy, frs = fr + fs;
 #pragma unroll
             for (int i = 0; i < NP; ++i)
-                div[i] = fmaR(cop<R>(ODS + i * NP + m), fs, fmaR(cop<R>(ODR + i * NP + m), fr, div[i]));
+                div[i] = fmaR(cop<R>(O0 + m * CA + NP + i), fs, fmaR(cop<R>(O0 + m * CA + i), fr, div[i]));
 #pragma unroll
             for (int j = 0; j < NFP; ++j)
-                fn[j] = fmaR(cop<R>(OI + j * NP + m), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+                fn[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
         }
         // traces in a second pass, so that they and the fluxes' sums are not live together
         R ut[NFP];
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
         for (int m = 0; m < NP; ++m) {
             const R um = sW[(el * NP + m) * NV + v];
 #pragma unroll
-            for (int j = 0; j < NFP; ++j) ut[j] = fmaR(cop<R>(OI + j * NP + m), um, ut[j]);
+            for (int j = 0; j < NFP; ++j) ut[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), um, ut[j]);
         }
 #pragma unroll
         for (int j = 0; j < NFP; ++j) sT[(el * NFP + j) * NV + v] = ut[j];
@@ -305,7 +318,7 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
         for (int j = 0; j < NFP; ++j) {
             const R d = sW[(el * NFP + j) * NV + v];
 #pragma unroll
-            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OL + i * NFP + j), d, div[i]);
+            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OLT + j * NP + i), d, div[i]);
         }
         const double dt = MODE == MODE_RESIDUAL ? 0.0 : *a.dt_ptr;
 #pragma unroll
@@ -353,7 +366,7 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
 template <typename R, int P>
 __global__ __launch_bounds__(BLOCK) void tri_trace_kernel(int E, const double* U, R* T_out) {
     using S = TriShape<P>;
-    constexpr int NP = S::NP, NFP = S::NFP, OI = ops_offset(P) + 2 * NP * NP;
+    constexpr int NP = S::NP, NFP = S::NFP, OI = ops_offset(P) + TriOps<P>::C;
     const int e = blockIdx.x * TILE + (threadIdx.x >> 2), v = threadIdx.x & 3;
     if (e >= E) return;
     R u[NP];
@@ -470,11 +483,18 @@ hipError_t load_tri_ops(int device) {
         tri::Operators O;
         if (!tri::operators(P, O)) return hipErrorInvalidValue;
         double* d = all.data() + ops_offset(P);
-        const int np = O.np, nf = O.nf;
-        std::copy(O.Dr.begin(), O.Dr.end(), d);
-        std::copy(O.Ds.begin(), O.Ds.end(), d + np * np);
-        std::copy(O.interp.begin(), O.interp.end(), d + 2 * np * np);
-        std::copy(O.lift.begin(), O.lift.end(), d + 2 * np * np + nf * np);
+        const int np = O.np, nf = O.nf, ca = 2 * np + nf;   // TriOps<P> layout
+        for (int m = 0; m < np; ++m) {
+            for (int i = 0; i < np; ++i) {
+                d[m * ca + i] = O.Dr[(size_t)i * np + m];
+                d[m * ca + np + i] = O.Ds[(size_t)i * np + m];
+            }
+            for (int jj = 0; jj < nf; ++jj) d[m * ca + 2 * np + jj] = O.interp[(size_t)jj * np + m];
+        }
+        double* b = d + (size_t)np * ca;
+        for (int jj = 0; jj < nf; ++jj)
+            for (int i = 0; i < np; ++i) b[(size_t)jj * np + i] = O.lift[(size_t)i * nf + jj];
+        std::copy(O.interp.begin(), O.interp.end(), b + (size_t)nf * np);
     }
     std::vector<float> allf(all.begin(), all.end());
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops64), all.data(), all.size() * sizeof(double), 0,
