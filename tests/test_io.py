@@ -27,8 +27,8 @@ from zhijian_amd._lib import ZJError
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 CONDA_PY = "/opt/conda/bin/python3.9"
-HAVE_H5PY = os.path.exists(CONDA_PY) and subprocess.run(
-    [CONDA_PY, "-c", "import h5py"], capture_output=True).returncode == 0 if os.path.exists(CONDA_PY) else False
+HAVE_H5PY = os.path.exists(CONDA_PY) and subprocess.run([CONDA_PY, "-c", "import h5py"],
+                                                         capture_output=True).returncode == 0
 
 
 def _sample(n_elem, seed):
@@ -340,3 +340,19 @@ def test_vtk_errors(tmp_path):
         io.write_vtk(tmp_path / "x.vtu", bad, [4], sol[:16], poly_order=1)
     with pytest.raises(ZJError, match="Failed to open"):
         io.write_vtk(tmp_path / "nodir" / "x.vtu", m, nsp, sol, poly_order=1)
+
+
+def test_restart_attribute_and_name_limits(tmp_path):
+    """Argument checks of zj_restart_write: a solution that does not match
+    elem_n_sp, a negative element count, negative points per element."""
+    import ctypes as C
+    from zhijian_amd import _lib
+    nsp, sol = _sample(3, 9)
+    with pytest.raises(ValueError):
+        io.write_restart(tmp_path / "x.h5", nsp, sol[:-1])
+    rc = _lib.lib().zj_restart_write(str(tmp_path / "y.h5").encode(), -1, None, None, 0.0, 0, 1, 1.4)
+    assert rc != 0 and b"elem_n_sp" in _lib.lib().zj_io_last_error()
+    bad = np.array([4, -1, 4], np.int32)
+    rc = _lib.lib().zj_restart_write(str(tmp_path / "z.h5").encode(), 3, bad.ctypes.data, sol.ctypes.data,
+                                     0.0, 0, 1, 1.4)
+    assert rc != 0 and b"negative" in _lib.lib().zj_io_last_error()

@@ -221,8 +221,13 @@ bool write_file(const std::string& path, const std::vector<Attribute>& attrs, co
     }
     std::set<std::string> aseen;
     for (const auto& a : attrs) {
-        if (a.name.empty() || a.name.size() > 60000 || !aseen.insert(a.name).second) {
+        if (a.name.empty() || !aseen.insert(a.name).second) {
             err = "invalid or duplicate attribute name '" + a.name + "'";
+            return false;
+        }
+        // a version-1 object header message records its size in 16 bits
+        if (align8(attribute_body(a).size()) > 0xFFF8) {
+            err = "attribute '" + a.name + "' is too large for a compact attribute message";
             return false;
         }
     }
@@ -741,6 +746,10 @@ bool Reader::open(const std::string& path, std::string& err) {
             if (sc.bad) { err = "bad symbol table message"; return false; }
             have_stab = true;
         } else if (m.type == MSG_ATTR) {
+            if (m.flags & 0x02) {
+                err = "shared attribute messages are not supported";
+                return false;
+            }
             std::string name;
             RawAttr a;
             if (!ReaderImpl::parse_attr(m, name, a, err)) return false;
