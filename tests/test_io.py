@@ -140,6 +140,26 @@ def test_restart_errors(tmp_path):
         io.write_restart(tmp_path / "no_dir" / "x.h5", nsp, sol)
 
 
+def test_restart_claimed_size_bounded_by_file(tmp_path):
+    """A dataspace and layout that claim more data than the file holds are
+    refused before any caller sizes an array from them."""
+    nsp, sol = _sample(4, 8)
+    good = tmp_path / "g.h5"
+    io.write_restart(good, nsp, sol, 0.0, 1, 2, 1.4)
+    raw = bytearray(good.read_bytes())
+    n = sol.size
+    # the solution dataset's dims and max dims, and its layout size, all say n doubles
+    old_dims = np.array([n, n], np.uint64).tobytes()
+    old_size = np.array([8 * n], np.uint64).tobytes()
+    assert raw.count(old_dims) == 1 and raw.count(old_size) >= 1
+    huge = 1 << 40
+    raw = raw.replace(old_dims, np.array([huge, huge], np.uint64).tobytes())
+    raw = raw.replace(old_size, np.array([8 * huge], np.uint64).tobytes(), 1)
+    (tmp_path / "b.h5").write_bytes(bytes(raw))
+    with pytest.raises(ZJError, match="past the end|smaller than"):
+        io.read_restart(tmp_path / "b.h5")
+
+
 def test_restart_corrupt_bytes_never_crash(tmp_path):
     """Flipped bytes anywhere in the metadata: an error or a successful read,
     never a crash (the reader bounds-checks every access)."""

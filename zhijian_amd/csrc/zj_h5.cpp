@@ -860,6 +860,11 @@ bool Reader::dataset(const std::string& name, DatasetInfo& info, std::string& er
         err = "dataset '" + name + "': stored size smaller than its dataspace";
         return false;
     }
+    // the data must lie inside the file, which also bounds what a caller sizes from count
+    if (info.count && !info.compact && (info.addr > file_size_ || need > file_size_ - info.addr)) {
+        err = "dataset '" + name + "': data extends past the end of the file";
+        return false;
+    }
     return true;
 }
 
