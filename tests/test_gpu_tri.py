@@ -129,3 +129,18 @@ def test_tri_fp32bulk_steps():
     r = rel(b.get_state(), a.get_state())
     say(f"  FP32-bulk vs FP64 after 50 steps: rel L2 {r:.3e}")
     assert r < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_tri_tiny_periodic_meshes(n):
+    """1x1 and 2x2 periodic meshes: two triangles share several faces (the
+    in-tile face points are solved once, by each face's left element, for
+    both sides)."""
+    say(f"case: triangle residual, periodic {n}x{n} P2 hllc and roe")
+    for riemann in ("hllc", "roe"):
+        s, o, m = _periodic(n, 2, 0.0, 0, riemann)
+        U = _state(m, 2, 3 + n)
+        s.set_state(U)
+        r = rel(s.residual(), o.residual(U))
+        say(f"  {riemann}: rel L2 {r:.3e}")
+        assert r < TOL
