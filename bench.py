@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Residual DOF-updates/s of the FR residual + SSP-RK3 step on one MI355X.
 
-Workload (BASELINE config "2D Taylor-Green vortex"): periodic [0,2pi]^2,
+Workload (the 2D Taylor-Green vortex benchmark): periodic [0,2pi]^2,
 256x256 uniform quads, P4 (25 solution points/element, 1,638,400 points,
 6,553,600 DOF), Mach 0.1, the reference's executed arithmetic (DG correction,
 inline Rusanov, inviscid), SSP-RK3 with the reference's positivity limiter, dt

@@ -19,7 +19,8 @@ LIB = os.path.join(PKG, "libzhijian_amd.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# packed FP32 instructions stay disabled on this target
+# packed FP32 instructions (v_pk_*_f32) stay disabled: some of their forms have given
+# silent wrong results on this hardware
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-function"]
