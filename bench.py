@@ -484,7 +484,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU owning the whole mesh; --gpus {args.gpus} "
-              f"(WORLD_SIZE={world}) is not supported by this build", file=sys.stderr)
+              f"(WORLD_SIZE={world}) is not supported", file=sys.stderr)
         sys.exit(2)
 
     from zhijian_amd import _lib, cases, hiprt, solver

@@ -63,7 +63,7 @@ public:
     void computeDerivedQuantities(std::vector<std::vector<Real>>& pressure,
                                   std::vector<std::vector<Real>>& mach) const;
 
-    // ---- this build
+    // ---- additions of this library
     // Residual dU/dt of the current state, AoS [E][S][4] on the host.
     std::vector<Real> residualField();
     // Resume from a restart file's solution (RestartIO::read): sets the state,

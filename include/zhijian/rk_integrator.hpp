@@ -1,7 +1,7 @@
 // zhijian/rk_integrator.hpp -- time-integration API.
 //
 // The reference declares these classes (include/time/rk_integrator.hpp:11-79)
-// and defines none of them; this build defines all of them.  The *_GPU
+// and defines none of them; this library defines all of them.  The *_GPU
 // functions take device pointers and run elementwise on the given HIP stream;
 // they do not apply the positivity limiter (FRSolver::advance does, as the
 // reference's advance does through gpu::rkStage*, kernels.cu:1305-1349).

@@ -3,7 +3,7 @@
 // Same names, fields and defaults as the reference's include/common/types.hpp
 // (Real, Index, LocalIndex, N_VARS, the enums, Vec2, State, Flux2D, SimParams,
 // PartitionInfo) so that code written against the reference compiles
-// unchanged.  SimParams gains three fields for this build: precision, compat
+// unchanged.  SimParams gains fields of its own in this library: precision, compat
 // and positivity.
 #pragma once
 
@@ -120,7 +120,7 @@ struct SimParams {
     };
     std::map<int, BCSpec> bc_specs;
 
-    // ---- this build
+    // ---- additions of this library
     Precision precision = Precision::FP64;  // FP32Bulk: FP32 flux/interp/correction, FP64 RK state
     Compat compat = Compat::Reference;      // the reference's executed arithmetic
     bool positivity = true;                 // the reference's limiter after every RK stage

@@ -48,7 +48,7 @@ extern "C" {
 #define ZJ_ERR_ARG 1          /* invalid argument or inconsistent tables */
 #define ZJ_ERR_HIP 2          /* HIP runtime error (message in zj_last_error) */
 #define ZJ_ERR_NOMEM 3
-#define ZJ_ERR_UNSUPPORTED 4  /* valid input this build does not handle */
+#define ZJ_ERR_UNSUPPORTED 4  /* valid input this library does not handle */
 #define ZJ_ERR_NONFINITE 5    /* state or residual holds NaN/Inf */
 #define ZJ_ERR_STATE 6        /* call out of order (e.g. no state set) */
 

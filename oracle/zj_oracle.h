@@ -61,7 +61,7 @@ void oz_quad_operators(int p, double* sol_xi, double* sol_eta, double* qw,
  * reference on its test meshes and generated meshes (bit-exact). */
 int oz_build_faces(int n_elem, int nv, int stride, const int64_t* elem_nodes,
                    int* face_le, int* face_ll, int* face_re, int* face_rl);
-/* Periodic pairing (the reference has none; convention of this build): the
+/* Periodic pairing (the reference has none; the library uses the same convention): the
  * boundary face with the lower index is kept in place and gets the partner's
  * left element/edge as its right side; the partner is removed; order stays
  * stable.  Partners are boundary faces whose midpoints differ by (+-Lx, 0) or
