@@ -410,6 +410,25 @@ def time_precision(sol, U0, dt, steps, warmup, stream, hiprt):
     return ms_step, st_ms
 
 
+def time_vortex(steps, warmup, stream, hiprt):
+    """The isentropic-vortex case: 10x10 quads on [-5,5]^2, P3 (6,400 DOF),
+    reference arithmetic, dt = min(CFL dt, 1e-3).  0.3 MB per stage: the step
+    is launch-bound, so steps/s is the figure, not a roofline fraction."""
+    from zhijian_amd import cases, solver
+    mesh = cases.vortex_mesh(10)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=(mesh.Lx, mesh.Ly))
+    U0 = cases.vortex_ic(t.pos).reshape(-1)
+    out = {"workload": "isentropic vortex, 10x10 quads on [-5,5]^2, P3, reference arithmetic", "dof": t.E * t.S * 4}
+    for name in ("fp64", "fp32bulk"):
+        s = solver.Solver(t, precision=name, cfl=0.5)
+        s.set_state(U0)
+        dt = min(s.dt_cfl(0.5), 1e-3)
+        ms, st = time_precision(s, U0, dt, steps, warmup, stream, hiprt)
+        out[name] = dict(ms_per_step=ms, rk_steps_per_s=1e3 / ms, stage_ms=st,
+                         dof_updates_per_s=3.0 * out["dof"] / (ms * 1e-3))
+    return out
+
+
 def cpu_baseline(mesh, p, U0, dt, budget_s):
     """The oracle (a plain-C restatement of the reference's executed kernels)
     on the same workload, OpenMP over elements and faces on this host."""
@@ -601,6 +620,9 @@ def main():
                        0.0 if args.no_cpu else 5.0)
         log(f"triangles: {tri}")
 
+    vortex = time_vortex(10 * args.steps, args.warmup, stream, hiprt)
+    log(f"vortex: {vortex}")
+
     io_res = None
     if args.io:
         io_res = time_io(s64, mesh, args.p)
@@ -643,6 +665,7 @@ def main():
         "textbook": textbook,
         "navier_stokes": viscous,
         "triangles": tri,
+        "vortex": vortex,
         "io": io_res,
         "copy_kernel_gbps": copy_gbps,
     }
