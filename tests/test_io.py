@@ -376,3 +376,35 @@ def test_restart_attribute_and_name_limits(tmp_path):
     rc = _lib.lib().zj_restart_write(str(tmp_path / "z.h5").encode(), 3, bad.ctypes.data, sol.ctypes.data,
                                      0.0, 0, 1, 1.4)
     assert rc != 0 and b"negative" in _lib.lib().zj_io_last_error()
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_vtu_interpolated_exact_on_curved_quads(tmp_path, p):
+    """A curved Q2 quadrilateral: a state polynomial of degree <= p in the
+    reference coordinates (xi, eta) is reproduced at every lattice point
+    (xi_i, eta_j) = (-1 + 2i/n, -1 + 2j/n), whatever the element's geometry."""
+    from zhijian_amd import _lib
+    ops = _lib.quad_operators(p)
+    xi, eta = ops["sol_xi"], ops["sol_eta"]
+    rng = np.random.default_rng(10 + p)
+    c = rng.normal(size=(4, p + 1, p + 1)) * 0.1
+    c[0, 0, 0] += 2.0
+    c[3, 0, 0] += 6.0
+
+    def field(v, X, Y):
+        return sum(c[v, a, b] * X ** a * Y ** b for a in range(p + 1) for b in range(p + 1 - a))
+
+    sol = np.stack([field(v, xi, eta) for v in range(4)], 1).ravel()
+    xy = [(0, 0), (2, 0.1), (2.2, 1.9), (-0.1, 2), (1, -0.2), (2.3, 1), (1.1, 2.2), (0.1, 1.05), (1.05, 0.95)]
+    m = io.VisMesh(np.array(xy, float), [list(range(9))], 1, 2)
+    n = 5
+    io.write_vtk(tmp_path / "c.vtu", m, (p + 1) ** 2, sol, poly_order=p, subdivisions=n,
+                 flags=io.BINARY | io.INTERPOLATE)
+    got = io.read_vtu_binary(tmp_path / "c.vtu")
+    X, Y = np.meshgrid(-1 + 2 * np.arange(n + 1) / n, -1 + 2 * np.arange(n + 1) / n)
+    X, Y = X.ravel(), Y.ravel()
+    assert np.allclose(got["Density"], field(0, X, Y), rtol=0, atol=1e-12)
+    assert np.allclose(got["Velocity"][:, 0], field(1, X, Y) / field(0, X, Y), rtol=0, atol=1e-11)
+    # the lattice points themselves follow the Q2 map (compare with the oracle's shape functions)
+    want = [V.subdivide(1, 2, xy, None, n)[0][k] for k in range(len(X))]
+    assert np.allclose(got["Points"][:, :2], np.array(want), rtol=0, atol=1e-14)
