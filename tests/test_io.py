@@ -408,3 +408,37 @@ def test_vtu_interpolated_exact_on_curved_quads(tmp_path, p):
     # the lattice points themselves follow the Q2 map (compare with the oracle's shape functions)
     want = [V.subdivide(1, 2, xy, None, n)[0][k] for k in range(len(X))]
     assert np.allclose(got["Points"][:, :2], np.array(want), rtol=0, atol=1e-14)
+
+
+def test_restart_read_checks_counts_before_allocating(tmp_path):
+    """zj_restart_read with an element count other than the file's is refused
+    (ZJ_ERR_ARG) from the dataset's header, before anything is sized from the
+    caller's count; the right counts read the file."""
+    from zhijian_amd import _lib
+    nsp, sol = _sample(6, 4)
+    path = tmp_path / "r.h5"
+    io.write_restart(path, nsp, sol, 0.5, 3, 2, 1.4)
+    L = _lib.lib()
+    for n_elem in (5, 7, 1 << 40):
+        rc = L.zj_restart_read(str(path).encode(), n_elem, None, sol.size, None)
+        assert rc == 1 and b"elem_n_sp holds 6 entries" in L.zj_io_last_error()
+    got_nsp = np.zeros(6, np.int32)
+    got = np.zeros(sol.size)
+    assert L.zj_restart_read(str(path).encode(), 6, got_nsp.ctypes.data, sol.size, got.ctypes.data) == 0
+    assert got.tobytes() == sol.tobytes() and got_nsp.tolist() == list(nsp)
+
+
+def test_vtk_refuses_bad_node_offsets(tmp_path):
+    """Element node offsets that are negative or decreasing are refused
+    before any node is read through them."""
+    xy, els, kinds, orders = _mixed_mesh(0)
+    m = io.VisMesh(xy, els, kinds, orders)
+    nsp = [4] * 8
+    sol = _states(8, nsp, 0)
+    m.start[0] = -4
+    with pytest.raises(ZJError, match="too few nodes"):
+        io.write_vtk(tmp_path / "x.vtu", m, nsp, sol, poly_order=1)
+    m = io.VisMesh(xy, els, kinds, orders)
+    m.start[3], m.start[4] = m.start[4], m.start[3]
+    with pytest.raises(ZJError, match="too few nodes"):
+        io.write_vtk(tmp_path / "x.vtu", m, nsp, sol, poly_order=1)

@@ -19,6 +19,7 @@
 #include <cctype>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <map>
 #include <set>
@@ -87,6 +88,12 @@ class Reader {
 public:
     explicit Reader(std::istream& in) : in_(in) {}
     bool line(std::string& s) { return (bool)std::getline(in_, s); }
+    // The next line of `section`; at end of file, false with the error set.
+    bool need(std::string& s, const char* section, std::string& err) {
+        if (line(s)) return true;
+        err = std::string("unexpected end of file in ") + section;
+        return false;
+    }
     // skip to the line containing `marker`
     void skip_to(const std::string& marker) {
         std::string s;
@@ -111,7 +118,10 @@ bool add_element(State& st, int type, int phys, std::istringstream& ls) {
     std::vector<int64_t> ids((size_t)nn);
     for (int j = 0; j < nn; ++j) {
         int64_t tag = 0;
-        ls >> tag;
+        if (!(ls >> tag)) {
+            st.out.err = "element line with fewer than " + std::to_string(nn) + " node tags";
+            return false;
+        }
         auto it = st.node_index.find(tag);
         if (it == st.node_index.end()) {
             st.out.err = "element references unknown node tag " + std::to_string(tag);
@@ -151,46 +161,58 @@ bool add_element(State& st, int type, int phys, std::istringstream& ls) {
     return true;
 }
 
+// Node and element sections.  Counts in section and block headers are
+// checked before use, storage grows with the lines actually read (a corrupt
+// count cannot size an allocation), and a section that ends early is an error.
 bool read_nodes(Reader& r, State& st, bool v4) {
-    std::string s;
-    r.line(s);
+    std::string s, &err = st.out.err;
+    if (!r.need(s, "$Nodes", err)) return false;
     std::istringstream h(s);
+    std::vector<double>& xy = st.out.xy;
+    xy.clear();
     if (!v4) {
-        int64_t n = 0;
-        h >> n;
-        st.out.xy.assign((size_t)(2 * n), 0.0);
+        int64_t n = -1;
+        if (!(h >> n) || n < 0) { err = "malformed $Nodes header"; return false; }
         for (int64_t i = 0; i < n; ++i) {
-            r.line(s);
+            if (!r.need(s, "$Nodes", err)) return false;
             std::istringstream ls(s);
             int64_t tag;
             double x, y, z;
-            ls >> tag >> x >> y >> z;
+            if (!(ls >> tag >> x >> y >> z)) { err = "malformed node line: " + s; return false; }
             st.node_index[tag] = i;
-            st.out.xy[(size_t)(2 * i)] = x;
-            st.out.xy[(size_t)(2 * i + 1)] = y;
+            xy.push_back(x);
+            xy.push_back(y);
         }
     } else {
-        int64_t blocks = 0, n = 0, tmin, tmax;
-        h >> blocks >> n >> tmin >> tmax;
-        st.out.xy.assign((size_t)(2 * n), 0.0);
+        int64_t blocks = -1, n = -1, tmin, tmax;
+        if (!(h >> blocks >> n >> tmin >> tmax) || blocks < 0 || n < 0) {
+            err = "malformed $Nodes header";
+            return false;
+        }
         int64_t base = 0;
         for (int64_t b = 0; b < blocks; ++b) {
-            r.line(s);
+            if (!r.need(s, "$Nodes", err)) return false;
             std::istringstream bl(s);
             int dim, ent, param;
-            int64_t cnt = 0;
-            bl >> dim >> ent >> param >> cnt;
-            for (int64_t i = 0; i < cnt; ++i) {
-                r.line(s);
-                st.node_index[std::stoll(s)] = base + i;
+            int64_t cnt = -1;
+            if (!(bl >> dim >> ent >> param >> cnt) || cnt < 0 || cnt > n - base) {
+                err = "malformed $Nodes entity block: " + s;
+                return false;
             }
             for (int64_t i = 0; i < cnt; ++i) {
-                r.line(s);
+                if (!r.need(s, "$Nodes", err)) return false;
+                std::istringstream ls(s);
+                int64_t tag;
+                if (!(ls >> tag)) { err = "malformed node tag: " + s; return false; }
+                st.node_index[tag] = base + i;
+            }
+            for (int64_t i = 0; i < cnt; ++i) {
+                if (!r.need(s, "$Nodes", err)) return false;
                 std::istringstream ls(s);
                 double x, y, z;
-                ls >> x >> y >> z;
-                st.out.xy[(size_t)(2 * (base + i))] = x;
-                st.out.xy[(size_t)(2 * (base + i) + 1)] = y;
+                if (!(ls >> x >> y >> z)) { err = "malformed node coordinates: " + s; return false; }
+                xy.push_back(x);
+                xy.push_back(y);
             }
             base += cnt;
         }
@@ -200,46 +222,49 @@ bool read_nodes(Reader& r, State& st, bool v4) {
 }
 
 bool read_elements(Reader& r, State& st, bool v4) {
-    std::string s;
-    r.line(s);
+    std::string s, &err = st.out.err;
+    if (!r.need(s, "$Elements", err)) return false;
     std::istringstream h(s);
     if (!v4) {
-        int64_t n = 0;
-        h >> n;
+        int64_t n = -1;
+        if (!(h >> n) || n < 0) { err = "malformed $Elements header"; return false; }
         for (int64_t i = 0; i < n; ++i) {
-            r.line(s);
+            if (!r.need(s, "$Elements", err)) return false;
             std::istringstream ls(s);
             int id, type, ntags;
-            ls >> id >> type >> ntags;
+            if (!(ls >> id >> type >> ntags) || ntags < 0) { err = "malformed element line: " + s; return false; }
             int phys = 0;
             for (int t = 0; t < ntags; ++t) {
                 int tag;
-                ls >> tag;
+                if (!(ls >> tag)) { err = "malformed element line: " + s; return false; }
                 if (t == 0) phys = tag;
             }
             if (nodes_of(type) < 0) continue;
             if (!add_element(st, type, phys, ls)) return false;
         }
     } else {
-        int64_t blocks = 0, n, tmin, tmax;
-        h >> blocks >> n >> tmin >> tmax;
+        int64_t blocks = -1, n, tmin, tmax;
+        if (!(h >> blocks >> n >> tmin >> tmax) || blocks < 0) { err = "malformed $Elements header"; return false; }
         for (int64_t b = 0; b < blocks; ++b) {
-            r.line(s);
+            if (!r.need(s, "$Elements", err)) return false;
             std::istringstream bl(s);
             int dim, ent, type;
-            int64_t cnt = 0;
-            bl >> dim >> ent >> type >> cnt;
+            int64_t cnt = -1;
+            if (!(bl >> dim >> ent >> type >> cnt) || cnt < 0) {
+                err = "malformed $Elements entity block: " + s;
+                return false;
+            }
             int phys = 0;
             if (dim == 1) {
                 auto it = st.curve_phys.find(ent);
                 if (it != st.curve_phys.end()) phys = it->second;
             }
             for (int64_t e = 0; e < cnt; ++e) {
-                r.line(s);
+                if (!r.need(s, "$Elements", err)) return false;
                 if (nodes_of(type) < 0 || type == 15) continue;
                 std::istringstream ls(s);
                 int64_t id;
-                ls >> id;
+                if (!(ls >> id)) { err = "malformed element line: " + s; return false; }
                 if (!add_element(st, type, phys, ls)) return false;
             }
         }
@@ -265,23 +290,28 @@ bool parse(const std::string& path, Parsed& out) {
     while (r.line(s)) {
         if (s.empty() || s[0] != '$') continue;
         if (has(s, "$MeshFormat")) {
-            r.line(s);
+            if (!r.need(s, "$MeshFormat", out.err)) return false;
             std::istringstream ls(s);
             int dsize;
-            ls >> version >> ftype >> dsize;
+            if (!(ls >> version >> ftype >> dsize) || version <= 0.0) {
+                out.err = "malformed $MeshFormat line: " + s;
+                return false;
+            }
             if (ftype != 0) {
                 out.err = "only ASCII Gmsh files are supported";
                 return false;
             }
             r.skip_to("$EndMeshFormat");
         } else if (has(s, "$PhysicalNames")) {
-            r.line(s);
-            const int n = std::stoi(s);
+            if (!r.need(s, "$PhysicalNames", out.err)) return false;
+            std::istringstream hs(s);
+            int n = -1;
+            if (!(hs >> n) || n < 0) { out.err = "malformed $PhysicalNames header"; return false; }
             for (int i = 0; i < n; ++i) {
-                r.line(s);
+                if (!r.need(s, "$PhysicalNames", out.err)) return false;
                 std::istringstream ls(s);
                 int dim, tag;
-                ls >> dim >> tag;
+                if (!(ls >> dim >> tag)) { out.err = "malformed physical name: " + s; return false; }
                 std::string name;
                 std::getline(ls, name);
                 const size_t a = name.find_first_not_of(" \t");
@@ -292,30 +322,36 @@ bool parse(const std::string& path, Parsed& out) {
             }
             r.skip_to("$EndPhysicalNames");
         } else if (has(s, "$Entities")) {
-            r.line(s);
+            if (!r.need(s, "$Entities", out.err)) return false;
             std::istringstream hs(s);
-            int np, nc, ns, nv;
-            hs >> np >> nc >> ns >> nv;
-            for (int i = 0; i < np; ++i) r.line(s);
+            int np = -1, nc = -1, ns, nv;
+            if (!(hs >> np >> nc >> ns >> nv) || np < 0 || nc < 0) {
+                out.err = "malformed $Entities header";
+                return false;
+            }
+            for (int i = 0; i < np; ++i)
+                if (!r.need(s, "$Entities", out.err)) return false;
             for (int i = 0; i < nc; ++i) {
-                r.line(s);
+                if (!r.need(s, "$Entities", out.err)) return false;
                 std::istringstream ls(s);
-                int tag, nphys;
+                int tag, nphys = 0;
                 double bb[6];
-                ls >> tag >> bb[0] >> bb[1] >> bb[2] >> bb[3] >> bb[4] >> bb[5] >> nphys;
-                if (nphys > 0) {
-                    int ph;
-                    ls >> ph;
-                    st.curve_phys[tag] = ph;
+                if (!(ls >> tag >> bb[0] >> bb[1] >> bb[2] >> bb[3] >> bb[4] >> bb[5] >> nphys)) {
+                    out.err = "malformed curve entity: " + s;
+                    return false;
                 }
+                int ph;
+                if (nphys > 0 && (ls >> ph)) st.curve_phys[tag] = ph;
             }
             r.skip_to("$EndEntities");
         } else if (has(s, "$Nodes")) {
             if (version == 0.0) { out.err = "$Nodes before $MeshFormat"; return false; }
+            if (have_nodes) { out.err = "more than one $Nodes section"; return false; }
             if (!read_nodes(r, st, version >= 4.0)) return false;
             have_nodes = true;
         } else if (has(s, "$Elements")) {
             if (!have_nodes) { out.err = "$Elements before $Nodes"; return false; }
+            if (have_elems) { out.err = "more than one $Elements section"; return false; }
             if (!read_elements(r, st, version >= 4.0)) return false;
             have_elems = true;
         } else if (!has(s, "$End")) {
@@ -358,7 +394,14 @@ extern "C" int zj_gmsh_read(const char* path, zj_gmsh** out) {
     *out = nullptr;
     zj_gmsh* g = new (std::nothrow) zj_gmsh();
     if (!g) return ZJ_ERR_NOMEM;
-    if (!zj::gmsh::parse(path, g->p)) {
+    bool ok;
+    try {
+        ok = zj::gmsh::parse(path, g->p);
+    } catch (const std::exception& e) {   // std::bad_alloc on a mesh too large for host memory
+        g->p.err = std::string("reading ") + path + ": " + e.what();
+        ok = false;
+    }
+    if (!ok) {
         g_gmsh_error = g->p.err;
         delete g;
         return ZJ_ERR_ARG;

@@ -32,6 +32,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -46,6 +47,20 @@ thread_local std::string g_io_err;
 int io_fail(int code, const std::string& msg) {
     g_io_err = msg;
     return code;
+}
+
+// Runs the body of a C entry point; a C++ exception (std::bad_alloc for a
+// mesh or file too large for host memory) becomes an error code instead of
+// crossing into the caller.
+template <class F>
+int guarded(F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return io_fail(ZJ_ERR_NOMEM, "out of host memory");
+    } catch (const std::exception& e) {
+        return io_fail(ZJ_ERR_ARG, e.what());
+    }
 }
 
 // ============================================================================
@@ -97,44 +112,55 @@ extern "C" const char* zj_io_last_error(void) { return g_io_err.c_str(); }
 
 extern "C" int zj_restart_write(const char* path, int64_t n_elem, const int* elem_n_sp, const double* solution,
                                 double time, int iteration, int poly_order, double gamma) {
-    return restart_write(path, n_elem, elem_n_sp, solution, time, iteration, poly_order, gamma);
+    return guarded([&]() -> int {
+        return restart_write(path, n_elem, elem_n_sp, solution, time, iteration, poly_order, gamma);
+    });
 }
 
 extern "C" int zj_restart_query(const char* path, zj_restart_info* info) {
-    if (!path || !info) return io_fail(ZJ_ERR_ARG, "null argument");
-    zj::h5::Reader r;
-    std::string err;
-    if (!r.open(path, err)) return io_fail(ZJ_ERR_ARG, err);
-    zj_restart_info o{};
-    zj::h5::DatasetInfo dn, ds;
-    if (!scalar_attr(r, "time", o.time, err) || !int_attr(r, "iteration", o.iteration, err) ||
-        !int_attr(r, "poly_order", o.poly_order, err) || !scalar_attr(r, "gamma", o.gamma, err) ||
-        !r.dataset("elem_n_sp", dn, err) || !r.dataset("solution", ds, err))
-        return io_fail(ZJ_ERR_ARG, std::string(path) + ": " + err);
-    o.n_elem = int64_t(dn.count);
-    o.n_values = int64_t(ds.count);
-    *info = o;
-    return ZJ_OK;
+    return guarded([&]() -> int {
+        if (!path || !info) return io_fail(ZJ_ERR_ARG, "null argument");
+        zj::h5::Reader r;
+        std::string err;
+        if (!r.open(path, err)) return io_fail(ZJ_ERR_ARG, err);
+        zj_restart_info o{};
+        zj::h5::DatasetInfo dn, ds;
+        if (!scalar_attr(r, "time", o.time, err) || !int_attr(r, "iteration", o.iteration, err) ||
+            !int_attr(r, "poly_order", o.poly_order, err) || !scalar_attr(r, "gamma", o.gamma, err) ||
+            !r.dataset("elem_n_sp", dn, err) || !r.dataset("solution", ds, err))
+            return io_fail(ZJ_ERR_ARG, std::string(path) + ": " + err);
+        o.n_elem = int64_t(dn.count);
+        o.n_values = int64_t(ds.count);
+        *info = o;
+        return ZJ_OK;
+    });
 }
 
 extern "C" int zj_restart_read(const char* path, int64_t n_elem, int* elem_n_sp, int64_t n_values, double* solution) {
-    if (!path || n_elem < 0 || n_values < 0) return io_fail(ZJ_ERR_ARG, "bad argument");
-    zj::h5::Reader r;
-    std::string err;
-    if (!r.open(path, err)) return io_fail(ZJ_ERR_ARG, err);
-    std::vector<int> nsp(static_cast<size_t>(n_elem));
-    if (!r.read("elem_n_sp", nsp.data(), uint64_t(n_elem), err)) return io_fail(ZJ_ERR_ARG, err);
-    uint64_t total = 0;
-    for (int v : nsp) {
-        if (v < 0) return io_fail(ZJ_ERR_ARG, "negative elem_n_sp in file");
-        total += uint64_t(v) * N_VARS_C;
-    }
-    if (total != uint64_t(n_values))
-        return io_fail(ZJ_ERR_ARG, "solution holds " + std::to_string(n_values) + " values but elem_n_sp adds up to " +
-                                       std::to_string(total));
-    if (solution && !r.read("solution", solution, uint64_t(n_values), err)) return io_fail(ZJ_ERR_ARG, err);
-    if (elem_n_sp) std::copy(nsp.begin(), nsp.end(), elem_n_sp);
-    return ZJ_OK;
+    return guarded([&]() -> int {
+        if (!path || n_elem < 0 || n_values < 0) return io_fail(ZJ_ERR_ARG, "bad argument");
+        zj::h5::Reader r;
+        std::string err;
+        if (!r.open(path, err)) return io_fail(ZJ_ERR_ARG, err);
+        zj::h5::DatasetInfo dn;
+        if (!r.dataset("elem_n_sp", dn, err)) return io_fail(ZJ_ERR_ARG, err);
+        if (dn.count != uint64_t(n_elem))
+            return io_fail(ZJ_ERR_ARG, std::string(path) + ": elem_n_sp holds " + std::to_string(dn.count) +
+                                           " entries, not " + std::to_string(n_elem));
+        std::vector<int> nsp(static_cast<size_t>(n_elem));
+        if (!r.read("elem_n_sp", nsp.data(), uint64_t(n_elem), err)) return io_fail(ZJ_ERR_ARG, err);
+        uint64_t total = 0;
+        for (int v : nsp) {
+            if (v < 0) return io_fail(ZJ_ERR_ARG, "negative elem_n_sp in file");
+            total += uint64_t(v) * N_VARS_C;
+        }
+        if (total != uint64_t(n_values))
+            return io_fail(ZJ_ERR_ARG, "solution holds " + std::to_string(n_values) +
+                                           " values but elem_n_sp adds up to " + std::to_string(total));
+        if (solution && !r.read("solution", solution, uint64_t(n_values), err)) return io_fail(ZJ_ERR_ARG, err);
+        if (elem_n_sp) std::copy(nsp.begin(), nsp.end(), elem_n_sp);
+        return ZJ_OK;
+    });
 }
 
 namespace {
@@ -262,23 +288,31 @@ HandleOps tri_ops(zj_tri_handle h) {
 }  // namespace
 
 extern "C" int zj_restart_save(zj_handle h, const char* path, double time, int iteration) {
-    if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
-    return handle_save(quad_ops(h), path, time, iteration);
+    return guarded([&]() -> int {
+        if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
+        return handle_save(quad_ops(h), path, time, iteration);
+    });
 }
 
 extern "C" int zj_restart_load(zj_handle h, const char* path, double* time, int* iteration) {
-    if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
-    return handle_load(quad_ops(h), path, time, iteration);
+    return guarded([&]() -> int {
+        if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
+        return handle_load(quad_ops(h), path, time, iteration);
+    });
 }
 
 extern "C" int zj_tri_restart_save(zj_tri_handle h, const char* path, double time, int iteration) {
-    if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
-    return handle_save(tri_ops(h), path, time, iteration);
+    return guarded([&]() -> int {
+        if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
+        return handle_save(tri_ops(h), path, time, iteration);
+    });
 }
 
 extern "C" int zj_tri_restart_load(zj_tri_handle h, const char* path, double* time, int* iteration) {
-    if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
-    return handle_load(tri_ops(h), path, time, iteration);
+    return guarded([&]() -> int {
+        if (!h) return io_fail(ZJ_ERR_ARG, "null handle");
+        return handle_load(tri_ops(h), path, time, iteration);
+    });
 }
 
 // ============================================================================
@@ -435,7 +469,10 @@ bool build_lattice(const VisIn& in, Lattice& out, std::string& err) {
         const Kind& k = *it->second;
         ek[e] = &k;
         const int64_t nn = m.elem_node_start[e + 1] - m.elem_node_start[e];
-        if (nn < k.nnodes) { err = "element " + std::to_string(e) + ": too few nodes for its order"; return false; }
+        if (m.elem_node_start[e] < 0 || nn < k.nnodes) {
+            err = "element " + std::to_string(e) + ": too few nodes for its order";
+            return false;
+        }
         if (in.nsp[e] < 0 || (in.interp && in.nsp[e] != k.nsp)) {
             err = "element " + std::to_string(e) + ": " + std::to_string(in.nsp[e]) +
                   " solution points do not match poly_order " + std::to_string(in.p);
@@ -446,7 +483,7 @@ bool build_lattice(const VisIn& in, Lattice& out, std::string& err) {
         coff[e + 1] = coff[e] + k.ncells;
     }
     if (soff[E] && !in.sol) { err = "null solution"; return false; }
-    for (int64_t i = 0; i < (E ? m.elem_node_start[E] : 0); ++i)
+    for (int64_t i = E ? m.elem_node_start[0] : 0; i < (E ? m.elem_node_start[E] : 0); ++i)
         if (m.elem_nodes[i] < 0 || m.elem_nodes[i] >= m.n_nodes) { err = "node index out of range"; return false; }
     out.npts = poff[E];
     out.ncells = coff[E];
@@ -563,8 +600,8 @@ bool write_vtu_ascii(const std::string& path, const Lattice& L, std::string& err
     h += "<?xml version=\"1.0\"?>\n";
     h += "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
     h += "  <UnstructuredGrid>\n";
-    h += "    <Piece NumberOfPoints=\"" + std::to_string(int(L.npts)) + "\" NumberOfCells=\"" +
-         std::to_string(int(L.ncells)) + "\">\n";
+    h += "    <Piece NumberOfPoints=\"" + std::to_string(L.npts) + "\" NumberOfCells=\"" +
+         std::to_string(L.ncells) + "\">\n";
     h += "      <Points>\n";
     h += "        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
     f.put(h);
@@ -579,7 +616,7 @@ bool write_vtu_ascii(const std::string& path, const Lattice& L, std::string& err
     });
     f.put("        </DataArray>\n        <DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
     put_lines(f, L.ncells, [&](std::string& s, int64_t i) {
-        s += IND; Fmt::integer(s, int(i + 1) * 3); s += '\n';
+        s += IND; Fmt::integer(s, (i + 1) * 3); s += '\n';
     });
     f.put("        </DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
     put_lines(f, L.ncells, [&](std::string& s, int64_t) { s += IND; s += "5\n"; });
@@ -674,7 +711,7 @@ bool write_legacy(const std::string& path, const Lattice& L, std::string& err) {
     put_lines(f, L.npts, [&](std::string& s, int64_t i) {
         e.num(s, L.xyz[3 * i]); s += ' '; e.num(s, L.xyz[3 * i + 1]); s += " 0.0\n";
     });
-    const int nc = int(L.ncells);
+    const int64_t nc = L.ncells;
     f.put("CELLS " + std::to_string(nc) + " " + std::to_string(nc * 4) + "\n");
     put_lines(f, L.ncells, [&](std::string& s, int64_t i) {
         s += "3 "; Fmt::integer(s, L.conn[3 * i]); s += ' '; Fmt::integer(s, L.conn[3 * i + 1]); s += ' ';
@@ -733,23 +770,27 @@ int vtk_write(const std::string& path, const VisIn& in, int flags, std::string& 
 
 extern "C" int zj_vtk_write(const char* path, const zj_vis_mesh* mesh, const int* elem_n_sp, const double* solution,
                             int poly_order, double gamma, int subdivisions, int flags) {
-    if (!path || !mesh) return io_fail(ZJ_ERR_ARG, "null argument");
-    if (flags & ~(ZJ_VTK_LEGACY | ZJ_VTK_BINARY | ZJ_VTK_INTERPOLATE)) return io_fail(ZJ_ERR_ARG, "unknown flags");
-    VisIn in{mesh, elem_n_sp, solution, poly_order, gamma, subdivisions, (flags & ZJ_VTK_INTERPOLATE) != 0};
-    std::string err;
-    const int rc = vtk_write(path, in, flags, err);
-    return rc ? io_fail(rc, err) : ZJ_OK;
+    return guarded([&]() -> int {
+        if (!path || !mesh) return io_fail(ZJ_ERR_ARG, "null argument");
+        if (flags & ~(ZJ_VTK_LEGACY | ZJ_VTK_BINARY | ZJ_VTK_INTERPOLATE)) return io_fail(ZJ_ERR_ARG, "unknown flags");
+        VisIn in{mesh, elem_n_sp, solution, poly_order, gamma, subdivisions, (flags & ZJ_VTK_INTERPOLATE) != 0};
+        std::string err;
+        const int rc = vtk_write(path, in, flags, err);
+        return rc ? io_fail(rc, err) : ZJ_OK;
+    });
 }
 
 extern "C" int zj_pvtu_write(const char* path, int n_pieces, const char* const* piece_files) {
-    if (!path || n_pieces < 0 || (n_pieces && !piece_files)) return io_fail(ZJ_ERR_ARG, "null argument");
-    std::vector<std::string> pieces;
-    for (int k = 0; k < n_pieces; ++k) {
-        if (!piece_files[k]) return io_fail(ZJ_ERR_ARG, "null piece name");
-        pieces.push_back(piece_files[k]);
-    }
-    std::string err;
-    return write_pvtu(path, pieces, err) ? ZJ_OK : io_fail(ZJ_ERR_ARG, err);
+    return guarded([&]() -> int {
+        if (!path || n_pieces < 0 || (n_pieces && !piece_files)) return io_fail(ZJ_ERR_ARG, "null argument");
+        std::vector<std::string> pieces;
+        for (int k = 0; k < n_pieces; ++k) {
+            if (!piece_files[k]) return io_fail(ZJ_ERR_ARG, "null piece name");
+            pieces.push_back(piece_files[k]);
+        }
+        std::string err;
+        return write_pvtu(path, pieces, err) ? ZJ_OK : io_fail(ZJ_ERR_ARG, err);
+    });
 }
 
 // ============================================================================

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <new>
 #include <vector>
 
 using namespace zj;
@@ -426,7 +427,13 @@ extern "C" int zj_create(zj_handle* out, const zj_mesh_desc* m, const zj_params*
         g_create_error = "out of host memory";
         return ZJ_ERR_NOMEM;
     }
-    int rc = build(h, m, p);
+    int rc;
+    try {
+        rc = build(h, m, p);
+    } catch (const std::bad_alloc&) {   // host tables of a mesh too large for host memory
+        h->err = "out of host memory";
+        rc = ZJ_ERR_NOMEM;
+    }
     if (rc != ZJ_OK) {
         g_create_error = h->err;
         release(h);
@@ -1248,8 +1255,17 @@ extern "C" int zj_batch_create(zj_batch** out, const zj_handle* members, int n) 
     }
     *out = nullptr;
     zj_batch* b = new (std::nothrow) zj_batch();
-    if (!b) return ZJ_ERR_NOMEM;
-    int rc = batch_build(b, members, n);
+    if (!b) {
+        g_batch_error = "out of host memory";
+        return ZJ_ERR_NOMEM;
+    }
+    int rc;
+    try {
+        rc = batch_build(b, members, n);
+    } catch (const std::bad_alloc&) {
+        b->err = "out of host memory";
+        rc = ZJ_ERR_NOMEM;
+    }
     if (rc) {
         g_batch_error = b->err;
         batch_release(b);

@@ -52,6 +52,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <new>
 #include <vector>
 
 using namespace zj;
@@ -777,7 +778,13 @@ extern "C" int zj_tri_create(zj_tri_handle* out, const zj_tri_desc* m, const zj_
         g_tri_error = "out of host memory";
         return ZJ_ERR_NOMEM;
     }
-    const int rc = tbuild(h, m, p);
+    int rc;
+    try {
+        rc = tbuild(h, m, p);
+    } catch (const std::bad_alloc&) {   // host tables of a mesh too large for host memory
+        h->err = "out of host memory";
+        rc = ZJ_ERR_NOMEM;
+    }
     if (rc != ZJ_OK) {
         g_tri_error = h->err;
         trelease(h);
