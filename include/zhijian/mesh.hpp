@@ -3,14 +3,18 @@
 // API of the reference's include/mesh/mesh.hpp (Face, Element, BCInfo, Mesh).
 // buildFaces() reproduces the reference's face order and left/right roles
 // (mesh.cpp:18-83): faces in ascending (min vertex, max vertex) order, left =
-// first element in element order.  pairPeriodic() has no counterpart in the
-// reference: it joins boundary faces one period apart (see its comment).
+// first element in element order.  pairPeriodic() and pairPeriodicNodes()
+// have no counterpart in the reference: they join boundary faces one period
+// apart, or whose end nodes a mesh file's $Periodic links make correspond.
 #pragma once
 
 #include "zhijian/types.hpp"
 
+#include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace zhijian {
@@ -86,6 +90,19 @@ public:
     // is removed and the order is otherwise kept.  A period <= 0 disables that
     // direction.  Throws std::runtime_error if a face has several partners.
     void pairPeriodic(Real Lx, Real Ly, Real tol = -1.0);
+    // Periodic pairing by node correspondence (slave node, master node), e.g.
+    // a Gmsh file's $Periodic section: two boundary faces become one interior
+    // face when their end nodes correspond (directly or through a chain of
+    // links) and one face is the other moved by a link's offset; kept,
+    // removed and ordered as pairPeriodic() does.  The no-argument form uses
+    // periodicNodes().  Throws std::runtime_error when a face has two
+    // candidates or a link names a node out of range.
+    void pairPeriodicNodes();
+    void pairPeriodicNodes(const std::vector<std::pair<Index, Index>>& links);
+    // Node correspondences read from the mesh file (GmshReader fills them;
+    // the reference's reader skips $Periodic and has no such field).
+    const std::vector<std::pair<Index, Index>>& periodicNodes() const { return periodic_nodes_; }
+    std::vector<std::pair<Index, Index>>& periodicNodes() { return periodic_nodes_; }
     void computeGeometry();
 
     Vec2 elementCentroid(Index elem_id) const;
@@ -97,11 +114,14 @@ public:
 
 private:
     void rebuildFaceIds();
+    void pairFaces(const std::function<int(int*, int*, int*, int*, int*, int, const int64_t*)>& pair,
+                   const char* what);
 
     std::vector<Vec2> nodes_;
     std::vector<Element> elements_;
     std::vector<Face> faces_;
     std::map<int, BCInfo> bc_info_;
+    std::vector<std::pair<Index, Index>> periodic_nodes_;
     std::vector<Vec2> elem_centroids_;
     std::vector<Real> elem_areas_;
     std::vector<Real> face_lengths_;

@@ -27,7 +27,8 @@
  *   zj_face_normals    Mesh::faceNormal (mesh.cpp:159-183)
  *   zj_elem_areas      Mesh::elementArea (mesh.cpp:122-146)
  *   zj_geometry        FRSolver::initGeometry (fr_solver.cpp:85-129)
- *   zj_pair_periodic   no counterpart (the reference has no periodic faces)
+ *   zj_pair_periodic, zj_pair_periodic_nodes
+ *                      no counterpart (the reference has no periodic faces)
  *
  * Array layouts at this boundary are the reference's (include/gpu/kernels.hpp):
  *   U, dUdt  [E][S][4] AoS (rho, rhou, rhov, rhoE), S = (p+1)^2
@@ -250,6 +251,10 @@ int zj_gmsh_nodes(zj_gmsh_handle g, double* xy);
 int zj_gmsh_elements(zj_gmsh_handle g, int* kind, int* order, int* n_nodes, int64_t* nodes, int stride);
 int zj_gmsh_edge_tag(zj_gmsh_handle g, int64_t a, int64_t b, int* tag);
 int zj_gmsh_group(zj_gmsh_handle g, int i, int* tag, int* bc_type, char* name, int name_cap);
+/* The file's $Periodic node correspondences as 0-based node indices (slave,
+ * master), in file order; *n_pairs gets the count and the arrays, when not
+ * null, are filled.  No counterpart: the reference reader skips $Periodic. */
+int zj_gmsh_periodic(zj_gmsh_handle g, int64_t* n_pairs, int64_t* slave, int64_t* master);
 
 /* ---- triangles (textbook mode only) ---------------------------------
  * The reference declares triangle elements (ElementType::Triangle,
@@ -394,6 +399,17 @@ int zj_build_faces(int n_elem, int nv, int stride, const int64_t* elem_nodes, in
 int zj_pair_periodic(int* n_faces, int* face_le, int* face_ll, int* face_re, int* face_rl,
                      int nv, int stride, const int64_t* elem_nodes, const double* xy,
                      double Lx, double Ly, double tol);
+/* Pairs boundary faces by node correspondence instead of by fixed periods:
+ * n_pairs links slave[i] -> master[i] (node indices, e.g. from
+ * zj_gmsh_periodic) join nodes into classes (transitively).  Two boundary
+ * faces become one interior face when their end nodes fall in the same two
+ * classes and one face is the other moved by a link offset
+ * xy[master] - xy[slave] (either sign, within tol).  Kept face, removal and
+ * order as zj_pair_periodic.  ZJ_ERR_ARG if a face has two candidates or an
+ * index is out of range.  Updates *n_faces. */
+int zj_pair_periodic_nodes(int* n_faces, int* face_le, int* face_ll, int* face_re, int* face_rl,
+                           int nv, int stride, const int64_t* elem_nodes, const double* xy, int64_t n_nodes,
+                           int64_t n_pairs, const int64_t* slave, const int64_t* master, double tol);
 int zj_face_normals(int n_faces, const int* face_le, const int* face_ll, int nv, int stride,
                     const int64_t* elem_nodes, const double* xy, double* normals);
 int zj_elem_areas(int n_elem, int nv, int stride, const int64_t* elem_nodes,

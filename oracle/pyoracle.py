@@ -169,6 +169,76 @@ def pair_periodic(f: Faces, elem_nodes: np.ndarray, xy: np.ndarray, Lx: float, L
     return Faces(*[a[:nf].copy() for a in arrs])
 
 
+def pair_periodic_nodes(f: Faces, elem_nodes: np.ndarray, xy, links, tol: Optional[float] = None,
+                        nv: Optional[int] = None) -> Faces:
+    """Periodic pairing by node correspondence (no reference counterpart: the
+    reference's Gmsh reader skips $Periodic).  Restated with plain loops:
+    node labels by propagation (each link sets both ends to the smaller label
+    until nothing changes); the link offsets xy[master] - xy[slave]; then, in
+    ascending face order, each unpaired boundary face f takes as partner the
+    later unpaired boundary face g whose end-node labels equal f's and whose
+    end nodes are f's moved by one common vector equal (within tol, either
+    sign) to a link offset.  Two candidates: ValueError.  The lower-indexed
+    face keeps its place with the partner's left side as its right side; the
+    partner is removed.  Pinned in the tests by equality with pair_periodic on
+    translation-periodic meshes, by the edge-cover and order invariants and
+    partner midpoints one spanning vector apart on sheared meshes that
+    pair_periodic cannot pair, and by wall edges left unpaired."""
+    en = np.asarray(elem_nodes, np.int64)
+    nv = en.shape[1] if nv is None else nv
+    P = np.asarray(xy, np.float64).reshape(-1, 2)
+    if tol is None:
+        tol = 1e-9 * float(np.ptp(P, axis=0).max())
+    lab = list(range(P.shape[0]))
+    pairs = [(int(a), int(b)) for a, b in np.asarray(links, np.int64).reshape(-1, 2)]
+    changed = True
+    while changed:
+        changed = False
+        for a, b in pairs:
+            m = min(lab[a], lab[b])
+            if lab[a] != m or lab[b] != m:
+                lab[a] = lab[b] = m
+                changed = True
+    offsets = [P[b] - P[a] for a, b in pairs]
+
+    def is_offset(d):
+        return any(max(abs(d[0] - s * o[0]), abs(d[1] - s * o[1])) <= tol for o in offsets for s in (1.0, -1.0))
+
+    def ends(k):
+        e, l = int(f.le[k]), int(f.ll[k])
+        return int(en[e, l]), int(en[e, (l + 1) % nv])
+
+    def moved(k, g):
+        a0, a1 = ends(k)
+        for c0, c1 in (ends(g), ends(g)[::-1]):
+            d0, d1 = P[c0] - P[a0], P[c1] - P[a1]
+            if max(abs(d0 - d1)) <= tol and is_offset(d0):
+                return True
+        return False
+
+    bnd = [k for k in range(f.n) if f.re[k] < 0]
+    key = {k: tuple(sorted(lab[n] for n in ends(k))) for k in bnd}
+    partner, taken = {}, set()
+    for x, k in enumerate(bnd):
+        if k in taken:
+            continue
+        cands = [g for g in bnd[x + 1:] if g not in taken and key[g] == key[k] and moved(k, g)]
+        if len(cands) > 1:
+            raise ValueError("ambiguous periodic partners")
+        if cands:
+            partner[k] = cands[0]
+            taken.update((k, cands[0]))
+    dead = set(partner.values())
+    rows = []
+    for k in range(f.n):
+        if k in dead:
+            continue
+        g = partner.get(k)
+        rows.append((f.le[k], f.ll[k], f.le[g] if g is not None else f.re[k], f.ll[g] if g is not None else f.rl[k]))
+    a = np.array(rows, np.int32).reshape(-1, 4)
+    return Faces(*[a[:, c].copy() for c in range(4)])
+
+
 def face_normals(f: Faces, elem_nodes, xy, nv=None) -> np.ndarray:
     en = np.ascontiguousarray(elem_nodes, dtype=np.int64)
     stride = en.shape[1]

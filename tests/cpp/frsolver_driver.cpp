@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -219,12 +220,21 @@ int main(int argc, char** argv) {
         return 0;
     }
     if (argc >= 4 && std::strcmp(argv[1], "gmsh") == 0) {
-        // faces of a Gmsh mesh read through GmshReader: le ll re rl bc_type bc_tag per face
+        // faces of a Gmsh mesh read through GmshReader: le ll re rl bc_type bc_tag per face;
+        // with a 5th argument "periodic", after pairing by the file's $Periodic links
         Mesh m;
         GmshReader reader;
         if (!reader.read(argv[2], m)) {
             std::fprintf(stderr, "%s\n", reader.errorMessage().c_str());
             return 3;
+        }
+        if (argc >= 5 && std::strcmp(argv[4], "periodic") == 0) {
+            try {
+                m.pairPeriodicNodes();
+            } catch (const std::exception& e) {
+                std::fprintf(stderr, "%s\n", e.what());
+                return 4;
+            }
         }
         std::vector<double> out;
         for (Index f = 0; f < m.numFaces(); ++f) {
@@ -233,8 +243,9 @@ int main(int argc, char** argv) {
                                    (double)fc.right_face, (double)static_cast<int>(fc.bc_type), (double)fc.bc_tag});
         }
         dump(argv[3], out);
-        std::printf("{\"nodes\": %lld, \"elements\": %lld, \"faces\": %lld}\n", (long long)m.numNodes(),
-                    (long long)m.numElements(), (long long)m.numFaces());
+        std::printf("{\"nodes\": %lld, \"elements\": %lld, \"faces\": %lld, \"periodic_links\": %lld}\n",
+                    (long long)m.numNodes(), (long long)m.numElements(), (long long)m.numFaces(),
+                    (long long)m.periodicNodes().size());
         return 0;
     }
     if (argc >= 3 && std::strcmp(argv[1], "io") == 0) return io_case(argv[2]);

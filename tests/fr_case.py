@@ -75,13 +75,15 @@ def bc_tables(faces: O.Faces, bc_type_per_face: Optional[np.ndarray], fs, outflo
 
 
 def build(xy, elem_nodes, p, periodic=None, gamma=1.4, bc_type_per_face=None, fs=None,
-          node_count=None) -> OracleCase:
+          node_count=None, periodic_nodes=None) -> OracleCase:
     en = np.ascontiguousarray(elem_nodes, np.int64)
     nv = 4
     ops = O.quad_operators(p)
     faces = O.build_faces(en, nv=nv)
     if periodic is not None:
         faces = O.pair_periodic(faces, en, xy, periodic[0], periodic[1], 1e-9 * max(periodic), nv=nv)
+    if periodic_nodes is not None:
+        faces = O.pair_periodic_nodes(faces, en[:, :nv], xy, periodic_nodes, nv=nv)
     normals = O.face_normals(faces, en, xy, nv=nv)
     J, Jinv = O.geometry(en, xy, ops, node_count=node_count)
     area = O.elem_areas(en, xy, nv=nv)

@@ -49,6 +49,29 @@ def test_gmsh_reader_api_matches_reference(tmp_path, golden_meshes, name):
     assert np.array_equal(got[:, 5], g["face_bc_tag"])
 
 
+@pytest.mark.parametrize("v4", [False, True])
+def test_gmsh_periodic_links_cpp_api(tmp_path, v4):
+    """GmshReader keeps a file's $Periodic links (Mesh::periodicNodes) and
+    Mesh::pairPeriodicNodes joins the faces they link, as the C ABI and the
+    oracle do, on a sheared doubly periodic mesh (no GPU)."""
+    _need_driver()
+    from tests.test_gmsh import _msh_with_periodic
+    n = 5
+    m = cases.sheared_quad_mesh(n, n, 1.0, 1.0, -0.4, jitter=0.15, seed=4)
+    links = cases.periodic_links(n, n)
+    msh = tmp_path / "s.msh"
+    msh.write_text(_msh_with_periodic(m.xy, m.elem_nodes, 3, links[: n + 1], links[n + 1:], v4))
+    out = tmp_path / "f.bin"
+    r = subprocess.run([DRIVER, "gmsh", str(msh), str(out), "periodic"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["periodic_links"] == len(links) and info["faces"] == 2 * n * n
+    got = np.fromfile(out, dtype=np.float64).reshape(-1, 6).astype(np.int64)
+    want = O.pair_periodic_nodes(O.build_faces(m.elem_nodes), m.elem_nodes, m.xy, links)
+    assert np.array_equal(got[:, :4], want.as_array())
+    assert np.all(got[:, 4] == 0) and np.all(got[:, 5] == -1)   # every face interior
+
+
 def _run(tmp_path, kind, n, p, steps, dt_max, prec):
     prefix = str(tmp_path / f"{kind}{n}p{p}")
     print(f"driver: {kind} n={n} P{p} steps={steps} {prec}", flush=True)

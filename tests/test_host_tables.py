@@ -57,6 +57,46 @@ def test_periodic_pairing_matches_oracle(nx, ny, jit):
     assert fp.shape[0] == 2 * nx * ny
 
 
+@pytest.mark.parametrize("kind,n,shear,jit", [("quad", 10, 0.0, 0.0), ("quad", 7, 0.0, 0.2), ("quad", 2, 0.0, 0.0),
+                                              ("quad", 1, 0.0, 0.0), ("sheared", 6, 0.45, 0.1),
+                                              ("sheared", 2, -0.7, 0.0), ("tri", 8, 0.0, 0.2), ("tri", 2, 0.0, 0.0)])
+def test_node_pairing_matches_oracle(kind, n, shear, jit):
+    """zj_pair_periodic_nodes against the oracle's restatement, bit for bit,
+    on square, sheared and triangle meshes with lattice links; on the
+    translation-periodic ones also against zj_pair_periodic."""
+    if kind == "tri":
+        m = cases.split_tri_mesh(n, jitter=jit, seed=3)
+        en, xy, nv = m.tris, m.xy, 3
+    elif kind == "quad":
+        m = cases.uniform_quad_mesh(n, n + 1, 0.0, 1.0, 0.0, 1.2, jitter=jit, seed=n)
+        en, xy, nv = m.elem_nodes, m.xy, 4
+    else:
+        m = cases.sheared_quad_mesh(n, n + 1, 1.0, 1.2, shear, jitter=jit, seed=n)
+        en, xy, nv = m.elem_nodes, m.xy, 4
+    ny = n if kind == "tri" else n + 1
+    links = cases.periodic_links(n, ny)
+    f = _lib.build_faces(en, nv=nv)
+    got = _lib.pair_periodic_nodes(f, en, xy, links, nv=nv)
+    want = O.pair_periodic_nodes(O.build_faces(en, nv=nv), en, xy, links, nv=nv)
+    assert np.array_equal(got, want.as_array())
+    assert np.all(got[:, 2] >= 0)
+    if kind != "sheared":
+        L = (1.0, 1.0) if kind == "tri" else (1.0, 1.2)
+        assert np.array_equal(got, _lib.pair_periodic(f, en, xy, L[0], L[1], nv=nv))
+
+
+def test_node_pairing_argument_checks():
+    mesh = cases.uniform_quad_mesh(2, 2, 0.0, 2.0, 0.0, 2.0)
+    f = _lib.build_faces(mesh.elem_nodes)
+    with pytest.raises(_lib.ZJError):   # a link to a node that does not exist
+        _lib.pair_periodic_nodes(f, mesh.elem_nodes, mesh.xy, [[2, 99]])
+    with pytest.raises(_lib.ZJError):   # half-period links: two candidates
+        _lib.pair_periodic_nodes(f, mesh.elem_nodes, mesh.xy,
+                                 np.concatenate([cases.periodic_links(2, 2), [[1, 0], [4, 3], [7, 6]]]))
+    same = _lib.pair_periodic_nodes(f, mesh.elem_nodes, mesh.xy, np.zeros((0, 2), np.int64))
+    assert np.array_equal(same, f)
+
+
 def test_tables_match_oracle_case():
     from tests import fr_case
     mesh = cases.uniform_quad_mesh(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.2, seed=4)

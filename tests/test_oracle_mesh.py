@@ -115,3 +115,82 @@ def test_periodic_pairing_vortex_counts():
     assert f0.n == 220 and int(np.sum(f0.re < 0)) == 40
     f = O.pair_periodic(f0, mesh.elem_nodes, mesh.xy, 10.0, 10.0)
     assert f.n == 200 and np.all(f.re >= 0)
+
+
+@pytest.mark.parametrize("nx,ny,jit", [(10, 10, 0.0), (3, 5, 0.0), (6, 4, 0.2), (2, 3, 0.1), (2, 2, 0.0),
+                                       (1, 1, 0.0), (2, 1, 0.0)])
+def test_node_pairing_equals_period_pairing(nx, ny, jit):
+    """On a translation-periodic mesh, pairing by the lattice's node links
+    gives the same face table as pairing by the periods, including meshes one
+    or two elements across, where the end-node classes alone do not decide."""
+    mesh = cases.uniform_quad_mesh(nx, ny, -1.0, 2.0, 0.5, 2.5, jitter=jit, seed=5)
+    f0 = O.build_faces(mesh.elem_nodes)
+    geo = O.pair_periodic(f0, mesh.elem_nodes, mesh.xy, mesh.Lx, mesh.Ly, 1e-9)
+    nod = O.pair_periodic_nodes(f0, mesh.elem_nodes, mesh.xy, cases.periodic_links(nx, ny))
+    assert np.array_equal(geo.as_array(), nod.as_array())
+    links = cases.periodic_links(nx, ny)[::-1, ::-1]   # other order, slave and master swapped
+    assert np.array_equal(O.pair_periodic_nodes(f0, mesh.elem_nodes, mesh.xy, links).as_array(),
+                          geo.as_array())
+
+
+@pytest.mark.parametrize("nx,ny,shear,jit", [(4, 3, 0.37, 0.0), (5, 6, -1.3, 0.2), (2, 2, 0.5, 0.0)])
+def test_node_pairing_on_sheared_mesh(nx, ny, shear, jit):
+    """A parallelogram domain: every edge is covered once, kept faces keep
+    their order and the lower index, and partner midpoints differ by one of
+    the spanning vectors (Lx, 0) or (shear, Ly).  Pairing by the axis
+    periods pairs only the x sides here."""
+    mesh = cases.sheared_quad_mesh(nx, ny, 2.0, 1.5, shear, jitter=jit, seed=2)
+    en, xy = mesh.elem_nodes, mesh.xy
+    f0 = O.build_faces(en)
+    f = O.pair_periodic_nodes(f0, en, xy, cases.periodic_links(nx, ny))
+    E = en.shape[0]
+    assert f.n == 2 * E
+    seen = np.zeros((E, 4), int)
+    for a in range(f.n):
+        seen[f.le[a], f.ll[a]] += 1
+        seen[f.re[a], f.rl[a]] += 1
+    assert np.all(seen == 1)
+    rows = [tuple(r[:2]) for r in f0.as_array()]
+    orig = {r: i for i, r in enumerate(rows)}
+    idx = [orig[(f.le[a], f.ll[a])] for a in range(f.n)]
+    assert idx == sorted(idx)
+    mid = lambda e, k: 0.5 * (xy[en[e, k]] + xy[en[e, (k + 1) % 4]])
+    spans = [np.array([2.0, 0.0]), np.array([shear, 1.5])]
+    n_periodic = 0
+    for a in range(f.n):
+        d = mid(f.re[a], f.rl[a]) - mid(f.le[a], f.ll[a])
+        if f0.re[orig[(f.le[a], f.ll[a])]] >= 0:
+            continue  # interior before pairing
+        n_periodic += 1
+        assert orig[(f.le[a], f.ll[a])] < orig[(f.re[a], f.rl[a])]
+        assert any(np.allclose(d, v) or np.allclose(d, -v) for v in spans)
+    assert n_periodic == nx + ny
+    by_axis = O.pair_periodic(f0, en, xy, 2.0, 1.5, 1e-9)
+    assert by_axis.n == f.n + nx   # the sheared top and bottom rows stay unpaired
+
+
+@pytest.mark.parametrize("nx", [2, 3])
+def test_node_pairing_leaves_walls_alone(nx):
+    """Periodic in x only, walls top and bottom: with two elements across,
+    both bottom wall edges join a corner class to the middle class, yet they
+    are not translates by a link offset, so every wall edge stays a boundary
+    face."""
+    mesh = cases.uniform_quad_mesh(nx, 3, 0.0, 1.0, 0.0, 1.0)
+    links = cases.periodic_links(nx, 3)[:4]   # the column links only
+    f0 = O.build_faces(mesh.elem_nodes)
+    f = O.pair_periodic_nodes(f0, mesh.elem_nodes, mesh.xy, links)
+    geo = O.pair_periodic(f0, mesh.elem_nodes, mesh.xy, 1.0, 0.0, 1e-9)
+    assert np.array_equal(f.as_array(), geo.as_array())
+    assert int(np.sum(f.re < 0)) == 2 * nx
+
+
+def test_node_pairing_ambiguous_and_unlinked():
+    """Links that also identify nodes half a period apart give a face two
+    candidates (refused); without links nothing changes."""
+    mesh = cases.uniform_quad_mesh(2, 2, 0.0, 2.0, 0.0, 2.0)
+    links = np.concatenate([cases.periodic_links(2, 2), [[1, 0], [4, 3], [7, 6]]])
+    f0 = O.build_faces(mesh.elem_nodes)
+    with pytest.raises(ValueError):
+        O.pair_periodic_nodes(f0, mesh.elem_nodes, mesh.xy, links)
+    same = O.pair_periodic_nodes(f0, mesh.elem_nodes, mesh.xy, np.zeros((0, 2), np.int64))
+    assert np.array_equal(same.as_array(), f0.as_array())

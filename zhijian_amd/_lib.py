@@ -87,7 +87,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_tri_set_xcd_groups", "zj_tri_get_info", "zj_tri_operators", "zj_get_scheme",
            "zj_tri_get_scheme", "zj_io_last_error", "zj_restart_write", "zj_restart_query", "zj_restart_read",
            "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
-           "zj_pvtu_write")
+           "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes")
 
 
 def lib() -> C.CDLL:
@@ -139,6 +139,9 @@ def lib() -> C.CDLL:
         L.zj_gmsh_elements.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.zj_gmsh_edge_tag.argtypes = [H, C.c_int64, C.c_int64, C.POINTER(C.c_int)]
         L.zj_gmsh_group.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
+        L.zj_gmsh_periodic.argtypes = [H, C.POINTER(C.c_int64), C.c_void_p, C.c_void_p]
+        L.zj_pair_periodic_nodes.argtypes = [C.POINTER(C.c_int), _ip, _ip, _ip, _ip, C.c_int, C.c_int, _lp, _dp,
+                                             C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_double]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_get_tiling.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
         L.zj_quad_operators.argtypes = [C.c_int] + [_dp] * 7
@@ -229,6 +232,24 @@ def pair_periodic(faces: np.ndarray, elem_nodes, xy, Lx, Ly, tol=None, nv=None):
     tol = 1e-9 * max(Lx, Ly) if tol is None else tol
     check(lib().zj_pair_periodic(C.byref(nf), *arrs, nv, stride, en.reshape(-1),
                                  np.ascontiguousarray(xy, np.float64).reshape(-1), Lx, Ly, tol))
+    return np.stack([a[: nf.value] for a in arrs], axis=1)
+
+
+def pair_periodic_nodes(faces: np.ndarray, elem_nodes, xy, links, tol=None, nv=None):
+    """zj_pair_periodic_nodes: links [n][2] (slave node, master node); tol
+    defaults to 1e-9 of the mesh's extent."""
+    en = np.ascontiguousarray(elem_nodes, np.int64)
+    stride = en.shape[1]
+    nv = stride if nv is None else nv
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    if tol is None:
+        tol = 1e-9 * float(np.ptp(xy, axis=0).max()) if xy.size else 0.0
+    lk = np.ascontiguousarray(np.asarray(links, np.int64).reshape(-1, 2))
+    sl, ms = np.ascontiguousarray(lk[:, 0]), np.ascontiguousarray(lk[:, 1])
+    arrs = [np.ascontiguousarray(faces[:, k], np.int32).copy() for k in range(4)]
+    nf = C.c_int(faces.shape[0])
+    check(lib().zj_pair_periodic_nodes(C.byref(nf), *arrs, nv, stride, en.reshape(-1), xy.reshape(-1), xy.shape[0],
+                                       lk.shape[0], sl.ctypes.data, ms.ctypes.data, float(tol)))
     return np.stack([a[: nf.value] for a in arrs], axis=1)
 
 

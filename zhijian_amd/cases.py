@@ -61,6 +61,30 @@ def uniform_quad_mesh(nx: int, ny: int, x0: float, x1: float, y0: float, y1: flo
     return QuadMesh(xy, en, nx, ny, x1 - x0, y1 - y0)
 
 
+def sheared_quad_mesh(nx: int, ny: int, Lx: float, Ly: float, shear: float, jitter: float = 0.0,
+                      seed: int = 0) -> QuadMesh:
+    """nx*ny parallelogram cells of the domain spanned by (Lx, 0) and
+    (shear, Ly), node (i, j) at (i Lx/nx + j shear/ny, j Ly/ny), numbered as
+    uniform_quad_mesh; interior nodes optionally jittered.  Periodic in both
+    spanning directions: the top row is the bottom row shifted by (shear, Ly),
+    so pairing by an axis period misses it and pairing by node links
+    (periodic_links) is needed."""
+    m = uniform_quad_mesh(nx, ny, 0.0, Lx, 0.0, Ly, jitter=jitter, seed=seed)
+    xy = m.xy.copy()
+    xy[:, 0] += shear * xy[:, 1] / Ly
+    return QuadMesh(xy, m.elem_nodes, nx, ny, Lx, Ly)
+
+
+def periodic_links(nx: int, ny: int) -> np.ndarray:
+    """(slave, master) node links of an (nx+1) x (ny+1) lattice numbered
+    i + j (nx+1), periodic in i and j: the last column onto the first and the
+    last row onto the first, corners included, as a Gmsh $Periodic section
+    lists them."""
+    col = [((j * (nx + 1) + nx), j * (nx + 1)) for j in range(ny + 1)]
+    row = [((ny * (nx + 1) + i), i) for i in range(nx + 1)]
+    return np.array(col + row, np.int64)
+
+
 def gauss_points(n: int) -> np.ndarray:
     """1D Gauss-Legendre points (numpy's table; used only to place ICs)."""
     return np.polynomial.legendre.leggauss(n)[0]

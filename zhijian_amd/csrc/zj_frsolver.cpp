@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -81,19 +82,49 @@ void Mesh::buildFaces() {
 }
 
 void Mesh::pairPeriodic(Real Lx, Real Ly, Real tol) {
+    if (tol < 0) tol = 1e-9 * std::max(Lx, Ly);
+    pairFaces([&](int* nf, int* le, int* ll, int* re, int* rl, int nv, const int64_t* en) {
+        const std::vector<double> xy = flat_xy(nodes_);
+        return zj_pair_periodic(nf, le, ll, re, rl, nv, nv, en, xy.data(), Lx, Ly, tol);
+    }, "pairPeriodic");
+}
+
+void Mesh::pairPeriodicNodes() { pairPeriodicNodes(periodic_nodes_); }
+
+void Mesh::pairPeriodicNodes(const std::vector<std::pair<Index, Index>>& links) {
+    std::vector<int64_t> slave(links.size()), master(links.size());
+    for (size_t i = 0; i < links.size(); ++i) {
+        slave[i] = links[i].first;
+        master[i] = links[i].second;
+    }
+    const std::vector<double> xy = flat_xy(nodes_);
+    double lo[2] = {0, 0}, hi[2] = {0, 0};
+    for (size_t i = 0; i < nodes_.size(); ++i)
+        for (int d = 0; d < 2; ++d) {
+            lo[d] = i ? std::min(lo[d], xy[2 * i + d]) : xy[2 * i + d];
+            hi[d] = i ? std::max(hi[d], xy[2 * i + d]) : xy[2 * i + d];
+        }
+    const double tol = 1e-9 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
+    pairFaces([&](int* nf, int* le, int* ll, int* re, int* rl, int nv, const int64_t* en) {
+        return zj_pair_periodic_nodes(nf, le, ll, re, rl, nv, nv, en, xy.data(), numNodes(),
+                                      (int64_t)links.size(), slave.data(), master.data(), tol);
+    }, "pairPeriodicNodes");
+}
+
+// Runs a pairing routine on the face table and keeps each paired face's BC
+// fields as an interior face's.
+void Mesh::pairFaces(const std::function<int(int*, int*, int*, int*, int*, int, const int64_t*)>& pair,
+                     const char* what) {
     int nv;
     std::vector<int64_t> en = corners(elements_, nv);
-    std::vector<double> xy = flat_xy(nodes_);
     const int F = (int)faces_.size();
     std::vector<int> le(std::max(F, 1)), ll(le.size()), re(le.size()), rl(le.size());
     for (int f = 0; f < F; ++f) {
         le[f] = faces_[f].left_elem; ll[f] = faces_[f].left_face;
         re[f] = faces_[f].right_elem; rl[f] = faces_[f].right_face;
     }
-    if (tol < 0) tol = 1e-9 * std::max(Lx, Ly);
     int nf = F;
-    need(zj_pair_periodic(&nf, le.data(), ll.data(), re.data(), rl.data(), nv, nv, en.data(), xy.data(), Lx, Ly, tol),
-         "pairPeriodic");
+    need(pair(&nf, le.data(), ll.data(), re.data(), rl.data(), nv, en.data()), what);
     std::vector<Face> out;
     out.reserve(nf);
     size_t src = 0;
@@ -478,6 +509,11 @@ bool GmshReader::read(const std::string& filename, Mesh& mesh) {
         el.partition = 0;
         mesh.elements().push_back(el);
     }
+    int64_t npl = 0;
+    zj_gmsh_periodic(g, &npl, nullptr, nullptr);
+    std::vector<int64_t> ps((size_t)npl), pm((size_t)npl);
+    zj_gmsh_periodic(g, &npl, ps.data(), pm.data());
+    for (int64_t i = 0; i < npl; ++i) mesh.periodicNodes().emplace_back(ps[(size_t)i], pm[(size_t)i]);
     for (int i = 0; i < ng; ++i) {
         int tag = 0, type = 0;
         char name[256];

@@ -10,7 +10,10 @@
 //     vertices, carrying the physical tag (v2: the first element tag; v4: the
 //     first physical tag of the curve entity);
 //   - boundary groups get the $PhysicalNames name (else "BC_<tag>") and a BC
-//     type inferred from it (gmsh_reader.cpp:117-158).
+//     type inferred from it (gmsh_reader.cpp:117-158);
+//   - $Periodic node correspondences (slave node -> master node) are kept as
+//     node-index pairs.  The reference skips this section; the pairs feed
+//     zj_pair_periodic_nodes, which joins the faces they make periodic.
 // Exposed through a small C ABI (include/zhijian_amd.h) and, in C++, through
 // zhijian::GmshReader (include/zhijian/mesh.hpp).
 #include "zhijian_amd.h"
@@ -47,6 +50,7 @@ struct Parsed {
     std::vector<Elem> elems;
     std::map<std::pair<int64_t, int64_t>, int> edge_tag;  // sorted vertex pair -> physical tag
     std::map<int, Group> groups;                           // physical tag -> group
+    std::vector<std::pair<int64_t, int64_t>> periodic;     // (slave, master) node indices
     std::string err;
 };
 
@@ -111,6 +115,7 @@ struct State {
     std::map<int, int> curve_phys;             // v4 curve entity -> physical tag
     std::map<std::pair<int, int>, std::string> names;
     std::vector<std::pair<std::vector<int64_t>, int>> bedges;  // (vertex nodes, physical tag)
+    std::vector<std::pair<int64_t, int64_t>> periodic_tags;    // $Periodic (slave, master) node tags
 };
 
 bool add_element(State& st, int type, int phys, std::istringstream& ls) {
@@ -273,6 +278,45 @@ bool read_elements(Reader& r, State& st, bool v4) {
     return true;
 }
 
+// $Periodic.  MSH 2.2: per link "dim slave master", an optional "Affine ..."
+// line, the node count, then "slave master" node-tag lines.  MSH 4.1: per link
+// "dim tag master_tag", a line "n_affine values...", the node count, then the
+// node-tag lines.  Only the node correspondences are kept.
+bool read_periodic(Reader& r, State& st, bool v4) {
+    std::string s, &err = st.out.err;
+    if (!r.need(s, "$Periodic", err)) return false;
+    std::istringstream h(s);
+    int64_t links = -1;
+    if (!(h >> links) || links < 0) { err = "malformed $Periodic header"; return false; }
+    for (int64_t k = 0; k < links; ++k) {
+        if (!r.need(s, "$Periodic", err)) return false;
+        std::istringstream ls(s);
+        int dim, tag, master;
+        if (!(ls >> dim >> tag >> master)) { err = "malformed periodic link: " + s; return false; }
+        if (!r.need(s, "$Periodic", err)) return false;
+        if (v4) {
+            std::istringstream as(s);
+            int64_t n_affine = -1;
+            if (!(as >> n_affine) || n_affine < 0) { err = "malformed periodic affine line: " + s; return false; }
+            if (!r.need(s, "$Periodic", err)) return false;
+        } else if (s.compare(0, 6, "Affine") == 0) {
+            if (!r.need(s, "$Periodic", err)) return false;
+        }
+        std::istringstream cs(s);
+        int64_t n = -1;
+        if (!(cs >> n) || n < 0) { err = "malformed periodic node count: " + s; return false; }
+        for (int64_t i = 0; i < n; ++i) {
+            if (!r.need(s, "$Periodic", err)) return false;
+            std::istringstream ns(s);
+            int64_t a, b;
+            if (!(ns >> a >> b)) { err = "malformed periodic node pair: " + s; return false; }
+            st.periodic_tags.emplace_back(a, b);
+        }
+    }
+    r.skip_to("$EndPeriodic");
+    return true;
+}
+
 }  // namespace
 
 bool parse(const std::string& path, Parsed& out) {
@@ -354,6 +398,9 @@ bool parse(const std::string& path, Parsed& out) {
             if (have_elems) { out.err = "more than one $Elements section"; return false; }
             if (!read_elements(r, st, version >= 4.0)) return false;
             have_elems = true;
+        } else if (has(s, "$Periodic")) {
+            if (version == 0.0) { out.err = "$Periodic before $MeshFormat"; return false; }
+            if (!read_periodic(r, st, version >= 4.0)) return false;
         } else if (!has(s, "$End")) {
             r.skip_to("$End" + s.substr(1));
         }
@@ -362,6 +409,15 @@ bool parse(const std::string& path, Parsed& out) {
     if (!have_nodes) { out.err = "no $Nodes section"; return false; }
     if (!have_elems) { out.err = "no $Elements section"; return false; }
     if (out.elems.empty()) { out.err = "no 2D elements (triangles or quadrilaterals) in the file"; return false; }
+    for (const auto& pt : st.periodic_tags) {
+        auto a = st.node_index.find(pt.first), b = st.node_index.find(pt.second);
+        if (a == st.node_index.end() || b == st.node_index.end()) {
+            out.err = "$Periodic names unknown node tag " +
+                      std::to_string(a == st.node_index.end() ? pt.first : pt.second);
+            return false;
+        }
+        out.periodic.emplace_back(a->second, b->second);
+    }
     std::set<int> tags;
     for (const auto& be : st.bedges)
         if (be.second > 0) tags.insert(be.second);
@@ -452,6 +508,16 @@ extern "C" int zj_gmsh_edge_tag(zj_gmsh* g, int64_t a, int64_t b, int* tag) {
     if (!g || !tag) return ZJ_ERR_ARG;
     auto it = g->p.edge_tag.find({std::min(a, b), std::max(a, b)});
     *tag = it == g->p.edge_tag.end() ? 0 : it->second;
+    return ZJ_OK;
+}
+
+extern "C" int zj_gmsh_periodic(zj_gmsh* g, int64_t* n_pairs, int64_t* slave, int64_t* master) {
+    if (!g || !n_pairs) return ZJ_ERR_ARG;
+    *n_pairs = (int64_t)g->p.periodic.size();
+    for (size_t i = 0; i < g->p.periodic.size(); ++i) {
+        if (slave) slave[i] = g->p.periodic[i].first;
+        if (master) master[i] = g->p.periodic[i].second;
+    }
     return ZJ_OK;
 }
 

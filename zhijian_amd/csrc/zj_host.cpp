@@ -5,7 +5,9 @@
 // the tables agree with what the reference uploads:
 //   operators   src/basis/basis.cpp:12-104 (Gauss), 169-224 (Lagrange),
 //               314-379 (Radau correction derivative), 464-613 (quad tables)
-//   faces       src/mesh/mesh.cpp:18-83 (std::map order of sorted vertex pairs)
+//   faces       src/mesh/mesh.cpp:18-83 (std::map order of sorted vertex pairs);
+//               periodic pairing (by period or by node links) has no
+//               counterpart in the reference
 //   normals     mesh.cpp:159-183;  areas mesh.cpp:122-146
 //   geometry    src/solver/fr_solver.cpp:85-129, basis.cpp:717-863
 #include "zhijian_amd.h"
@@ -151,6 +153,111 @@ extern "C" int zj_pair_periodic(int* n_faces, int* le, int* ll, int* re, int* rl
         if (dead[f]) continue;
         int g = partner[f];
         int nre = g >= 0 ? le[g] : re[f], nrl = g >= 0 ? ll[g] : rl[f];
+        le[out] = le[f]; ll[out] = ll[f]; re[out] = nre; rl[out] = nrl; ++out;
+    }
+    *n_faces = out;
+    return ZJ_OK;
+}
+
+// Pairing by node correspondence (a mesh file's $Periodic links).  Two
+// boundary faces are partners when
+//   - their end nodes correspond: the links join nodes into classes
+//     (transitively, so a corner of a doubly periodic mesh is one class with
+//     its three images), and both faces have the same two end-node classes;
+//   - and one is the other translated: both end nodes move by the same
+//     vector, which is (up to sign and tol) the offset master - slave of some
+//     link.
+// The first rule alone would pair the two wall edges of a mesh two elements
+// across a periodic direction; the second alone would pair faces the file
+// does not link.  More than one candidate is an error.  Kept face, removal
+// and order follow zj_pair_periodic, so on a translation-periodic mesh both
+// give the same table.
+extern "C" int zj_pair_periodic_nodes(int* n_faces, int* le, int* ll, int* re, int* rl, int nv, int stride,
+                                      const int64_t* en, const double* xy, int64_t n_nodes, int64_t n_pairs,
+                                      const int64_t* slave, const int64_t* master, double tol) {
+    if (!n_faces || *n_faces < 0 || nv < 3 || stride < nv || n_nodes < 0 || n_pairs < 0 || !(tol >= 0.0) ||
+        (n_pairs > 0 && (!slave || !master || !xy)) || (*n_faces > 0 && (!le || !ll || !re || !rl || !en)))
+        return ZJ_ERR_ARG;
+    const int nf = *n_faces;
+    std::vector<int64_t> parent((size_t)n_nodes);
+    std::iota(parent.begin(), parent.end(), int64_t(0));
+    auto root = [&](int64_t a) {
+        while (parent[(size_t)a] != a) {
+            parent[(size_t)a] = parent[(size_t)parent[(size_t)a]];
+            a = parent[(size_t)a];
+        }
+        return a;
+    };
+    std::vector<std::pair<double, double>> shifts;   // distinct link offsets, one sign
+    auto close = [&](double x0, double y0, double x1, double y1) {
+        return std::fabs(x0 - x1) <= tol && std::fabs(y0 - y1) <= tol;
+    };
+    for (int64_t i = 0; i < n_pairs; ++i) {
+        const int64_t a = slave[i], b = master[i];
+        if (a < 0 || a >= n_nodes || b < 0 || b >= n_nodes) return ZJ_ERR_ARG;
+        const int64_t ra = root(a), rb = root(b);
+        if (ra != rb) parent[(size_t)std::max(ra, rb)] = std::min(ra, rb);
+        const double dx = xy[2 * b] - xy[2 * a], dy = xy[2 * b + 1] - xy[2 * a + 1];
+        bool known = false;
+        for (const auto& v : shifts)
+            if (close(dx, dy, v.first, v.second) || close(-dx, -dy, v.first, v.second)) { known = true; break; }
+        if (!known) shifts.emplace_back(dx, dy);
+    }
+    struct K { int64_t lo, hi; int f; };
+    std::vector<K> keys;
+    for (int f = 0; f < nf; ++f) {
+        if (re[f] >= 0) continue;
+        const int64_t v0 = node_of(en, stride, le[f], ll[f]), v1 = node_of(en, stride, le[f], (ll[f] + 1) % nv);
+        if (v0 < 0 || v0 >= n_nodes || v1 < 0 || v1 >= n_nodes) return ZJ_ERR_ARG;
+        const int64_t a = root(v0), b = root(v1);
+        keys.push_back({std::min(a, b), std::max(a, b), f});
+    }
+    std::stable_sort(keys.begin(), keys.end(), [](const K& x, const K& y) {
+        return x.lo != y.lo ? x.lo < y.lo : x.hi < y.hi;
+    });
+    // g is f translated by a link offset: both end nodes move by +-v
+    auto translate_of = [&](int f, int g) {
+        const int64_t a0 = node_of(en, stride, le[f], ll[f]), a1 = node_of(en, stride, le[f], (ll[f] + 1) % nv);
+        const int64_t b0 = node_of(en, stride, le[g], ll[g]), b1 = node_of(en, stride, le[g], (ll[g] + 1) % nv);
+        for (int swap = 0; swap < 2; ++swap) {
+            const int64_t c0 = swap ? b1 : b0, c1 = swap ? b0 : b1;
+            const double dx = xy[2 * c0] - xy[2 * a0], dy = xy[2 * c0 + 1] - xy[2 * a0 + 1];
+            if (!close(xy[2 * c1] - xy[2 * a1], xy[2 * c1 + 1] - xy[2 * a1 + 1], dx, dy)) continue;
+            for (const auto& v : shifts)
+                if (close(dx, dy, v.first, v.second) || close(-dx, -dy, v.first, v.second)) return true;
+        }
+        return false;
+    };
+    std::vector<char> dead((size_t)std::max(nf, 1), 0), taken(dead.size(), 0);
+    std::vector<int> partner((size_t)std::max(nf, 1), -1);
+    std::vector<int> group;
+    for (size_t i = 0; i < keys.size();) {
+        size_t j = i + 1;
+        while (j < keys.size() && keys[j].lo == keys[i].lo && keys[j].hi == keys[i].hi) ++j;
+        group.clear();
+        for (size_t k = i; k < j; ++k) group.push_back(keys[k].f);   // ascending face order
+        for (size_t x = 0; x < group.size(); ++x) {
+            const int f = group[x];
+            if (taken[(size_t)f]) continue;
+            int found = -1, count = 0;
+            for (size_t y = x + 1; y < group.size(); ++y) {
+                const int g = group[y];
+                if (!taken[(size_t)g] && translate_of(f, g)) { found = g; ++count; }
+            }
+            if (count > 1) return ZJ_ERR_ARG;  // ambiguous partners
+            if (count == 1) {
+                partner[(size_t)f] = found;
+                taken[(size_t)f] = taken[(size_t)found] = 1;
+                dead[(size_t)found] = 1;
+            }
+        }
+        i = j;
+    }
+    int out = 0;
+    for (int f = 0; f < nf; ++f) {
+        if (dead[(size_t)f]) continue;
+        const int g = partner[(size_t)f];
+        const int nre = g >= 0 ? le[g] : re[f], nrl = g >= 0 ? ll[g] : rl[f];
         le[out] = le[f]; ll[out] = ll[f]; re[out] = nre; rl[out] = nrl; ++out;
     }
     *n_faces = out;

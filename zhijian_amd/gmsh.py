@@ -26,20 +26,25 @@ class GmshMesh:
     elem_n_nodes: np.ndarray
     groups: Dict[int, Tuple[str, int]]  # physical tag -> (name, BC type)
     _edge_tags: Dict[Tuple[int, int], int]
+    periodic_nodes: np.ndarray = None   # [n][2] $Periodic (slave, master) node indices; the reference skips them
 
     def edge_tag(self, a: int, b: int) -> int:
         return self._edge_tags.get((min(a, b), max(a, b)), 0)
 
-    def faces(self):
+    def faces(self, periodic: bool = False):
         """Faces (zj_build_faces) with each boundary face's physical tag and BC
         type stamped as GmshReader::applyBoundaryTags does; untagged boundary
         faces keep buildFaces' FarField / tag 0, interior faces Interior / tag -1
-        (mesh.cpp:40-57)."""
+        (mesh.cpp:40-57).  periodic=True first joins the faces the file's
+        $Periodic links make correspond (zj_pair_periodic_nodes)."""
         kinds = set(self.elem_kind.tolist())
         if len(kinds) != 1:
             raise ValueError("mixed triangle/quadrilateral meshes are not supported here")
         nv = 3 if 0 in kinds else 4
-        f = _lib.build_faces(np.ascontiguousarray(self.elem_nodes[:, :nv]), nv=nv)
+        en = np.ascontiguousarray(self.elem_nodes[:, :nv])
+        f = _lib.build_faces(en, nv=nv)
+        if periodic:
+            f = _lib.pair_periodic_nodes(f, en, self.xy, self.periodic_nodes, nv=nv)
         tag = np.full(f.shape[0], -1, np.int32)   # interior faces: tag -1, type Interior
         typ = np.zeros(f.shape[0], np.int32)
         for i in np.nonzero(f[:, 2] < 0)[0]:
@@ -87,6 +92,12 @@ def read(path: str) -> GmshMesh:
                 L.zj_gmsh_edge_tag(h, a, b, C.byref(t))
                 if t.value:
                     edge_tags[(min(a, b), max(a, b))] = t.value
-        return GmshMesh(xy, nodes, kind, order, cnt, groups, edge_tags)
+        npl = C.c_int64()
+        L.zj_gmsh_periodic(h, C.byref(npl), None, None)
+        links = np.zeros((npl.value, 2), np.int64)
+        sl, ms = np.zeros(npl.value, np.int64), np.zeros(npl.value, np.int64)
+        L.zj_gmsh_periodic(h, C.byref(npl), sl.ctypes.data, ms.ctypes.data)
+        links[:, 0], links[:, 1] = sl, ms
+        return GmshMesh(xy, nodes, kind, order, cnt, groups, edge_tags, links)
     finally:
         L.zj_gmsh_free(h)

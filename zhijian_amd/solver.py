@@ -80,12 +80,18 @@ def bc_tables(faces: np.ndarray, bc_type_per_face=None, fs=None, outflow_p=None)
 
 
 def build_tables(xy, elem_nodes, p, periodic=None, bc_type_per_face=None, fs=None,
-                 node_count=None, want_pos=True) -> Tables:
+                 node_count=None, want_pos=True, periodic_nodes=None) -> Tables:
+    """Host tables of a quadrilateral mesh.  periodic=(Lx, Ly) joins boundary
+    faces one period apart; periodic_nodes=[n][2] (slave, master) node links,
+    e.g. a Gmsh file's $Periodic section, joins the faces whose end nodes
+    correspond."""
     en = np.ascontiguousarray(elem_nodes, np.int64)
     ops = _lib.quad_operators(p)
     faces = _lib.build_faces(en[:, :4], nv=4)
     if periodic is not None:
         faces = _lib.pair_periodic(faces, en[:, :4], xy, periodic[0], periodic[1])
+    if periodic_nodes is not None:
+        faces = _lib.pair_periodic_nodes(faces, en[:, :4], xy, periodic_nodes)
     normals = _lib.face_normals(faces, en[:, :4], xy, nv=4)
     geo = _lib.geometry(en, xy, ops, node_count=node_count, want_pos=want_pos)
     area = _lib.elem_areas(en[:, :4], xy, nv=4)

@@ -27,12 +27,16 @@ def _check(rc, h=None):
     return rc
 
 
-def tri_faces(xy, tris, periodic=None):
+def tri_faces(xy, tris, periodic=None, periodic_nodes=None):
     """Faces [F][4] (left elem, left edge, right elem, right edge) and unit
-    normals [F][2]; `periodic` = (Lx, Ly) pairs the boundary faces."""
+    normals [F][2]; `periodic` = (Lx, Ly) pairs the boundary faces one period
+    apart, `periodic_nodes` = [n][2] (slave, master) node links those whose
+    end nodes correspond."""
     faces = _lib.build_faces(tris, nv=3)
     if periodic is not None:
         faces = _lib.pair_periodic(faces, tris, xy, periodic[0], periodic[1], nv=3)
+    if periodic_nodes is not None:
+        faces = _lib.pair_periodic_nodes(faces, tris, xy, periodic_nodes, nv=3)
     return faces, _lib.face_normals(faces, tris, xy, nv=3)
 
 
