@@ -219,7 +219,7 @@ def tri_alg_bytes_per_stage(E, p, stage, precision, cross_edges):
 
 
 def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
-    """Triangles (textbook FR, DG-equivalent correction, HLLC): n x n periodic
+    """Triangles (textbook FR, GA correction field, HLLC): n x n periodic
     unit square split along the cell diagonals, interior nodes jittered
     U(-0.2h, 0.2h) by PCG64(2024); Mach 0.5 at 2 degrees with 8 Fourier
     velocity modes of amplitude 0.05 (PCG64(2024)).  FP64 vs FP32-bulk time
@@ -233,11 +233,12 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
     dof = m.n_elem * Np * 4
     out = {"workload": f"{n}x{n} split-quad periodic triangles ({m.n_elem} elements), P{p} ({Np} points), "
                        f"jitter 0.2h seed 2024, Mach 0.5 at 2 deg + 8 Fourier modes (amp 0.05, seed 2024), "
-                       f"textbook FR (DG correction), HLLC, inviscid, no limiter, CFL 0.5",
+                       f"textbook FR (GA correction, energy-stable family on triangles), HLLC, inviscid, "
+                       f"no limiter, CFL 0.5",
            "elements": m.n_elem, "dof": dof}
     sols = {}
     for name in ("fp64", "fp32bulk"):
-        s = TriSolver(m.xy, m.tris, p, periodic=(1.0, 1.0), riemann="hllc", precision=name)
+        s = TriSolver(m.xy, m.tris, p, periodic=(1.0, 1.0), riemann="hllc", precision=name, corr="ga")
         s.set_state(U0)
         if name == "fp64":
             out["dt"] = dt = s.dt_cfl(0.5)
@@ -277,7 +278,7 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
         ns = min(n, 64)
         ms_ = cases.split_tri_mesh(ns, jitter=0.2, seed=2024)
         faces, nrm = tri_faces(ms_.xy, ms_.tris, periodic=(1.0, 1.0))
-        o = T.TriTextbook(p, ms_.xy, ms_.tris, faces, nrm, riemann="hllc")
+        o = T.TriTextbook(p, ms_.xy, ms_.tris, faces, nrm, riemann="hllc", corr="ga")
         U = cases.fourier_freestream_ic(sp_coords(ms_.xy, ms_.tris, p), seed=2024).reshape(-1)
         U = o.advance(U, dt)
         k, t0 = 0, time.perf_counter()

@@ -262,10 +262,12 @@ int zj_gmsh_periodic(zj_gmsh_handle g, int64_t* n_pairs, int64_t* slave, int64_t
  * (FROperators::initTriangle, src/basis/basis.cpp:615-669); FRSolver assumes
  * quadrilaterals (fr_solver.cpp:138-140).  These entry points run the scheme
  * the reference's settings name on straight-sided triangles: nodal FR with the
- * DG-equivalent correction field (L2 lifting), warp-and-blend solution
- * points, p+1 Gauss flux points per edge, the configured Riemann solver
- * (euler.hpp:204-375), SSP-RK3 (include/time/rk_integrator.hpp:33-58).
- * Params: compat = ZJ_COMPAT_TEXTBOOK, corr = ZJ_CORR_DG, positivity = 0.
+ * energy-stable correction field of the configured family (Castonguay,
+ * Vincent and Jameson 2012: DG = the L2 lifting; SD = GA and HU with the
+ * one-dimensional VCJH c), warp-and-blend solution points, p+1 Gauss flux
+ * points per edge, the configured Riemann solver (euler.hpp:204-375), SSP-RK3
+ * (include/time/rk_integrator.hpp:33-58).
+ * Params: compat = ZJ_COMPAT_TEXTBOOK, corr = ZJ_CORR_*, positivity = 0.
  * State layout [E][Np][4], Np = (p+1)(p+2)/2, solution points in the order
  * zj_tri_operators returns.  Faces come from zj_build_faces (nv = 3) and
  * zj_pair_periodic; normals from zj_face_normals. */
@@ -310,6 +312,10 @@ int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_block, int* 
  * lift [Np][3(p+1)] (before the |e|/(2J) edge factor).  Any pointer may be NULL. */
 int zj_tri_operators(int p, double* sp, double* fp, double* wg, double* Dr, double* Ds,
                      double* interp, double* lift);
+/* The lifting matrix [Np][3(p+1)] of correction family `corr` (ZJ_CORR_*):
+ * the DG lifting filtered by (M + K)^-1 M, K the c-term of the family's
+ * energy norm; equal to zj_tri_operators' lift for ZJ_CORR_DG. */
+int zj_tri_lift(int p, int corr, double* lift);
 
 /* Polynomial order and gamma a handle was created with. */
 int zj_get_scheme(zj_handle h, int* poly_order, double* gamma);

@@ -23,6 +23,19 @@ follow.  This restates the textbook scheme for straight-sided triangles:
   LIFT[sp, j] = w_j sum_m psi_m(x_sp) psi_m(x_j), the L2 lifting of a unit
   value at flux point j.
 
+Correction fields other than DG (esfr_lift): the energy-stable family on
+triangles of Castonguay, Vincent & Jameson (J. Sci. Comput. 51, 2012), norm
+int_T u^2 + (c/2) sum_v C(p,v) (d^p u / dr^v ds^(p-v))^2, whose correction
+field is the DG lifting filtered by (M + K)^-1 M.  Restated nodally: M the
+nodal mass matrix (V V^T)^-1, K = c sum_v C(p,v) d_v d_v^T with d_v the
+p-th derivative of each nodal basis function, taken as a row of
+Dr^v Ds^(p-v) (constant over the element; the reference triangle's area 2
+times the c/2).  c is the one-dimensional VCJH value of the family (Vincent,
+Castonguay & Jameson, J. Sci. Comput. 47, 2011), SD = GA.  Pinned by c = 0
+giving the DG lifting, conservation (the volume weights of the lifting do
+not depend on c), K annihilating degree < p, linear stability and the
+library's independent modal construction.
+
 Boundary faces take the ghost state of the quadrilateral path
 (kernels.cu:453-525 via the C oracle) at each flux point.
 
@@ -185,6 +198,35 @@ def operators(p):
     return dict(sp=sp, fp=fp, wg=np.asarray(wg), Dr=Dr, Ds=Ds, interp=I, lift=LIFT, V=V)
 
 
+def vcjh_c(p, family):
+    """One-dimensional VCJH c: DG 0, SD = GA 2p/((2p+1)(p+1)(a_p p!)^2),
+    HU 2(p+1)/((2p+1) p (a_p p!)^2), a_p = C(2p, p) / 2^p."""
+    ap_pf = math.comb(2 * p, p) / 2 ** p * math.factorial(p)
+    if family == "dg":
+        return 0.0
+    if family in ("sd", "ga"):
+        return 2.0 * p / ((2 * p + 1) * (p + 1) * ap_pf ** 2)
+    if family == "hu":
+        return 2.0 * (p + 1) / ((2 * p + 1) * p * ap_pf ** 2)
+    raise ValueError(family)
+
+
+def esfr_lift(p, family, ops=None):
+    """Lifting [Np][3(p+1)] of the family's energy-stable correction field."""
+    ops = operators(p) if ops is None else ops
+    c = vcjh_c(p, family)
+    if c == 0.0:
+        return ops["lift"].copy()
+    V, Dr, Ds = ops["V"], ops["Dr"], ops["Ds"]
+    M = np.linalg.inv(V @ V.T)
+    K = np.zeros_like(M)
+    for v in range(p + 1):
+        D = np.linalg.matrix_power(Dr, v) @ np.linalg.matrix_power(Ds, p - v)
+        d = D.mean(axis=0)             # every row holds the same constant derivatives
+        K += c * math.comb(p, v) * np.outer(d, d)
+    return np.linalg.solve(M + K, M @ ops["lift"])
+
+
 # ------------------------------------------------------------ meshes
 def split_quad_mesh(n, jitter=0.0, seed=0, L=1.0):
     """n x n periodic square of side L, each cell split along (i,j)->(i+1,j+1)
@@ -230,9 +272,10 @@ class TriTextbook:
     """Textbook FR residual / SSP-RK3 on straight-sided triangles."""
 
     def __init__(self, p, xy, tris, faces, normals, riemann="roe", gamma=1.4, bc_type=None, bc_face_map=None,
-                 bc_data=None):
+                 bc_data=None, corr="dg"):
         self.p, self.g = p, gamma
         self.ops = operators(p)
+        self.ops["lift"] = esfr_lift(p, corr, self.ops)
         self.Np = self.ops["sp"].shape[0]
         self.Fp = p + 1
         self.E = tris.shape[0]

@@ -23,10 +23,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _periodic(n, p, jitter, seed, riemann, precision="fp64", order="hilbert", L=1.0, x0=0.0):
+def _periodic(n, p, jitter, seed, riemann, precision="fp64", order="hilbert", L=1.0, x0=0.0, corr="dg"):
     m = cases.split_tri_mesh(n, jitter=jitter, seed=seed, L=L, x0=x0)
-    s = TriSolver(m.xy, m.tris, p, periodic=(L, L), riemann=riemann, precision=precision, order=order)
-    o = T.TriTextbook(p, m.xy, m.tris, s.faces, s.normals, riemann=riemann)
+    s = TriSolver(m.xy, m.tris, p, periodic=(L, L), riemann=riemann, precision=precision, order=order, corr=corr)
+    o = T.TriTextbook(p, m.xy, m.tris, s.faces, s.normals, riemann=riemann, corr=corr)
     return s, o, m
 
 
@@ -144,3 +144,53 @@ def test_tri_tiny_periodic_meshes(n):
         r = rel(s.residual(), o.residual(U))
         say(f"  {riemann}: rel L2 {r:.3e}")
         assert r < TOL
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("corr", ["sd", "hu", "ga"])
+def test_tri_residual_correction_families(p, corr):
+    """Energy-stable correction fields (the lifting the handle reads is chosen
+    per handle): residual against the oracle's (M + K)^-1 M filtered lifting."""
+    say(f"case: triangle residual, periodic 6x6 jitter 0.2 P{p} hllc corr {corr}")
+    s, o, m = _periodic(6, p, 0.2, p, "hllc", corr=corr)
+    U = _state(m, p, 30 + p)
+    s.set_state(U)
+    r = rel(s.residual(), o.residual(U))
+    say(f"  rel L2 {r:.3e}")
+    assert r < TOL
+
+
+def test_tri_ga_steps_and_fp32bulk():
+    """The C3 configuration's correction (GA): 20 steps of the vortex against
+    the oracle, FP32-bulk against FP64 after 50 steps, and handles of two
+    families side by side keep their own lifting."""
+    say("case: triangle GA, vortex 16x16 P3 roe 20 steps; FP32-bulk vs FP64 50 steps; DG and GA handles")
+    s, o, m = _periodic(16, 3, 0.0, 0, "roe", L=10.0, x0=-5.0, corr="ga")
+    d, _, _ = _periodic(16, 3, 0.0, 0, "roe", L=10.0, x0=-5.0, corr="dg")
+    pos = sp_coords(m.xy, m.tris, 3)
+    U = cases.vortex_ic(pos).reshape(-1)
+    s.set_state(U)
+    d.set_state(U)
+    dt = 0.5 * s.dt_cfl(1.0)
+    V = U.copy()
+    for _ in range(20):
+        s.rk3_step(dt)
+        d.rk3_step(dt)
+        V = o.advance(V, dt)
+    r = rel(s.get_state(), V)
+    say(f"  GA rel L2 after 20 steps {r:.3e}")
+    assert r < 1e-10
+    assert rel(d.get_state(), V) > 1e-8   # the DG handle is a different scheme
+    m2 = cases.split_tri_mesh(24, jitter=0.2, seed=7)
+    a = TriSolver(m2.xy, m2.tris, 3, periodic=(1.0, 1.0), riemann="hllc", corr="ga")
+    b = TriSolver(m2.xy, m2.tris, 3, periodic=(1.0, 1.0), riemann="hllc", corr="ga", precision="fp32bulk")
+    W = cases.fourier_freestream_ic(sp_coords(m2.xy, m2.tris, 3), seed=1).reshape(-1)
+    a.set_state(W)
+    b.set_state(W)
+    dt2 = a.dt_cfl(0.5)
+    for _ in range(50):
+        a.rk3_step(dt2)
+        b.rk3_step(dt2)
+    r2 = rel(b.get_state(), a.get_state())
+    say(f"  GA FP32-bulk vs FP64 after 50 steps: rel L2 {r2:.3e}")
+    assert r2 < 1e-5

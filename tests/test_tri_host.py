@@ -21,6 +21,19 @@ def test_operators_match_oracle(p):
     assert np.abs(a["fp"] - b["fp"].reshape(-1, 2)).max() <= 1e-15
 
 
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_esfr_lift_matches_oracle(p):
+    """zj_tri_lift (modal construction: monomial fit of the Dubiner modes'
+    p-th derivatives) against the oracle's nodal one (powers of Dr and Ds)."""
+    ops = T.operators(p)
+    assert np.array_equal(_lib.tri_lift(p, 0), _lib.tri_operators(p)["lift"])
+    for code, fam in ((1, "sd"), (2, "hu"), (3, "ga")):
+        ref = T.esfr_lift(p, fam, ops)
+        assert np.abs(_lib.tri_lift(p, code) - ref).max() <= 1e-11 * np.abs(ref).max(), fam
+    with pytest.raises(ZJError):
+        _lib.tri_lift(p, 4)
+
+
 def test_mesh_matches_oracle_mesh():
     for n, j, seed in ((4, 0.2, 3), (7, 0.0, 0), (9, 0.15, 11)):
         m = cases.split_tri_mesh(n, jitter=j, seed=seed)
@@ -63,7 +76,7 @@ def test_create_rejects_quad_only_modes():
     d.n_faces = faces.shape[0]
     d.face_left_elem, d.face_left_local, d.face_right_elem, d.face_right_local = (c.ctypes.data for c in cols)
     d.normals = nrm.ctypes.data
-    for compat, positivity, corr, code in ((0, 0, 0, 4), (1, 1, 0, 4), (1, 0, 2, 4), (1, 0, 9, 4), (7, 0, 0, 4)):
+    for compat, positivity, corr, code in ((0, 0, 0, 4), (1, 1, 0, 4), (1, 0, 9, 1), (1, 0, -1, 1), (7, 0, 0, 4)):
         prm = _lib.Params(1.4, 0.5, 0, compat, positivity, 0, 1, corr)
         h = C.c_void_p()
         assert lib().zj_tri_create(C.byref(h), C.byref(d), C.byref(prm)) == code

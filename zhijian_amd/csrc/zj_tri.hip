@@ -1,12 +1,13 @@
 // zj_tri.hip -- flux reconstruction on straight-sided triangles (textbook
-// scheme, DG-equivalent correction) and its SSP-RK3 step: one fused kernel per
-// stage, C ABI zj_tri_* (include/zhijian_amd.h).
+// scheme) and its SSP-RK3 step: one fused kernel per stage, C ABI zj_tri_*
+// (include/zhijian_amd.h).
 //
 // The reference has no triangle semantics to follow: FROperators::initTriangle
 // leaves every table zero (src/basis/basis.cpp:615-669) and FRSolver assumes
 // quadrilaterals (src/solver/fr_solver.cpp:138-140).  The scheme is the one its
-// settings name on simplices: nodal FR with the L2 lifting as correction field
-// (Huynh 2007 with c = 0; Castonguay, Vincent & Jameson 2012), transformed
+// settings name on simplices: nodal FR with an energy-stable correction field
+// (Castonguay, Vincent & Jameson 2012; the L2 lifting for DG, c = 0, and its
+// (M + K)^-1 M filtering for SD = GA and HU, zj_tri_ops.hpp), transformed
 // fluxes through the constant affine metric, one common flux per face point
 // from the configured Riemann solver (include/physics/euler.hpp:204-375).
 //
@@ -76,14 +77,24 @@ constexpr int GEO = 10;           // geometry words per element
 //      interpolation to the flux points (NFP), contiguous -- the point loop of
 //      the volume phase reads one such block per iteration;
 //   B  the lifting matrix by columns: column j (NP values) contiguous;
-//   C  the interpolation matrix by rows (row j: NP values), for the traces.
+//   C  the interpolation matrix by rows (row j: NP values), for the traces;
+//   D  the lifting matrices of the SD (= GA) and HU correction fields, laid
+//      out as B, one after the other.
+// The kernel reads the lifting matrix at a per-handle offset (B or D).
 template <int P> struct TriOps {
     static constexpr int NP = (P + 1) * (P + 2) / 2, NFP = 3 * (P + 1);
     static constexpr int CA = 2 * NP + NFP;         // words per block of A
-    static constexpr int B = NP * CA, C = B + NFP * NP;
+    static constexpr int B = NP * CA, C = B + NFP * NP, D = C + NFP * NP;
 };
 constexpr int ops_words(int P) {
-    return 2 * ((P + 1) * (P + 2) / 2) * ((P + 1) * (P + 2) / 2) + 3 * 3 * (P + 1) * ((P + 1) * (P + 2) / 2);
+    return 2 * ((P + 1) * (P + 2) / 2) * ((P + 1) * (P + 2) / 2) + 5 * 3 * (P + 1) * ((P + 1) * (P + 2) / 2);
+}
+// Offset of a correction family's lifting matrix within the order-P block.
+inline int lift_offset(int P, int family) {
+    const int np = (P + 1) * (P + 2) / 2, nfp = 3 * (P + 1), b = np * (2 * np + nfp);
+    if (family == ZJ_CORR_DG) return b;
+    const int d = b + 2 * nfp * np;
+    return family == ZJ_CORR_HU ? d + nfp * np : d;   // SD and GA share one field
 }
 constexpr int ops_offset(int P) {
     int o = 0;
@@ -130,6 +141,7 @@ struct TriArgs {
     double gamma;
     int riemann;
     int xcd_groups;
+    int lift_off;         // the correction family's lifting matrix in the order's operator block
 };
 
 __device__ __forceinline__ int link_edge(int info) { return info & 7; }
@@ -153,7 +165,8 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
     constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
     constexpr int NW = NP > NFP ? NP : NFP;
     using OP = TriOps<P>;
-    constexpr int O0 = ops_offset(P), CA = OP::CA, OLT = O0 + OP::B, OI = O0 + OP::C;
+    constexpr int O0 = ops_offset(P), CA = OP::CA, OI = O0 + OP::C;
+    const int OLT = O0 + a.lift_off;
     __shared__ R sT[TILE * NFP * NV];  // own traces
     __shared__ R sW[TILE * NW * NV];   // the stage input state, then the transformed normal fluxes,
                                        // then the flux differences
@@ -481,8 +494,9 @@ hipError_t load_tri_ops(int device) {
     if (device >= 0 && device < 64 && (g_ops_loaded >> device) & 1) return hipSuccess;
     std::vector<double> all((size_t)OPS_TOTAL);
     for (int P = 1; P <= 5; ++P) {
-        tri::Operators O;
-        if (!tri::operators(P, O)) return hipErrorInvalidValue;
+        tri::Operators O, Osd, Ohu;
+        if (!tri::operators(P, O) || !tri::operators(P, Osd, ZJ_CORR_SD) || !tri::operators(P, Ohu, ZJ_CORR_HU))
+            return hipErrorInvalidValue;
         double* d = all.data() + ops_offset(P);
         const int np = O.np, nf = O.nf, ca = 2 * np + nf;   // TriOps<P> layout
         for (int m = 0; m < np; ++m) {
@@ -496,6 +510,12 @@ hipError_t load_tri_ops(int device) {
         for (int jj = 0; jj < nf; ++jj)
             for (int i = 0; i < np; ++i) b[(size_t)jj * np + i] = O.lift[(size_t)i * nf + jj];
         std::copy(O.interp.begin(), O.interp.end(), b + (size_t)nf * np);
+        for (int k = 0; k < 2; ++k) {
+            const tri::Operators& F = k ? Ohu : Osd;
+            double* dl = d + lift_offset(P, k ? ZJ_CORR_HU : ZJ_CORR_SD);
+            for (int jj = 0; jj < nf; ++jj)
+                for (int i = 0; i < np; ++i) dl[(size_t)jj * np + i] = F.lift[(size_t)i * nf + jj];
+        }
     }
     std::vector<float> allf(all.begin(), all.end());
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops64), all.data(), all.size() * sizeof(double), 0,
@@ -517,6 +537,7 @@ struct zj_tri {
     int prec = ZJ_PREC_FP64;
     double gamma = 1.4;
     int riemann = 0;
+    int corr = ZJ_CORR_DG;
     int xcd_groups = 1;
     int epb = 0, nblk = 0, nred = 0;
     int64_t cross_edges = 0;
@@ -596,8 +617,7 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
         return tfail(h, ZJ_ERR_UNSUPPORTED, "triangles exist in textbook mode only (the reference's triangle "
                                             "operators are zero)");
     if (p->riemann < 0 || p->riemann > 2) return tfail(h, ZJ_ERR_ARG, "unknown Riemann solver");
-    if (p->corr != ZJ_CORR_DG)
-        return tfail(h, ZJ_ERR_UNSUPPORTED, "triangles use the DG-equivalent correction field only");
+    if (p->corr < ZJ_CORR_DG || p->corr > ZJ_CORR_GA) return tfail(h, ZJ_ERR_ARG, "unknown correction function");
     if (p->positivity) return tfail(h, ZJ_ERR_UNSUPPORTED, "the positivity limiter is quadrilateral-only");
     if (!(p->gamma > 1.0)) return tfail(h, ZJ_ERR_ARG, "gamma must exceed 1");
     const int NP = tri::n_sp(P), NFP = tri::n_fp(P);
@@ -605,7 +625,8 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
         return tfail(h, ZJ_ERR_UNSUPPORTED, "mesh too large for 32-bit element indices");
 
     h->P = P; h->NP = NP; h->NFP = NFP; h->E = m->n_elem; h->F = m->n_faces;
-    h->prec = p->precision; h->gamma = p->gamma; h->riemann = p->riemann; h->device = p->device;
+    h->prec = p->precision; h->gamma = p->gamma; h->riemann = p->riemann; h->corr = p->corr;
+    h->device = p->device;
     h->epb = TILE;
     h->nblk = (h->E + h->epb - 1) / h->epb;
     const int E = h->E;
@@ -730,6 +751,7 @@ static TriArgs<R> tri_args(zj_tri* h, int mode) {
     a.gamma = h->gamma;
     a.riemann = h->riemann;
     a.xcd_groups = h->xcd_groups;
+    a.lift_off = lift_offset(h->P, h->corr);
     R** T = (R**)h->T;
     switch (mode) {
     case MODE_RESIDUAL:
@@ -963,5 +985,12 @@ extern "C" int zj_tri_operators(int p, double* sp, double* fp, double* wg, doubl
     };
     put(sp, O.sp); put(fp, O.fp); put(wg, O.wg);
     put(Dr, O.Dr); put(Ds, O.Ds); put(interp, O.interp); put(lift, O.lift);
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_lift(int p, int corr, double* lift) {
+    tri::Operators O;
+    if (!lift || corr < ZJ_CORR_DG || corr > ZJ_CORR_GA || !tri::operators(p, O, corr)) return ZJ_ERR_ARG;
+    std::memcpy(lift, O.lift.data(), O.lift.size() * sizeof(double));
     return ZJ_OK;
 }

@@ -15,6 +15,18 @@
 //   Dr = Vr V^-1, Ds = Vs V^-1, interp = Vf V^-1;
 //   lift[i][j] = w_j sum_m psi_m(x_i) psi_m(y_j), the L2 lifting of a unit
 //   value at flux point j (the DG-equivalent correction field, c = 0).
+// Other correction fields: the energy-stable (VCJH) family on triangles of
+// Castonguay, Vincent and Jameson (J. Sci. Comput. 51, 2012), whose norm is
+//   |u|^2 = int_T u^2 + (c/2) sum_v C(p,v) (d^p u / dr^v ds^(p-v))^2;
+// its correction field is the DG lifting filtered by (M + K)^-1 M, with K the
+// c-term's matrix.  In the orthonormal modal basis M = I and, the p-th
+// derivatives of degree-p modes being constants (zero below degree p) and the
+// reference triangle's area 2, K = c sum_v C(p,v) d_v d_v^T with
+// d_v[m] = d^p psi_m / dr^v ds^(p-v).  c takes the one-dimensional VCJH
+// values of the family (Vincent, Castonguay and Jameson, J. Sci. Comput. 47,
+// 2011): c_SD = 2p / ((2p+1)(p+1)(a_p p!)^2), c_HU = 2(p+1) / ((2p+1) p
+// (a_p p!)^2), a_p = (2p)! / (2^p (p!)^2); GA is Huynh's g_Ga, the SD member,
+// as on quadrilaterals.
 #pragma once
 
 #include "zj_host_math.hpp"
@@ -178,6 +190,53 @@ inline bool invert(int n, std::vector<double>& A) {
     return true;
 }
 
+// One-dimensional VCJH parameter of a correction family (0 DG, 1 SD, 2 HU,
+// 3 GA = SD); negative for an unknown family.
+inline double vcjh_c(int p, int family) {
+    double fact = 1.0, fact2 = 1.0;
+    for (int k = 2; k <= p; ++k) fact *= k;
+    for (int k = 2; k <= 2 * p; ++k) fact2 *= k;
+    const double ap = fact2 / (std::pow(2.0, p) * fact * fact), q = (ap * fact) * (ap * fact);
+    switch (family) {
+    case 0: return 0.0;
+    case 1: case 3: return 2.0 * p / ((2.0 * p + 1) * (p + 1) * q);
+    case 2: return 2.0 * (p + 1) / ((2.0 * p + 1) * p * q);
+    default: return -1.0;
+    }
+}
+
+// d^p psi_m / dr^v ds^(p-v) of every Dubiner mode (constant for degree p,
+// zero below): each mode is fitted exactly by the monomials r^a s^b, a+b <= p,
+// on the np solution points, and the coefficient of r^v s^(p-v) times
+// v! (p-v)! is the derivative.  Out: d[v][m], v = 0..p.
+inline bool top_derivatives(int p, const std::vector<double>& sp, const std::vector<double>& V,
+                            std::vector<double>& d) {
+    const int np = n_sp(p);
+    std::vector<double> Mono((size_t)np * np);   // [point][monomial], monomials by degree, then a descending
+    std::vector<std::pair<int, int>> mono;
+    for (int deg = 0; deg <= p; ++deg)
+        for (int a = deg; a >= 0; --a) mono.push_back({a, deg - a});
+    for (int q = 0; q < np; ++q)
+        for (int k = 0; k < np; ++k)
+            Mono[(size_t)q * np + k] =
+                std::pow(sp[2 * q], mono[k].first) * std::pow(sp[2 * q + 1], mono[k].second);
+    if (!invert(np, Mono)) return false;
+    d.assign((size_t)(p + 1) * np, 0.0);
+    for (int v = 0; v <= p; ++v) {
+        int k = 0;
+        while (!(mono[k].first == v && mono[k].second == p - v)) ++k;
+        double f = 1.0;
+        for (int t = 2; t <= v; ++t) f *= t;
+        for (int t = 2; t <= p - v; ++t) f *= t;
+        for (int m = 0; m < np; ++m) {
+            double coef = 0.0;   // monomial coefficient k of mode m = (Mono^-1 V)[k][m]
+            for (int q = 0; q < np; ++q) coef += Mono[(size_t)k * np + q] * V[(size_t)q * np + m];
+            d[(size_t)v * np + m] = coef * f;
+        }
+    }
+    return true;
+}
+
 struct Operators {
     int p = 0, np = 0, nf = 0;
     std::vector<double> sp, fp, wg;   // [np][2], [nf][2], [p+1]
@@ -186,8 +245,10 @@ struct Operators {
     std::vector<double> lift;         // [np][nf]
 };
 
-inline bool operators(int p, Operators& o) {
+inline bool operators(int p, Operators& o, int family = 0) {
     if (p < 1 || p > 5) return false;
+    const double c = vcjh_c(p, family);
+    if (c < 0.0) return false;
     const int np = n_sp(p), nf = n_fp(p);
     o.p = p; o.np = np; o.nf = nf;
     o.sp.assign((size_t)np * 2, 0.0);
@@ -213,13 +274,36 @@ inline bool operators(int p, Operators& o) {
     mul(Vr, np, o.Dr);
     mul(Vs, np, o.Ds);
     mul(Vf, nf, o.interp);
+    // modal lifting coefficients of a unit value at flux point j, filtered by
+    // (I + K)^-1 (the identity for DG)
+    std::vector<double> F((size_t)np * np, 0.0);   // (I + K)^-1, modal
+    for (int m = 0; m < np; ++m) F[(size_t)m * np + m] = 1.0;
+    if (c > 0.0) {
+        std::vector<double> d;
+        if (!top_derivatives(p, o.sp, V, d)) return false;
+        double binom = 1.0;
+        for (int v = 0; v <= p; ++v) {
+            for (int m = 0; m < np; ++m)
+                for (int n = 0; n < np; ++n)
+                    F[(size_t)m * np + n] += c * binom * d[(size_t)v * np + m] * d[(size_t)v * np + n];
+            binom = binom * (p - v) / (v + 1);
+        }
+        if (!invert(np, F)) return false;
+    }
     o.lift.assign((size_t)np * nf, 0.0);
-    for (int i = 0; i < np; ++i)
-        for (int j = 0; j < nf; ++j) {
+    std::vector<double> hat(np);
+    for (int j = 0; j < nf; ++j) {
+        for (int m = 0; m < np; ++m) {
             double acc = 0.0;
-            for (int m = 0; m < np; ++m) acc += V[(size_t)i * np + m] * Vf[(size_t)j * np + m];
+            for (int n = 0; n < np; ++n) acc += F[(size_t)m * np + n] * Vf[(size_t)j * np + n];
+            hat[m] = acc;
+        }
+        for (int i = 0; i < np; ++i) {
+            double acc = 0.0;
+            for (int m = 0; m < np; ++m) acc += V[(size_t)i * np + m] * hat[m];
             o.lift[(size_t)i * nf + j] = acc * o.wg[(size_t)(j % (p + 1))];
         }
+    }
     return true;
 }
 

@@ -2,8 +2,9 @@
 
 The reference declares triangles but leaves their operators zero
 (src/basis/basis.cpp:615-669); ``TriSolver`` runs the textbook scheme its
-settings name on straight-sided triangles (nodal FR, DG-equivalent correction,
-configured Riemann solver, SSP-RK3).  Faces, periodic pairing and normals come
+settings name on straight-sided triangles (nodal FR, the energy-stable
+correction field of the configured family, configured Riemann solver,
+SSP-RK3).  Faces, periodic pairing and normals come
 from the library's native mesh routines (Mesh::buildFaces /
 Mesh::faceNormal counterparts with three corners per element).
 """
@@ -15,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ZJError, lib
-from .solver import PRECISIONS, RIEMANNS
+from .solver import CORRS, PRECISIONS, RIEMANNS
 
 ZJ_CORR_DG = 0
 
@@ -45,7 +46,8 @@ class TriSolver:
     solution points in the order of ``_lib.tri_operators(p)["sp"]``."""
 
     def __init__(self, xy, tris, p, faces=None, normals=None, periodic=None, bc_type=None, bc_face_map=None,
-                 bc_data=None, gamma=1.4, precision="fp64", riemann="roe", device=0, order="hilbert"):
+                 bc_data=None, gamma=1.4, precision="fp64", riemann="roe", device=0, order="hilbert",
+                 corr="dg"):
         xy = np.ascontiguousarray(xy, np.float64)
         tris = np.ascontiguousarray(tris, np.int64)
         if faces is None:
@@ -73,7 +75,7 @@ class TriSolver:
             keep.append(bd); d.bc_data = bd.ctypes.data; d.n_bc_data = bd.size // 8
         d.reorder = 1 if order == "hilbert" else 0
         prm = _lib.Params(gamma, 0.5, PRECISIONS[precision], _lib.ZJ_COMPAT_TEXTBOOK, 0, device,
-                          RIEMANNS[riemann], ZJ_CORR_DG)
+                          RIEMANNS[riemann], CORRS[corr])
         h = C.c_void_p()
         _check(lib().zj_tri_create(C.byref(h), C.byref(d), C.byref(prm)))
         self._h = h
