@@ -194,3 +194,49 @@ def test_tri_ga_steps_and_fp32bulk():
     r2 = rel(b.get_state(), a.get_state())
     say(f"  GA FP32-bulk vs FP64 after 50 steps: rel L2 {r2:.3e}")
     assert r2 < 1e-5
+
+
+def test_tri_c3_full_size_properties():
+    """The triangle configuration at full size (500x500 split quads = 500,000
+    elements, P3, HLLC, GA), through properties that do not depend on size:
+    a uniform flow has zero residual; the residual of the Fourier state is
+    discretely conservative (sum over elements of J w^T dU/dt, w the nodal
+    quadrature weights, vanishes to round-off on the periodic mesh); one step
+    keeps the totals, and the FP32-bulk step agrees with the FP64 one.  (The
+    FP32 residual itself differs from FP64 by ~5e-4 here: at h = 0.002 it is
+    a small difference of fluxes dominated by the mean pressure, which FP32
+    rounds at 1e-7 of the flux; the states that it updates differ by ~1e-8.)"""
+    say("case: triangles C3 full size 500x500 P3 hllc GA: freestream, conservation, FP32-bulk, one step")
+    m = cases.split_tri_mesh(500, jitter=0.2, seed=2024)
+    a = TriSolver(m.xy, m.tris, 3, periodic=(1.0, 1.0), riemann="hllc", corr="ga")
+    b = TriSolver(m.xy, m.tris, 3, periodic=(1.0, 1.0), riemann="hllc", corr="ga", precision="fp32bulk")
+    Np = 10
+    ops = T.operators(3)
+    w = np.ones(Np) @ np.linalg.inv(ops["V"] @ ops["V"].T)          # nodal quadrature weights
+    P = m.xy[m.tris]
+    J = 0.25 * ((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1])
+                - (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1]))   # area / 2
+    free = np.tile(cases.prim_to_cons(np.array(1.0), np.array(0.8), np.array(0.3), np.array(1 / 1.4 / 0.25), 1.4),
+                   (m.n_elem * Np, 1)).reshape(-1)
+    a.set_state(free)
+    assert np.abs(a.residual()).max() < 1e-9   # round-off of O(1) fluxes over h = 0.002; a defect is O(1/h)
+    U = cases.fourier_freestream_ic(sp_coords(m.xy, m.tris, 3), seed=2024).reshape(-1)
+    a.set_state(U)
+    b.set_state(U)
+    R = a.residual().reshape(m.n_elem, Np, 4)
+    tot = np.einsum("e,i,eiv->v", J, w, R)
+    scale = np.einsum("e,i,eiv->v", np.abs(J), np.abs(w), np.abs(R))
+    say(f"  conservation |sum J w R| / sum|.|: {np.abs(tot / scale).max():.2e}")
+    assert np.all(np.abs(tot) <= 1e-11 * scale)
+    before = np.einsum("e,i,eiv->v", J, w, U.reshape(m.n_elem, Np, 4))
+    dt = a.dt_cfl(0.5)
+    a.rk3_step(dt)
+    b.rk3_step(dt)
+    Ua = a.get_state()
+    after = np.einsum("e,i,eiv->v", J, w, Ua.reshape(m.n_elem, Np, 4))
+    assert np.all(np.abs(after - before) <= 1e-12 * np.abs(before).max())
+    r = rel(b.get_state(), Ua)
+    say(f"  FP32-bulk vs FP64 after one step: rel L2 {r:.3e}")
+    assert r < 1e-7
+    a.close()
+    b.close()
