@@ -156,3 +156,37 @@ def test_adjoint_affine_equals_per_point():
         s.rk3_step_vjp(5e-3)
         out.append(s.adjoint_get())
     assert rel(out[0], out[1]) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_adjoint_c5_full_size(precision):
+    """The adjoint configuration at full size (TGV 128x128 P3, Mach 0.1,
+    16,384 elements, limiter on): the residual VJP and one step's VJP
+    (state and dt cotangents) against central differences of the oracle's
+    forward map, through the dot-product identity."""
+    say(f"case: adjoint at full size, TGV 128x128 P3 {precision}: residual VJP, step VJP, dJ/ddt")
+    mesh = cases.tgv_mesh(128)
+    s, oc, t = make(mesh, 3, precision=precision)
+    U = tgv_state(t, amp=0.02, seed=41, mach=0.1)
+    o = oc.solver()
+    tol = TOL if precision == "fp64" else 2e-4
+    w = np.random.default_rng(42).standard_normal(U.shape)
+    v = direction(U, 43)
+    s.set_state(U)
+    if precision == "fp64":
+        lhs = float(w @ fd(o.residual, U, v))
+        assert dot_check(lhs, float(s.residual_vjp(w) @ v), "residual") < tol
+    dt = 0.5 * s.dt_cfl(0.5)
+    lhs = float(w @ fd(lambda X: o.advance(X, dt, True), U, v))
+    s.set_state(U)
+    s.adjoint_set(w)
+    dtb = s.rk3_step_vjp(dt)
+    assert dot_check(lhs, float(s.adjoint_get() @ v), "step") < tol
+    # the step is polynomial in dt (no clamp changes branch), so a wide central
+    # difference is accurate to (h/dt)^2; a narrow one drowns in the round-off
+    # of w.U (1e5 in size here) against 2h dJ/ddt
+    h = 1e-4 * dt
+    dfd = (float(w @ o.advance(U, dt + h, True)) - float(w @ o.advance(U, dt - h, True))) / (2 * h)
+    rd = abs(dtb - dfd) / abs(dfd)
+    say(f"  dJ/ddt: adjoint {dtb:.12e}  FD {dfd:.12e}  rel {rd:.2e}")
+    assert rd < tol
