@@ -70,3 +70,40 @@ def test_autotune_keeps_results():
         s.rk3_step(1e-3)
         ref.rk3_step(1e-3)
     assert np.array_equal(s.get_state(), ref.get_state())
+
+
+def test_batch_c4_full_size():
+    """The batched configuration at full size, as the benchmark builds it
+    (64 seeded meshes, N ~ U{16..128} elements per side, P ~ U{1..5}, Mach ~
+    U[0.1, 0.8], alternating Taylor-Green / vortex, FP32-bulk, PCG64 seed 4;
+    35.8 M DOF): two batched steps (one workgroup per tile, then four per CU
+    looping) equal every member's own steps bit for bit."""
+    import math
+    say("case: batch C4 full size, 64 seeded meshes P1-P5, fp32bulk, 2 steps, grid policies 0 and 4")
+    rng = np.random.default_rng(4)
+    a, b, dts = [], [], []
+    for k in range(64):
+        n, p, mach = int(rng.integers(16, 129)), int(rng.integers(1, 6)), float(rng.uniform(0.1, 0.8))
+        mesh = cases.tgv_mesh(n) if k % 2 == 0 else cases.vortex_mesh(n)
+        t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+        U = (cases.tgv_ic(t.pos, mach=mach) if k % 2 == 0
+             else cases.vortex_ic(t.pos, u_inf=mach * math.sqrt(1.4))).reshape(-1)
+        for lst in (a, b):
+            s = solver.Solver(t, precision="fp32bulk", cfl=0.5)
+            s.set_state(U)
+            lst.append(s)
+        dts.append(a[-1].dt_cfl(0.5))
+    assert sum(s.n for s in a) == 35785688
+    dts = np.array(dts)
+    batch = solver.Batch(a)
+    for policy in (0, 4):
+        batch.set_grid(policy)
+        batch.rk3_step(dts)
+        for s, dt in zip(b, dts):
+            s.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), k
+    batch.close()
+    for s in a + b:
+        s.close()
