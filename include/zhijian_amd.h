@@ -7,7 +7,9 @@
  * handle owns one mesh's device state on one GPU.  Host arrays are copied at
  * the call; the caller keeps ownership.  Streams are hipStream_t passed as
  * void* (NULL = the handle's own stream).  Calls on one handle are not
- * thread-safe.
+ * thread-safe; different handles may be used from different threads.  The
+ * error text of calls without a handle (a failed create, zj_gmsh_read, the
+ * I/O calls) is kept per thread.
  *
  * Reference interfaces replaced (Zhijian, file:line):
  *   zj_create        FRSolver::initialize/allocateGPUMemory/initGeometry

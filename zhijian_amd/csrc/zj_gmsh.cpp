@@ -443,7 +443,7 @@ struct zj_gmsh {
     std::vector<int> group_tags;
 };
 
-static std::string g_gmsh_error;
+static thread_local std::string g_gmsh_error;
 
 extern "C" int zj_gmsh_read(const char* path, zj_gmsh** out) {
     if (!path || !out) return ZJ_ERR_ARG;

@@ -112,7 +112,7 @@ static int boundary_link(zj_solver* h, int f, const int* bc_type, const int* bc_
     return ZJ_OK;
 }
 
-static std::string g_create_error;
+static thread_local std::string g_create_error;   // error text of handle-less calls, per thread
 
 #define ZJ_HIP(h, call)                                                                  \
     do {                                                                                 \
@@ -1106,7 +1106,7 @@ struct zj_batch {
     std::string err;
 };
 
-static std::string g_batch_error;
+static thread_local std::string g_batch_error;
 
 static void batch_release(zj_batch* b) {
     if (b->gexec) hipGraphExecDestroy(b->gexec);

@@ -563,7 +563,7 @@ struct zj_tri {
     std::string err;
 };
 
-static std::string g_tri_error;
+static thread_local std::string g_tri_error;
 
 #define ZJ_THIP(h, call)                                                                 \
     do {                                                                                 \
