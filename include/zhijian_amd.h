@@ -221,7 +221,9 @@ int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_ao
  * state and dt) with one launch per stage and order/metric group, the groups
  * running side by side in one captured graph.  Members keep working on their
  * own; a member must not be stepped on another stream while its batch runs.
- * Members must use the full tile of their order (the default). */
+ * Members must use the full tile of their order (the default), and must not
+ * be destroyed before the batch: its graph reads their device buffers.
+ * dts of zj_batch_rk3_step holds one time step per member. */
 typedef struct zj_batch* zj_batch_handle;
 int zj_batch_create(zj_batch_handle* out, const zj_handle* members, int n_members);
 int zj_batch_destroy(zj_batch_handle b);

@@ -176,3 +176,19 @@ def test_create_rejects_bad_input_before_touching_the_gpu():
     d.bc_type = bct.ctypes.data; d.bc_face_map = bmap.ctypes.data
     rc, msg = _create(d)
     assert rc == 1 and "bc_data" in msg
+
+
+def test_batch_step_argument_checks_before_the_library():
+    """Batch.rk3_step refuses a dts array of the wrong length and a closed
+    member before any call into the library (no GPU needed for the checks)."""
+    class _Member:
+        def __init__(self, v):
+            self._h = C.c_void_p(v)
+    b = solver.Batch.__new__(solver.Batch)
+    b.members = [_Member(1), _Member(2)]
+    b._h = C.c_void_p()
+    with pytest.raises(ValueError, match="2 members"):
+        b.rk3_step(np.zeros(3))
+    b.members[1]._h = C.c_void_p()
+    with pytest.raises(RuntimeError, match="closed"):
+        b.rk3_step(np.zeros(2))

@@ -313,8 +313,14 @@ class Batch:
             raise ZJError(rc, msg.decode() if msg else "")
 
     def rk3_step(self, dts, stream=None):
-        d = np.ascontiguousarray(dts, np.float64)
-        assert d.size == len(self.members)
+        """One SSP-RK3 step of every member, dts[k] for member k.  The batch
+        launches read the members' device buffers, so every member must
+        still be open."""
+        d = np.ascontiguousarray(dts, np.float64).reshape(-1)
+        if d.size != len(self.members):
+            raise ValueError(f"{d.size} time steps for {len(self.members)} members")
+        if not all(getattr(m, "_h", None) and m._h.value for m in self.members):
+            raise RuntimeError("a member of the batch has been closed")
         self._chk(lib().zj_batch_rk3_step(self._h, d, stream))
 
     def synchronize(self):
