@@ -16,7 +16,7 @@ follow.  This restates the textbook scheme for straight-sided triangles:
 * flux points: p+1 Gauss-Legendre points per edge, ordered along the edge's
   counter-clockwise direction (neighbours meet at slot p-f);
 * orthonormal Dubiner basis (ibid., Simplex2DP / GradSimplex2DP);
-* residual (DG-equivalent FR, c = 0):
+* residual (DG-equivalent FR, c = 0; other correction fields below):
       dU/dt = -div F^D - sum_j LIFT[:, j] (|e_j| / (2 J)) (F^I - F^D) . n_out
   with F^D the polynomial interpolant of the flux, F^I the Riemann flux
   (euler.hpp:184-375 via the C oracle) once per face point, and
