@@ -248,6 +248,41 @@ int main(int argc, char** argv) {
                     (long long)m.periodicNodes().size());
         return 0;
     }
+    if (argc >= 7 && std::strcmp(argv[1], "gmshrun") == 0) {
+        // a periodic Gmsh mesh ($Periodic links) through GmshReader, Mesh::pairPeriodicNodes
+        // and FRSolver in textbook mode (Roe, HU, no limiter, inviscid): TGV state, fixed dt
+        Mesh mesh;
+        GmshReader reader;
+        if (!reader.read(argv[2], mesh)) {
+            std::fprintf(stderr, "%s\n", reader.errorMessage().c_str());
+            return 3;
+        }
+        mesh.pairPeriodicNodes();
+        SimParams params;
+        params.poly_order = std::atoi(argv[3]);
+        params.compat = Compat::Textbook;
+        params.flux_type = FluxType::HU;
+        params.riemann = RiemannSolver::Roe;
+        params.positivity = false;
+        params.viscous = false;
+        FRSolver solver;
+        solver.initialize(mesh, params);
+        const Real g = params.gamma;
+        IdealGas gas(g);
+        solver.setInitialCondition([&](Vec2 x) {
+            return gas.primToConserv(1.0, std::sin(x.x) * std::cos(x.y), -std::cos(x.x) * std::sin(x.y),
+                                     1.0 / (g * 0.3 * 0.3) + 0.25 * (std::cos(2 * x.x) + std::cos(2 * x.y)));
+        });
+        const std::string prefix = argv[6];
+        solver.copyToHost();
+        dump(prefix + "_U0.bin", state(solver));
+        for (int it = 0; it < std::atoi(argv[4]); ++it) solver.advance(std::atof(argv[5]));
+        solver.copyToHost();
+        dump(prefix + "_U.bin", state(solver));
+        std::printf("{\"elements\": %lld, \"faces\": %lld, \"boundary_faces\": %lld}\n",
+                    (long long)mesh.numElements(), (long long)mesh.numFaces(), (long long)mesh.numBoundaryFaces());
+        return 0;
+    }
     if (argc >= 3 && std::strcmp(argv[1], "io") == 0) return io_case(argv[2]);
     if (argc >= 7 && std::strcmp(argv[1], "resume") == 0)
         return resume_case(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6]));

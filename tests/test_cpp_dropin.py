@@ -176,3 +176,34 @@ def test_frsolver_resume_from_restart_bit_exact(tmp_path):
     b = np.fromfile(tmp_path / "straight.bin", dtype=np.float64)
     assert a.tobytes() == b.tobytes()
     assert info["time"] == info["time_straight"] and info["iteration"] == info["iteration_straight"] == 10
+
+
+@pytest.mark.gpu
+def test_frsolver_on_periodic_gmsh_file(tmp_path):
+    """GmshReader + Mesh::pairPeriodicNodes + FRSolver (textbook, Roe, HU) on
+    a sheared doubly periodic mesh given as an MSH 4.1 file with $Periodic
+    links: 5 steps against the textbook oracle on the same face table."""
+    _need_driver()
+    import math
+    from tests.test_gmsh import _msh_with_periodic
+    n, p, steps, dt = 6, 3, 5, 2e-3
+    m = cases.sheared_quad_mesh(n, n, 2 * math.pi, 2 * math.pi, 1.1, jitter=0.1, seed=6)
+    links = cases.periodic_links(n, n)
+    msh = tmp_path / "tgv.msh"
+    msh.write_text(_msh_with_periodic(m.xy, m.elem_nodes, 3, links[: n + 1], links[n + 1:], True))
+    prefix = str(tmp_path / "g")
+    print(f"driver: gmshrun sheared {n}x{n} P{p} {steps} steps", flush=True)
+    r = subprocess.run([DRIVER, "gmshrun", str(msh), str(p), str(steps), repr(dt), prefix],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["boundary_faces"] == 0 and info["faces"] == 2 * n * n
+    U0 = np.fromfile(prefix + "_U0.bin", dtype=np.float64)
+    U = np.fromfile(prefix + "_U.bin", dtype=np.float64)
+    oc = fr_case.build(m.xy, m.elem_nodes, p, periodic_nodes=links)
+    tb = oc.textbook("roe", "hu")
+    V = U0.copy()
+    for _ in range(steps):
+        V = tb.advance(V, dt, False)
+    assert np.linalg.norm(U - V) / np.linalg.norm(V) < 1e-10
+    assert np.linalg.norm(U - U0) > 1e-6 * np.linalg.norm(U0)
