@@ -79,3 +79,25 @@ def test_gmsh_periodic_section_under_sanitizers(driver, tmp_path):
 def test_pairing_under_sanitizers(driver, tmp_path):
     out = _run([driver, "pair", "200", "15"], tmp_path)
     assert out["ok"] > 0
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    """The CPU oracle itself (test infrastructure) under gcc AddressSanitizer
+    + UBSan: every order's reference-arithmetic residual, limited steps, CFL dt
+    and norm, and the textbook residual for every Riemann solver and
+    correction family with the BR2 terms (tests/cpp/sanitize_oracle.c)."""
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "sanitize_oracle")
+    ora = os.path.join(ROOT, "oracle")
+    cmd = ["gcc", "-std=gnu11", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", "-ffp-contract=off", "-I", ora,
+           os.path.join(ROOT, "tests", "cpp", "sanitize_oracle.c"), os.path.join(ora, "zj_oracle.c"),
+           os.path.join(ora, "zj_textbook.c"), "-lm", "-o", exe]
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 and "asan" in (r.stderr + r.stdout).lower():
+        pytest.skip("the sanitizer runtime is not installed: " + r.stderr[-200:])
+    assert r.returncode == 0, r.stderr
+    out = _run([exe, "4"], tmp_path)
+    assert out["finite"] == 1
