@@ -5,9 +5,11 @@ What fixes each kernel independently of the oracle's own formula:
 * divergence: exact -grad p for a rest state with polynomial pressure;
 * interpolation: polynomial values at the physical flux points, with edges 2
   and 3 stored in reversed order;
-* F_diff: slot placement written out from kernels.cu:402-425 (the right element
-  uses its own slot fp, the common flux pairs U_L[fp] with U_R[Fp-1-fp]);
-* correction: impulse responses written out from kernels.cu:971-1020;
+* F_diff and the correction: the residual of element-wise constant states
+  from the golden correction table and Rusanov flux; common fluxes and flux
+  differences of smooth states checked point by point from physical
+  positions; the correction of flux differences given as functions of
+  position (no slot arithmetic in any expectation);
 * whole residual: freestream preservation, periodic and far field.
 """
 import math
@@ -74,60 +76,200 @@ def test_interpolation_exact_at_physical_flux_points(p):
             assert np.allclose(ufp[e, edge, :, 0], f(X[:, 0], X[:, 1]), atol=1e-12), (e, edge)
 
 
-def test_flux_diff_slot_placement():
-    """Two elements side by side; face between elem0 edge1 (left) and elem1 edge3."""
-    mesh = cases.uniform_quad_mesh(2, 1, 0.0, 2.0, 0.0, 1.0)
-    p = 2
-    case = fr_case.build(mesh.xy, mesh.elem_nodes, p)
+# ---- physical pins of F_diff and the correction -------------------------------
+# Expectations below come from the physical positions of flux points and
+# solution points, from states given as functions of position, and from the
+# compiled reference's tables (golden).  They never index a slot by arithmetic:
+# every pairing is found by position.
+
+def _fp_positions(xy, en, e, k, g):
+    """Flux points of element e on edge k: the edge runs from corner k to
+    corner k+1, the points sit at the 1D Gauss points g of that traversal
+    (the layout test_interpolation_exact_at_physical_flux_points checks by
+    value)."""
+    P = xy[en[e, :4]]
+    a, b = P[k], P[(k + 1) % 4]
+    t = 0.5 * (1.0 + g)
+    return (1 - t)[:, None] * a + t[:, None] * b, a, b
+
+
+def _match(X, pts, periods):
+    """Index of the point of pts at X, up to a whole period in x or y."""
+    hits = []
+    for sx in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            d = pts - (X + np.array([sx * periods[0], sy * periods[1]]))
+            hits += list(np.nonzero(np.abs(d).max(axis=1) < 1e-9)[0])
+    assert len(set(hits)) == 1, (X, pts)
+    return hits[0]
+
+
+def _flux_n(U, n):
+    fx, fy = O.flux_xy(U)
+    return fx * n[0] + fy * n[1]
+
+
+def _golden_case(golden_ops, mesh, p, periodic):
+    case = fr_case.from_quadmesh(mesh, p, periodic=periodic)
+    for k in ("sol_xi", "sol_eta", "diff_xi", "diff_eta", "interp", "corr"):
+        assert np.array_equal(case.ops[k], golden_ops[f"P{p}_{k}"]), k
+    return case
+
+
+@pytest.mark.parametrize("p", [1, 3, 4])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_residual_of_elementwise_constant_states(p, periodic, golden_ops):
+    """Each element holds one constant state.  Every flux point of an element
+    then holds that state whatever slot it sits in, the interior divergence
+    vanishes (rows of D sum to 0), and each face's common flux is one vector.
+    So, per solution point,
+        R[e][sp] = -sum_k corr[k][sp] * Delta[e][k],
+    Delta = F^c - F(U_e).n on an edge where e is the face's left element and
+    -F^c + F(U_e).n where it is the right one (kernels.cu:346-450), n the face
+    normal, F^c the inline Rusanov flux of the two elements' states, or of the
+    state and its far-field ghost on an open boundary (kernels.cu:453-701),
+    corr the compiled reference's table.  A correction that summed the wrong
+    edge's difference, or a flux difference with either sign flipped, fails."""
+    mesh = cases.uniform_quad_mesh(4, 3, 0.0, 1.0, 0.0, 0.75, jitter=0.2, seed=7)
+    case = _golden_case(golden_ops, mesh, p, periodic)
+    E, S = case.E, case.S
+    rng = np.random.default_rng(11 + p)
+    rho0, u0, v0, p0 = fr_case.freestream()
+    Ue = cases.prim_to_cons(rho0 * (1 + 0.2 * rng.uniform(-1, 1, E)), u0 * (1 + 0.3 * rng.uniform(-1, 1, E)),
+                            0.3 * u0 * rng.uniform(-1, 1, E), p0 * (1 + 0.2 * rng.uniform(-1, 1, E)), 1.4)
+    R = case.solver().residual(np.repeat(Ue[:, None, :], S, axis=1)).reshape(E, S, 4)
+    delta = np.full((E, 4, 4), np.nan)
+    f = case.faces
+    for i in range(f.n):
+        n = case.normals[i]
+        le, ll, re, rl = int(f.le[i]), int(f.ll[i]), int(f.re[i]), int(f.rl[i])
+        if re >= 0:
+            UR = Ue[re]
+        else:
+            UR = O.ghost_state(Ue[le], n[0], n[1], int(case.bc_type[i]), case.bc_data,
+                               int(case.bc_face_map[i]))
+        Fc = O.rusanov(Ue[le], UR, n[0], n[1])
+        delta[le, ll] = Fc - _flux_n(Ue[le], n)
+        if re >= 0:
+            delta[re, rl] = -Fc + _flux_n(Ue[re], n)
+    assert not np.isnan(delta).any()
+    assert periodic or (f.re < 0).sum() == 14
+    corr = golden_ops[f"P{p}_corr"].reshape(4, S)
+    expect = -np.einsum("ks,ekv->esv", corr, delta)
+    scale = np.abs(expect).max()
+    assert scale > 0
+    assert np.abs(R - expect).max() <= 1e-12 * scale
+
+
+def _poly_state(pos, p, c_elem):
+    """A globally smooth state of degree p in x and y (each element's own
+    physical coordinates) plus a constant jump c_elem[e] per element."""
+    x, y = pos[..., 0], pos[..., 1]
+    rho = 1.0 + 0.1 * x ** p - 0.05 * y ** (p - 1) + 0.02 * x * y
+    ru = 0.3 + 0.2 * x - 0.1 * y ** p
+    rv = -0.2 + 0.15 * y + 0.05 * x ** (p - 1) * y
+    rE = 3.0 + 0.3 * y ** p + 0.1 * x
+    return np.stack([rho, ru, rv, rE], axis=-1) + c_elem[:, None, :]
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+@pytest.mark.parametrize("jitter", [0.0, 0.2])
+def test_flux_diff_at_physical_points(p, jitter, golden_ops):
+    """Common flux and flux differences of a smooth state with a constant jump
+    per element, checked point by point from positions (kernels.cu:346-450,
+    528-701).
+      * The common flux at a left flux point X is the Rusanov flux of the left
+        state at X and the right element's state at its flux point that
+        coincides with X (up to a period).
+      * The left element's difference at X is that flux minus F(U_left(X)).n.
+      * The right element's difference at its flux point X_R is
+        -F^c(X') + F(U_right(X_R)).n, where X' is the mirror image of X_R
+        through the face's midpoint: the reference stores each side's flux
+        points along its own counter-clockwise traversal and pairs equal
+        positions in those two sequences (kernels.cu:402-425 against :577).
+    A common flux paired with a mismatched right-hand point, or a right-hand
+    difference taken at the coincident point instead of the mirrored one,
+    fails."""
+    mesh = cases.uniform_quad_mesh(3, 3, 0.0, 1.0, 0.0, 1.0, jitter=jitter, seed=5)
+    case = _golden_case(golden_ops, mesh, p, True)
+    E, S, Fp = case.E, case.S, p + 1
+    g = golden_ops[f"P{p}_sol_xi"][:Fp]
+    c_elem = 0.02 * np.random.default_rng(p).uniform(-1, 1, (E, 4))
+    pos = O.sp_coords(case.elem_nodes, case.xy, case.ops)
     s = case.solver()
-    Fp = p + 1
-    rng = np.random.default_rng(5)
-    ufp = np.zeros((2, 4, Fp, 4))
-    for e in range(2):
-        for k in range(4):
-            for f in range(Fp):
-                ufp[e, k, f] = [1 + 0.1 * rng.random(), 0.2 * rng.random(), 0.1 * rng.random(),
-                                3 + rng.random()]
-    s.set_ufp(ufp)
+    s.k_interp(_poly_state(pos, p, c_elem))
     fc = s.k_riemann().reshape(case.faces.n, Fp, 4)
-    fd = s.k_flux_diff().reshape(2, 4, Fp, 4)
-    a = [i for i in range(case.faces.n) if case.faces.re[i] >= 0]
-    assert len(a) == 1
-    fi = a[0]
-    assert (case.faces.le[fi], case.faces.ll[fi], case.faces.re[fi], case.faces.rl[fi]) == (0, 1, 1, 3)
-    n = case.normals[fi]
-    assert np.allclose(n, [1.0, 0.0])
-    for f in range(Fp):
-        UL = ufp[0, 1, f]
-        UR = ufp[1, 3, Fp - 1 - f]            # kernels.cu:577
-        Fc = O.rusanov(UL, UR, n[0], n[1])
-        assert np.allclose(fc[fi, f], Fc, rtol=1e-15)
-        fxL, fyL = O.flux_xy(UL)
-        assert np.allclose(fd[0, 1, f], Fc - (fxL * n[0] + fyL * n[1]), rtol=1e-14)
-        fxR, fyR = O.flux_xy(ufp[1, 3, f])    # kernels.cu:403-405: same fp
-        assert np.allclose(fd[1, 3, f], -Fc + (fxR * n[0] + fyR * n[1]), rtol=1e-14)
+    fd = s.k_flux_diff().reshape(E, 4, Fp, 4)
+    per = (mesh.Lx, mesh.Ly)
+    f = case.faces
+    checked = 0
+    for i in range(f.n):
+        n = case.normals[i]
+        le, ll, re, rl = int(f.le[i]), int(f.ll[i]), int(f.re[i]), int(f.rl[i])
+        XL, _, _ = _fp_positions(case.xy, case.elem_nodes, le, ll, g)
+        XR, a, b = _fp_positions(case.xy, case.elem_nodes, re, rl, g)
+        UL = _poly_state(XL[None], p, c_elem[le][None])[0]
+        UR = _poly_state(XR[None], p, c_elem[re][None])[0]
+        Fc = np.array([O.rusanov(UL[q], UR[_match(XL[q], XR, per)], n[0], n[1]) for q in range(Fp)])
+        assert np.allclose(fc[i], Fc, rtol=1e-13, atol=1e-14)
+        for q in range(Fp):
+            assert np.allclose(fd[le, ll, q], Fc[q] - _flux_n(UL[q], n), rtol=1e-12, atol=1e-13)
+            m = _match(a + b - XR[q], XL, per)
+            assert np.allclose(fd[re, rl, q], -Fc[m] + _flux_n(UR[q], n), rtol=1e-12, atol=1e-13)
+            checked += 1
+    assert checked == f.n * Fp
 
 
-@pytest.mark.parametrize("p", [1, 2, 4])
-def test_correction_impulse_responses(p):
-    mesh = cases.uniform_quad_mesh(1, 1, 0.0, 1.0, 0.0, 1.0)
-    case = fr_case.build(mesh.xy, mesh.elem_nodes, p)
+@pytest.mark.parametrize("p", [1, 2, 4, 5])
+def test_correction_at_physical_points(p, golden_ops):
+    """The correction of flux differences given as smooth functions of
+    position, on one bilinear (non-affine) element (kernels.cu:948-1034).
+    Solution point sp sits at reference (xi_sp, eta_sp) (the compiled
+    reference's sol_xi / sol_eta).  The reference adds, with no metric,
+        -corr[k][sp] * Fd_k(Y_k(sp))
+    for each edge k, where Y_k(sp) is the edge point at reference coordinates
+      edge 0: (xi_sp, -1), edge 1: (1, eta_sp),
+      edge 2: (-xi_sp, 1), edge 3: (-1, -eta_sp):
+    the flux point on the sp's own grid line for edges 0 and 1, and its mirror
+    image along the edge for edges 2 and 3, whose points the reference stores
+    in reversed order but reads with the unreversed line index
+    (basis.cpp:546-552 against kernels.cu:991-1007).  Taking the unmirrored
+    point on edges 2 and 3, or the wrong line on any edge, fails."""
+    xy = np.array([[0.0, 0.0], [1.1, 0.1], [1.3, 1.2], [-0.1, 0.9]])
+    en = np.array([[0, 1, 2, 3]], np.int64)
+    case = fr_case.build(xy, en, p)
+    for k in ("sol_xi", "sol_eta", "corr"):
+        assert np.array_equal(case.ops[k], golden_ops[f"P{p}_{k}"]), k
+    S, Fp = case.S, p + 1
+    g = golden_ops[f"P{p}_sol_xi"][:Fp]
+
+    def phys(xi, eta):  # the bilinear map of the element
+        N = np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)]) / 4
+        return N @ xy
+
+    coef = np.random.default_rng(40 + p).standard_normal((4, 4, 5))
+
+    def fdiff(k, X):  # per-edge smooth functions of position, 4 variables
+        x, y = X[..., 0], X[..., 1]
+        c = coef[k]
+        return np.stack([c[v, 0] + c[v, 1] * x + c[v, 2] * y + c[v, 3] * x * x + c[v, 4] * y ** 3
+                         for v in range(4)], axis=-1)
+
+    fd = np.zeros((1, 4, Fp, 4))
+    for k in range(4):
+        X, _, _ = _fp_positions(xy, en, 0, k, g)
+        fd[0, k] = fdiff(k, X)
     s = case.solver()
-    Fp = p + 1; S = Fp * Fp
-    corr = case.ops["corr"].reshape(4, S)
-    for edge in range(4):
-        for f in range(Fp):
-            for v in (0, 3):
-                fd = np.zeros((1, 4, Fp, 4)); fd[0, edge, f, v] = 1.0
-                s.set_fdiff(fd)
-                out = s.k_correction(np.zeros(S * 4)).reshape(S, 4)
-                expect = np.zeros((S, 4))
-                for sp in range(S):
-                    eta_i, xi_j = sp // Fp, sp % Fp    # kernels.cu:966-967
-                    hit = (xi_j == f) if edge in (0, 2) else (eta_i == f)
-                    if hit:
-                        expect[sp, v] = -corr[edge, sp]
-                assert np.array_equal(out, expect), (edge, f, v)
+    s.set_fdiff(fd)
+    out = s.k_correction(np.zeros(S * 4)).reshape(S, 4)
+    corr = golden_ops[f"P{p}_corr"].reshape(4, S)
+    xi = golden_ops[f"P{p}_sol_xi"]; eta = golden_ops[f"P{p}_sol_eta"]
+    expect = np.zeros((S, 4))
+    for sp in range(S):
+        Y = [phys(xi[sp], -1.0), phys(1.0, eta[sp]), phys(-xi[sp], 1.0), phys(-1.0, -eta[sp])]
+        for k in range(4):
+            expect[sp] -= corr[k, sp] * fdiff(k, Y[k])
+    assert np.abs(out - expect).max() <= 1e-12 * np.abs(expect).max()
 
 
 def test_freestream_periodic_is_preserved():

@@ -16,7 +16,7 @@ from tests.conftest import ROOT
 
 CSRC = os.path.join(ROOT, "zhijian_amd", "csrc")
 SRC = os.path.join(ROOT, "tests", "cpp", "sanitize_driver.cpp")
-OUT = os.path.join(ROOT, "tests", "cpp", "_build", "sanitize_driver")
+OUT = os.path.join(ROOT, "tests", "cpp", "_build_san", "sanitize_driver")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
@@ -88,7 +88,7 @@ def test_oracle_under_sanitizers(tmp_path):
     correction family with the BR2 terms (tests/cpp/sanitize_oracle.c)."""
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
-    exe = os.path.join(ROOT, "tests", "cpp", "_build", "sanitize_oracle")
+    exe = os.path.join(ROOT, "tests", "cpp", "_build_san", "sanitize_oracle")
     ora = os.path.join(ROOT, "oracle")
     cmd = ["gcc", "-std=gnu11", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
            "-fno-omit-frame-pointer", "-ffp-contract=off", "-I", ora,
