@@ -53,6 +53,28 @@ def alg_bytes_per_stage(E, p, stage, precision, affine, cross_edges=None, faces_
     return E * (S * per_pt + geo + faces_per_elem * (2 * w + 16)) + ce * Fp * 4 * w * 2
 
 
+def textbook_alg_bytes_per_stage(E, p, stage, precision, affine, cross_edges, viscous=False):
+    """The textbook scheme's stage launch: the reference-arithmetic bytes plus
+    the per-edge factor (edge half length over J, one word per element edge).
+    With the Navier-Stokes terms each cross-tile trace carries U, dU/dx and
+    dU/dy (12 words per flux point instead of 4)."""
+    w = 8 if precision == "fp64" else 4
+    by = alg_bytes_per_stage(E, p, stage, precision, affine, cross_edges) + E * 4 * w
+    if viscous:
+        by += cross_edges * (p + 1) * 8 * w * 2
+    return by
+
+
+def roofline_of(bytes_per_step, stage_ms, kernel, **extra):
+    """HBM roofline of a stage kernel: algorithmic bytes of the three launches
+    over their summed kernel execution time."""
+    achieved = bytes_per_step / (sum(stage_ms) * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                kernel=kernel, bytes_per_launch=bytes_per_step / 3.0, avg_launch_ms=sum(stage_ms) / 3.0,
+                timing="kernel execution (start/stop events recorded by the runtime at each stage kernel)",
+                **extra)
+
+
 def cross_tile_edges(faces, epb):
     """Element edges whose trace crosses tiles of `epb` consecutive elements."""
     f = np.asarray(faces)
@@ -110,13 +132,21 @@ def batch_members(n_inst, seed, precision):
     return out
 
 
-def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt):
-    """One batched SSP-RK3 step of all members vs each member's own graph step."""
+def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=5):
+    """One batched SSP-RK3 step of all members vs each member's own graph step;
+    the FP32-bulk batch against an FP64 batch of the same members after
+    err_steps steps (reference arithmetic: short window)."""
     from zhijian_amd import solver
     mem = batch_members(n_inst, seed, precision)
     sols = [m[0] for m in mem]
     dts = np.array([s.dt_cfl(0.5) for s in sols])
     dof = sum(s.n for s in sols)
+    by = 0
+    for s, (_, _, meta) in zip(sols, mem):
+        info = s.info()
+        n_el = info["n_dof"] // (4 * info["n_sp"])
+        by += sum(alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"], info["cross_tile_edges"])
+                  for k in (1, 2, 3))
     batch = solver.Batch(sols)
     res = {}
     for policy in (0, 2, 4):
@@ -145,11 +175,37 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt):
     e1.synchronize()
     seq = e0.elapsed_ms(e1) / steps
     batch.close()
+    err = None
+    if err_steps > 0 and precision != "fp64":
+        ref = batch_members(n_inst, seed, "fp64")
+        for (s, U, _), (r, _, _) in zip(mem, ref):
+            s.set_state(U)
+        b32 = solver.Batch(sols)
+        b64 = solver.Batch([m[0] for m in ref])
+        for _ in range(err_steps):
+            b32.rk3_step(dts)
+            b64.rk3_step(dts)
+        rel = []
+        for s, (r, _, _) in zip(sols, ref):
+            a, b = r.get_state(), s.get_state()
+            rel.append((float(np.linalg.norm(b - a) / np.linalg.norm(a)), float(np.abs(b - a).max() / np.abs(a).max())))
+        b32.close()
+        b64.close()
+        for r, _, _ in ref:
+            r.close()
+        err = dict(steps=err_steps, max_rel_l2=max(x[0] for x in rel), max_rel_linf=max(x[1] for x in rel),
+                   median_rel_l2=float(np.median([x[0] for x in rel])), over="members (FP64 batch of the same meshes)")
+    achieved = by / (res[best] * 1e-3) / 1e9
     return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], "
                          f"alternating TGV/vortex", precision=precision, dof=dof,
                 ms_per_step_by_grid_policy={str(k): v for k, v in res.items()}, grid_policy=best,
                 batched_ms_per_step=res[best], sequential_ms_per_step=seq, speedup=seq / res[best],
-                dof_updates_per_s=3.0 * dof / (res[best] * 1e-3))
+                dof_updates_per_s=3.0 * dof / (res[best] * 1e-3),
+                roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=achieved / HBM_PEAK_GBS, bytes_per_step=by,
+                              timing="graph-replayed batched step (launch gaps included: the groups' kernels "
+                                     "run side by side on parallel graph branches)"),
+                fp32bulk_error_vs_fp64=err)
 
 
 def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
@@ -165,7 +221,12 @@ def time_textbook(mesh, tables, U0, steps, warmup, err_steps, stream, hiprt):
     out = {"workload": "same mesh and state, textbook FR (VCJH HU, Roe, inviscid), no limiter", "dt": dt}
     for name, sol in (("fp64", s64), ("fp32bulk", s32)):
         ms, st = time_precision(sol, U0, dt, steps, warmup, stream, hiprt)
-        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3))
+        info = sol.info()
+        by = sum(textbook_alg_bytes_per_stage(tables.E, tables.p, k, name, info["affine"], info["cross_tile_edges"])
+                 for k in (1, 2, 3))
+        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3),
+                         roofline=roofline_of(by, st, f"stage_kernel<{'double' if name == 'fp64' else 'float'},"
+                                                      f"P{tables.p},textbook>"))
     s64.set_state(U0)
     s32.set_state(U0)
     for _ in range(err_steps):
@@ -192,7 +253,12 @@ def time_viscous(tables, U0, steps, warmup, err_steps, stream, hiprt, Re=1600.0)
                        f"penalty 2), no limiter", "dt": dt}
     for name, sol in (("fp64", s64), ("fp32bulk", s32)):
         ms, st = time_precision(sol, U0, dt, steps, warmup, stream, hiprt)
-        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3))
+        info = sol.info()
+        by = sum(textbook_alg_bytes_per_stage(tables.E, tables.p, k, name, True, info["cross_tile_edges"], True)
+                 for k in (1, 2, 3))
+        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * sol.n / (ms * 1e-3),
+                         roofline=roofline_of(by, st, f"visc_stage_kernel<{'double' if name == 'fp64' else 'float'},"
+                                                      f"P{tables.p}>"))
     s64.set_state(U0)
     s32.set_state(U0)
     for _ in range(err_steps):
@@ -252,14 +318,12 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
         e1.record(stream)
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / steps
+        st = s.time_stages(dt, max(steps // 2, 5), stream.ptr)
         ce = s.info()["cross_tile_edges"]
-        by = sum(tri_alg_bytes_per_stage(m.n_elem, p, st, name, ce) for st in (1, 2, 3))
-        achieved = by / (ms * 1e-3) / 1e9
-        out[name] = dict(ms_per_step=ms, dof_updates_per_s=3.0 * dof / (ms * 1e-3),
-                         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                                       frac=achieved / HBM_PEAK_GBS, bytes_per_launch=by / 3.0,
-                                       kernel=f"tri_stage_kernel<{'double' if name == 'fp64' else 'float'},P{p}>",
-                                       cross_tile_edge_fraction=ce / (3.0 * m.n_elem)))
+        by = sum(tri_alg_bytes_per_stage(m.n_elem, p, k, name, ce) for k in (1, 2, 3))
+        out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * dof / (ms * 1e-3),
+                         roofline=roofline_of(by, st, f"tri_stage_kernel<{'double' if name == 'fp64' else 'float'},P{p}>",
+                                              cross_tile_edge_fraction=ce / (3.0 * m.n_elem)))
         sols[name] = s
     for s in sols.values():
         s.set_state(U0)
@@ -301,7 +365,9 @@ def measured_traffic(n, p):
     for line in open(path):
         if line.startswith("traffic:"):
             f = line.split()
-            return {"fp32bulk": float(f[2]), "fp64": float(f[5])}
+            return {"fp32bulk": float(f[2]), "fp64": float(f[5]),
+                    "source": f"profiles/{os.path.basename(path)} (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                              f"scaled by a plain copy kernel's readings; not measured in this run)"}
     return None
 
 
@@ -397,17 +463,10 @@ def time_precision(sol, U0, dt, steps, warmup, stream, hiprt):
     e1.synchronize()
     hiprt.device_synchronize()
     ms_step = e0.elapsed_ms(e1) / steps
-    # per-stage kernel durations: events between individually launched stages
-    nst = max(steps // 2, 5)
-    evs = [[hiprt.Event() for _ in range(4)] for _ in range(nst)]
-    for k in range(nst):
-        evs[k][0].record(stream)
-        for st in (1, 2, 3):
-            sol.rk3_stage(st, dt, stream.ptr)
-            evs[k][st].record(stream)
-    evs[-1][3].synchronize()
+    # each stage kernel's own execution time: start/stop events that the
+    # runtime records at the kernel's start and end (no launch gaps)
+    st_ms = sol.time_stages(dt, max(steps // 2, 5), stream.ptr)
     hiprt.device_synchronize()
-    st_ms = [float(np.mean([evs[k][st - 1].elapsed_ms(evs[k][st]) for k in range(nst)])) for st in (1, 2, 3)]
     return ms_step, st_ms
 
 
@@ -420,6 +479,7 @@ def time_vortex(steps, warmup, stream, hiprt):
     t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=(mesh.Lx, mesh.Ly))
     U0 = cases.vortex_ic(t.pos).reshape(-1)
     out = {"workload": "isentropic vortex, 10x10 quads on [-5,5]^2, P3, reference arithmetic", "dof": t.E * t.S * 4}
+    sols = {}
     for name in ("fp64", "fp32bulk"):
         s = solver.Solver(t, precision=name, cfl=0.5)
         s.set_state(U0)
@@ -427,6 +487,28 @@ def time_vortex(steps, warmup, stream, hiprt):
         ms, st = time_precision(s, U0, dt, steps, warmup, stream, hiprt)
         out[name] = dict(ms_per_step=ms, rk_steps_per_s=1e3 / ms, stage_ms=st,
                          dof_updates_per_s=3.0 * out["dof"] / (ms * 1e-3))
+        sols[name] = s
+    out["dt"] = dt
+
+    def err_after(a, b, n):
+        for s in (a, b):
+            s.set_state(U0)
+        errs = {}
+        for k in range(1, n + 1):
+            a.rk3_step(dt)
+            b.rk3_step(dt)
+            if k in (20, n):
+                x, y = a.get_state(), b.get_state()
+                errs[str(k)] = dict(rel_l2=float(np.linalg.norm(y - x) / np.linalg.norm(x)),
+                                    rel_linf=float(np.abs(y - x).max() / np.abs(x).max()))
+        return errs
+    # the configuration's 100 steps; the reference arithmetic is linearly
+    # unstable, so its FP32 error is also stated at 20 steps, before the
+    # limiter's clamps engage, and the stable textbook scheme is shown beside
+    out["fp32bulk_error_vs_fp64"] = err_after(sols["fp64"], sols["fp32bulk"], 100)
+    kw = dict(compat="textbook", riemann="rusanov", corr="dg", positivity=False, cfl=0.5)
+    out["textbook_fp32bulk_error_vs_fp64"] = err_after(solver.Solver(t, precision="fp64", **kw),
+                                                       solver.Solver(t, precision="fp32bulk", **kw), 100)
     return out
 
 
@@ -565,19 +647,16 @@ def main():
         affine = info["affine"]
         ce = info["cross_tile_edges"]
         by = [alg_bytes_per_stage(E, args.p, st, name, affine, ce) for st in (1, 2, 3)]
-        achieved = sum(by) / (sum(st_ms) * 1e-3) / 1e9
+        rl = roofline_of(sum(by), st_ms, f"stage_kernel<{'double' if w == 8 else 'float'},P{args.p}>",
+                         affine_metric=affine, cross_tile_edge_fraction=ce / (4.0 * E),
+                         step_gap_ms=ms_step - sum(st_ms))
+        rl["traffic"] = None if traffic is None else traffic[name]
+        rl["traffic_source"] = None if traffic is None else traffic["source"]
         res[name] = dict(ms_per_step=ms_step, stage_ms=st_ms,
                          dof_updates_per_s=3.0 * dof / (ms_step * 1e-3),
-                         rk_steps_per_s=1e3 / ms_step,
-                         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                                       frac=achieved / HBM_PEAK_GBS,
-                                       traffic=None if traffic is None else traffic[name],
-                                       kernel=f"stage_kernel<{'double' if w == 8 else 'float'},P{args.p}>",
-                                       bytes_per_launch=sum(by) / 3.0, affine_metric=affine,
-                                       cross_tile_edge_fraction=ce / (4.0 * E),
-                                       avg_launch_ms=sum(st_ms) / 3.0))
-        log(f"{name}: {ms_step:.4f} ms/step, stages {['%.4f' % x for x in st_ms]} ms, "
-            f"{res[name]['dof_updates_per_s']:.3e} DOF-upd/s, {achieved:.0f} GB/s")
+                         rk_steps_per_s=1e3 / ms_step, roofline=rl)
+        log(f"{name}: {ms_step:.4f} ms/step, stage kernels {['%.4f' % x for x in st_ms]} ms, "
+            f"{res[name]['dof_updates_per_s']:.3e} DOF-upd/s, {rl['achieved']:.0f} GB/s")
 
     if args.only:
         print(json.dumps({"only": args.only, "copy_gbps": copy_gbps, **res[args.only]}), flush=True)
@@ -606,7 +685,8 @@ def main():
 
     batched = None
     if args.batch > 0:
-        batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt)
+        batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt,
+                             err_steps=args.err_steps)
         log(f"batched: {batched}")
 
     adj = {}
@@ -653,7 +733,10 @@ def main():
                                f"P{args.p}, Mach 0.1, reference arithmetic (DG correction, inline Rusanov, "
                                f"inviscid), SSP-RK3 + positivity limiter, CFL 0.5",
                    "elements": E, "solution_points": E * S, "dof": dof, "dt": dt,
-                   "precision": "fp32bulk (FP32 flux/interp/correction, FP64 RK state)"},
+                   "precision": "fp32bulk: flux, interpolation, Riemann flux, divergence and correction in FP32 "
+                                "(VALU; no MFMA in the stage kernels); U^n, the RK combination, limiter, norms "
+                                "and dt in FP64; the intermediate stage states stored as FP32 increments over "
+                                "U^n"},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "rk_steps_per_s": head["rk_steps_per_s"],
@@ -673,5 +756,18 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def reap_children():
+    """Waits for any child process still attached (none should be left)."""
+    try:
+        import psutil
+        for c in psutil.Process().children(recursive=True):
+            c.wait(timeout=30)
+    except Exception:
+        pass
+
+
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        reap_children()

@@ -155,6 +155,12 @@ int zj_rk3_step(zj_handle h, double dt, void* stream);
 int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream);
 /* Waits for the work queued on the handle's own stream. */
 int zj_synchronize(zj_handle h);
+/* Measurement (no reference counterpart): `steps` SSP-RK3 steps of size dt,
+ * each stage launched on its own with events that the runtime records when
+ * the kernel starts and ends; stage_ms[3] receives each stage kernel's mean
+ * execution time in ms (the quantity a kernel profiler reports, without the
+ * gaps between launches).  The state advances by `steps` steps. */
+int zj_time_stages(zj_handle h, double dt, int steps, double* stage_ms, void* stream);
 /* CFL time step of the current state (synchronises). */
 int zj_dt_cfl(zj_handle h, double cfl, double* dt);
 /* L2 norm of the last residual: that of zj_residual, or of stage 3 of the last
@@ -221,8 +227,11 @@ int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_ao
  * state and dt) with one launch per stage and order/metric group, the groups
  * running side by side in one captured graph.  Members keep working on their
  * own; a member must not be stepped on another stream while its batch runs.
- * Members must use the full tile of their order (the default), and must not
- * be destroyed before the batch: its graph reads their device buffers.
+ * Members must use the full tile of their order (the default).  While a
+ * batch holds a handle, zj_set_tile, zj_set_geometry_mode (a change),
+ * zj_autotune and zj_destroy on it return ZJ_ERR_STATE: the batch's graph
+ * reads the member's device buffers at that tiling.  zj_set_boundary on a
+ * member is taken up by the batch's next step.
  * dts of zj_batch_rk3_step holds one time step per member. */
 typedef struct zj_batch* zj_batch_handle;
 int zj_batch_create(zj_batch_handle* out, const zj_handle* members, int n_members);
@@ -304,6 +313,8 @@ int zj_tri_get_state_device(zj_tri_handle h, double* d_U_aos, void* stream);
 int zj_tri_residual(zj_tri_handle h, double* dUdt_aos, void* stream);  /* host array or NULL */
 int zj_tri_residual_device(zj_tri_handle h, double* d_dUdt_aos, void* stream);
 int zj_tri_rk3_step(zj_tri_handle h, double dt, void* stream);
+/* As zj_time_stages, for a triangle mesh. */
+int zj_tri_time_stages(zj_tri_handle h, double dt, int steps, double* stage_ms, void* stream);
 /* dt = cfl * d_e / ((2p+1) max(|u|+c)), d_e the inscribed-circle diameter */
 int zj_tri_dt_cfl(zj_tri_handle h, double cfl, double* dt);
 int zj_tri_residual_l2(zj_tri_handle h, double* l2);
@@ -418,7 +429,7 @@ int zj_pair_periodic(int* n_faces, int* face_le, int* face_ll, int* face_re, int
  * order as zj_pair_periodic.  ZJ_ERR_ARG if a face has two candidates or an
  * index is out of range.  Updates *n_faces. */
 int zj_pair_periodic_nodes(int* n_faces, int* face_le, int* face_ll, int* face_re, int* face_rl,
-                           int nv, int stride, const int64_t* elem_nodes, const double* xy, int64_t n_nodes,
+                           int nv, int stride, int n_elem, const int64_t* elem_nodes, const double* xy, int64_t n_nodes,
                            int64_t n_pairs, const int64_t* slave, const int64_t* master, double tol);
 int zj_face_normals(int n_faces, const int* face_le, const int* face_ll, int nv, int stride,
                     const int64_t* elem_nodes, const double* xy, double* normals);

@@ -102,7 +102,7 @@ int pair_case(int n, std::mt19937_64& rng) {
     for (int k = 0; k < extra; ++k) { sl.push_back((int64_t)(rng() % (N + 2)) - 1); ms.push_back((int64_t)(rng() % N)); }
     std::vector<int> a = le, b = ll, c = re, d = rl;
     int nf2 = nf;
-    const int rc1 = zj_pair_periodic_nodes(&nf2, a.data(), b.data(), c.data(), d.data(), 4, 4, en.data(), xy.data(), N,
+    const int rc1 = zj_pair_periodic_nodes(&nf2, a.data(), b.data(), c.data(), d.data(), 4, 4, E, en.data(), xy.data(), N,
                                            (int64_t)sl.size(), sl.data(), ms.data(), 1e-9);
     int nf3 = nf;
     const int rc2 = zj_pair_periodic(&nf3, le.data(), ll.data(), re.data(), rl.data(), 4, 4, en.data(), xy.data(),

@@ -52,6 +52,43 @@ def test_batch_equals_members(precision):
     batch.close()
 
 
+def test_batch_member_changes():
+    """While a batch holds a handle, changes that would leave the batch's
+    launch arguments stale are refused (tile, metric form, autotune, destroy),
+    and a boundary-table change that moves the member's bc_data buffer is
+    taken up by the batch's next step."""
+    say("case: batch member changes: refusals and a moved bc_data buffer")
+    from tests import fr_case
+    mesh = cases.uniform_quad_mesh(6, 5, 0.0, 1.0, 0.0, 1.0)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=None)
+    rho, u, v, p = fr_case.freestream()
+    U = cases.perturb(np.tile(cases.prim_to_cons(np.array(rho), np.array(u), np.array(v), np.array(p), 1.4),
+                              (t.E * t.S, 1)), 0.01, 9).reshape(-1)
+    a, b = solver.Solver(t), solver.Solver(t)
+    for s in (a, b):
+        s.set_state(U)
+    batch = solver.Batch([a])
+    for call in (lambda: a.set_tile(5), lambda: a.autotune(1), lambda: a.close(),
+                 lambda: a.set_geometry_mode(0 if a.set_geometry_mode(-1) else 1)):
+        with pytest.raises(solver.ZJError) as ei:
+            call()
+        assert ei.value.code == 6
+    batch.rk3_step([1e-6])
+    b.rk3_step(1e-6)
+    # a far-field state per boundary face instead of one shared row: the bc_data buffer grows
+    bnd = t.faces[:, 2] < 0
+    assert t.bc_data.size == 8 * int(bnd.sum()) and bnd.sum() == 22
+    fs = np.array([rho * 1.01, u, v, p * 0.99, 0, 0, 0, 0])
+    for s in (a, b):
+        s.set_boundary(t.bc_type, np.where(bnd, np.cumsum(bnd) - 1, -1), np.tile(fs, 2 * int(bnd.sum())))
+    batch.rk3_step([1e-6])
+    b.rk3_step(1e-6)
+    batch.synchronize()
+    assert np.array_equal(a.get_state(), b.get_state())
+    batch.close()
+    a.close()
+
+
 def test_autotune_keeps_results():
     say("case: autotune tile and XCD order, TGV 32x32 P4")
     mesh = cases.tgv_mesh(32)

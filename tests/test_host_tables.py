@@ -95,6 +95,12 @@ def test_node_pairing_argument_checks():
                                  np.concatenate([cases.periodic_links(2, 2), [[1, 0], [4, 3], [7, 6]]]))
     same = _lib.pair_periodic_nodes(f, mesh.elem_nodes, mesh.xy, np.zeros((0, 2), np.int64))
     assert np.array_equal(same, f)
+    links = cases.periodic_links(2, 2)
+    for col, bad in ((0, 4), (0, -1), (1, 4), (2, 7)):   # a face naming an element or edge that does not exist
+        g = f.copy()
+        g[np.nonzero(g[:, 2] < 0)[0][0] if col != 2 else 0, col] = bad
+        with pytest.raises(_lib.ZJError):
+            _lib.pair_periodic_nodes(g, mesh.elem_nodes, mesh.xy, links)
 
 
 def test_tables_match_oracle_case():

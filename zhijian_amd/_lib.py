@@ -88,7 +88,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_tri_get_scheme", "zj_io_last_error", "zj_restart_write", "zj_restart_query", "zj_restart_read",
            "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
            "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes",
-           "zj_tri_lift")
+           "zj_tri_lift", "zj_time_stages", "zj_tri_time_stages")
 
 
 def lib() -> C.CDLL:
@@ -141,7 +141,7 @@ def lib() -> C.CDLL:
         L.zj_gmsh_edge_tag.argtypes = [H, C.c_int64, C.c_int64, C.POINTER(C.c_int)]
         L.zj_gmsh_group.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
         L.zj_gmsh_periodic.argtypes = [H, C.POINTER(C.c_int64), C.c_void_p, C.c_void_p]
-        L.zj_pair_periodic_nodes.argtypes = [C.POINTER(C.c_int), _ip, _ip, _ip, _ip, C.c_int, C.c_int, _lp, _dp,
+        L.zj_pair_periodic_nodes.argtypes = [C.POINTER(C.c_int), _ip, _ip, _ip, _ip, C.c_int, C.c_int, C.c_int, _lp, _dp,
                                              C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_double]
         L.zj_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.zj_get_tiling.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
@@ -169,6 +169,8 @@ def lib() -> C.CDLL:
         L.zj_tri_residual.argtypes = [H, C.c_void_p, C.c_void_p]
         L.zj_tri_residual_device.argtypes = [H, C.c_void_p, C.c_void_p]
         L.zj_tri_rk3_step.argtypes = [H, C.c_double, C.c_void_p]
+        L.zj_tri_time_stages.argtypes = [H, C.c_double, C.c_int, _dp, C.c_void_p]
+        L.zj_time_stages.argtypes = [H, C.c_double, C.c_int, _dp, C.c_void_p]
         L.zj_tri_dt_cfl.argtypes = [H, C.c_double, C.POINTER(C.c_double)]
         L.zj_tri_residual_l2.argtypes = [H, C.POINTER(C.c_double)]
         L.zj_tri_synchronize.argtypes = [H]
@@ -250,7 +252,8 @@ def pair_periodic_nodes(faces: np.ndarray, elem_nodes, xy, links, tol=None, nv=N
     sl, ms = np.ascontiguousarray(lk[:, 0]), np.ascontiguousarray(lk[:, 1])
     arrs = [np.ascontiguousarray(faces[:, k], np.int32).copy() for k in range(4)]
     nf = C.c_int(faces.shape[0])
-    check(lib().zj_pair_periodic_nodes(C.byref(nf), *arrs, nv, stride, en.reshape(-1), xy.reshape(-1), xy.shape[0],
+    check(lib().zj_pair_periodic_nodes(C.byref(nf), *arrs, nv, stride, en.shape[0], en.reshape(-1), xy.reshape(-1),
+                                       xy.shape[0],
                                        lk.shape[0], sl.ctypes.data, ms.ctypes.data, float(tol)))
     return np.stack([a[: nf.value] for a in arrs], axis=1)
 

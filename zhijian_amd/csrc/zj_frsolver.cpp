@@ -106,7 +106,7 @@ void Mesh::pairPeriodicNodes(const std::vector<std::pair<Index, Index>>& links) 
         }
     const double tol = 1e-9 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
     pairFaces([&](int* nf, int* le, int* ll, int* re, int* rl, int nv, const int64_t* en) {
-        return zj_pair_periodic_nodes(nf, le, ll, re, rl, nv, nv, en, xy.data(), numNodes(),
+        return zj_pair_periodic_nodes(nf, le, ll, re, rl, nv, nv, (int)elements_.size(), en, xy.data(), numNodes(),
                                       (int64_t)links.size(), slave.data(), master.data(), tol);
     }, "pairPeriodicNodes");
 }

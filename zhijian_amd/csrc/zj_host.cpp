@@ -173,12 +173,17 @@ extern "C" int zj_pair_periodic(int* n_faces, int* le, int* ll, int* re, int* rl
 // and order follow zj_pair_periodic, so on a translation-periodic mesh both
 // give the same table.
 extern "C" int zj_pair_periodic_nodes(int* n_faces, int* le, int* ll, int* re, int* rl, int nv, int stride,
-                                      const int64_t* en, const double* xy, int64_t n_nodes, int64_t n_pairs,
-                                      const int64_t* slave, const int64_t* master, double tol) {
-    if (!n_faces || *n_faces < 0 || nv < 3 || stride < nv || n_nodes < 0 || n_pairs < 0 || !(tol >= 0.0) ||
-        (n_pairs > 0 && (!slave || !master || !xy)) || (*n_faces > 0 && (!le || !ll || !re || !rl || !en)))
+                                      int n_elem, const int64_t* en, const double* xy, int64_t n_nodes,
+                                      int64_t n_pairs, const int64_t* slave, const int64_t* master, double tol) {
+    if (!n_faces || *n_faces < 0 || nv < 3 || stride < nv || n_elem < 0 || n_nodes < 0 || n_pairs < 0 ||
+        !(tol >= 0.0) || (n_pairs > 0 && (!slave || !master || !xy)) ||
+        (*n_faces > 0 && (!le || !ll || !re || !rl || !en || !xy)))
         return ZJ_ERR_ARG;
     const int nf = *n_faces;
+    for (int f = 0; f < nf; ++f)  // every face's left side indexes en
+        if (le[f] < 0 || le[f] >= n_elem || ll[f] < 0 || ll[f] >= nv || re[f] >= n_elem ||
+            (re[f] >= 0 && (rl[f] < 0 || rl[f] >= nv)))
+            return ZJ_ERR_ARG;
     std::vector<int64_t> parent((size_t)n_nodes);
     std::iota(parent.begin(), parent.end(), int64_t(0));
     auto root = [&](int64_t a) {

@@ -479,6 +479,11 @@ __global__ __launch_bounds__(BLOCK) void finalize_kernel(const double* __restric
 }  // namespace
 
 // ------------------------------------------------------------------ launch
+KernelTimer& kernel_timer() {
+    static thread_local KernelTimer t;
+    return t;
+}
+
 int max_epb(int P) { return BLOCK / ((P + 1) * (P + 1)); }
 int default_epb(int P) { return max_epb(P); }
 int num_blocks(int E, int epb) { return (E + epb - 1) / epb; }
@@ -489,10 +494,10 @@ template <typename R, int P, int EPB, bool AFF, bool TB>
 static hipError_t stage_dispatch_mode(int mode, const StageArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
     const dim3 grid(num_blocks(a.E, EPB)), block(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: stage_kernel<R, P, EPB, MODE_RESIDUAL, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE1: stage_kernel<R, P, EPB, MODE_STAGE1, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE2: stage_kernel<R, P, EPB, MODE_STAGE2, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE3: stage_kernel<R, P, EPB, MODE_STAGE3, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_RESIDUAL: klaunch(stage_kernel<R, P, EPB, MODE_RESIDUAL, AFF, TB>, grid, block, 0, s, a, ops); break;
+    case MODE_STAGE1: klaunch(stage_kernel<R, P, EPB, MODE_STAGE1, AFF, TB>, grid, block, 0, s, a, ops); break;
+    case MODE_STAGE2: klaunch(stage_kernel<R, P, EPB, MODE_STAGE2, AFF, TB>, grid, block, 0, s, a, ops); break;
+    case MODE_STAGE3: klaunch(stage_kernel<R, P, EPB, MODE_STAGE3, AFF, TB>, grid, block, 0, s, a, ops); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -558,10 +563,10 @@ static hipError_t batched_mode(int mode, const BatchInst<R>* insts, const int2* 
                                hipStream_t s) {
     const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: batched_stage_kernel<R, P, MODE_RESIDUAL, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
-    case MODE_STAGE1: batched_stage_kernel<R, P, MODE_STAGE1, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
-    case MODE_STAGE2: batched_stage_kernel<R, P, MODE_STAGE2, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
-    case MODE_STAGE3: batched_stage_kernel<R, P, MODE_STAGE3, AFF><<<g, b, 0, s>>>(insts, items, n_items); break;
+    case MODE_RESIDUAL: klaunch(batched_stage_kernel<R, P, MODE_RESIDUAL, AFF>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE1: klaunch(batched_stage_kernel<R, P, MODE_STAGE1, AFF>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE2: klaunch(batched_stage_kernel<R, P, MODE_STAGE2, AFF>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE3: klaunch(batched_stage_kernel<R, P, MODE_STAGE3, AFF>, g, b, 0, s, insts, items, n_items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -3,9 +3,64 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
+#include <new>
 
 namespace zj {
+
+// Kernel execution timing.  While `start` is set (zj_time_stages and the
+// triangle equivalent), stage launches pass these events to
+// hipExtLaunchKernelGGL, which records them when the kernel starts and ends:
+// the kernel's own execution, without the gaps between launches.  Host state,
+// per thread; never set while a graph is captured.
+struct KernelTimer {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+KernelTimer& kernel_timer();
+
+// Runs `steps` rounds of the three stage launches enq(1), enq(2), enq(3), each
+// with kernel timing events, and returns in stage_ms[3] the mean execution
+// time of each stage's kernel (ms).  enq returns a hipError_t.
+template <typename Enq>
+hipError_t time_stage_kernels(Enq enq, int steps, hipStream_t s, double* stage_ms) {
+    const int n = 3 * steps;
+    hipEvent_t* ev = new (std::nothrow) hipEvent_t[2 * (size_t)n]();
+    if (!ev) return hipErrorOutOfMemory;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 * n && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+    KernelTimer& kt = kernel_timer();
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        kt.start = ev[2 * i];
+        kt.stop = ev[2 * i + 1];
+        e = enq(1 + i % 3);
+    }
+    kt = KernelTimer{};
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+        acc[i % 3] += ms;
+    }
+    for (int i = 0; i < 2 * n; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    delete[] ev;
+    if (e == hipSuccess)
+        for (int k = 0; k < 3; ++k) stage_ms[k] = acc[k] / steps;
+    return e;
+}
+
+template <typename... Formals, typename... Actuals>
+inline void klaunch(void (*k)(Formals...), dim3 grid, dim3 block, unsigned shmem, hipStream_t s,
+                    Actuals... args) {
+    const KernelTimer& t = kernel_timer();
+    if (t.start)
+        hipExtLaunchKernelGGL(k, grid, block, shmem, s, t.start, t.stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(k, grid, block, shmem, s, args...);
+}
 
 constexpr int NV = 4;           // conservative variables
 constexpr int BLOCK = 256;      // threads per workgroup of the stage kernels

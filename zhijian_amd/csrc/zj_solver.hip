@@ -94,6 +94,12 @@ struct zj_solver {
     int A_cur = 0;
     bool A_has_tb = false;
     bool A_seeded = false;
+    // batches that hold this handle read its device buffers through kernel
+    // arguments uploaded at batch creation: `gen` moves whenever those
+    // arguments change, and tile, metric form and lifetime stay fixed while
+    // `in_batches` > 0
+    unsigned gen = 0;
+    int in_batches = 0;
 };
 
 // Boundary link of one face (kernels.cu:584-595): a missing entry falls back
@@ -446,6 +452,7 @@ extern "C" int zj_create(zj_handle* out, const zj_mesh_desc* m, const zj_params*
 
 extern "C" int zj_destroy(zj_handle h) {
     if (!h) return ZJ_ERR_ARG;
+    if (h->in_batches > 0) return fail(h, ZJ_ERR_STATE, "the handle belongs to a batch: destroy the batch first");
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
     release(h);
@@ -622,6 +629,17 @@ extern "C" int zj_rk3_stage(zj_handle h, int stage, double dt, void* stream) {
     return ZJ_OK;
 }
 
+extern "C" int zj_time_stages(zj_handle h, double dt, int steps, double* stage_ms, void* stream) {
+    if (!h || steps < 1 || !stage_ms) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, set_dt(h, dt, s));
+    ZJ_HIP(h, time_stage_kernels([&](int st) { return enqueue_stage(h, st, s); }, steps, s, stage_ms));
+    return ZJ_OK;
+}
+
 static int capture_step(zj_solver* h) {
     drop_graph(h);
     ZJ_HIP(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -700,6 +718,7 @@ extern "C" int zj_set_tile(zj_handle h, int epb) {
     if (h->textbook && epb != M) return fail(h, ZJ_ERR_UNSUPPORTED, "the textbook scheme runs at the full tile");
     if (epb == h->epb) return ZJ_OK;
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "tile change in the middle of a step");
+    if (h->in_batches > 0) return fail(h, ZJ_ERR_STATE, "the handle belongs to a batch, which runs the full tile");
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     // the partial-sum buffer holds one entry per element, enough for any tile
@@ -722,6 +741,7 @@ extern "C" int zj_set_xcd_groups(zj_handle h, int groups) {
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     h->xcd_groups = groups;
+    ++h->gen;
     drop_graph(h);
     return ZJ_OK;
 }
@@ -732,6 +752,7 @@ extern "C" int zj_set_geometry_mode(zj_handle h, int mode, int* active) {
     if (mode == 0 && h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the viscous terms run with the per-element metric");
     if (mode >= 0 && (mode == 1) != h->affine) {
         if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "geometry change in the middle of a step");
+        if (h->in_batches > 0) return fail(h, ZJ_ERR_STATE, "the handle belongs to a batch, grouped by metric form");
         ZJ_HIP(h, hipSetDevice(h->device));
         ZJ_HIP(h, hipStreamSynchronize(h->stream));
         h->affine = mode == 1;
@@ -837,6 +858,7 @@ extern "C" int zj_set_boundary(zj_handle h, const int* bc_type, const int* bc_fa
         ZJ_HIP(h, hipMemcpy(h->bc_data, bc_data, (size_t)n_bc_data * 8 * sizeof(double), hipMemcpyHostToDevice));
     ZJ_HIP(h, hipMemcpy(h->links, links.data(), links.size() * sizeof(EdgeLink), hipMemcpyHostToDevice));
     h->links_host = std::move(links);
+    ++h->gen;  // batches holding the handle re-read its bc_data pointer
     return ZJ_OK;
 }
 
@@ -1096,6 +1118,8 @@ struct zj_batch {
         hipEvent_t join = nullptr;
     };
     std::vector<Group> groups;
+    std::vector<std::vector<int>> group_members;  // member indices of each group
+    std::vector<unsigned> gens;                    // member generations the uploaded arguments match
     double* d_dt = nullptr;
     std::vector<double> dt_on_device;
     hipEvent_t fork = nullptr;
@@ -1147,7 +1171,7 @@ static int batch_upload_group(zj_batch* b, zj_batch::Group& g, const std::vector
             in.affine = h->affine ? 1 : 0;
             v.push_back(in);
         }
-        ZJ_BHIP(b, hipMalloc(&g.insts[mode], v.size() * sizeof(BatchInst<R>)));
+        if (!g.insts[mode]) ZJ_BHIP(b, hipMalloc(&g.insts[mode], v.size() * sizeof(BatchInst<R>)));
         ZJ_BHIP(b, hipMemcpy(g.insts[mode], v.data(), v.size() * sizeof(BatchInst<R>), hipMemcpyHostToDevice));
     }
     return ZJ_OK;
@@ -1205,10 +1229,30 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
                 ZJ_BHIP(b, hipMemcpy(gg.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
             ZJ_BHIP(b, hipStreamCreateWithFlags(&gg.branch, hipStreamNonBlocking));
             ZJ_BHIP(b, hipEventCreateWithFlags(&gg.join, hipEventDisableTiming));
+            b->group_members.push_back(members);
             int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_group<double>(b, gg, members)
                                              : batch_upload_group<float>(b, gg, members);
             if (rc) return rc;
         }
+    for (int i = 0; i < n; ++i) b->gens.push_back(hs[i]->gen);
+    return ZJ_OK;
+}
+
+// Re-uploads the per-member kernel arguments after a member changed them
+// (zj_set_boundary can move its bc_data buffer); the graph is re-captured.
+static int batch_refresh(zj_batch* b) {
+    bool moved = false;
+    for (size_t i = 0; i < b->m.size(); ++i) moved |= b->m[i]->gen != b->gens[i];
+    if (!moved) return ZJ_OK;
+    ZJ_BHIP(b, hipStreamSynchronize(b->stream));
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+        int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_group<double>(b, b->groups[k], b->group_members[k])
+                                         : batch_upload_group<float>(b, b->groups[k], b->group_members[k]);
+        if (rc) return rc;
+    }
+    for (size_t i = 0; i < b->m.size(); ++i) b->gens[i] = b->m[i]->gen;
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     return ZJ_OK;
 }
 
@@ -1272,6 +1316,7 @@ extern "C" int zj_batch_create(zj_batch** out, const zj_handle* members, int n) 
         delete b;
         return rc;
     }
+    for (zj_solver* h : b->m) ++h->in_batches;
     *out = b;
     return ZJ_OK;
 }
@@ -1280,6 +1325,7 @@ extern "C" int zj_batch_destroy(zj_batch* b) {
     if (!b) return ZJ_ERR_ARG;
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
+    for (zj_solver* h : b->m) --h->in_batches;
     batch_release(b);
     delete b;
     return ZJ_OK;
@@ -1307,6 +1353,8 @@ extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
     }
     ZJ_BHIP(b, hipSetDevice(b->device));
     hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    int rc0 = batch_refresh(b);
+    if (rc0) return rc0;
     if (!b->gexec) {
         int rc = batch_capture(b);
         if (rc) return rc;
@@ -1336,6 +1384,7 @@ extern "C" int zj_batch_synchronize(zj_batch* b) {
 extern "C" int zj_autotune(zj_handle h, int reps, int* epb_out, int* xcd_out, double* ms_out) {
     if (!h || reps < 1) return ZJ_ERR_ARG;
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "autotuning needs a state");
+    if (h->in_batches > 0) return fail(h, ZJ_ERR_STATE, "the handle belongs to a batch, which fixes its launch");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "autotune in the middle of a step");
     ZJ_HIP(h, hipSetDevice(h->device));
     int nxcc = 1;

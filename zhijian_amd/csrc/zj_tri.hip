@@ -443,10 +443,10 @@ template <typename R, int P>
 hipError_t tri_launch_p(int mode, const TriArgs<R>& a, hipStream_t s) {
     const int grid = (std::max(a.E, 1) + TILE - 1) / TILE;
     switch (mode) {
-    case MODE_RESIDUAL: tri_stage_kernel<R, P, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a); break;
-    case MODE_STAGE1: tri_stage_kernel<R, P, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a); break;
-    case MODE_STAGE2: tri_stage_kernel<R, P, MODE_STAGE2><<<grid, BLOCK, 0, s>>>(a); break;
-    default: tri_stage_kernel<R, P, MODE_STAGE3><<<grid, BLOCK, 0, s>>>(a); break;
+    case MODE_RESIDUAL: klaunch(tri_stage_kernel<R, P, MODE_RESIDUAL>, grid, dim3(BLOCK), 0, s, a); break;
+    case MODE_STAGE1: klaunch(tri_stage_kernel<R, P, MODE_STAGE1>, grid, dim3(BLOCK), 0, s, a); break;
+    case MODE_STAGE2: klaunch(tri_stage_kernel<R, P, MODE_STAGE2>, grid, dim3(BLOCK), 0, s, a); break;
+    default: klaunch(tri_stage_kernel<R, P, MODE_STAGE3>, grid, dim3(BLOCK), 0, s, a); break;
     }
     return hipGetLastError();
 }
@@ -890,6 +890,19 @@ extern "C" int zj_tri_residual_device(zj_tri_handle h, double* d_dUdt_aos, void*
     hipStream_t s = tpick(h, stream);
     ZJ_THIP(h, tenqueue(h, MODE_RESIDUAL, s));
     ZJ_THIP(h, tri_convert(false, h->Rres, d_dUdt_aos, h->E, h->NP, h->perm, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_time_stages(zj_tri_handle h, double dt, int steps, double* stage_ms, void* stream) {
+    if (!h || steps < 1 || !stage_ms) return ZJ_ERR_ARG;
+    if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
+    ZJ_THIP(h, hipSetDevice(h->device));
+    hipStream_t s = tpick(h, stream);
+    if (dt != h->dt_on_device) {
+        h->dt_on_device = dt;
+        ZJ_THIP(h, launch_set_scalar(h->d_scalars, dt, s));
+    }
+    ZJ_THIP(h, time_stage_kernels([&](int st) { return tenqueue(h, st, s); }, steps, s, stage_ms));
     return ZJ_OK;
 }
 

@@ -418,10 +418,10 @@ hipError_t visc_launch_p(int mode, const StageArgs<R>& a, const OpTables<R>& ops
     constexpr int EPB = BLOCK / ((P + 1) * (P + 1));
     const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
     switch (mode) {
-    case MODE_RESIDUAL: visc_stage_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    case MODE_STAGE1: visc_stage_kernel<R, P, EPB, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    case MODE_STAGE2: visc_stage_kernel<R, P, EPB, MODE_STAGE2><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    default: visc_stage_kernel<R, P, EPB, MODE_STAGE3><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    case MODE_RESIDUAL: klaunch(visc_stage_kernel<R, P, EPB, MODE_RESIDUAL>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    case MODE_STAGE1: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE1>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    case MODE_STAGE2: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE2>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    default: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE3>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
     }
     return hipGetLastError();
 }

@@ -153,7 +153,7 @@ class Solver:
     def close(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().zj_destroy(h)
+            self._chk(lib().zj_destroy(h))
             self._h = None
 
     def __del__(self):
@@ -220,6 +220,16 @@ class Solver:
     def rk3_stage(self, stage, dt, stream=None):
         self._chk(lib().zj_rk3_stage(self._h, int(stage), float(dt), stream))
 
+    def synchronize(self):
+        self._chk(lib().zj_synchronize(self._h))
+
+    def time_stages(self, dt, steps, stream=None):
+        """zj_time_stages: advances `steps` steps, each stage launched on its
+        own; returns each stage kernel's mean execution time (ms, list of 3)."""
+        out = np.zeros(3)
+        self._chk(lib().zj_time_stages(self._h, float(dt), int(steps), out, stream))
+        return [float(x) for x in out]
+
     def dt_cfl(self, cfl):
         v = C.c_double()
         self._chk(lib().zj_dt_cfl(self._h, float(cfl), C.byref(v)))
@@ -234,6 +244,14 @@ class Solver:
         v = C.c_int64()
         self._chk(lib().zj_check_finite(self._h, C.byref(v)))
         return v.value
+
+    def set_boundary(self, bc_type, bc_face_map, bc_data):
+        """zj_set_boundary: new BC tables in the layout of zj_create (per face
+        of the creation face list; 8 doubles per bc_data entry)."""
+        bt = np.ascontiguousarray(bc_type, np.int32)
+        bm = np.ascontiguousarray(bc_face_map, np.int32)
+        bd = np.ascontiguousarray(bc_data, np.float64).reshape(-1)
+        self._chk(lib().zj_set_boundary(self._h, bt.ctypes.data, bm.ctypes.data, bd.ctypes.data, bd.size // 8))
 
     def set_tile(self, epb):
         self._chk(lib().zj_set_tile(self._h, int(epb)))

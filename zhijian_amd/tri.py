@@ -137,6 +137,12 @@ class TriSolver:
     def rk3_step(self, dt, stream=None):
         self._chk(lib().zj_tri_rk3_step(self._h, float(dt), stream))
 
+    def time_stages(self, dt, steps, stream=None):
+        """zj_tri_time_stages: each stage kernel's mean execution time (ms)."""
+        out = np.zeros(3)
+        self._chk(lib().zj_tri_time_stages(self._h, float(dt), int(steps), out, stream))
+        return [float(x) for x in out]
+
     def dt_cfl(self, cfl):
         v = C.c_double()
         self._chk(lib().zj_tri_dt_cfl(self._h, float(cfl), C.byref(v)))
