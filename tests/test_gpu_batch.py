@@ -68,7 +68,8 @@ def test_batch_member_changes():
     for s in (a, b):
         s.set_state(U)
     batch = solver.Batch([a])
-    for call in (lambda: a.set_tile(5), lambda: a.autotune(1), lambda: a.close(),
+    half = a.info()["elems_per_block"] // 2
+    for call in (lambda: a.set_tile(half), lambda: a.autotune(1), lambda: a.close(),
                  lambda: a.set_geometry_mode(0 if a.set_geometry_mode(-1) else 1)):
         with pytest.raises(solver.ZJError) as ei:
             call()

@@ -25,15 +25,20 @@ def driver():
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    out = "%s.%d" % (OUT, os.getpid())
+    tmp = out + ".tmp"
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
            "-fno-omit-frame-pointer", "-I", os.path.join(ROOT, "include"), "-I", CSRC, SRC,
            os.path.join(CSRC, "zj_h5.cpp"), os.path.join(CSRC, "zj_gmsh.cpp"), os.path.join(CSRC, "zj_host.cpp"),
-           "-o", OUT]
+           "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0 and "asan" in (r.stderr + r.stdout).lower():
         pytest.skip("the sanitizer runtime is not installed: " + r.stderr[-200:])
     assert r.returncode == 0, r.stderr
-    return OUT
+    # each test process builds its own copy and swaps it in whole, so parallel
+    # workers never execute a file another one is still writing
+    os.replace(tmp, out)
+    return out
 
 
 def _run(args, tmp_path):
