@@ -149,19 +149,22 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
                   for k in (1, 2, 3))
     batch = solver.Batch(sols)
     res = {}
-    for policy in (0, 2, 4):
-        batch.set_grid(policy)
-        for _ in range(warmup):
-            batch.rk3_step(dts, stream.ptr)
-        stream.synchronize()
-        e0, e1 = hiprt.Event(), hiprt.Event()
-        e0.record(stream)
-        for _ in range(steps):
-            batch.rk3_step(dts, stream.ptr)
-        e1.record(stream)
-        e1.synchronize()
-        res[policy] = e0.elapsed_ms(e1) / steps
+    for single in (False, True):
+        batch.set_launch(single)
+        for policy in (0, 2, 4):
+            batch.set_grid(policy)
+            for _ in range(warmup):
+                batch.rk3_step(dts, stream.ptr)
+            stream.synchronize()
+            e0, e1 = hiprt.Event(), hiprt.Event()
+            e0.record(stream)
+            for _ in range(steps):
+                batch.rk3_step(dts, stream.ptr)
+            e1.record(stream)
+            e1.synchronize()
+            res[("single" if single else "grouped", policy)] = e0.elapsed_ms(e1) / steps
     best = min(res, key=res.get)
+    tuned = batch.autotune(dts, reps=max(2, steps // 4))
     for _ in range(warmup):
         for s, dt in zip(sols, dts):
             s.rk3_step(dt, stream.ptr)
@@ -198,13 +201,18 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
     achieved = by / (res[best] * 1e-3) / 1e9
     return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], "
                          f"alternating TGV/vortex", precision=precision, dof=dof,
-                ms_per_step_by_grid_policy={str(k): v for k, v in res.items()}, grid_policy=best,
+                ms_per_step_by_launch={f"{k[0]}/grid{k[1]}": v for k, v in res.items()},
+                launch=best[0], grid_policy=best[1],
+                autotuner=dict(launch="single" if tuned[0] else "grouped", grid_policy=tuned[1], ms_per_step=tuned[2]),
+                launches_per_stage=1 if best[0] == "single" else len({(m[2]["p"], s.info()["affine"])
+                                                                       for s, m in zip(sols, mem)}),
                 batched_ms_per_step=res[best], sequential_ms_per_step=seq, speedup=seq / res[best],
                 dof_updates_per_s=3.0 * dof / (res[best] * 1e-3),
                 roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=achieved / HBM_PEAK_GBS, bytes_per_step=by,
-                              timing="graph-replayed batched step (launch gaps included: the groups' kernels "
-                                     "run side by side on parallel graph branches)"),
+                              timing="graph-replayed batched step, launch gaps included (grouped: the "
+                                     "groups' kernels run side by side on parallel graph branches; single: "
+                                     "one launch per stage)"),
                 fp32bulk_error_vs_fp64=err)
 
 
@@ -436,6 +444,59 @@ def time_io(sol, mesh, p, reps=3):
     finally:
         shutil.rmtree(d, ignore_errors=True)
     return out
+
+
+def time_gpu_api(tables, U0, reps, stream, hiprt):
+    """The gpu:: dense operator products (interpolateToFluxPoints,
+    computeReferenceGradients, computeFluxDivergence: the reference's K1, K2,
+    K11 on AoS arrays) on the FP64 matrix cores and on the VALU, same mesh.
+    Algorithmic bytes: the AoS inputs read and outputs written once (8-byte
+    words); the operator tables are negligible.  Flops: 2 per multiply-add of
+    the dense products."""
+    from zhijian_amd import gpu_ops as G
+    E, S, p = tables.E, tables.S, tables.p
+    Fp = p + 1
+    from zhijian_amd import _lib
+    ops = _lib.quad_operators(p)
+    DA = hiprt.DeviceArray
+    U = DA(U0)
+    Jinv = DA(np.tile(np.array([1.0, 0.0, 0.0, 1.0]), E * S))
+    ufp = DA(E * 4 * Fp * 4)
+    gx, ge, fx, fy, div = (DA(E * S * 4) for _ in range(5))
+    interp, dxi, deta = DA(ops["interp"]), DA(ops["diff_xi"]), DA(ops["diff_eta"])
+    G.compute_inviscid_flux_sp(U.ptr, fx.ptr, fy.ptr, 1.4, E, S, stream.ptr)
+    calls = {
+        "interpolateToFluxPoints": (lambda: G.interpolate_to_flux_points(U.ptr, ufp.ptr, interp.ptr, E, S, 4, Fp,
+                                                                         stream.ptr),
+                                    8 * E * (S * 4 + 4 * Fp * 4), 2.0 * E * 4 * Fp * S * 4),
+        "computeReferenceGradients": (lambda: G.compute_reference_gradients(U.ptr, gx.ptr, ge.ptr, dxi.ptr, deta.ptr,
+                                                                            E, S, stream.ptr),
+                                      8 * E * S * 4 * 3, 2.0 * 2 * E * S * S * 4),
+        "computeFluxDivergence": (lambda: G.compute_flux_divergence(fx.ptr, fy.ptr, div.ptr, dxi.ptr, deta.ptr,
+                                                                    Jinv.ptr, E, S, stream.ptr),
+                                  8 * E * S * (4 * 3 + 4), 2.0 * 4 * E * S * S * 4),
+    }
+    out = {}
+    for eng, name in ((G.ENGINE_MFMA, "mfma_f64"), (G.ENGINE_VALU, "valu_f64")):
+        G.set_operator_engine(eng)
+        legs = {}
+        for fn, (call, by, fl) in calls.items():
+            for _ in range(3):
+                call()
+            e0, e1 = hiprt.Event(), hiprt.Event()
+            e0.record(stream)
+            for _ in range(reps):
+                call()
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            legs[fn] = dict(ms=ms, gbps=by / (ms * 1e-3) / 1e9, hbm_frac=by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            tflops=fl / (ms * 1e-3) / 1e12, bytes=by, flops=fl)
+        out[name] = legs
+    G.set_operator_engine(G.ENGINE_MFMA)
+    return dict(workload=f"TGV {int(round(math.sqrt(E)))}x{int(round(math.sqrt(E)))} P{p} AoS arrays, FP64 "
+                         f"(the reference's gpu:: layout)",
+                timing="HIP events around back-to-back launches on one stream", **out)
 
 
 def _timed(fn):
@@ -695,6 +756,9 @@ def main():
             adj[prec] = time_adjoint(128, 3, prec, args.steps, args.warmup, stream, hiprt)
             log(f"adjoint {prec}: {adj[prec]}")
 
+    gpu_api = time_gpu_api(tables, U0, max(args.steps // 5, 10), stream, hiprt)
+    log(f"gpu api: {gpu_api}")
+
     tri = None
     if args.tri_n > 0:
         tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.textbook_steps, stream, hiprt,
@@ -749,6 +813,7 @@ def main():
         "textbook": textbook,
         "navier_stokes": viscous,
         "triangles": tri,
+        "gpu_api": gpu_api,
         "vortex": vortex,
         "io": io_res,
         "copy_kernel_gbps": copy_gbps,

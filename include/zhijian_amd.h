@@ -142,6 +142,15 @@ int zj_set_state(zj_handle h, const double* U_aos);          /* host [E][S][4] *
 int zj_get_state(zj_handle h, double* U_aos);                /* host, synchronises */
 int zj_set_state_device(zj_handle h, const double* d_U_aos, void* stream);
 int zj_get_state_device(zj_handle h, double* d_U_aos, void* stream);
+/* Device-resident state slots (no reference counterpart): checkpoints of a
+ * trajectory kept on the GPU in the library's own layout, for checkpointed
+ * adjoints and restarts within a run.  zj_state_slots allocates n slots of
+ * the handle's state (n = 0 frees them; existing slots are dropped);
+ * zj_state_save copies the current state into a slot and zj_state_load makes
+ * a slot the current state, both asynchronous on `stream`. */
+int zj_state_slots(zj_handle h, int n);
+int zj_state_save(zj_handle h, int slot, void* stream);
+int zj_state_load(zj_handle h, int slot, void* stream);
 
 /* dUdt of the current state into dUdt_aos (host pointer; NULL keeps it on the
  * device for zj_residual_l2).  Synchronises when dUdt_aos is not NULL. */
@@ -240,9 +249,19 @@ const char* zj_batch_last_error(zj_batch_handle b);
 /* dts[i] is member i's step (FRSolver::advance, fr_solver.cpp:305-333, for all members). */
 int zj_batch_rk3_step(zj_batch_handle b, const double* dts, void* stream);
 int zj_batch_synchronize(zj_batch_handle b);
-/* Workgroups of each group's launch: 0 = one per work item (element tile);
+/* Workgroups of each launch: 0 = one per work item (element tile);
  * k > 0 = k per CU, each looping over its share of the items (persistent). */
 int zj_batch_set_grid(zj_batch_handle b, int policy);
+/* single = 1: ONE launch per stage over every member's tiles, whatever the
+ * members' orders and metric forms (the stage body is picked per tile, the
+ * LDS sized for the largest order present); 0 (default): one launch per
+ * (order, metric form) group, the groups on parallel graph branches.
+ * Results are the same either way. */
+int zj_batch_set_launch(zj_batch_handle b, int single);
+/* Times `reps` batched steps of sizes dts for each launch form and grid
+ * policy (0, 2, 4), keeps the fastest (*single, *policy, *ms per step; any may
+ * be NULL), and restores every member's state as it was before the call. */
+int zj_batch_autotune(zj_batch_handle b, const double* dts, int reps, int* single, int* policy, double* ms);
 /* Times the residual launch for every tile (full, half) and XCD order and
  * keeps the fastest; results do not depend on the choice. */
 int zj_autotune(zj_handle h, int reps, int* elems_per_block, int* xcd_groups, double* ms);
@@ -406,6 +425,106 @@ int zj_pvtu_write(const char* path, int n_pieces, const char* const* piece_files
  * bytes over kernel time, hipEvents).  Used to calibrate HBM counters and the
  * achievable bandwidth next to the stage kernels' roofline. */
 int zj_copy_probe(int device, int64_t bytes, int reps, double* gbps);
+
+/* ---- the gpu:: free functions (include/gpu/kernels.hpp:17-266) ---------
+ * Raw device arrays in the reference's layouts: U-like arrays [E][S][4];
+ * element-indexed flux-point arrays [E][n_edges][Fp][4]; face-indexed
+ * [F][Fp][4]; Jinv [E][S][4]; interp [n_edges][Fp][S]; diff [S][S];
+ * corr [n_edges][S]; normals [F][2] (one per face) unless stated.  Each
+ * function is asynchronous on `stream` (NULL: the default stream) except
+ * zj_gpu_compute_time_step and zj_gpu_residual_norm, which block and return
+ * the value through their last pointer, as the reference's return it.  The
+ * caller owns every array; nothing is allocated except those two functions'
+ * partial-result buffers.  Errors: ZJ_ERR_ARG for sizes or null arrays,
+ * ZJ_ERR_HIP for a failed launch (text in zj_last_error(NULL)).
+ * The C++ declarations with the reference's names and hipStream_t are in
+ * include/zhijian/gpu_kernels.hpp. */
+#define ZJ_ENGINE_MFMA 0      /* dense operator products on the FP64 matrix cores (default) */
+#define ZJ_ENGINE_VALU 1      /* the same products as FMA chains on the vector ALUs */
+/* Engine of zj_gpu_interpolate_to_flux_points, zj_gpu_reference_gradients and
+ * zj_gpu_flux_divergence (process-wide). */
+int zj_gpu_set_operator_engine(int engine);
+/* gpu::interpolateToFluxPoints (kernels.cu:30-67); n_edges * Fp <= 64, S <= 64. */
+int zj_gpu_interpolate_to_flux_points(const double* U_sp, double* U_fp, const double* interp, int n_elem,
+                                      int n_sp, int n_edges, int n_fp_per_edge, void* stream);
+/* gpu::computeReferenceGradients (kernels.cu:73-108). */
+int zj_gpu_reference_gradients(const double* U, double* dUdxi, double* dUdeta, const double* diff_xi,
+                               const double* diff_eta, int n_elem, int n_sp, void* stream);
+/* gpu::transformGradients (kernels.cu:110-151). */
+int zj_gpu_transform_gradients(const double* dUdxi, const double* dUdeta, double* dUdx, double* dUdy,
+                               const double* Jinv, int n_elem, int n_sp, void* stream);
+/* gpu::computeInviscidFluxSP (kernels.cu:157-194). */
+int zj_gpu_inviscid_flux_sp(const double* U, double* Fx, double* Fy, double gamma, int n_elem, int n_sp,
+                            void* stream);
+/* gpu::computeRiemannFlux (kernels.hpp:58-64; the reference's body is empty,
+ * kernels.cu:704-714): RiemannSolvers::rusanov/roe/hllc (euler.hpp:184-375,
+ * solver_type ZJ_RIEMANN_*) on face-indexed states, normals [F][Fp][2]. */
+int zj_gpu_riemann_flux(const double* U_L, const double* U_R, double* F_common, const double* normals,
+                        double gamma, int solver_type, int n_faces, int n_fp_per_face, void* stream);
+/* gpu::computeRiemannFluxWithBC (kernels.cu:453-701): inline clamped Rusanov
+ * (solver_type ignored, as there), U_R from the right slot Fp-1-f or the ghost
+ * state of bc_type[face] with bc_data row bc_face_map[face]. */
+int zj_gpu_riemann_flux_with_bc(const double* U_fp, double* F_common, const int* face_left_elem,
+                                const int* face_left_local, const int* face_right_elem, const int* face_right_local,
+                                const int* bc_type, const double* bc_data, const int* bc_face_map,
+                                const double* normals, double gamma, int solver_type, int n_faces,
+                                int n_fp_per_face, int n_faces_per_elem, void* stream);
+/* gpu::computeViscousFluxSP (kernels.cu:720-807): mu = T^0.7 / Re. */
+int zj_gpu_viscous_flux_sp(const double* U, const double* dUdx, const double* dUdy, double* Fv_x, double* Fv_y,
+                           double gamma, double Re, double Pr, int n_elem, int n_sp, void* stream);
+/* gpu::computeInteriorFluxAtFace (kernels.cu:198-257). */
+int zj_gpu_interior_flux_at_face(const double* U_fp, double* F_int, const int* face_left_elem,
+                                 const int* face_left_local, const double* face_normals, double gamma, int n_faces,
+                                 int n_fp_per_face, int n_faces_per_elem, void* stream);
+/* gpu::computeInviscidFluxAtFP (kernels.cu:260-319): reference-element edge
+ * normals; face_normals and the face arrays are not read. */
+int zj_gpu_inviscid_flux_at_fp(const double* U_fp, double* F_fp, const double* face_normals,
+                               const int* face_left_elem, const int* face_left_local, double gamma, int n_elem,
+                               int n_edges, int n_fp_per_edge, void* stream);
+/* gpu::computeFluxDifference (kernels.cu:322-340): F_diff = F_common - F_int. */
+int zj_gpu_flux_difference(const double* F_common, const double* F_int, double* F_diff, int n, void* stream);
+/* gpu::computeFluxDiffForFR (kernels.cu:346-450); slots of faces not listed
+ * are left untouched (the reference's caller zeroes the array first). */
+int zj_gpu_flux_diff_for_fr(const double* U_fp, const double* F_common, double* F_diff_elem,
+                            const int* face_left_elem, const int* face_left_local, const int* face_right_elem,
+                            const int* face_right_local, const double* face_normals, double gamma, int n_faces,
+                            int n_fp_per_face, int n_faces_per_elem, void* stream);
+/* gpu::computeFluxDivergence (kernels.cu:813-872): div_F = -(physical
+ * divergence); J is not read. */
+int zj_gpu_flux_divergence(const double* Fx, const double* Fy, double* div_F, const double* diff_xi,
+                           const double* diff_eta, const double* Jinv, const double* J, int n_elem, int n_sp,
+                           void* stream);
+/* gpu::scatterFluxToElements (kernels.cu:875-934). */
+int zj_gpu_scatter_flux_to_elements(const double* F_face, double* F_elem, const int* face_left_elem,
+                                    const int* face_left_local, const int* face_right_elem,
+                                    const int* face_right_local, int n_faces, int n_fp_per_face,
+                                    int n_faces_per_elem, void* stream);
+/* gpu::applyFRCorrection (kernels.cu:948-1034): quadrilaterals (n_edges 4,
+ * n_sp = Fp^2); J is not read. */
+int zj_gpu_apply_fr_correction(double* div_F, const double* F_diff, const double* corr_deriv, const double* J,
+                               int n_elem, int n_sp, int n_edges, int n_fp_per_edge, void* stream);
+/* gpu::applyBoundaryConditions (kernels.cu:1040-1228): characteristic far
+ * field, reflecting walls, outflow pressure, total-condition inflow; normals
+ * [n_bc_faces][Fp][2] and bc_data [n_bc_faces][8] per boundary face. */
+int zj_gpu_apply_boundary_conditions(double* U_fp, const int* elem_idx, const int* local_face, const int* bc_type,
+                                     const double* bc_data, const double* normals, double gamma, int n_bc_faces,
+                                     int n_fp_per_face, int n_faces_per_elem, void* stream);
+/* gpu::enforcePositivity (kernels.cu:1236-1295) on AoS states. */
+int zj_gpu_enforce_positivity(double* U, int n_states, void* stream);
+/* gpu::computeTimeStep (kernels.cu:1355-1474): min over points of
+ * CFL h_min[e] / (maxWaveSpeed(U) sqrt|J|), over the points its per-element
+ * tree reduction reaches (all points when n_sp is a power of two). */
+int zj_gpu_compute_time_step(const double* U, const double* J, const double* h_min, double CFL, double gamma,
+                             int n_elem, int n_sp, double* dt, void* stream);
+/* gpu::computeResidualNorm (kernels.cu:1476-1524): sqrt of the sum of squares
+ * of the n_elem * n_sp * 4 values, summed in a fixed order. */
+int zj_gpu_residual_norm(const double* R, int n_elem, int n_sp, double* norm, void* stream);
+/* gpu::setInitialCondition, copySolution, scaleArray, axpy (kernels.cu:1526-1582). */
+int zj_gpu_set_initial_condition(double* U, double rho, double u, double v, double p, double gamma, int n_elem,
+                                 int n_sp, void* stream);
+int zj_gpu_copy_solution(double* dst, const double* src, int n, void* stream);
+int zj_gpu_scale_array(double* dst, const double* src, double alpha, int n, void* stream);
+int zj_gpu_axpy(double* y, const double* x, double alpha, int n, void* stream);
 
 /* ---- host mesh and operator tables (native, used by FRSolver) -------- */
 int zj_quad_operators(int poly_order, double* sol_xi, double* sol_eta, double* quad_w,

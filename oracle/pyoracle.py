@@ -7,6 +7,7 @@ oracle/zj_oracle.h for what each function restates and how it is pinned.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass
 from typing import Optional
@@ -535,3 +536,119 @@ def riemann_batch(kind, UL, UR, nrm, gamma=1.4):
     out = np.zeros_like(UL)
     lib().oz_riemann_batch(RIEMANN[kind], UL.shape[0], UL, UR, nrm, gamma, out)
     return out
+
+
+# --------------------------------------------- the reference's gpu:: helpers
+# Restatements of the free functions of kernels.cu that FRSolver's residual
+# does not call, for the tests of the library's gpu:: layer (small inputs,
+# plain Python loops).
+
+def _pressure(U, gamma):
+    """IdealGas::pressure (euler.hpp:16-23)."""
+    rho = U[0]
+    u, v = U[1] / rho, U[2] / rho
+    return (gamma - 1.0) * rho * (U[3] / rho - 0.5 * (u * u + v * v))
+
+
+def _sound_speed(U, gamma):
+    """IdealGas::soundSpeed (euler.hpp:32-41)."""
+    return math.sqrt(gamma * max(_pressure(U, gamma), 1e-10) / max(U[0], 1e-10))
+
+
+def _prim2cons(rho, u, v, p, gamma):
+    return np.array([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])
+
+
+def apply_boundary_conditions(U_fp, elem_idx, local_face, bc_type, bc_data, normals, gamma, nfp, nfe):
+    """applyBoundaryConditionsKernel (kernels.cu:1040-1211): returns a copy of
+    U_fp [E][nfe][nfp][4] with the ghost state written over each boundary
+    face's interior flux-point states; normals [n_bc][nfp][2], bc_data
+    [n_bc][8], both indexed by boundary face.  Pinned by the relations each
+    branch must satisfy (tests/test_gpu_api_oracle.py)."""
+    out = np.array(U_fp, np.float64).reshape(-1, nfe, nfp, 4).copy()
+    nrm = np.asarray(normals, np.float64).reshape(-1, nfp, 2)
+    bd = np.asarray(bc_data, np.float64).reshape(-1, 8)
+    g = gamma
+    for b in range(len(elem_idx)):
+        for f in range(nfp):
+            U = out[elem_idx[b], local_face[b], f].copy()
+            nx, ny = nrm[b, f]
+            t = int(bc_type[b])
+            if t in (BC_WALL, BC_SLIPWALL, BC_SYMMETRY):
+                rho = U[0]; u = U[1] / rho; v = U[2] / rho; p = _pressure(U, g)
+                vn = u * nx + v * ny
+                G = _prim2cons(rho, u - 2.0 * vn * nx, v - 2.0 * vn * ny, p, g)
+            elif t == BC_FARFIELD:
+                rinf, uinf, vinf, pinf = bd[b, :4]
+                cinf = _sound_speed(_prim2cons(rinf, uinf, vinf, pinf, g), g)
+                ri = U[0]; ui = U[1] / ri; vi = U[2] / ri; ci = _sound_speed(U, g)
+                vni = ui * nx + vi * ny
+                vninf = uinf * nx + vinf * ny
+                Rp = vni + 2.0 * ci / (g - 1.0)
+                Rm = vninf - 2.0 * cinf / (g - 1.0)
+                vnb = 0.5 * (Rp + Rm)
+                cb = 0.25 * (g - 1.0) * (Rp - Rm)
+                if vnb >= 0:
+                    vt = ui * (-ny) + vi * nx
+                    s = _pressure(U, g) / ri ** g
+                else:
+                    vt = uinf * (-ny) + vinf * nx
+                    s = pinf / rinf ** g
+                rb = (cb * cb / (g * s)) ** (1.0 / (g - 1.0))
+                pb = rb * cb * cb / g
+                G = _prim2cons(rb, vnb * nx - vt * ny, vnb * ny + vt * nx, pb, g)
+            elif t == BC_OUTFLOW:
+                rho = U[0]
+                G = _prim2cons(rho, U[1] / rho, U[2] / rho, bd[b, 0], g)
+            elif t == BC_INFLOW:
+                pt, Tt, dx, dy = bd[b, :4]
+                dm = math.sqrt(dx * dx + dy * dy)
+                if dm > 1e-10:
+                    dx, dy = dx / dm, dy / dm
+                else:
+                    dx, dy = -nx, -ny
+                ps = _pressure(U, g)
+                M2 = max(2.0 / (g - 1.0) * ((pt / ps) ** ((g - 1.0) / g) - 1.0), 0.0)
+                M = math.sqrt(M2)
+                Ts = Tt / (1.0 + 0.5 * (g - 1.0) * M2)
+                rb = ps / Ts
+                vel = M * math.sqrt(g * ps / rb)
+                G = _prim2cons(rb, vel * dx, vel * dy, ps, g)
+            else:
+                G = U
+            out[elem_idx[b], local_face[b], f] = G
+    return out.reshape(-1)
+
+
+def dt_tree_points(n_sp):
+    """Solution points whose value reaches computeTimeStepKernel's per-element
+    minimum (kernels.cu:1389-1397, blockDim = n_sp): the tree
+    `for s = n_sp/2; s > 0; s >>= 1: if sp < s and sp + s < n_sp` run on index
+    sets."""
+    sets = [{i} for i in range(n_sp)]
+    s = n_sp // 2
+    while s > 0:
+        for sp in range(s):
+            if sp + s < n_sp:
+                sets[sp] |= sets[sp + s]
+        s >>= 1
+    return sorted(sets[0])
+
+
+def dt_gpu(U, J, h_min, cfl, gamma, n_sp):
+    """gpu::computeTimeStep (kernels.cu:1355-1474): min of
+    CFL h_min[e] / (maxWaveSpeed(U) sqrt|J|) over the points dt_tree_points keeps."""
+    U = np.asarray(U, np.float64).reshape(-1, n_sp, 4)
+    J = np.asarray(J, np.float64).reshape(-1, n_sp)
+    keep = dt_tree_points(n_sp)
+    best = 1e20
+    for e in range(U.shape[0]):
+        for sp in keep:
+            ws = max_wave_speed(U[e, sp], gamma)
+            best = min(best, cfl * h_min[e] / (ws * math.sqrt(abs(J[e, sp]))))
+    return best
+
+
+def max_wave_speed(U, gamma=1.4):
+    """IdealGas::maxWaveSpeed (euler.hpp:166-171) through the oracle library."""
+    return lib().oz_max_wave_speed(np.ascontiguousarray(U, np.float64), gamma)

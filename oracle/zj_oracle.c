@@ -19,6 +19,7 @@
 
 static int g_threads = 1;
 void oz_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int oz_threads(void) { return g_threads; }
 
 /* ======================================================================= */
 /* Operators                                                               */

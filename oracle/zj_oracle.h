@@ -118,6 +118,7 @@ typedef struct {
 oz_solver* oz_create(const oz_desc* d);
 void oz_destroy(oz_solver* s);
 void oz_set_threads(int n);   /* OpenMP threads for the loops below (1 = serial) */
+int oz_threads(void);
 
 /* Individual kernels, each a literal restatement of one reference kernel.
  * They read/write the solver's internal buffers; accessors below expose them. */

@@ -445,7 +445,11 @@ static void visc_common(const oz_tb* t, const double* UL, const double* qxL, con
 void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
     const int n1 = t->n1, S = t->S, E = t->E, P = t->p;
     const double g = t->g;
-    /* traces, then (viscous) gradients and jumps */
+    /* traces, then (viscous) gradients and jumps.  Every loop below writes
+     * each output from one iteration only, so the thread count (oz_set_threads)
+     * does not change the result. */
+    const int nt = oz_threads();
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int e = 0; e < E; ++e)
         for (int k = 0; k < 4; ++k)
             for (int f = 0; f < n1; ++f)
@@ -454,6 +458,7 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
                         edge_value(t, U + (size_t)e * S * NV + v, NV, k, f);
     if (t->viscous) visc_gradients(t, U);
     /* transformed total fluxes: inviscid minus viscous (with the corrected gradient) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int e = 0; e < E; ++e) {
         for (int sp = 0; sp < S; ++sp) {
             const size_t gi = (size_t)e * S + sp;
@@ -474,6 +479,7 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
         }
     }
     /* common fluxes: one evaluation per face point, outward on each side */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int fc = 0; fc < t->F; ++fc) {
         const int le = t->le[fc], ll = t->ll[fc], re = t->re[fc], rl = t->rl[fc];
         const double nx = t->normals[2 * fc], ny = t->normals[2 * fc + 1];
@@ -509,6 +515,7 @@ void oz_tb_residual(oz_tb* t, const double* U, double* dUdt) {
         }
     }
     /* divergence and correction */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (int e = 0; e < E; ++e) {
         double delta[4][6][NV];
         const double sgn[4] = {-1.0, 1.0, 1.0, -1.0};

@@ -1,0 +1,126 @@
+"""Discrete adjoint of the textbook scheme (ZJ_COMPAT_TEXTBOOK: VCJH correction
+with metrics, Rusanov / Roe / HLLC of euler.hpp) on an MI355X.
+
+As for the reference arithmetic (test_gpu_adjoint.py), the pin is the
+identity <w, J v> = <J^T w, v> with J v a central finite difference of the
+textbook ORACLE's forward map (oracle/zj_textbook.c, FP64) and J^T w the HIP
+adjoint; bar 1e-6 relative.  The Riemann solvers' VJPs, the transposed VCJH
+correction, the transposed interpolation of the transformed flux and the 1/J
+of non-affine elements each move the HIP side by O(1) when wrong.
+"""
+import numpy as np
+import pytest
+
+from tests import fr_case
+from tests.test_gpu_adjoint import TOL, direction, dot_check, fd
+from tests.test_gpu_parity import say, tgv_state, vortex_state
+from zhijian_amd import cases, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(mesh, p, riemann, corr, periodic=True, precision="fp64", bc_types=None, fs=None, positivity=False):
+    per = (mesh.Lx, mesh.Ly) if periodic else None
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=per, bc_type_per_face=bc_types, fs=fs)
+    s = solver.Solver(t, precision=precision, compat="textbook", riemann=riemann, corr=corr,
+                      positivity=positivity)
+    oc = fr_case.build(mesh.xy, mesh.elem_nodes, p, periodic=per, bc_type_per_face=bc_types, fs=fs)
+    return s, oc.textbook(riemann, corr), t
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("riemann", ["rusanov", "roe", "hllc"])
+def test_textbook_residual_vjp(p, riemann):
+    """Every order and Riemann solver, affine and jittered (per-point metric) meshes."""
+    for jit in (0.0, 0.12):
+        say(f"case: textbook residual VJP, periodic 5x4 P{p} {riemann}/hu jitter {jit}")
+        mesh = cases.uniform_quad_mesh(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=jit, seed=p)
+        s, tb, t = _pair(mesh, p, riemann, "hu")
+        U = tgv_state(t, amp=0.05, seed=40 + p, mach=0.3)
+        s.set_state(U)
+        w = np.random.default_rng(p).standard_normal(U.shape)
+        v = direction(U, 50 + p)
+        assert dot_check(float(w @ fd(tb.residual, U, v)), float(s.residual_vjp(w) @ v), "residual") < TOL
+
+
+@pytest.mark.parametrize("corr", ["dg", "sd", "ga"])
+def test_textbook_residual_vjp_corrections(corr):
+    say(f"case: textbook residual VJP, vortex 6x6 P3 roe/{corr}")
+    s, tb, t = _pair(cases.vortex_mesh(6), 3, "roe", corr)
+    U = vortex_state(t, amp=0.05, seed=61)
+    s.set_state(U)
+    w = np.random.default_rng(62).standard_normal(U.shape)
+    v = direction(U, 63)
+    assert dot_check(float(w @ fd(tb.residual, U, v)), float(s.residual_vjp(w) @ v), corr) < TOL
+
+
+@pytest.mark.parametrize("bct", [1, 2, 4, 6])
+@pytest.mark.parametrize("riemann", ["roe", "hllc"])
+def test_textbook_residual_vjp_boundaries(bct, riemann):
+    """Open jittered mesh, every boundary face of one type (wall, slip wall, far field, outflow)."""
+    say(f"case: textbook residual VJP, open 5x4 P3 {riemann}, boundary type {bct}")
+    mesh = cases.uniform_quad_mesh(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.15, seed=3)
+    faces = solver._lib.build_faces(mesh.elem_nodes)
+    types = np.zeros(faces.shape[0], np.int32)
+    types[faces[:, 2] < 0] = bct
+    fs = (1.1, 0.3, 0.1, 1.5e3)
+    s, tb, t = _pair(mesh, 3, riemann, "hu", periodic=False, bc_types=types, fs=fs)
+    shp = t.pos.shape[:2]
+    U = cases.perturb(cases.prim_to_cons(np.full(shp, 1.0), np.full(shp, 0.3), np.full(shp, -0.1),
+                                         np.full(shp, 2e3), 1.4), 0.05, 4).reshape(-1)
+    s.set_state(U)
+    w = np.random.default_rng(5).standard_normal(U.shape)
+    v = direction(U, 6)
+    assert dot_check(float(w @ fd(tb.residual, U, v)), float(s.residual_vjp(w) @ v), f"BC {bct}") < TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+@pytest.mark.parametrize("p", [2, 4])
+def test_textbook_step_vjp_and_dt(precision, p):
+    """One SSP-RK3 step of the textbook scheme: state and dt cotangents
+    (P2 runs its adjoint at the half tile, P4 at the full one)."""
+    say(f"case: textbook step VJP, TGV 6x6 P{p} roe/hu {precision}")
+    mesh = cases.uniform_quad_mesh(6, 6, 0.0, 2 * np.pi, 0.0, 2 * np.pi, jitter=0.1, seed=p)
+    s, tb, t = _pair(mesh, p, "roe", "hu", precision=precision)
+    U = tgv_state(t, amp=0.02, seed=70 + p, mach=0.3)
+    dt = 2e-3
+    ubar = np.random.default_rng(71).standard_normal(U.shape)
+    v = direction(U, 72)
+    lhs = float(ubar @ fd(lambda X: tb.advance(X, dt, False), U, v))
+    s.set_state(U)
+    s.adjoint_set(ubar)
+    dtb = s.rk3_step_vjp(dt)
+    g = s.adjoint_get()
+    tol = TOL if precision == "fp64" else 2e-4
+    assert dot_check(lhs, float(g @ v), "step") < tol
+    h = 1e-7
+    dfd = (float(ubar @ tb.advance(U, dt + h, False)) - float(ubar @ tb.advance(U, dt - h, False))) / (2 * h)
+    rd = abs(dtb - dfd) / abs(dfd)
+    say(f"  dJ/ddt: adjoint {dtb:.12e}  FD {dfd:.12e}  rel {rd:.2e}")
+    assert rd < (1e-6 if precision == "fp64" else 2e-4)
+
+
+def test_textbook_step_vjp_with_limiter():
+    """The textbook step with the reference's positivity limiter switched on."""
+    say("case: textbook step VJP with the limiter, vortex 6x6 P3 hllc/ga")
+    s, tb, t = _pair(cases.vortex_mesh(6), 3, "hllc", "ga", positivity=True)
+    U = vortex_state(t, amp=0.05, seed=81)
+    dt = 1e-3
+    ubar = np.random.default_rng(82).standard_normal(U.shape)
+    v = direction(U, 83)
+    lhs = float(ubar @ fd(lambda X: tb.advance(X, dt, True), U, v))
+    s.set_state(U)
+    s.adjoint_set(ubar)
+    s.rk3_step_vjp(dt)
+    assert dot_check(lhs, float(s.adjoint_get() @ v), "step+limiter") < TOL
+
+
+def test_navier_stokes_adjoint_refused():
+    mesh = cases.tgv_mesh(4)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 2, periodic=(mesh.Lx, mesh.Ly))
+    s = solver.Solver(t, compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Re=100.0)
+    U = tgv_state(t, amp=0.02, seed=1, mach=0.3)
+    s.set_state(U)
+    with pytest.raises(solver.ZJError) as e:
+        s.residual_vjp(np.ones_like(U))
+    assert e.value.code == 4

@@ -52,6 +52,44 @@ def test_batch_equals_members(precision):
     batch.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_single_launch_equals_members(precision):
+    """One launch per stage over every member (orders P1-P5, both metric forms
+    in the same kernel) gives each member's own step bit for bit, under both
+    grid policies; the batch autotuner leaves the states as they were."""
+    say(f"case: single-launch batch of {len(MEMBERS)} meshes vs own steps, {precision}")
+    built = [_member(*m, precision, seed=50 + k) for k, m in enumerate(MEMBERS)]
+    a = [solver.Solver(t, precision=precision) for t, _ in built]
+    b = [solver.Solver(t, precision=precision) for t, _ in built]
+    for s1, s2, (t, U) in zip(a, b, built):
+        s1.set_state(U)
+        s2.set_state(U)
+    batch = solver.Batch(a)
+    batch.set_launch(True)
+    dts = np.array([1e-3 * (1 + 0.05 * k) for k in range(len(a))])
+    for step in range(3):
+        batch.set_grid(0 if step < 2 else 4)
+        batch.rk3_step(dts)
+        for s2, dt in zip(b, dts):
+            s2.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), MEMBERS[k]
+    before = [s1.get_state() for s1 in a]
+    single, policy, ms = batch.autotune(dts, reps=2)
+    say(f"  autotune: single={single} policy={policy} {ms:.3f} ms per step")
+    assert ms > 0 and policy in (0, 2, 4)
+    for k, s1 in enumerate(a):
+        assert np.array_equal(s1.get_state(), before[k]), MEMBERS[k]
+    batch.rk3_step(dts)
+    for s2, dt in zip(b, dts):
+        s2.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), MEMBERS[k]
+    batch.close()
+
+
 def test_batch_member_changes():
     """While a batch holds a handle, changes that would leave the batch's
     launch arguments stale are refused (tile, metric form, autotune, destroy),

@@ -79,6 +79,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
            "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
+           "zj_batch_set_launch", "zj_batch_autotune", "zj_state_slots", "zj_state_save", "zj_state_load",
            "zj_gmsh_read", "zj_gmsh_error", "zj_gmsh_free", "zj_gmsh_counts", "zj_gmsh_nodes",
            "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group", "zj_get_tiling",
            "zj_tri_create", "zj_tri_destroy", "zj_tri_last_error", "zj_tri_set_state", "zj_tri_get_state",
@@ -88,7 +89,16 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_tri_get_scheme", "zj_io_last_error", "zj_restart_write", "zj_restart_query", "zj_restart_read",
            "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
            "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes",
-           "zj_tri_lift", "zj_time_stages", "zj_tri_time_stages")
+           "zj_tri_lift", "zj_time_stages", "zj_tri_time_stages",
+           # the gpu:: free functions (zhijian_amd.gpu_ops binds them)
+           "zj_gpu_set_operator_engine", "zj_gpu_interpolate_to_flux_points", "zj_gpu_reference_gradients",
+           "zj_gpu_transform_gradients", "zj_gpu_inviscid_flux_sp", "zj_gpu_riemann_flux",
+           "zj_gpu_riemann_flux_with_bc", "zj_gpu_viscous_flux_sp", "zj_gpu_interior_flux_at_face",
+           "zj_gpu_inviscid_flux_at_fp", "zj_gpu_flux_difference", "zj_gpu_flux_diff_for_fr",
+           "zj_gpu_flux_divergence", "zj_gpu_scatter_flux_to_elements", "zj_gpu_apply_fr_correction",
+           "zj_gpu_apply_boundary_conditions", "zj_gpu_enforce_positivity", "zj_gpu_compute_time_step",
+           "zj_gpu_residual_norm", "zj_gpu_set_initial_condition", "zj_gpu_copy_solution", "zj_gpu_scale_array",
+           "zj_gpu_axpy")
 
 
 def lib() -> C.CDLL:
@@ -130,6 +140,12 @@ def lib() -> C.CDLL:
         L.zj_batch_rk3_step.argtypes = [H, _dp, C.c_void_p]
         L.zj_batch_synchronize.argtypes = [H]
         L.zj_batch_set_grid.argtypes = [H, C.c_int]
+        L.zj_batch_set_launch.argtypes = [H, C.c_int]
+        L.zj_state_slots.argtypes = [H, C.c_int]
+        L.zj_state_save.argtypes = [H, C.c_int, C.c_void_p]
+        L.zj_state_load.argtypes = [H, C.c_int, C.c_void_p]
+        L.zj_batch_autotune.argtypes = [H, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double)]
         L.zj_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.zj_gmsh_read.argtypes = [C.c_char_p, C.POINTER(H)]
         L.zj_gmsh_error.restype = C.c_char_p

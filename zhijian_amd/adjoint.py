@@ -8,8 +8,11 @@ with the reference's quadrature weights (FROperators quad_weights,
 basis.cpp:484) and Jacobian determinants (FRSolver::initGeometry,
 fr_solver.cpp:85-129).  The forward trajectory keeps one state every
 `every` steps; the reverse sweep recomputes each segment from its checkpoint
-and runs zj_rk3_step_vjp backwards through it.  The library holds the
-cotangent on the device between steps.
+and runs zj_rk3_step_vjp backwards through it.  Checkpoints and the
+segment's states stay on the GPU (zj_state_slots / zj_state_save /
+zj_state_load: no host copy per step), and the library holds the cotangent
+on the device between steps; the host sees the final state (for J and its
+gradient) and the result.
 """
 from __future__ import annotations
 
@@ -39,30 +42,34 @@ def kinetic_energy_grad(U: np.ndarray, t: Tables) -> np.ndarray:
     return g.reshape(-1)
 
 
-def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every: int = 16
-             ) -> Tuple[float, np.ndarray, List[float]]:
-    """J(U^N) after len(dts) steps from U0, dJ/dU0 and dJ/d(dt_n)."""
+def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every: int = 16, want_dt: bool = True,
+             stream=None) -> Tuple[float, np.ndarray, List[float]]:
+    """J(U^N) after len(dts) steps from U0, dJ/dU0 and dJ/d(dt_n) (None
+    each with want_dt=False, which keeps the reverse sweep free of host
+    synchronisation)."""
     steps = len(dts)
-    checkpoints = {0: np.array(U0, np.float64).reshape(-1)}
-    sol.set_state(checkpoints[0])
+    n_ckpt = (steps + every - 1) // every
+    # slots 0 .. n_ckpt-1: checkpoints; n_ckpt .. n_ckpt+every-2: the segment's later states
+    sol.state_slots(n_ckpt + every - 1)
+    sol.set_state(np.array(U0, np.float64).reshape(-1))
     for n in range(steps):
-        sol.rk3_step(dts[n])
-        if (n + 1) % every == 0 and n + 1 < steps:
-            checkpoints[n + 1] = sol.get_state()
+        if n % every == 0:
+            sol.state_save(n // every, stream)
+        sol.rk3_step(dts[n], stream)
     UN = sol.get_state()
     J = kinetic_energy(UN, t)
     sol.adjoint_set(kinetic_energy_grad(UN, t))
-    dtb = [0.0] * steps
-    seg_starts = sorted(checkpoints, reverse=True)
-    for start in seg_starts:
+    dtb = [None] * steps
+    for c in range(n_ckpt - 1, -1, -1):
+        start = c * every
         end = min(start + every, steps)
-        # states start .. end-1 of this segment
-        traj = [checkpoints[start]]
-        sol.set_state(traj[0])
-        for n in range(start, end - 1):
-            sol.rk3_step(dts[n])
-            traj.append(sol.get_state())
+        sol.state_load(c, stream)
+        for n in range(start, end - 1):  # states start+1 .. end-1 of the segment
+            sol.rk3_step(dts[n], stream)
+            sol.state_save(n_ckpt + n - start, stream)
         for n in range(end - 1, start - 1, -1):
-            sol.set_state(traj[n - start])
-            dtb[n] = sol.rk3_step_vjp(dts[n])
-    return J, sol.adjoint_get(), dtb
+            sol.state_load(c if n == start else n_ckpt + n - start - 1, stream)
+            dtb[n] = sol.rk3_step_vjp(dts[n], stream, want_dt=want_dt)
+    g = sol.adjoint_get()
+    sol.state_slots(0)
+    return J, g, dtb

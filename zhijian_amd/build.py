@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-f
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", INC, "-I", CSRC, "-Wall",
              "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include"]
 
-HIP_SRCS = ["zj_kernels.hip", "zj_adjoint.hip", "zj_solver.hip", "zj_tri.hip", "zj_visc.hip"]
+HIP_SRCS = ["zj_kernels.hip", "zj_adjoint.hip", "zj_solver.hip", "zj_tri.hip", "zj_visc.hip", "zj_gpu_api.hip"]
 CXX_SRCS = ["zj_host.cpp", "zj_frsolver.cpp", "zj_gmsh.cpp", "zj_h5.cpp", "zj_io.cpp"]
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "frsolver_driver.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "frsolver_driver")

@@ -24,6 +24,7 @@
 // zero for being non-finite has zero derivative.
 #include "zj_adjoint.hpp"
 #include "zj_device.hpp"
+#include "zj_riemann_vjp.hpp"
 
 #include <cmath>
 #include <type_traits>
@@ -234,7 +235,7 @@ __device__ __forceinline__ void positivity_vjp(const double* q, const double* ob
 }
 
 // ------------------------------------------------------------------ kernel
-template <typename R, int P, int EPB, int MODE, bool AFF>
+template <typename R, int P, int EPB, int MODE, bool AFF, bool TB>
 __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops) {
     using Y = Lay<R, P>;
     constexpr bool INC = std::is_same<R, float>::value;
@@ -263,7 +264,11 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma);
     for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
-    for (int t = tid; t < 4 * S; t += BLOCK) sC[t] = ops.corr[t];
+    if (TB) {  // textbook: g_L', g_R' at the 1D points
+        if (tid < N1) { sC[tid] = ops.gL[tid]; sC[N1 + tid] = ops.gR[tid]; }
+    } else {
+        for (int t = tid; t < 4 * S; t += BLOCK) sC[t] = ops.corr[t];
+    }
     if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
 
     const int el = tid / S, sp = tid - (tid / S) * S;
@@ -276,6 +281,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     // ---- forward A: state, flux
     double q[NV], u0[NV];
     R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
+    R jd = R(1);  // textbook, non-affine: the transformed fluxes carry J
     R uq[NV];
     if (active) {
         if (AFF) {
@@ -283,6 +289,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             kx = a.Jinv[(int64_t)e * 4 + 2]; ky = a.Jinv[(int64_t)e * 4 + 3];
         } else {
             jx = a.Jinv[gi]; jy = a.Jinv[PS + gi]; kx = a.Jinv[2 * PS + gi]; ky = a.Jinv[3 * PS + gi];
+            if (TB) jd = a.Jpt[gi];
         }
         R fx[NV], fy[NV];
 #pragma unroll
@@ -299,6 +306,9 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             if (AFF) {
                 sFx[ub + v * Y::UV] = jx * fx[v] + jy * fy[v];
                 sFy[ub + v * Y::UV] = kx * fx[v] + ky * fy[v];
+            } else if (TB) {
+                sFx[ub + v * Y::UV] = jd * (jx * fx[v] + jy * fy[v]);
+                sFy[ub + v * Y::UV] = jd * (kx * fx[v] + ky * fy[v]);
             } else {
                 sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
             }
@@ -339,20 +349,35 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             R own[NV], mirr[NV], other[NV], UL[NV], UR[NV], Fc[NV], Fn[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { own[v] = sTr[Y::tr(el2, k, f, v)]; mirr[v] = sTr[Y::tr(el2, k, P - f, v)]; }
+            // reference: the right role reads the neighbour's slot f (kernels.cu:577);
+            // textbook: the geometrically matching slot p-f on both sides
+            const int oslot = (TB || !right) ? P - f : f;
             if (L.role == ROLE_BOUNDARY) {
                 ghost_state(own, nx, ny, L.nbr_edge, a.bc_data + (int64_t)L.bc_idx * 8, g, other);
             } else if (inside) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, right ? f : P - f, v)];
+                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
             } else {
-                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (right ? f : P - f)) * NV, other);
+                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * NV, other);
             }
+            if (TB) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
-            rusanov(UL, UR, nx, ny, g, Fc);
-            normal_flux(own, g, nx, ny, Fn);
+                for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? own[v] : other[v]; }
+                riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
+                const R ef = a.edge_f[(int64_t)e2 * 4 + k];
+                const TraceLine tl = trace_line<P>(k, f);
+                const R* Fbase = ((k & 1) ? sFx : sFy) + el2 * Y::UE;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
+                for (int v = 0; v < NV; ++v)
+                    Fd[v] = ef * (right ? -Fc[v] : Fc[v]) - trace_point<R, P>(Fbase + v * Y::UV, tl, sLm, sLp);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? mirr[v] : other[v]; }
+                rusanov(UL, UR, nx, ny, g, Fc);
+                normal_flux(own, g, nx, ny, Fn);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fd[v] = right ? Fn[v] - Fc[v] : Fc[v] - Fn[v];
+            }
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) sFd[Y::tr(el2, k, f, v)] = Fd[v];
@@ -369,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             const R* Fx = sFx + el * Y::UE + v * Y::UV;
             const R* Fy = sFy + el * Y::UE + v * Y::UV;
             R div;
-            if (AFF) {
+            if (AFF || TB) {
                 R dxi = 0, deta = 0;
 #pragma unroll
                 for (int m = 0; m < N1; ++m) {
@@ -389,9 +414,15 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
                 }
                 div = (dFdxi * jx + dFdeta * kx) + (dGdxi * jy + dGdeta * ky);
             }
-            const R c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
-                        sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
-            res[v] = double(-div - c);
+            if (TB) {
+                const R c = sFd[Y::tr(el, 3, P - j, v)] * sC[i] + sFd[Y::tr(el, 1, j, v)] * sC[N1 + i] +
+                            sFd[Y::tr(el, 0, i, v)] * sC[j] + sFd[Y::tr(el, 2, P - i, v)] * sC[N1 + j];
+                res[v] = double(-(div + c) * (AFF ? R(1) : rcpR(jd)));
+            } else {
+                const R c = sFd[Y::tr(el, 0, i, v)] * sC[sp] + sFd[Y::tr(el, 1, j, v)] * sC[S + sp] +
+                            sFd[Y::tr(el, 2, i, v)] * sC[2 * S + sp] + sFd[Y::tr(el, 3, j, v)] * sC[3 * S + sp];
+                res[v] = double(-div - c);
+            }
         }
     }
     __syncthreads();  // sFx, sFy, sFd are reused below
@@ -430,9 +461,11 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
                 rsum += cc * res[v] * unb[v];
             }
         }
+        // textbook, non-affine: R = -(div + c) / J, so div and c see rb / J
+        const R rs = (TB && !AFF) ? rcpR(jd) : R(1);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sFx[ub + v * Y::UV] = R(rb[v]);
-        if (!AFF) {  // the per-point metric for the transposed divergence
+        for (int v = 0; v < NV; ++v) sFx[ub + v * Y::UV] = R(rb[v]) * rs;
+        if (!AFF && !TB) {  // the per-point metric for the transposed divergence
             sFy[ub] = jx; sFy[ub + Y::UV] = jy; sFy[ub + 2 * Y::UV] = kx; sFy[ub + 3 * Y::UV] = ky;
         }
     }
@@ -442,6 +475,101 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     }
     __syncthreads();
 
+    if (TB) {
+    // ---- R2b (textbook): per (edge, flux point) the cotangent of
+    // Delta = ef (+-F^I) - (interpolated transformed flux): the correction's
+    // transpose gathers it from the points of the slot's grid line; its flux
+    // part waits in sTm for the points, its common-flux part runs through the
+    // Riemann solver's VJP to this side (sFd) and the other (sTb or HBM)
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        // edges 1, 3 correct along the xi line j = f (1) / p-f (3); edges 0, 2 the eta line i = f / p-f
+        const R* Rb = sFx + el2 * Y::UE;
+        const bool odd = (k & 1) != 0;
+        const int line = (k == 0 || k == 1) ? f : P - f;
+        const R* gw = sC + ((k == 1 || k == 2) ? N1 : 0);
+        R Fdb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R acc = 0;
+#pragma unroll
+            for (int m = 0; m < N1; ++m) acc -= gw[m] * Rb[v * Y::UV + (odd ? line * N1 + m : m * N1 + line)];
+            Fdb[v] = acc;
+            sTm[Y::tr(el2, k, f, v)] = -acc;
+        }
+        R ownb[NV] = {R(0), R(0), R(0), R(0)};
+        if (L.role != ROLE_NONE) {
+            const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+            const bool right = L.role == ROLE_RIGHT;
+            const int nl = L.nbr - e0;
+            const bool inside = L.role != ROLE_BOUNDARY && nl >= 0 && nl < EPB;
+            R own[NV], other[NV], UL[NV], UR[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) own[v] = sTr[Y::tr(el2, k, f, v)];
+            const double* bcd = a.bc_data + (int64_t)(L.role == ROLE_BOUNDARY ? L.bc_idx : 0) * 8;
+            if (L.role == ROLE_BOUNDARY) {
+                ghost_state(own, nx, ny, L.nbr_edge, bcd, g, other);
+            } else if (inside) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) other[v] = sTr[Y::tr(nl, L.nbr_edge, P - f, v)];
+            } else {
+                load4(a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (P - f)) * NV, other);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { UL[v] = right ? other[v] : own[v]; UR[v] = right ? own[v] : other[v]; }
+            const R ef = a.edge_f[(int64_t)e2 * 4 + k];
+            R Fcb[NV], ULb[NV] = {R(0), R(0), R(0), R(0)}, URb[NV] = {R(0), R(0), R(0), R(0)};
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Fcb[v] = ef * (right ? -Fdb[v] : Fdb[v]);
+            riemann_by_kind_vjp(a.riemann, UL, UR, nx, ny, g, Fcb, ULb, URb);
+            R othb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                ownb[v] = right ? URb[v] : ULb[v];
+                othb[v] = right ? ULb[v] : URb[v];
+            }
+            if (L.role == ROLE_BOUNDARY) {
+                ghost_state_vjp(own, nx, ny, L.nbr_edge, bcd, g, othb, ownb);
+            } else if (inside) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sTb[Y::tr(nl, L.nbr_edge, P - f, v)] = othb[v];
+            } else {
+                store4(a.Tb_out + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + (P - f)) * NV, othb);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sFd[Y::tr(el2, k, f, v)] = ownb[v];  // sFd was last read in forward C
+    }
+    __syncthreads();
+
+    // ---- R2a (textbook): transposed divergence plus the interpolated-flux
+    // cotangents of the edge points on this point's lines, then the metric and
+    // the Euler flux Jacobian
+    if (active) {
+        const R* Rb = sFx + el * Y::UE;
+        R Fxb[NV], Fyb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R* rv = Rb + v * Y::UV;
+            R ftb = sLp[i] * sTm[Y::tr(el, 1, j, v)] + sLm[i] * sTm[Y::tr(el, 3, P - j, v)];
+            R gtb = sLm[j] * sTm[Y::tr(el, 0, i, v)] + sLp[j] * sTm[Y::tr(el, 2, P - i, v)];
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                ftb -= sD[m * N1 + i] * rv[j * N1 + m];
+                gtb -= sD[m * N1 + j] * rv[m * N1 + i];
+            }
+            Fxb[v] = jd * (jx * ftb + kx * gtb);
+            Fyb[v] = jd * (jy * ftb + ky * gtb);
+        }
+        R ubq[NV] = {R(0), R(0), R(0), R(0)};
+        euler_flux_vjp(uq, g, Fxb, Fyb, ubq);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qb[v] += double(ubq[v]);
+    }
+    } else {
     // ---- R2a: transposed divergence and Euler flux Jacobian at the solution points
     if (active) {
         const R* Rb = sFx + el * Y::UE;
@@ -551,6 +679,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             sTm[Y::tr(el2, k, P - f, v)] = mirb[v];  // one writer per slot
         }
     }
+    }  // textbook / reference reverse phases
     __syncthreads();
 
     // ---- R3: transposed trace operator; cotangent of q
@@ -560,10 +689,11 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
         for (int v = 0; v < NV; ++v) {
             const int s0 = Y::tr(el, 0, i, v), s1 = Y::tr(el, 1, j, v);
             const int s2 = Y::tr(el, 2, P - i, v), s3 = Y::tr(el, 3, P - j, v);
-            const R t0 = sFd[s0] + sTm[s0] + (in0 ? sTb[s0] : R(0));
-            const R t1 = sFd[s1] + sTm[s1] + (in1 ? sTb[s1] : R(0));
-            const R t2 = sFd[s2] + sTm[s2] + (in2 ? sTb[s2] : R(0));
-            const R t3 = sFd[s3] + sTm[s3] + (in3 ? sTb[s3] : R(0));
+            // (textbook: sTm holds the flux part, already taken up in R2a)
+            const R t0 = sFd[s0] + (TB ? R(0) : sTm[s0]) + (in0 ? sTb[s0] : R(0));
+            const R t1 = sFd[s1] + (TB ? R(0) : sTm[s1]) + (in1 ? sTb[s1] : R(0));
+            const R t2 = sFd[s2] + (TB ? R(0) : sTm[s2]) + (in2 ? sTb[s2] : R(0));
+            const R t3 = sFd[s3] + (TB ? R(0) : sTm[s3]) + (in3 ? sTb[s3] : R(0));
             const R tsum = sLm[j] * t0 + sLp[j] * t2 + sLp[i] * t1 + sLm[i] * t3;
             double out = qb[v] + double(tsum);
             if (MODE == MODE_STAGE1) out += a.U0b[v * PS + gi];
@@ -616,15 +746,15 @@ constexpr int adj_tile_max() {
     return P <= 2 ? (BLOCK / ((P + 1) * (P + 1))) / 2 : BLOCK / ((P + 1) * (P + 1));
 }
 
-template <typename R, int P, bool AFF>
+template <typename R, int P, bool AFF, bool TB>
 static hipError_t vjp_dispatch(int mode, const VjpArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
     constexpr int T = adj_tile_max<P>();
     const dim3 grid(num_blocks(a.E, T)), block(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: vjp_kernel<R, P, T, MODE_RESIDUAL, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE1: vjp_kernel<R, P, T, MODE_STAGE1, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE2: vjp_kernel<R, P, T, MODE_STAGE2, AFF><<<grid, block, 0, s>>>(a, ops); break;
-    case MODE_STAGE3: vjp_kernel<R, P, T, MODE_STAGE3, AFF><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_RESIDUAL: vjp_kernel<R, P, T, MODE_RESIDUAL, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE1: vjp_kernel<R, P, T, MODE_STAGE1, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE2: vjp_kernel<R, P, T, MODE_STAGE2, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
+    case MODE_STAGE3: vjp_kernel<R, P, T, MODE_STAGE3, AFF, TB><<<grid, block, 0, s>>>(a, ops); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -651,11 +781,16 @@ int adjoint_tile(int P) {
 }
 
 template <typename R>
-hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, bool affine, hipStream_t s) {
+hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, bool affine, hipStream_t s,
+                      bool textbook) {
     if (a.E <= 0) return hipSuccess;
-#define ZJ_VJP_P(PP)                                                                              \
-    case PP:                                                                                      \
-        return affine ? vjp_dispatch<R, PP, true>(mode, a, ops, s) : vjp_dispatch<R, PP, false>(mode, a, ops, s);
+#define ZJ_VJP_P(PP)                                                                                      \
+    case PP:                                                                                              \
+        if (textbook)                                                                                     \
+            return affine ? vjp_dispatch<R, PP, true, true>(mode, a, ops, s)                              \
+                          : vjp_dispatch<R, PP, false, true>(mode, a, ops, s);                            \
+        return affine ? vjp_dispatch<R, PP, true, false>(mode, a, ops, s)                                 \
+                      : vjp_dispatch<R, PP, false, false>(mode, a, ops, s);
     switch (P) {
         ZJ_VJP_P(1) ZJ_VJP_P(2) ZJ_VJP_P(3) ZJ_VJP_P(4) ZJ_VJP_P(5)
     default: return hipErrorInvalidValue;
@@ -676,8 +811,10 @@ hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& o
     }
 }
 
-template hipError_t launch_vjp<double>(int, int, const VjpArgs<double>&, const OpTables<double>&, bool, hipStream_t);
-template hipError_t launch_vjp<float>(int, int, const VjpArgs<float>&, const OpTables<float>&, bool, hipStream_t);
+template hipError_t launch_vjp<double>(int, int, const VjpArgs<double>&, const OpTables<double>&, bool, hipStream_t,
+                                      bool);
+template hipError_t launch_vjp<float>(int, int, const VjpArgs<float>&, const OpTables<float>&, bool, hipStream_t,
+                                     bool);
 template hipError_t launch_complete<double>(int, const CompleteArgs<double>&, const OpTables<double>&, hipStream_t);
 template hipError_t launch_complete<float>(int, const CompleteArgs<float>&, const OpTables<float>&, hipStream_t);
 

@@ -31,6 +31,10 @@ struct VjpArgs {
     double gamma;
     int positivity;
     int xcd_groups;
+    // textbook mode (as StageArgs)
+    const R* edge_f;         // [E][4] sign * edge half length (/ J on affine meshes)
+    const R* Jpt;            // [E*S] Jacobian determinant (non-affine meshes)
+    int riemann;             // 0 Rusanov, 1 Roe, 2 HLLC
 };
 
 template <typename R>
@@ -44,8 +48,11 @@ struct CompleteArgs {
 };
 
 int adjoint_tile(int P);
+// textbook: the VJP of the textbook scheme's stage (VCJH correction with
+// metrics, the configured Riemann solver), else of the reference arithmetic.
 template <typename R>
-hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, bool affine, hipStream_t s);
+hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, bool affine, hipStream_t s,
+                      bool textbook = false);
 template <typename R>
 hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& ops, hipStream_t s);
 

@@ -35,6 +35,9 @@
 
 namespace zj {
 
+// Elements per workgroup of the full tile at order P.
+#define ZJ_EPB(P) (BLOCK / (((P) + 1) * ((P) + 1)))
+
 namespace {
 
 // LDS of one element tile of the stage kernel, as a policy of accessors.
@@ -358,6 +361,36 @@ __global__ __launch_bounds__(BLOCK) void batched_stage_kernel(const BatchInst<R>
     }
 }
 
+// One launch per stage for a whole batch, every order and metric form: work
+// item w = (instance, tile) in a single list; the instance's order and metric
+// form select the stage body, and the LDS buffer is sized for the largest tile
+// layout among the instances (dynamic shared memory).
+template <typename R, int MODE>
+__global__ __launch_bounds__(BLOCK) void batched_all_kernel(const BatchInst<R>* __restrict__ insts,
+                                                            const int2* __restrict__ items, int n_items) {
+    extern __shared__ __align__(16) unsigned char raw_all[];
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const int2 it = items[w];
+        const BatchInst<R>& in = insts[it.x];
+        const int kind = __builtin_amdgcn_readfirstlane(in.P * 2 + in.affine);
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+#define ZJ_BALL(PP)                                                                                          \
+    case 2 * PP:                                                                                             \
+        stage_body<R, PP, ZJ_EPB(PP), MODE, false>(in.args, in.ops, it.y, BatchSmem<R, PP, ZJ_EPB(PP)>{raw_all}, tid); \
+        break;                                                                                               \
+    case 2 * PP + 1:                                                                                         \
+        stage_body<R, PP, ZJ_EPB(PP), MODE, true>(in.args, in.ops, it.y, BatchSmem<R, PP, ZJ_EPB(PP)>{raw_all}, tid); \
+        break;
+        switch (kind) {
+            ZJ_BALL(1) ZJ_BALL(2) ZJ_BALL(3) ZJ_BALL(4) ZJ_BALL(5)
+        default: break;
+        }
+#undef ZJ_BALL
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
 template <typename R, int P, int EPB>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const TraceArgs<R> a, const OpTables<R> ops) {
     using Y = Lay<R, P>;
@@ -488,7 +521,6 @@ int max_epb(int P) { return BLOCK / ((P + 1) * (P + 1)); }
 int default_epb(int P) { return max_epb(P); }
 int num_blocks(int E, int epb) { return (E + epb - 1) / epb; }
 
-#define ZJ_EPB(P) (BLOCK / (((P) + 1) * ((P) + 1)))
 
 template <typename R, int P, int EPB, bool AFF, bool TB>
 static hipError_t stage_dispatch_mode(int mode, const StageArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
@@ -507,8 +539,11 @@ template <typename R, int P, bool AFF>
 static hipError_t stage_dispatch_epb(int mode, const StageArgs<R>& a, const OpTables<R>& ops, int epb, bool tb,
                                      hipStream_t s) {
     constexpr int M = ZJ_EPB(P);
-    if (tb) {  // the textbook scheme is built at the full tile only
+    if (tb) {  // the textbook scheme runs at the full tile; P1 and P2 also at half (the adjoint's tile)
         if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF, true>(mode, a, ops, s);
+        if constexpr (P <= 2) {
+            if (epb == M / 2) return stage_dispatch_mode<R, P, M / 2, AFF, true>(mode, a, ops, s);
+        }
         return hipErrorInvalidValue;
     }
     if (epb == 0 || epb == M) return stage_dispatch_mode<R, P, M, AFF, false>(mode, a, ops, s);
@@ -587,6 +622,33 @@ hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>
 #undef ZJ_BP
 }
 
+int batched_all_lds(int P, bool fp32) {
+    switch (P) {
+#define ZJ_BL(PP) \
+    case PP: return fp32 ? BatchSmem<float, PP, ZJ_EPB(PP)>::bytes : BatchSmem<double, PP, ZJ_EPB(PP)>::bytes;
+        ZJ_BL(1) ZJ_BL(2) ZJ_BL(3) ZJ_BL(4) ZJ_BL(5)
+#undef ZJ_BL
+    default: return 0;
+    }
+}
+
+template <typename R>
+hipError_t launch_batched_all(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                              int lds_bytes, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
+    const unsigned lds = (unsigned)lds_bytes;
+    switch (mode) {
+    case MODE_STAGE1: klaunch(batched_all_kernel<R, MODE_STAGE1>, g, b, lds, s, insts, items, n_items); break;
+    case MODE_STAGE2: klaunch(batched_all_kernel<R, MODE_STAGE2>, g, b, lds, s, insts, items, n_items); break;
+    case MODE_STAGE3: klaunch(batched_all_kernel<R, MODE_STAGE3>, g, b, lds, s, insts, items, n_items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_batched_all<double>(int, const BatchInst<double>*, const int2*, int, int, int, hipStream_t);
+template hipError_t launch_batched_all<float>(int, const BatchInst<float>*, const int2*, int, int, int, hipStream_t);
 template hipError_t launch_batched_stage<double>(int, bool, int, const BatchInst<double>*, const int2*, int, int,
                                                  hipStream_t);
 template hipError_t launch_batched_stage<float>(int, bool, int, const BatchInst<float>*, const int2*, int, int,

@@ -154,6 +154,13 @@ hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, in
 template <typename R>
 hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>* insts, const int2* items,
                                 int n_items, int grid, hipStream_t s);
+// The same stages for every member of a batch in ONE launch per stage:
+// items[k] = (index into insts, tile), insts across all orders and metric
+// forms; lds_bytes = batched_all_lds of the largest order present.
+template <typename R>
+hipError_t launch_batched_all(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                              int lds_bytes, hipStream_t s);
+int batched_all_lds(int P, bool fp32);
 int default_epb(int P);
 int max_epb(int P);
 

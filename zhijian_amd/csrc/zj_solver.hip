@@ -91,6 +91,8 @@ struct zj_solver {
     double* A_U0b = nullptr;       // cotangent of U^n within a step
     void* A_Tb[2] = {nullptr, nullptr};  // cross-tile edge cotangents, ping-pong
     double* A_dtb = nullptr;       // per-tile dt cotangents of the three stages
+    double* slots = nullptr;       // device copies of states (SoA planes), zj_state_slots
+    int n_slots = 0;
     int A_cur = 0;
     bool A_has_tb = false;
     bool A_seeded = false;
@@ -119,6 +121,11 @@ static int boundary_link(zj_solver* h, int f, const int* bc_type, const int* bc_
 }
 
 static thread_local std::string g_create_error;   // error text of handle-less calls, per thread
+
+namespace zj {
+// Error text of a handle-less call in another source file (zj_last_error(NULL)).
+void set_thread_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace zj
 
 #define ZJ_HIP(h, call)                                                                  \
     do {                                                                                 \
@@ -196,7 +203,7 @@ static void release(zj_solver* h) {
     if (h->graph) hipGraphDestroy(h->graph);
     void* bufs[] = {h->perm, h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
-                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb};
+                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb, h->slots};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -557,6 +564,53 @@ extern "C" int zj_set_state_device(zj_handle h, const double* d_aos, void* strea
     if (rc) return rc;
     h->has_state = true;
     h->stage_next = 1;
+    return ZJ_OK;
+}
+
+// ---- device state slots: checkpoints that never leave the GPU
+extern "C" int zj_state_slots(zj_handle h, int n) {
+    if (!h || n < 0) return ZJ_ERR_ARG;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->slots) {
+        ZJ_HIP(h, hipFree(h->slots));
+        h->slots = nullptr;
+        h->n_slots = 0;
+    }
+    if (n > 0) {
+        const size_t bytes = (size_t)n * h->E * h->S * 4 * sizeof(double);
+        if (hipMalloc((void**)&h->slots, bytes) != hipSuccess) {
+            h->slots = nullptr;
+            return fail(h, ZJ_ERR_NOMEM, "state slots: device allocation failed");
+        }
+        h->n_slots = n;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_state_save(zj_handle h, int slot, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (slot < 0 || slot >= h->n_slots) return fail(h, ZJ_ERR_ARG, "slot out of range");
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "state requested in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->slots + (size_t)slot * n, h->U, n * sizeof(double), hipMemcpyDeviceToDevice,
+                             pick(h, stream)));
+    return ZJ_OK;
+}
+
+extern "C" int zj_state_load(zj_handle h, int slot, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (slot < 0 || slot >= h->n_slots) return fail(h, ZJ_ERR_ARG, "slot out of range");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "state change in the middle of a step");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->U, h->slots + (size_t)slot * n, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int rc = refresh_trace(h, s);
+    if (rc) return rc;
+    h->has_state = true;
     return ZJ_OK;
 }
 
@@ -924,6 +978,9 @@ static VjpArgs<R> vjp_args(zj_solver* h) {
     a.gamma = h->gamma;
     a.positivity = h->positivity;
     a.xcd_groups = h->xcd_groups;
+    a.edge_f = (const R*)(h->affine ? h->edge_f_aff : h->edge_f_pt);
+    a.Jpt = (const R*)h->Jpt;
+    a.riemann = h->riemann;
     return a;
 }
 
@@ -935,13 +992,13 @@ static hipError_t recompute_stages(zj_solver* h, const OpTables<R>& ops, hipStre
     StageArgs<R> a = stage_args<R>(h, MODE_STAGE1);
     a.T_in = T[0]; a.T_out = T[1]; a.l2_partial = nullptr;
     if (inc) { a.dW_out = (float*)h->A_W1; a.U_out = nullptr; } else { a.U_out = h->A_W1; }
-    hipError_t e = launch_stage<R>(h->P, MODE_STAGE1, a, ops, h->adj_epb, h->affine, s);
+    hipError_t e = launch_stage<R>(h->P, MODE_STAGE1, a, ops, h->adj_epb, h->affine, s, h->textbook);
     if (e != hipSuccess) return e;
     StageArgs<R> b = stage_args<R>(h, MODE_STAGE2);
     b.T_in = T[1]; b.T_out = T[2]; b.l2_partial = nullptr;
     if (inc) { b.dW_in = (const float*)h->A_W1; b.dW_out = (float*)h->A_W2; b.U_in = nullptr; b.U_out = nullptr; }
     else { b.U_in = h->A_W1; b.U_out = h->A_W2; }
-    return launch_stage<R>(h->P, MODE_STAGE2, b, ops, h->adj_epb, h->affine, s);
+    return launch_stage<R>(h->P, MODE_STAGE2, b, ops, h->adj_epb, h->affine, s, h->textbook);
 }
 
 template <typename R>
@@ -965,7 +1022,7 @@ static hipError_t adjoint_stages(zj_solver* h, const OpTables<R>& ops, hipStream
         a.Tb_out = Tb[1 - h->A_cur];
         a.U0b = h->A_U0b;
         a.dtb_partial = h->A_dtb + (size_t)(3 - st) * h->adj_nt;
-        hipError_t e = launch_vjp<R>(h->P, st, a, ops, h->affine, s);
+        hipError_t e = launch_vjp<R>(h->P, st, a, ops, h->affine, s, h->textbook);
         if (e != hipSuccess) return e;
         h->A_cur = 1 - h->A_cur;
         h->A_has_tb = true;
@@ -1027,7 +1084,7 @@ extern "C" int zj_adjoint_get(zj_handle h, double* ubar_aos) {
 
 extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream) {
     if (!h) return ZJ_ERR_ARG;
-    if (h->textbook) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint is built for the reference arithmetic");
+    if (h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint of the Navier-Stokes terms is not built");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
     if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
@@ -1056,7 +1113,7 @@ extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* str
 
 extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream) {
     if (!h || !d_w_aos || !d_ubar_aos) return ZJ_ERR_ARG;
-    if (h->textbook) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint is built for the reference arithmetic");
+    if (h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint of the Navier-Stokes terms is not built");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual VJP requested in the middle of a step");
     int rc = ensure_adjoint(h);
@@ -1073,13 +1130,13 @@ extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double
         VjpArgs<double> a = vjp_args<double>(h);
         a.U_in = h->U; a.T_in = (const double*)h->T[0];
         a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (double*)tb;
-        e = launch_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->affine, s);
+        e = launch_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->affine, s, h->textbook);
         if (e == hipSuccess) e = complete_into<double>(h, h->Rres, tb, h->W, h->ops64, s);
     } else {
         VjpArgs<float> a = vjp_args<float>(h);
         a.U_in = h->U; a.T_in = (const float*)h->T[0];
         a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (float*)tb;
-        e = launch_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->affine, s);
+        e = launch_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->affine, s, h->textbook);
         if (e == hipSuccess) e = complete_into<float>(h, h->Rres, tb, h->W, h->ops32, s);
     }
     ZJ_HIP(h, e);
@@ -1119,6 +1176,12 @@ struct zj_batch {
     };
     std::vector<Group> groups;
     std::vector<std::vector<int>> group_members;  // member indices of each group
+    // single launch per stage over every member (all orders and metric forms)
+    bool single = false;
+    void* all_insts[4] = {nullptr, nullptr, nullptr, nullptr};  // BatchInst<R>[members] per mode
+    int2* all_items = nullptr;  // (member, tile), larger members first
+    int n_all = 0;
+    int all_lds = 0;            // LDS bytes of the largest order present
     std::vector<unsigned> gens;                    // member generations the uploaded arguments match
     double* d_dt = nullptr;
     std::vector<double> dt_on_device;
@@ -1142,6 +1205,9 @@ static void batch_release(zj_batch* b) {
         if (g.join) hipEventDestroy(g.join);
         if (g.branch) hipStreamDestroy(g.branch);
     }
+    for (void* p : b->all_insts)
+        if (p) hipFree(p);
+    if (b->all_items) hipFree(b->all_items);
     if (b->d_dt) hipFree(b->d_dt);
     if (b->fork) hipEventDestroy(b->fork);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -1157,7 +1223,7 @@ static void batch_release(zj_batch* b) {
     } while (0)
 
 template <typename R>
-static int batch_upload_group(zj_batch* b, zj_batch::Group& g, const std::vector<int>& members) {
+static int batch_upload_insts(zj_batch* b, void** insts, const std::vector<int>& members) {
     for (int mode = MODE_STAGE1; mode <= MODE_STAGE3; ++mode) {
         std::vector<BatchInst<R>> v;
         for (int idx : members) {
@@ -1171,10 +1237,17 @@ static int batch_upload_group(zj_batch* b, zj_batch::Group& g, const std::vector
             in.affine = h->affine ? 1 : 0;
             v.push_back(in);
         }
-        if (!g.insts[mode]) ZJ_BHIP(b, hipMalloc(&g.insts[mode], v.size() * sizeof(BatchInst<R>)));
-        ZJ_BHIP(b, hipMemcpy(g.insts[mode], v.data(), v.size() * sizeof(BatchInst<R>), hipMemcpyHostToDevice));
+        if (!insts[mode]) ZJ_BHIP(b, hipMalloc(&insts[mode], v.size() * sizeof(BatchInst<R>)));
+        ZJ_BHIP(b, hipMemcpy(insts[mode], v.data(), v.size() * sizeof(BatchInst<R>), hipMemcpyHostToDevice));
     }
     return ZJ_OK;
+}
+
+static int batch_upload_all(zj_batch* b) {
+    std::vector<int> all(b->m.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    return b->prec == ZJ_PREC_FP64 ? batch_upload_insts<double>(b, b->all_insts, all)
+                                   : batch_upload_insts<float>(b, b->all_insts, all);
 }
 
 static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
@@ -1230,10 +1303,27 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
             ZJ_BHIP(b, hipStreamCreateWithFlags(&gg.branch, hipStreamNonBlocking));
             ZJ_BHIP(b, hipEventCreateWithFlags(&gg.join, hipEventDisableTiming));
             b->group_members.push_back(members);
-            int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_group<double>(b, gg, members)
-                                             : batch_upload_group<float>(b, gg, members);
+            int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_insts<double>(b, gg.insts, members)
+                                             : batch_upload_insts<float>(b, gg.insts, members);
             if (rc) return rc;
         }
+    // the single-launch form: one list of every member's tiles, larger members first
+    {
+        std::vector<int> order(n);
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hs[x]->E > hs[y]->E; });
+        std::vector<int2> items;
+        for (int i : order) {
+            for (int t = 0; t < hs[i]->nblk; ++t) items.push_back(make_int2(i, t));
+            b->all_lds = std::max(b->all_lds, batched_all_lds(hs[i]->P, b->prec == ZJ_PREC_FP32BULK));
+        }
+        b->n_all = (int)items.size();
+        ZJ_BHIP(b, hipMalloc((void**)&b->all_items, std::max<size_t>(items.size(), 1) * sizeof(int2)));
+        if (!items.empty())
+            ZJ_BHIP(b, hipMemcpy(b->all_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+        int rc = batch_upload_all(b);
+        if (rc) return rc;
+    }
     for (int i = 0; i < n; ++i) b->gens.push_back(hs[i]->gen);
     return ZJ_OK;
 }
@@ -1246,10 +1336,11 @@ static int batch_refresh(zj_batch* b) {
     if (!moved) return ZJ_OK;
     ZJ_BHIP(b, hipStreamSynchronize(b->stream));
     for (size_t k = 0; k < b->groups.size(); ++k) {
-        int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_group<double>(b, b->groups[k], b->group_members[k])
-                                         : batch_upload_group<float>(b, b->groups[k], b->group_members[k]);
+        int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_insts<double>(b, b->groups[k].insts, b->group_members[k])
+                                         : batch_upload_insts<float>(b, b->groups[k].insts, b->group_members[k]);
         if (rc) return rc;
     }
+    if (int rc = batch_upload_all(b)) return rc;
     for (size_t i = 0; i < b->m.size(); ++i) b->gens[i] = b->m[i]->gen;
     if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -1260,13 +1351,24 @@ static int batch_grid(const zj_batch* b, const zj_batch::Group& g) {
     return b->policy <= 0 ? g.n_items : std::min(g.n_items, b->policy * b->n_cu);
 }
 
-static int batch_capture(zj_batch* b) {
-    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
-    ZJ_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+// The three stages of a batched step on `s`: one launch per stage over every
+// member (single), or one per (order, metric form) group on parallel branches.
+static hipError_t batch_enqueue(zj_batch* b, hipStream_t s) {
     hipError_t e = hipSuccess;
+    if (b->single) {
+        const int grid = b->policy <= 0 ? b->n_all : std::min(b->n_all, b->policy * b->n_cu);
+        for (int mode = MODE_STAGE1; mode <= MODE_STAGE3 && e == hipSuccess; ++mode) {
+            if (b->prec == ZJ_PREC_FP64)
+                e = launch_batched_all<double>(mode, (const BatchInst<double>*)b->all_insts[mode], b->all_items,
+                                               b->n_all, grid, b->all_lds, s);
+            else
+                e = launch_batched_all<float>(mode, (const BatchInst<float>*)b->all_insts[mode], b->all_items,
+                                              b->n_all, grid, b->all_lds, s);
+        }
+        return e;
+    }
     for (int mode = MODE_STAGE1; mode <= MODE_STAGE3 && e == hipSuccess; ++mode) {
-        e = hipEventRecord(b->fork, b->stream);
+        e = hipEventRecord(b->fork, s);
         for (auto& g : b->groups) {
             if (e == hipSuccess) e = hipStreamWaitEvent(g.branch, b->fork, 0);
             if (e == hipSuccess) {
@@ -1278,9 +1380,17 @@ static int batch_capture(zj_batch* b) {
                                                     g.items, g.n_items, batch_grid(b, g), g.branch);
             }
             if (e == hipSuccess) e = hipEventRecord(g.join, g.branch);
-            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, g.join, 0);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, g.join, 0);
         }
     }
+    return e;
+}
+
+static int batch_capture(zj_batch* b) {
+    if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    ZJ_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = batch_enqueue(b, b->stream);
     hipGraph_t gr = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -1343,6 +1453,16 @@ extern "C" int zj_batch_set_grid(zj_batch* b, int policy) {
     return ZJ_OK;
 }
 
+extern "C" int zj_batch_set_launch(zj_batch* b, int single) {
+    if (!b || (single != 0 && single != 1)) return ZJ_ERR_ARG;
+    if ((single != 0) != b->single) {
+        b->single = single != 0;
+        if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    }
+    return ZJ_OK;
+}
+
 extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
     if (!b || !dts) return ZJ_ERR_ARG;
     for (zj_solver* h : b->m) {
@@ -1369,6 +1489,83 @@ extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
     ZJ_BHIP(b, hipGraphLaunch(b->gexec, s));
     for (zj_solver* h : b->m) h->l2_blocks = h->nblk;
     return ZJ_OK;
+}
+
+// Times `reps` batched steps (after one untimed step) for each launch form
+// (grouped branches, single launch) and grid policy (one workgroup per tile;
+// 2 or 4 per CU, persistent), keeps the fastest, and puts every member's state
+// back as it was: the members' states are saved on the device first and
+// restored, with their traces, at the end.
+extern "C" int zj_batch_autotune(zj_batch* b, const double* dts, int reps, int* single, int* policy, double* ms) {
+    if (!b || !dts || reps < 1) return ZJ_ERR_ARG;
+    for (zj_solver* h : b->m)
+        if (!h->has_state || h->stage_next != 1) {
+            b->err = "every member needs a state and no partial step";
+            return ZJ_ERR_STATE;
+        }
+    ZJ_BHIP(b, hipSetDevice(b->device));
+    if (int rc = batch_refresh(b)) return rc;
+    size_t total = 0;
+    for (zj_solver* h : b->m) total += (size_t)4 * h->E * h->S;
+    double* save = nullptr;
+    ZJ_BHIP(b, hipMalloc((void**)&save, std::max<size_t>(total, 1) * sizeof(double)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ZJ_OK;
+    auto run = [&]() -> int {
+        size_t off = 0;
+        for (zj_solver* h : b->m) {
+            const size_t n = (size_t)4 * h->E * h->S;
+            ZJ_BHIP(b, hipMemcpyAsync(save + off, h->U, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream));
+            off += n;
+        }
+        ZJ_BHIP(b, hipEventCreate(&e0));
+        ZJ_BHIP(b, hipEventCreate(&e1));
+        const int cand_single[2] = {0, 1};
+        const int cand_policy[3] = {0, 2, 4};
+        double best = 1e300;
+        int bs = b->single ? 1 : 0, bp = b->policy;
+        for (int cs : cand_single)
+            for (int cp : cand_policy) {
+                if (int r = zj_batch_set_launch(b, cs)) return r;
+                if (int r = zj_batch_set_grid(b, cp)) return r;
+                if (int r = zj_batch_rk3_step(b, dts, b->stream)) return r;  // captures and warms up
+                ZJ_BHIP(b, hipEventRecord(e0, b->stream));
+                for (int k = 0; k < reps; ++k)
+                    if (int r = zj_batch_rk3_step(b, dts, b->stream)) return r;
+                ZJ_BHIP(b, hipEventRecord(e1, b->stream));
+                ZJ_BHIP(b, hipEventSynchronize(e1));
+                float t = 0.f;
+                ZJ_BHIP(b, hipEventElapsedTime(&t, e0, e1));
+                const double per = t / reps;
+                if (per < best) { best = per; bs = cs; bp = cp; }
+            }
+        if (int r = zj_batch_set_launch(b, bs)) return r;
+        if (int r = zj_batch_set_grid(b, bp)) return r;
+        if (single) *single = bs;
+        if (policy) *policy = bp;
+        if (ms) *ms = best;
+        return ZJ_OK;
+    };
+    rc = run();
+    // restore every member's state and the traces its next step reads
+    size_t off = 0;
+    for (zj_solver* h : b->m) {
+        const size_t n = (size_t)4 * h->E * h->S;
+        if (hipMemcpyAsync(h->U, save + off, n * sizeof(double), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+            refresh_trace(h, b->stream) != ZJ_OK) {
+            if (!rc) { b->err = "restoring the member states failed"; rc = ZJ_ERR_HIP; }
+        }
+        h->l2_blocks = 0;
+        off += n;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) {
+        b->err = "restoring the member states failed";
+        rc = ZJ_ERR_HIP;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(save);
+    return rc;
 }
 
 extern "C" int zj_batch_synchronize(zj_batch* b) {

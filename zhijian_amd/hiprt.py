@@ -30,6 +30,10 @@ def hip() -> C.CDLL:
         H.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         H.hipDeviceSynchronize.argtypes = []
         H.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
     return _hip
 
 
@@ -90,3 +94,55 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = hip().hipGetDeviceCount(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+_H2D, _D2H = 1, 2
+
+
+class DeviceArray:
+    """A device buffer of `n` items of a numpy dtype (hipMalloc / hipMemcpy),
+    filled from a host array or zeroed."""
+
+    def __init__(self, n_or_host, dtype=None):
+        import numpy as np
+        if isinstance(n_or_host, (int, np.integer)):
+            self.dtype = np.dtype(dtype or np.float64)
+            self.n = int(n_or_host)
+            host = None
+        else:
+            host = np.ascontiguousarray(n_or_host, dtype=dtype or n_or_host.dtype)
+            self.dtype = host.dtype
+            self.n = host.size
+        self.nbytes = max(self.n * self.dtype.itemsize, 8)
+        self.h = C.c_void_p()
+        _ok(hip().hipMalloc(C.byref(self.h), self.nbytes), "hipMalloc")
+        if host is not None and host.size:
+            _ok(hip().hipMemcpy(self.h, host.ctypes.data, host.nbytes, _H2D), "hipMemcpy")
+        elif host is None:
+            _ok(hip().hipMemset(self.h, 0, self.nbytes), "hipMemset")
+
+    @property
+    def ptr(self):
+        return self.h.value
+
+    def upload(self, host):
+        import numpy as np
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.size == self.n
+        _ok(hip().hipMemcpy(self.h, host.ctypes.data, host.nbytes, _H2D), "hipMemcpy")
+
+    def get(self):
+        import numpy as np
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+            _ok(hip().hipMemcpy(out.ctypes.data, self.h, out.nbytes, _D2H), "hipMemcpy")
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                hip().hipFree(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass

@@ -196,6 +196,16 @@ class Solver:
     def get_state_device(self, ptr, stream=None):
         self._chk(lib().zj_get_state_device(self._h, ptr, stream))
 
+    def state_slots(self, n):
+        """Allocates n device-resident state slots (zj_state_slots)."""
+        self._chk(lib().zj_state_slots(self._h, int(n)))
+
+    def state_save(self, slot, stream=None):
+        self._chk(lib().zj_state_save(self._h, int(slot), stream))
+
+    def state_load(self, slot, stream=None):
+        self._chk(lib().zj_state_load(self._h, int(slot), stream))
+
     def adjoint_set_device(self, ptr, stream=None):
         self._chk(lib().zj_adjoint_set_device(self._h, ptr, stream))
 
@@ -346,6 +356,20 @@ class Batch:
 
     def set_grid(self, policy):
         self._chk(lib().zj_batch_set_grid(self._h, int(policy)))
+
+    def set_launch(self, single):
+        """True: one launch per stage over every member; False: one per (order, metric) group."""
+        self._chk(lib().zj_batch_set_launch(self._h, int(bool(single))))
+
+    def autotune(self, dts, reps=5):
+        """Times each launch form and grid policy, keeps the fastest and leaves
+        every member's state unchanged; returns (single, policy, ms per step)."""
+        d = np.ascontiguousarray(dts, np.float64).reshape(-1)
+        if d.size != len(self.members):
+            raise ValueError(f"{d.size} time steps for {len(self.members)} members")
+        sg, pol, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(lib().zj_batch_autotune(self._h, d, int(reps), C.byref(sg), C.byref(pol), C.byref(ms)))
+        return bool(sg.value), pol.value, ms.value
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
