@@ -519,6 +519,16 @@ int zj_gpu_compute_time_step(const double* U, const double* J, const double* h_m
 /* gpu::computeResidualNorm (kernels.cu:1476-1524): sqrt of the sum of squares
  * of the n_elem * n_sp * 4 values, summed in a fixed order. */
 int zj_gpu_residual_norm(const double* R, int n_elem, int n_sp, double* norm, void* stream);
+/* zj_gpu_residual_norm without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a static device array of 256 KiB per GPU, held until
+ * the process ends; at most 67,108,864 values (n_elem * n_sp * 4), larger
+ * inputs return ZJ_ERR_ARG.  The result equals zj_gpu_residual_norm's bit for
+ * bit (same kernels, same summation order).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (different streams or threads)
+ * race.  Blocks like zj_gpu_residual_norm. */
+int zj_gpu_residual_norm_noalloc(const double* R, int n_elem, int n_sp, double* norm, void* stream);
 /* gpu::setInitialCondition, copySolution, scaleArray, axpy (kernels.cu:1526-1582). */
 int zj_gpu_set_initial_condition(double* U, double rho, double u, double v, double p, double gamma, int n_elem,
                                  int n_sp, void* stream);

@@ -97,8 +97,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_gpu_inviscid_flux_at_fp", "zj_gpu_flux_difference", "zj_gpu_flux_diff_for_fr",
            "zj_gpu_flux_divergence", "zj_gpu_scatter_flux_to_elements", "zj_gpu_apply_fr_correction",
            "zj_gpu_apply_boundary_conditions", "zj_gpu_enforce_positivity", "zj_gpu_compute_time_step",
-           "zj_gpu_residual_norm", "zj_gpu_set_initial_condition", "zj_gpu_copy_solution", "zj_gpu_scale_array",
-           "zj_gpu_axpy")
+           "zj_gpu_residual_norm", "zj_gpu_residual_norm_noalloc", "zj_gpu_set_initial_condition",
+           "zj_gpu_copy_solution", "zj_gpu_scale_array", "zj_gpu_axpy")
 
 
 def lib() -> C.CDLL:

@@ -30,6 +30,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -611,6 +612,40 @@ int residual_norm_on(const double* R, int64_t n, double* work, int nchunk, doubl
     return ZJ_OK;
 }
 
+// ---- allocation-free residual norm
+// The partial sums live in a file-scope device array instead of a buffer the
+// call allocates.  In host code `scratch` names a host-side stand-in, so the
+// device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that address is passed to the kernels.
+constexpr int NOALLOC_MAX_CHUNKS = 32768;  // 256 KiB of partials: up to 32768 * RED_CHUNK values
+constexpr int MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_MAX_CHUNKS + 1];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_of_current_device(double** out, const char* fn) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess && (dev < 0 || dev >= MAX_DEVICES)) e = hipErrorInvalidDevice;
+    if (e != hipSuccess) {
+        set_thread_error(std::string(fn) + ": " + hipGetErrorString(e));
+        return ZJ_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_thread_error(std::string(fn) + ": hipGetSymbolAddress: " + hipGetErrorString(e));
+            return ZJ_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ZJ_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -884,6 +919,23 @@ extern "C" int zj_gpu_residual_norm(const double* R, int n_elem, int n_sp, doubl
     const int rc = residual_norm_on(R, n, work, nchunk, norm, (hipStream_t)stream, fn);
     (void)hipFree(work);
     return rc;
+}
+
+extern "C" int zj_gpu_residual_norm_noalloc(const double* R, int n_elem, int n_sp, double* norm, void* stream) {
+    static const char* fn = "zj_gpu_residual_norm_noalloc";
+    if (n_elem < 0 || n_sp < 1 || !norm) return err(fn, "sizes out of range");
+    const int64_t n = (int64_t)n_elem * n_sp * 4;
+    if (n > (int64_t)NOALLOC_MAX_CHUNKS * RED_CHUNK)
+        return err(fn, "n_elem * n_sp * 4 exceeds the allocation-free maximum of 67108864 values");
+    if (n > 0 && !R) return err(fn, "null array");
+    if (n == 0) {
+        *norm = 0.0;
+        return ZJ_OK;
+    }
+    double* work = nullptr;
+    if (int rc = scratch_of_current_device(&work, fn)) return rc;
+    const int nchunk = (int)((n + RED_CHUNK - 1) / RED_CHUNK);
+    return residual_norm_on(R, n, work, nchunk, norm, (hipStream_t)stream, fn);
 }
 
 extern "C" int zj_gpu_set_initial_condition(double* U, double rho, double u, double v, double p, double gamma,

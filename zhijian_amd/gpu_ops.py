@@ -34,6 +34,7 @@ _SIG = {
     "zj_gpu_enforce_positivity": [P, I, P],
     "zj_gpu_compute_time_step": [P, P, P, D, D, I, I, C.POINTER(D), P],
     "zj_gpu_residual_norm": [P, I, I, C.POINTER(D), P],
+    "zj_gpu_residual_norm_noalloc": [P, I, I, C.POINTER(D), P],
     "zj_gpu_set_initial_condition": [P, D, D, D, D, D, I, I, P],
     "zj_gpu_copy_solution": [P, P, I, P],
     "zj_gpu_scale_array": [P, P, D, I, P],
@@ -147,6 +148,13 @@ def compute_time_step(U, J, h_min, cfl, gamma, n_elem, n_sp, stream=None):
 def compute_residual_norm(R, n_elem, n_sp, stream=None):
     out = C.c_double()
     _call("zj_gpu_residual_norm", R, n_elem, n_sp, C.byref(out), stream)
+    return out.value
+
+
+def compute_residual_norm_noalloc(R, n_elem, n_sp, stream=None):
+    """zj_gpu_residual_norm_noalloc: the same norm with no device allocation."""
+    out = C.c_double()
+    _call("zj_gpu_residual_norm_noalloc", R, n_elem, n_sp, C.byref(out), stream)
     return out.value
 
 
