@@ -351,6 +351,24 @@ int zj_tri_operators(int p, double* sp, double* fp, double* wg, double* Dr, doub
  * energy norm; equal to zj_tri_operators' lift for ZJ_CORR_DG. */
 int zj_tri_lift(int p, int corr, double* lift);
 
+/* Discrete adjoint on triangles (no reference counterpart; the quadrilateral
+ * zj_adjoint_* protocol): vector-Jacobian products of the triangle stage --
+ * the transposed lifting, the configured Riemann solver's and the ghost
+ * states' VJPs (euler.hpp:204-375, kernels.cu:453-525), the transposed
+ * divergence, interpolation and Euler flux Jacobian, SSP-RK3
+ * (rk_integrator.hpp:33-58).  Reverse stepping: zj_tri_adjoint_set(dJ/dU^N);
+ * for n = N-1 .. 0: zj_tri_set_state(U^n), zj_tri_rk3_step_vjp(dt_n)
+ * (stages recomputed from U^n; dt_bar, may be NULL, receives dJ/d(dt_n));
+ * then zj_tri_adjoint_get.  Arrays are [E][Np][4]. */
+int zj_tri_adjoint_set(zj_tri_handle h, const double* ubar_aos);
+int zj_tri_adjoint_set_device(zj_tri_handle h, const double* d_ubar_aos, void* stream);
+int zj_tri_adjoint_get(zj_tri_handle h, double* ubar_aos);
+int zj_tri_adjoint_get_device(zj_tri_handle h, double* d_ubar_aos, void* stream);
+int zj_tri_rk3_step_vjp(zj_tri_handle h, double dt, double* dt_bar, void* stream);
+/* (dR/dU)^T w at the handle's state, R = zj_tri_residual's dU/dt. */
+int zj_tri_residual_vjp(zj_tri_handle h, const double* w_aos, double* ubar_aos);
+int zj_tri_residual_vjp_device(zj_tri_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream);
+
 /* Polynomial order and gamma a handle was created with. */
 int zj_get_scheme(zj_handle h, int* poly_order, double* gamma);
 int zj_tri_get_scheme(zj_tri_handle h, int* poly_order, double* gamma);

@@ -90,6 +90,8 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_restart_save", "zj_restart_load", "zj_tri_restart_save", "zj_tri_restart_load", "zj_vtk_write",
            "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes",
            "zj_tri_lift", "zj_time_stages", "zj_tri_time_stages",
+           "zj_tri_adjoint_set", "zj_tri_adjoint_set_device", "zj_tri_adjoint_get", "zj_tri_adjoint_get_device",
+           "zj_tri_rk3_step_vjp", "zj_tri_residual_vjp", "zj_tri_residual_vjp_device",
            # the gpu:: free functions (zhijian_amd.gpu_ops binds them)
            "zj_gpu_set_operator_engine", "zj_gpu_interpolate_to_flux_points", "zj_gpu_reference_gradients",
            "zj_gpu_transform_gradients", "zj_gpu_inviscid_flux_sp", "zj_gpu_riemann_flux",
@@ -193,6 +195,13 @@ def lib() -> C.CDLL:
         L.zj_tri_set_xcd_groups.argtypes = [H, C.c_int]
         L.zj_tri_get_info.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int64)]
+        L.zj_tri_adjoint_set.argtypes = [H, _dp]
+        L.zj_tri_adjoint_get.argtypes = [H, _dp]
+        L.zj_tri_adjoint_set_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_adjoint_get_device.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.zj_tri_rk3_step_vjp.argtypes = [H, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+        L.zj_tri_residual_vjp.argtypes = [H, _dp, _dp]
+        L.zj_tri_residual_vjp_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
         L.zj_tri_operators.argtypes = [C.c_int] + [C.c_void_p] * 7
         L.zj_tri_lift.argtypes = [C.c_int, C.c_int, C.c_void_p]
         L.zj_get_scheme.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_double)]

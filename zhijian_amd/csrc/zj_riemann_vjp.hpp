@@ -7,6 +7,8 @@
 // forms them (floors, guarded denominators, Harten's entropy fix) and runs
 // them backwards; every max, floor, guard and branch takes the side the
 // forward took, so the result is the derivative of the executed arithmetic.
+// The Euler flux and ghost-state VJPs shared by the quadrilateral and
+// triangle adjoints sit here too.
 #pragma once
 
 #include "zj_device.hpp"
@@ -310,6 +312,89 @@ __device__ __forceinline__ void riemann_hllc_vjp(const R* UL, const R* UR, R nx,
     br.u += vnRb * nx; br.v += vnRb * ny;
     side_vjp(UL, L, g, bl, ULb);
     side_vjp(UR, Q, g, br, URb);
+}
+
+// ------------------------------------------------------------ physics VJPs
+// Ub += (dFx/dU)^T Fxb + (dFy/dU)^T Fyb for IdealGas::fluxX/fluxY (euler.hpp:101-129).
+template <typename R>
+__device__ __forceinline__ void euler_flux_vjp(const R* U, R g, const R* a, const R* b, R* Ub) {
+    const R inv = rcpR(U[0]), gm = g - R(1);
+    const R u = U[1] * inv, v = U[2] * inv;
+    const R p = gm * (U[3] - R(0.5) * (U[1] * u + U[2] * v));
+    const R H = U[3] + p;
+    R ub = a[1] * U[1] + a[3] * H + b[1] * U[2];
+    R vb = a[2] * U[1] + b[2] * U[2] + b[3] * H;
+    const R pb = a[1] + a[3] * u + b[2] + b[3] * v;
+    R U1b = a[0] + a[1] * u + a[2] * v - pb * gm * u;
+    R U2b = b[0] + b[1] * u + b[2] * v - pb * gm * v;
+    const R U3b = a[3] * u + b[3] * v + pb * gm;
+    U1b += ub * inv;
+    U2b += vb * inv;
+    const R U0b = -(ub * u + vb * v) * inv + pb * gm * R(0.5) * (u * u + v * v);
+    Ub[0] += U0b; Ub[1] += U1b; Ub[2] += U2b; Ub[3] += U3b;
+}
+
+// VJP of computeGhostState (kernels.cu:453-525): Ub += (dG/dUi)^T Gb.
+template <typename R>
+__device__ void ghost_state_vjp(const R* Ui, R nx, R ny, int bct, const double* bc, R g, const R* Gb, R* Ub) {
+    const R gm = g - R(1);
+    if (bct == 1 || bct == 2 || bct == 3) {
+        const R rho = Ui[0], u = Ui[1] / rho, v = Ui[2] / rho;
+        const R vn = u * nx + v * ny;
+        const R ur = u - R(2) * vn * nx, vr = v - R(2) * vn * ny;
+        R rb = Gb[0] + Gb[1] * ur + Gb[2] * vr + Gb[3] * R(0.5) * (ur * ur + vr * vr);
+        const R urb = Gb[1] * rho + Gb[3] * rho * ur, vrb = Gb[2] * rho + Gb[3] * rho * vr;
+        const R pb = Gb[3] / gm;
+        const R vnb = -R(2) * (urb * nx + vrb * ny);
+        R ub = urb + vnb * nx, vb = vrb + vnb * ny;
+        // p = (g-1) (U3 - rho (u^2+v^2)/2)
+        Ub[3] += pb * gm;
+        rb -= pb * gm * R(0.5) * (u * u + v * v);
+        ub -= pb * gm * rho * u;
+        vb -= pb * gm * rho * v;
+        Ub[1] += ub / rho; Ub[2] += vb / rho;
+        rb -= (ub * u + vb * v) / rho;
+        Ub[0] += rb;
+    } else if (bct == 4) {
+        const R rinf = R(bc[0]), uinf = R(bc[1]), vinf = R(bc[2]), pinf = R(bc[3]);
+        const bool rok = Ui[0] >= R(1e-10);
+        const R ri = fmaxR(Ui[0], R(1e-10));
+        const R ui = Ui[1] / ri, vi = Ui[2] / ri;
+        const R praw = gm * (Ui[3] - R(0.5) * ri * (ui * ui + vi * vi));
+        const bool pok = praw >= R(1e-10);
+        const R pi = fmaxR(praw, R(1e-10));
+        const bool inflow = uinf * nx + vinf * ny > R(0);
+        const R rb0 = inflow ? rinf : ri, ub0 = inflow ? uinf : ui, vb0 = inflow ? vinf : vi;
+        const R pb0 = R(0.5) * (pinf + pi);
+        const R rb = fmaxR(fminR(rb0, R(100)), R(0.01));
+        const R ub = fmaxR(fminR(ub0, R(1000)), R(-1000));
+        const R vb = fmaxR(fminR(vb0, R(1000)), R(-1000));
+        // G = prim2cons(rb, ub, vb, pb)
+        R rbb = Gb[0] + Gb[1] * ub + Gb[2] * vb + Gb[3] * R(0.5) * (ub * ub + vb * vb);
+        R ubb = Gb[1] * rb + Gb[3] * rb * ub, vbb = Gb[2] * rb + Gb[3] * rb * vb;
+        R pbb = Gb[3] / gm;
+        if (!(rb0 >= R(0.01) && rb0 <= R(100))) rbb = R(0);
+        if (!(ub0 >= R(-1000) && ub0 <= R(1000))) ubb = R(0);
+        if (!(vb0 >= R(-1000) && vb0 <= R(1000))) vbb = R(0);
+        if (!(pb0 >= R(1e3) && pb0 <= R(1e7))) pbb = R(0);
+        // p_i = (g-1)(U3 - ri (ui^2 + vi^2)/2) in the variables (ri, ui, vi, U3)
+        const R pib = pok ? R(0.5) * pbb : R(0);
+        R rib = -pib * gm * R(0.5) * (ui * ui + vi * vi);
+        Ub[3] += pib * gm;
+        R uib = -pib * gm * ri * ui, vib = -pib * gm * ri * vi;
+        if (!inflow) { rib += rbb; uib += ubb; vib += vbb; }
+        Ub[1] += uib / ri; Ub[2] += vib / ri;
+        rib -= (uib * ui + vib * vi) / ri;
+        if (rok) Ub[0] += rib;
+    } else if (bct == 6) {
+        const R rho = Ui[0], u = Ui[1] / rho, v = Ui[2] / rho;
+        Ub[0] += Gb[0] - Gb[3] * R(0.5) * (u * u + v * v);
+        Ub[1] += Gb[1] + Gb[3] * u;
+        Ub[2] += Gb[2] + Gb[3] * v;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) Ub[k] += Gb[k];
+    }
 }
 
 template <typename R>

@@ -44,6 +44,7 @@
 #include "zj_device.hpp"
 #include "zj_host_math.hpp"
 #include "zj_kernels.hpp"
+#include "zj_riemann_vjp.hpp"
 #include "zj_tri_ops.hpp"
 
 #include <hip/hip_runtime.h>
@@ -376,6 +377,403 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
     }
 }
 
+// ------------------------------------------------------------ adjoint
+// Vector-Jacobian product of one triangle stage u_n = a U^n + b q + c dt R(q)
+// (MODE_STAGE1..3), or of the residual alone (MODE_RESIDUAL).  The forward
+// values are recomputed as tri_stage_kernel forms them; then, per lane
+// (element, variable):
+//   R1  cotangent of u_out: its partial value plus the transposed trace
+//       operator on the edge cotangents other tiles left (cross-tile edges);
+//       the RK combination's cotangents of U^n, q and dt
+//   R2  the lifting transposed: the flux differences' cotangents (LDS)
+//   R3  per solved face point (as the forward solves them): the common flux's
+//       cotangent from both sides' differences, the Riemann solver's VJP, to
+//       this side's trace cotangent (LDS), an in-tile neighbour's (LDS) or a
+//       cross-tile neighbour's (HBM, at its slot) or through the ghost state
+//   R4  per solution point: Dr, Ds and the interpolated normal flux
+//       transposed, the metric, the Euler flux Jacobian (the four variables'
+//       cotangents gathered across the element's DPP quad), and the
+//       transposed trace operator
+template <typename R>
+struct TriVjpArgs {
+    int E;
+    const double* U_in;     // q
+    const double* U0;       // U^n (stages 2, 3)
+    const R* geo;
+    const int2* links;
+    const R* T_in;          // traces of q on edges read across tiles
+    const double* bc_data;
+    const double* dt_ptr;
+    const double* Ub_in;    // partial cotangent of u_out, or of R
+    const R* Tb_in;         // cross-tile edge cotangents completing Ub_in (nullptr: none)
+    double* Ub_out;         // partial cotangent of q (may alias Ub_in)
+    R* Tb_out;              // cross-tile edge cotangents of q, at the neighbour's slots
+    double* U0b;            // cotangent of U^n: stage 3 writes, stage 2 adds, stage 1 reads
+    double* dtb_partial;    // per-tile sum of c R . du_n (nullptr: skip)
+    double gamma;
+    int riemann;
+    int xcd_groups;
+    int lift_off;
+};
+
+template <typename R, int P, int MODE>
+__global__ __launch_bounds__(BLOCK) void tri_vjp_kernel(TriVjpArgs<R> a) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int NW = NP > NFP ? NP : NFP;
+    using OP = TriOps<P>;
+    constexpr int O0 = ops_offset(P), CA = OP::CA, OI = O0 + OP::C;
+    constexpr double ca = MODE == MODE_STAGE2 ? 0.75 : (MODE == MODE_STAGE3 ? 1.0 / 3.0 : 0.0);
+    constexpr double cb = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    constexpr double cc = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    const int OLT = O0 + a.lift_off;
+    __shared__ R sT[TILE * NFP * NV];   // own traces
+    __shared__ R sW[TILE * NW * NV];    // q, then Fn, then the flux differences d, then their cotangents
+    __shared__ R sTb[TILE * NFP * NV];  // trace cotangents
+    __shared__ double sred[BLOCK / 64];
+    __shared__ R sEdge[TILE * 3];
+    __shared__ unsigned char sList[TILE * 3];
+    __shared__ int sCnt[BLOCK / 64];
+
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
+    const int tile = tile_of_block(a.xcd_groups);
+    const int e0 = tile * TILE, e = e0 + el;
+    const bool live = e < a.E;
+    const R g = R(a.gamma);
+    const int64_t base = blk(e, 0, NP) + v;
+
+    double q[NP];
+    R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
+    int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) q[m] = a.U_in[base + (int64_t)m * EB * NV];
+        const R* G = a.geo + (int64_t)e * GEO;
+        rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) sW[(el * NP + m) * NV + v] = R(q[m]);
+        if (v < 3) {
+            const R ax = v == 0 ? sx : (v == 1 ? rx + sx : rx), ay = v == 0 ? sy : (v == 1 ? ry + sy : ry);
+            sEdge[el * 3 + v] = sqrtR(ax * ax + ay * ay);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) q[m] = 0.0;
+    }
+    bool owned = false;
+    if (live && v < 3) {
+        const int2 Lv = v == 0 ? L[0] : (v == 1 ? L[1] : L[2]);
+        const int nl = Lv.x - e0;
+        owned = !(link_role(Lv.y) == ROLE_RIGHT && nl >= 0 && nl < TILE);
+    }
+    const unsigned long long own_mask = __ballot(owned);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) sCnt[wave] = __popcll(own_mask);
+    __syncthreads();
+
+    // ---- forward 1: divergence, interpolated normal fluxes, traces
+    R div[NP], fn[NFP];
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) div[i] = R(0);
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) fn[j] = R(0);
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R um = sW[(el * NP + m) * NV + v];
+            const R q0 = quad_get<0>(um), q1 = quad_get<1>(um), q2 = quad_get<2>(um), q3 = quad_get<3>(um);
+            const R inv = fast_rcp(q0);
+            const R ux = q1 * inv, uy = q2 * inv;
+            const R p = (g - R(1)) * (q3 - R(0.5) * (q1 * ux + q2 * uy));
+            const R fx = v == 0 ? q1 : (v == 1 ? q1 * ux + p : (v == 2 ? q1 * uy : (q3 + p) * ux));
+            const R fy = v == 0 ? q2 : (v == 1 ? q2 * ux : (v == 2 ? q2 * uy + p : (q3 + p) * uy));
+            const R fr = rx * fx + ry * fy, fs = sx * fx + sy * fy, frs = fr + fs;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                div[i] = fmaR(cop<R>(O0 + m * CA + NP + i), fs, fmaR(cop<R>(O0 + m * CA + i), fr, div[i]));
+#pragma unroll
+            for (int j = 0; j < NFP; ++j)
+                fn[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+        }
+        R ut[NFP];
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) ut[j] = R(0);
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R um = sW[(el * NP + m) * NV + v];
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) ut[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), um, ut[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) sT[(el * NFP + j) * NV + v] = ut[j];
+    }
+    if (owned) {
+        int pos = __popcll(own_mask & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) pos += sCnt[w];
+        sList[pos] = (unsigned char)(el * 3 + v);
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) sW[(el * NFP + j) * NV + v] = (j >= FP && j < 2 * FP) ? fn[j] : -fn[j];
+    }
+    __syncthreads();
+    int nsolved = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) nsolved += sCnt[w];
+
+    // ---- forward 2: flux differences d = sc F^I - Fn of both sides of every solved face point
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += BLOCK) {
+        const int c = sList[t / FP], f = t - (t / FP) * FP;
+        const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
+        const int2 Lk = a.links[(int64_t)(e0 + elt) * 3 + kk];
+        const int role = link_role(Lk.y);
+        const R nx = a.geo[(int64_t)(e0 + elt) * GEO + 4 + 2 * kk];
+        const R ny = a.geo[(int64_t)(e0 + elt) * GEO + 5 + 2 * kk];
+        R own[NV], oth[NV], UL[NV], UR[NV], Fc[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) own[w] = sT[(elt * NFP + j) * NV + w];
+        int nl = -1, oj = 0;
+        if (role == ROLE_BOUNDARY) {
+            ghost_state(own, nx, ny, link_edge(Lk.y), a.bc_data + (int64_t)link_bc(Lk.y) * 8, g, oth);
+        } else {
+            nl = Lk.x - e0;
+            oj = link_edge(Lk.y) * FP + (P - f);
+            if (nl >= 0 && nl < TILE) {
+#pragma unroll
+                for (int w = 0; w < NV; ++w) oth[w] = sT[(nl * NFP + oj) * NV + w];
+            } else {
+                load4(a.T_in + trix(Lk.x, oj, NFP), oth);
+                nl = -1;
+            }
+        }
+        const bool right = role == ROLE_RIGHT;
+#pragma unroll
+        for (int w = 0; w < NV; ++w) { UL[w] = right ? oth[w] : own[w]; UR[w] = right ? own[w] : oth[w]; }
+        riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
+        {
+            const R sc = right ? -sEdge[c] : sEdge[c];
+            R* d = sW + (elt * NFP + j) * NV;
+#pragma unroll
+            for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
+        }
+        if (nl >= 0) {
+            const R sc = -sEdge[nl * 3 + link_edge(Lk.y)];
+            R* d = sW + (nl * NFP + oj) * NV;
+#pragma unroll
+            for (int w = 0; w < NV; ++w) d[w] = sc * Fc[w] - d[w];
+        }
+    }
+    __syncthreads();
+
+    // ---- forward 3: residual; R1: cotangent of the stage output, RK combination
+    double rsum = 0.0;
+    R divb[NP];
+    double qb[NP];
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            const R d = sW[(el * NFP + j) * NV + v];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OLT + j * NP + i), d, div[i]);
+        }
+        // incoming edge cotangents of u_out on this element's cross-tile edges
+        R tin[NFP];
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) tin[j] = R(0);
+        if (MODE != MODE_RESIDUAL && a.Tb_in) {
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const int nl = L[kk].x - e0;
+                const int role = link_role(L[kk].y);
+                if (role == ROLE_BOUNDARY || (nl >= 0 && nl < TILE)) continue;
+#pragma unroll
+                for (int f = 0; f < FP; ++f) tin[kk * FP + f] = a.Tb_in[trix(e, kk * FP + f, NFP) + v];
+            }
+        }
+        const double dt = MODE == MODE_RESIDUAL ? 0.0 : *a.dt_ptr;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int64_t gi = base + (int64_t)i * EB * NV;
+            double rb;
+            if (MODE == MODE_RESIDUAL) {
+                rb = a.Ub_in[gi];
+                qb[i] = 0.0;
+            } else {
+                R tsum = R(0);
+#pragma unroll
+                for (int j = 0; j < NFP; ++j) tsum = fmaR(cop<R>(OI + j * NP + i), tin[j], tsum);
+                const double unb = a.Ub_in[gi] + double(tsum);
+                if (MODE == MODE_STAGE3) a.U0b[gi] = ca * unb;
+                if (MODE == MODE_STAGE2) a.U0b[gi] += ca * unb;
+                qb[i] = cb * unb;
+                rb = cc * dt * unb;
+                rsum += cc * (-double(div[i])) * unb;
+            }
+            divb[i] = R(-rb);  // R = -div
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) { divb[i] = R(0); qb[i] = 0.0; }
+    }
+    if (a.dtb_partial) {
+        const double s = block_reduce<OpSum>(rsum, sred);
+        if (tid == 0) a.dtb_partial[tile] = s;
+    }
+    __syncthreads();  // every lane's reads of sW (forward 3) are done
+
+    // ---- R2: the lifting transposed, d_j's cotangent into sW; trace cotangents cleared
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            R acc = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) acc = fmaR(cop<R>(OLT + j * NP + i), divb[i], acc);
+            sW[(el * NFP + j) * NV + v] = acc;
+            sTb[(el * NFP + j) * NV + v] = R(0);
+        }
+    }
+    __syncthreads();
+
+    // ---- R3: the common flux of every solved face point, backwards
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += BLOCK) {
+        const int c = sList[t / FP], f = t - (t / FP) * FP;
+        const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
+        const int2 Lk = a.links[(int64_t)(e0 + elt) * 3 + kk];
+        const int role = link_role(Lk.y);
+        const R nx = a.geo[(int64_t)(e0 + elt) * GEO + 4 + 2 * kk];
+        const R ny = a.geo[(int64_t)(e0 + elt) * GEO + 5 + 2 * kk];
+        R own[NV], oth[NV], UL[NV], UR[NV], Fcb[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) own[w] = sT[(elt * NFP + j) * NV + w];
+        int nl = -1, oj = 0;
+        bool cross = false;
+        const double* bcd = a.bc_data + (int64_t)(role == ROLE_BOUNDARY ? link_bc(Lk.y) : 0) * 8;
+        if (role == ROLE_BOUNDARY) {
+            ghost_state(own, nx, ny, link_edge(Lk.y), bcd, g, oth);
+        } else {
+            nl = Lk.x - e0;
+            oj = link_edge(Lk.y) * FP + (P - f);
+            if (nl >= 0 && nl < TILE) {
+#pragma unroll
+                for (int w = 0; w < NV; ++w) oth[w] = sT[(nl * NFP + oj) * NV + w];
+            } else {
+                load4(a.T_in + trix(Lk.x, oj, NFP), oth);
+                nl = -1;
+                cross = true;
+            }
+        }
+        const bool right = role == ROLE_RIGHT;
+        const R sc = right ? -sEdge[c] : sEdge[c];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            UL[w] = right ? oth[w] : own[w];
+            UR[w] = right ? own[w] : oth[w];
+            Fcb[w] = sc * sW[(elt * NFP + j) * NV + w];
+        }
+        if (nl >= 0) {
+            const R scn = -sEdge[nl * 3 + link_edge(Lk.y)];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) Fcb[w] += scn * sW[(nl * NFP + oj) * NV + w];
+        }
+        R ULb[NV] = {R(0), R(0), R(0), R(0)}, URb[NV] = {R(0), R(0), R(0), R(0)};
+        riemann_by_kind_vjp(a.riemann, UL, UR, nx, ny, g, Fcb, ULb, URb);
+        R ownb[NV], othb[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            ownb[w] = right ? URb[w] : ULb[w];
+            othb[w] = right ? ULb[w] : URb[w];
+        }
+        if (role == ROLE_BOUNDARY) {
+            ghost_state_vjp(own, nx, ny, link_edge(Lk.y), bcd, g, othb, ownb);
+        } else if (nl >= 0) {
+#pragma unroll
+            for (int w = 0; w < NV; ++w) sTb[(nl * NFP + oj) * NV + w] = othb[w];
+        } else if (cross) {
+            store4(a.Tb_out + trix(Lk.x, oj, NFP), othb);
+        }
+#pragma unroll
+        for (int w = 0; w < NV; ++w) sTb[(elt * NFP + j) * NV + w] = ownb[w];
+    }
+    __syncthreads();
+
+    // ---- R4: volume terms at the solution points, transposed traces; cotangent of q
+    if (!live) return;
+#pragma unroll 1
+    for (int m = 0; m < NP; ++m) {
+        R frb = R(0), fsb = R(0);
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            frb = fmaR(cop<R>(O0 + m * CA + i), divb[i], frb);
+            fsb = fmaR(cop<R>(O0 + m * CA + NP + i), divb[i], fsb);
+        }
+        R tb = R(0);
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) {
+            const R im = cop<R>(O0 + m * CA + 2 * NP + j);
+            // Fn = -fn (edges 0, 2), +fn (edge 1); d = sc F^I - Fn; fn_j = sum_m interp[j][m] X_m
+            // with X = fs, fr + fs, fr on edges 0, 1, 2
+            const R db = sW[(el * NFP + j) * NV + v];
+            const R fnb = (j >= FP && j < 2 * FP) ? -db : db;
+            if (j >= FP) frb = fmaR(im, fnb, frb);
+            if (j < 2 * FP) fsb = fmaR(im, fnb, fsb);
+            tb = fmaR(im, sTb[(el * NFP + j) * NV + v], tb);
+        }
+        const R fxb = rx * frb + sx * fsb, fyb = ry * frb + sy * fsb;
+        const R um = R(q[m]);
+        const R Uq[NV] = {quad_get<0>(um), quad_get<1>(um), quad_get<2>(um), quad_get<3>(um)};
+        const R A[NV] = {quad_get<0>(fxb), quad_get<1>(fxb), quad_get<2>(fxb), quad_get<3>(fxb)};
+        const R B[NV] = {quad_get<0>(fyb), quad_get<1>(fyb), quad_get<2>(fyb), quad_get<3>(fyb)};
+        R Ub[NV] = {R(0), R(0), R(0), R(0)};
+        euler_flux_vjp(Uq, g, A, B, Ub);
+        const R mine = v == 0 ? Ub[0] : (v == 1 ? Ub[1] : (v == 2 ? Ub[2] : Ub[3]));
+        const int64_t gi = base + (int64_t)m * EB * NV;
+        double out = qb[m] + double(mine) + double(tb);
+        if (MODE == MODE_STAGE1) out += a.U0b[gi];
+        a.Ub_out[gi] = out;
+    }
+}
+
+// Completion of a cotangent: U_bar plus the transposed traces of the
+// cross-tile edge cotangents that the last adjoint launch left for this tile.
+template <typename R, int P>
+__global__ __launch_bounds__(BLOCK) void tri_complete_kernel(int E, const int2* __restrict__ links,
+                                                             const double* __restrict__ Ub, const R* __restrict__ Tb,
+                                                             double* __restrict__ out, int xcd_groups) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int OI = ops_offset(P) + TriOps<P>::C;
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
+    const int e0 = tile_of_block(xcd_groups) * TILE, e = e0 + el;
+    if (e >= E) return;
+    const int64_t base = blk(e, 0, NP) + v;
+    R tin[NFP];
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) tin[j] = R(0);
+    if (Tb) {
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const int2 Lk = links[(int64_t)e * 3 + kk];
+            const int nl = Lk.x - e0;
+            if (link_role(Lk.y) == ROLE_BOUNDARY || (nl >= 0 && nl < TILE)) continue;
+#pragma unroll
+            for (int f = 0; f < FP; ++f) tin[kk * FP + f] = Tb[trix(e, kk * FP + f, NFP) + v];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        R tsum = R(0);
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) tsum = fmaR(cop<R>(OI + j * NP + i), tin[j], tsum);
+        const int64_t gi = base + (int64_t)i * EB * NV;
+        out[gi] = Ub[gi] + double(tsum);
+    }
+}
+
 // Traces of U on every edge (after a new state is set).
 template <typename R, int P>
 __global__ __launch_bounds__(BLOCK) void tri_trace_kernel(int E, const double* U, R* T_out) {
@@ -460,6 +858,43 @@ hipError_t tri_launch(int P, int mode, const TriArgs<R>& a, hipStream_t s) {
     case 4: return tri_launch_p<R, 4>(mode, a, s);
     default: return tri_launch_p<R, 5>(mode, a, s);
     }
+}
+
+template <typename R, int P>
+hipError_t tri_vjp_p(int mode, const TriVjpArgs<R>& a, hipStream_t s) {
+    const int grid = (std::max(a.E, 1) + TILE - 1) / TILE;
+    switch (mode) {
+    case MODE_RESIDUAL: tri_vjp_kernel<R, P, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a); break;
+    case MODE_STAGE1: tri_vjp_kernel<R, P, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a); break;
+    case MODE_STAGE2: tri_vjp_kernel<R, P, MODE_STAGE2><<<grid, BLOCK, 0, s>>>(a); break;
+    default: tri_vjp_kernel<R, P, MODE_STAGE3><<<grid, BLOCK, 0, s>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t tri_vjp(int P, int mode, const TriVjpArgs<R>& a, hipStream_t s) {
+    switch (P) {
+    case 1: return tri_vjp_p<R, 1>(mode, a, s);
+    case 2: return tri_vjp_p<R, 2>(mode, a, s);
+    case 3: return tri_vjp_p<R, 3>(mode, a, s);
+    case 4: return tri_vjp_p<R, 4>(mode, a, s);
+    default: return tri_vjp_p<R, 5>(mode, a, s);
+    }
+}
+
+template <typename R>
+hipError_t tri_complete(int P, int E, const int2* links, const double* Ub, const R* Tb, double* out, int xcd,
+                        hipStream_t s) {
+    const int grid = (std::max(E, 1) + TILE - 1) / TILE;
+    switch (P) {
+    case 1: tri_complete_kernel<R, 1><<<grid, BLOCK, 0, s>>>(E, links, Ub, Tb, out, xcd); break;
+    case 2: tri_complete_kernel<R, 2><<<grid, BLOCK, 0, s>>>(E, links, Ub, Tb, out, xcd); break;
+    case 3: tri_complete_kernel<R, 3><<<grid, BLOCK, 0, s>>>(E, links, Ub, Tb, out, xcd); break;
+    case 4: tri_complete_kernel<R, 4><<<grid, BLOCK, 0, s>>>(E, links, Ub, Tb, out, xcd); break;
+    default: tri_complete_kernel<R, 5><<<grid, BLOCK, 0, s>>>(E, links, Ub, Tb, out, xcd); break;
+    }
+    return hipGetLastError();
 }
 
 template <typename R>
@@ -560,6 +995,20 @@ struct zj_tri {
     bool has_state = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // discrete adjoint, allocated on first use: the cotangent of the state
+    // (blocked layout), of U^n within a step, the recomputed stage states and
+    // their traces, the cross-tile edge cotangents (ping-pong, A_cur holds the
+    // pending ones), per-tile dt partials of the three stages
+    double* A_Ub = nullptr;
+    double* A_U0b = nullptr;
+    double* A_W1 = nullptr;
+    double* A_W2 = nullptr;
+    double* A_dtb = nullptr;
+    void* A_T[3] = {nullptr, nullptr, nullptr};
+    void* A_Tb[2] = {nullptr, nullptr};
+    int A_cur = 0;
+    bool A_has_tb = false;
+    bool A_seeded = false;
     std::string err;
 };
 
@@ -597,7 +1046,9 @@ static void trelease(zj_tri* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     void* bufs[] = {h->U, h->W, h->Rres, h->aos, h->geo, h->links, h->T[0], h->T[1], h->T[2],
-                    h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->perm};
+                    h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->perm,
+                    h->A_Ub, h->A_U0b, h->A_W1, h->A_W2, h->A_dtb, h->A_T[0], h->A_T[1], h->A_T[2],
+                    h->A_Tb[0], h->A_Tb[1]};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
@@ -986,6 +1437,178 @@ extern "C" int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_b
     if (elems_per_block) *elems_per_block = h->epb;
     if (n_sp) *n_sp = h->NP;
     if (cross_tile_edges) *cross_tile_edges = h->cross_edges;
+    return ZJ_OK;
+}
+
+// ------------------------------------------------------------ adjoint (host)
+static int tensure_adjoint(zj_tri* h) {
+    if (h->A_Ub) return ZJ_OK;
+    ZJ_THIP(h, hipSetDevice(h->device));
+    const size_t eb = ((size_t)h->E + EB - 1) / EB * EB;
+    const size_t nst = eb * h->NP * 4 * sizeof(double);
+    const size_t ntr = eb * h->NFP * 4 * (h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float));
+    for (double** b : {&h->A_Ub, &h->A_U0b, &h->A_W1, &h->A_W2}) ZJ_THIP(h, hipMalloc((void**)b, nst));
+    for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->A_T[k], ntr));
+    for (int k = 0; k < 2; ++k) ZJ_THIP(h, hipMalloc(&h->A_Tb[k], ntr));
+    ZJ_THIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * std::max(h->nblk, 1) * sizeof(double)));
+    return ZJ_OK;
+}
+
+template <typename R>
+static TriVjpArgs<R> tvjp_args(zj_tri* h) {
+    TriVjpArgs<R> a{};
+    a.E = h->E;
+    a.geo = (const R*)h->geo;
+    a.links = h->links;
+    a.bc_data = h->bc_data;
+    a.dt_ptr = h->d_scalars;
+    a.gamma = h->gamma;
+    a.riemann = h->riemann;
+    a.xcd_groups = h->xcd_groups;
+    a.lift_off = lift_offset(h->P, h->corr);
+    return a;
+}
+
+// One step backwards: traces of U^n, stages 1 and 2 recomputed into A_W1 and
+// A_W2 (with their cross-tile traces), then the stage VJPs 3, 2, 1, each
+// consuming the edge cotangents the previous launch left.
+template <typename R>
+static hipError_t tri_adjoint_step(zj_tri* h, bool want_dt, hipStream_t s) {
+    R** AT = (R**)h->A_T;
+    R** Tb = (R**)h->A_Tb;
+    hipError_t e = tri_trace<R>(h->P, h->E, h->U, AT[0], s);
+    if (e != hipSuccess) return e;
+    TriArgs<R> f = tri_args<R>(h, MODE_STAGE1);
+    f.U_in = h->U; f.U_out = h->A_W1; f.T_in = AT[0]; f.T_out = AT[1]; f.l2_partial = nullptr;
+    if ((e = tri_launch<R>(h->P, MODE_STAGE1, f, s)) != hipSuccess) return e;
+    f = tri_args<R>(h, MODE_STAGE2);
+    f.U0 = h->U; f.U_in = h->A_W1; f.U_out = h->A_W2; f.T_in = AT[1]; f.T_out = AT[2]; f.l2_partial = nullptr;
+    if ((e = tri_launch<R>(h->P, MODE_STAGE2, f, s)) != hipSuccess) return e;
+    for (int st = 3; st >= 1; --st) {
+        TriVjpArgs<R> a = tvjp_args<R>(h);
+        a.U_in = st == 3 ? h->A_W2 : (st == 2 ? h->A_W1 : h->U);
+        a.U0 = h->U;
+        a.T_in = AT[st - 1];
+        a.Ub_in = h->A_Ub;
+        a.Ub_out = h->A_Ub;
+        a.Tb_in = h->A_has_tb ? Tb[h->A_cur] : nullptr;
+        a.Tb_out = Tb[1 - h->A_cur];
+        a.U0b = h->A_U0b;
+        a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->nblk : nullptr;
+        if ((e = tri_vjp<R>(h->P, st, a, s)) != hipSuccess) return e;
+        h->A_cur = 1 - h->A_cur;
+        h->A_has_tb = true;
+    }
+    return hipSuccess;
+}
+
+extern "C" int zj_tri_adjoint_set_device(zj_tri_handle h, const double* d_ubar_aos, void* stream) {
+    if (!h || !d_ubar_aos) return ZJ_ERR_ARG;
+    const int rc = tensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_THIP(h, hipSetDevice(h->device));
+    ZJ_THIP(h, tri_convert(true, d_ubar_aos, h->A_Ub, h->E, h->NP, h->perm, tpick(h, stream)));
+    h->A_has_tb = false;
+    h->A_seeded = true;
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_adjoint_set(zj_tri_handle h, const double* ubar_aos) {
+    if (!h || !ubar_aos) return ZJ_ERR_ARG;
+    ZJ_THIP(h, hipSetDevice(h->device));
+    ZJ_THIP(h, hipMemcpyAsync(h->aos, ubar_aos, (size_t)h->E * h->NP * 4 * sizeof(double), hipMemcpyHostToDevice,
+                              h->stream));
+    const int rc = zj_tri_adjoint_set_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    ZJ_THIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_adjoint_get_device(zj_tri_handle h, double* d_ubar_aos, void* stream) {
+    if (!h || !d_ubar_aos) return ZJ_ERR_ARG;
+    if (!h->A_seeded) return tfail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    ZJ_THIP(h, hipSetDevice(h->device));
+    hipStream_t s = tpick(h, stream);
+    const void* tb = h->A_has_tb ? h->A_Tb[h->A_cur] : nullptr;
+    hipError_t e = h->prec == ZJ_PREC_FP64
+                       ? tri_complete<double>(h->P, h->E, h->links, h->A_Ub, (const double*)tb, h->Rres,
+                                              h->xcd_groups, s)
+                       : tri_complete<float>(h->P, h->E, h->links, h->A_Ub, (const float*)tb, h->Rres,
+                                             h->xcd_groups, s);
+    ZJ_THIP(h, e);
+    ZJ_THIP(h, tri_convert(false, h->Rres, d_ubar_aos, h->E, h->NP, h->perm, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_adjoint_get(zj_tri_handle h, double* ubar_aos) {
+    if (!h || !ubar_aos) return ZJ_ERR_ARG;
+    const int rc = zj_tri_adjoint_get_device(h, h->aos, nullptr);
+    if (rc) return rc;
+    ZJ_THIP(h, hipMemcpyAsync(ubar_aos, h->aos, (size_t)h->E * h->NP * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
+    ZJ_THIP(h, hipStreamSynchronize(h->stream));
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_rk3_step_vjp(zj_tri_handle h, double dt, double* dt_bar, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
+    if (!h->A_seeded) return tfail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    ZJ_THIP(h, hipSetDevice(h->device));
+    hipStream_t s = tpick(h, stream);
+    if (dt != h->dt_on_device) {
+        h->dt_on_device = dt;
+        ZJ_THIP(h, launch_set_scalar(h->d_scalars, dt, s));
+    }
+    const bool want = dt_bar != nullptr;
+    ZJ_THIP(h, h->prec == ZJ_PREC_FP64 ? tri_adjoint_step<double>(h, want, s) : tri_adjoint_step<float>(h, want, s));
+    if (want) {
+        ZJ_THIP(h, launch_finalize(h->A_dtb, 3 * h->nblk, FIN_SUM, h->d_scalars + 3, s));
+        ZJ_THIP(h, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+        ZJ_THIP(h, hipStreamSynchronize(s));
+        *dt_bar = h->h_pinned[3];
+    }
+    return ZJ_OK;
+}
+
+template <typename R>
+static hipError_t tri_residual_vjp(zj_tri* h, hipStream_t s) {
+    R** AT = (R**)h->A_T;
+    R* tb = (R*)h->A_Tb[1 - h->A_cur];  // the pending edge cotangents of the accumulator stay in A_Tb[A_cur]
+    hipError_t e = tri_trace<R>(h->P, h->E, h->U, AT[0], s);
+    if (e != hipSuccess) return e;
+    TriVjpArgs<R> a = tvjp_args<R>(h);
+    a.U_in = h->U;
+    a.T_in = AT[0];
+    a.Ub_in = h->Rres;
+    a.Ub_out = h->Rres;
+    a.Tb_out = tb;
+    if ((e = tri_vjp<R>(h->P, MODE_RESIDUAL, a, s)) != hipSuccess) return e;
+    return tri_complete<R>(h->P, h->E, h->links, h->Rres, tb, h->W, h->xcd_groups, s);
+}
+
+extern "C" int zj_tri_residual_vjp_device(zj_tri_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream) {
+    if (!h || !d_w_aos || !d_ubar_aos) return ZJ_ERR_ARG;
+    if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
+    const int rc = tensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_THIP(h, hipSetDevice(h->device));
+    hipStream_t s = tpick(h, stream);
+    ZJ_THIP(h, tri_convert(true, d_w_aos, h->Rres, h->E, h->NP, h->perm, s));
+    ZJ_THIP(h, h->prec == ZJ_PREC_FP64 ? tri_residual_vjp<double>(h, s) : tri_residual_vjp<float>(h, s));
+    ZJ_THIP(h, tri_convert(false, h->W, d_ubar_aos, h->E, h->NP, h->perm, s));
+    return ZJ_OK;
+}
+
+extern "C" int zj_tri_residual_vjp(zj_tri_handle h, const double* w_aos, double* ubar_aos) {
+    if (!h || !w_aos || !ubar_aos) return ZJ_ERR_ARG;
+    ZJ_THIP(h, hipSetDevice(h->device));
+    const size_t n = (size_t)h->E * h->NP * 4;
+    ZJ_THIP(h, hipMemcpyAsync(h->aos, w_aos, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const int rc = zj_tri_residual_vjp_device(h, h->aos, h->aos, nullptr);  // w is consumed before aos is rewritten
+    if (rc) return rc;
+    ZJ_THIP(h, hipMemcpyAsync(ubar_aos, h->aos, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_THIP(h, hipStreamSynchronize(h->stream));
     return ZJ_OK;
 }
 

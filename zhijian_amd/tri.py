@@ -137,6 +137,32 @@ class TriSolver:
     def rk3_step(self, dt, stream=None):
         self._chk(lib().zj_tri_rk3_step(self._h, float(dt), stream))
 
+    def adjoint_set(self, ubar):
+        """Seed the state cotangent (zj_tri_adjoint_set)."""
+        ubar = np.ascontiguousarray(ubar, np.float64).reshape(-1)
+        assert ubar.size == self.n
+        self._chk(lib().zj_tri_adjoint_set(self._h, ubar))
+
+    def adjoint_get(self):
+        out = np.zeros(self.n)
+        self._chk(lib().zj_tri_adjoint_get(self._h, out))
+        return out
+
+    def rk3_step_vjp(self, dt, stream=None, want_dt=True):
+        """One SSP-RK3 step backwards from the current state (zj_tri_rk3_step_vjp);
+        returns dJ/d(dt) (None with want_dt=False)."""
+        v = C.c_double()
+        self._chk(lib().zj_tri_rk3_step_vjp(self._h, float(dt), C.byref(v) if want_dt else None, stream))
+        return v.value if want_dt else None
+
+    def residual_vjp(self, w):
+        """(dR/dU)^T w at the current state (zj_tri_residual_vjp)."""
+        w = np.ascontiguousarray(w, np.float64).reshape(-1)
+        assert w.size == self.n
+        out = np.zeros(self.n)
+        self._chk(lib().zj_tri_residual_vjp(self._h, w, out))
+        return out
+
     def time_stages(self, dt, steps, stream=None):
         """zj_tri_time_stages: each stage kernel's mean execution time (ms)."""
         out = np.zeros(3)
