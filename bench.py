@@ -110,15 +110,17 @@ def time_adjoint(n, p, precision, steps, warmup, stream, hiprt):
                 note="adjoint step = traces of U^n + stages 1-2 recomputed + 3 VJP launches")
 
 
-def time_gradient(n, p, precision, steps, every, stream, hiprt):
+def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None):
     """The whole checkpointed gradient of the adjoint configuration (TGV n x n,
-    order p, Mach 0.1, textbook VCJH HU / Roe, kinetic energy after `steps`
-    SSP-RK3 steps, checkpoints every `every` steps on the GPU) against the
-    same `steps` forward steps alone (graph-replayed), wall time each."""
+    order p, Mach 0.1, textbook VCJH HU / Roe, with the Navier-Stokes terms at
+    Re when given, kinetic energy after `steps` SSP-RK3 steps, checkpoints
+    every `every` steps on the GPU) against the same `steps` forward steps
+    alone (graph-replayed), wall time each."""
     from zhijian_amd import adjoint, cases, solver
     mesh = cases.tgv_mesh(n)
     t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
-    s = solver.Solver(t, precision=precision, compat="textbook", riemann="roe", corr="hu", positivity=False)
+    visc = dict(viscous=True, Re=Re, Pr=0.72) if Re else {}
+    s = solver.Solver(t, precision=precision, compat="textbook", riemann="roe", corr="hu", positivity=False, **visc)
     U0 = cases.tgv_ic(t.pos, mach=0.1).reshape(-1)
     s.set_state(U0)
     dt = s.dt_cfl(0.5)
@@ -140,11 +142,13 @@ def time_gradient(n, p, precision, steps, every, stream, hiprt):
     adjoint.gradient(s, t, U0, dts, every=every, want_dt=False)
     hiprt.hip().hipDeviceSynchronize()
     grad_ms = (time.perf_counter() - t0) * 1e3
-    return dict(workload=f"TGV {n}x{n} P{p}, Mach 0.1, textbook VCJH HU / Roe, {steps} steps, checkpoints every "
-                         f"{every} (device-resident), kinetic-energy objective", precision=precision,
-                forward_ms=fwd_ms, gradient_ms=grad_ms, gradient_over_forward=grad_ms / fwd_ms,
-                note="gradient = forward with checkpoints + per-segment recompute + reverse sweep "
-                     "(each reverse step recomputes its stages 1-2); host wall time around the whole call")
+    physics = f"Navier-Stokes (BR2) Re {Re:g}" if Re else "Euler"
+    return dict(workload=f"TGV {n}x{n} P{p}, Mach 0.1, textbook VCJH HU / Roe, {physics}, {steps} steps, "
+                         f"checkpoints every {every} (device-resident), kinetic-energy objective",
+                precision=precision, forward_ms=fwd_ms, gradient_ms=grad_ms, gradient_over_forward=grad_ms / fwd_ms,
+                note="gradient = forward with checkpoints + per segment: forward recompute recorded on a device "
+                     "tape + stage VJPs from the tape (both passes native); host wall time around the whole "
+                     "call, including the host objective and two state transfers")
 
 
 def batch_members(n_inst, seed, precision):
@@ -792,8 +796,10 @@ def main():
         for prec in ("fp64", "fp32bulk"):
             adj[prec] = time_adjoint(128, 3, prec, args.steps, args.warmup, stream, hiprt)
             log(f"adjoint {prec}: {adj[prec]}")
-            adj[prec]["checkpointed_gradient"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt)
+            adj[prec]["checkpointed_gradient"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt, Re=1600.0)
             log(f"gradient {prec}: {adj[prec]['checkpointed_gradient']}")
+            adj[prec]["checkpointed_gradient_euler"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt)
+            log(f"gradient {prec} (Euler terms): {adj[prec]['checkpointed_gradient_euler']}")
 
     gpu_api = time_gpu_api(tables, U0, max(args.steps // 5, 10), stream, hiprt)
     log(f"gpu api: {gpu_api}")
@@ -848,7 +854,8 @@ def main():
         "fp32bulk_error_vs_fp64": err,
         "adjoint_over_forward": adj.get("fp64", {}).get("checkpointed_gradient", {}).get("gradient_over_forward"),
         "adjoint_over_forward_basis": "whole checkpointed gradient (256 steps, checkpoints every 16, TGV 128x128 "
-                                      "P3 textbook, FP64) over the same 256 forward steps",
+                                      "P3 textbook with the Navier-Stokes terms at Re 1600, FP64) over the same "
+                                      "256 forward steps",
         "adjoint": adj or None,
         "batched": batched,
         "textbook": textbook,

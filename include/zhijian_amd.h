@@ -214,7 +214,9 @@ int zj_set_xcd_groups(zj_handle h, int groups);
  * of every executed operation (fluxes kernels.cu:157-194, Rusanov and ghost
  * states :453-701, flux difference :346-450, divergence and correction
  * :813-1034, SSP-RK3 and the limiter :1236-1349), taking the branch of each
- * clamp the forward took.
+ * clamp the forward took.  In textbook mode, of the textbook scheme (VCJH
+ * correction with metrics, Roe / HLLC / Rusanov of euler.hpp:204-375) and,
+ * with the Navier-Stokes terms, of BR2 (navierstokes.hpp:53-99, 170-205).
  *
  * The handle keeps the cotangent of its state on the device.  Reverse time
  * stepping: zj_adjoint_set(dJ/dU^N); then for n = N-1 .. 0:
@@ -226,6 +228,24 @@ int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void* stream);
 int zj_adjoint_get(zj_handle h, double* ubar_aos);
 int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stream);
 int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream);
+/* Checkpointed gradient, both passes native (no reference counterpart).
+ * zj_forward_checkpointed: n_steps SSP-RK3 steps from the handle's state with
+ * dt = dts[n], U^{c*every} saved in state slot c (the slots grow to
+ * ceil(n_steps / every) if fewer).  zj_adjoint_sweep: with the cotangent of
+ * U^N seeded (zj_adjoint_set), every step backwards: per segment, its
+ * checkpoint, the segment's forward steps recorded on a device tape (stage
+ * states and traces, every steps), then the stage VJPs from the tape; the
+ * handle then holds U^0 and the cotangent of U^0 (zj_adjoint_get).  dt_bar
+ * (n_steps values, or NULL) receives dJ/d(dt_n); with NULL the sweep does not
+ * synchronize. */
+int zj_forward_checkpointed(zj_handle h, const double* dts, int n_steps, int every, void* stream);
+int zj_adjoint_sweep(zj_handle h, const double* dts, int n_steps, int every, double* dt_bar, void* stream);
+/* With the Navier-Stokes terms (textbook mode, BR2, navierstokes.hpp:53-99,
+ * 170-205): dJ/dRe accumulated over the steps run backwards since
+ * zj_adjoint_set (0 without viscous terms).  The viscous fluxes scale with
+ * mu = 1/Re (kappa with it), so the kernels sum <cotangent of F_v, F_v> over
+ * every point and face point; dJ/dRe = -mu * that sum.  Synchronizes the device. */
+int zj_adjoint_get_re(zj_handle h, double* dJ_dRe);
 /* (dR/dU)^T w at the handle's state, R = FRSolver::computeResidual_GPU's dU/dt
  * (fr_solver.cpp:335-608); w and ubar are [E][S][4]. */
 int zj_residual_vjp(zj_handle h, const double* w_aos, double* ubar_aos);

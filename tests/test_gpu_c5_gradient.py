@@ -1,19 +1,17 @@
 """The adjoint configuration at its stated size: Taylor-Green vortex on 128x128
-quadrilaterals, P3, Mach 0.1, objective = domain-integrated kinetic energy
-after 256 SSP-RK3 steps, checkpoints every 16 steps kept on the GPU
-(zhijian_amd.adjoint.gradient), the textbook scheme (VCJH HU, Roe), FP64.
+quadrilaterals, P3, Re 1600, Mach 0.1, objective = domain-integrated kinetic
+energy after 256 SSP-RK3 steps, checkpoints every 16 steps kept on the GPU
+(zhijian_amd.adjoint.gradient), the textbook scheme with its Navier-Stokes
+terms (BR2, VCJH HU, Roe), FP64.
 
 dJ/dU0 is checked on 32 seeded directions against central differences
-(eps = 1e-6 relative) of the FP64 forward map, and dJ/dMach through the
-closed-form initial state.  The forward map of these differences is the GPU
-textbook path, which equals the textbook oracle to 1e-10 over 100 steps
-(test_gpu_textbook.py); one of the directions is also differenced with the
-oracle itself (oracle/zj_textbook.c) over the full 256 steps.  Bar: 1e-6
+(eps = 1e-6 relative) of the FP64 forward map; dJ/dMach through the
+closed-form initial state and dJ/dRe against Richardson-extrapolated central
+differences.  The forward map of these differences is the GPU viscous path,
+which equals the textbook oracle with its viscous terms to 1e-10 over 30
+steps (test_gpu_viscous.py); one of the directions is also differenced with
+the oracle itself (oracle/zj_textbook.c) over the full 256 steps.  Bar: 1e-6
 relative.  Runs on an MI355X.
-
-The configuration names Re = 1600; the adjoint of the Navier-Stokes (BR2)
-terms is not built, so this runs the Euler terms of the same scheme, where
-dJ/dRe is identically zero.
 """
 import os
 
@@ -27,28 +25,33 @@ from zhijian_amd import adjoint, cases, solver
 
 pytestmark = pytest.mark.gpu
 
-N, P, MACH, STEPS, EVERY, EPS = 128, 3, 0.1, 256, 16, 1e-6
+N, P, RE, MACH, STEPS, EVERY, EPS = 128, 3, 1600.0, 0.1, 256, 16, 1e-6
+ETA = 2.0  # BR2 penalty (the library's default)
+
+
+def _solver(t, Re):
+    return solver.Solver(t, compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Re=Re,
+                         Pr=0.72, br2_eta=ETA)
 
 
 @pytest.fixture(scope="module")
 def c5():
     mesh = cases.tgv_mesh(N)
     t = solver.build_tables(mesh.xy, mesh.elem_nodes, P, periodic=(mesh.Lx, mesh.Ly))
-    mk = lambda: solver.Solver(t, compat="textbook", riemann="roe", corr="hu", positivity=False)
-    s, f = mk(), mk()
+    s, f = _solver(t, RE), _solver(t, RE)
     U0 = cases.tgv_ic(t.pos, mach=MACH).reshape(-1)
     s.set_state(U0)
     dt = s.dt_cfl(0.5)  # the CFL step of the initial state, then held fixed
-    say(f"case: C5 gradient, TGV {N}x{N} P{P}, {STEPS} steps of dt {dt:.4e}, checkpoints every {EVERY}")
-    J, g, _ = adjoint.gradient(s, t, U0, [dt] * STEPS, every=EVERY, want_dt=False)
+    say(f"case: C5 gradient, TGV {N}x{N} P{P} Re {RE:g}, {STEPS} steps of dt {dt:.4e}, checkpoints every {EVERY}")
+    J, g, _, dre = adjoint.gradient(s, t, U0, [dt] * STEPS, every=EVERY, want_dt=False, want_re=True)
 
-    def forward(U):
-        f.set_state(U)
+    def run(sol, U):
+        sol.set_state(U)
         for _ in range(STEPS):
-            f.rk3_step(dt)
-        return adjoint.kinetic_energy(f.get_state(), t)
+            sol.rk3_step(dt)
+        return adjoint.kinetic_energy(sol.get_state(), t)
 
-    return dict(mesh=mesh, t=t, U0=U0, dt=dt, J=J, g=g, forward=forward)
+    return dict(mesh=mesh, t=t, U0=U0, dt=dt, J=J, g=g, dre=dre, forward=lambda U: run(f, U), run=run)
 
 
 def test_c5_objective_matches_forward(c5):
@@ -93,12 +96,28 @@ def test_c5_gradient_mach(c5):
     assert r < 1e-6
 
 
+def test_c5_gradient_re(c5):
+    """dJ/dRe from the adjoint's accumulated <cotangent of F_v, F_v> against
+    Richardson's extrapolation of central differences at h = 2% and 1% of Re."""
+    t, U0, run = c5["t"], c5["U0"], c5["run"]
+
+    def central(h):
+        return (run(_solver(t, RE + h), U0) - run(_solver(t, RE - h), U0)) / (2 * h)
+    d1, d2 = central(0.02 * RE), central(0.01 * RE)
+    fd = (4.0 * d2 - d1) / 3.0
+    ad = c5["dre"]
+    r = abs(ad - fd) / abs(fd)
+    say(f"  dJ/dRe: adjoint {ad:+.12e}  Richardson FD {fd:+.12e} (central {d2:+.12e})  rel {r:.2e}")
+    assert r < 1e-6
+
+
 def test_c5_gradient_against_oracle_forward(c5):
     """One direction differenced with the textbook oracle's own forward map."""
     mesh, U0, g, dt = c5["mesh"], c5["U0"], c5["g"], c5["dt"]
     O.set_threads(min(16, len(os.sched_getaffinity(0))))
     try:
         tb = fr_case.build(mesh.xy, mesh.elem_nodes, P, periodic=(mesh.Lx, mesh.Ly)).textbook("roe", "hu")
+        tb.set_viscous(RE, eta=ETA)
         tt = c5["t"]
 
         def oracle_J(U):

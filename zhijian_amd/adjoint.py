@@ -7,16 +7,15 @@ The objective is the kinetic energy of the final state,
 with the reference's quadrature weights (FROperators quad_weights,
 basis.cpp:484) and Jacobian determinants (FRSolver::initGeometry,
 fr_solver.cpp:85-129).  The forward trajectory keeps one state every
-`every` steps; the reverse sweep recomputes each segment from its checkpoint
-and runs zj_rk3_step_vjp backwards through it.  Checkpoints and the
-segment's states stay on the GPU (zj_state_slots / zj_state_save /
-zj_state_load: no host copy per step), and the library holds the cotangent
-on the device between steps; the host sees the final state (for J and its
-gradient) and the result.
+`every` steps (zj_forward_checkpointed); the reverse sweep
+(zj_adjoint_sweep) recomputes each segment from its checkpoint, recording the
+stage states and traces of its steps on a device tape, and runs the stage
+VJPs backwards from the tape.  Both passes run in the library, on the GPU:
+the host sees the final state (for J and its gradient) and the result.
 """
 from __future__ import annotations
 
-from typing import List, Sequence, Tuple
+from typing import Sequence
 
 import numpy as np
 
@@ -43,33 +42,22 @@ def kinetic_energy_grad(U: np.ndarray, t: Tables) -> np.ndarray:
 
 
 def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every: int = 16, want_dt: bool = True,
-             stream=None) -> Tuple[float, np.ndarray, List[float]]:
+             stream=None, want_re: bool = False):
     """J(U^N) after len(dts) steps from U0, dJ/dU0 and dJ/d(dt_n) (None
     each with want_dt=False, which keeps the reverse sweep free of host
-    synchronisation)."""
+    synchronisation); with want_re, also dJ/dRe (Navier-Stokes terms) as a
+    fourth value."""
     steps = len(dts)
-    n_ckpt = (steps + every - 1) // every
-    # slots 0 .. n_ckpt-1: checkpoints; n_ckpt .. n_ckpt+every-2: the segment's later states
-    sol.state_slots(n_ckpt + every - 1)
     sol.set_state(np.array(U0, np.float64).reshape(-1))
-    for n in range(steps):
-        if n % every == 0:
-            sol.state_save(n // every, stream)
-        sol.rk3_step(dts[n], stream)
+    sol.forward_checkpointed(dts, every, stream)
     UN = sol.get_state()
     J = kinetic_energy(UN, t)
     sol.adjoint_set(kinetic_energy_grad(UN, t))
-    dtb = [None] * steps
-    for c in range(n_ckpt - 1, -1, -1):
-        start = c * every
-        end = min(start + every, steps)
-        sol.state_load(c, stream)
-        for n in range(start, end - 1):  # states start+1 .. end-1 of the segment
-            sol.rk3_step(dts[n], stream)
-            sol.state_save(n_ckpt + n - start, stream)
-        for n in range(end - 1, start - 1, -1):
-            sol.state_load(c if n == start else n_ckpt + n - start - 1, stream)
-            dtb[n] = sol.rk3_step_vjp(dts[n], stream, want_dt=want_dt)
+    dtb = sol.adjoint_sweep(dts, every, want_dt, stream)
+    if dtb is None:
+        dtb = [None] * steps
     g = sol.adjoint_get()
     sol.state_slots(0)
+    if want_re:
+        return J, g, dtb, sol.adjoint_get_re()
     return J, g, dtb

@@ -195,6 +195,10 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     const int ub = el * Y::UE + sp;
     const int i = sp % N1, j = sp / N1;
 
+    // The forward residual is needed only for the dt cotangent and the
+    // limiter's VJP; without either, the kernel keeps just the traces.
+    const bool need_res = MODE != MODE_RESIDUAL && (a.dtb_partial != nullptr || a.positivity);
+
     // ---- forward A: state, flux
     double q[NV], u0[NV];
     R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
@@ -216,10 +220,12 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             else q[v] = a.U_in[v * PS + gi];
             uq[v] = R(q[v]);
         }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sU[ub + v * Y::UV] = uq[v];
+        if (need_res) {
         euler_flux(uq, g, fx, fy);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            sU[ub + v * Y::UV] = uq[v];
             if (AFF) {
                 sFx[ub + v * Y::UV] = jx * fx[v] + jy * fy[v];
                 sFy[ub + v * Y::UV] = kx * fx[v] + ky * fy[v];
@@ -229,6 +235,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
             } else {
                 sFx[ub + v * Y::UV] = fx[v]; sFy[ub + v * Y::UV] = fy[v];
             }
+        }
         }
     }
     __syncthreads();
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
 #pragma unroll
         for (int v = 0; v < NV; ++v) sTb[Y::tr(el2, k, f, v)] = tb[v];
         R Fd[NV] = {R(0), R(0), R(0), R(0)};
-        if (L.role != ROLE_NONE) {
+        if (need_res && L.role != ROLE_NONE) {
             const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
             const bool right = L.role == ROLE_RIGHT;
             R own[NV], mirr[NV], other[NV], UL[NV], UR[NV], Fc[NV], Fn[NV];
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     double rsum = 0.0;  // dt cotangent partial
     double qb[NV] = {0.0, 0.0, 0.0, 0.0};
     double res[NV] = {0.0, 0.0, 0.0, 0.0};
-    if (active) {
+    if (active && need_res) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const R* Fx = sFx + el * Y::UE + v * Y::UV;

@@ -28,6 +28,7 @@ struct VjpArgs {
     R* Tb_out;               // cross-tile edge cotangents of q, written at the neighbour's slots
     double* U0b;             // cotangent of U^n: stage 3 writes, stage 2 adds, stage 1 reads
     double* dtb_partial;     // per-tile sum of c * R . du_n (nullptr: skip)
+    double* mub_partial;     // viscous terms: per-tile sum of <cotangent of F_v, F_v> (nullptr: skip)
     double gamma;
     int positivity;
     int xcd_groups;
@@ -55,5 +56,19 @@ hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& o
                       bool textbook = false);
 template <typename R>
 hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& ops, hipStream_t s);
+
+// The BR2 (Navier-Stokes) stage's adjoint (zj_visc.hip): its own tile
+// (visc_adjoint_tile), the forward stages 1 and 2 recomputed at that tile,
+// the stage VJP, and the completion of 12-word edge cotangents (U, grad U).
+int visc_adjoint_tile(int P, bool fp64);
+template <typename R>
+hipError_t launch_visc_stage_adj(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
+                                 const ViscParams& vp, hipStream_t s);
+template <typename R>
+hipError_t launch_visc_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                           hipStream_t s);
+template <typename R>
+hipError_t launch_visc_complete(int P, const CompleteArgs<R>& a, const R* Jinv_e, const OpTables<R>& ops,
+                                hipStream_t s);
 
 }  // namespace zj

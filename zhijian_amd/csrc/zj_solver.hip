@@ -61,7 +61,7 @@ struct zj_solver {
     void* normals = nullptr;
     double* bc_data = nullptr;
     double* hcell = nullptr;
-    double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl
+    double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl, [3] dt_bar, [4] sum of <F_v cotangent, F_v>
     double* partial = nullptr;     // per-workgroup sums of R^2 of the last residual / stage 3
     double* partial_dt = nullptr;  // per-workgroup minima of the CFL dt
     int l2_blocks = 0;             // valid entries of `partial` (0: no residual yet)
@@ -91,8 +91,15 @@ struct zj_solver {
     double* A_U0b = nullptr;       // cotangent of U^n within a step
     void* A_Tb[2] = {nullptr, nullptr};  // cross-tile edge cotangents, ping-pong
     double* A_dtb = nullptr;       // per-tile dt cotangents of the three stages
+    double* A_mub = nullptr;       // viscous terms: per-tile <F_v cotangent, F_v> of the three stages
     double* slots = nullptr;       // device copies of states (SoA planes), zj_state_slots
     int n_slots = 0;
+    // tape of the native reverse sweep: per step of a segment, U^n, U1, U2
+    // and their traces at the adjoint tile (tape_steps entries)
+    char* tape = nullptr;
+    int tape_steps = 0;
+    double* d_dtb_steps = nullptr;  // dJ/d(dt_n) of every step of a sweep
+    int n_dtb_steps = 0;
     int A_cur = 0;
     bool A_has_tb = false;
     bool A_seeded = false;
@@ -203,7 +210,7 @@ static void release(zj_solver* h) {
     if (h->graph) hipGraphDestroy(h->graph);
     void* bufs[] = {h->perm, h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
-                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb, h->slots};
+                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb, h->A_mub, h->slots, h->tape, h->d_dtb_steps};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -952,8 +959,9 @@ static int ensure_adjoint(zj_solver* h) {
     if (h->adj_ready) return ZJ_OK;
     const size_t npts = (size_t)h->E * h->S;
     const size_t rsz = h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float);
-    const size_t ntr = std::max<size_t>((size_t)h->E * 4 * h->N1 * 4, 1);
-    h->adj_epb = adjoint_tile(h->P);
+    // edge cotangents: U (4 words per face point), with the viscous terms also grad U (12)
+    const size_t ntr = std::max<size_t>((size_t)h->E * 4 * h->N1 * (h->viscous ? 12 : 4), 1);
+    h->adj_epb = h->viscous ? visc_adjoint_tile(h->P, h->prec == ZJ_PREC_FP64) : adjoint_tile(h->P);
     h->adj_nt = num_blocks(std::max(h->E, 1), h->adj_epb);
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipMalloc((void**)&h->A_W1, std::max<size_t>(npts * 4, 1) * sizeof(double)));
@@ -962,6 +970,7 @@ static int ensure_adjoint(zj_solver* h) {
     ZJ_HIP(h, hipMalloc((void**)&h->A_U0b, std::max<size_t>(npts * 4, 1) * sizeof(double)));
     for (int k = 0; k < 2; ++k) ZJ_HIP(h, hipMalloc(&h->A_Tb[k], ntr * rsz));
     ZJ_HIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * h->adj_nt * sizeof(double)));
+    if (h->viscous) ZJ_HIP(h, hipMalloc((void**)&h->A_mub, (size_t)3 * h->adj_nt * sizeof(double)));
     h->adj_ready = true;
     return ZJ_OK;
 }
@@ -984,45 +993,71 @@ static VjpArgs<R> vjp_args(zj_solver* h) {
     return a;
 }
 
-// The forward stages 1 and 2 at the adjoint tile, keeping U1 and U2 apart.
+// One step's forward data at the adjoint tile: U^n, the stage states U1, U2
+// (FP64, or FP32 increments over U^n) and the traces of U^n, U1, U2 on the
+// edges another adjoint tile reads.
+struct StepBufs {
+    double* U;
+    double* W1;
+    double* W2;
+    void* T[3];
+};
+
+static StepBufs handle_step(zj_solver* h) { return StepBufs{h->U, h->A_W1, h->A_W2, {h->T[0], h->T[1], h->T[2]}}; }
+
 template <typename R>
-static hipError_t recompute_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s) {
-    R** T = (R**)h->T;
+static hipError_t launch_stage_adj(zj_solver* h, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
+                                   hipStream_t s) {
+    return h->viscous ? launch_visc_stage_adj<R>(h->P, mode, a, ops, h->vp, s)
+                      : launch_stage<R>(h->P, mode, a, ops, h->adj_epb, h->affine, s, h->textbook);
+}
+
+// The forward stages 1 and 2 at the adjoint tile, keeping U1 and U2 apart;
+// with U_next, stage 3 too (into U_next and its traces T_next).
+template <typename R>
+static hipError_t recompute_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const StepBufs& b,
+                                   double* U_next = nullptr, void* T_next = nullptr) {
     const bool inc = std::is_same<R, float>::value;
     StageArgs<R> a = stage_args<R>(h, MODE_STAGE1);
-    a.T_in = T[0]; a.T_out = T[1]; a.l2_partial = nullptr;
-    if (inc) { a.dW_out = (float*)h->A_W1; a.U_out = nullptr; } else { a.U_out = h->A_W1; }
-    hipError_t e = launch_stage<R>(h->P, MODE_STAGE1, a, ops, h->adj_epb, h->affine, s, h->textbook);
+    a.U_in = b.U; a.T_in = (const R*)b.T[0]; a.T_out = (R*)b.T[1]; a.l2_partial = nullptr;
+    if (inc) { a.dW_out = (float*)b.W1; a.U_out = nullptr; } else { a.U_out = b.W1; }
+    hipError_t e = launch_stage_adj<R>(h, MODE_STAGE1, a, ops, s);
     if (e != hipSuccess) return e;
-    StageArgs<R> b = stage_args<R>(h, MODE_STAGE2);
-    b.T_in = T[1]; b.T_out = T[2]; b.l2_partial = nullptr;
-    if (inc) { b.dW_in = (const float*)h->A_W1; b.dW_out = (float*)h->A_W2; b.U_in = nullptr; b.U_out = nullptr; }
-    else { b.U_in = h->A_W1; b.U_out = h->A_W2; }
-    return launch_stage<R>(h->P, MODE_STAGE2, b, ops, h->adj_epb, h->affine, s, h->textbook);
+    StageArgs<R> c = stage_args<R>(h, MODE_STAGE2);
+    c.U0 = b.U; c.T_in = (const R*)b.T[1]; c.T_out = (R*)b.T[2]; c.l2_partial = nullptr;
+    if (inc) { c.dW_in = (const float*)b.W1; c.dW_out = (float*)b.W2; c.U_in = nullptr; c.U_out = nullptr; }
+    else { c.U_in = b.W1; c.U_out = b.W2; }
+    if ((e = launch_stage_adj<R>(h, MODE_STAGE2, c, ops, s)) != hipSuccess || !U_next) return e;
+    StageArgs<R> d = stage_args<R>(h, MODE_STAGE3);
+    d.U0 = b.U; d.U_out = U_next; d.T_in = (const R*)b.T[2]; d.T_out = (R*)T_next; d.l2_partial = nullptr;
+    if (inc) { d.dW_in = (const float*)b.W2; d.U_in = nullptr; } else { d.U_in = b.W2; }
+    return launch_stage_adj<R>(h, MODE_STAGE3, d, ops, s);
 }
 
 template <typename R>
-static hipError_t adjoint_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s) {
-    R** T = (R**)h->T;
+static hipError_t adjoint_stages(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const StepBufs& b,
+                                 bool want_dt) {
     R** Tb = (R**)h->A_Tb;
     const bool inc = std::is_same<R, float>::value;
     for (int st = 3; st >= 1; --st) {
         VjpArgs<R> a = vjp_args<R>(h);
-        double* W = st == 3 ? h->A_W2 : h->A_W1;
+        double* W = st == 3 ? b.W2 : b.W1;
         if (st > 1) {
-            a.U0 = h->U;
+            a.U0 = b.U;
             if (inc) a.dW_in = (const float*)W; else a.U_in = W;
         } else {
-            a.U_in = h->U;
+            a.U_in = b.U;
         }
-        a.T_in = T[st - 1];
+        a.T_in = (const R*)b.T[st - 1];
         a.Ub_in = h->A_Ub;
         a.Ub_out = h->A_Ub;
         a.Tb_in = h->A_has_tb ? Tb[h->A_cur] : nullptr;
         a.Tb_out = Tb[1 - h->A_cur];
         a.U0b = h->A_U0b;
-        a.dtb_partial = h->A_dtb + (size_t)(3 - st) * h->adj_nt;
-        hipError_t e = launch_vjp<R>(h->P, st, a, ops, h->affine, s, h->textbook);
+        a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->adj_nt : nullptr;
+        if (h->viscous) a.mub_partial = h->A_mub + (size_t)(3 - st) * h->adj_nt;
+        hipError_t e = h->viscous ? launch_visc_vjp<R>(h->P, st, a, ops, h->vp, s)
+                                  : launch_vjp<R>(h->P, st, a, ops, h->affine, s, h->textbook);
         if (e != hipSuccess) return e;
         h->A_cur = 1 - h->A_cur;
         h->A_has_tb = true;
@@ -1038,6 +1073,7 @@ extern "C" int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void
     ZJ_HIP(h, launch_aos_to_soa(d_ubar_aos, h->A_Ub, (int64_t)h->E * h->S, pick(h, stream), h->perm, h->S));
     h->A_has_tb = false;
     h->A_seeded = true;
+    ZJ_HIP(h, launch_set_scalar(h->d_scalars + 4, 0.0, pick(h, stream)));
     return ZJ_OK;
 }
 
@@ -1056,6 +1092,7 @@ template <typename R>
 static hipError_t complete_into(zj_solver* h, const double* Ub, const void* Tb, double* out, const OpTables<R>& ops,
                                 hipStream_t s) {
     CompleteArgs<R> c{h->E, Ub, (const R*)Tb, out, h->links, h->xcd_groups};
+    if (h->viscous) return launch_visc_complete<R>(h->P, c, (const R*)h->Jinv_e, ops, s);
     return launch_complete<R>(h->P, c, ops, s);
 }
 
@@ -1084,7 +1121,6 @@ extern "C" int zj_adjoint_get(zj_handle h, double* ubar_aos) {
 
 extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream) {
     if (!h) return ZJ_ERR_ARG;
-    if (h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint of the Navier-Stokes terms is not built");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
     if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
@@ -1094,14 +1130,16 @@ extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* str
     int rc = refresh_trace(h, s);  // traces of U^n on every edge, valid for the adjoint tile
     if (rc) return rc;
     hipError_t e;
+    const StepBufs b = handle_step(h);
     if (h->prec == ZJ_PREC_FP64) {
-        e = recompute_stages<double>(h, h->ops64, s);
-        if (e == hipSuccess) e = adjoint_stages<double>(h, h->ops64, s);
+        e = recompute_stages<double>(h, h->ops64, s, b);
+        if (e == hipSuccess) e = adjoint_stages<double>(h, h->ops64, s, b, dt_bar != nullptr);
     } else {
-        e = recompute_stages<float>(h, h->ops32, s);
-        if (e == hipSuccess) e = adjoint_stages<float>(h, h->ops32, s);
+        e = recompute_stages<float>(h, h->ops32, s, b);
+        if (e == hipSuccess) e = adjoint_stages<float>(h, h->ops32, s, b, dt_bar != nullptr);
     }
     ZJ_HIP(h, e);
+    if (h->viscous) ZJ_HIP(h, launch_finalize(h->A_mub, 3 * h->adj_nt, FIN_ACC, h->d_scalars + 4, s));
     if (dt_bar) {
         ZJ_HIP(h, launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_scalars + 3, s));
         ZJ_HIP(h, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1111,9 +1149,150 @@ extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* str
     return ZJ_OK;
 }
 
+// ------------------------------------------------- native checkpointed sweep
+static size_t tape_state_bytes(const zj_solver* h) {
+    return ((size_t)h->E * h->S * 4 * sizeof(double) + 255) / 256 * 256;
+}
+static size_t tape_trace_bytes(const zj_solver* h) {
+    const size_t w = (size_t)h->E * 4 * h->N1 * (h->viscous ? 12 : 4);
+    return (std::max<size_t>(w, 1) * (h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float)) + 255) / 256 * 256;
+}
+static StepBufs tape_step(zj_solver* h, int k) {
+    const size_t sb = tape_state_bytes(h), tb = tape_trace_bytes(h);
+    char* base = h->tape + (size_t)k * (3 * sb + 3 * tb);
+    return StepBufs{(double*)base, (double*)(base + sb), (double*)(base + 2 * sb),
+                    {base + 3 * sb, base + 3 * sb + tb, base + 3 * sb + 2 * tb}};
+}
+
+static int ensure_tape(zj_solver* h, int steps) {
+    if (h->tape && h->tape_steps >= steps) return ZJ_OK;
+    if (h->tape) {
+        ZJ_HIP(h, hipStreamSynchronize(h->stream));
+        ZJ_HIP(h, hipDeviceSynchronize());
+        ZJ_HIP(h, hipFree(h->tape));
+        h->tape = nullptr;
+        h->tape_steps = 0;
+    }
+    const size_t bytes = (size_t)steps * (3 * tape_state_bytes(h) + 3 * tape_trace_bytes(h));
+    if (hipMalloc((void**)&h->tape, bytes) != hipSuccess) {
+        h->tape = nullptr;
+        return fail(h, ZJ_ERR_NOMEM, "reverse-sweep tape: device allocation failed");
+    }
+    h->tape_steps = steps;
+    return ZJ_OK;
+}
+
+template <typename R>
+static hipError_t trace_into(zj_solver* h, const double* U, void* T, const OpTables<R>& ops, hipStream_t s) {
+    if (h->viscous) return launch_visc_trace<R>(h->P, h->E, U, (const R*)h->Jinv_e, (R*)T, ops, h->xcd_groups, s);
+    return launch_trace<R>(h->P, TraceArgs<R>{h->E, U, (R*)T, h->xcd_groups}, ops, 0, s);
+}
+
+// One segment backwards: its checkpoint, the forward steps recorded on the
+// tape (stage states and traces written where the VJPs read them: no step is
+// computed twice), then the stage VJPs from the last step to the first.
+template <typename R>
+static hipError_t sweep_segment(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const double* dts, int start,
+                                int m, int slot, bool want_dt) {
+    const size_t n = (size_t)h->E * h->S * 4;
+    StepBufs b0 = tape_step(h, 0);
+    hipError_t e = hipMemcpyAsync(b0.U, h->slots + (size_t)slot * n, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = trace_into<R>(h, b0.U, b0.T[0], ops, s);
+    for (int k = 0; k < m && e == hipSuccess; ++k) {
+        if ((e = set_dt(h, dts[start + k], s)) != hipSuccess) break;
+        const StepBufs b = tape_step(h, k);
+        if (k + 1 < m) {
+            const StepBufs nb = tape_step(h, k + 1);
+            e = recompute_stages<R>(h, ops, s, b, nb.U, nb.T[0]);
+        } else {
+            e = recompute_stages<R>(h, ops, s, b);
+        }
+    }
+    for (int k = m - 1; k >= 0 && e == hipSuccess; --k) {
+        if ((e = set_dt(h, dts[start + k], s)) != hipSuccess) break;
+        if ((e = adjoint_stages<R>(h, ops, s, tape_step(h, k), want_dt)) != hipSuccess) break;
+        if (h->viscous && (e = launch_finalize(h->A_mub, 3 * h->adj_nt, FIN_ACC, h->d_scalars + 4, s)) != hipSuccess)
+            break;
+        if (want_dt) e = launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_dtb_steps + start + k, s);
+    }
+    return e;
+}
+
+extern "C" int zj_forward_checkpointed(zj_handle h, const double* dts, int n_steps, int every, void* stream) {
+    if (!h || !dts || n_steps < 1 || every < 1) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    const int n_ckpt = (n_steps + every - 1) / every;
+    if (h->n_slots < n_ckpt) {
+        const int rc = zj_state_slots(h, n_ckpt);
+        if (rc) return rc;
+    }
+    for (int n = 0; n < n_steps; ++n) {
+        int rc;
+        if (n % every == 0 && (rc = zj_state_save(h, n / every, stream))) return rc;
+        if ((rc = zj_rk3_step(h, dts[n], stream))) return rc;
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_sweep(zj_handle h, const double* dts, int n_steps, int every, double* dt_bar,
+                                void* stream) {
+    if (!h || !dts || n_steps < 1 || every < 1) return ZJ_ERR_ARG;
+    if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    const int n_ckpt = (n_steps + every - 1) / every;
+    if (h->n_slots < n_ckpt) return fail(h, ZJ_ERR_STATE, "fewer state slots than checkpoints");
+    int rc = ensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    if ((rc = ensure_tape(h, std::min(every, n_steps)))) return rc;
+    if (dt_bar && h->n_dtb_steps < n_steps) {
+        if (h->d_dtb_steps) {
+            ZJ_HIP(h, hipDeviceSynchronize());
+            ZJ_HIP(h, hipFree(h->d_dtb_steps));
+            h->d_dtb_steps = nullptr;
+            h->n_dtb_steps = 0;
+        }
+        ZJ_HIP(h, hipMalloc((void**)&h->d_dtb_steps, (size_t)n_steps * sizeof(double)));
+        h->n_dtb_steps = n_steps;
+    }
+    hipStream_t s = pick(h, stream);
+    for (int c = n_ckpt - 1; c >= 0; --c) {
+        const int start = c * every, m = std::min(start + every, n_steps) - start;
+        const hipError_t e = h->prec == ZJ_PREC_FP64
+                                 ? sweep_segment<double>(h, h->ops64, s, dts, start, m, c, dt_bar != nullptr)
+                                 : sweep_segment<float>(h, h->ops32, s, dts, start, m, c, dt_bar != nullptr);
+        ZJ_HIP(h, e);
+    }
+    // the handle's state is U^0 again
+    const size_t n = (size_t)h->E * h->S * 4;
+    ZJ_HIP(h, hipMemcpyAsync(h->U, h->slots, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if ((rc = refresh_trace(h, s))) return rc;
+    h->has_state = true;
+    if (dt_bar) {
+        ZJ_HIP(h, hipMemcpyAsync(dt_bar, h->d_dtb_steps, (size_t)n_steps * sizeof(double), hipMemcpyDeviceToHost, s));
+        ZJ_HIP(h, hipStreamSynchronize(s));
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_get_re(zj_handle h, double* dJ_dRe) {
+    if (!h || !dJ_dRe) return ZJ_ERR_ARG;
+    if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    if (!h->viscous) {
+        *dJ_dRe = 0.0;
+        return ZJ_OK;
+    }
+    ZJ_HIP(h, hipSetDevice(h->device));
+    ZJ_HIP(h, hipDeviceSynchronize());
+    ZJ_HIP(h, hipMemcpy(h->h_pinned + 4, h->d_scalars + 4, sizeof(double), hipMemcpyDeviceToHost));
+    // F_v scales with mu = 1/Re: dJ/dmu = S / mu, dJ/dRe = -dJ/dmu / Re^2 = -S / Re
+    *dJ_dRe = -h->h_pinned[4] * h->vp.mu;
+    return ZJ_OK;
+}
+
 extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_aos, void* stream) {
     if (!h || !d_w_aos || !d_ubar_aos) return ZJ_ERR_ARG;
-    if (h->viscous) return fail(h, ZJ_ERR_UNSUPPORTED, "the adjoint of the Navier-Stokes terms is not built");
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "residual VJP requested in the middle of a step");
     int rc = ensure_adjoint(h);
@@ -1130,13 +1309,15 @@ extern "C" int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double
         VjpArgs<double> a = vjp_args<double>(h);
         a.U_in = h->U; a.T_in = (const double*)h->T[0];
         a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (double*)tb;
-        e = launch_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->affine, s, h->textbook);
+        e = h->viscous ? launch_visc_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->vp, s)
+                       : launch_vjp<double>(h->P, MODE_RESIDUAL, a, h->ops64, h->affine, s, h->textbook);
         if (e == hipSuccess) e = complete_into<double>(h, h->Rres, tb, h->W, h->ops64, s);
     } else {
         VjpArgs<float> a = vjp_args<float>(h);
         a.U_in = h->U; a.T_in = (const float*)h->T[0];
         a.Ub_in = h->Rres; a.Ub_out = h->Rres; a.Tb_out = (float*)tb;
-        e = launch_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->affine, s, h->textbook);
+        e = h->viscous ? launch_visc_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->vp, s)
+                       : launch_vjp<float>(h->P, MODE_RESIDUAL, a, h->ops32, h->affine, s, h->textbook);
         if (e == hipSuccess) e = complete_into<float>(h, h->Rres, tb, h->W, h->ops32, s);
     }
     ZJ_HIP(h, e);

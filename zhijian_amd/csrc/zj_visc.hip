@@ -22,7 +22,9 @@
 // edges another tile reads; a neighbour in the same tile is read from LDS.
 #include "zj_kernels.hpp"
 
+#include "zj_adjoint.hpp"
 #include "zj_device.hpp"
+#include "zj_riemann_vjp.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -50,6 +52,70 @@ __device__ __forceinline__ void visc_flux(const R* U, const R* Qx, const R* Qy, 
     const R txx = R(2) * mu * dudx + lam * div, tyy = R(2) * mu * dvdy + lam * div, txy = mu * (dudy + dvdx);
     Fv[0] = R(0); Fv[1] = txx; Fv[2] = txy; Fv[3] = u * txx + v * txy + kappa * dTdx;
     Gv[0] = R(0); Gv[1] = txy; Gv[2] = tyy; Gv[3] = u * txy + v * tyy + kappa * dTdy;
+}
+
+// visc_flux backwards: Ub += (dF_v/dU)^T Fb + (dG_v/dU)^T Gb, and likewise
+// the cotangents of the gradient Qx, Qy.
+template <typename R>
+__device__ __forceinline__ void visc_flux_vjp(const R* U, const R* Qx, const R* Qy, R g, R mu, R kappa, const R* Fb,
+                                              const R* Gb, R* Ub, R* Qxb, R* Qyb) {
+    const R gm = g - R(1);
+    const R rho = U[0], ir = rcpR(rho), u = U[1] * ir, v = U[2] * ir;
+    const R dudx = (Qx[1] - u * Qx[0]) * ir, dudy = (Qy[1] - u * Qy[0]) * ir;
+    const R dvdx = (Qx[2] - v * Qx[0]) * ir, dvdy = (Qy[2] - v * Qy[0]) * ir;
+    const R q2 = u * u + v * v;
+    const R p = gm * rho * (U[3] * ir - R(0.5) * q2);
+    const R dpdx = gm * (Qx[3] - R(0.5) * Qx[0] * q2 - rho * (u * dudx + v * dvdx));
+    const R dpdy = gm * (Qy[3] - R(0.5) * Qy[0] * q2 - rho * (u * dudy + v * dvdy));
+    const R T = p * ir;
+    const R div = dudx + dvdy, lam = R(-2.0 / 3.0) * mu;
+    const R txx = R(2) * mu * dudx + lam * div, tyy = R(2) * mu * dvdy + lam * div, txy = mu * (dudy + dvdx);
+    // F_v = [0, txx, txy, u txx + v txy + kappa dT/dx], G_v = [0, txy, tyy, u txy + v tyy + kappa dT/dy]
+    const R txxb = Fb[1] + u * Fb[3], txyb = Fb[2] + v * Fb[3] + Gb[1] + u * Gb[3], tyyb = Gb[2] + v * Gb[3];
+    R ub = txx * Fb[3] + txy * Gb[3], vb = txy * Fb[3] + tyy * Gb[3];
+    const R dTdxb = kappa * Fb[3], dTdyb = kappa * Gb[3];
+    const R divb = lam * (txxb + tyyb);
+    R dudxb = R(2) * mu * txxb + divb, dvdyb = R(2) * mu * tyyb + divb, dudyb = mu * txyb, dvdxb = mu * txyb;
+    // dT/dx = (dp/dx - T Qx0) / rho, T = p / rho
+    const R dpdxb = dTdxb * ir, dpdyb = dTdyb * ir;
+    const R Tb = -(dTdxb * Qx[0] + dTdyb * Qy[0]) * ir;
+    R Qx0b = -dTdxb * T * ir, Qy0b = -dTdyb * T * ir;
+    R irb = dTdxb * (dpdx - T * Qx[0]) + dTdyb * (dpdy - T * Qy[0]);
+    const R pb = Tb * ir;
+    irb += Tb * p;
+    // dp/dx = (g-1) (Qx3 - Qx0 q2 / 2 - rho (u du/dx + v dv/dx))
+    Qxb[3] += gm * dpdxb;
+    Qyb[3] += gm * dpdyb;
+    Qx0b -= R(0.5) * gm * q2 * dpdxb;
+    Qy0b -= R(0.5) * gm * q2 * dpdyb;
+    R q2b = -R(0.5) * gm * (Qx[0] * dpdxb + Qy[0] * dpdyb);
+    R rhob = -gm * (dpdxb * (u * dudx + v * dvdx) + dpdyb * (u * dudy + v * dvdy));
+    ub -= gm * rho * (dpdxb * dudx + dpdyb * dudy);
+    vb -= gm * rho * (dpdxb * dvdx + dpdyb * dvdy);
+    dudxb -= gm * rho * u * dpdxb;
+    dvdxb -= gm * rho * v * dpdxb;
+    dudyb -= gm * rho * u * dpdyb;
+    dvdyb -= gm * rho * v * dpdyb;
+    // p = (g-1) rho (U3 / rho - q2 / 2)
+    rhob += pb * gm * (U[3] * ir - R(0.5) * q2);
+    Ub[3] += pb * gm * rho * ir;
+    irb += pb * gm * rho * U[3];
+    q2b -= R(0.5) * pb * gm * rho;
+    ub += R(2) * u * q2b;
+    vb += R(2) * v * q2b;
+    // velocity gradients (Q - u Q0) / rho
+    Qxb[1] += dudxb * ir; ub -= dudxb * Qx[0] * ir; Qx0b -= dudxb * u * ir; irb += dudxb * (Qx[1] - u * Qx[0]);
+    Qyb[1] += dudyb * ir; ub -= dudyb * Qy[0] * ir; Qy0b -= dudyb * u * ir; irb += dudyb * (Qy[1] - u * Qy[0]);
+    Qxb[2] += dvdxb * ir; vb -= dvdxb * Qx[0] * ir; Qx0b -= dvdxb * v * ir; irb += dvdxb * (Qx[2] - v * Qx[0]);
+    Qyb[2] += dvdyb * ir; vb -= dvdyb * Qy[0] * ir; Qy0b -= dvdyb * v * ir; irb += dvdyb * (Qy[2] - v * Qy[0]);
+    Qxb[0] += Qx0b;
+    Qyb[0] += Qy0b;
+    // u = U1 / rho, v = U2 / rho
+    Ub[1] += ub * ir;
+    Ub[2] += vb * ir;
+    irb += ub * U[1] + vb * U[2];
+    rhob -= irb * ir * ir;
+    Ub[0] += rhob;
 }
 
 // A tile's arrays.  G holds the gradient (x then y, each [var][point]), then
@@ -412,6 +478,607 @@ __global__ __launch_bounds__(BLOCK) void visc_trace_kernel(int E, const double* 
     }
 }
 
+// ------------------------------------------------------------ adjoint
+// Vector-Jacobian product of one BR2 stage (MODE_STAGE1..3) or of the
+// residual (MODE_RESIDUAL).  The forward values are recomputed as
+// visc_stage_kernel forms them; then, in reverse:
+//   R1  cotangent of the stage output: its partial value, plus the transposed
+//       traces of U and of grad U (through the transposed gradient) of the
+//       edge cotangents other tiles left; the RK combination
+//   R2  the correction transposed: the flux differences' cotangents
+//   R3  per point: the transposed divergence and traces of the contravariant
+//       fluxes, the metric, the Euler and viscous flux VJPs, which give the
+//       cotangent of the corrected gradient Q; its correction terms
+//       transposed onto the jumps
+//   R4  per face point, as the forward solves it (both sides solve each
+//       face): the Riemann and viscous common-flux VJPs, the BR2 gradients
+//       and jumps backwards, to this side's trace and gradient-trace
+//       cotangents and the partner's (LDS in the tile, HBM across tiles)
+//   R5  the transposed traces; the gradient's cotangent through the
+//       transposed gradient operator; the cotangent of q
+// Traces and their cotangents across tiles carry 12 words per face point:
+// U, dU/dx, dU/dy.
+template <typename R, int P>
+constexpr int visc_adj_epb() {
+    using Y = Lay<R, P>;
+    constexpr int full = BLOCK / ((P + 1) * (P + 1));
+    constexpr int cap = 65536 / ((6 * Y::UE + 9 * Y::TE) * (int)sizeof(R));  // LDS within 64 KiB
+    return cap < 1 ? 1 : (cap < full ? cap : full);
+}
+
+// Both traces and both BR2 lifting coefficients of one face point of
+// element el2 (edge k, point f), the partner read from LDS or HBM.
+template <typename R, int P>
+struct ViscFacePoint {
+    R own[NV], oth[NV], gxo[NV], gyo[NV], gxn[NV], gyn[NV];
+    R ax, ay, bx, by;
+};
+
+template <typename R, int P>
+__device__ __forceinline__ void visc_face_point(ViscFacePoint<R, P>& F, const R* sU, const R* sGx, const R* sGy,
+                                                const R* sTr, const R* sLm, const R* sLp, const R* T_in,
+                                                const R* Jinv, int e0, int epb, int el2, int e2, int k, int f,
+                                                const EdgeLink& L, R eta, R gLe, R gRe) {
+    using Y = Lay<R, P>;
+    constexpr int FP = P + 1;
+    const int nl = L.nbr - e0;
+    const bool inside = nl >= 0 && nl < epb;
+    const int oslot = P - f;
+    const TraceLine tl = trace_line<P>(k, f);
+    const TraceLine tn = trace_line<P>(L.nbr_edge, oslot);
+    const R* T = T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
+    const R* Jo = Jinv + (int64_t)e2 * 4;
+    const R* Jn = Jinv + (int64_t)L.nbr * 4;
+    const bool oxi = (k & 1) != 0, nxi = (L.nbr_edge & 1) != 0;
+    const R go = eta * ((k == 0 || k == 3) ? gLe : gRe);
+    const R gn = eta * ((L.nbr_edge == 0 || L.nbr_edge == 3) ? gLe : gRe);
+    F.ax = go * (oxi ? Jo[0] : Jo[2]); F.ay = go * (oxi ? Jo[1] : Jo[3]);
+    F.bx = gn * (nxi ? Jn[0] : Jn[2]); F.by = gn * (nxi ? Jn[1] : Jn[3]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        F.own[v] = sTr[Y::tr(el2, k, f, v)];
+        if (inside) {
+            F.oth[v] = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
+            F.gxn[v] = trace_point<R, P>(sGx + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+            F.gyn[v] = trace_point<R, P>(sGy + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+        } else {
+            F.oth[v] = T[v];
+            F.gxn[v] = T[NV + v];
+            F.gyn[v] = T[2 * NV + v];
+        }
+        F.gxo[v] = trace_point<R, P>(sGx + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+        F.gyo[v] = trace_point<R, P>(sGy + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+    }
+    (void)sU;
+}
+
+template <typename R, int P, int EPB, int MODE>
+__global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops,
+                                                         const ViscParams vp) {
+    using Y = Lay<R, P>;
+    constexpr bool INC = std::is_same<R, float>::value;
+    constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
+    constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
+    constexpr int NT = EPB * S, NF = EPB * 4 * FP;
+    static_assert(NT <= BLOCK, "tile exceeds the workgroup");
+    constexpr double ca = MODE == MODE_STAGE2 ? 0.75 : (MODE == MODE_STAGE3 ? 1.0 / 3.0 : 0.0);
+    constexpr double cb = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    constexpr double cc = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    __shared__ R sU[EPB * Y::UE], sGx[EPB * Y::UE], sGy[EPB * Y::UE];
+    __shared__ R sW1[EPB * Y::UE], sW2[EPB * Y::UE], sW3[EPB * Y::UE];
+    __shared__ R sTr[EPB * Y::TE], sJ[EPB * Y::TE], sFI[EPB * Y::TE];
+    __shared__ R sAu[EPB * Y::TE], sAx[EPB * Y::TE], sAy[EPB * Y::TE];  // incoming, then own-side cotangents
+    __shared__ R sNu[EPB * Y::TE], sNx[EPB * Y::TE], sNy[EPB * Y::TE];  // cotangents from in-tile partners
+    __shared__ R sD[N1 * N1], sLm[N1], sLp[N1], sC[2 * N1];
+    __shared__ unsigned char sIn[EPB * 4];
+    __shared__ double sRed[BLOCK / 64];
+
+    const int tid = threadIdx.x;
+    const int tile = tile_of_block(a.xcd_groups);
+    const int e0 = tile * EPB;
+    const int64_t PS = (int64_t)a.E * S;
+    const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
+    const R gLe = R(vp.gLe), gRe = R(vp.gRe);
+    for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
+    if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; sC[tid] = ops.gL[tid]; sC[N1 + tid] = ops.gR[tid]; }
+
+    const int el = tid / S, sp = tid - (tid / S) * S;
+    const int e = e0 + el;
+    const bool active = tid < NT && e < a.E;
+    const int64_t gi = (int64_t)e * S + sp;
+    const int ub = el * Y::UE + sp;
+    const int i = sp % N1, j = sp / N1;
+    const bool incoming = MODE != MODE_RESIDUAL && a.Tb_in != nullptr;
+    // the forward residual is needed only for the dt cotangent
+    const bool need_res = MODE != MODE_RESIDUAL && a.dtb_partial != nullptr;
+
+    // ---- forward A: stage input
+    double q[NV], u0[NV];
+    R jx = R(0), jy = R(0), kx = R(0), ky = R(0);
+    if (active) {
+        jx = a.Jinv[(int64_t)e * 4]; jy = a.Jinv[(int64_t)e * 4 + 1];
+        kx = a.Jinv[(int64_t)e * 4 + 2]; ky = a.Jinv[(int64_t)e * 4 + 3];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            u0[v] = HAS_U0 ? a.U0[v * PS + gi] : 0.0;
+            q[v] = (INC && HAS_U0) ? u0[v] + double(a.dW_in[v * PS + gi]) : a.U_in[v * PS + gi];
+            sU[ub + v * Y::UV] = R(q[v]);
+        }
+    }
+    __syncthreads();
+
+    // ---- forward B1: gradient, own traces; incoming edge cotangents
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            point_gradient<R, P>(sU + el * Y::UE + v * Y::UV, sD, i, j, jx, jy, kx, ky, sGx[ub + v * Y::UV],
+                                 sGy[ub + v * Y::UV]);
+    }
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const TraceLine tl = trace_line<P>(k, f);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            sTr[Y::tr(el2, k, f, v)] = trace_point<R, P>(sU + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        const int nl = L.nbr - e0;
+        const bool inside = nl >= 0 && nl < EPB;
+        if (f == 0) sIn[el2 * 4 + k] = inside;
+        R tb[TW];
+#pragma unroll
+        for (int w = 0; w < TW; ++w) tb[w] = R(0);
+        if (incoming && !inside) {
+            const R* src = a.Tb_in + (((int64_t)e2 * 4 + k) * FP + f) * TW;
+            load4(src, tb);
+            load4(src + NV, tb + NV);
+            load4(src + 2 * NV, tb + 2 * NV);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sAu[Y::tr(el2, k, f, v)] = tb[v];
+            sAx[Y::tr(el2, k, f, v)] = tb[NV + v];
+            sAy[Y::tr(el2, k, f, v)] = tb[2 * NV + v];
+        }
+    }
+    __syncthreads();
+
+    // ---- forward B2: jumps, common fluxes (viscous and Riemann), outward and scaled
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+        ViscFacePoint<R, P> F;
+        visc_face_point<R, P>(F, sU, sGx, sGy, sTr, sLm, sLp, a.T_in, a.Jinv, e0, EPB, el2, e2, k, f, L, eta, gLe,
+                              gRe);
+        R qx[NV], qy[NV], Um[NV], Fv[NV], Gv[NV], UL[NV], UR[NV], Fc[NV];
+        const bool right = L.role == ROLE_RIGHT;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R uc = R(0.5) * (F.own[v] + F.oth[v]);
+            const R jo = uc - F.own[v], jn = uc - F.oth[v];
+            sJ[Y::tr(el2, k, f, v)] = jo;
+            qx[v] = R(0.5) * ((F.gxo[v] + F.ax * jo) + (F.gxn[v] + F.bx * jn));
+            qy[v] = R(0.5) * ((F.gyo[v] + F.ay * jo) + (F.gyn[v] + F.by * jn));
+            Um[v] = uc;
+            UL[v] = right ? F.oth[v] : F.own[v];
+            UR[v] = right ? F.own[v] : F.oth[v];
+        }
+        if (!need_res) continue;  // the jumps are all the reverse phases use
+        visc_flux(Um, qx, qy, g, mu, kappa, Fv, Gv);
+        riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
+        const R ef = a.edge_f[(int64_t)e2 * 4 + k];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R fc = Fc[v] - (Fv[v] * nx + Gv[v] * ny);
+            sFI[Y::tr(el2, k, f, v)] = ef * (right ? -fc : fc);
+        }
+    }
+    __syncthreads();
+
+    // ---- forward C: corrected gradient; contravariant total fluxes
+    auto corrected = [&](int v, R& Qx, R& Qy) {
+        const R cxi = sC[i] * sJ[Y::tr(el, 3, P - j, v)] + sC[N1 + i] * sJ[Y::tr(el, 1, j, v)];
+        const R ceta = sC[j] * sJ[Y::tr(el, 0, i, v)] + sC[N1 + j] * sJ[Y::tr(el, 2, P - i, v)];
+        Qx = sGx[ub + v * Y::UV] + jx * cxi + kx * ceta;
+        Qy = sGy[ub + v * Y::UV] + jy * cxi + ky * ceta;
+    };
+    if (active && need_res) {
+        R Uq[NV], Qx[NV], Qy[NV], fx[NV], fy[NV], Fv[NV], Gv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Uq[v] = sU[ub + v * Y::UV];
+            corrected(v, Qx[v], Qy[v]);
+        }
+        euler_flux(Uq, g, fx, fy);
+        visc_flux(Uq, Qx, Qy, g, mu, kappa, Fv, Gv);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R Fx = fx[v] - Fv[v], Fy = fy[v] - Gv[v];
+            sW1[ub + v * Y::UV] = jx * Fx + jy * Fy;
+            sW2[ub + v * Y::UV] = kx * Fx + ky * Fy;
+        }
+    }
+    __syncthreads();
+
+    // ---- forward D: flux differences (in place of the common fluxes)
+    for (int t = tid; need_res && t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        if (e0 + el2 >= a.E) continue;
+        const TraceLine tl = trace_line<P>(k, f);
+        const R* Fb = ((k & 1) ? sW1 : sW2) + el2 * Y::UE;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            sFI[Y::tr(el2, k, f, v)] = sFI[Y::tr(el2, k, f, v)] - trace_point<R, P>(Fb + v * Y::UV, tl, sLm, sLp);
+    }
+    __syncthreads();
+
+    // ---- forward E: residual
+    double res[NV] = {0.0, 0.0, 0.0, 0.0};
+    if (active && need_res) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R* Fx = sW1 + el * Y::UE + v * Y::UV;
+            const R* Fy = sW2 + el * Y::UE + v * Y::UV;
+            R dxi = 0, deta = 0;
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                dxi += sD[i * N1 + m] * Fx[j * N1 + m];
+                deta += sD[j * N1 + m] * Fy[m * N1 + i];
+            }
+            const R c = sFI[Y::tr(el, 3, P - j, v)] * sC[i] + sFI[Y::tr(el, 1, j, v)] * sC[N1 + i] +
+                        sFI[Y::tr(el, 0, i, v)] * sC[j] + sFI[Y::tr(el, 2, P - i, v)] * sC[N1 + j];
+            res[v] = double(-(dxi + deta + c));
+        }
+    }
+    __syncthreads();  // the fluxes in sW1, sW2 are read
+
+    // transposed traces (edges 0..3 at this point's slots) of a trace array
+    auto trace_t = [&](const R* A, int v) -> R {
+        return sLm[j] * A[Y::tr(el, 0, i, v)] + sLp[j] * A[Y::tr(el, 2, P - i, v)] +
+               sLp[i] * A[Y::tr(el, 1, j, v)] + sLm[i] * A[Y::tr(el, 3, P - j, v)];
+    };
+    // transposed gradient at this point of the (dxi, deta) cotangents in sW1, sW2
+    auto grad_t = [&](int v) -> R {
+        const R* X = sW1 + el * Y::UE + v * Y::UV;
+        const R* H = sW2 + el * Y::UE + v * Y::UV;
+        R acc = R(0);
+#pragma unroll
+        for (int m = 0; m < N1; ++m) acc += sD[m * N1 + i] * X[j * N1 + m] + sD[m * N1 + j] * H[m * N1 + i];
+        return acc;
+    };
+
+    // ---- R1a: incoming gradient-trace cotangents onto the reference derivatives
+    if (incoming && active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R gbx = trace_t(sAx, v), gby = trace_t(sAy, v);
+            sW1[ub + v * Y::UV] = jx * gbx + jy * gby;
+            sW2[ub + v * Y::UV] = kx * gbx + ky * gby;
+        }
+    }
+    __syncthreads();
+
+    // ---- R1b: cotangent of the stage output, RK combination
+    double rsum = 0.0;
+    double qb[NV] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double rb;
+            if (MODE == MODE_RESIDUAL) {
+                rb = a.Ub_in[v * PS + gi];
+            } else {
+                const double dt = *a.dt_ptr;
+                double unb = a.Ub_in[v * PS + gi];
+                if (incoming) unb += double(trace_t(sAu, v)) + double(grad_t(v));
+                if (MODE == MODE_STAGE3) a.U0b[v * PS + gi] = ca * unb;
+                if (MODE == MODE_STAGE2) a.U0b[v * PS + gi] += ca * unb;
+                qb[v] = cb * unb;
+                rb = cc * dt * unb;
+                rsum += cc * res[v] * unb;
+            }
+            sW3[ub + v * Y::UV] = R(rb);
+        }
+    }
+    if (a.dtb_partial) {
+        const double s = block_reduce<OpSum>(rsum, sRed);
+        if (tid == 0) a.dtb_partial[tile] = s;
+    }
+    __syncthreads();
+
+    // ---- R2: cotangents of the flux differences (in place)
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        if (e0 + el2 >= a.E) continue;
+        const R* Rb = sW3 + el2 * Y::UE;
+        const bool odd = (k & 1) != 0;
+        const int line = (k == 0 || k == 1) ? f : P - f;
+        const R* gw = sC + ((k == 1 || k == 2) ? N1 : 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R acc = R(0);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) acc -= gw[m] * Rb[v * Y::UV + (odd ? line * N1 + m : m * N1 + line)];
+            sFI[Y::tr(el2, k, f, v)] = acc;
+        }
+    }
+    __syncthreads();
+
+    // ---- R3: fluxes at the points backwards; the corrected gradient's cotangent
+    R Qxb[NV] = {R(0), R(0), R(0), R(0)}, Qyb[NV] = {R(0), R(0), R(0), R(0)};
+    double musum = 0.0;  // <cotangent of F_v, F_v> over this thread's points and face points
+    if (active) {
+        const R* rbp = sW3 + el * Y::UE;
+        R Uq[NV], Qx[NV], Qy[NV], Fxb[NV], Fyb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Uq[v] = sU[ub + v * Y::UV];
+            corrected(v, Qx[v], Qy[v]);
+            const R* rv = rbp + v * Y::UV;
+            // the flux traces enter the differences negatively
+            R ftb = -(sLp[i] * sFI[Y::tr(el, 1, j, v)] + sLm[i] * sFI[Y::tr(el, 3, P - j, v)]);
+            R gtb = -(sLm[j] * sFI[Y::tr(el, 0, i, v)] + sLp[j] * sFI[Y::tr(el, 2, P - i, v)]);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                ftb -= sD[m * N1 + i] * rv[j * N1 + m];
+                gtb -= sD[m * N1 + j] * rv[m * N1 + i];
+            }
+            Fxb[v] = jx * ftb + kx * gtb;
+            Fyb[v] = jy * ftb + ky * gtb;
+        }
+        R ubq[NV] = {R(0), R(0), R(0), R(0)}, nFx[NV], nFy[NV];
+        euler_flux_vjp(Uq, g, Fxb, Fyb, ubq);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { nFx[v] = -Fxb[v]; nFy[v] = -Fyb[v]; }
+        visc_flux_vjp(Uq, Qx, Qy, g, mu, kappa, nFx, nFy, ubq, Qxb, Qyb);
+        if (a.mub_partial) {  // F_v is proportional to mu (kappa too): dF_v/dmu = F_v / mu
+            R Fv[NV], Gv[NV];
+            visc_flux(Uq, Qx, Qy, g, mu, kappa, Fv, Gv);
+#pragma unroll
+            for (int v = 1; v < NV; ++v) musum += double(nFx[v] * Fv[v] + nFy[v] * Gv[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            qb[v] += double(ubq[v]);
+            sW1[ub + v * Y::UV] = jx * Qxb[v] + jy * Qyb[v];  // cotangent of the xi correction term
+            sW2[ub + v * Y::UV] = kx * Qxb[v] + ky * Qyb[v];  // and of the eta one
+        }
+    }
+    __syncthreads();
+
+    // ---- R4: face points backwards
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+        const int nl = L.nbr - e0;
+        const bool inside = nl >= 0 && nl < EPB;
+        const int oslot = P - f;
+        // the jump's cotangent from the correction terms of the points on this slot's line
+        const bool odd = (k & 1) != 0;
+        const int line = (k == 0 || k == 1) ? f : P - f;
+        const R* gw = sC + ((k == 1 || k == 2) ? N1 : 0);
+        const R* Cb = (odd ? sW1 : sW2) + el2 * Y::UE;
+        ViscFacePoint<R, P> F;
+        visc_face_point<R, P>(F, sU, sGx, sGy, sTr, sLm, sLp, a.T_in, a.Jinv, e0, EPB, el2, e2, k, f, L, eta, gLe,
+                              gRe);
+        const bool right = L.role == ROLE_RIGHT;
+        R Jb[NV], uc[NV], jo[NV], jn[NV], qx[NV], qy[NV], UL[NV], UR[NV], Fcb[NV], Fvb[NV], Gvb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            R acc = R(0);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) acc += gw[m] * Cb[v * Y::UV + (odd ? line * N1 + m : m * N1 + line)];
+            Jb[v] = acc;
+            uc[v] = R(0.5) * (F.own[v] + F.oth[v]);
+            jo[v] = uc[v] - F.own[v];
+            jn[v] = uc[v] - F.oth[v];
+            qx[v] = R(0.5) * ((F.gxo[v] + F.ax * jo[v]) + (F.gxn[v] + F.bx * jn[v]));
+            qy[v] = R(0.5) * ((F.gyo[v] + F.ay * jo[v]) + (F.gyn[v] + F.by * jn[v]));
+            UL[v] = right ? F.oth[v] : F.own[v];
+            UR[v] = right ? F.own[v] : F.oth[v];
+        }
+        const R ef = a.edge_f[(int64_t)e2 * 4 + k];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R d = sFI[Y::tr(el2, k, f, v)];
+            Fcb[v] = ef * (right ? -d : d);
+            Fvb[v] = -Fcb[v] * nx;
+            Gvb[v] = -Fcb[v] * ny;
+        }
+        R ULb[NV] = {R(0), R(0), R(0), R(0)}, URb[NV] = {R(0), R(0), R(0), R(0)};
+        riemann_by_kind_vjp(a.riemann, UL, UR, nx, ny, g, Fcb, ULb, URb);
+        R ucb[NV] = {R(0), R(0), R(0), R(0)}, qxb[NV] = {R(0), R(0), R(0), R(0)}, qyb[NV] = {R(0), R(0), R(0), R(0)};
+        visc_flux_vjp(uc, qx, qy, g, mu, kappa, Fvb, Gvb, ucb, qxb, qyb);
+        if (a.mub_partial) {
+            R Fv[NV], Gv[NV];
+            visc_flux(uc, qx, qy, g, mu, kappa, Fv, Gv);
+#pragma unroll
+            for (int v = 1; v < NV; ++v) musum += double(Fvb[v] * Fv[v] + Gvb[v] * Gv[v]);
+        }
+        R nb[TW];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R job = Jb[v] + R(0.5) * (F.ax * qxb[v] + F.ay * qyb[v]);
+            const R jnb = R(0.5) * (F.bx * qxb[v] + F.by * qyb[v]);
+            const R uct = ucb[v] + job + jnb;
+            const R ownb = (right ? URb[v] : ULb[v]) + R(0.5) * uct - job;
+            nb[v] = (right ? ULb[v] : URb[v]) + R(0.5) * uct - jnb;
+            nb[NV + v] = R(0.5) * qxb[v];
+            nb[2 * NV + v] = R(0.5) * qyb[v];
+            sAu[Y::tr(el2, k, f, v)] = ownb;
+            sAx[Y::tr(el2, k, f, v)] = R(0.5) * qxb[v];
+            sAy[Y::tr(el2, k, f, v)] = R(0.5) * qyb[v];
+            if (inside) {
+                sNu[Y::tr(nl, L.nbr_edge, oslot, v)] = nb[v];
+                sNx[Y::tr(nl, L.nbr_edge, oslot, v)] = nb[NV + v];
+                sNy[Y::tr(nl, L.nbr_edge, oslot, v)] = nb[2 * NV + v];
+            }
+        }
+        if (!inside) {
+            R* dst = a.Tb_out + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
+            store4(dst, nb);
+            store4(dst + NV, nb + NV);
+            store4(dst + 2 * NV, nb + 2 * NV);
+        }
+    }
+    __syncthreads();
+
+    // ---- R5: transposed traces of U and grad U; the gradient's cotangent onto
+    // the reference derivatives
+    if (active) {
+        // a partner's slot is read only where the partner is in this tile
+        // (elsewhere it holds no value: selected, never multiplied by 0)
+        const bool in0 = sIn[el * 4 + 0], in1 = sIn[el * 4 + 1], in2 = sIn[el * 4 + 2], in3 = sIn[el * 4 + 3];
+        auto both_t = [&](const R* A, const R* N, int v) -> R {
+            const int s0 = Y::tr(el, 0, i, v), s1 = Y::tr(el, 1, j, v);
+            const int s2 = Y::tr(el, 2, P - i, v), s3 = Y::tr(el, 3, P - j, v);
+            return sLm[j] * (A[s0] + (in0 ? N[s0] : R(0))) + sLp[j] * (A[s2] + (in2 ? N[s2] : R(0))) +
+                   sLp[i] * (A[s1] + (in1 ? N[s1] : R(0))) + sLm[i] * (A[s3] + (in3 ? N[s3] : R(0)));
+        };
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            qb[v] += double(both_t(sAu, sNu, v));
+            const R gbx = Qxb[v] + both_t(sAx, sNx, v), gby = Qyb[v] + both_t(sAy, sNy, v);
+            sW3[ub + v * Y::UV] = jx * gbx + jy * gby;
+            sGx[ub + v * Y::UV] = kx * gbx + ky * gby;
+        }
+    }
+    __syncthreads();
+
+    if (a.mub_partial) {
+        const double sm = block_reduce<OpSum>(musum, sRed);
+        if (tid == 0) a.mub_partial[tile] = sm;
+    }
+
+    // ---- R6: transposed gradient operator; cotangent of q
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R* X = sW3 + el * Y::UE + v * Y::UV;
+            const R* H = sGx + el * Y::UE + v * Y::UV;
+            R acc = R(0);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) acc += sD[m * N1 + i] * X[j * N1 + m] + sD[m * N1 + j] * H[m * N1 + i];
+            double out = qb[v] + double(acc);
+            if (MODE == MODE_STAGE1) out += a.U0b[v * PS + gi];
+            a.Ub_out[v * PS + gi] = out;
+        }
+    }
+}
+
+// Completion of a BR2 cotangent: U_bar plus the transposed traces of U and
+// grad U of the cross-tile edge cotangents the last adjoint launch left.
+template <typename R, int P, int EPB>
+__global__ __launch_bounds__(BLOCK) void visc_complete_kernel(const CompleteArgs<R> a, const R* __restrict__ Jinv,
+                                                              const OpTables<R> ops) {
+    using Y = Lay<R, P>;
+    constexpr int N1 = P + 1, S = N1 * N1, FP = N1, NT = EPB * S, NF = EPB * 4 * FP;
+    __shared__ R sAu[EPB * Y::TE], sAx[EPB * Y::TE], sAy[EPB * Y::TE];
+    __shared__ R sX[EPB * Y::UE], sH[EPB * Y::UE];
+    __shared__ R sD[N1 * N1], sLm[N1], sLp[N1];
+    const int tid = threadIdx.x;
+    const int e0 = tile_of_block(a.xcd_groups) * EPB;
+    const int64_t PS = (int64_t)a.E * S;
+    for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
+    if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; }
+    for (int t = tid; t < NF; t += BLOCK) {
+        const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
+        const int e2 = e0 + el2;
+        if (e2 >= a.E) continue;
+        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+        const int nl = L.nbr - e0;
+        R tb[TW];
+#pragma unroll
+        for (int w = 0; w < TW; ++w) tb[w] = R(0);
+        if (a.Tb && !(nl >= 0 && nl < EPB)) {
+            const R* src = a.Tb + (((int64_t)e2 * 4 + k) * FP + f) * TW;
+            load4(src, tb);
+            load4(src + NV, tb + NV);
+            load4(src + 2 * NV, tb + 2 * NV);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sAu[Y::tr(el2, k, f, v)] = tb[v];
+            sAx[Y::tr(el2, k, f, v)] = tb[NV + v];
+            sAy[Y::tr(el2, k, f, v)] = tb[2 * NV + v];
+        }
+    }
+    __syncthreads();
+    const int el = tid / S, sp = tid - (tid / S) * S;
+    const int e = e0 + el;
+    const bool active = tid < NT && e < a.E;
+    const int ub = el * Y::UE + sp;
+    const int i = sp % N1, j = sp / N1;
+    auto trace_t = [&](const R* A, int v) -> R {
+        return sLm[j] * A[Y::tr(el, 0, i, v)] + sLp[j] * A[Y::tr(el, 2, P - i, v)] +
+               sLp[i] * A[Y::tr(el, 1, j, v)] + sLm[i] * A[Y::tr(el, 3, P - j, v)];
+    };
+    if (active) {
+        const R jx = Jinv[(int64_t)e * 4], jy = Jinv[(int64_t)e * 4 + 1];
+        const R kx = Jinv[(int64_t)e * 4 + 2], ky = Jinv[(int64_t)e * 4 + 3];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R gbx = trace_t(sAx, v), gby = trace_t(sAy, v);
+            sX[ub + v * Y::UV] = jx * gbx + jy * gby;
+            sH[ub + v * Y::UV] = kx * gbx + ky * gby;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int64_t gi = (int64_t)e * S + sp;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const R* X = sX + el * Y::UE + v * Y::UV;
+        const R* H = sH + el * Y::UE + v * Y::UV;
+        R acc = trace_t(sAu, v);
+#pragma unroll
+        for (int m = 0; m < N1; ++m) acc += sD[m * N1 + i] * X[j * N1 + m] + sD[m * N1 + j] * H[m * N1 + i];
+        a.out[v * PS + gi] = a.Ub[v * PS + gi] + double(acc);
+    }
+}
+
+template <typename R, int P>
+hipError_t visc_adj_stage_p(int mode, const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                            hipStream_t s) {
+    constexpr int EPB = visc_adj_epb<R, P>();
+    const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
+    switch (mode) {
+    case MODE_STAGE1: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE1>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    case MODE_STAGE2: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE2>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    case MODE_STAGE3: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE3>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int P>
+hipError_t visc_vjp_p(int mode, const VjpArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp, hipStream_t s) {
+    constexpr int EPB = visc_adj_epb<R, P>();
+    const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
+    switch (mode) {
+    case MODE_RESIDUAL: visc_vjp_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    case MODE_STAGE1: visc_vjp_kernel<R, P, EPB, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    case MODE_STAGE2: visc_vjp_kernel<R, P, EPB, MODE_STAGE2><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    default: visc_vjp_kernel<R, P, EPB, MODE_STAGE3><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int P>
+hipError_t visc_complete_p(const CompleteArgs<R>& a, const R* Jinv, const OpTables<R>& ops, hipStream_t s) {
+    constexpr int EPB = visc_adj_epb<R, P>();
+    const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
+    visc_complete_kernel<R, P, EPB><<<grid, BLOCK, 0, s>>>(a, Jinv, ops);
+    return hipGetLastError();
+}
+
 template <typename R, int P>
 hipError_t visc_launch_p(int mode, const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
                          hipStream_t s) {
@@ -438,6 +1105,67 @@ hipError_t visc_trace_p(int E, const double* U, const R* Jinv, R* T, const OpTab
 }  // namespace
 
 int visc_epb(int P) { return BLOCK / ((P + 1) * (P + 1)); }
+
+int visc_adjoint_tile(int P, bool fp64) {
+    switch (P) {
+    case 1: return fp64 ? visc_adj_epb<double, 1>() : visc_adj_epb<float, 1>();
+    case 2: return fp64 ? visc_adj_epb<double, 2>() : visc_adj_epb<float, 2>();
+    case 3: return fp64 ? visc_adj_epb<double, 3>() : visc_adj_epb<float, 3>();
+    case 4: return fp64 ? visc_adj_epb<double, 4>() : visc_adj_epb<float, 4>();
+    case 5: return fp64 ? visc_adj_epb<double, 5>() : visc_adj_epb<float, 5>();
+    default: return 0;
+    }
+}
+
+#define ZJ_VISC_P(call)                 \
+    switch (P) {                         \
+    case 1: return call(1);              \
+    case 2: return call(2);              \
+    case 3: return call(3);              \
+    case 4: return call(4);              \
+    case 5: return call(5);              \
+    default: return hipErrorInvalidValue; \
+    }
+
+template <typename R>
+hipError_t launch_visc_stage_adj(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops,
+                                 const ViscParams& vp, hipStream_t s) {
+#define ZJ_C(PP) visc_adj_stage_p<R, PP>(mode, a, ops, vp, s)
+    ZJ_VISC_P(ZJ_C)
+#undef ZJ_C
+}
+
+template <typename R>
+hipError_t launch_visc_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                           hipStream_t s) {
+    if (a.E <= 0) return hipSuccess;
+#define ZJ_C(PP) visc_vjp_p<R, PP>(mode, a, ops, vp, s)
+    ZJ_VISC_P(ZJ_C)
+#undef ZJ_C
+}
+
+template <typename R>
+hipError_t launch_visc_complete(int P, const CompleteArgs<R>& a, const R* Jinv_e, const OpTables<R>& ops,
+                                hipStream_t s) {
+    if (a.E <= 0) return hipSuccess;
+#define ZJ_C(PP) visc_complete_p<R, PP>(a, Jinv_e, ops, s)
+    ZJ_VISC_P(ZJ_C)
+#undef ZJ_C
+}
+#undef ZJ_VISC_P
+
+template hipError_t launch_visc_stage_adj<double>(int, int, const StageArgs<double>&, const OpTables<double>&,
+                                                  const ViscParams&, hipStream_t);
+template hipError_t launch_visc_stage_adj<float>(int, int, const StageArgs<float>&, const OpTables<float>&,
+                                                 const ViscParams&, hipStream_t);
+template hipError_t launch_visc_vjp<double>(int, int, const VjpArgs<double>&, const OpTables<double>&,
+                                            const ViscParams&, hipStream_t);
+template hipError_t launch_visc_vjp<float>(int, int, const VjpArgs<float>&, const OpTables<float>&,
+                                           const ViscParams&, hipStream_t);
+template hipError_t launch_visc_complete<double>(int, const CompleteArgs<double>&, const double*,
+                                                 const OpTables<double>&, hipStream_t);
+template hipError_t launch_visc_complete<float>(int, const CompleteArgs<float>&, const float*,
+                                                const OpTables<float>&, hipStream_t);
 
 template <typename R>
 hipError_t launch_visc_stage(int P, int mode, const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,

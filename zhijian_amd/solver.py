@@ -285,6 +285,26 @@ class Solver:
         self._chk(lib().zj_adjoint_get(self._h, out))
         return out
 
+    def forward_checkpointed(self, dts, every, stream=None):
+        """len(dts) steps, U^{c*every} kept in state slot c (zj_forward_checkpointed)."""
+        d = np.ascontiguousarray(dts, np.float64)
+        self._chk(lib().zj_forward_checkpointed(self._h, d, d.size, int(every), stream))
+
+    def adjoint_sweep(self, dts, every, want_dt=True, stream=None):
+        """Every step backwards from the seeded cotangent (zj_adjoint_sweep);
+        returns dJ/d(dt_n) per step, or None with want_dt=False (asynchronous)."""
+        d = np.ascontiguousarray(dts, np.float64)
+        out = np.zeros(d.size) if want_dt else None
+        self._chk(lib().zj_adjoint_sweep(self._h, d, d.size, int(every), None if out is None else out.ctypes.data,
+                                         stream))
+        return None if out is None else [float(x) for x in out]
+
+    def adjoint_get_re(self):
+        """dJ/dRe accumulated since adjoint_set (Navier-Stokes terms; 0 without)."""
+        v = C.c_double()
+        self._chk(lib().zj_adjoint_get_re(self._h, C.byref(v)))
+        return v.value
+
     def rk3_step_vjp(self, dt, stream=None, want_dt=True):
         """Maps the held cotangent of U^{n+1} to that of U^n (the state set on
         the handle); returns dJ/d(dt) (synchronising), or None with
