@@ -320,26 +320,29 @@ def time_viscous(tables, U0, steps, warmup, err_steps, stream, hiprt, Re=1600.0)
     return out
 
 
-def tri_alg_bytes_per_stage(E, p, stage, precision, cross_edges):
+def tri_alg_bytes_per_stage(E, p, stage, precision, cross_edges, viscous=False):
     """Algorithmic HBM bytes of one triangle stage launch: the FP64 stage state
     per solution point (stage 1 reads U^n and writes U1; stages 2 and 3 read
     U^n and the intermediate state and write one), 10 geometry words per
     element (affine metric, three face normals), 3 int2 links, and the
     (p+1) x 4 trace of every element edge whose neighbour lies in another tile,
-    written once and read once in the compute precision."""
+    written once and read once in the compute precision (with the
+    Navier-Stokes terms (p+1) x 12: U, dU/dx, dU/dy)."""
     Np = (p + 1) * (p + 2) // 2
     w = 8 if precision == "fp64" else 4
     per_pt = 64 if stage == 1 else 96
-    return E * (Np * per_pt + 10 * w + 24) + cross_edges * (p + 1) * 4 * w * 2
+    tw = 12 if viscous else 4
+    return E * (Np * per_pt + 10 * w + 24) + cross_edges * (p + 1) * tw * w * 2
 
 
-def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
+def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds, Re=None):
     """Triangles (textbook FR, GA correction field, HLLC): n x n periodic
     unit square split along the cell diagonals, interior nodes jittered
     U(-0.2h, 0.2h) by PCG64(2024); Mach 0.5 at 2 degrees with 8 Fourier
-    velocity modes of amplitude 0.05 (PCG64(2024)).  FP64 vs FP32-bulk time
-    per step, the FP32 error after err_steps steps, and the triangle oracle
-    (numpy, one core) on a bounded sample of the same workload."""
+    velocity modes of amplitude 0.05 (PCG64(2024)); with the Navier-Stokes
+    terms (BR2) at Re when given.  FP64 vs FP32-bulk time per step, the FP32
+    error after err_steps steps, and the triangle oracle (numpy, one core) on
+    a bounded sample of the same workload."""
     from zhijian_amd import cases
     from zhijian_amd.tri import TriSolver, sp_coords
     m = cases.split_tri_mesh(n, jitter=0.2, seed=2024)
@@ -348,12 +351,13 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
     dof = m.n_elem * Np * 4
     out = {"workload": f"{n}x{n} split-quad periodic triangles ({m.n_elem} elements), P{p} ({Np} points), "
                        f"jitter 0.2h seed 2024, Mach 0.5 at 2 deg + 8 Fourier modes (amp 0.05, seed 2024), "
-                       f"textbook FR (GA correction, energy-stable family on triangles), HLLC, inviscid, "
-                       f"no limiter, CFL 0.5",
+                       f"textbook FR (GA correction, energy-stable family on triangles), HLLC, "
+                       f"{f'Navier-Stokes (BR2) Re {Re:g}' if Re else 'inviscid'}, no limiter, CFL 0.5",
            "elements": m.n_elem, "dof": dof}
     sols = {}
     for name in ("fp64", "fp32bulk"):
-        s = TriSolver(m.xy, m.tris, p, periodic=(1.0, 1.0), riemann="hllc", precision=name, corr="ga")
+        visc = dict(viscous=True, Re=Re) if Re else {}
+        s = TriSolver(m.xy, m.tris, p, periodic=(1.0, 1.0), riemann="hllc", precision=name, corr="ga", **visc)
         s.set_state(U0)
         if name == "fp64":
             out["dt"] = dt = s.dt_cfl(0.5)
@@ -369,9 +373,10 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
         ms = e0.elapsed_ms(e1) / steps
         st = s.time_stages(dt, max(steps // 2, 5), stream.ptr)
         ce = s.info()["cross_tile_edges"]
-        by = sum(tri_alg_bytes_per_stage(m.n_elem, p, k, name, ce) for k in (1, 2, 3))
+        by = sum(tri_alg_bytes_per_stage(m.n_elem, p, k, name, ce, viscous=bool(Re)) for k in (1, 2, 3))
+        kern = "tri_visc_stage_kernel" if Re else "tri_stage_kernel"
         out[name] = dict(ms_per_step=ms, stage_ms=st, dof_updates_per_s=3.0 * dof / (ms * 1e-3),
-                         roofline=roofline_of(by, st, f"tri_stage_kernel<{'double' if name == 'fp64' else 'float'},P{p}>",
+                         roofline=roofline_of(by, st, f"{kern}<{'double' if name == 'fp64' else 'float'},P{p}>",
                                               cross_tile_edge_fraction=ce / (3.0 * m.n_elem)))
         sols[name] = s
     for s in sols.values():
@@ -392,6 +397,8 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds):
         ms_ = cases.split_tri_mesh(ns, jitter=0.2, seed=2024)
         faces, nrm = tri_faces(ms_.xy, ms_.tris, periodic=(1.0, 1.0))
         o = T.TriTextbook(p, ms_.xy, ms_.tris, faces, nrm, riemann="hllc", corr="ga")
+        if Re:
+            o.set_viscous(Re)
         U = cases.fourier_freestream_ic(sp_coords(ms_.xy, ms_.tris, p), seed=2024).reshape(-1)
         U = o.advance(U, dt)
         k, t0 = 0, time.perf_counter()
@@ -809,6 +816,9 @@ def main():
         tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.textbook_steps, stream, hiprt,
                        0.0 if args.no_cpu else 5.0)
         log(f"triangles: {tri}")
+        tri["navier_stokes"] = time_tri(args.tri_n, 3, args.steps, args.warmup, args.textbook_steps, stream, hiprt,
+                                        0.0 if args.no_cpu else 5.0, Re=5000.0)
+        log(f"triangles, Navier-Stokes: {tri['navier_stokes']}")
 
     vortex = time_vortex(10 * args.steps, args.warmup, stream, hiprt)
     log(f"vortex: {vortex}")

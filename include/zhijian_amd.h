@@ -354,7 +354,9 @@ int zj_tri_residual_device(zj_tri_handle h, double* d_dUdt_aos, void* stream);
 int zj_tri_rk3_step(zj_tri_handle h, double dt, void* stream);
 /* As zj_time_stages, for a triangle mesh. */
 int zj_tri_time_stages(zj_tri_handle h, double dt, int steps, double* stage_ms, void* stream);
-/* dt = cfl * d_e / ((2p+1) max(|u|+c)), d_e the inscribed-circle diameter */
+/* dt = cfl / max((2p+1)(|u|+c) / d_e + (p+1)^4 nu_k / (rho d_e^2)), d_e the
+ * inscribed-circle diameter; nu_k = mu max(4/3, gamma/Pr) with the viscous
+ * terms, else 0 */
 int zj_tri_dt_cfl(zj_tri_handle h, double cfl, double* dt);
 int zj_tri_residual_l2(zj_tri_handle h, double* l2);
 int zj_tri_synchronize(zj_tri_handle h);

@@ -286,6 +286,87 @@ class TriTextbook:
         self.bc_type = None if bc_type is None else np.asarray(bc_type)
         self.bc_face_map = None if bc_face_map is None else np.asarray(bc_face_map)
         self.bc_data = None if bc_data is None else np.asarray(bc_data, np.float64).reshape(-1)
+        self.viscous = False
+
+    def set_viscous(self, Re, Pr=0.72, eta=2.0):
+        """Navier-Stokes terms (BR2) with mu = 1/Re, kappa = mu gamma / ((gamma-1) Pr)
+        and BR2 penalty eta; periodic meshes only."""
+        if (self.faces[:, 2] < 0).any():
+            raise ValueError("the triangle viscous terms are built for meshes without boundary faces")
+        self.mu = 1.0 / Re
+        self.kappa = self.mu * self.g / ((self.g - 1.0) * Pr)
+        self.eta = eta
+        self.viscous = True
+        return self
+
+    def _visc_flux(self, U, Qx, Qy):
+        """ViscousFlux::fluxX/fluxY (navierstokes.hpp:53-99) with mu, kappa as
+        parameters, in the steps of oz_visc_flux, over arrays [..., 4]."""
+        g, mu, kappa = self.g, self.mu, self.kappa
+        rho = U[..., 0]
+        u, v = U[..., 1] / rho, U[..., 2] / rho
+        dudx, dudy = (Qx[..., 1] - u * Qx[..., 0]) / rho, (Qy[..., 1] - u * Qy[..., 0]) / rho
+        dvdx, dvdy = (Qx[..., 2] - v * Qx[..., 0]) / rho, (Qy[..., 2] - v * Qy[..., 0]) / rho
+        p = (g - 1.0) * rho * (U[..., 3] / rho - 0.5 * (u * u + v * v))
+        dpdx = (g - 1.0) * (Qx[..., 3] - 0.5 * Qx[..., 0] * (u * u + v * v) - rho * (u * dudx + v * dvdx))
+        dpdy = (g - 1.0) * (Qy[..., 3] - 0.5 * Qy[..., 0] * (u * u + v * v) - rho * (u * dudy + v * dvdy))
+        T = p / rho
+        dTdx, dTdy = (dpdx - T * Qx[..., 0]) / rho, (dpdy - T * Qy[..., 0]) / rho
+        div, lam = dudx + dvdy, -2.0 / 3.0 * mu
+        txx, tyy, txy = 2.0 * mu * dudx + lam * div, 2.0 * mu * dvdy + lam * div, mu * (dudy + dvdx)
+        z = np.zeros_like(rho)
+        Fv = np.stack([z, txx, txy, u * txx + v * txy + kappa * dTdx], -1)
+        Gv = np.stack([z, txy, tyy, u * txy + v * tyy + kappa * dTdy], -1)
+        return Fv, Gv
+
+    def gradient(self, U):
+        """Corrected gradient Q at the solution points ([E][Np][4] each of x, y):
+        grad U plus the lifting of the jumps U^C - U_own, U^C = (U_L + U_R) / 2,
+        along the outward normals (DG-equivalent FR applied to grad u)."""
+        return self._viscous_parts(U.reshape(self.E, self.Np, 4))[:2]
+
+    def _viscous_parts(self, U):
+        E, Fp, p = self.E, self.Fp, self.p
+        rx, ry, sx, sy, J = (self.met[:, k] for k in range(5))
+        Dr, Ds, I, LIFT = self.ops["Dr"], self.ops["Ds"], self.ops["interp"], self.ops["lift"]
+        Ur, Us = np.einsum("ij,ejv->eiv", Dr, U), np.einsum("ij,ejv->eiv", Ds, U)
+        gx = Ur * rx[:, None, None] + Us * sx[:, None, None]
+        gy = Ur * ry[:, None, None] + Us * sy[:, None, None]
+        Uf = np.einsum("ij,ejv->eiv", I, U).reshape(E, 3, Fp, 4)
+        f = self.faces
+        le, ll, re, rl = f[:, 0], f[:, 1], f[:, 2], f[:, 3]
+        fr = np.arange(Fp)
+        UL = Uf[le[:, None], ll[:, None], fr[None, :]]
+        UR = Uf[re[:, None], rl[:, None], (p - fr)[None, :]]
+        UC = 0.5 * (UL + UR)
+        jump = np.zeros((E, 3, Fp, 4))
+        jump[le[:, None], ll[:, None], fr[None, :]] = UC - UL
+        jump[re[:, None], rl[:, None], (p - fr)[None, :]] = UC - UR
+        nrm_out = np.zeros((E, 3, 2))
+        nrm_out[le, ll] = self.normals
+        nrm_out[re, rl] = -self.normals
+        sc = self.elen / (2 * J[:, None])                      # |e| / (2J) per element edge
+        wx = (jump * (sc * nrm_out[:, :, 0])[:, :, None, None]).reshape(E, 3 * Fp, 4)
+        wy = (jump * (sc * nrm_out[:, :, 1])[:, :, None, None]).reshape(E, 3 * Fp, 4)
+        Qx = gx + np.einsum("ij,ejv->eiv", LIFT, wx)
+        Qy = gy + np.einsum("ij,ejv->eiv", LIFT, wy)
+        # BR2: each side's gradient at a face point is its own trace of grad U
+        # plus eta times the lifting of this face's jumps alone, evaluated at
+        # the face's points (LF = interp . lift restricted to the face)
+        gxf = np.einsum("ij,ejv->eiv", I, gx).reshape(E, 3, Fp, 4)
+        gyf = np.einsum("ij,ejv->eiv", I, gy).reshape(E, 3, Fp, 4)
+        rxf, ryf = np.zeros_like(gxf), np.zeros_like(gyf)
+        for k in range(3):
+            LF = I[k * Fp:(k + 1) * Fp] @ LIFT[:, k * Fp:(k + 1) * Fp]
+            lj = np.einsum("ab,ebv->eav", LF, jump[:, k])
+            rxf[:, k] = self.eta * (sc[:, k] * nrm_out[:, k, 0])[:, None, None] * lj
+            ryf[:, k] = self.eta * (sc[:, k] * nrm_out[:, k, 1])[:, None, None] * lj
+        qx, qy = gxf + rxf, gyf + ryf
+        Qbx = 0.5 * (qx[le[:, None], ll[:, None], fr[None, :]] + qx[re[:, None], rl[:, None], (p - fr)[None, :]])
+        Qby = 0.5 * (qy[le[:, None], ll[:, None], fr[None, :]] + qy[re[:, None], rl[:, None], (p - fr)[None, :]])
+        Fvf, Gvf = self._visc_flux(UC, Qbx, Qby)
+        FvN = Fvf * self.normals[:, None, 0:1] + Gvf * self.normals[:, None, 1:2]   # along the left normal
+        return Qx, Qy, FvN
 
     def _flux(self, U):
         rho = U[..., 0]
@@ -299,6 +380,11 @@ class TriTextbook:
         E, Np, Fp, p = self.E, self.Np, self.Fp, self.p
         U = U.reshape(E, Np, 4)
         Fx, Fy = self._flux(U)
+        FvN = None
+        if self.viscous:  # total flux F_inv - F_v(U, Q) at the points
+            Qx, Qy, FvN = self._viscous_parts(U)
+            Fv, Gv = self._visc_flux(U, Qx, Qy)
+            Fx, Fy = Fx - Fv, Fy - Gv
         rx, ry, sx, sy, J = (self.met[:, k] for k in range(5))
         Dr, Ds, I, LIFT = self.ops["Dr"], self.ops["Ds"], self.ops["interp"], self.ops["lift"]
         div = (np.einsum("ij,ejv->eiv", Dr, Fx) * rx[:, None, None] + np.einsum("ij,ejv->eiv", Ds, Fx) * sx[:, None, None]
@@ -320,6 +406,8 @@ class TriTextbook:
                                           int(self.bc_type[fi]), self.bc_data, max(bci, 0), self.g)
         nrm = np.repeat(self.normals[:, None, :], Fp, axis=1)
         Fc = O.riemann_batch(list(O.RIEMANN)[self.kind], UL, UR, nrm, self.g).reshape(-1, Fp, 4)
+        if FvN is not None:  # common viscous normal flux F_v(U^C, Qbar) . n (BR2)
+            Fc = Fc - FvN
         Fout = np.zeros((E, 3, Fp, 4))
         Fout[le[:, None], ll[:, None], fr[None, :]] = Fc
         Fout[re[inner][:, None], rl[inner][:, None], (p - fr)[None, :]] = -Fc[inner]

@@ -213,3 +213,114 @@ def test_closed_box_conserves_mass(riemann):
     tot = np.einsum("e,ij,ej->", s.met[:, 4], M, R[..., 0])
     scale = np.einsum("e,ij,ej->", s.met[:, 4], np.abs(M), np.abs(R[..., 0]))
     assert abs(tot) <= 1e-12 * scale
+
+
+# ------------------------------------------------- Navier-Stokes terms (BR2)
+def test_visc_flux_matches_c_oracle():
+    """The numpy viscous flux against oz_visc_flux (itself pinned bit for bit
+    against the compiled reference's ViscousFlux with mu and kappa given)."""
+    s, _, _ = _case(2, 1)
+    s.set_viscous(100.0)
+    rng = np.random.default_rng(5)
+    for _ in range(16):
+        U = np.array([1.0, 0.3, -0.2, 2.5]) + 0.1 * rng.standard_normal(4)
+        Qx, Qy = rng.standard_normal(4), rng.standard_normal(4)
+        F, G = s._visc_flux(U[None], Qx[None], Qy[None])
+        Fc, Gc = O.visc_flux(U, Qx, Qy, s.mu, s.kappa)
+        assert np.allclose(F[0], Fc, rtol=1e-14, atol=1e-16) and np.allclose(G[0], Gc, rtol=1e-14, atol=1e-16)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_viscous_gradient_exact_inside(p):
+    """For a polynomial field of degree <= p every jump away from the periodic
+    seam is zero, so the corrected gradient equals the exact gradient on every
+    element whose edges are all interior (jittered mesh)."""
+    s, xy, tris = _case(6, p, jitter=0.2, seed=7)
+    s.set_viscous(100.0)
+    pos = T.sp_coords(xy, tris, s.ops)
+    x, y = pos[..., 0], pos[..., 1]
+    c = np.array([0.3, -0.7, 1.1, 0.5])
+    U = np.stack([c[v] * (x + 0.5 * y) ** p + (v + 1) * x * y ** (p - 1) + 2.0 for v in range(4)], -1)
+    dUx = np.stack([c[v] * p * (x + 0.5 * y) ** (p - 1) + (v + 1) * y ** (p - 1) for v in range(4)], -1)
+    dUy = np.stack([c[v] * 0.5 * p * (x + 0.5 * y) ** (p - 1) + (v + 1) * (p - 1) * x * y ** max(p - 2, 0)
+                    for v in range(4)], -1) if p > 1 else np.stack([c[v] * 0.5 * np.ones_like(x) for v in range(4)], -1)
+    Qx, Qy = s.gradient(U.reshape(-1))
+    tol = 1e-12 * max(1.0, np.abs(dUx).max())
+    inner = [e for e in range(s.E) if xy[tris[e]][:, 0].min() > 1e-12 and xy[tris[e]][:, 0].max() < 1 - 1e-12
+             and xy[tris[e]][:, 1].min() > 1e-12 and xy[tris[e]][:, 1].max() < 1 - 1e-12]
+    assert len(inner) > s.E // 3
+    assert np.abs(Qx[inner] - dUx[inner]).max() < tol and np.abs(Qy[inner] - dUy[inner]).max() < tol
+
+
+@pytest.mark.parametrize("corr", ["dg", "hu"])
+def test_viscous_gradient_discrete_gauss(corr):
+    """On a periodic mesh the integral of the corrected gradient vanishes
+    (the common solution is single-valued on every face)."""
+    s, xy, tris = _case(4, 3, jitter=0.2, seed=8, corr=corr)
+    s.set_viscous(100.0)
+    pos = T.sp_coords(xy, tris, s.ops)
+    U = cases.perturb(cases.tgv_ic(pos * 2 * math.pi, mach=0.3), 0.05, 9).reshape(-1)
+    Qx, Qy = s.gradient(U)
+    M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+    w = np.ones(s.Np) @ M
+    for Q in (Qx, Qy):
+        tot = np.einsum("e,i,eiv->v", s.met[:, 4], w, Q)
+        assert np.all(np.abs(tot) <= 1e-12 * np.einsum("e,i,eiv->v", s.met[:, 4], np.abs(w), np.abs(Q)))
+
+
+@pytest.mark.parametrize("p,corr", [(2, "dg"), (3, "ga"), (3, "hu")])
+def test_viscous_freestream_and_conservation(p, corr):
+    s, xy, tris = _case(4, p, jitter=0.2, seed=10, corr=corr)
+    s.set_viscous(50.0)
+    U = np.tile(cases.prim_to_cons(np.array(1.2), np.array(0.4), np.array(-0.3), np.array(1.1), 1.4),
+                (s.E * s.Np, 1)).reshape(-1)
+    assert np.abs(s.residual(U)).max() < 1e-11
+    pos = T.sp_coords(xy, tris, s.ops)
+    U = cases.perturb(cases.tgv_ic(pos * 2 * math.pi, mach=0.3), 0.05, 11).reshape(-1)
+    R = s.residual(U).reshape(s.E, s.Np, 4)
+    M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+    tot = np.einsum("e,i,ij,ejv->v", s.met[:, 4], np.ones(s.Np), M, R)
+    scale = np.einsum("e,i,ij,ejv->v", s.met[:, 4], np.ones(s.Np), np.abs(M), np.abs(R))
+    assert np.all(np.abs(tot) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("eta,stable", [(2.0, True), (0.0, False)])
+def test_viscous_linear_stability(eta, stable):
+    """Jacobian spectrum at a uniform state, low Re so the viscous terms
+    dominate: the default penalty keeps every eigenvalue in the left half
+    plane; without the penalty (eta = 0) the scheme has growing modes."""
+    s, xy, tris = _case(2, 2, corr="dg")
+    s.set_viscous(2.0, eta=eta)
+    U0 = np.tile(cases.prim_to_cons(np.array(1.0), np.array(0.5), np.array(0.2), np.array(1 / 1.4), 1.4),
+                 (s.E * s.Np, 1)).reshape(-1)
+    n = U0.size
+    Jm = np.zeros((n, n))
+    for k in range(n):
+        d = np.zeros(n)
+        d[k] = 1e-7
+        Jm[:, k] = (s.residual(U0 + d) - s.residual(U0 - d)) / 2e-7
+    lam = np.linalg.eigvals(Jm)
+    growth = lam.real.max() / np.abs(lam).max()
+    assert (growth < 1e-6) == stable, growth
+
+
+def test_viscous_tgv_decay_rate():
+    """Two-dimensional Taylor-Green vortex at low Mach: the kinetic energy
+    decays as exp(-4 t / Re) (on [0, 2 pi]^2, mu = 1/Re, rho = 1)."""
+    Re = 10.0
+    s, xy, tris = _case(8, 3, L=2 * math.pi, corr="dg")
+    s.set_viscous(Re)
+    pos = T.sp_coords(xy, tris, s.ops)
+    U = cases.tgv_ic(pos, mach=0.05).reshape(-1)
+    M = np.linalg.inv(s.ops["V"] @ s.ops["V"].T)
+
+    def ke(U):
+        q = U.reshape(s.E, s.Np, 4)
+        k = 0.5 * (q[..., 1] ** 2 + q[..., 2] ** 2) / q[..., 0]
+        return float(np.einsum("e,i,ij,ej->", s.met[:, 4], np.ones(s.Np), M, k))
+    dt, steps = 2e-4, 250
+    k0 = ke(U)
+    for _ in range(steps):
+        U = s.advance(U, dt)
+    rate = math.log(ke(U) / k0) / (dt * steps)
+    assert abs(rate - (-4.0 / Re)) < 0.02 * 4.0 / Re, rate

@@ -369,17 +369,22 @@ __device__ __forceinline__ double wave_reduce(double v) {
     return Op::f(Op::f(read_lane(v, 0), read_lane(v, 16)), Op::f(read_lane(v, 32), read_lane(v, 48)));
 }
 
-// Block reduction over BLOCK threads; the result is valid in thread 0.
-template <typename Op>
-__device__ __forceinline__ double block_reduce(double v, double* sred) {
+// Block reduction over NW waves; the result is valid in thread 0.
+template <typename Op, int NW>
+__device__ __forceinline__ double block_reduce_n(double v, double* sred) {
     v = wave_reduce<Op>(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) sred[w] = v;
     __syncthreads();
     double r = sred[0];
     if (threadIdx.x == 0)
-        for (int k = 1; k < BLOCK / 64; ++k) r = Op::f(r, sred[k]);
+        for (int k = 1; k < NW; ++k) r = Op::f(r, sred[k]);
     return r;
+}
+// ... over BLOCK threads.
+template <typename Op>
+__device__ __forceinline__ double block_reduce(double v, double* sred) {
+    return block_reduce_n<Op, BLOCK / 64>(v, sred);
 }
 
 // ------------------------------------------------------------ operators

@@ -46,6 +46,7 @@
 #include "zj_kernels.hpp"
 #include "zj_riemann_vjp.hpp"
 #include "zj_tri_ops.hpp"
+#include "zj_viscous.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -107,6 +108,23 @@ constexpr int OPS_TOTAL = ops_offset(6);
 __constant__ double c_ops64[OPS_TOTAL];
 __constant__ float c_ops32[OPS_TOTAL];
 
+// BR2 face liftings, per order and correction family (DG, SD = GA, HU):
+// LF_k = interp(edge k rows) . lift(edge k columns), [3][FP][FP] -- the lifting
+// of one edge's jumps evaluated at that edge's own flux points.
+constexpr int lf_words(int P) { return 9 * (P + 1) * (P + 1); }
+constexpr int lf_offset(int P) {
+    int o = 0;
+    for (int q = 1; q < P; ++q) o += 3 * lf_words(q);
+    return o;
+}
+constexpr int LF_TOTAL = lf_offset(6);
+inline int lf_family(int corr) { return corr == ZJ_CORR_DG ? 0 : (corr == ZJ_CORR_HU ? 2 : 1); }
+__constant__ double c_lf64[LF_TOTAL];
+__constant__ float c_lf32[LF_TOTAL];
+template <typename R> __device__ __forceinline__ R clf(int k);
+template <> __device__ __forceinline__ double clf<double>(int k) { return c_lf64[k]; }
+template <> __device__ __forceinline__ float clf<float>(int k) { return c_lf32[k]; }
+
 template <typename R> __device__ __forceinline__ R cop(int k);
 template <> __device__ __forceinline__ double cop<double>(int k) { return c_ops64[k]; }
 template <> __device__ __forceinline__ float cop<float>(int k) { return c_ops32[k]; }
@@ -143,6 +161,10 @@ struct TriArgs {
     int riemann;
     int xcd_groups;
     int lift_off;         // the correction family's lifting matrix in the order's operator block
+    // Navier-Stokes terms (tri_visc_stage_kernel): mu = 1/Re, kappa, BR2 penalty,
+    // and the family's face liftings in c_lf
+    double mu, kappa, eta;
+    int lf_off;
 };
 
 __device__ __forceinline__ int link_edge(int info) { return info & 7; }
@@ -374,6 +396,422 @@ __global__ __launch_bounds__(BLOCK, (TriWaves<R, P>::value)) void tri_stage_kern
             for (int m = 0; m < NP; ++m) t = fmaR(cop<R>(OI + j * NP + m), R(un[m]), t);
             a.T_out[trix(e, j, NFP) + v] = t;
         }
+    }
+}
+
+// ------------------------------------------------------------ Navier-Stokes terms
+// The stage with the compressible Navier-Stokes viscous terms, BR2 (textbook
+// scheme; Bassi-Rebay 2 on the triangle's FR correction field), periodic
+// meshes:
+//   U^C = (U_own + U_nbr)/2 at every face point, jump U^C - U_own
+//   Q = grad U + sum_j lift[:, j] s_j n_j (U^C - U_own)_j   (corrected gradient)
+//   F = F_inv(U) - F_v(U, Q) at the points (F_v: ViscousFlux, mu = 1/Re)
+//   F^I = Riemann(U_L, U_R) - F_v(U^C, Qbar) . n, Qbar the mean of both sides'
+//       trace of grad U plus eta times the lifting of this face's jumps
+//       alone at the face's points (LF_k = interp . lift of edge k)
+//   dU/dt = -(Dr Fr + Ds Fs) - sum_j lift[:, j] (s_j F^I_j - Fn_j)
+// Lanes as in tri_stage_kernel (one per element and variable), but the point
+// values (U, then Q) live in LDS, and each lane reads the four variables of a
+// point from there: the per-lane registers hold only the divergence and the
+// normal-flux sums.  A tile holds TT elements, sized so that the trace arrays
+// (U, dU/dx, dU/dy) and the point arrays (U, Qx, Qy; later the flux
+// differences) fit the LDS.  Traces across tiles carry 12 words per flux
+// point ([E][3(p+1)][12]: U, dU/dx, dU/dy).
+constexpr int TWV = 12;
+
+template <typename R, int P> struct TriViscTile {
+    static constexpr int NP = (P + 1) * (P + 2) / 2, NFP = 3 * (P + 1);
+    static constexpr int bytes(int tt) { return tt * (3 * NFP + 3 * NP) * NV * (int)sizeof(R); }
+    static constexpr int value = bytes(64) <= 61440 ? 64 : (bytes(32) <= 61440 ? 32 : 16);
+    static_assert(bytes(16) <= 61440, "the smallest tile exceeds the LDS");
+};
+
+template <typename R, int P, int MODE, int TT>
+__global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int NB = TT * NV, NWV = NB / 64;
+    static_assert(NB % 64 == 0 && 2 * NP >= NFP, "tile and point-array layout");
+    using OP = TriOps<P>;
+    constexpr int O0 = ops_offset(P), CA = OP::CA, OI = O0 + OP::C;
+    const int OLT = O0 + a.lift_off;
+    const int LFO = lf_offset(P) + a.lf_off;
+    __shared__ R sTu[TT * NFP * NV], sTx[TT * NFP * NV], sTy[TT * NFP * NV];
+    __shared__ R sU[TT * NP * NV];       // the stage input, later the stage output
+    __shared__ R sQ[2 * TT * NP * NV];   // Qx, Qy; later the flux differences [TT][NFP][NV]
+    __shared__ double sred[NWV];
+    __shared__ R sEdge[TT * 3];
+    __shared__ unsigned char sList[TT * 3];
+    __shared__ int sCnt[NWV];
+    R* const sQx = sQ;
+    R* const sQy = sQ + TT * NP * NV;
+    R* const sD = sQ;
+
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
+    const int tile = tile_of_block(a.xcd_groups);
+    const int e0 = tile * TT, e = e0 + el;
+    const bool live = e < a.E;
+    const R g = R(a.gamma), mu = R(a.mu), kappa = R(a.kappa), eta = R(a.eta);
+    const int64_t base = blk(e, 0, NP) + v;
+    auto sl = [](int elx, int j, int w) { return (elx * NFP + j) * NV + w; };   // trace slots
+    auto sp = [](int elx, int m, int w) { return (elx * NP + m) * NV + w; };    // point slots
+
+    // physical gradient of this lane's variable at the points, from sU into gx, gy
+    auto gradient_of = [&](R rx_, R ry_, R sx_, R sy_, R* gx, R* gy) {
+#pragma unroll 1
+        for (int i = 0; i < NP; ++i) {
+            R gr = R(0), gs = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const R um = sU[sp(el, m, v)];
+                gr = fmaR(cop<R>(O0 + m * CA + i), um, gr);
+                gs = fmaR(cop<R>(O0 + m * CA + NP + i), um, gs);
+            }
+            gx[sp(el, i, v)] = rx_ * gr + sx_ * gs;
+            gy[sp(el, i, v)] = ry_ * gr + sy_ * gs;
+        }
+    };
+
+    // ---- 0: state, gradient, traces of U and grad U
+    R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
+    int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
+    if (live) {
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) sU[sp(el, m, v)] = R(a.U_in[base + (int64_t)m * EB * NV]);
+        const R* G = a.geo + (int64_t)e * GEO;
+        rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
+        if (v < 3) {
+            const R ax = v == 0 ? sx : (v == 1 ? rx + sx : rx), ay = v == 0 ? sy : (v == 1 ? ry + sy : ry);
+            sEdge[el * 3 + v] = sqrtR(ax * ax + ay * ay);
+        }
+        gradient_of(rx, ry, sx, sy, sQx, sQy);
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            R tu = R(0), tx = R(0), ty = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const R w = cop<R>(O0 + m * CA + 2 * NP + j);
+                tu = fmaR(w, sU[sp(el, m, v)], tu);
+                tx = fmaR(w, sQx[sp(el, m, v)], tx);
+                ty = fmaR(w, sQy[sp(el, m, v)], ty);
+            }
+            sTu[sl(el, j, v)] = tu;
+            sTx[sl(el, j, v)] = tx;
+            sTy[sl(el, j, v)] = ty;
+        }
+    }
+    bool owned = false;
+    if (live && v < 3) {
+        const int2 Lv = v == 0 ? L[0] : (v == 1 ? L[1] : L[2]);
+        const int nl = Lv.x - e0;
+        owned = !(link_role(Lv.y) == ROLE_RIGHT && nl >= 0 && nl < TT);
+    }
+    const unsigned long long own_mask = __ballot(owned);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) sCnt[wave] = __popcll(own_mask);
+    __syncthreads();
+
+    // ---- 1: corrected gradient: the lifted jumps along the outward normals
+    if (live) {
+        R jwx[NFP], jwy[NFP];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const int2 Lk = L[kk];
+            const R sg = link_role(Lk.y) == ROLE_RIGHT ? R(-1) : R(1);
+            const R s = sEdge[el * 3 + kk];
+            const R wx = sg * s * a.geo[(int64_t)e * GEO + 4 + 2 * kk];
+            const R wy = sg * s * a.geo[(int64_t)e * GEO + 5 + 2 * kk];
+            const int nl = Lk.x - e0;
+            const bool in = nl >= 0 && nl < TT;
+            const int ek = link_edge(Lk.y);
+#pragma unroll
+            for (int f = 0; f < FP; ++f) {
+                const int j = kk * FP + f, oj = ek * FP + (P - f);
+                const R own = sTu[sl(el, j, v)];
+                R oth;
+                if (in) oth = sTu[sl(nl, oj, v)];
+                else oth = a.T_in[((int64_t)Lk.x * NFP + oj) * TWV + v];
+                const R jmp = R(0.5) * (own + oth) - own;
+                jwx[j] = wx * jmp;
+                jwy[j] = wy * jmp;
+            }
+        }
+#pragma unroll 1
+        for (int i = 0; i < NP; ++i) {
+            R qx = sQx[sp(el, i, v)], qy = sQy[sp(el, i, v)];
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) {
+                const R l = cop<R>(OLT + j * NP + i);
+                qx = fmaR(l, jwx[j], qx);
+                qy = fmaR(l, jwy[j], qy);
+            }
+            sQx[sp(el, i, v)] = qx;
+            sQy[sp(el, i, v)] = qy;
+        }
+    }
+    if (owned) {
+        int pos = __popcll(own_mask & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) pos += sCnt[w];
+        sList[pos] = (unsigned char)(el * 3 + v);
+    }
+    __syncthreads();  // Q of every variable in place
+
+    // ---- 2: total fluxes at the points (this lane's component, from the
+    // point's four variables in LDS); divergence and interpolated normal fluxes
+    R div[NP], fn[NFP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) div[i] = R(0);
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) fn[j] = R(0);
+    if (live) {
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            R U4[NV], X4[NV], Y4[NV];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                U4[w] = sU[sp(el, m, w)];
+                X4[w] = sQx[sp(el, m, w)];
+                Y4[w] = sQy[sp(el, m, w)];
+            }
+            const R inv = fast_rcp(U4[0]);
+            const R ux = U4[1] * inv, uy = U4[2] * inv;
+            const R p = (g - R(1)) * (U4[3] - R(0.5) * (U4[1] * ux + U4[2] * uy));
+            R Fv[NV], Gv[NV];
+            visc_flux(U4, X4, Y4, g, mu, kappa, Fv, Gv);
+            const R fx = v == 0 ? U4[1] : (v == 1 ? U4[1] * ux + p : (v == 2 ? U4[1] * uy : (U4[3] + p) * ux));
+            const R fy = v == 0 ? U4[2] : (v == 1 ? U4[2] * ux : (v == 2 ? U4[2] * uy + p : (U4[3] + p) * uy));
+            const R Fx = fx - (v == 0 ? Fv[0] : (v == 1 ? Fv[1] : (v == 2 ? Fv[2] : Fv[3])));
+            const R Fy = fy - (v == 0 ? Gv[0] : (v == 1 ? Gv[1] : (v == 2 ? Gv[2] : Gv[3])));
+            const R fr = rx * Fx + ry * Fy, fs = sx * Fx + sy * Fy, frs = fr + fs;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                div[i] = fmaR(cop<R>(O0 + m * CA + NP + i), fs, fmaR(cop<R>(O0 + m * CA + i), fr, div[i]));
+#pragma unroll
+            for (int j = 0; j < NFP; ++j)
+                fn[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+        }
+    }
+    __syncthreads();  // every read of Q done: the flux differences take its place
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) sD[sl(el, j, v)] = (j >= FP && j < 2 * FP) ? fn[j] : -fn[j];
+    }
+    __syncthreads();
+    int nsolved = 0;
+#pragma unroll
+    for (int w = 0; w < NWV; ++w) nsolved += sCnt[w];
+
+    // ---- 3: common fluxes of the solved face points: Riemann minus the BR2
+    // viscous flux; flux differences of both sides (in-tile partner included)
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += NB) {
+        const int c = sList[t / FP], f = t - (t / FP) * FP;
+        const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
+        const int2 Lk = a.links[(int64_t)(e0 + elt) * 3 + kk];
+        const bool right = link_role(Lk.y) == ROLE_RIGHT;
+        const R nx = a.geo[(int64_t)(e0 + elt) * GEO + 4 + 2 * kk];
+        const R ny = a.geo[(int64_t)(e0 + elt) * GEO + 5 + 2 * kk];
+        const int ek = link_edge(Lk.y), oj = ek * FP + (P - f);
+        const int nl = Lk.x - e0;
+        const bool in = nl >= 0 && nl < TT;
+        const R* Tn = a.T_in + (int64_t)Lk.x * NFP * TWV;  // the partner's traces (across tiles)
+        R own[NV], oth[NV], gxo[NV], gyo[NV], gxn[NV], gyn[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            own[w] = sTu[sl(elt, j, w)];
+            gxo[w] = sTx[sl(elt, j, w)];
+            gyo[w] = sTy[sl(elt, j, w)];
+        }
+        if (in) {
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                oth[w] = sTu[sl(nl, oj, w)];
+                gxn[w] = sTx[sl(nl, oj, w)];
+                gyn[w] = sTy[sl(nl, oj, w)];
+            }
+        } else {
+            load4(Tn + (int64_t)oj * TWV, oth);
+            load4(Tn + (int64_t)oj * TWV + NV, gxn);
+            load4(Tn + (int64_t)oj * TWV + 2 * NV, gyn);
+        }
+        // each side's edge factor |e|/(2J)
+        const R so = sEdge[elt * 3 + kk];
+        R sn;
+        if (in) {
+            sn = sEdge[nl * 3 + ek];
+        } else {
+            const R* Gn = a.geo + (int64_t)Lk.x * GEO;
+            const R ax = ek == 0 ? Gn[2] : (ek == 1 ? Gn[0] + Gn[2] : Gn[0]);
+            const R ay = ek == 0 ? Gn[3] : (ek == 1 ? Gn[1] + Gn[3] : Gn[1]);
+            sn = sqrtR(ax * ax + ay * ay);
+        }
+        // liftings of this face's jumps at both sides' points; the partner's
+        // jump at its slot P-f2 is minus this side's at f2
+        R lo[NV] = {R(0), R(0), R(0), R(0)}, ln[NV] = {R(0), R(0), R(0), R(0)};
+#pragma unroll 1
+        for (int f2 = 0; f2 < FP; ++f2) {
+            const int j2 = kk * FP + f2, o2 = ek * FP + (P - f2);
+            R uo[NV], un[NV];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) uo[w] = sTu[sl(elt, j2, w)];
+            if (in) {
+#pragma unroll
+                for (int w = 0; w < NV; ++w) un[w] = sTu[sl(nl, o2, w)];
+            } else {
+                load4(Tn + (int64_t)o2 * TWV, un);
+            }
+            const R wo = clf<R>(LFO + (kk * FP + f) * FP + f2);
+            const R wn = clf<R>(LFO + (ek * FP + (P - f)) * FP + (P - f2));
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R jo = R(0.5) * (uo[w] + un[w]) - uo[w];
+                lo[w] = fmaR(wo, jo, lo[w]);
+                ln[w] = fmaR(wn, jo, ln[w]);
+            }
+        }
+        const R sg = right ? R(-1) : R(1);
+        const R nox = sg * nx, noy = sg * ny;  // this side's outward normal
+        R Uc[NV], Qbx[NV], Qby[NV], UL[NV], UR[NV], Fc[NV], Fv[NV], Gv[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            Uc[w] = R(0.5) * (own[w] + oth[w]);
+            Qbx[w] = R(0.5) * ((gxo[w] + eta * so * nox * lo[w]) + (gxn[w] + eta * sn * nox * ln[w]));
+            Qby[w] = R(0.5) * ((gyo[w] + eta * so * noy * lo[w]) + (gyn[w] + eta * sn * noy * ln[w]));
+            UL[w] = right ? oth[w] : own[w];
+            UR[w] = right ? own[w] : oth[w];
+        }
+        visc_flux(Uc, Qbx, Qby, g, mu, kappa, Fv, Gv);
+        riemann_by_kind(a.riemann, UL, UR, nx, ny, g, Fc);
+        {
+            const R sc = right ? -so : so;
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R fi = Fc[w] - (Fv[w] * nx + Gv[w] * ny);
+                sD[sl(elt, j, w)] = sc * fi - sD[sl(elt, j, w)];
+            }
+        }
+        if (in) {  // the in-tile partner, whose side is the right one
+            const R sc = -sEdge[nl * 3 + ek];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R fi = Fc[w] - (Fv[w] * nx + Gv[w] * ny);
+                sD[sl(nl, oj, w)] = sc * fi - sD[sl(nl, oj, w)];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- 4: lifted flux differences, RK stage update (the stage input read
+    // again from memory); the stage output to sU for its traces
+    double l2 = 0.0;
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            const R d = sD[sl(el, j, v)];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OLT + j * NP + i), d, div[i]);
+        }
+        const double dt = MODE == MODE_RESIDUAL ? 0.0 : *a.dt_ptr;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double res = -double(div[i]);
+            const int64_t gi = base + (int64_t)i * EB * NV;
+            if (MODE == MODE_RESIDUAL) {
+                a.R_out[gi] = res;
+                l2 += res * res;
+            } else {
+                const double q = a.U_in[gi];
+                double un;
+                if (MODE == MODE_STAGE1) {
+                    un = q + dt * res;
+                } else {
+                    const double u0 = a.U0[gi];
+                    un = MODE == MODE_STAGE2 ? 0.75 * u0 + 0.25 * (q + dt * res)
+                                             : (1.0 / 3.0) * u0 + (2.0 / 3.0) * (q + dt * res);
+                    if (MODE == MODE_STAGE3) l2 += res * res;
+                }
+                a.U_out[gi] = un;
+                sU[sp(el, i, v)] = R(un);  // only this lane reads its variable's points back
+            }
+        }
+    }
+    if ((MODE == MODE_RESIDUAL || MODE == MODE_STAGE3) && a.l2_partial) {
+        const double s2 = block_reduce_n<OpSum, NWV>(l2, sred);
+        if (tid == 0) a.l2_partial[tile] = s2;
+    }
+    if (MODE == MODE_RESIDUAL) return;
+    __syncthreads();  // every read of the flux differences done: their space is reused below
+    if (!live) return;
+
+    // ---- 5: traces of the new stage state and of its gradient on edges
+    // another tile reads (the gradient into the Q arrays, own variable only)
+    bool any_out = false;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+        const int nl = L[kk].x - e0;
+        any_out = any_out || !(nl >= 0 && nl < TT);
+    }
+    if (!any_out) return;
+    R* const gx = sQx;  // each lane reads only its own variable's slots from here on
+    R* const gy = sQy;
+    gradient_of(rx, ry, sx, sy, gx, gy);
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+        const int nl = L[kk].x - e0;
+        if (nl >= 0 && nl < TT) continue;
+#pragma unroll 1
+        for (int j = kk * FP; j < kk * FP + FP; ++j) {
+            R tu = R(0), tx = R(0), ty = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const R w = cop<R>(OI + j * NP + m);
+                tu = fmaR(w, sU[sp(el, m, v)], tu);
+                tx = fmaR(w, gx[sp(el, m, v)], tx);
+                ty = fmaR(w, gy[sp(el, m, v)], ty);
+            }
+            R* T = a.T_out + ((int64_t)e * NFP + j) * TWV;
+            T[v] = tu;
+            T[NV + v] = tx;
+            T[2 * NV + v] = ty;
+        }
+    }
+}
+
+// Traces of U and grad U on every edge (after a new state is set).
+template <typename R, int P>
+__global__ __launch_bounds__(BLOCK) void tri_visc_trace_kernel(int E, const double* U, const R* geo, R* T_out) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, NFP = S::NFP, O0 = ops_offset(P), CA = TriOps<P>::CA;
+    const int e = blockIdx.x * TILE + (threadIdx.x >> 2), v = threadIdx.x & 3;
+    if (e >= E) return;
+    const R* G = geo + (int64_t)e * GEO;
+    const R rx = G[0], ry = G[1], sx = G[2], sy = G[3];
+    R u[NP], gr[NP], gs[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) { u[m] = R(U[blk(e, m, NP) + v]); gr[m] = R(0); gs[m] = R(0); }
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            gr[i] = fmaR(cop<R>(O0 + m * CA + i), u[m], gr[i]);
+            gs[i] = fmaR(cop<R>(O0 + m * CA + NP + i), u[m], gs[i]);
+        }
+#pragma unroll 1
+    for (int j = 0; j < NFP; ++j) {
+        R tu = R(0), tx = R(0), ty = R(0);
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const R w = cop<R>(O0 + m * CA + 2 * NP + j);
+            tu = fmaR(w, u[m], tu);
+            tx = fmaR(w, rx * gr[m] + sx * gs[m], tx);
+            ty = fmaR(w, ry * gr[m] + sy * gs[m], ty);
+        }
+        R* T = T_out + ((int64_t)e * NFP + j) * TWV;
+        T[v] = tu;
+        T[NV + v] = tx;
+        T[2 * NV + v] = ty;
     }
 }
 
@@ -816,10 +1254,14 @@ __global__ void tri_from_blk_kernel(const double* __restrict__ in, double* __res
     store4(aos + ((int64_t)dst * NP + i) * NV, q);
 }
 
-// CFL step per point of the blocked state, minimum per workgroup.
+// CFL step per point of the blocked state, minimum per workgroup.  With the
+// viscous terms (nu_k = mu max(4/3, gamma/Pr), else 0) the diffusive rate
+// (p+1)^4 nu_k / (rho d^2) joins the convective (2p+1)(|u|+c)/d: the
+// spectral radius of the BR2 operator on triangles is about 1.3 (p+1)^4
+// nu / d^2 (P2-P4; 1.7 at P1), d the inscribed-circle diameter.
 __global__ __launch_bounds__(BLOCK) void tri_dt_kernel(const double* __restrict__ U, const double* __restrict__ hcell,
                                                        int E, int NP, double gamma, double cfl, double cf,
-                                                       double* __restrict__ partial) {
+                                                       double visc, double* __restrict__ partial) {
     __shared__ double sRed[BLOCK / 64];
     const int64_t n = (int64_t)E * NP;
     double mn = 1e20;
@@ -830,7 +1272,8 @@ __global__ __launch_bounds__(BLOCK) void tri_dt_kernel(const double* __restrict_
         const double u = q[1] / q[0], w = q[2] / q[0];
         const double p = (gamma - 1.0) * q[0] * (q[3] / q[0] - 0.5 * (u * u + w * w));
         const double ws = sqrt(u * u + w * w) + sqrt(gamma * fmax(p, 1e-10) / fmax(q[0], 1e-10));
-        const double d = cfl * hcell[e] / (ws * cf);
+        const double h = hcell[e];
+        const double d = cfl / (cf * ws / h + visc / (fmax(q[0], 1e-10) * h * h));
         mn = d < mn ? d : mn;
     }
     const double r = block_reduce<OpMin>(mn, sRed);
@@ -857,6 +1300,54 @@ hipError_t tri_launch(int P, int mode, const TriArgs<R>& a, hipStream_t s) {
     case 3: return tri_launch_p<R, 3>(mode, a, s);
     case 4: return tri_launch_p<R, 4>(mode, a, s);
     default: return tri_launch_p<R, 5>(mode, a, s);
+    }
+}
+
+template <typename R, int P>
+hipError_t tri_visc_launch_p(int mode, const TriArgs<R>& a, hipStream_t s) {
+    constexpr int TT = TriViscTile<R, P>::value;
+    const int grid = (std::max(a.E, 1) + TT - 1) / TT;
+    const dim3 blk(TT * NV);
+    switch (mode) {
+    case MODE_RESIDUAL: klaunch(tri_visc_stage_kernel<R, P, MODE_RESIDUAL, TT>, grid, blk, 0, s, a); break;
+    case MODE_STAGE1: klaunch(tri_visc_stage_kernel<R, P, MODE_STAGE1, TT>, grid, blk, 0, s, a); break;
+    case MODE_STAGE2: klaunch(tri_visc_stage_kernel<R, P, MODE_STAGE2, TT>, grid, blk, 0, s, a); break;
+    default: klaunch(tri_visc_stage_kernel<R, P, MODE_STAGE3, TT>, grid, blk, 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t tri_visc_launch(int P, int mode, const TriArgs<R>& a, hipStream_t s) {
+    switch (P) {
+    case 1: return tri_visc_launch_p<R, 1>(mode, a, s);
+    case 2: return tri_visc_launch_p<R, 2>(mode, a, s);
+    case 3: return tri_visc_launch_p<R, 3>(mode, a, s);
+    case 4: return tri_visc_launch_p<R, 4>(mode, a, s);
+    default: return tri_visc_launch_p<R, 5>(mode, a, s);
+    }
+}
+
+template <typename R>
+hipError_t tri_visc_trace(int P, int E, const double* U, const R* geo, R* T, hipStream_t s) {
+    const int grid = (std::max(E, 1) + TILE - 1) / TILE;
+    switch (P) {
+    case 1: tri_visc_trace_kernel<R, 1><<<grid, BLOCK, 0, s>>>(E, U, geo, T); break;
+    case 2: tri_visc_trace_kernel<R, 2><<<grid, BLOCK, 0, s>>>(E, U, geo, T); break;
+    case 3: tri_visc_trace_kernel<R, 3><<<grid, BLOCK, 0, s>>>(E, U, geo, T); break;
+    case 4: tri_visc_trace_kernel<R, 4><<<grid, BLOCK, 0, s>>>(E, U, geo, T); break;
+    default: tri_visc_trace_kernel<R, 5><<<grid, BLOCK, 0, s>>>(E, U, geo, T); break;
+    }
+    return hipGetLastError();
+}
+
+int tri_visc_tile(int P, bool fp64) {
+    switch (P) {
+    case 1: return fp64 ? TriViscTile<double, 1>::value : TriViscTile<float, 1>::value;
+    case 2: return fp64 ? TriViscTile<double, 2>::value : TriViscTile<float, 2>::value;
+    case 3: return fp64 ? TriViscTile<double, 3>::value : TriViscTile<float, 3>::value;
+    case 4: return fp64 ? TriViscTile<double, 4>::value : TriViscTile<float, 4>::value;
+    default: return fp64 ? TriViscTile<double, 5>::value : TriViscTile<float, 5>::value;
     }
 }
 
@@ -952,12 +1443,34 @@ hipError_t load_tri_ops(int device) {
                 for (int i = 0; i < np; ++i) dl[(size_t)jj * np + i] = F.lift[(size_t)i * nf + jj];
         }
     }
-    std::vector<float> allf(all.begin(), all.end());
+    std::vector<double> lf((size_t)LF_TOTAL, 0.0);
+    for (int P = 1; P <= 5; ++P) {
+        for (int fam = 0; fam < 3; ++fam) {
+            tri::Operators O;
+            if (!tri::operators(P, O, fam == 0 ? ZJ_CORR_DG : (fam == 1 ? ZJ_CORR_SD : ZJ_CORR_HU)))
+                return hipErrorInvalidValue;
+            const int np = O.np, nf = O.nf, fp = P + 1;
+            double* d = lf.data() + lf_offset(P) + (size_t)fam * lf_words(P);
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < fp; ++a)
+                    for (int b = 0; b < fp; ++b) {
+                        double acc = 0.0;
+                        for (int i = 0; i < np; ++i)
+                            acc += O.interp[(size_t)(k * fp + a) * np + i] * O.lift[(size_t)i * nf + k * fp + b];
+                        d[(k * fp + a) * fp + b] = acc;
+                    }
+        }
+    }
+    std::vector<float> allf(all.begin(), all.end()), lff(lf.begin(), lf.end());
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops64), all.data(), all.size() * sizeof(double), 0,
                                      hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpyToSymbol(HIP_SYMBOL(c_ops32), allf.data(), allf.size() * sizeof(float), 0,
                               hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_lf64), lf.data(), lf.size() * sizeof(double), 0, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbol(HIP_SYMBOL(c_lf32), lff.data(), lff.size() * sizeof(float), 0, hipMemcpyHostToDevice);
     if (e == hipSuccess && device >= 0 && device < 64) g_ops_loaded |= 1ull << device;
     return e;
 }
@@ -1009,6 +1522,9 @@ struct zj_tri {
     int A_cur = 0;
     bool A_has_tb = false;
     bool A_seeded = false;
+    // Navier-Stokes terms (BR2): mu = 1/Re, kappa, penalty; traces of 12 words
+    bool viscous = false;
+    double mu = 0.0, kappa = 0.0, eta = 2.0;
     std::string err;
 };
 
@@ -1078,7 +1594,17 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
     h->P = P; h->NP = NP; h->NFP = NFP; h->E = m->n_elem; h->F = m->n_faces;
     h->prec = p->precision; h->gamma = p->gamma; h->riemann = p->riemann; h->corr = p->corr;
     h->device = p->device;
-    h->epb = TILE;
+    h->viscous = p->viscous != 0;
+    if (h->viscous) {
+        if (!(p->Re > 0.0) || !(p->Pr > 0.0)) return tfail(h, ZJ_ERR_ARG, "viscous terms need Re > 0 and Pr > 0");
+        if (P == 4 && p->precision == ZJ_PREC_FP64)
+            return tfail(h, ZJ_ERR_UNSUPPORTED, "FP64 P4 triangles with the viscous terms are switched off: an earlier "
+                                                "build of their kernel faulted on the GPU, with no cause found");
+        h->mu = 1.0 / p->Re;
+        h->kappa = h->mu * p->gamma / ((p->gamma - 1.0) * p->Pr);
+        h->eta = p->br2_eta > 0.0 ? p->br2_eta : 2.0;
+    }
+    h->epb = h->viscous ? tri_visc_tile(P, h->prec == ZJ_PREC_FP64) : TILE;
     h->nblk = (h->E + h->epb - 1) / h->epb;
     const int E = h->E;
 
@@ -1140,6 +1666,9 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
             geo[(size_t)dr * GEO + 5 + 2 * rl] = ny;
             if (dl / h->epb != dr / h->epb) h->cross_edges += 2;
         } else {
+            if (h->viscous)
+                return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle viscous terms are built for meshes without "
+                                                    "boundary faces");
             if (!m->bc_type) return tfail(h, ZJ_ERR_ARG, "boundary face without bc_type");
             const int bct = m->bc_type[f];
             if (bct < 0 || bct > 7) return tfail(h, ZJ_ERR_ARG, "unknown BC type");
@@ -1168,7 +1697,7 @@ static int tbuild(zj_tri* h, const zj_tri_desc* m, const zj_params* p) {
     ZJ_THIP(h, hipMalloc((void**)&h->W, eb * NP * 4 * sizeof(double)));
     ZJ_THIP(h, hipMalloc((void**)&h->Rres, eb * NP * 4 * sizeof(double)));
     ZJ_THIP(h, hipMalloc((void**)&h->aos, npts * 4 * sizeof(double)));
-    for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->T[k], eb * NFP * 4 * rsz));
+    for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->T[k], eb * NFP * (h->viscous ? TWV : 4) * rsz));
     int rc;
     if (h->prec == ZJ_PREC_FP64) {
         if ((rc = tupload_as<double>(h, &h->geo, geo))) return rc;
@@ -1203,6 +1732,10 @@ static TriArgs<R> tri_args(zj_tri* h, int mode) {
     a.riemann = h->riemann;
     a.xcd_groups = h->xcd_groups;
     a.lift_off = lift_offset(h->P, h->corr);
+    a.mu = h->mu;
+    a.kappa = h->kappa;
+    a.eta = h->eta;
+    a.lf_off = lf_family(h->corr) * lf_words(h->P);
     R** T = (R**)h->T;
     switch (mode) {
     case MODE_RESIDUAL:
@@ -1222,15 +1755,26 @@ static TriArgs<R> tri_args(zj_tri* h, int mode) {
 }
 
 static hipError_t tenqueue(zj_tri* h, int mode, hipStream_t s) {
-    hipError_t e = h->prec == ZJ_PREC_FP64 ? tri_launch<double>(h->P, mode, tri_args<double>(h, mode), s)
-                                           : tri_launch<float>(h->P, mode, tri_args<float>(h, mode), s);
+    hipError_t e;
+    if (h->viscous)
+        e = h->prec == ZJ_PREC_FP64 ? tri_visc_launch<double>(h->P, mode, tri_args<double>(h, mode), s)
+                                    : tri_visc_launch<float>(h->P, mode, tri_args<float>(h, mode), s);
+    else
+        e = h->prec == ZJ_PREC_FP64 ? tri_launch<double>(h->P, mode, tri_args<double>(h, mode), s)
+                                    : tri_launch<float>(h->P, mode, tri_args<float>(h, mode), s);
     if (e == hipSuccess && (mode == MODE_RESIDUAL || mode == MODE_STAGE3)) h->l2_blocks = h->nblk;
     return e;
 }
 
 static int trefresh(zj_tri* h, hipStream_t s) {
-    hipError_t e = h->prec == ZJ_PREC_FP64 ? tri_trace<double>(h->P, h->E, h->U, (double*)h->T[0], s)
-                                           : tri_trace<float>(h->P, h->E, h->U, (float*)h->T[0], s);
+    hipError_t e;
+    if (h->viscous)
+        e = h->prec == ZJ_PREC_FP64
+                ? tri_visc_trace<double>(h->P, h->E, h->U, (const double*)h->geo, (double*)h->T[0], s)
+                : tri_visc_trace<float>(h->P, h->E, h->U, (const float*)h->geo, (float*)h->T[0], s);
+    else
+        e = h->prec == ZJ_PREC_FP64 ? tri_trace<double>(h->P, h->E, h->U, (double*)h->T[0], s)
+                                    : tri_trace<float>(h->P, h->E, h->U, (float*)h->T[0], s);
     ZJ_THIP(h, e);
     return ZJ_OK;
 }
@@ -1387,8 +1931,11 @@ extern "C" int zj_tri_dt_cfl(zj_tri_handle h, double cfl, double* dt) {
     if (!h || !dt) return ZJ_ERR_ARG;
     if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
     ZJ_THIP(h, hipSetDevice(h->device));
+    const double p1 = h->P + 1.0;
+    const double visc = h->viscous ? p1 * p1 * p1 * p1 * std::max(h->mu * 4.0 / 3.0, h->kappa * (h->gamma - 1.0))
+                                   : 0.0;  // kappa (gamma-1) = mu gamma / Pr
     tri_dt_kernel<<<h->nred, BLOCK, 0, h->stream>>>(h->U, h->hcell, h->E, h->NP, h->gamma, cfl, 2.0 * h->P + 1.0,
-                                                    h->partial_dt);
+                                                    visc, h->partial_dt);
     ZJ_THIP(h, hipGetLastError());
     ZJ_THIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, h->stream));
     ZJ_THIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1442,6 +1989,7 @@ extern "C" int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_b
 
 // ------------------------------------------------------------ adjoint (host)
 static int tensure_adjoint(zj_tri* h) {
+    if (h->viscous) return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint covers the Euler terms");
     if (h->A_Ub) return ZJ_OK;
     ZJ_THIP(h, hipSetDevice(h->device));
     const size_t eb = ((size_t)h->E + EB - 1) / EB * EB;
@@ -1554,6 +2102,7 @@ extern "C" int zj_tri_rk3_step_vjp(zj_tri_handle h, double dt, double* dt_bar, v
     if (!h) return ZJ_ERR_ARG;
     if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
     if (!h->A_seeded) return tfail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    if (h->viscous) return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint covers the Euler terms");
     ZJ_THIP(h, hipSetDevice(h->device));
     hipStream_t s = tpick(h, stream);
     if (dt != h->dt_on_device) {

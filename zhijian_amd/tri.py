@@ -47,7 +47,7 @@ class TriSolver:
 
     def __init__(self, xy, tris, p, faces=None, normals=None, periodic=None, bc_type=None, bc_face_map=None,
                  bc_data=None, gamma=1.4, precision="fp64", riemann="roe", device=0, order="hilbert",
-                 corr="dg"):
+                 corr="dg", viscous=False, Re=1600.0, Pr=0.72, br2_eta=0.0):
         xy = np.ascontiguousarray(xy, np.float64)
         tris = np.ascontiguousarray(tris, np.int64)
         if faces is None:
@@ -75,7 +75,7 @@ class TriSolver:
             keep.append(bd); d.bc_data = bd.ctypes.data; d.n_bc_data = bd.size // 8
         d.reorder = 1 if order == "hilbert" else 0
         prm = _lib.Params(gamma, 0.5, PRECISIONS[precision], _lib.ZJ_COMPAT_TEXTBOOK, 0, device,
-                          RIEMANNS[riemann], CORRS[corr])
+                          RIEMANNS[riemann], CORRS[corr], int(bool(viscous)), float(Re), float(Pr), float(br2_eta))
         h = C.c_void_p()
         _check(lib().zj_tri_create(C.byref(h), C.byref(d), C.byref(prm)))
         self._h = h
