@@ -151,14 +151,22 @@ def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None):
                      "call, including the host objective and two state transfers")
 
 
+BATCH_SCHEME = dict(compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Pr=0.72)
+
+
 def batch_members(n_inst, seed, precision):
     """Seeded batch: N ~ U{16..128} elements per side, P ~ U{1..5},
-    Mach ~ U[0.1, 0.8], alternating Taylor-Green / isentropic vortex."""
+    Mach ~ U[0.1, 0.8] (PCG64 seed), Re ~ U[100, 5000] (PCG64 [seed, 1]),
+    alternating Taylor-Green /
+    isentropic vortex; textbook FR (VCJH HU, Roe) with the Navier-Stokes terms
+    (BR2, Pr 0.72), no limiter."""
     from zhijian_amd import cases, solver
     rng = np.random.default_rng(seed)
+    re_rng = np.random.default_rng([seed, 1])  # its own stream: the meshes do not depend on it
     out = []
     for k in range(n_inst):
         n, p, mach = int(rng.integers(16, 129)), int(rng.integers(1, 6)), float(rng.uniform(0.1, 0.8))
+        Re = float(re_rng.uniform(100.0, 5000.0))
         if k % 2 == 0:
             mesh = cases.tgv_mesh(n)
             t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
@@ -167,16 +175,16 @@ def batch_members(n_inst, seed, precision):
             mesh = cases.vortex_mesh(n)
             t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
             U = cases.vortex_ic(t.pos, u_inf=mach * math.sqrt(1.4)).reshape(-1)
-        s = solver.Solver(t, precision=precision, cfl=0.5)
+        s = solver.Solver(t, precision=precision, cfl=0.5, Re=Re, **BATCH_SCHEME)
         s.set_state(U)
-        out.append((s, U, dict(n=n, p=p, mach=mach, kind="tgv" if k % 2 == 0 else "vortex")))
+        out.append((s, U, dict(n=n, p=p, mach=mach, Re=Re, kind="tgv" if k % 2 == 0 else "vortex")))
     return out
 
 
-def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=5):
+def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=20):
     """One batched SSP-RK3 step of all members vs each member's own graph step;
     the FP32-bulk batch against an FP64 batch of the same members after
-    err_steps steps (reference arithmetic: short window)."""
+    err_steps steps."""
     from zhijian_amd import solver
     mem = batch_members(n_inst, seed, precision)
     sols = [m[0] for m in mem]
@@ -186,11 +194,11 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
     for s, (_, _, meta) in zip(sols, mem):
         info = s.info()
         n_el = info["n_dof"] // (4 * info["n_sp"])
-        by += sum(alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"], info["cross_tile_edges"])
-                  for k in (1, 2, 3))
+        by += sum(textbook_alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"],
+                                               info["cross_tile_edges"], viscous=True) for k in (1, 2, 3))
     batch = solver.Batch(sols)
     res = {}
-    for single in (False, True):
+    for single in (False,):  # the single launch covers reference-arithmetic members only
         batch.set_launch(single)
         for policy in (0, 2, 4):
             batch.set_grid(policy)
@@ -241,7 +249,8 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
                    median_rel_l2=float(np.median([x[0] for x in rel])), over="members (FP64 batch of the same meshes)")
     achieved = by / (res[best] * 1e-3) / 1e9
     return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], "
-                         f"alternating TGV/vortex", precision=precision, dof=dof,
+                         f"Re~U[100,5000], alternating TGV/vortex; textbook FR (VCJH HU, Roe) + Navier-Stokes "
+                         f"BR2 (Pr 0.72), no limiter, dt = each member's CFL 0.5 step", precision=precision, dof=dof,
                 ms_per_step_by_launch={f"{k[0]}/grid{k[1]}": v for k, v in res.items()},
                 launch=best[0], grid_policy=best[1],
                 autotuner=dict(launch="single" if tuned[0] else "grouped", grid_policy=tuned[1], ms_per_step=tuned[2]),

@@ -170,7 +170,10 @@ int zj_synchronize(zj_handle h);
  * execution time in ms (the quantity a kernel profiler reports, without the
  * gaps between launches).  The state advances by `steps` steps. */
 int zj_time_stages(zj_handle h, double dt, int steps, double* stage_ms, void* stream);
-/* CFL time step of the current state (synchronises). */
+/* CFL time step of the current state (synchronises): the minimum over the
+ * points of cfl h / ((2p+1) ws), h = sqrt(element area), ws = |u| + c; with
+ * the viscous terms, of cfl / ((2p+1) ws / h + (p+1)^4 nu_k / (rho h^2)),
+ * nu_k = max(4/3 mu, (gamma-1) kappa). */
 int zj_dt_cfl(zj_handle h, double cfl, double* dt);
 /* L2 norm of the last residual: that of zj_residual, or of stage 3 of the last
  * step (synchronises). */
@@ -253,8 +256,11 @@ int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_ao
 
 /* ---- batches and tuning (no reference counterpart) -------------------
  * A batch steps many independent meshes (each its own zj_handle, own order,
- * state and dt) with one launch per stage and order/metric group, the groups
- * running side by side in one captured graph.  Members keep working on their
+ * state, dt, scheme and Reynolds number) with one launch per stage and
+ * (order, metric form, scheme) group -- reference arithmetic, textbook
+ * Euler, textbook Navier-Stokes -- the groups running side by side in one
+ * captured graph; each member's step equals its own zj_rk3_step bit for
+ * bit.  Members keep working on their
  * own; a member must not be stepped on another stream while its batch runs.
  * Members must use the full tile of their order (the default).  While a
  * batch holds a handle, zj_set_tile, zj_set_geometry_mode (a change),
@@ -276,11 +282,13 @@ int zj_batch_set_grid(zj_batch_handle b, int policy);
  * members' orders and metric forms (the stage body is picked per tile, the
  * LDS sized for the largest order present); 0 (default): one launch per
  * (order, metric form) group, the groups on parallel graph branches.
- * Results are the same either way. */
+ * Results are the same either way.  single = 1 covers reference-arithmetic
+ * members only: with a textbook member it returns ZJ_ERR_UNSUPPORTED. */
 int zj_batch_set_launch(zj_batch_handle b, int single);
 /* Times `reps` batched steps of sizes dts for each launch form and grid
  * policy (0, 2, 4), keeps the fastest (*single, *policy, *ms per step; any may
- * be NULL), and restores every member's state as it was before the call. */
+ * be NULL), and restores every member's state as it was before the call;
+ * with textbook members the single launch is not among the candidates. */
 int zj_batch_autotune(zj_batch_handle b, const double* dts, int reps, int* single, int* policy, double* ms);
 /* Times the residual launch for every tile (full, half) and XCD order and
  * keeps the fastest; results do not depend on the choice. */

@@ -2,7 +2,8 @@
 
 The batched launches run the same stage arithmetic as a single mesh's
 kernels, so a batched step must equal each member's own step bit for bit, for
-every order, both metric forms and every workgroup-grid policy.
+every order, both metric forms, every scheme (reference arithmetic, textbook
+Euler, textbook Navier-Stokes) and every workgroup-grid policy.
 """
 import numpy as np
 import pytest
@@ -90,6 +91,70 @@ def test_single_launch_equals_members(precision):
     batch.close()
 
 
+# (order, elements per side, scheme, Reynolds number, Riemann solver, correction, jitter)
+SCHEME_MEMBERS = [
+    (1, 10, "ns", 100.0, "roe", "hu", 0.0), (2, 9, "ns", 4000.0, "hllc", "dg", 0.0),
+    (3, 8, "ns", 800.0, "roe", "hu", 0.0), (4, 7, "ns", 250.0, "roe", "sd", 0.0),
+    (5, 6, "ns", 1600.0, "hllc", "hu", 0.0), (3, 11, "ns", 5000.0, "roe", "hu", 0.0),
+    (2, 8, "euler", None, "roe", "hu", 0.0), (4, 9, "euler", None, "hllc", "dg", 0.1),
+    (3, 10, "reference", None, None, None, 0.0), (5, 7, "reference", None, None, None, 0.08),
+]
+
+
+def _scheme_member(p, n, scheme, Re, riemann, corr, jit, precision, seed):
+    mesh = cases.uniform_quad_mesh(n, n, 0.0, 2 * np.pi, 0.0, 2 * np.pi, jitter=jit, seed=seed)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
+    U = tgv_state(t, amp=0.02, seed=seed, mach=0.3)
+    if scheme == "reference":
+        kw = {}
+    else:
+        kw = dict(compat="textbook", riemann=riemann, corr=corr, positivity=False)
+        if scheme == "ns":
+            kw.update(viscous=True, Re=Re, Pr=0.72)
+    return t, U, kw
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_batch_textbook_and_navier_stokes_members(precision):
+    """Textbook Euler and Navier-Stokes (BR2, each its own Reynolds number)
+    members batched with reference-arithmetic ones: one launch per (order,
+    metric form, scheme) group; every member's step equals its own bit for
+    bit under both grid policies, and the single launch (reference arithmetic
+    only) is refused."""
+    say(f"case: batch of {len(SCHEME_MEMBERS)} meshes: NS Re 100-5000, textbook Euler, reference; {precision}")
+    built = [_scheme_member(*m, precision, seed=70 + k) for k, m in enumerate(SCHEME_MEMBERS)]
+    a = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    b = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    for s1, s2, (t, U, _) in zip(a, b, built):
+        s1.set_state(U)
+        s2.set_state(U)
+    dts = np.array([0.5 * s.dt_cfl(0.5) for s in a])
+    batch = solver.Batch(a)
+    with pytest.raises(solver.ZJError) as ei:
+        batch.set_launch(True)
+    assert ei.value.code == 4
+    for step in range(4):
+        batch.set_grid(0 if step < 2 else 4)
+        batch.rk3_step(dts)
+        for s2, dt in zip(b, dts):
+            s2.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), SCHEME_MEMBERS[k]
+        assert s1.residual_l2() == s2.residual_l2(), SCHEME_MEMBERS[k]
+    single, policy, ms = batch.autotune(dts, reps=2)
+    say(f"  autotune: single={single} policy={policy} {ms:.3f} ms per step")
+    assert not single and policy in (0, 2, 4)
+    batch.rk3_step(dts)
+    for s2, dt in zip(b, dts):
+        s2.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), SCHEME_MEMBERS[k]
+        assert np.all(np.isfinite(s1.get_state()))
+    batch.close()
+
+
 def test_batch_member_changes():
     """While a batch holds a handle, changes that would leave the batch's
     launch arguments stale are refused (tile, metric form, autotune, destroy),
@@ -148,24 +213,31 @@ def test_autotune_keeps_results():
     assert np.array_equal(s.get_state(), ref.get_state())
 
 
-def test_batch_c4_full_size():
-    """The batched configuration at full size, as the benchmark builds it
-    (64 seeded meshes, N ~ U{16..128} elements per side, P ~ U{1..5}, Mach ~
-    U[0.1, 0.8], alternating Taylor-Green / vortex, FP32-bulk, PCG64 seed 4;
-    35.8 M DOF): two batched steps (one workgroup per tile, then four per CU
-    looping) equal every member's own steps bit for bit."""
+@pytest.mark.parametrize("scheme", ["reference", "navier_stokes"])
+def test_batch_c4_full_size(scheme):
+    """The batched configuration at full size (64 seeded meshes, N ~
+    U{16..128} elements per side, P ~ U{1..5}, Mach ~ U[0.1, 0.8], alternating
+    Taylor-Green / vortex, FP32-bulk, PCG64 seed 4; 35.8 M DOF) with the
+    reference arithmetic, and as the benchmark builds it: textbook FR (VCJH
+    HU, Roe) with the Navier-Stokes terms, Re ~ U[100, 5000] (PCG64 [4, 1]).
+    Two batched steps (one workgroup per tile, then four per CU looping) equal
+    every member's own steps bit for bit."""
     import math
-    say("case: batch C4 full size, 64 seeded meshes P1-P5, fp32bulk, 2 steps, grid policies 0 and 4")
+    say(f"case: batch C4 full size, 64 seeded meshes P1-P5, {scheme}, fp32bulk, 2 steps, grid policies 0 and 4")
     rng = np.random.default_rng(4)
+    re_rng = np.random.default_rng([4, 1])
     a, b, dts = [], [], []
     for k in range(64):
         n, p, mach = int(rng.integers(16, 129)), int(rng.integers(1, 6)), float(rng.uniform(0.1, 0.8))
+        Re = float(re_rng.uniform(100.0, 5000.0))
         mesh = cases.tgv_mesh(n) if k % 2 == 0 else cases.vortex_mesh(n)
         t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
         U = (cases.tgv_ic(t.pos, mach=mach) if k % 2 == 0
              else cases.vortex_ic(t.pos, u_inf=mach * math.sqrt(1.4))).reshape(-1)
+        kw = {} if scheme == "reference" else dict(compat="textbook", riemann="roe", corr="hu", positivity=False,
+                                                   viscous=True, Re=Re, Pr=0.72)
         for lst in (a, b):
-            s = solver.Solver(t, precision="fp32bulk", cfl=0.5)
+            s = solver.Solver(t, precision="fp32bulk", cfl=0.5, **kw)
             s.set_state(U)
             lst.append(s)
         dts.append(a[-1].dt_cfl(0.5))

@@ -140,6 +140,33 @@ def test_viscous_fp32_after_100_steps():
     assert r < 1e-6
 
 
+@pytest.mark.parametrize("Re", [20.0, 1600.0])
+def test_viscous_dt_rule(Re):
+    """zj_dt_cfl of a viscous handle against the rule evaluated here: the
+    minimum over the points of cfl / ((2p+1)(|u|+c)/h + (p+1)^4 nu_k /
+    (rho h^2)), h = sqrt(area), nu_k = mu max(4/3, gamma/Pr).  The BR2
+    operator's spectral radius on squares of side h is at most 2.0 (p+1)^4
+    nu_k / h^2 for P1-P5 (measured on the oracle's Jacobian), so at cfl 0.5
+    the viscous part alone keeps dt * rho(J) near 1, inside SSP-RK3's real
+    interval (-2.51, 0).  Sheared parallelograms: h varies by element."""
+    p, g, Pr = 3, 1.4, 0.72
+    s, _, t = _pair(_sheared_mesh(8), p, "hllc", "hu", Re=Re)
+    U = _state(t, 60)
+    s.set_state(U)
+    q = U.reshape(t.E, t.S, 4)
+    u, w = q[..., 1] / q[..., 0], q[..., 2] / q[..., 0]
+    pr = (g - 1) * q[..., 0] * (q[..., 3] / q[..., 0] - 0.5 * (u * u + w * w))
+    ws = np.sqrt(u * u + w * w) + np.sqrt(g * np.maximum(pr, 1e-10) / np.maximum(q[..., 0], 1e-10))
+    h = np.sqrt(t.area)[:, None]
+    visc = (p + 1) ** 4 * (1.0 / Re) * max(4.0 / 3.0, g / Pr)
+    dt = float(np.min(0.5 / ((2 * p + 1) * ws / h + visc / (q[..., 0] * h * h))))
+    got = s.dt_cfl(0.5)
+    conv = float(np.min(0.5 * h / ((2 * p + 1) * ws)))
+    say(f"  Re {Re:g}: dt {got:.6e} vs {dt:.6e} (convective alone {conv:.6e})")
+    assert abs(got - dt) <= 1e-13 * dt
+    assert got < conv
+
+
 def test_viscous_rejects_unsupported_meshes():
     say("case: viscous terms on jittered / open meshes are refused")
     jit = cases.uniform_quad_mesh(4, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=1)

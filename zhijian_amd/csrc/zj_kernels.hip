@@ -346,7 +346,7 @@ struct BatchSmem {
 // instance uses its order's full tile, the tiling its trace buffers were
 // written with.  The launches of one stage are independent branches of the
 // step graph, so they share the GPU.
-template <typename R, int P, int MODE, bool AFF>
+template <typename R, int P, int MODE, bool AFF, bool TB>
 __global__ __launch_bounds__(BLOCK) void batched_stage_kernel(const BatchInst<R>* __restrict__ insts,
                                                               const int2* __restrict__ items, int n_items) {
     constexpr int T = BLOCK / ((P + 1) * (P + 1));
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(BLOCK) void batched_stage_kernel(const BatchInst<R>
         const int2 it = items[w];
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));  // keeps the per-thread index math in the loop body (registers)
-        stage_body<R, P, T, MODE, AFF>(insts[it.x].args, insts[it.x].ops, it.y, SM{raw}, tid);
+        stage_body<R, P, T, MODE, AFF, TB>(insts[it.x].args, insts[it.x].ops, it.y, SM{raw}, tid);
         __syncthreads();  // the next item reuses the LDS
     }
 }
@@ -470,7 +470,7 @@ __global__ void set_many_kernel(double* dst, ScalarChunk c) {
 __global__ __launch_bounds__(BLOCK) void dt_partial_kernel(const double* __restrict__ U,
                                                            const double* __restrict__ hcell,
                                                            int E, int S, double gamma, double cfl,
-                                                           double cf, double* __restrict__ partial) {
+                                                           double cf, double visc, double* __restrict__ partial) {
 #pragma clang fp contract(off)
     __shared__ double sRed[BLOCK / 64];
     const int64_t n = (int64_t)E * S;
@@ -481,7 +481,8 @@ __global__ __launch_bounds__(BLOCK) void dt_partial_kernel(const double* __restr
         const double p = (gamma - 1.0) * rho * (rE / rho - 0.5 * (u * u + v * v));
         const double ws = sqrt(u * u + v * v) + sqrt(gamma * fmax(p, 1e-10) / fmax(rho, 1e-10));
         const double h = hcell[i / S];
-        const double d = cfl * h / (ws * cf);
+        // with the viscous terms, their rate (p+1)^4 nu_k / h^2 adds to the convective one
+        const double d = visc == 0.0 ? cfl * h / (ws * cf) : cfl / (cf * ws / h + visc / (fmax(rho, 1e-10) * h * h));
         m = d < m ? d : m;
     }
     const double r = block_reduce<OpMin>(m, sRed);
@@ -593,15 +594,14 @@ hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, in
     }
 }
 
-template <typename R, int P, bool AFF>
+template <typename R, int P, bool AFF, bool TB>
 static hipError_t batched_mode(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
                                hipStream_t s) {
     const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
     switch (mode) {
-    case MODE_RESIDUAL: klaunch(batched_stage_kernel<R, P, MODE_RESIDUAL, AFF>, g, b, 0, s, insts, items, n_items); break;
-    case MODE_STAGE1: klaunch(batched_stage_kernel<R, P, MODE_STAGE1, AFF>, g, b, 0, s, insts, items, n_items); break;
-    case MODE_STAGE2: klaunch(batched_stage_kernel<R, P, MODE_STAGE2, AFF>, g, b, 0, s, insts, items, n_items); break;
-    case MODE_STAGE3: klaunch(batched_stage_kernel<R, P, MODE_STAGE3, AFF>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE1: klaunch(batched_stage_kernel<R, P, MODE_STAGE1, AFF, TB>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE2: klaunch(batched_stage_kernel<R, P, MODE_STAGE2, AFF, TB>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE3: klaunch(batched_stage_kernel<R, P, MODE_STAGE3, AFF, TB>, g, b, 0, s, insts, items, n_items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -609,12 +609,15 @@ static hipError_t batched_mode(int mode, const BatchInst<R>* insts, const int2* 
 
 template <typename R>
 hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>* insts, const int2* items,
-                                int n_items, int grid, hipStream_t s) {
+                                int n_items, int grid, hipStream_t s, bool textbook) {
     if (n_items <= 0) return hipSuccess;
 #define ZJ_BP(PP)                                                                                    \
     case PP:                                                                                         \
-        return affine ? batched_mode<R, PP, true>(mode, insts, items, n_items, grid, s)              \
-                      : batched_mode<R, PP, false>(mode, insts, items, n_items, grid, s);
+        if (textbook)                                                                                \
+            return affine ? batched_mode<R, PP, true, true>(mode, insts, items, n_items, grid, s)    \
+                          : batched_mode<R, PP, false, true>(mode, insts, items, n_items, grid, s);  \
+        return affine ? batched_mode<R, PP, true, false>(mode, insts, items, n_items, grid, s)       \
+                      : batched_mode<R, PP, false, false>(mode, insts, items, n_items, grid, s);
     switch (P) {
         ZJ_BP(1) ZJ_BP(2) ZJ_BP(3) ZJ_BP(4) ZJ_BP(5)
     default: return hipErrorInvalidValue;
@@ -650,9 +653,9 @@ hipError_t launch_batched_all(int mode, const BatchInst<R>* insts, const int2* i
 template hipError_t launch_batched_all<double>(int, const BatchInst<double>*, const int2*, int, int, int, hipStream_t);
 template hipError_t launch_batched_all<float>(int, const BatchInst<float>*, const int2*, int, int, int, hipStream_t);
 template hipError_t launch_batched_stage<double>(int, bool, int, const BatchInst<double>*, const int2*, int, int,
-                                                 hipStream_t);
+                                                 hipStream_t, bool);
 template hipError_t launch_batched_stage<float>(int, bool, int, const BatchInst<float>*, const int2*, int, int,
-                                                hipStream_t);
+                                                hipStream_t, bool);
 
 template hipError_t launch_stage<double>(int, int, const StageArgs<double>&, const OpTables<double>&, int, bool,
                                         hipStream_t, bool);
@@ -702,8 +705,8 @@ hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt,
 }
 
 hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma, double cfl,
-                             double cf, double* partial, int nblk, hipStream_t s) {
-    dt_partial_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, hcell, E, S, gamma, cfl, cf, partial);
+                             double cf, double visc, double* partial, int nblk, hipStream_t s) {
+    dt_partial_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, hcell, E, S, gamma, cfl, cf, visc, partial);
     return hipGetLastError();
 }
 

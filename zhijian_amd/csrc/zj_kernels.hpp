@@ -132,14 +132,24 @@ struct TraceArgs {
     int xcd_groups;
 };
 
+// Viscous terms of the textbook scheme (zj_visc.hip): constant viscosity and
+// conductivity, BR2 penalty, and the correction derivatives g_L'(-1), g_R'(+1).
+struct ViscParams {
+    double mu, kappa, eta, gLe, gRe;
+};
+
 // One mesh of a batched launch: its stage arguments for one mode, its
-// operator tables and order.
+// operator tables and order, its scheme (reference arithmetic or textbook,
+// with the viscous parameters of the Navier-Stokes terms).
 template <typename R>
 struct BatchInst {
     StageArgs<R> args;
     OpTables<R> ops;
     int P;
     int affine;
+    int textbook;
+    int viscous;
+    ViscParams vp;
 };
 
 // Launch wrappers (host).  `epb` = elements per workgroup (0 = default).
@@ -153,7 +163,7 @@ hipError_t launch_trace(int P, const TraceArgs<R>& a, const OpTables<R>& ops, in
 // instances' arguments for `mode`.
 template <typename R>
 hipError_t launch_batched_stage(int P, bool affine, int mode, const BatchInst<R>* insts, const int2* items,
-                                int n_items, int grid, hipStream_t s);
+                                int n_items, int grid, hipStream_t s, bool textbook = false);
 // The same stages for every member of a batch in ONE launch per stage:
 // items[k] = (index into insts, tile), insts across all orders and metric
 // forms; lds_bytes = batched_all_lds of the largest order present.
@@ -164,11 +174,6 @@ int batched_all_lds(int P, bool fp32);
 int default_epb(int P);
 int max_epb(int P);
 
-// Viscous terms of the textbook scheme (zj_visc.hip): constant viscosity and
-// conductivity, BR2 penalty, and the correction derivatives g_L'(-1), g_R'(+1).
-struct ViscParams {
-    double mu, kappa, eta, gLe, gRe;
-};
 // Stage launch with viscous terms (affine quadrilaterals, no boundary faces,
 // traces of 12 words per flux point: U, dU/dx, dU/dy); tile = max_epb(P).
 template <typename R>
@@ -177,6 +182,10 @@ hipError_t launch_visc_stage(int P, int mode, const StageArgs<R>& a, const OpTab
 template <typename R>
 hipError_t launch_visc_trace(int P, int E, const double* U, const R* Jinv_e, R* T, const OpTables<R>& ops,
                              int xcd_groups, hipStream_t s);
+// The viscous stage for many meshes of order P (items = (instance, tile)).
+template <typename R>
+hipError_t launch_batched_visc(int P, int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                               hipStream_t s);
 int num_blocks(int E, int epb);
 
 // perm[e] = caller index of device element e (nullptr: same order), S points per element
@@ -194,9 +203,11 @@ hipError_t launch_set_many(double* dst, const double* v, int n, hipStream_t s);
 hipError_t launch_rk3_combine(int stage, double* out, const double* U0, const double* Uk, const double* R,
                               int64_t n, double dt, hipStream_t s);
 hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt, hipStream_t s);
-// dt = min over points of cfl*h_e/(ws*cf), h_e = sqrt(area_e); partial per block.
+// dt = min over points of cfl*h_e/(ws*cf), h_e = sqrt(area_e); with visc > 0
+// (the viscous terms: (p+1)^4 max(4/3 mu, (gamma-1) kappa)) the minimum of
+// cfl / (cf*ws/h_e + visc/(rho h_e^2)); partial per block.
 hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma,
-                             double cfl, double cf, double* partial, int nblk, hipStream_t s);
+                             double cfl, double cf, double visc, double* partial, int nblk, hipStream_t s);
 hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count,
                                   int nblk, hipStream_t s);
 // Deterministic reduction of block partials (fixed order): sqrt of the sum

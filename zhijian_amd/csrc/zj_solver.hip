@@ -738,7 +738,12 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
     if (!h->has_area) return fail(h, ZJ_ERR_STATE, "element areas were not provided");
     if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "dt requested in the middle of a step");
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, h->partial_dt,
+    // BR2 on quadrilaterals: the viscous operator's spectral radius is below
+    // 2.0 (p+1)^4 nu_k / h^2 (h the side; measured on the oracle's Jacobian, P1-P5)
+    const double p1 = h->P + 1.0;
+    const double visc =
+        h->viscous ? p1 * p1 * p1 * p1 * std::max(h->vp.mu * 4.0 / 3.0, h->vp.kappa * (h->gamma - 1.0)) : 0.0;
+    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, visc, h->partial_dt,
                                 h->nred, h->stream));
     ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, h->stream));
     ZJ_HIP(h, hipMemcpyAsync(h->h_pinned, h->d_scalars + 2, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -852,10 +857,21 @@ extern "C" int zj_get_info(zj_handle h, int64_t* n_dof, int* epb, int* n_sp) {
 
 // ------------------------------------------------------------ measurement
 namespace {
+// Grid-stride copy of 32-byte elements, U loads in flight per thread before
+// their stores.
+template <int U>
 __global__ __launch_bounds__(256) void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst,
                                                    int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        double4 v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = src[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < U; ++k) dst[i + k * stride] = v[k];
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 }  // namespace
 
@@ -881,18 +897,28 @@ extern "C" int zj_copy_probe(int device, int64_t bytes, int reps, double* gbps) 
         h->err = "copy probe: allocation failed";
         return fin(ZJ_ERR_HIP);
     }
-    const int grid = 256 * 8 * 4;
-    copy_kernel<<<grid, 256>>>((const double4*)a, (double4*)b, n);  // warm-up
-    hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; ++r) copy_kernel<<<grid, 256>>>((const double4*)a, (double4*)b, n);
-    hipEventRecord(e1, nullptr);
-    if (hipEventSynchronize(e1) != hipSuccess) {
-        h->err = "copy probe: kernel failed";
-        return fin(ZJ_ERR_HIP);
+    // the best of a few launch shapes: 8192..32768 workgroups, 1 or 8 loads in flight
+    double best = 0.0;
+    for (int grid : {8192, 16384, 32768}) {
+        for (int unroll : {1, 8}) {
+            auto launch = [&] {
+                if (unroll == 1) copy_kernel<1><<<grid, 256>>>((const double4*)a, (double4*)b, n);
+                else copy_kernel<8><<<grid, 256>>>((const double4*)a, (double4*)b, n);
+            };
+            launch();  // warm-up
+            hipEventRecord(e0, nullptr);
+            for (int r = 0; r < reps; ++r) launch();
+            hipEventRecord(e1, nullptr);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                h->err = "copy probe: kernel failed";
+                return fin(ZJ_ERR_HIP);
+            }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            best = std::max(best, 2.0 * (double)n * 32.0 * reps / (ms * 1e-3) / 1e9);
+        }
     }
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    *gbps = 2.0 * (double)n * 32.0 * reps / (ms * 1e-3) / 1e9;
+    *gbps = best;
     return fin(rc);
 }
 
@@ -1349,6 +1375,8 @@ struct zj_batch {
     struct Group {
         int P = 0;
         bool affine = false;
+        bool textbook = false;
+        bool viscous = false;
         int n_items = 0;
         int2* items = nullptr;
         void* insts[4] = {nullptr, nullptr, nullptr, nullptr};  // BatchInst<R>[members of the group] per mode
@@ -1357,8 +1385,10 @@ struct zj_batch {
     };
     std::vector<Group> groups;
     std::vector<std::vector<int>> group_members;  // member indices of each group
-    // single launch per stage over every member (all orders and metric forms)
+    // single launch per stage over every member (all orders and metric forms;
+    // reference-arithmetic members only)
     bool single = false;
+    bool any_textbook = false;
     void* all_insts[4] = {nullptr, nullptr, nullptr, nullptr};  // BatchInst<R>[members] per mode
     int2* all_items = nullptr;  // (member, tile), larger members first
     int n_all = 0;
@@ -1416,6 +1446,9 @@ static int batch_upload_insts(zj_batch* b, void** insts, const std::vector<int>&
                                                                                              : (const void*)&h->ops32);
             in.P = h->P;
             in.affine = h->affine ? 1 : 0;
+            in.textbook = h->textbook ? 1 : 0;
+            in.viscous = h->viscous ? 1 : 0;
+            in.vp = h->vp;
             v.push_back(in);
         }
         if (!insts[mode]) ZJ_BHIP(b, hipMalloc(&insts[mode], v.size() * sizeof(BatchInst<R>)));
@@ -1446,10 +1479,7 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
             b->err = "members must use the full tile of their order";
             return ZJ_ERR_UNSUPPORTED;
         }
-        if (h->textbook) {
-            b->err = "batches run the reference arithmetic";
-            return ZJ_ERR_UNSUPPORTED;
-        }
+        b->any_textbook |= h->textbook;
         for (int j = 0; j < i; ++j)
             if (hs[j] == h) { b->err = "a handle appears twice"; return ZJ_ERR_ARG; }
     }
@@ -1461,33 +1491,39 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
     b->n_cu = std::max(1, prop.multiProcessorCount);
     ZJ_BHIP(b, hipMalloc((void**)&b->d_dt, (size_t)n * sizeof(double)));
     b->dt_on_device.assign(n, NAN);
+    // groups by (order, metric form, scheme): reference arithmetic, textbook
+    // Euler, textbook Navier-Stokes
+    auto scheme = [](const zj_solver* h) { return h->viscous ? 2 : h->textbook ? 1 : 0; };
     for (int P = 1; P <= 5; ++P)
-        for (int af = 0; af < 2; ++af) {
-            std::vector<int> members;
-            for (int i = 0; i < n; ++i)
-                if (hs[i]->P == P && (hs[i]->affine ? 1 : 0) == af) members.push_back(i);
-            if (members.empty()) continue;
-            // larger meshes first; tiles of one mesh stay consecutive
-            std::stable_sort(members.begin(), members.end(), [&](int x, int y) { return hs[x]->E > hs[y]->E; });
-            zj_batch::Group g;
-            g.P = P;
-            g.affine = af != 0;
-            std::vector<int2> items;
-            for (size_t k = 0; k < members.size(); ++k)
-                for (int t = 0; t < hs[members[k]]->nblk; ++t) items.push_back(make_int2((int)k, t));
-            g.n_items = (int)items.size();
-            b->groups.push_back(g);
-            zj_batch::Group& gg = b->groups.back();
-            ZJ_BHIP(b, hipMalloc((void**)&gg.items, std::max<size_t>(items.size(), 1) * sizeof(int2)));
-            if (!items.empty())
-                ZJ_BHIP(b, hipMemcpy(gg.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
-            ZJ_BHIP(b, hipStreamCreateWithFlags(&gg.branch, hipStreamNonBlocking));
-            ZJ_BHIP(b, hipEventCreateWithFlags(&gg.join, hipEventDisableTiming));
-            b->group_members.push_back(members);
-            int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_insts<double>(b, gg.insts, members)
-                                             : batch_upload_insts<float>(b, gg.insts, members);
-            if (rc) return rc;
-        }
+        for (int af = 0; af < 2; ++af)
+            for (int sch = 0; sch < 3; ++sch) {
+                std::vector<int> members;
+                for (int i = 0; i < n; ++i)
+                    if (hs[i]->P == P && (hs[i]->affine ? 1 : 0) == af && scheme(hs[i]) == sch) members.push_back(i);
+                if (members.empty()) continue;
+                // larger meshes first; tiles of one mesh stay consecutive
+                std::stable_sort(members.begin(), members.end(), [&](int x, int y) { return hs[x]->E > hs[y]->E; });
+                zj_batch::Group g;
+                g.P = P;
+                g.affine = af != 0;
+                g.textbook = sch >= 1;
+                g.viscous = sch == 2;
+                std::vector<int2> items;
+                for (size_t k = 0; k < members.size(); ++k)
+                    for (int t = 0; t < hs[members[k]]->nblk; ++t) items.push_back(make_int2((int)k, t));
+                g.n_items = (int)items.size();
+                b->groups.push_back(g);
+                zj_batch::Group& gg = b->groups.back();
+                ZJ_BHIP(b, hipMalloc((void**)&gg.items, std::max<size_t>(items.size(), 1) * sizeof(int2)));
+                if (!items.empty())
+                    ZJ_BHIP(b, hipMemcpy(gg.items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+                ZJ_BHIP(b, hipStreamCreateWithFlags(&gg.branch, hipStreamNonBlocking));
+                ZJ_BHIP(b, hipEventCreateWithFlags(&gg.join, hipEventDisableTiming));
+                b->group_members.push_back(members);
+                int rc = b->prec == ZJ_PREC_FP64 ? batch_upload_insts<double>(b, gg.insts, members)
+                                                 : batch_upload_insts<float>(b, gg.insts, members);
+                if (rc) return rc;
+            }
     // the single-launch form: one list of every member's tiles, larger members first
     {
         std::vector<int> order(n);
@@ -1553,12 +1589,18 @@ static hipError_t batch_enqueue(zj_batch* b, hipStream_t s) {
         for (auto& g : b->groups) {
             if (e == hipSuccess) e = hipStreamWaitEvent(g.branch, b->fork, 0);
             if (e == hipSuccess) {
-                if (b->prec == ZJ_PREC_FP64)
+                if (g.viscous && b->prec == ZJ_PREC_FP64)
+                    e = launch_batched_visc<double>(g.P, mode, (const BatchInst<double>*)g.insts[mode], g.items,
+                                                    g.n_items, batch_grid(b, g), g.branch);
+                else if (g.viscous)
+                    e = launch_batched_visc<float>(g.P, mode, (const BatchInst<float>*)g.insts[mode], g.items,
+                                                   g.n_items, batch_grid(b, g), g.branch);
+                else if (b->prec == ZJ_PREC_FP64)
                     e = launch_batched_stage<double>(g.P, g.affine, mode, (const BatchInst<double>*)g.insts[mode],
-                                                     g.items, g.n_items, batch_grid(b, g), g.branch);
+                                                     g.items, g.n_items, batch_grid(b, g), g.branch, g.textbook);
                 else
                     e = launch_batched_stage<float>(g.P, g.affine, mode, (const BatchInst<float>*)g.insts[mode],
-                                                    g.items, g.n_items, batch_grid(b, g), g.branch);
+                                                    g.items, g.n_items, batch_grid(b, g), g.branch, g.textbook);
             }
             if (e == hipSuccess) e = hipEventRecord(g.join, g.branch);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, g.join, 0);
@@ -1636,6 +1678,10 @@ extern "C" int zj_batch_set_grid(zj_batch* b, int policy) {
 
 extern "C" int zj_batch_set_launch(zj_batch* b, int single) {
     if (!b || (single != 0 && single != 1)) return ZJ_ERR_ARG;
+    if (single && b->any_textbook) {
+        b->err = "the single launch runs the reference arithmetic; this batch has textbook members";
+        return ZJ_ERR_UNSUPPORTED;
+    }
     if ((single != 0) != b->single) {
         b->single = single != 0;
         if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
@@ -1707,6 +1753,7 @@ extern "C" int zj_batch_autotune(zj_batch* b, const double* dts, int reps, int* 
         int bs = b->single ? 1 : 0, bp = b->policy;
         for (int cs : cand_single)
             for (int cp : cand_policy) {
+                if (cs && b->any_textbook) continue;
                 if (int r = zj_batch_set_launch(b, cs)) return r;
                 if (int r = zj_batch_set_grid(b, cp)) return r;
                 if (int r = zj_batch_rk3_step(b, dts, b->stream)) return r;  // captures and warms up

@@ -73,9 +73,10 @@ __device__ __forceinline__ void point_gradient(const R* u, const R* D, int i, in
     gy = jy * dxi + ky * deta;
 }
 
+// One tile's viscous stage (the workgroup's threads, the tile's elements).
 template <typename R, int P, int EPB, int MODE>
-__global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
-                                                           const ViscParams vp) {
+__device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                                          const int tile) {
     using Y = Lay<R, P>;
     using SM = ViscSmem<R, P, EPB>;
     constexpr bool INC = std::is_same<R, float>::value;
@@ -97,7 +98,6 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
     unsigned char* const sOut = SM::Out();
 
     const int tid = threadIdx.x;
-    const int tile = tile_of_block(a.xcd_groups);
     const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
@@ -340,6 +340,25 @@ __global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a,
         store4(T, tu);
         store4(T + NV, tx);
         store4(T + 2 * NV, ty);
+    }
+}
+
+template <typename R, int P, int EPB, int MODE>
+__global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
+                                                           const ViscParams vp) {
+    visc_body<R, P, EPB, MODE>(a, ops, vp, tile_of_block(a.xcd_groups));
+}
+
+// The stage for many meshes of order P: work item w = (instance, tile); a
+// workgroup takes items blockIdx.x, blockIdx.x + gridDim.x, ...
+template <typename R, int P, int EPB, int MODE>
+__global__ __launch_bounds__(BLOCK) void batched_visc_kernel(const BatchInst<R>* __restrict__ insts,
+                                                             const int2* __restrict__ items, int n_items) {
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const int2 it = items[w];
+        const BatchInst<R>& in = insts[it.x];
+        visc_body<R, P, EPB, MODE>(in.args, in.ops, in.vp, it.y);
+        __syncthreads();  // the next item reuses the LDS
     }
 }
 
@@ -1097,6 +1116,34 @@ hipError_t launch_visc_stage(int P, int mode, const StageArgs<R>& a, const OpTab
     }
 }
 
+template <typename R, int P>
+hipError_t batched_visc_p(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                          hipStream_t s) {
+    constexpr int EPB = BLOCK / ((P + 1) * (P + 1));
+    const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
+    switch (mode) {
+    case MODE_STAGE1: klaunch(batched_visc_kernel<R, P, EPB, MODE_STAGE1>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE2: klaunch(batched_visc_kernel<R, P, EPB, MODE_STAGE2>, g, b, 0, s, insts, items, n_items); break;
+    case MODE_STAGE3: klaunch(batched_visc_kernel<R, P, EPB, MODE_STAGE3>, g, b, 0, s, insts, items, n_items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_batched_visc(int P, int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                               hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    switch (P) {
+    case 1: return batched_visc_p<R, 1>(mode, insts, items, n_items, grid, s);
+    case 2: return batched_visc_p<R, 2>(mode, insts, items, n_items, grid, s);
+    case 3: return batched_visc_p<R, 3>(mode, insts, items, n_items, grid, s);
+    case 4: return batched_visc_p<R, 4>(mode, insts, items, n_items, grid, s);
+    case 5: return batched_visc_p<R, 5>(mode, insts, items, n_items, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename R>
 hipError_t launch_visc_trace(int P, int E, const double* U, const R* Jinv, R* T, const OpTables<R>& ops, int groups,
                              hipStream_t s) {
@@ -1113,6 +1160,10 @@ template hipError_t launch_visc_stage<double>(int, int, const StageArgs<double>&
                                               const ViscParams&, hipStream_t);
 template hipError_t launch_visc_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&,
                                              const ViscParams&, hipStream_t);
+template hipError_t launch_batched_visc<double>(int, int, const BatchInst<double>*, const int2*, int, int,
+                                                hipStream_t);
+template hipError_t launch_batched_visc<float>(int, int, const BatchInst<float>*, const int2*, int, int,
+                                               hipStream_t);
 template hipError_t launch_visc_trace<double>(int, int, const double*, const double*, double*,
                                               const OpTables<double>&, int, hipStream_t);
 template hipError_t launch_visc_trace<float>(int, int, const double*, const float*, float*, const OpTables<float>&,
