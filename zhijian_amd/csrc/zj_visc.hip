@@ -73,8 +73,19 @@ __device__ __forceinline__ void point_gradient(const R* u, const R* D, int i, in
     gy = jy * dxi + ky * deta;
 }
 
-// One tile's viscous stage (the workgroup's threads, the tile's elements).
-template <typename R, int P, int EPB, int MODE>
+// Threads of a workgroup for a tile of EPB elements: its points or its face
+// points, whichever are more, in whole waves, at most BLOCK (the full tile
+// takes BLOCK; the adjoint's smaller tiles leave no wave idle).
+template <int P, int EPB>
+constexpr int visc_threads() {
+    constexpr int S = (P + 1) * (P + 1), NF = EPB * 4 * (P + 1);
+    constexpr int n = EPB * S > NF ? EPB * S : NF;
+    constexpr int r = (n + 63) / 64 * 64;
+    return r < BLOCK ? r : BLOCK;
+}
+
+// One tile's viscous stage (the workgroup's NTHR threads, the tile's elements).
+template <typename R, int P, int EPB, int MODE, int NTHR>
 __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
                                           const int tile) {
     using Y = Lay<R, P>;
@@ -83,7 +94,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
-    static_assert(NT <= BLOCK, "tile exceeds the workgroup");
+    static_assert(NT <= NTHR, "tile exceeds the workgroup");
     R* const sU = SM::U();
     R* const sGx = SM::Gx();
     R* const sGy = SM::Gy();
@@ -103,7 +114,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
     const R gLe = R(vp.gLe), gRe = R(vp.gRe);
 
-    for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
+    for (int t = tid; t < N1 * N1; t += NTHR) sD[t] = ops.D[t];
     if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; sC[tid] = ops.gL[tid]; sC[N1 + tid] = ops.gR[tid]; }
 
     const int el = tid / S, sp = tid - (tid / S) * S;
@@ -134,7 +145,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
             point_gradient<R, P>(sU + el * Y::UE + v * Y::UV, sD, i, j, jx, jy, kx, ky, sGx[ub + v * Y::UV],
                                  sGy[ub + v * Y::UV]);
     }
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         if (e0 + el2 < a.E) {
             const TraceLine tl = trace_line<P>(k, f);
@@ -148,7 +159,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     // ---- B2a: jumps and the common viscous normal flux per (edge, flux point)
     // (kept apart from the Riemann solve so that the two never hold their
     // registers at once)
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
@@ -200,7 +211,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     __syncthreads();
 
     // ---- B2b: Riemann flux; total common normal flux, outward and scaled
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
@@ -253,7 +264,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     __syncthreads();
 
     // ---- D: flux differences (into the jump slots)
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         if (e0 + el2 >= a.E) continue;
         const TraceLine tl = trace_line<P>(k, f);
@@ -310,7 +321,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
         }
     }
     if (a.l2_partial) {
-        const double s = block_reduce<OpSum>(rsq, sRed);
+        const double s = block_reduce_n<OpSum, NTHR / 64>(rsq, sRed);
         if (tid == 0) a.l2_partial[tile] = s;
     }
     if (MODE == MODE_RESIDUAL) return;
@@ -324,7 +335,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
                                  sGy[ub + v * Y::UV]);
     }
     __syncthreads();
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E || !sOut[el2 * 4 + k]) continue;
@@ -343,10 +354,10 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     }
 }
 
-template <typename R, int P, int EPB, int MODE>
-__global__ __launch_bounds__(BLOCK) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
-                                                           const ViscParams vp) {
-    visc_body<R, P, EPB, MODE>(a, ops, vp, tile_of_block(a.xcd_groups));
+template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
+__global__ __launch_bounds__(NTHR) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
+                                                          const ViscParams vp) {
+    visc_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups));
 }
 
 // The stage for many meshes of order P: work item w = (instance, tile); a
@@ -357,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void batched_visc_kernel(const BatchInst<R>*
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
         const int2 it = items[w];
         const BatchInst<R>& in = insts[it.x];
-        visc_body<R, P, EPB, MODE>(in.args, in.ops, in.vp, it.y);
+        visc_body<R, P, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, it.y);
         __syncthreads();  // the next item reuses the LDS
     }
 }
@@ -489,15 +500,15 @@ __device__ __forceinline__ void visc_face_point(ViscFacePoint<R, P>& F, const R*
     (void)sU;
 }
 
-template <typename R, int P, int EPB, int MODE>
-__global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops,
+template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
+__global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops,
                                                          const ViscParams vp) {
     using Y = Lay<R, P>;
     constexpr bool INC = std::is_same<R, float>::value;
     constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
-    static_assert(NT <= BLOCK, "tile exceeds the workgroup");
+    static_assert(NT <= NTHR, "tile exceeds the workgroup");
     constexpr double ca = MODE == MODE_STAGE2 ? 0.75 : (MODE == MODE_STAGE3 ? 1.0 / 3.0 : 0.0);
     constexpr double cb = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
     constexpr double cc = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
@@ -516,7 +527,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
     const R gLe = R(vp.gLe), gRe = R(vp.gRe);
-    for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
+    for (int t = tid; t < N1 * N1; t += NTHR) sD[t] = ops.D[t];
     if (tid < N1) { sLm[tid] = ops.Lm[tid]; sLp[tid] = ops.Lp[tid]; sC[tid] = ops.gL[tid]; sC[N1 + tid] = ops.gR[tid]; }
 
     const int el = tid / S, sp = tid - (tid / S) * S;
@@ -551,7 +562,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
             point_gradient<R, P>(sU + el * Y::UE + v * Y::UV, sD, i, j, jx, jy, kx, ky, sGx[ub + v * Y::UV],
                                  sGy[ub + v * Y::UV]);
     }
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
@@ -582,7 +593,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
     __syncthreads();
 
     // ---- forward B2: jumps, common fluxes (viscous and Riemann), outward and scaled
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
@@ -642,7 +653,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
     __syncthreads();
 
     // ---- forward D: flux differences (in place of the common fluxes)
-    for (int t = tid; need_res && t < NF; t += BLOCK) {
+    for (int t = tid; need_res && t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         if (e0 + el2 >= a.E) continue;
         const TraceLine tl = trace_line<P>(k, f);
@@ -722,13 +733,13 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
         }
     }
     if (a.dtb_partial) {
-        const double s = block_reduce<OpSum>(rsum, sRed);
+        const double s = block_reduce_n<OpSum, NTHR / 64>(rsum, sRed);
         if (tid == 0) a.dtb_partial[tile] = s;
     }
     __syncthreads();
 
     // ---- R2: cotangents of the flux differences (in place)
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         if (e0 + el2 >= a.E) continue;
         const R* Rb = sW3 + el2 * Y::UE;
@@ -788,7 +799,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
     __syncthreads();
 
     // ---- R4: face points backwards
-    for (int t = tid; t < NF; t += BLOCK) {
+    for (int t = tid; t < NF; t += NTHR) {
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
         if (e2 >= a.E) continue;
@@ -890,7 +901,7 @@ __global__ __launch_bounds__(BLOCK) void visc_vjp_kernel(const VjpArgs<R> a, con
     __syncthreads();
 
     if (a.mub_partial) {
-        const double sm = block_reduce<OpSum>(musum, sRed);
+        const double sm = block_reduce_n<OpSum, NTHR / 64>(musum, sRed);
         if (tid == 0) a.mub_partial[tile] = sm;
     }
 
@@ -984,12 +995,12 @@ __global__ __launch_bounds__(BLOCK) void visc_complete_kernel(const CompleteArgs
 template <typename R, int P>
 hipError_t visc_adj_stage_p(int mode, const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
                             hipStream_t s) {
-    constexpr int EPB = visc_adj_epb<R, P>();
+    constexpr int EPB = visc_adj_epb<R, P>(), NTHR = visc_threads<P, EPB>();
     const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
     switch (mode) {
-    case MODE_STAGE1: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE1>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
-    case MODE_STAGE2: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE2>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
-    case MODE_STAGE3: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE3>, grid, dim3(BLOCK), 0, s, a, ops, vp); break;
+    case MODE_STAGE1: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE1>, grid, dim3(NTHR), 0, s, a, ops, vp); break;
+    case MODE_STAGE2: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE2>, grid, dim3(NTHR), 0, s, a, ops, vp); break;
+    case MODE_STAGE3: klaunch(visc_stage_kernel<R, P, EPB, MODE_STAGE3>, grid, dim3(NTHR), 0, s, a, ops, vp); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -997,13 +1008,13 @@ hipError_t visc_adj_stage_p(int mode, const StageArgs<R>& a, const OpTables<R>& 
 
 template <typename R, int P>
 hipError_t visc_vjp_p(int mode, const VjpArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp, hipStream_t s) {
-    constexpr int EPB = visc_adj_epb<R, P>();
+    constexpr int EPB = visc_adj_epb<R, P>(), NTHR = visc_threads<P, EPB>();
     const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
     switch (mode) {
-    case MODE_RESIDUAL: visc_vjp_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    case MODE_STAGE1: visc_vjp_kernel<R, P, EPB, MODE_STAGE1><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    case MODE_STAGE2: visc_vjp_kernel<R, P, EPB, MODE_STAGE2><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
-    default: visc_vjp_kernel<R, P, EPB, MODE_STAGE3><<<grid, BLOCK, 0, s>>>(a, ops, vp); break;
+    case MODE_RESIDUAL: visc_vjp_kernel<R, P, EPB, MODE_RESIDUAL><<<grid, NTHR, 0, s>>>(a, ops, vp); break;
+    case MODE_STAGE1: visc_vjp_kernel<R, P, EPB, MODE_STAGE1><<<grid, NTHR, 0, s>>>(a, ops, vp); break;
+    case MODE_STAGE2: visc_vjp_kernel<R, P, EPB, MODE_STAGE2><<<grid, NTHR, 0, s>>>(a, ops, vp); break;
+    default: visc_vjp_kernel<R, P, EPB, MODE_STAGE3><<<grid, NTHR, 0, s>>>(a, ops, vp); break;
     }
     return hipGetLastError();
 }
