@@ -110,12 +110,13 @@ def time_adjoint(n, p, precision, steps, warmup, stream, hiprt):
                 note="adjoint step = traces of U^n + stages 1-2 recomputed + 3 VJP launches")
 
 
-def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None):
+def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None, tape_segments=1):
     """The whole checkpointed gradient of the adjoint configuration (TGV n x n,
     order p, Mach 0.1, textbook VCJH HU / Roe, with the Navier-Stokes terms at
     Re when given, kinetic energy after `steps` SSP-RK3 steps, checkpoints
-    every `every` steps on the GPU) against the same `steps` forward steps
-    alone (graph-replayed), wall time each."""
+    every `every` steps on the GPU, the last `tape_segments` segments recorded
+    on the device tape by the forward pass) against the same `steps` forward
+    steps alone (graph-replayed), wall time each."""
     from zhijian_amd import adjoint, cases, solver
     mesh = cases.tgv_mesh(n)
     t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
@@ -136,19 +137,24 @@ def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None):
     e1.record(stream)
     e1.synchronize()
     fwd_ms = e0.elapsed_ms(e1)
-    adjoint.gradient(s, t, U0, dts[:2 * every], every=every, want_dt=False)  # warm-up (graph capture, allocation)
+    # warm-up (graph capture, allocation of the slots and of the whole tape)
+    adjoint.gradient(s, t, U0, dts, every=every, want_dt=False, tape_segments=tape_segments)
     hiprt.hip().hipDeviceSynchronize()
     t0 = time.perf_counter()
-    adjoint.gradient(s, t, U0, dts, every=every, want_dt=False)
+    adjoint.gradient(s, t, U0, dts, every=every, want_dt=False, tape_segments=tape_segments)
     hiprt.hip().hipDeviceSynchronize()
     grad_ms = (time.perf_counter() - t0) * 1e3
     physics = f"Navier-Stokes (BR2) Re {Re:g}" if Re else "Euler"
+    n_seg = (steps + every - 1) // every
+    rec = min(tape_segments, n_seg)
     return dict(workload=f"TGV {n}x{n} P{p}, Mach 0.1, textbook VCJH HU / Roe, {physics}, {steps} steps, "
                          f"checkpoints every {every} (device-resident), kinetic-energy objective",
-                precision=precision, forward_ms=fwd_ms, gradient_ms=grad_ms, gradient_over_forward=grad_ms / fwd_ms,
-                note="gradient = forward with checkpoints + per segment: forward recompute recorded on a device "
-                     "tape + stage VJPs from the tape (both passes native); host wall time around the whole "
-                     "call, including the host objective and two state transfers")
+                precision=precision, tape_segments=rec, forward_ms=fwd_ms, gradient_ms=grad_ms,
+                gradient_over_forward=grad_ms / fwd_ms,
+                note=f"gradient = forward with checkpoints (its last {rec} of {n_seg} segments recorded on the "
+                     f"device tape as it steps) + per other segment a forward recompute recorded on the tape + "
+                     f"stage VJPs from the tape (both passes native); host wall time around the whole call, "
+                     f"including the host objective and two state transfers")
 
 
 BATCH_SCHEME = dict(compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Pr=0.72)
@@ -816,6 +822,12 @@ def main():
             log(f"gradient {prec}: {adj[prec]['checkpointed_gradient']}")
             adj[prec]["checkpointed_gradient_euler"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt)
             log(f"gradient {prec} (Euler terms): {adj[prec]['checkpointed_gradient_euler']}")
+            # the whole trajectory on the tape (0.1 GB per step here: 26 GB of HBM), no step computed twice
+            adj[prec]["taped_gradient"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt, Re=1600.0,
+                                                        tape_segments=16)
+            log(f"gradient {prec}, whole tape: {adj[prec]['taped_gradient']}")
+            adj[prec]["taped_gradient_euler"] = time_gradient(128, 3, prec, 256, 16, stream, hiprt, tape_segments=16)
+            log(f"gradient {prec} (Euler terms), whole tape: {adj[prec]['taped_gradient_euler']}")
 
     gpu_api = time_gpu_api(tables, U0, max(args.steps // 5, 10), stream, hiprt)
     log(f"gpu api: {gpu_api}")

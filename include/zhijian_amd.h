@@ -240,9 +240,17 @@ int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream);
  * states and traces, every steps), then the stage VJPs from the tape; the
  * handle then holds U^0 and the cotangent of U^0 (zj_adjoint_get).  dt_bar
  * (n_steps values, or NULL) receives dJ/d(dt_n); with NULL the sweep does not
- * synchronize. */
+ * synchronize.
+ * zj_adjoint_tape_segments: the last n segments (default 1; 0 = none) are
+ * recorded on the tape by zj_forward_checkpointed itself, as it steps (at the
+ * adjoint tile: same results), and zj_adjoint_sweep of the same trajectory
+ * (n_steps, every, dts) then skips their recompute.  The tape holds
+ * n * every steps of 3 states and 3 trace sets in device memory (TGV 128^2
+ * P3 with the Navier-Stokes terms, FP64: 0.1 GB per step).  A sweep that
+ * recomputes any segment, zj_state_save and zj_state_slots end the record. */
 int zj_forward_checkpointed(zj_handle h, const double* dts, int n_steps, int every, void* stream);
 int zj_adjoint_sweep(zj_handle h, const double* dts, int n_steps, int every, double* dt_bar, void* stream);
+int zj_adjoint_tape_segments(zj_handle h, int n);
 /* With the Navier-Stokes terms (textbook mode, BR2, navierstokes.hpp:53-99,
  * 170-205): dJ/dRe accumulated over the steps run backwards since
  * zj_adjoint_set (0 without viscous terms).  The viscous fluxes scale with

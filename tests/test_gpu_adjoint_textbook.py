@@ -155,3 +155,43 @@ def test_native_sweep_equals_stepwise(p, viscous):
     rd = max(abs(a - b) / abs(b) for a, b in zip(dtb1, dtb2))
     say(f"  cotangent rel L2 {r:.3e}, dt cotangents max rel {rd:.3e}")
     assert r < 1e-12 and rd < 1e-10
+
+
+@pytest.mark.parametrize("precision,viscous", [("fp64", False), ("fp64", True), ("fp32bulk", True)])
+def test_tape_recorded_by_forward_equals_recomputed(precision, viscous):
+    """Segments recorded on the tape by the forward pass itself (none, the
+    last one, all three) give the same final state, cotangents, dt and Re
+    cotangents, bit for bit; a second sweep of the same trajectory after a
+    sweep that recomputed reuses nothing stale."""
+    say(f"case: tape recorded by the forward pass vs recomputed, TGV 8x8 P3 {precision} viscous {viscous}")
+    mesh = cases.uniform_quad_mesh(8, 8, 0.0, 2 * np.pi, 0.0, 2 * np.pi)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 3, periodic=(mesh.Lx, mesh.Ly))
+    extra = dict(viscous=True, Re=300.0) if viscous else {}
+    s = solver.Solver(t, precision=precision, compat="textbook", riemann="hllc", corr="hu", positivity=False,
+                      **extra)
+    U0 = tgv_state(t, amp=0.02, seed=95, mach=0.3)
+    dts = [1e-3 * (1.0 + 0.05 * k) for k in range(8)]
+    ubar = np.random.default_rng(96).standard_normal(U0.shape)
+    out = []
+    for segs in (0, 1, 3, 0):
+        s.adjoint_tape_segments(segs)
+        s.set_state(U0)
+        s.forward_checkpointed(dts, 3)
+        UN = s.get_state()
+        s.adjoint_set(ubar)
+        dtb = s.adjoint_sweep(dts, 3, want_dt=True)
+        out.append((UN, s.adjoint_get(), dtb, s.adjoint_get_re()))
+    for k in (1, 2, 3):
+        assert np.array_equal(out[k][0], out[0][0]), k
+        assert np.array_equal(out[k][1], out[0][1]), k
+        assert out[k][2] == out[0][2], k
+        assert out[k][3] == out[0][3], k
+    # the handle steps on as before: the recorded segments left its state and traces consistent
+    s.adjoint_tape_segments(3)
+    s.set_state(U0)
+    s.forward_checkpointed(dts, 3)
+    s.rk3_step(dts[0])
+    a = s.get_state()
+    s.set_state(out[0][0])
+    s.rk3_step(dts[0])
+    assert np.array_equal(a, s.get_state())

@@ -7,10 +7,11 @@ The objective is the kinetic energy of the final state,
 with the reference's quadrature weights (FROperators quad_weights,
 basis.cpp:484) and Jacobian determinants (FRSolver::initGeometry,
 fr_solver.cpp:85-129).  The forward trajectory keeps one state every
-`every` steps (zj_forward_checkpointed); the reverse sweep
-(zj_adjoint_sweep) recomputes each segment from its checkpoint, recording the
-stage states and traces of its steps on a device tape, and runs the stage
-VJPs backwards from the tape.  Both passes run in the library, on the GPU:
+`every` steps (zj_forward_checkpointed) and records its last segments on a
+device tape as it steps; the reverse sweep (zj_adjoint_sweep) recomputes
+every other segment from its checkpoint, recording the stage states and
+traces of its steps on the tape, and runs the stage VJPs backwards from the
+tape.  Both passes run in the library, on the GPU:
 the host sees the final state (for J and its gradient) and the result.
 """
 from __future__ import annotations
@@ -42,12 +43,16 @@ def kinetic_energy_grad(U: np.ndarray, t: Tables) -> np.ndarray:
 
 
 def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every: int = 16, want_dt: bool = True,
-             stream=None, want_re: bool = False):
+             stream=None, want_re: bool = False, tape_segments: int | None = None):
     """J(U^N) after len(dts) steps from U0, dJ/dU0 and dJ/d(dt_n) (None
     each with want_dt=False, which keeps the reverse sweep free of host
     synchronisation); with want_re, also dJ/dRe (Navier-Stokes terms) as a
-    fourth value."""
+    fourth value.  tape_segments: trailing segments the forward pass records
+    on the device tape (None: the library's setting, 1 by default); with all
+    of them recorded, no step is computed twice."""
     steps = len(dts)
+    if tape_segments is not None:
+        sol.adjoint_tape_segments(tape_segments)
     sol.set_state(np.array(U0, np.float64).reshape(-1))
     sol.forward_checkpointed(dts, every, stream)
     UN = sol.get_state()

@@ -98,6 +98,13 @@ struct zj_solver {
     // and their traces at the adjoint tile (tape_steps entries)
     char* tape = nullptr;
     int tape_steps = 0;
+    // trailing segments that zj_forward_checkpointed records on the tape as it
+    // steps, and what the tape holds from it: segments rec_first ..
+    // rec_first + rec_n - 1 of a trajectory of rec_steps steps cut every
+    // rec_every, step sizes rec_dts (rec_n = 0: nothing usable)
+    int tape_segs = 1;
+    int rec_first = 0, rec_n = 0, rec_every = 0, rec_steps = 0;
+    std::vector<double> rec_dts;
     double* d_dtb_steps = nullptr;  // dJ/d(dt_n) of every step of a sweep
     int n_dtb_steps = 0;
     int A_cur = 0;
@@ -577,6 +584,7 @@ extern "C" int zj_set_state_device(zj_handle h, const double* d_aos, void* strea
 // ---- device state slots: checkpoints that never leave the GPU
 extern "C" int zj_state_slots(zj_handle h, int n) {
     if (!h || n < 0) return ZJ_ERR_ARG;
+    h->rec_n = 0;
     ZJ_HIP(h, hipSetDevice(h->device));
     ZJ_HIP(h, hipStreamSynchronize(h->stream));
     if (h->slots) {
@@ -604,6 +612,7 @@ extern "C" int zj_state_save(zj_handle h, int slot, void* stream) {
     const size_t n = (size_t)h->E * h->S * 4;
     ZJ_HIP(h, hipMemcpyAsync(h->slots + (size_t)slot * n, h->U, n * sizeof(double), hipMemcpyDeviceToDevice,
                              pick(h, stream)));
+    h->rec_n = 0;  // a recorded segment may start from this slot
     return ZJ_OK;
 }
 
@@ -1192,6 +1201,7 @@ static StepBufs tape_step(zj_solver* h, int k) {
 
 static int ensure_tape(zj_solver* h, int steps) {
     if (h->tape && h->tape_steps >= steps) return ZJ_OK;
+    h->rec_n = 0;
     if (h->tape) {
         ZJ_HIP(h, hipStreamSynchronize(h->stream));
         ZJ_HIP(h, hipDeviceSynchronize());
@@ -1214,29 +1224,41 @@ static hipError_t trace_into(zj_solver* h, const double* U, void* T, const OpTab
     return launch_trace<R>(h->P, TraceArgs<R>{h->E, U, (R*)T, h->xcd_groups}, ops, 0, s);
 }
 
-// One segment backwards: its checkpoint, the forward steps recorded on the
-// tape (stage states and traces written where the VJPs read them: no step is
-// computed twice), then the stage VJPs from the last step to the first.
+// m forward steps from `src` recorded on the tape from entry `base` on: the
+// stage states and traces written where the VJPs read them.  With U_end,
+// the last step's stage 3 too (U^{n+m} into U_end; its traces into T_end).
 template <typename R>
-static hipError_t sweep_segment(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const double* dts, int start,
-                                int m, int slot, bool want_dt) {
+static hipError_t record_steps(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const double* src,
+                               const double* dts, int m, int base, double* U_end = nullptr, void* T_end = nullptr) {
     const size_t n = (size_t)h->E * h->S * 4;
-    StepBufs b0 = tape_step(h, 0);
-    hipError_t e = hipMemcpyAsync(b0.U, h->slots + (size_t)slot * n, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    StepBufs b0 = tape_step(h, base);
+    hipError_t e = hipMemcpyAsync(b0.U, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = trace_into<R>(h, b0.U, b0.T[0], ops, s);
     for (int k = 0; k < m && e == hipSuccess; ++k) {
-        if ((e = set_dt(h, dts[start + k], s)) != hipSuccess) break;
-        const StepBufs b = tape_step(h, k);
+        if ((e = set_dt(h, dts[k], s)) != hipSuccess) break;
+        const StepBufs b = tape_step(h, base + k);
         if (k + 1 < m) {
-            const StepBufs nb = tape_step(h, k + 1);
+            const StepBufs nb = tape_step(h, base + k + 1);
             e = recompute_stages<R>(h, ops, s, b, nb.U, nb.T[0]);
         } else {
-            e = recompute_stages<R>(h, ops, s, b);
+            e = recompute_stages<R>(h, ops, s, b, U_end, T_end);
         }
     }
+    return e;
+}
+
+// One segment backwards: unless the forward pass recorded it, its checkpoint
+// and its forward steps recorded on the tape (no step is computed twice);
+// then the stage VJPs from the last step to the first.
+template <typename R>
+static hipError_t sweep_segment(zj_solver* h, const OpTables<R>& ops, hipStream_t s, const double* dts, int start,
+                                int m, int slot, bool want_dt, bool recorded, int base) {
+    const size_t n = (size_t)h->E * h->S * 4;
+    hipError_t e = hipSuccess;
+    if (!recorded) e = record_steps<R>(h, ops, s, h->slots + (size_t)slot * n, dts + start, m, base);
     for (int k = m - 1; k >= 0 && e == hipSuccess; --k) {
         if ((e = set_dt(h, dts[start + k], s)) != hipSuccess) break;
-        if ((e = adjoint_stages<R>(h, ops, s, tape_step(h, k), want_dt)) != hipSuccess) break;
+        if ((e = adjoint_stages<R>(h, ops, s, tape_step(h, base + k), want_dt)) != hipSuccess) break;
         if (h->viscous && (e = launch_finalize(h->A_mub, 3 * h->adj_nt, FIN_ACC, h->d_scalars + 4, s)) != hipSuccess)
             break;
         if (want_dt) e = launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_dtb_steps + start + k, s);
@@ -1253,11 +1275,49 @@ extern "C" int zj_forward_checkpointed(zj_handle h, const double* dts, int n_ste
         const int rc = zj_state_slots(h, n_ckpt);
         if (rc) return rc;
     }
-    for (int n = 0; n < n_steps; ++n) {
-        int rc;
-        if (n % every == 0 && (rc = zj_state_save(h, n / every, stream))) return rc;
-        if ((rc = zj_rk3_step(h, dts[n], stream))) return rc;
+    h->rec_n = 0;
+    // the trailing segments run at the adjoint tile, recorded on the tape
+    const int n_rec = std::min(h->tape_segs, n_ckpt), c0 = n_ckpt - n_rec;
+    if (n_rec > 0) {
+        int rc = ensure_adjoint(h);
+        if (!rc) rc = ensure_tape(h, n_rec * every);
+        if (rc) return rc;
     }
+    hipStream_t s = pick(h, stream);
+    for (int c = 0; c < n_ckpt; ++c) {
+        const int start = c * every, m = std::min(start + every, n_steps) - start;
+        int rc;
+        if ((rc = zj_state_save(h, c, stream))) return rc;
+        if (c < c0) {
+            for (int k = 0; k < m; ++k)
+                if ((rc = zj_rk3_step(h, dts[start + k], stream))) return rc;
+            continue;
+        }
+        ZJ_HIP(h, hipSetDevice(h->device));
+        const int base = (c - c0) * every;
+        // U^{start+m} into the handle's state; the stage-3 traces at the
+        // adjoint tile land in a stage buffer, the step traces are rebuilt
+        const hipError_t e =
+            h->prec == ZJ_PREC_FP64
+                ? record_steps<double>(h, h->ops64, s, h->U, dts + start, m, base, h->U, h->T[1])
+                : record_steps<float>(h, h->ops32, s, h->U, dts + start, m, base, h->U, h->T[1]);
+        ZJ_HIP(h, e);
+        if ((rc = refresh_trace(h, s))) return rc;
+        h->l2_blocks = 0;  // these steps leave no residual norm
+    }
+    if (n_rec > 0) {
+        h->rec_first = c0;
+        h->rec_n = n_rec;
+        h->rec_every = every;
+        h->rec_steps = n_steps;
+        h->rec_dts.assign(dts, dts + n_steps);
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_tape_segments(zj_handle h, int n) {
+    if (!h || n < 0) return ZJ_ERR_ARG;
+    h->tape_segs = n;
     return ZJ_OK;
 }
 
@@ -1271,7 +1331,10 @@ extern "C" int zj_adjoint_sweep(zj_handle h, const double* dts, int n_steps, int
     int rc = ensure_adjoint(h);
     if (rc) return rc;
     ZJ_HIP(h, hipSetDevice(h->device));
-    if ((rc = ensure_tape(h, std::min(every, n_steps)))) return rc;
+    // segments the forward pass recorded for this very trajectory
+    const bool have_rec = h->rec_n > 0 && h->rec_every == every && h->rec_steps == n_steps &&
+                          std::equal(dts, dts + n_steps, h->rec_dts.begin());
+    if (!have_rec && (rc = ensure_tape(h, std::min(every, n_steps)))) return rc;
     if (dt_bar && h->n_dtb_steps < n_steps) {
         if (h->d_dtb_steps) {
             ZJ_HIP(h, hipDeviceSynchronize());
@@ -1283,13 +1346,22 @@ extern "C" int zj_adjoint_sweep(zj_handle h, const double* dts, int n_steps, int
         h->n_dtb_steps = n_steps;
     }
     hipStream_t s = pick(h, stream);
+    bool overwrote = false;
     for (int c = n_ckpt - 1; c >= 0; --c) {
         const int start = c * every, m = std::min(start + every, n_steps) - start;
-        const hipError_t e = h->prec == ZJ_PREC_FP64
-                                 ? sweep_segment<double>(h, h->ops64, s, dts, start, m, c, dt_bar != nullptr)
-                                 : sweep_segment<float>(h, h->ops32, s, dts, start, m, c, dt_bar != nullptr);
+        // a recorded segment sits at its own place on the tape; the others
+        // are recomputed into the first place, free once the recorded
+        // segments (all later than them) are done
+        const bool rec = have_rec && c >= h->rec_first;
+        const int base = rec ? (c - h->rec_first) * every : 0;
+        overwrote |= !rec;
+        const hipError_t e =
+            h->prec == ZJ_PREC_FP64
+                ? sweep_segment<double>(h, h->ops64, s, dts, start, m, c, dt_bar != nullptr, rec, base)
+                : sweep_segment<float>(h, h->ops32, s, dts, start, m, c, dt_bar != nullptr, rec, base);
         ZJ_HIP(h, e);
     }
+    if (overwrote) h->rec_n = 0;
     // the handle's state is U^0 again
     const size_t n = (size_t)h->E * h->S * 4;
     ZJ_HIP(h, hipMemcpyAsync(h->U, h->slots, n * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -290,6 +290,12 @@ class Solver:
         d = np.ascontiguousarray(dts, np.float64)
         self._chk(lib().zj_forward_checkpointed(self._h, d, d.size, int(every), stream))
 
+    def adjoint_tape_segments(self, n):
+        """Trailing segments that forward_checkpointed records on the device
+        tape as it steps, so that adjoint_sweep need not recompute them
+        (zj_adjoint_tape_segments; default 1, 0 = none)."""
+        self._chk(lib().zj_adjoint_tape_segments(self._h, int(n)))
+
     def adjoint_sweep(self, dts, every, want_dt=True, stream=None):
         """Every step backwards from the seeded cotangent (zj_adjoint_sweep);
         returns dJ/d(dt_n) per step, or None with want_dt=False (asynchronous)."""
