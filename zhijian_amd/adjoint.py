@@ -62,7 +62,8 @@ def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every
     if dtb is None:
         dtb = [None] * steps
     g = sol.adjoint_get()
-    sol.state_slots(0)
+    # the checkpoint slots stay allocated for the next gradient, as the tape does
+    # (sol.state_slots(0) releases them)
     if want_re:
         return J, g, dtb, sol.adjoint_get_re()
     return J, g, dtb

@@ -28,7 +28,7 @@ struct VjpArgs {
     R* Tb_out;               // cross-tile edge cotangents of q, written at the neighbour's slots
     double* U0b;             // cotangent of U^n: stage 3 writes, stage 2 adds, stage 1 reads
     double* dtb_partial;     // per-tile sum of c * R . du_n (nullptr: skip)
-    double* mub_partial;     // viscous terms: per-tile sum of <cotangent of F_v, F_v> (nullptr: skip)
+    double* mub_partial;     // viscous terms: per-tile running sum of <cotangent of F_v, F_v> (nullptr: skip)
     double gamma;
     int positivity;
     int xcd_groups;

@@ -507,7 +507,7 @@ __global__ __launch_bounds__(BLOCK) void finalize_kernel(const double* __restric
     for (int i = threadIdx.x; i < n; i += BLOCK)
         acc = is_min ? fmin(acc, partial[i]) : acc + partial[i];
     const double r = is_min ? block_reduce<OpMin>(acc, sRed) : block_reduce<OpSum>(acc, sRed);
-    if (threadIdx.x == 0) *out = op == FIN_SQRT_SUM ? sqrt(r) : (op == FIN_ACC ? *out + r : r);
+    if (threadIdx.x == 0) *out = op == FIN_SQRT_SUM ? sqrt(r) : r;
 }
 
 }  // namespace

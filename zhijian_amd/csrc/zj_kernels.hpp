@@ -212,7 +212,7 @@ hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long
                                   int nblk, hipStream_t s);
 // Deterministic reduction of block partials (fixed order): sqrt of the sum
 // (an L2 norm), the minimum, or the plain sum.
-enum FinalizeOp : int { FIN_SQRT_SUM = 0, FIN_MIN = 1, FIN_SUM = 2, FIN_ACC = 3 };  // FIN_ACC: *out += sum
+enum FinalizeOp : int { FIN_SQRT_SUM = 0, FIN_MIN = 1, FIN_SUM = 2 };
 hipError_t launch_finalize(const double* partial, int n, int op, double* out, hipStream_t s);
 
 }  // namespace zj

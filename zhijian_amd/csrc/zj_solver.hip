@@ -91,7 +91,7 @@ struct zj_solver {
     double* A_U0b = nullptr;       // cotangent of U^n within a step
     void* A_Tb[2] = {nullptr, nullptr};  // cross-tile edge cotangents, ping-pong
     double* A_dtb = nullptr;       // per-tile dt cotangents of the three stages
-    double* A_mub = nullptr;       // viscous terms: per-tile <F_v cotangent, F_v> of the three stages
+    double* A_mub = nullptr;       // viscous terms: per-tile <F_v cotangent, F_v>, summed since the seed
     double* slots = nullptr;       // device copies of states (SoA planes), zj_state_slots
     int n_slots = 0;
     // tape of the native reverse sweep: per step of a segment, U^n, U1, U2
@@ -1005,7 +1005,7 @@ static int ensure_adjoint(zj_solver* h) {
     ZJ_HIP(h, hipMalloc((void**)&h->A_U0b, std::max<size_t>(npts * 4, 1) * sizeof(double)));
     for (int k = 0; k < 2; ++k) ZJ_HIP(h, hipMalloc(&h->A_Tb[k], ntr * rsz));
     ZJ_HIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * h->adj_nt * sizeof(double)));
-    if (h->viscous) ZJ_HIP(h, hipMalloc((void**)&h->A_mub, (size_t)3 * h->adj_nt * sizeof(double)));
+    if (h->viscous) ZJ_HIP(h, hipMalloc((void**)&h->A_mub, (size_t)h->adj_nt * sizeof(double)));
     h->adj_ready = true;
     return ZJ_OK;
 }
@@ -1090,7 +1090,7 @@ static hipError_t adjoint_stages(zj_solver* h, const OpTables<R>& ops, hipStream
         a.Tb_out = Tb[1 - h->A_cur];
         a.U0b = h->A_U0b;
         a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->adj_nt : nullptr;
-        if (h->viscous) a.mub_partial = h->A_mub + (size_t)(3 - st) * h->adj_nt;
+        if (h->viscous) a.mub_partial = h->A_mub;
         hipError_t e = h->viscous ? launch_visc_vjp<R>(h->P, st, a, ops, h->vp, s)
                                   : launch_vjp<R>(h->P, st, a, ops, h->affine, s, h->textbook);
         if (e != hipSuccess) return e;
@@ -1108,7 +1108,8 @@ extern "C" int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void
     ZJ_HIP(h, launch_aos_to_soa(d_ubar_aos, h->A_Ub, (int64_t)h->E * h->S, pick(h, stream), h->perm, h->S));
     h->A_has_tb = false;
     h->A_seeded = true;
-    ZJ_HIP(h, launch_set_scalar(h->d_scalars + 4, 0.0, pick(h, stream)));
+    if (h->viscous)
+        ZJ_HIP(h, hipMemsetAsync(h->A_mub, 0, (size_t)h->adj_nt * sizeof(double), pick(h, stream)));
     return ZJ_OK;
 }
 
@@ -1174,7 +1175,6 @@ extern "C" int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* str
         if (e == hipSuccess) e = adjoint_stages<float>(h, h->ops32, s, b, dt_bar != nullptr);
     }
     ZJ_HIP(h, e);
-    if (h->viscous) ZJ_HIP(h, launch_finalize(h->A_mub, 3 * h->adj_nt, FIN_ACC, h->d_scalars + 4, s));
     if (dt_bar) {
         ZJ_HIP(h, launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_scalars + 3, s));
         ZJ_HIP(h, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1259,8 +1259,6 @@ static hipError_t sweep_segment(zj_solver* h, const OpTables<R>& ops, hipStream_
     for (int k = m - 1; k >= 0 && e == hipSuccess; --k) {
         if ((e = set_dt(h, dts[start + k], s)) != hipSuccess) break;
         if ((e = adjoint_stages<R>(h, ops, s, tape_step(h, base + k), want_dt)) != hipSuccess) break;
-        if (h->viscous && (e = launch_finalize(h->A_mub, 3 * h->adj_nt, FIN_ACC, h->d_scalars + 4, s)) != hipSuccess)
-            break;
         if (want_dt) e = launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_dtb_steps + start + k, s);
     }
     return e;
@@ -1382,8 +1380,10 @@ extern "C" int zj_adjoint_get_re(zj_handle h, double* dJ_dRe) {
         return ZJ_OK;
     }
     ZJ_HIP(h, hipSetDevice(h->device));
-    ZJ_HIP(h, hipDeviceSynchronize());
-    ZJ_HIP(h, hipMemcpy(h->h_pinned + 4, h->d_scalars + 4, sizeof(double), hipMemcpyDeviceToHost));
+    ZJ_HIP(h, hipDeviceSynchronize());  // the sweep may have run on a caller's stream
+    ZJ_HIP(h, launch_finalize(h->A_mub, h->adj_nt, FIN_SUM, h->d_scalars + 4, h->stream));
+    ZJ_HIP(h, hipMemcpyAsync(h->h_pinned + 4, h->d_scalars + 4, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ZJ_HIP(h, hipStreamSynchronize(h->stream));
     // F_v scales with mu = 1/Re: dJ/dmu = S / mu, dJ/dRe = -dJ/dmu / Re^2 = -S / Re
     *dJ_dRe = -h->h_pinned[4] * h->vp.mu;
     return ZJ_OK;

@@ -902,7 +902,7 @@ __global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, cons
 
     if (a.mub_partial) {
         const double sm = block_reduce_n<OpSum, NTHR / 64>(musum, sRed);
-        if (tid == 0) a.mub_partial[tile] = sm;
+        if (tid == 0) a.mub_partial[tile] += sm;  // summed over the launches since the seed
     }
 
     // ---- R6: transposed gradient operator; cotangent of q
