@@ -160,12 +160,13 @@ def time_gradient(n, p, precision, steps, every, stream, hiprt, Re=None, tape_se
 BATCH_SCHEME = dict(compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Pr=0.72)
 
 
-def batch_members(n_inst, seed, precision):
+def batch_members(n_inst, seed, precision, scheme="navier_stokes"):
     """Seeded batch: N ~ U{16..128} elements per side, P ~ U{1..5},
     Mach ~ U[0.1, 0.8] (PCG64 seed), Re ~ U[100, 5000] (PCG64 [seed, 1]),
     alternating Taylor-Green /
     isentropic vortex; textbook FR (VCJH HU, Roe) with the Navier-Stokes terms
-    (BR2, Pr 0.72), no limiter."""
+    (BR2, Pr 0.72), no limiter; scheme="reference": the reference arithmetic
+    (DG correction, inline Rusanov, positivity limiter) on the same meshes."""
     from zhijian_amd import cases, solver
     rng = np.random.default_rng(seed)
     re_rng = np.random.default_rng([seed, 1])  # its own stream: the meshes do not depend on it
@@ -181,18 +182,20 @@ def batch_members(n_inst, seed, precision):
             mesh = cases.vortex_mesh(n)
             t = solver.build_tables(mesh.xy, mesh.elem_nodes, p, periodic=(mesh.Lx, mesh.Ly))
             U = cases.vortex_ic(t.pos, u_inf=mach * math.sqrt(1.4)).reshape(-1)
-        s = solver.Solver(t, precision=precision, cfl=0.5, Re=Re, **BATCH_SCHEME)
+        kw = dict(Re=Re, **BATCH_SCHEME) if scheme == "navier_stokes" else {}
+        s = solver.Solver(t, precision=precision, cfl=0.5, **kw)
         s.set_state(U)
         out.append((s, U, dict(n=n, p=p, mach=mach, Re=Re, kind="tgv" if k % 2 == 0 else "vortex")))
     return out
 
 
-def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=20):
+def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=20, scheme="navier_stokes"):
     """One batched SSP-RK3 step of all members vs each member's own graph step;
     the FP32-bulk batch against an FP64 batch of the same members after
     err_steps steps."""
     from zhijian_amd import solver
-    mem = batch_members(n_inst, seed, precision)
+    ns = scheme == "navier_stokes"
+    mem = batch_members(n_inst, seed, precision, scheme)
     sols = [m[0] for m in mem]
     dts = np.array([s.dt_cfl(0.5) for s in sols])
     dof = sum(s.n for s in sols)
@@ -200,11 +203,15 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
     for s, (_, _, meta) in zip(sols, mem):
         info = s.info()
         n_el = info["n_dof"] // (4 * info["n_sp"])
-        by += sum(textbook_alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"],
-                                               info["cross_tile_edges"], viscous=True) for k in (1, 2, 3))
+        if ns:
+            by += sum(textbook_alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"],
+                                                   info["cross_tile_edges"], viscous=True) for k in (1, 2, 3))
+        else:
+            by += sum(alg_bytes_per_stage(n_el, meta["p"], k, precision, info["affine"], info["cross_tile_edges"])
+                      for k in (1, 2, 3))
     batch = solver.Batch(sols)
     res = {}
-    for single in (False,):  # the single launch covers reference-arithmetic members only
+    for single in ((False,) if ns else (False, True)):  # the single launch covers reference-arithmetic members
         batch.set_launch(single)
         for policy in (0, 2, 4):
             batch.set_grid(policy)
@@ -235,7 +242,7 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
     batch.close()
     err = None
     if err_steps > 0 and precision != "fp64":
-        ref = batch_members(n_inst, seed, "fp64")
+        ref = batch_members(n_inst, seed, "fp64", scheme)
         for (s, U, _), (r, _, _) in zip(mem, ref):
             s.set_state(U)
         b32 = solver.Batch(sols)
@@ -254,9 +261,11 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
         err = dict(steps=err_steps, max_rel_l2=max(x[0] for x in rel), max_rel_linf=max(x[1] for x in rel),
                    median_rel_l2=float(np.median([x[0] for x in rel])), over="members (FP64 batch of the same meshes)")
     achieved = by / (res[best] * 1e-3) / 1e9
-    return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], "
-                         f"Re~U[100,5000], alternating TGV/vortex; textbook FR (VCJH HU, Roe) + Navier-Stokes "
-                         f"BR2 (Pr 0.72), no limiter, dt = each member's CFL 0.5 step", precision=precision, dof=dof,
+    physics = ("Re~U[100,5000], alternating TGV/vortex; textbook FR (VCJH HU, Roe) + Navier-Stokes BR2 (Pr 0.72), "
+               "no limiter" if ns else "alternating TGV/vortex; reference arithmetic (DG, inline Rusanov, "
+               "positivity limiter)")
+    return dict(workload=f"{n_inst} meshes, seed {seed}: N~U{{16..128}}, P~U{{1..5}}, Mach~U[0.1,0.8], {physics}, "
+                         f"dt = each member's CFL 0.5 step", precision=precision, dof=dof,
                 ms_per_step_by_launch={f"{k[0]}/grid{k[1]}": v for k, v in res.items()},
                 launch=best[0], grid_policy=best[1],
                 autotuner=dict(launch="single" if tuned[0] else "grouped", grid_policy=tuned[1], ms_per_step=tuned[2]),
@@ -812,6 +821,9 @@ def main():
         batched = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt,
                              err_steps=args.err_steps)
         log(f"batched: {batched}")
+        batched["reference_arithmetic"] = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream,
+                                                     hiprt, err_steps=args.err_steps, scheme="reference")
+        log(f"batched, reference arithmetic: {batched['reference_arithmetic']}")
 
     adj = {}
     if args.adjoint:
