@@ -701,6 +701,8 @@ def main():
                     help="members of the batched-step measurement (0: skip)")
     ap.add_argument("--tri-n", type=int, default=500,
                     help="cells per side of the triangle leg (each split in two; 0: skip)")
+    ap.add_argument("--tri-re", type=float, default=0.0,
+                    help="with --tri-only: the Navier-Stokes terms at this Reynolds number (0: Euler)")
     ap.add_argument("--tri-only", action="store_true",
                     help="run only the triangle leg and print its JSON (profiling runs)")
     ap.add_argument("--io-only", action="store_true",
@@ -728,7 +730,7 @@ def main():
         if args.probe_bytes > 0:
             log(f"copy kernel: {_lib.copy_probe(int(args.probe_bytes), 10):.0f} GB/s")
         tri = time_tri(args.tri_n, 3, args.steps, args.warmup, args.err_steps, hiprt.Stream(), hiprt,
-                       0.0 if args.no_cpu else 5.0)
+                       0.0 if args.no_cpu else 5.0, Re=args.tri_re if args.tri_re > 0 else None)
         print(json.dumps({"triangles": tri}), flush=True)
         return
 

@@ -456,20 +456,47 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     auto sl = [](int elx, int j, int w) { return (elx * NFP + j) * NV + w; };   // trace slots
     auto sp = [](int elx, int m, int w) { return (elx * NP + m) * NV + w; };    // point slots
 
-    // physical gradient of this lane's variable at the points, from sU into gx, gy
-    auto gradient_of = [&](R rx_, R ry_, R sx_, R sy_, R* gx, R* gy) {
-#pragma unroll 1
-        for (int i = 0; i < NP; ++i) {
-            R gr = R(0), gs = R(0);
+    // physical gradient of this lane's variable at the points, from sU into gx,
+    // gy (every point's sums advance together: 2 NP independent FMA chains)
+    auto gradient_of = [&](R rx_, R ry_, R sx_, R sy_, R* gx, R* gy) __attribute__((always_inline)) {
+        R gr[NP], gs[NP];
 #pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const R um = sU[sp(el, m, v)];
-                gr = fmaR(cop<R>(O0 + m * CA + i), um, gr);
-                gs = fmaR(cop<R>(O0 + m * CA + NP + i), um, gs);
+        for (int i = 0; i < NP; ++i) { gr[i] = R(0); gs[i] = R(0); }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const R um = sU[sp(el, m, v)];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                gr[i] = fmaR(cop<R>(O0 + m * CA + i), um, gr[i]);
+                gs[i] = fmaR(cop<R>(O0 + m * CA + NP + i), um, gs[i]);
             }
-            gx[sp(el, i, v)] = rx_ * gr + sx_ * gs;
-            gy[sp(el, i, v)] = ry_ * gr + sy_ * gs;
         }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            gx[sp(el, i, v)] = rx_ * gr[i] + sx_ * gs[i];
+            gy[sp(el, i, v)] = ry_ * gr[i] + sy_ * gs[i];
+        }
+    };
+    // traces of U, gx, gy (this lane's variable) at the FP points of edge kk
+    // (3 FP independent FMA chains), for each through out(j, tu, tx, ty)
+    auto edge_traces = [&](int kk, int op_row, const R* gx, const R* gy, auto&& out) __attribute__((always_inline)) {
+        R tu[FP], tx[FP], ty[FP];
+#pragma unroll
+        for (int f = 0; f < FP; ++f) { tu[f] = R(0); tx[f] = R(0); ty[f] = R(0); }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const R um = sU[sp(el, m, v)], xm = gx[sp(el, m, v)], ym = gy[sp(el, m, v)];
+#pragma unroll
+            for (int f = 0; f < FP; ++f) {
+                const int j = kk * FP + f;
+                const R w = op_row ? cop<R>(OI + j * NP + m) : cop<R>(O0 + m * CA + 2 * NP + j);
+                tu[f] = fmaR(w, um, tu[f]);
+                tx[f] = fmaR(w, xm, tx[f]);
+                ty[f] = fmaR(w, ym, ty[f]);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < FP; ++f) out(kk * FP + f, tu[f], tx[f], ty[f]);
     };
 
     // ---- 0: state, gradient, traces of U and grad U
@@ -488,19 +515,12 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
         }
         gradient_of(rx, ry, sx, sy, sQx, sQy);
 #pragma unroll 1
-        for (int j = 0; j < NFP; ++j) {
-            R tu = R(0), tx = R(0), ty = R(0);
-#pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const R w = cop<R>(O0 + m * CA + 2 * NP + j);
-                tu = fmaR(w, sU[sp(el, m, v)], tu);
-                tx = fmaR(w, sQx[sp(el, m, v)], tx);
-                ty = fmaR(w, sQy[sp(el, m, v)], ty);
-            }
-            sTu[sl(el, j, v)] = tu;
-            sTx[sl(el, j, v)] = tx;
-            sTy[sl(el, j, v)] = ty;
-        }
+        for (int kk = 0; kk < 3; ++kk)
+            edge_traces(kk, 0, sQx, sQy, [&](int j, R tu, R tx, R ty) __attribute__((always_inline)) {
+                sTu[sl(el, j, v)] = tu;
+                sTx[sl(el, j, v)] = tx;
+                sTy[sl(el, j, v)] = ty;
+            });
     }
     bool owned = false;
     if (live && v < 3) {
@@ -761,21 +781,12 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     for (int kk = 0; kk < 3; ++kk) {
         const int nl = L[kk].x - e0;
         if (nl >= 0 && nl < TT) continue;
-#pragma unroll 1
-        for (int j = kk * FP; j < kk * FP + FP; ++j) {
-            R tu = R(0), tx = R(0), ty = R(0);
-#pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const R w = cop<R>(OI + j * NP + m);
-                tu = fmaR(w, sU[sp(el, m, v)], tu);
-                tx = fmaR(w, gx[sp(el, m, v)], tx);
-                ty = fmaR(w, gy[sp(el, m, v)], ty);
-            }
+        edge_traces(kk, 1, gx, gy, [&](int j, R tu, R tx, R ty) __attribute__((always_inline)) {
             R* T = a.T_out + ((int64_t)e * NFP + j) * TWV;
             T[v] = tu;
             T[NV + v] = tx;
             T[2 * NV + v] = ty;
-        }
+        });
     }
 }
 
