@@ -578,16 +578,17 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     }
     __syncthreads();  // Q of every variable in place
 
-    // ---- 2: total fluxes at the points (this lane's component, from the
-    // point's four variables in LDS); divergence and interpolated normal fluxes
-    R div[NP], fn[NFP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) div[i] = R(0);
-#pragma unroll
-    for (int j = 0; j < NFP; ++j) fn[j] = R(0);
+    // ---- 2: total fluxes at the points, each point's four components formed
+    // once: this lane takes points v, v + 4, ... of its element and leaves
+    // their contravariant fluxes where Q was; then every lane accumulates its
+    // own component's divergence and interpolated normal fluxes
+    constexpr int MPL = (NP + NV - 1) / NV;  // points per lane
+    R FR[MPL][NV], FS[MPL][NV];
     if (live) {
-#pragma unroll 1
-        for (int m = 0; m < NP; ++m) {
+#pragma unroll
+        for (int q = 0; q < MPL; ++q) {
+            const int m = v + NV * q;
+            if (m >= NP) break;
             R U4[NV], X4[NV], Y4[NV];
 #pragma unroll
             for (int w = 0; w < NV; ++w) {
@@ -600,11 +601,39 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
             const R p = (g - R(1)) * (U4[3] - R(0.5) * (U4[1] * ux + U4[2] * uy));
             R Fv[NV], Gv[NV];
             visc_flux(U4, X4, Y4, g, mu, kappa, Fv, Gv);
-            const R fx = v == 0 ? U4[1] : (v == 1 ? U4[1] * ux + p : (v == 2 ? U4[1] * uy : (U4[3] + p) * ux));
-            const R fy = v == 0 ? U4[2] : (v == 1 ? U4[2] * ux : (v == 2 ? U4[2] * uy + p : (U4[3] + p) * uy));
-            const R Fx = fx - (v == 0 ? Fv[0] : (v == 1 ? Fv[1] : (v == 2 ? Fv[2] : Fv[3])));
-            const R Fy = fy - (v == 0 ? Gv[0] : (v == 1 ? Gv[1] : (v == 2 ? Gv[2] : Gv[3])));
-            const R fr = rx * Fx + ry * Fy, fs = sx * Fx + sy * Fy, frs = fr + fs;
+            const R fx[NV] = {U4[1], U4[1] * ux + p, U4[1] * uy, (U4[3] + p) * ux};
+            const R fy[NV] = {U4[2], U4[2] * ux, U4[2] * uy + p, (U4[3] + p) * uy};
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R Fx = fx[w] - Fv[w], Fy = fy[w] - Gv[w];
+                FR[q][w] = rx * Fx + ry * Fy;
+                FS[q][w] = sx * Fx + sy * Fy;
+            }
+        }
+    }
+    __syncthreads();  // every read of Q done: the contravariant fluxes take its place
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < MPL; ++q) {
+            const int m = v + NV * q;
+            if (m >= NP) break;
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                sQx[sp(el, m, w)] = FR[q][w];
+                sQy[sp(el, m, w)] = FS[q][w];
+            }
+        }
+    }
+    __syncthreads();
+    R div[NP], fn[NFP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) div[i] = R(0);
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) fn[j] = R(0);
+    if (live) {
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R fr = sQx[sp(el, m, v)], fs = sQy[sp(el, m, v)], frs = fr + fs;
 #pragma unroll
             for (int i = 0; i < NP; ++i)
                 div[i] = fmaR(cop<R>(O0 + m * CA + NP + i), fs, fmaR(cop<R>(O0 + m * CA + i), fr, div[i]));
@@ -613,7 +642,7 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
                 fn[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
         }
     }
-    __syncthreads();  // every read of Q done: the flux differences take its place
+    __syncthreads();  // every read of the point fluxes done: the flux differences take their place
     if (live) {
 #pragma unroll
         for (int j = 0; j < NFP; ++j) sD[sl(el, j, v)] = (j >= FP && j < 2 * FP) ? fn[j] : -fn[j];
