@@ -701,6 +701,9 @@ def main():
                     help="members of the batched-step measurement (0: skip)")
     ap.add_argument("--tri-n", type=int, default=500,
                     help="cells per side of the triangle leg (each split in two; 0: skip)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the whole C5 gradient (Navier-Stokes, checkpointed and whole-tape), FP64 and FP32-bulk")
+    ap.add_argument("--batch-only", action="store_true", help="only the Navier-Stokes batch (--batch members)")
     ap.add_argument("--tri-re", type=float, default=0.0,
                     help="with --tri-only: the Navier-Stokes terms at this Reynolds number (0: Euler)")
     ap.add_argument("--tri-only", action="store_true",
@@ -742,6 +745,22 @@ def main():
         ns = time_viscous(tables, cases.tgv_ic(tables.pos, mach=0.1).reshape(-1), args.steps, args.warmup,
                           args.err_steps, hiprt.Stream(), hiprt)
         print(json.dumps({"navier_stokes": ns}), flush=True)
+        return
+
+    if args.adjoint_only:
+        stream = hiprt.Stream()
+        adj = {}
+        for prec in ("fp64", "fp32bulk"):
+            adj[prec] = dict(checkpointed_gradient=time_gradient(128, 3, prec, 256, 16, stream, hiprt, Re=1600.0),
+                             taped_gradient=time_gradient(128, 3, prec, 256, 16, stream, hiprt, Re=1600.0,
+                                                          tape_segments=16))
+        print(json.dumps({"adjoint": adj}), flush=True)
+        return
+
+    if args.batch_only:
+        stream = hiprt.Stream()
+        b = time_batch(args.batch, 4, "fp32bulk", max(args.steps // 5, 10), 3, stream, hiprt, err_steps=args.err_steps)
+        print(json.dumps({"batched": b}), flush=True)
         return
 
     if args.io_only:
