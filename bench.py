@@ -920,6 +920,7 @@ def main():
         "adjoint_over_forward_basis": "whole checkpointed gradient (256 steps, checkpoints every 16, TGV 128x128 "
                                       "P3 textbook with the Navier-Stokes terms at Re 1600, FP64) over the same "
                                       "256 forward steps",
+        "adjoint_over_forward_whole_tape": adj.get("fp64", {}).get("taped_gradient", {}).get("gradient_over_forward"),
         "adjoint": adj or None,
         "batched": batched,
         "textbook": textbook,
