@@ -90,7 +90,7 @@ constexpr int visc_threads() {
 // One tile's viscous stage (the workgroup's NTHR threads, the tile's elements).
 template <typename R, int P, int EPB, int MODE, int NTHR>
 __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
-                                          const int tile) {
+                                          const int tile, const int tid) {
     using Y = Lay<R, P>;
     using SM = ViscSmem<R, P, EPB>;
     constexpr bool INC = std::is_same<R, float>::value;
@@ -118,7 +118,6 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     double* const sRed = SM::Red();
     unsigned char* const sOut = SM::Out();
 
-    const int tid = threadIdx.x;
     const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
@@ -386,7 +385,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
 template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
 __global__ __launch_bounds__(NTHR) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
                                                           const ViscParams vp) {
-    visc_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups));
+    visc_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups), threadIdx.x);
 }
 
 // The stage for many meshes of order P: work item w = (instance, tile); a
@@ -395,9 +394,14 @@ template <typename R, int P, int EPB, int MODE>
 __global__ __launch_bounds__(BLOCK) void batched_visc_kernel(const BatchInst<R>* __restrict__ insts,
                                                              const int2* __restrict__ items, int n_items) {
     for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
-        const int2 it = items[w];
-        const BatchInst<R>& in = insts[it.x];
-        visc_body<R, P, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, it.y);
+        // the item is the workgroup's: in scalar registers, so that the
+        // instance's arguments are scalar loads, not per-lane copies
+        const int inst = __builtin_amdgcn_readfirstlane(items[w].x);
+        const int tile = __builtin_amdgcn_readfirstlane(items[w].y);
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // keeps the per-thread index math in the loop body (registers)
+        const BatchInst<R>& in = insts[inst];
+        visc_body<R, P, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, tile, tid);
         __syncthreads();  // the next item reuses the LDS
     }
 }
