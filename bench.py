@@ -258,8 +258,11 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
         b64.close()
         for r, _, _ in ref:
             r.close()
+        worst = int(np.argmax([x[0] for x in rel]))
         err = dict(steps=err_steps, max_rel_l2=max(x[0] for x in rel), max_rel_linf=max(x[1] for x in rel),
-                   median_rel_l2=float(np.median([x[0] for x in rel])), over="members (FP64 batch of the same meshes)")
+                   median_rel_l2=float(np.median([x[0] for x in rel])), over="members (FP64 batch of the same meshes)",
+                   worst_member=dict(index=worst, rel_l2=rel[worst][0], **mem[worst][2]),
+                   members_above_1e_6=int(sum(x[0] > 1e-6 for x in rel)))
     achieved = by / (res[best] * 1e-3) / 1e9
     physics = ("Re~U[100,5000], alternating TGV/vortex; textbook FR (VCJH HU, Roe) + Navier-Stokes BR2 (Pr 0.72), "
                "no limiter" if ns else "alternating TGV/vortex; reference arithmetic (DG, inline Rusanov, "

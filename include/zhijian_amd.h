@@ -269,7 +269,10 @@ int zj_residual_vjp_device(zj_handle h, const double* d_w_aos, double* d_ubar_ao
  * Euler, textbook Navier-Stokes -- the groups running side by side in one
  * captured graph; each member's step equals its own zj_rk3_step bit for
  * bit.  Members keep working on their
- * own; a member must not be stepped on another stream while its batch runs.
+ * own: a member's calls queue their work after the batch's last step (an
+ * event the step records), so a state read right after zj_batch_rk3_step sees
+ * the stepped state; a member must not be stepped on another stream while its
+ * batch runs.
  * Members must use the full tile of their order (the default).  While a
  * batch holds a handle, zj_set_tile, zj_set_geometry_mode (a change),
  * zj_autotune and zj_destroy on it return ZJ_ERR_STATE: the batch's graph

@@ -155,6 +155,35 @@ def test_batch_textbook_and_navier_stokes_members(precision):
     batch.close()
 
 
+def test_member_calls_wait_for_the_batch_step():
+    """A member's own calls right after zj_batch_rk3_step (no
+    zj_batch_synchronize) see the stepped state: its state read, its next
+    own step and its CFL step wait for the event the batch step records."""
+    say("case: member calls after an unsynchronized batch step, 3 x TGV 96x96 P4 Navier-Stokes fp32bulk")
+    built = [_scheme_member(4, 96, "ns", Re, "roe", "hu", 0.0, "fp32bulk", seed=110 + k)
+             for k, Re in enumerate((200.0, 1000.0, 3000.0))]
+    a = [solver.Solver(t, precision="fp32bulk", **kw) for t, _, kw in built]
+    b = [solver.Solver(t, precision="fp32bulk", **kw) for t, _, kw in built]
+    for s1, s2, (t, U, _) in zip(a, b, built):
+        s1.set_state(U)
+        s2.set_state(U)
+    dts = np.array([0.5 * s.dt_cfl(0.5) for s in a])
+    batch = solver.Batch(a)
+    for s2, dt in zip(b, dts):
+        s2.rk3_step(dt)
+    batch.rk3_step(dts)
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), k
+    batch.rk3_step(dts)
+    for k, (s1, s2) in enumerate(zip(a, b)):  # own step right after an unsynchronized batch step
+        s1.rk3_step(dts[k])
+        s2.rk3_step(dts[k])
+        s2.rk3_step(dts[k])
+        assert s1.dt_cfl(0.5) == s2.dt_cfl(0.5), k
+        assert np.array_equal(s1.get_state(), s2.get_state()), k
+    batch.close()
+
+
 def test_batch_member_changes():
     """While a batch holds a handle, changes that would leave the batch's
     launch arguments stale are refused (tile, metric form, autotune, destroy),

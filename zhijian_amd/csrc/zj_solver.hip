@@ -98,6 +98,7 @@ struct zj_solver {
     // and their traces at the adjoint tile (tape_steps entries)
     char* tape = nullptr;
     int tape_steps = 0;
+    hipEvent_t after_batch = nullptr;  // recorded after the last batch step of a batch holding this handle
     // trailing segments that zj_forward_checkpointed records on the tape as it
     // steps, and what the tape holds from it: segments rec_first ..
     // rec_first + rec_n - 1 of a trajectory of rec_steps steps cut every
@@ -155,7 +156,14 @@ static int fail(zj_solver* h, int code, const std::string& msg) {
     return code;
 }
 
-static hipStream_t pick(zj_solver* h, void* s) { return s ? (hipStream_t)s : h->stream; }
+// The stream a call runs on: the caller's, or the handle's own.  Work queued
+// there waits for the last batch step that wrote the handle's buffers (a
+// batch steps on its own stream, asynchronously).
+static hipStream_t pick(zj_solver* h, void* s) {
+    hipStream_t st = s ? (hipStream_t)s : h->stream;
+    if (h->after_batch) (void)hipStreamWaitEvent(st, h->after_batch, 0);
+    return st;
+}
 
 // ---------------------------------------------------------------- tables
 // The reference's dense tables (basis.cpp:508-569) are tensor products of 1D
@@ -752,6 +760,7 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
     const double p1 = h->P + 1.0;
     const double visc =
         h->viscous ? p1 * p1 * p1 * p1 * std::max(h->vp.mu * 4.0 / 3.0, h->vp.kappa * (h->gamma - 1.0)) : 0.0;
+    (void)pick(h, nullptr);
     ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, 2.0 * h->P + 1.0, visc, h->partial_dt,
                                 h->nred, h->stream));
     ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, h->stream));
@@ -777,6 +786,7 @@ extern "C" int zj_check_finite(zj_handle h, int64_t* n_bad) {
     if (!h || !n_bad) return ZJ_ERR_ARG;
     if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
     ZJ_HIP(h, hipSetDevice(h->device));
+    (void)pick(h, nullptr);
     ZJ_HIP(h, hipMemsetAsync(h->d_count, 0, sizeof(unsigned long long), h->stream));
     ZJ_HIP(h, launch_count_nonfinite(h->U, (int64_t)h->E * h->S * 4, h->d_count, h->nred, h->stream));
     unsigned long long c = 0;
@@ -1469,6 +1479,7 @@ struct zj_batch {
     double* d_dt = nullptr;
     std::vector<double> dt_on_device;
     hipEvent_t fork = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each batched step; members' calls wait for it
     int policy = 0;  // 0: one work item per workgroup; k > 0: k workgroups per CU, persistent
     int n_cu = 1;
     hipGraph_t graph = nullptr;
@@ -1493,6 +1504,7 @@ static void batch_release(zj_batch* b) {
     if (b->all_items) hipFree(b->all_items);
     if (b->d_dt) hipFree(b->d_dt);
     if (b->fork) hipEventDestroy(b->fork);
+    if (b->done) hipEventDestroy(b->done);
     if (b->stream) hipStreamDestroy(b->stream);
 }
 
@@ -1558,6 +1570,7 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
     ZJ_BHIP(b, hipSetDevice(b->device));
     ZJ_BHIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     ZJ_BHIP(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    ZJ_BHIP(b, hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     hipDeviceProp_t prop;
     ZJ_BHIP(b, hipGetDeviceProperties(&prop, b->device));
     b->n_cu = std::max(1, prop.multiProcessorCount);
@@ -1730,7 +1743,11 @@ extern "C" int zj_batch_destroy(zj_batch* b) {
     if (!b) return ZJ_ERR_ARG;
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
-    for (zj_solver* h : b->m) --h->in_batches;
+    if (b->done) hipEventSynchronize(b->done);  // a step may have run on a caller's stream
+    for (zj_solver* h : b->m) {
+        --h->in_batches;
+        if (h->after_batch == b->done) h->after_batch = nullptr;
+    }
     batch_release(b);
     delete b;
     return ZJ_OK;
@@ -1786,7 +1803,11 @@ extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
         b->dt_on_device.assign(dts, dts + n);
     }
     ZJ_BHIP(b, hipGraphLaunch(b->gexec, s));
-    for (zj_solver* h : b->m) h->l2_blocks = h->nblk;
+    ZJ_BHIP(b, hipEventRecord(b->done, s));
+    for (zj_solver* h : b->m) {
+        h->l2_blocks = h->nblk;
+        h->after_batch = b->done;
+    }
     return ZJ_OK;
 }
 
