@@ -6,7 +6,9 @@
  * plain pointers and sizes, int status codes (ZJ_OK = 0), never exit().  One
  * handle owns one mesh's device state on one GPU.  Host arrays are copied at
  * the call; the caller keeps ownership.  Streams are hipStream_t passed as
- * void* (NULL = the handle's own stream).  Calls on one handle are not
+ * void* (NULL = the handle's own stream); work a call queues on a caller's
+ * stream is ordered against the handle's other calls by that caller
+ * (synchronize that stream, or pass it again).  Calls on one handle are not
  * thread-safe; different handles may be used from different threads.  The
  * error text of calls without a handle (a failed create, zj_gmsh_read, the
  * I/O calls) is kept per thread.
