@@ -87,7 +87,7 @@ def test_viscous_steps():
     assert r < 1e-10
 
 
-@pytest.mark.parametrize("p", [2, 4])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
 def test_viscous_fp32bulk_residual(p):
     say(f"case: viscous FP32-bulk residual vs oracle, TGV 6x6 P{p} roe/hu Re 100")
     s, tb, t = _pair(cases.tgv_mesh(6), p, "roe", "hu", precision="fp32bulk")
