@@ -229,6 +229,13 @@ int zj_set_xcd_groups(zj_handle h, int groups);
  * is recomputed from U^n); finally zj_adjoint_get.  dt_bar (may be NULL)
  * receives dJ/d(dt_n) for that step. */
 int zj_adjoint_set(zj_handle h, const double* ubar_aos);
+/* Objective on the device (no reference counterpart): J = sum over the
+ * points of w (m_x^2 + m_y^2) / (2 rho) of the current state, the domain
+ * integral of the kinetic energy for w = quadrature weight x |J| (basis.cpp:484,
+ * fr_solver.cpp:85-129); seeds the adjoint with dJ/dU as zj_adjoint_set would,
+ * with no state or cotangent leaving the GPU.  d_weights: device [E][S] in the
+ * caller's element order.  J (may be NULL) receives the value (synchronizes). */
+int zj_adjoint_seed_kinetic_energy(zj_handle h, const double* d_weights, double* J, void* stream);
 int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void* stream);
 int zj_adjoint_get(zj_handle h, double* ubar_aos);
 int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stream);

@@ -55,7 +55,11 @@ def c5():
 
 
 def test_c5_objective_matches_forward(c5):
-    assert c5["J"] == c5["forward"](c5["U0"])
+    """The gradient's J (summed on the GPU after the checkpointed forward
+    pass) against the same 256 steps run one by one and J summed on the host:
+    the trajectories are the same bits, the two sums differ in order only."""
+    J, Jf = c5["J"], c5["forward"](c5["U0"])
+    assert abs(J - Jf) <= 1e-13 * abs(Jf)
 
 
 def test_c5_gradient_32_directions(c5):

@@ -91,6 +91,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_pvtu_write", "zj_gmsh_periodic", "zj_pair_periodic_nodes",
            "zj_tri_lift", "zj_time_stages", "zj_tri_time_stages", "zj_adjoint_get_re",
            "zj_forward_checkpointed", "zj_adjoint_sweep", "zj_adjoint_tape_segments",
+           "zj_adjoint_seed_kinetic_energy",
            "zj_tri_adjoint_set", "zj_tri_adjoint_set_device", "zj_tri_adjoint_get", "zj_tri_adjoint_get_device",
            "zj_tri_rk3_step_vjp", "zj_tri_residual_vjp", "zj_tri_residual_vjp_device",
            # the gpu:: free functions (zhijian_amd.gpu_ops binds them)
@@ -200,6 +201,7 @@ def lib() -> C.CDLL:
         L.zj_forward_checkpointed.argtypes = [H, _dp, C.c_int, C.c_int, C.c_void_p]
         L.zj_adjoint_sweep.argtypes = [H, _dp, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.zj_adjoint_tape_segments.argtypes = [H, C.c_int]
+        L.zj_adjoint_seed_kinetic_energy.argtypes = [H, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
         L.zj_tri_adjoint_set.argtypes = [H, _dp]
         L.zj_tri_adjoint_get.argtypes = [H, _dp]
         L.zj_tri_adjoint_set_device.argtypes = [H, C.c_void_p, C.c_void_p]

@@ -11,8 +11,8 @@ fr_solver.cpp:85-129).  The forward trajectory keeps one state every
 device tape as it steps; the reverse sweep (zj_adjoint_sweep) recomputes
 every other segment from its checkpoint, recording the stage states and
 traces of its steps on the tape, and runs the stage VJPs backwards from the
-tape.  Both passes run in the library, on the GPU:
-the host sees the final state (for J and its gradient) and the result.
+tape.  Both passes and the objective run in the library, on the GPU:
+the host sees J and the resulting gradient.
 """
 from __future__ import annotations
 
@@ -42,6 +42,18 @@ def kinetic_energy_grad(U: np.ndarray, t: Tables) -> np.ndarray:
     return g.reshape(-1)
 
 
+def _device_weights(sol: Solver, t: Tables):
+    """The objective's weights (quadrature weight x |J| per point, [E][S]) on
+    the GPU, uploaded once per solver and tables."""
+    from .hiprt import DeviceArray
+    cached = getattr(sol, "_ke_weights", None)
+    if cached is None or cached[0] is not t:
+        w = np.ascontiguousarray((t.ops["qw"][None, :] * t.J.reshape(t.E, t.S)).reshape(-1), np.float64)
+        cached = (t, DeviceArray(w))
+        sol._ke_weights = cached
+    return cached[1]
+
+
 def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every: int = 16, want_dt: bool = True,
              stream=None, want_re: bool = False, tape_segments: int | None = None):
     """J(U^N) after len(dts) steps from U0, dJ/dU0 and dJ/d(dt_n) (None
@@ -55,9 +67,8 @@ def gradient(sol: Solver, t: Tables, U0: np.ndarray, dts: Sequence[float], every
         sol.adjoint_tape_segments(tape_segments)
     sol.set_state(np.array(U0, np.float64).reshape(-1))
     sol.forward_checkpointed(dts, every, stream)
-    UN = sol.get_state()
-    J = kinetic_energy(UN, t)
-    sol.adjoint_set(kinetic_energy_grad(UN, t))
+    # J and its gradient on the GPU: the final state never leaves it
+    J = sol.adjoint_seed_kinetic_energy(_device_weights(sol, t).ptr, stream)
     dtb = sol.adjoint_sweep(dts, every, want_dt, stream)
     if dtb is None:
         dtb = [None] * steps

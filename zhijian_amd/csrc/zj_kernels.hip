@@ -499,6 +499,31 @@ __global__ __launch_bounds__(BLOCK) void nonfinite_kernel(const double* __restri
     if (threadIdx.x == 0 && r > 0) atomicAdd(count, (unsigned long long)r);
 }
 
+// Kinetic energy sum_i w_i (m_x^2 + m_y^2) / (2 rho) of a state in SoA planes
+// (device element order; the weights in the caller's order, through perm),
+// one partial per workgroup, and its gradient into the planes G.
+__global__ __launch_bounds__(BLOCK) void kinetic_energy_kernel(const double* __restrict__ U,
+                                                               const double* __restrict__ w,
+                                                               const int* __restrict__ perm, int E, int S,
+                                                               double* __restrict__ G, double* __restrict__ partial) {
+    __shared__ double sRed[BLOCK / 64];
+    const int64_t n = (int64_t)E * S;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const int64_t e = i / S, sp = i - e * S;
+        const double wi = w[(perm ? (int64_t)perm[e] : e) * S + sp];
+        const double rho = U[i], mx = U[n + i], my = U[2 * n + i];
+        const double u = mx / rho, v = my / rho;
+        acc += wi * (0.5 * (mx * u + my * v));
+        G[i] = -0.5 * wi * (u * u + v * v);
+        G[n + i] = wi * u;
+        G[2 * n + i] = wi * v;
+        G[3 * n + i] = 0.0;
+    }
+    const double r = block_reduce<OpSum>(acc, sRed);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
 __global__ __launch_bounds__(BLOCK) void finalize_kernel(const double* __restrict__ partial, int n,
                                                          int op, double* __restrict__ out) {
     __shared__ double sRed[BLOCK / 64];
@@ -712,6 +737,12 @@ hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S,
 
 hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count, int nblk, hipStream_t s) {
     nonfinite_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, n, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_kinetic_energy(const double* U, const double* w, const int* perm, int E, int S, double* G,
+                                 double* partial, int nblk, hipStream_t s) {
+    kinetic_energy_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, w, perm, E, S, G, partial);
     return hipGetLastError();
 }
 

@@ -210,6 +210,11 @@ hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S,
                              double cfl, double cf, double visc, double* partial, int nblk, hipStream_t s);
 hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count,
                                   int nblk, hipStream_t s);
+// Sum over the points of w (m_x^2 + m_y^2) / (2 rho) as nblk partials, and its
+// gradient with respect to the state into the planes G (perm as above, w in
+// the caller's element order [E][S]).
+hipError_t launch_kinetic_energy(const double* U, const double* w, const int* perm, int E, int S, double* G,
+                                 double* partial, int nblk, hipStream_t s);
 // Deterministic reduction of block partials (fixed order): sqrt of the sum
 // (an L2 norm), the minimum, or the plain sum.
 enum FinalizeOp : int { FIN_SQRT_SUM = 0, FIN_MIN = 1, FIN_SUM = 2 };

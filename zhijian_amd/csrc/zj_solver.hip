@@ -61,7 +61,8 @@ struct zj_solver {
     void* normals = nullptr;
     double* bc_data = nullptr;
     double* hcell = nullptr;
-    double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl, [3] dt_bar, [4] sum of <F_v cotangent, F_v>
+    double* d_scalars = nullptr;  // [0] dt, [1] l2, [2] dt_cfl, [3] dt_bar, [4] sum of <F_v cotangent, F_v>,
+                                  // [5] kinetic energy
     double* partial = nullptr;     // per-workgroup sums of R^2 of the last residual / stage 3
     double* partial_dt = nullptr;  // per-workgroup minima of the CFL dt
     int l2_blocks = 0;             // valid entries of `partial` (0: no residual yet)
@@ -1120,6 +1121,27 @@ extern "C" int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void
     h->A_seeded = true;
     if (h->viscous)
         ZJ_HIP(h, hipMemsetAsync(h->A_mub, 0, (size_t)h->adj_nt * sizeof(double), pick(h, stream)));
+    return ZJ_OK;
+}
+
+extern "C" int zj_adjoint_seed_kinetic_energy(zj_handle h, const double* d_weights, double* J, void* stream) {
+    if (!h || !d_weights) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    int rc = ensure_adjoint(h);
+    if (rc) return rc;
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    ZJ_HIP(h, launch_kinetic_energy(h->U, d_weights, h->perm, h->E, h->S, h->A_Ub, h->partial_dt, h->nred, s));
+    ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, FIN_SUM, h->d_scalars + 5, s));
+    h->A_has_tb = false;
+    h->A_seeded = true;
+    if (h->viscous) ZJ_HIP(h, hipMemsetAsync(h->A_mub, 0, (size_t)h->adj_nt * sizeof(double), s));
+    if (J) {
+        ZJ_HIP(h, hipMemcpyAsync(h->h_pinned + 5, h->d_scalars + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+        ZJ_HIP(h, hipStreamSynchronize(s));
+        *J = h->h_pinned[5];
+    }
     return ZJ_OK;
 }
 

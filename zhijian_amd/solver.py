@@ -209,6 +209,14 @@ class Solver:
     def adjoint_set_device(self, ptr, stream=None):
         self._chk(lib().zj_adjoint_set_device(self._h, ptr, stream))
 
+    def adjoint_seed_kinetic_energy(self, weights_ptr, stream=None):
+        """J = sum of w (m_x^2 + m_y^2) / (2 rho) of the current state, computed
+        on the GPU, and the adjoint seeded with dJ/dU there
+        (zj_adjoint_seed_kinetic_energy); weights_ptr: device [E][S]."""
+        J = C.c_double()
+        self._chk(lib().zj_adjoint_seed_kinetic_energy(self._h, weights_ptr, C.byref(J), stream))
+        return J.value
+
     def adjoint_get_device(self, ptr, stream=None):
         self._chk(lib().zj_adjoint_get_device(self._h, ptr, stream))
 
