@@ -211,7 +211,7 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
                       for k in (1, 2, 3))
     batch = solver.Batch(sols)
     res = {}
-    for single in ((False,) if ns else (False, True)):  # the single launch covers reference-arithmetic members
+    for single in (False, True):
         batch.set_launch(single)
         for policy in (0, 2, 4):
             batch.set_grid(policy)

@@ -302,13 +302,15 @@ int zj_batch_set_grid(zj_batch_handle b, int policy);
  * members' orders and metric forms (the stage body is picked per tile, the
  * LDS sized for the largest order present); 0 (default): one launch per
  * (order, metric form) group, the groups on parallel graph branches.
- * Results are the same either way.  single = 1 covers reference-arithmetic
- * members only: with a textbook member it returns ZJ_ERR_UNSUPPORTED. */
+ * Results are the same either way.  single = 1 takes a batch whose members
+ * all run the reference arithmetic, or all have the Navier-Stokes terms (one
+ * BR2 launch per stage across the orders); any other batch returns
+ * ZJ_ERR_UNSUPPORTED. */
 int zj_batch_set_launch(zj_batch_handle b, int single);
 /* Times `reps` batched steps of sizes dts for each launch form and grid
  * policy (0, 2, 4), keeps the fastest (*single, *policy, *ms per step; any may
  * be NULL), and restores every member's state as it was before the call;
- * with textbook members the single launch is not among the candidates. */
+ * the single launch is a candidate where it applies. */
 int zj_batch_autotune(zj_batch_handle b, const double* dts, int reps, int* single, int* policy, double* ms);
 /* Times the residual launch for every tile (full, half) and XCD order and
  * keeps the fastest; results do not depend on the choice. */

@@ -120,7 +120,7 @@ def test_batch_textbook_and_navier_stokes_members(precision):
     members batched with reference-arithmetic ones: one launch per (order,
     metric form, scheme) group; every member's step equals its own bit for
     bit under both grid policies, and the single launch (reference arithmetic
-    only) is refused."""
+    or Navier-Stokes members only, not a mix) is refused."""
     say(f"case: batch of {len(SCHEME_MEMBERS)} meshes: NS Re 100-5000, textbook Euler, reference; {precision}")
     built = [_scheme_member(*m, precision, seed=70 + k) for k, m in enumerate(SCHEME_MEMBERS)]
     a = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
@@ -152,6 +152,41 @@ def test_batch_textbook_and_navier_stokes_members(precision):
     for k, (s1, s2) in enumerate(zip(a, b)):
         assert np.array_equal(s1.get_state(), s2.get_state()), SCHEME_MEMBERS[k]
         assert np.all(np.isfinite(s1.get_state()))
+    batch.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_single_launch_navier_stokes_members(precision):
+    """A batch of Navier-Stokes members only (P1-P5, each its own Re, Roe or
+    HLLC, HU or DG): one BR2 launch per stage across the orders gives every
+    member's own step bit for bit, under both grid policies; the autotuner
+    may pick it."""
+    say(f"case: single-launch Navier-Stokes batch P1-P5 vs own steps, {precision}")
+    ns = [m for m in SCHEME_MEMBERS if m[2] == "ns"]
+    built = [_scheme_member(*m, precision, seed=130 + k) for k, m in enumerate(ns)]
+    a = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    b = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    for s1, s2, (t, U, _) in zip(a, b, built):
+        s1.set_state(U)
+        s2.set_state(U)
+    dts = np.array([0.5 * s.dt_cfl(0.5) for s in a])
+    batch = solver.Batch(a)
+    batch.set_launch(True)
+    for step in range(3):
+        batch.set_grid(0 if step < 2 else 4)
+        batch.rk3_step(dts)
+        for s2, dt in zip(b, dts):
+            s2.rk3_step(dt)
+    batch.synchronize()
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), ns[k]
+    single, policy, ms = batch.autotune(dts, reps=2)
+    say(f"  autotune: single={single} policy={policy} {ms:.3f} ms per step")
+    batch.rk3_step(dts)
+    for s2, dt in zip(b, dts):
+        s2.rk3_step(dt)
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        assert np.array_equal(s1.get_state(), s2.get_state()), ns[k]
     batch.close()
 
 

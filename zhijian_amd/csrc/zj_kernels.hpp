@@ -186,6 +186,12 @@ hipError_t launch_visc_trace(int P, int E, const double* U, const R* Jinv_e, R* 
 template <typename R>
 hipError_t launch_batched_visc(int P, int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
                                hipStream_t s);
+// The viscous stage for many meshes of every order in one launch
+// (lds_bytes = visc_all_lds of the largest order present).
+template <typename R>
+hipError_t launch_batched_visc_all(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                                   int lds_bytes, hipStream_t s);
+int visc_all_lds(int P, bool fp32);
 int num_blocks(int E, int epb);
 
 // perm[e] = caller index of device element e (nullptr: same order), S points per element

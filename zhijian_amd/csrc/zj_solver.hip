@@ -1493,6 +1493,8 @@ struct zj_batch {
     // reference-arithmetic members only)
     bool single = false;
     bool any_textbook = false;
+    bool all_viscous = false;   // every member has the Navier-Stokes terms: the one-launch BR2 kernel
+    int visc_lds = 0;           // its LDS bytes (largest order present)
     void* all_insts[4] = {nullptr, nullptr, nullptr, nullptr};  // BatchInst<R>[members] per mode
     int2* all_items = nullptr;  // (member, tile), larger members first
     int n_all = 0;
@@ -1631,6 +1633,11 @@ static int batch_build(zj_batch* b, const zj_handle* hs, int n) {
                                                  : batch_upload_insts<float>(b, gg.insts, members);
                 if (rc) return rc;
             }
+    b->all_viscous = true;
+    for (int i = 0; i < n; ++i) {
+        b->all_viscous &= hs[i]->viscous;
+        b->visc_lds = std::max(b->visc_lds, visc_all_lds(hs[i]->P, b->prec == ZJ_PREC_FP32BULK));
+    }
     // the single-launch form: one list of every member's tiles, larger members first
     {
         std::vector<int> order(n);
@@ -1682,7 +1689,13 @@ static hipError_t batch_enqueue(zj_batch* b, hipStream_t s) {
     if (b->single) {
         const int grid = b->policy <= 0 ? b->n_all : std::min(b->n_all, b->policy * b->n_cu);
         for (int mode = MODE_STAGE1; mode <= MODE_STAGE3 && e == hipSuccess; ++mode) {
-            if (b->prec == ZJ_PREC_FP64)
+            if (b->all_viscous && b->prec == ZJ_PREC_FP64)
+                e = launch_batched_visc_all<double>(mode, (const BatchInst<double>*)b->all_insts[mode], b->all_items,
+                                                    b->n_all, grid, b->visc_lds, s);
+            else if (b->all_viscous)
+                e = launch_batched_visc_all<float>(mode, (const BatchInst<float>*)b->all_insts[mode], b->all_items,
+                                                   b->n_all, grid, b->visc_lds, s);
+            else if (b->prec == ZJ_PREC_FP64)
                 e = launch_batched_all<double>(mode, (const BatchInst<double>*)b->all_insts[mode], b->all_items,
                                                b->n_all, grid, b->all_lds, s);
             else
@@ -1789,8 +1802,9 @@ extern "C" int zj_batch_set_grid(zj_batch* b, int policy) {
 
 extern "C" int zj_batch_set_launch(zj_batch* b, int single) {
     if (!b || (single != 0 && single != 1)) return ZJ_ERR_ARG;
-    if (single && b->any_textbook) {
-        b->err = "the single launch runs the reference arithmetic; this batch has textbook members";
+    if (single && b->any_textbook && !b->all_viscous) {
+        b->err = "the single launch takes reference-arithmetic members or Navier-Stokes members, not a mix "
+                 "or textbook Euler members";
         return ZJ_ERR_UNSUPPORTED;
     }
     if ((single != 0) != b->single) {
@@ -1868,7 +1882,7 @@ extern "C" int zj_batch_autotune(zj_batch* b, const double* dts, int reps, int* 
         int bs = b->single ? 1 : 0, bp = b->policy;
         for (int cs : cand_single)
             for (int cp : cand_policy) {
-                if (cs && b->any_textbook) continue;
+                if (cs && b->any_textbook && !b->all_viscous) continue;
                 if (int r = zj_batch_set_launch(b, cs)) return r;
                 if (int r = zj_batch_set_grid(b, cp)) return r;
                 if (int r = zj_batch_rk3_step(b, dts, b->stream)) return r;  // captures and warms up

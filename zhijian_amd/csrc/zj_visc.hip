@@ -38,8 +38,10 @@ constexpr int TW = 3 * NV;  // trace words per flux point: U, dU/dx, dU/dy
 
 // A tile's arrays.  G holds the gradient (x then y, each [var][point]), then
 // the contravariant total fluxes in the same slots; Tr, TGx, TGy the traces
-// of U and of the gradient at the tile's own face points; J holds the jumps,
-// then the flux differences.
+// of U and (FP32 only) of the gradient at the tile's own face points; J holds
+// the jumps, then the flux differences.  ViscSmem: static arrays of the
+// single-mesh and per-order kernels; ViscSmemDyn: the same arrays in one
+// dynamic LDS block (the one-launch batch, sized for its largest order).
 template <typename R, int P, int EPB>
 struct ViscSmem {
     using Y = Lay<R, P>;
@@ -58,6 +60,35 @@ struct ViscSmem {
     __device__ static R* C() { __shared__ R a[2 * N1]; return a; }
     __device__ static double* Red() { __shared__ double a[BLOCK / 64]; return a; }
     __device__ static unsigned char* Out() { __shared__ unsigned char a[EPB * 4]; return a; }
+};
+
+template <typename R, int P, int EPB>
+struct ViscSmemDyn {
+    using Y = Lay<R, P>;
+    static constexpr int N1 = P + 1;
+    static constexpr bool PRE = std::is_same<R, float>::value;
+    static constexpr int oU = 0, oGx = oU + EPB * Y::UE, oGy = oGx + EPB * Y::UE, oTr = oGy + EPB * Y::UE;
+    static constexpr int oTGx = oTr + EPB * Y::TE, oTGy = oTGx + (PRE ? EPB * Y::TE : 0);
+    static constexpr int oJ = oTGy + (PRE ? EPB * Y::TE : 0), oFI = oJ + EPB * Y::TE, oD = oFI + EPB * Y::TE;
+    static constexpr int oLm = oD + N1 * N1, oLp = oLm + N1, oC = oLp + N1, nR = oC + 2 * N1;
+    static constexpr int red_byte = ((nR * (int)sizeof(R) + 15) / 16) * 16;
+    static constexpr int bytes = red_byte + (BLOCK / 64) * 8 + EPB * 4;
+    unsigned char* base;
+    __device__ R* r(int o) const { return reinterpret_cast<R*>(base) + o; }
+    __device__ R* U() const { return r(oU); }
+    __device__ R* Gx() const { return r(oGx); }
+    __device__ R* Gy() const { return r(oGy); }
+    __device__ R* Tr() const { return r(oTr); }
+    __device__ R* TGx() const { return r(oTGx); }
+    __device__ R* TGy() const { return r(oTGy); }
+    __device__ R* J() const { return r(oJ); }
+    __device__ R* FI() const { return r(oFI); }
+    __device__ R* D() const { return r(oD); }
+    __device__ R* Lm() const { return r(oLm); }
+    __device__ R* Lp() const { return r(oLp); }
+    __device__ R* C() const { return r(oC); }
+    __device__ double* Red() const { return reinterpret_cast<double*>(base + red_byte); }
+    __device__ unsigned char* Out() const { return base + red_byte + (BLOCK / 64) * 8; }
 };
 
 // gradient of one variable at point (i, j) of an element (u[S], xi index
@@ -88,35 +119,34 @@ constexpr int visc_threads() {
 }
 
 // One tile's viscous stage (the workgroup's NTHR threads, the tile's elements).
-template <typename R, int P, int EPB, int MODE, int NTHR>
+template <typename R, int P, int EPB, int MODE, int NTHR, class SM>
 __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
-                                          const int tile, const int tid) {
+                                          const int tile, const int tid, const SM& sm) {
     using Y = Lay<R, P>;
-    using SM = ViscSmem<R, P, EPB>;
     constexpr bool INC = std::is_same<R, float>::value;
     constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1;
     constexpr int NT = EPB * S, NF = EPB * 4 * FP;
     static_assert(NT <= NTHR, "tile exceeds the workgroup");
-    R* const sU = SM::U();
-    R* const sGx = SM::Gx();
-    R* const sGy = SM::Gy();
+    R* const sU = sm.U();
+    R* const sGx = sm.Gx();
+    R* const sGy = sm.Gy();
     // FP32: the gradient traces once per face point, in LDS (fewer VALU
     // instructions, 71 instead of 112 VGPRs at P4).  FP64 recomputes them where
     // they are read: with the two extra arrays its LDS caps it at two
     // workgroups per CU and the extra phase measured 2.5% slower at P4.
     constexpr bool PRE = std::is_same<R, float>::value;
-    R* const sTr = SM::Tr();
-    R* const sTGx = PRE ? SM::TGx() : nullptr;
-    R* const sTGy = PRE ? SM::TGy() : nullptr;
-    R* const sJ = SM::J();
-    R* const sFI = SM::FI();
-    R* const sD = SM::D();
-    R* const sLm = SM::Lm();
-    R* const sLp = SM::Lp();
-    R* const sC = SM::C();
-    double* const sRed = SM::Red();
-    unsigned char* const sOut = SM::Out();
+    R* const sTr = sm.Tr();
+    R* const sTGx = PRE ? sm.TGx() : nullptr;
+    R* const sTGy = PRE ? sm.TGy() : nullptr;
+    R* const sJ = sm.J();
+    R* const sFI = sm.FI();
+    R* const sD = sm.D();
+    R* const sLm = sm.Lm();
+    R* const sLp = sm.Lp();
+    R* const sC = sm.C();
+    double* const sRed = sm.Red();
+    unsigned char* const sOut = sm.Out();
 
     const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
@@ -385,7 +415,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
 template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
 __global__ __launch_bounds__(NTHR) void visc_stage_kernel(const StageArgs<R> a, const OpTables<R> ops,
                                                           const ViscParams vp) {
-    visc_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups), threadIdx.x);
+    visc_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups), threadIdx.x, ViscSmem<R, P, EPB>{});
 }
 
 // The stage for many meshes of order P: work item w = (instance, tile); a
@@ -401,7 +431,36 @@ __global__ __launch_bounds__(BLOCK) void batched_visc_kernel(const BatchInst<R>*
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));  // keeps the per-thread index math in the loop body (registers)
         const BatchInst<R>& in = insts[inst];
-        visc_body<R, P, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, tile, tid);
+        visc_body<R, P, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, tile, tid, ViscSmem<R, P, EPB>{});
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
+// One launch per stage for a batch of Navier-Stokes members of every order:
+// the item's instance picks the stage body; the LDS buffer is dynamic, sized
+// for the largest order present.
+template <typename R, int MODE>
+__global__ __launch_bounds__(BLOCK) void batched_visc_all_kernel(const BatchInst<R>* __restrict__ insts,
+                                                                 const int2* __restrict__ items, int n_items) {
+    extern __shared__ __align__(16) unsigned char raw_visc[];
+    for (int w = blockIdx.x; w < n_items; w += gridDim.x) {
+        const int inst = __builtin_amdgcn_readfirstlane(items[w].x);
+        const int tile = __builtin_amdgcn_readfirstlane(items[w].y);
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const BatchInst<R>& in = insts[inst];
+        const int P = __builtin_amdgcn_readfirstlane(in.P);
+#define ZJ_VALL(PP)                                                                                            \
+    case PP: {                                                                                                 \
+        constexpr int EPB = BLOCK / ((PP + 1) * (PP + 1));                                                     \
+        visc_body<R, PP, EPB, MODE, BLOCK>(in.args, in.ops, in.vp, tile, tid, ViscSmemDyn<R, PP, EPB>{raw_visc}); \
+        break;                                                                                                 \
+    }
+        switch (P) {
+            ZJ_VALL(1) ZJ_VALL(2) ZJ_VALL(3) ZJ_VALL(4) ZJ_VALL(5)
+        default: break;
+        }
+#undef ZJ_VALL
         __syncthreads();  // the next item reuses the LDS
     }
 }
@@ -411,14 +470,9 @@ template <typename R, int P, int EPB>
 __global__ __launch_bounds__(BLOCK) void visc_trace_kernel(int E, const double* U, const R* Jinv, R* T_out,
                                                            const OpTables<R> ops, int xcd_groups) {
     using Y = Lay<R, P>;
-    using SM = ViscSmem<R, P, EPB>;
     constexpr int N1 = P + 1, S = N1 * N1, FP = N1, NT = EPB * S, NF = EPB * 4 * FP;
-    R* const sU = SM::U();
-    R* const sGx = SM::Gx();
-    R* const sGy = SM::Gy();
-    R* const sD = SM::D();
-    R* const sLm = SM::Lm();
-    R* const sLp = SM::Lp();
+    __shared__ R sU[EPB * Y::UE], sGx[EPB * Y::UE], sGy[EPB * Y::UE];
+    __shared__ R sD[N1 * N1], sLm[N1], sLp[N1];
     const int tid = threadIdx.x;
     const int e0 = tile_of_block(xcd_groups) * EPB;
     const int64_t PS = (int64_t)E * S;
@@ -1189,6 +1243,34 @@ hipError_t launch_batched_visc(int P, int mode, const BatchInst<R>* insts, const
 }
 
 template <typename R>
+hipError_t launch_batched_visc_all(int mode, const BatchInst<R>* insts, const int2* items, int n_items, int grid,
+                                   int lds_bytes, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    const dim3 g(std::max(1, std::min(grid, n_items))), b(BLOCK);
+    const unsigned lds = (unsigned)lds_bytes;
+    switch (mode) {
+    case MODE_STAGE1: klaunch(batched_visc_all_kernel<R, MODE_STAGE1>, g, b, lds, s, insts, items, n_items); break;
+    case MODE_STAGE2: klaunch(batched_visc_all_kernel<R, MODE_STAGE2>, g, b, lds, s, insts, items, n_items); break;
+    case MODE_STAGE3: klaunch(batched_visc_all_kernel<R, MODE_STAGE3>, g, b, lds, s, insts, items, n_items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int visc_all_lds(int P, bool fp32) {
+    switch (P) {
+#define ZJ_VL(PP)                                                                       \
+    case PP: {                                                                          \
+        constexpr int EPB = BLOCK / ((PP + 1) * (PP + 1));                              \
+        return fp32 ? ViscSmemDyn<float, PP, EPB>::bytes : ViscSmemDyn<double, PP, EPB>::bytes; \
+    }
+        ZJ_VL(1) ZJ_VL(2) ZJ_VL(3) ZJ_VL(4) ZJ_VL(5)
+#undef ZJ_VL
+    default: return 0;
+    }
+}
+
+template <typename R>
 hipError_t launch_visc_trace(int P, int E, const double* U, const R* Jinv, R* T, const OpTables<R>& ops, int groups,
                              hipStream_t s) {
     switch (P) {
@@ -1204,6 +1286,10 @@ template hipError_t launch_visc_stage<double>(int, int, const StageArgs<double>&
                                               const ViscParams&, hipStream_t);
 template hipError_t launch_visc_stage<float>(int, int, const StageArgs<float>&, const OpTables<float>&,
                                              const ViscParams&, hipStream_t);
+template hipError_t launch_batched_visc_all<double>(int, const BatchInst<double>*, const int2*, int, int, int,
+                                                    hipStream_t);
+template hipError_t launch_batched_visc_all<float>(int, const BatchInst<float>*, const int2*, int, int, int,
+                                                   hipStream_t);
 template hipError_t launch_batched_visc<double>(int, int, const BatchInst<double>*, const int2*, int, int,
                                                 hipStream_t);
 template hipError_t launch_batched_visc<float>(int, int, const BatchInst<float>*, const int2*, int, int,
