@@ -503,7 +503,7 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
     int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
     if (live) {
-#pragma unroll 1
+#pragma unroll
         for (int m = 0; m < NP; ++m) sU[sp(el, m, v)] = R(a.U_in[base + (int64_t)m * EB * NV]);
         const R* G = a.geo + (int64_t)e * GEO;
         rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
