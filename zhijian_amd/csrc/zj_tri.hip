@@ -699,7 +699,7 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
         // liftings of this face's jumps at both sides' points; the partner's
         // jump at its slot P-f2 is minus this side's at f2
         R lo[NV] = {R(0), R(0), R(0), R(0)}, ln[NV] = {R(0), R(0), R(0), R(0)};
-#pragma unroll 1
+#pragma unroll 2
         for (int f2 = 0; f2 < FP; ++f2) {
             const int j2 = kk * FP + f2, o2 = ek * FP + (P - f2);
             R uo[NV], un[NV];
