@@ -441,6 +441,8 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     __shared__ R sQ[2 * TT * NP * NV];   // Qx, Qy; later the flux differences [TT][NFP][NV]
     __shared__ double sred[NWV];
     __shared__ R sEdge[TT * 3];
+    __shared__ int2 sLink[TT * 3];  // the tile's edge links and face normals
+    __shared__ R sNrm[TT * 3 * 2];
     __shared__ unsigned char sList[TT * 3];
     __shared__ int sCnt[NWV];
     R* const sQx = sQ;
@@ -512,6 +514,10 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
         if (v < 3) {
             const R ax = v == 0 ? sx : (v == 1 ? rx + sx : rx), ay = v == 0 ? sy : (v == 1 ? ry + sy : ry);
             sEdge[el * 3 + v] = sqrtR(ax * ax + ay * ay);
+            // the edge's link and normal, for whichever lane solves its face points
+            sLink[el * 3 + v] = v == 0 ? L[0] : (v == 1 ? L[1] : L[2]);
+            sNrm[(el * 3 + v) * 2] = G[4 + 2 * v];
+            sNrm[(el * 3 + v) * 2 + 1] = G[5 + 2 * v];
         }
         gradient_of(rx, ry, sx, sy, sQx, sQy);
 #pragma unroll 1
@@ -658,10 +664,10 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
     for (int t = tid; t < nsolved * FP; t += NB) {
         const int c = sList[t / FP], f = t - (t / FP) * FP;
         const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
-        const int2 Lk = a.links[(int64_t)(e0 + elt) * 3 + kk];
+        const int2 Lk = sLink[elt * 3 + kk];
         const bool right = link_role(Lk.y) == ROLE_RIGHT;
-        const R nx = a.geo[(int64_t)(e0 + elt) * GEO + 4 + 2 * kk];
-        const R ny = a.geo[(int64_t)(e0 + elt) * GEO + 5 + 2 * kk];
+        const R nx = sNrm[(elt * 3 + kk) * 2];
+        const R ny = sNrm[(elt * 3 + kk) * 2 + 1];
         const int ek = link_edge(Lk.y), oj = ek * FP + (P - f);
         const int nl = Lk.x - e0;
         const bool in = nl >= 0 && nl < TT;
