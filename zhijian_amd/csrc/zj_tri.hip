@@ -564,7 +564,7 @@ __global__ __launch_bounds__(TT * NV) void tri_visc_stage_kernel(TriArgs<R> a) {
                 jwy[j] = wy * jmp;
             }
         }
-#pragma unroll 1
+#pragma unroll 2
         for (int i = 0; i < NP; ++i) {
             R qx = sQx[sp(el, i, v)], qy = sQy[sp(el, i, v)];
 #pragma unroll
