@@ -415,6 +415,24 @@ def time_tri(n, p, steps, warmup, err_steps, stream, hiprt, cpu_seconds, Re=None
     out["fp32bulk_error_vs_fp64"] = dict(steps=err_steps, rel_l2=float(np.linalg.norm(b - a) / np.linalg.norm(a)),
                                          rel_linf=float(np.abs(b - a).max() / np.abs(a).max()))
     out["fp32bulk_over_fp64"] = out["fp32bulk"]["dof_updates_per_s"] / out["fp64"]["dof_updates_per_s"]
+    # one step backwards (its cost does not depend on the cotangent's values)
+    for name, s in sols.items():
+        s.set_state(U0)
+        s.adjoint_set(U0)
+        s.rk3_step_vjp(dt, stream.ptr, want_dt=False)
+        stream.synchronize()
+        k = max(steps // 4, 3)
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        for _ in range(k):
+            s.rk3_step_vjp(dt, stream.ptr, want_dt=False)
+        e1.record(stream)
+        e1.synchronize()
+        ms_adj = e0.elapsed_ms(e1) / k
+        out[name]["adjoint"] = dict(
+            adj_ms_per_step=ms_adj, adjoint_over_forward=ms_adj / out[name]["ms_per_step"],
+            note="adjoint step = traces of U^n " + ("and of the two recomputed stages " if Re else "")
+                 + "+ stages 1-2 recomputed + 3 VJP launches")
     for s in sols.values():
         s.close()
     if cpu_seconds > 0:

@@ -898,6 +898,8 @@ struct TriVjpArgs {
     int riemann;
     int xcd_groups;
     int lift_off;
+    double mu, kappa, eta;  // Navier-Stokes terms (tri_visc_vjp_kernel)
+    int lf_off;
 };
 
 template <typename R, int P, int MODE>
@@ -1258,6 +1260,705 @@ __global__ __launch_bounds__(BLOCK) void tri_complete_kernel(int E, const int2* 
     }
 }
 
+// ------------------------------------------------------------ adjoint, BR2
+// Vector-Jacobian product of one triangle stage with the Navier-Stokes terms
+// (tri_visc_stage_kernel), or of its residual alone.  The forward values are
+// recomputed as that kernel forms them (traces of U and of its gradient, the
+// corrected gradient, the point fluxes, the flux differences), with the point
+// fluxes kept apart from the corrected gradient; then, per lane (element,
+// variable) unless stated:
+//   R1  cotangent of u_out: its partial value plus, on cross-tile edges, the
+//       transposed traces of U and of its gradient; the RK combination
+//   R2  the lifting transposed: the flux differences' cotangents
+//   R4  the divergence and the interpolated normal fluxes transposed; then one
+//       lane per point (all four variables): the Euler and viscous flux VJPs,
+//       the cotangents of U and of the corrected gradient at the point
+//   R5  the corrected gradient's lifted jumps transposed: each edge point's
+//       jump cotangent
+//   R3  per solved face point: the common flux's cotangent, the Riemann and
+//       viscous flux VJPs at the face, the jump cotangents of both sides'
+//       corrected gradients; the cotangents of both sides' traces (of U and
+//       of the gradient) at their slots (LDS, or the cross-tile neighbour's
+//       slot in HBM, 12 words); then the face liftings transposed, gathered
+//       over the edge's points
+//   R6  the traces and the gradient transposed: cotangent of q
+template <typename R, int P> struct TriViscVjpTile {
+    static constexpr int NP = (P + 1) * (P + 2) / 2, NFP = 3 * (P + 1);
+    static constexpr int XW = 2 * (NP > NFP ? NP : NFP);  // the point fluxes, later the jump cotangents
+    static constexpr int words = 4 * NFP + 3 * NP + XW;    // per element and variable
+    static constexpr int bytes(int tt) { return tt * words * NV * (int)sizeof(R); }
+    static constexpr int value = bytes(64) <= 61440 ? 64 : (bytes(32) <= 61440 ? 32 : 16);
+    static constexpr bool fits = bytes(16) <= 61440;
+};
+
+template <typename R, int P, int MODE, int TT>
+__global__ __launch_bounds__(TT * NV) void tri_visc_vjp_kernel(TriVjpArgs<R> a) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int NB = TT * NV, NWV = NB / 64;
+    static_assert(NB % 64 == 0, "tile layout");
+    using OP = TriOps<P>;
+    constexpr int O0 = ops_offset(P), CA = OP::CA, OI = O0 + OP::C;
+    constexpr int XW = TriViscVjpTile<R, P>::XW;
+    constexpr double ca = MODE == MODE_STAGE2 ? 0.75 : (MODE == MODE_STAGE3 ? 1.0 / 3.0 : 0.0);
+    constexpr double cb = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    constexpr double cc = MODE == MODE_STAGE2 ? 0.25 : (MODE == MODE_STAGE3 ? 2.0 / 3.0 : 1.0);
+    const int OLT = O0 + a.lift_off;
+    const int LFO = lf_offset(P) + a.lf_off;
+    __shared__ R sTu[TT * NFP * NV];   // traces of U
+    __shared__ R sTx[TT * NFP * NV];   // traces of the gradient, then their cotangents
+    __shared__ R sTy[TT * NFP * NV];
+    __shared__ R sD[TT * NFP * NV];    // flux differences, their cotangents, then the U traces' cotangents
+    __shared__ R sU[TT * NP * NV];     // q, then the flux VJPs' cotangents of U at the points
+    __shared__ R sQx[TT * NP * NV];    // corrected gradient, then its cotangent
+    __shared__ R sQy[TT * NP * NV];
+    __shared__ R sX[TT * XW * NV];     // contravariant point fluxes, then their x / y cotangents;
+                                       // later the jump cotangents and the face liftings' seeds
+    __shared__ double sred[NWV];
+    __shared__ R sEdge[TT * 3];
+    __shared__ int2 sLink[TT * 3];
+    __shared__ R sNrm[TT * 3 * 2];
+    __shared__ unsigned char sList[TT * 3];
+    __shared__ int sCnt[NWV];
+    R* const sFR = sX;
+    R* const sFS = sX + TT * NP * NV;
+    R* const sJ = sX;
+    R* const sC = sX + TT * NFP * NV;
+
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
+    const int tile = tile_of_block(a.xcd_groups);
+    const int e0 = tile * TT, e = e0 + el;
+    const bool live = e < a.E;
+    const R g = R(a.gamma), mu = R(a.mu), kappa = R(a.kappa), eta = R(a.eta);
+    const int64_t base = blk(e, 0, NP) + v;
+    auto sl = [](int elx, int j, int w) { return (elx * NFP + j) * NV + w; };
+    auto sp = [](int elx, int m, int w) { return (elx * NP + m) * NV + w; };
+
+    // ---- forward 0: state, gradient, traces of U and of the gradient
+    R rx = R(0), ry = R(0), sx = R(0), sy = R(0);
+    R jwx[3] = {R(0), R(0), R(0)}, jwy[3] = {R(0), R(0), R(0)};  // each edge's jump weights
+    int2 L[3] = {make_int2(0, 0), make_int2(0, 0), make_int2(0, 0)};
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) sU[sp(el, m, v)] = R(a.U_in[base + (int64_t)m * EB * NV]);
+        const R* G = a.geo + (int64_t)e * GEO;
+        rx = G[0]; ry = G[1]; sx = G[2]; sy = G[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[k] = a.links[(int64_t)e * 3 + k];
+        R se[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const R ax = k == 0 ? sx : (k == 1 ? rx + sx : rx), ay = k == 0 ? sy : (k == 1 ? ry + sy : ry);
+            se[k] = sqrtR(ax * ax + ay * ay);
+            const R sg = link_role(L[k].y) == ROLE_RIGHT ? R(-1) : R(1);
+            jwx[k] = sg * se[k] * G[4 + 2 * k];
+            jwy[k] = sg * se[k] * G[5 + 2 * k];
+        }
+        if (v < 3) {
+            sEdge[el * 3 + v] = se[v];
+            sLink[el * 3 + v] = L[v];
+            sNrm[(el * 3 + v) * 2] = G[4 + 2 * v];
+            sNrm[(el * 3 + v) * 2 + 1] = G[5 + 2 * v];
+        }
+        R gr[NP], gs[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) { gr[i] = R(0); gs[i] = R(0); }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const R um = sU[sp(el, m, v)];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                gr[i] = fmaR(cop<R>(O0 + m * CA + i), um, gr[i]);
+                gs[i] = fmaR(cop<R>(O0 + m * CA + NP + i), um, gs[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            sQx[sp(el, i, v)] = rx * gr[i] + sx * gs[i];
+            sQy[sp(el, i, v)] = ry * gr[i] + sy * gs[i];
+        }
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            R tu = R(0), tx = R(0), ty = R(0);
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const R w = cop<R>(OI + j * NP + m);
+                tu = fmaR(w, sU[sp(el, m, v)], tu);
+                tx = fmaR(w, sQx[sp(el, m, v)], tx);
+                ty = fmaR(w, sQy[sp(el, m, v)], ty);
+            }
+            sTu[sl(el, j, v)] = tu;
+            sTx[sl(el, j, v)] = tx;
+            sTy[sl(el, j, v)] = ty;
+        }
+    }
+    bool owned = false;
+    if (live && v < 3) {
+        const int2 Lv = v == 0 ? L[0] : (v == 1 ? L[1] : L[2]);
+        const int nl = Lv.x - e0;
+        owned = !(link_role(Lv.y) == ROLE_RIGHT && nl >= 0 && nl < TT);
+    }
+    const unsigned long long own_mask = __ballot(owned);
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) sCnt[wave] = __popcll(own_mask);
+    __syncthreads();
+
+    // ---- forward 1: corrected gradient
+    if (live) {
+        R jx[NFP], jy[NFP];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const int2 Lk = L[kk];
+            const int nl = Lk.x - e0;
+            const bool in = nl >= 0 && nl < TT;
+            const int ek = link_edge(Lk.y);
+#pragma unroll
+            for (int f = 0; f < FP; ++f) {
+                const int j = kk * FP + f, oj = ek * FP + (P - f);
+                const R own = sTu[sl(el, j, v)];
+                R oth;
+                if (in) oth = sTu[sl(nl, oj, v)];
+                else oth = a.T_in[((int64_t)Lk.x * NFP + oj) * TWV + v];
+                const R jmp = R(0.5) * (own + oth) - own;
+                jx[j] = jwx[kk] * jmp;
+                jy[j] = jwy[kk] * jmp;
+            }
+        }
+#pragma unroll 1
+        for (int i = 0; i < NP; ++i) {
+            R qx = sQx[sp(el, i, v)], qy = sQy[sp(el, i, v)];
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) {
+                const R l = cop<R>(OLT + j * NP + i);
+                qx = fmaR(l, jx[j], qx);
+                qy = fmaR(l, jy[j], qy);
+            }
+            sQx[sp(el, i, v)] = qx;
+            sQy[sp(el, i, v)] = qy;
+        }
+    }
+    if (owned) {
+        int pos = __popcll(own_mask & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) pos += sCnt[w];
+        sList[pos] = (unsigned char)(el * 3 + v);
+    }
+    __syncthreads();
+
+    // ---- forward 2: contravariant total fluxes at the points (points v, v + 4,
+    // ... of the element, four variables each), then this lane's divergence
+    // and interpolated normal fluxes
+    constexpr int MPL = (NP + NV - 1) / NV;
+    if (live) {
+#pragma unroll 1
+        for (int q = 0; q < MPL; ++q) {
+            const int m = v + NV * q;
+            if (m >= NP) break;
+            R U4[NV], X4[NV], Y4[NV];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                U4[w] = sU[sp(el, m, w)];
+                X4[w] = sQx[sp(el, m, w)];
+                Y4[w] = sQy[sp(el, m, w)];
+            }
+            const R inv = fast_rcp(U4[0]);
+            const R ux = U4[1] * inv, uy = U4[2] * inv;
+            const R p = (g - R(1)) * (U4[3] - R(0.5) * (U4[1] * ux + U4[2] * uy));
+            R Fv[NV], Gv[NV];
+            visc_flux(U4, X4, Y4, g, mu, kappa, Fv, Gv);
+            const R fx[NV] = {U4[1], U4[1] * ux + p, U4[1] * uy, (U4[3] + p) * ux};
+            const R fy[NV] = {U4[2], U4[2] * ux, U4[2] * uy + p, (U4[3] + p) * uy};
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R Fx = fx[w] - Fv[w], Fy = fy[w] - Gv[w];
+                sFR[sp(el, m, w)] = rx * Fx + ry * Fy;
+                sFS[sp(el, m, w)] = sx * Fx + sy * Fy;
+            }
+        }
+    }
+    __syncthreads();
+    R div[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) div[i] = R(0);
+    if (live) {
+        R fn[NFP];
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) fn[j] = R(0);
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            const R fr = sFR[sp(el, m, v)], fs = sFS[sp(el, m, v)], frs = fr + fs;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                div[i] = fmaR(cop<R>(O0 + m * CA + NP + i), fs, fmaR(cop<R>(O0 + m * CA + i), fr, div[i]));
+#pragma unroll
+            for (int j = 0; j < NFP; ++j)
+                fn[j] = fmaR(cop<R>(O0 + m * CA + 2 * NP + j), j < FP ? fs : (j < 2 * FP ? frs : fr), fn[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) sD[sl(el, j, v)] = (j >= FP && j < 2 * FP) ? fn[j] : -fn[j];
+    }
+    __syncthreads();
+    int nsolved = 0;
+#pragma unroll
+    for (int w = 0; w < NWV; ++w) nsolved += sCnt[w];
+
+    // the values of one solved face point (slot j of element elt, edge kk):
+    // both sides' traces, edge factors, the face liftings of the jumps and
+    // the averaged state and BR2 gradient
+    struct Face {
+        int elt, kk, j, ek, oj, nl;
+        bool in, right;
+        int2 Lk;
+        R nx, ny, so, sn;
+        R own[NV], oth[NV], Uc[NV], Qbx[NV], Qby[NV];
+    };
+    auto face_at = [&](int t, Face& F) __attribute__((always_inline)) {
+        const int c = sList[t / FP], f = t - (t / FP) * FP;
+        F.elt = c / 3;
+        F.kk = c - F.elt * 3;
+        F.j = F.kk * FP + f;
+        F.Lk = sLink[F.elt * 3 + F.kk];
+        F.right = link_role(F.Lk.y) == ROLE_RIGHT;
+        F.nx = sNrm[(F.elt * 3 + F.kk) * 2];
+        F.ny = sNrm[(F.elt * 3 + F.kk) * 2 + 1];
+        F.ek = link_edge(F.Lk.y);
+        F.oj = F.ek * FP + (P - f);
+        F.nl = F.Lk.x - e0;
+        F.in = F.nl >= 0 && F.nl < TT;
+        const R* Tn = a.T_in + (int64_t)F.Lk.x * NFP * TWV;
+        R gxo[NV], gyo[NV], gxn[NV], gyn[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            F.own[w] = sTu[sl(F.elt, F.j, w)];
+            gxo[w] = sTx[sl(F.elt, F.j, w)];
+            gyo[w] = sTy[sl(F.elt, F.j, w)];
+        }
+        if (F.in) {
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                F.oth[w] = sTu[sl(F.nl, F.oj, w)];
+                gxn[w] = sTx[sl(F.nl, F.oj, w)];
+                gyn[w] = sTy[sl(F.nl, F.oj, w)];
+            }
+            F.sn = sEdge[F.nl * 3 + F.ek];
+        } else {
+            load4(Tn + (int64_t)F.oj * TWV, F.oth);
+            load4(Tn + (int64_t)F.oj * TWV + NV, gxn);
+            load4(Tn + (int64_t)F.oj * TWV + 2 * NV, gyn);
+            const R* Gn = a.geo + (int64_t)F.Lk.x * GEO;
+            const R ax = F.ek == 0 ? Gn[2] : (F.ek == 1 ? Gn[0] + Gn[2] : Gn[0]);
+            const R ay = F.ek == 0 ? Gn[3] : (F.ek == 1 ? Gn[1] + Gn[3] : Gn[1]);
+            F.sn = sqrtR(ax * ax + ay * ay);
+        }
+        F.so = sEdge[F.elt * 3 + F.kk];
+        R lo[NV] = {R(0), R(0), R(0), R(0)}, ln[NV] = {R(0), R(0), R(0), R(0)};
+#pragma unroll 1
+        for (int f2 = 0; f2 < FP; ++f2) {
+            const int j2 = F.kk * FP + f2, o2 = F.ek * FP + (P - f2);
+            R uo[NV], un[NV];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) uo[w] = sTu[sl(F.elt, j2, w)];
+            if (F.in) {
+#pragma unroll
+                for (int w = 0; w < NV; ++w) un[w] = sTu[sl(F.nl, o2, w)];
+            } else {
+                load4(Tn + (int64_t)o2 * TWV, un);
+            }
+            const R wo = clf<R>(LFO + (F.kk * FP + f) * FP + f2);
+            const R wn = clf<R>(LFO + (F.ek * FP + (P - f)) * FP + (P - f2));
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                const R jo = R(0.5) * (uo[w] + un[w]) - uo[w];
+                lo[w] = fmaR(wo, jo, lo[w]);
+                ln[w] = fmaR(wn, jo, ln[w]);
+            }
+        }
+        const R sg = F.right ? R(-1) : R(1);
+        const R nox = sg * F.nx, noy = sg * F.ny;
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            F.Uc[w] = R(0.5) * (F.own[w] + F.oth[w]);
+            F.Qbx[w] = R(0.5) * ((gxo[w] + eta * F.so * nox * lo[w]) + (gxn[w] + eta * F.sn * nox * ln[w]));
+            F.Qby[w] = R(0.5) * ((gyo[w] + eta * F.so * noy * lo[w]) + (gyn[w] + eta * F.sn * noy * ln[w]));
+        }
+    };
+
+    // ---- forward 3: flux differences of both sides of every solved face point
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += NB) {
+        Face F;
+        face_at(t, F);
+        R UL[NV], UR[NV], Fc[NV], Fv[NV], Gv[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            UL[w] = F.right ? F.oth[w] : F.own[w];
+            UR[w] = F.right ? F.own[w] : F.oth[w];
+        }
+        visc_flux(F.Uc, F.Qbx, F.Qby, g, mu, kappa, Fv, Gv);
+        riemann_by_kind(a.riemann, UL, UR, F.nx, F.ny, g, Fc);
+        const R sc = F.right ? -F.so : F.so;
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            const R fi = Fc[w] - (Fv[w] * F.nx + Gv[w] * F.ny);
+            sD[sl(F.elt, F.j, w)] = sc * fi - sD[sl(F.elt, F.j, w)];
+            if (F.in) sD[sl(F.nl, F.oj, w)] = -F.sn * fi - sD[sl(F.nl, F.oj, w)];
+        }
+    }
+    __syncthreads();
+
+    // ---- forward 4 and R1: residual; cotangent of u_out, RK combination
+    double rsum = 0.0;
+    R divb[NP];
+    double qb[NP];
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            const R d = sD[sl(el, j, v)];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) div[i] = fmaR(cop<R>(OLT + j * NP + i), d, div[i]);
+        }
+        double unb[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) unb[i] = a.Ub_in[base + (int64_t)i * EB * NV];
+        if (MODE != MODE_RESIDUAL && a.Tb_in) {
+            // edge cotangents of u_out's traces (U, gradient) left on this element's cross-tile edges
+            R tu[NFP], tx[NFP], ty[NFP];
+            bool any = false;
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const int nl = L[kk].x - e0;
+                const bool cross = !(nl >= 0 && nl < TT);
+                any = any || cross;
+#pragma unroll
+                for (int f = 0; f < FP; ++f) {
+                    const int j = kk * FP + f;
+                    const R* Tb = a.Tb_in + ((int64_t)e * NFP + j) * TWV;
+                    tu[j] = cross ? Tb[v] : R(0);
+                    tx[j] = cross ? Tb[NV + v] : R(0);
+                    ty[j] = cross ? Tb[2 * NV + v] : R(0);
+                }
+            }
+            if (any) {
+                R grb[NP], gsb[NP];
+#pragma unroll
+                for (int m = 0; m < NP; ++m) {
+                    R su = R(0), qx = R(0), qy = R(0);
+#pragma unroll
+                    for (int j = 0; j < NFP; ++j) {
+                        const R w = cop<R>(OI + j * NP + m);
+                        su = fmaR(w, tu[j], su);
+                        qx = fmaR(w, tx[j], qx);
+                        qy = fmaR(w, ty[j], qy);
+                    }
+                    unb[m] += double(su);
+                    grb[m] = rx * qx + ry * qy;
+                    gsb[m] = sx * qx + sy * qy;
+                }
+#pragma unroll
+                for (int m = 0; m < NP; ++m) {
+                    R acc = R(0);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i)
+                        acc = fmaR(cop<R>(O0 + m * CA + NP + i), gsb[i], fmaR(cop<R>(O0 + m * CA + i), grb[i], acc));
+                    unb[m] += double(acc);
+                }
+            }
+        }
+        const double dt = MODE == MODE_RESIDUAL ? 0.0 : *a.dt_ptr;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int64_t gi = base + (int64_t)i * EB * NV;
+            double rb;
+            if (MODE == MODE_RESIDUAL) {
+                rb = unb[i];
+                qb[i] = 0.0;
+            } else {
+                if (MODE == MODE_STAGE3) a.U0b[gi] = ca * unb[i];
+                if (MODE == MODE_STAGE2) a.U0b[gi] += ca * unb[i];
+                qb[i] = cb * unb[i];
+                rb = cc * dt * unb[i];
+                rsum += cc * (-double(div[i])) * unb[i];
+            }
+            divb[i] = R(-rb);  // R = -div
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) { divb[i] = R(0); qb[i] = 0.0; }
+    }
+    if (a.dtb_partial) {
+        const double s2 = block_reduce_n<OpSum, NWV>(rsum, sred);
+        if (tid == 0) a.dtb_partial[tile] = s2;
+    }
+    __syncthreads();
+
+    // ---- R2, R4: the lifting transposed; the divergence and normal fluxes
+    // transposed to the x / y flux cotangents at the points
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            R acc = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) acc = fmaR(cop<R>(OLT + j * NP + i), divb[i], acc);
+            sD[sl(el, j, v)] = acc;
+        }
+#pragma unroll 1
+        for (int m = 0; m < NP; ++m) {
+            R frb = R(0), fsb = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                frb = fmaR(cop<R>(O0 + m * CA + i), divb[i], frb);
+                fsb = fmaR(cop<R>(O0 + m * CA + NP + i), divb[i], fsb);
+            }
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) {
+                // d = sc F^I - Fn with Fn = -fn (edges 0, 2), +fn (edge 1); fn_j
+                // interpolates fs, fr + fs, fr on edges 0, 1, 2
+                const R im = cop<R>(O0 + m * CA + 2 * NP + j);
+                const R db = sD[sl(el, j, v)];
+                const R fnb = (j >= FP && j < 2 * FP) ? -db : db;
+                if (j >= FP) frb = fmaR(im, fnb, frb);
+                if (j < 2 * FP) fsb = fmaR(im, fnb, fsb);
+            }
+            sFR[sp(el, m, v)] = rx * frb + sx * fsb;  // cotangent of F_x = f_x - F_v
+            sFS[sp(el, m, v)] = ry * frb + sy * fsb;  // and of F_y
+        }
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll 1
+        for (int q = 0; q < MPL; ++q) {
+            const int m = v + NV * q;
+            if (m >= NP) break;
+            R U4[NV], X4[NV], Y4[NV], A4[NV], B4[NV], Fb[NV], Gb[NV];
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                U4[w] = sU[sp(el, m, w)];
+                X4[w] = sQx[sp(el, m, w)];
+                Y4[w] = sQy[sp(el, m, w)];
+                A4[w] = sFR[sp(el, m, w)];
+                B4[w] = sFS[sp(el, m, w)];
+                Fb[w] = -A4[w];
+                Gb[w] = -B4[w];
+            }
+            R Ub4[NV] = {R(0), R(0), R(0), R(0)}, Xb4[NV] = {R(0), R(0), R(0), R(0)},
+              Yb4[NV] = {R(0), R(0), R(0), R(0)};
+            euler_flux_vjp(U4, g, A4, B4, Ub4);
+            visc_flux_vjp(U4, X4, Y4, g, mu, kappa, Fb, Gb, Ub4, Xb4, Yb4);
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                sU[sp(el, m, w)] = Ub4[w];
+                sQx[sp(el, m, w)] = Xb4[w];
+                sQy[sp(el, m, w)] = Yb4[w];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- R5: the corrected gradient's lifted jumps transposed
+    if (live) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+            R bx = R(0), by = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const R l = cop<R>(OLT + j * NP + i);
+                bx = fmaR(l, sQx[sp(el, i, v)], bx);
+                by = fmaR(l, sQy[sp(el, i, v)], by);
+            }
+            const int kk = j / FP;
+            sJ[sl(el, j, v)] = (kk == 0 ? jwx[0] : (kk == 1 ? jwx[1] : jwx[2])) * bx +
+                               (kk == 0 ? jwy[0] : (kk == 1 ? jwy[1] : jwy[2])) * by;
+        }
+    }
+    __syncthreads();
+
+    // ---- R3: every solved face point backwards
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += NB) {
+        Face F;
+        face_at(t, F);
+        R UL[NV], UR[NV], Fcb[NV], Fvb[NV], Gvb[NV];
+        const R sc = F.right ? -F.so : F.so;
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            UL[w] = F.right ? F.oth[w] : F.own[w];
+            UR[w] = F.right ? F.own[w] : F.oth[w];
+            R fb = sc * sD[sl(F.elt, F.j, w)];
+            if (F.in) fb += -F.sn * sD[sl(F.nl, F.oj, w)];
+            Fcb[w] = fb;
+            Fvb[w] = -fb * F.nx;
+            Gvb[w] = -fb * F.ny;
+        }
+        R ULb[NV] = {R(0), R(0), R(0), R(0)}, URb[NV] = {R(0), R(0), R(0), R(0)};
+        riemann_by_kind_vjp(a.riemann, UL, UR, F.nx, F.ny, g, Fcb, ULb, URb);
+        R Ucb[NV] = {R(0), R(0), R(0), R(0)}, Qxb[NV] = {R(0), R(0), R(0), R(0)}, Qyb[NV] = {R(0), R(0), R(0), R(0)};
+        visc_flux_vjp(F.Uc, F.Qbx, F.Qby, g, mu, kappa, Fvb, Gvb, Ucb, Qxb, Qyb);
+        const R sg = F.right ? R(-1) : R(1);
+        const R nox = sg * F.nx, noy = sg * F.ny;
+        R ownb[NV], othb[NV];
+#pragma unroll
+        for (int w = 0; w < NV; ++w) {
+            const R jo = sJ[sl(F.elt, F.j, w)], jn = F.in ? sJ[sl(F.nl, F.oj, w)] : R(0);
+            ownb[w] = (F.right ? URb[w] : ULb[w]) + R(0.5) * Ucb[w] - R(0.5) * jo + R(0.5) * jn;
+            othb[w] = (F.right ? ULb[w] : URb[w]) + R(0.5) * Ucb[w] + R(0.5) * jo - R(0.5) * jn;
+            Qxb[w] *= R(0.5);
+            Qyb[w] *= R(0.5);
+            sC[sl(F.elt, F.j, w)] = eta * (nox * Qxb[w] + noy * Qyb[w]);
+            sD[sl(F.elt, F.j, w)] = ownb[w];
+            sTx[sl(F.elt, F.j, w)] = Qxb[w];
+            sTy[sl(F.elt, F.j, w)] = Qyb[w];
+        }
+        if (F.in) {
+#pragma unroll
+            for (int w = 0; w < NV; ++w) {
+                sD[sl(F.nl, F.oj, w)] = othb[w];
+                sTx[sl(F.nl, F.oj, w)] = Qxb[w];
+                sTy[sl(F.nl, F.oj, w)] = Qyb[w];
+            }
+        } else {
+            R* Tb = a.Tb_out + ((int64_t)F.Lk.x * NFP + F.oj) * TWV;
+            store4(Tb, othb);
+            store4(Tb + NV, Qxb);
+            store4(Tb + 2 * NV, Qyb);
+        }
+    }
+    __syncthreads();
+    // the face liftings transposed: the jump at this point from every point's
+    // lifting cotangents (the seeds c, times each side's edge factor)
+#pragma unroll 1
+    for (int t = tid; t < nsolved * FP; t += NB) {
+        const int c = sList[t / FP], f = t - (t / FP) * FP;
+        const int elt = c / 3, kk = c - elt * 3, j = kk * FP + f;
+        const int2 Lk = sLink[elt * 3 + kk];
+        const int ek = link_edge(Lk.y), oj = ek * FP + (P - f), nl = Lk.x - e0;
+        const bool in = nl >= 0 && nl < TT;
+        const R so = sEdge[elt * 3 + kk];
+        R sn;
+        if (in) {
+            sn = sEdge[nl * 3 + ek];
+        } else {
+            const R* Gn = a.geo + (int64_t)Lk.x * GEO;
+            const R ax = ek == 0 ? Gn[2] : (ek == 1 ? Gn[0] + Gn[2] : Gn[0]);
+            const R ay = ek == 0 ? Gn[3] : (ek == 1 ? Gn[1] + Gn[3] : Gn[1]);
+            sn = sqrtR(ax * ax + ay * ay);
+        }
+        R jb[NV] = {R(0), R(0), R(0), R(0)};
+#pragma unroll 1
+        for (int f1 = 0; f1 < FP; ++f1) {
+            const R w1 = so * clf<R>(LFO + (kk * FP + f1) * FP + f) +
+                         sn * clf<R>(LFO + (ek * FP + (P - f1)) * FP + (P - f));
+#pragma unroll
+            for (int w = 0; w < NV; ++w) jb[w] = fmaR(w1, sC[sl(elt, kk * FP + f1, w)], jb[w]);
+        }
+        // jump = (u_n - u_o) / 2 at this point
+#pragma unroll
+        for (int w = 0; w < NV; ++w) sD[sl(elt, j, w)] -= R(0.5) * jb[w];
+        if (in) {
+#pragma unroll
+            for (int w = 0; w < NV; ++w) sD[sl(nl, oj, w)] += R(0.5) * jb[w];
+        } else {
+            R* Tb = a.Tb_out + ((int64_t)Lk.x * NFP + oj) * TWV;
+#pragma unroll
+            for (int w = 0; w < NV; ++w) Tb[w] += R(0.5) * jb[w];
+        }
+    }
+    __syncthreads();
+
+    // ---- R6: traces and gradient transposed; cotangent of q
+    if (!live) return;
+    R grb[NP], gsb[NP];
+    double out[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        R su = R(0), qx = sQx[sp(el, m, v)], qy = sQy[sp(el, m, v)];
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) {
+            const R w = cop<R>(OI + j * NP + m);
+            su = fmaR(w, sD[sl(el, j, v)], su);
+            qx = fmaR(w, sTx[sl(el, j, v)], qx);
+            qy = fmaR(w, sTy[sl(el, j, v)], qy);
+        }
+        out[m] = qb[m] + double(sU[sp(el, m, v)]) + double(su);
+        grb[m] = rx * qx + ry * qy;
+        gsb[m] = sx * qx + sy * qy;
+    }
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        R acc = R(0);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            acc = fmaR(cop<R>(O0 + m * CA + NP + i), gsb[i], fmaR(cop<R>(O0 + m * CA + i), grb[i], acc));
+        const int64_t gi = base + (int64_t)m * EB * NV;
+        double o = out[m] + double(acc);
+        if (MODE == MODE_STAGE1) o += a.U0b[gi];
+        a.Ub_out[gi] = o;
+    }
+}
+
+// Completion of a cotangent with the Navier-Stokes terms: U_bar plus the
+// transposed traces of U and of its gradient for the cross-tile edge
+// cotangents (12 words per point) that the last adjoint launch left.
+template <typename R, int P, int TT>
+__global__ __launch_bounds__(TT * NV) void tri_visc_complete_kernel(int E, const int2* __restrict__ links,
+                                                                    const R* __restrict__ geo,
+                                                                    const double* __restrict__ Ub,
+                                                                    const R* __restrict__ Tb,
+                                                                    double* __restrict__ out, int xcd_groups) {
+    using S = TriShape<P>;
+    constexpr int NP = S::NP, FP = S::FP, NFP = S::NFP;
+    constexpr int O0 = ops_offset(P), CA = TriOps<P>::CA, OI = O0 + TriOps<P>::C;
+    const int tid = threadIdx.x, el = tid >> 2, v = tid & 3;
+    const int e0 = tile_of_block(xcd_groups) * TT, e = e0 + el;
+    if (e >= E) return;
+    const int64_t base = blk(e, 0, NP) + v;
+    double acc[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) acc[i] = Ub[base + (int64_t)i * EB * NV];
+    if (Tb) {
+        R tu[NFP], tx[NFP], ty[NFP];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const int2 Lk = links[(int64_t)e * 3 + kk];
+            const int nl = Lk.x - e0;
+            const bool cross = !(nl >= 0 && nl < TT);
+#pragma unroll
+            for (int f = 0; f < FP; ++f) {
+                const int j = kk * FP + f;
+                const R* T = Tb + ((int64_t)e * NFP + j) * TWV;
+                tu[j] = cross ? T[v] : R(0);
+                tx[j] = cross ? T[NV + v] : R(0);
+                ty[j] = cross ? T[2 * NV + v] : R(0);
+            }
+        }
+        const R* G = geo + (int64_t)e * GEO;
+        const R rx = G[0], ry = G[1], sx = G[2], sy = G[3];
+        R grb[NP], gsb[NP];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            R su = R(0), qx = R(0), qy = R(0);
+#pragma unroll
+            for (int j = 0; j < NFP; ++j) {
+                const R w = cop<R>(OI + j * NP + m);
+                su = fmaR(w, tu[j], su);
+                qx = fmaR(w, tx[j], qx);
+                qy = fmaR(w, ty[j], qy);
+            }
+            acc[m] += double(su);
+            grb[m] = rx * qx + ry * qy;
+            gsb[m] = sx * qx + sy * qy;
+        }
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            R s = R(0);
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                s = fmaR(cop<R>(O0 + m * CA + NP + i), gsb[i], fmaR(cop<R>(O0 + m * CA + i), grb[i], s));
+            acc[m] += double(s);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) out[base + (int64_t)i * EB * NV] = acc[i];
+}
+
 // Traces of U on every edge (after a new state is set).
 template <typename R, int P>
 __global__ __launch_bounds__(BLOCK) void tri_trace_kernel(int E, const double* U, R* T_out) {
@@ -1434,6 +2135,74 @@ hipError_t tri_complete(int P, int E, const int2* links, const double* Ub, const
     return hipGetLastError();
 }
 
+// The BR2 adjoint's elements per workgroup for an order and precision; 0
+// where its LDS arrays exceed 60 KB at 16 elements (FP64 P4, P5).
+int tri_visc_vjp_tile(int P, bool fp64) {
+    switch (P) {
+#define ZJ_TV(PP)                                                                                           \
+    case PP:                                                                                                \
+        return fp64 ? (TriViscVjpTile<double, PP>::fits ? TriViscVjpTile<double, PP>::value : 0)            \
+                    : (TriViscVjpTile<float, PP>::fits ? TriViscVjpTile<float, PP>::value : 0);
+        ZJ_TV(1) ZJ_TV(2) ZJ_TV(3) ZJ_TV(4) ZJ_TV(5)
+#undef ZJ_TV
+    default: return 0;
+    }
+}
+
+template <typename R, int P>
+hipError_t tri_visc_vjp_p(int mode, const TriVjpArgs<R>& a, hipStream_t s) {
+    if constexpr (!TriViscVjpTile<R, P>::fits) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr int TT = TriViscVjpTile<R, P>::value;
+        const int grid = (std::max(a.E, 1) + TT - 1) / TT;
+        const dim3 blk(TT * NV);
+        switch (mode) {
+        case MODE_RESIDUAL: klaunch(tri_visc_vjp_kernel<R, P, MODE_RESIDUAL, TT>, grid, blk, 0, s, a); break;
+        case MODE_STAGE1: klaunch(tri_visc_vjp_kernel<R, P, MODE_STAGE1, TT>, grid, blk, 0, s, a); break;
+        case MODE_STAGE2: klaunch(tri_visc_vjp_kernel<R, P, MODE_STAGE2, TT>, grid, blk, 0, s, a); break;
+        default: klaunch(tri_visc_vjp_kernel<R, P, MODE_STAGE3, TT>, grid, blk, 0, s, a); break;
+        }
+        return hipGetLastError();
+    }
+}
+
+template <typename R>
+hipError_t tri_visc_vjp(int P, int mode, const TriVjpArgs<R>& a, hipStream_t s) {
+    switch (P) {
+    case 1: return tri_visc_vjp_p<R, 1>(mode, a, s);
+    case 2: return tri_visc_vjp_p<R, 2>(mode, a, s);
+    case 3: return tri_visc_vjp_p<R, 3>(mode, a, s);
+    case 4: return tri_visc_vjp_p<R, 4>(mode, a, s);
+    default: return tri_visc_vjp_p<R, 5>(mode, a, s);
+    }
+}
+
+template <typename R, int P>
+hipError_t tri_visc_complete_p(int E, const int2* links, const R* geo, const double* Ub, const R* Tb, double* out,
+                               int xcd, hipStream_t s) {
+    if constexpr (!TriViscVjpTile<R, P>::fits) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr int TT = TriViscVjpTile<R, P>::value;
+        const int grid = (std::max(E, 1) + TT - 1) / TT;
+        tri_visc_complete_kernel<R, P, TT><<<grid, TT * NV, 0, s>>>(E, links, geo, Ub, Tb, out, xcd);
+        return hipGetLastError();
+    }
+}
+
+template <typename R>
+hipError_t tri_visc_complete(int P, int E, const int2* links, const R* geo, const double* Ub, const R* Tb, double* out,
+                             int xcd, hipStream_t s) {
+    switch (P) {
+    case 1: return tri_visc_complete_p<R, 1>(E, links, geo, Ub, Tb, out, xcd, s);
+    case 2: return tri_visc_complete_p<R, 2>(E, links, geo, Ub, Tb, out, xcd, s);
+    case 3: return tri_visc_complete_p<R, 3>(E, links, geo, Ub, Tb, out, xcd, s);
+    case 4: return tri_visc_complete_p<R, 4>(E, links, geo, Ub, Tb, out, xcd, s);
+    default: return tri_visc_complete_p<R, 5>(E, links, geo, Ub, Tb, out, xcd, s);
+    }
+}
+
 template <typename R>
 hipError_t tri_trace(int P, int E, const double* U, R* T, hipStream_t s) {
     const int grid = (std::max(E, 1) + TILE - 1) / TILE;
@@ -1566,6 +2335,7 @@ struct zj_tri {
     void* A_T[3] = {nullptr, nullptr, nullptr};
     void* A_Tb[2] = {nullptr, nullptr};
     int A_cur = 0;
+    int A_blk = 0;   // the adjoint kernels' tiles (the BR2 adjoint's tile may be smaller than the stage kernel's)
     bool A_has_tb = false;
     bool A_seeded = false;
     // Navier-Stokes terms (BR2): mu = 1/Re, kappa, penalty; traces of 12 words
@@ -2035,16 +2805,23 @@ extern "C" int zj_tri_get_info(zj_tri_handle h, int64_t* n_dof, int* elems_per_b
 
 // ------------------------------------------------------------ adjoint (host)
 static int tensure_adjoint(zj_tri* h) {
-    if (h->viscous) return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint covers the Euler terms");
     if (h->A_Ub) return ZJ_OK;
+    int tt = TILE;
+    if (h->viscous) {
+        tt = tri_visc_vjp_tile(h->P, h->prec == ZJ_PREC_FP64);
+        if (tt == 0)
+            return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint with the viscous terms covers FP64 P1-P3 "
+                                                "and FP32-bulk P1-P5 (its tile's LDS arrays)");
+    }
     ZJ_THIP(h, hipSetDevice(h->device));
+    h->A_blk = (h->E + tt - 1) / tt;
     const size_t eb = ((size_t)h->E + EB - 1) / EB * EB;
     const size_t nst = eb * h->NP * 4 * sizeof(double);
-    const size_t ntr = eb * h->NFP * 4 * (h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float));
+    const size_t ntr = eb * h->NFP * (h->viscous ? TWV : 4) * (h->prec == ZJ_PREC_FP64 ? sizeof(double) : sizeof(float));
     for (double** b : {&h->A_Ub, &h->A_U0b, &h->A_W1, &h->A_W2}) ZJ_THIP(h, hipMalloc((void**)b, nst));
     for (int k = 0; k < 3; ++k) ZJ_THIP(h, hipMalloc(&h->A_T[k], ntr));
     for (int k = 0; k < 2; ++k) ZJ_THIP(h, hipMalloc(&h->A_Tb[k], ntr));
-    ZJ_THIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * std::max(h->nblk, 1) * sizeof(double)));
+    ZJ_THIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * std::max(h->A_blk, 1) * sizeof(double)));
     return ZJ_OK;
 }
 
@@ -2060,6 +2837,10 @@ static TriVjpArgs<R> tvjp_args(zj_tri* h) {
     a.riemann = h->riemann;
     a.xcd_groups = h->xcd_groups;
     a.lift_off = lift_offset(h->P, h->corr);
+    a.mu = h->mu;
+    a.kappa = h->kappa;
+    a.eta = h->eta;
+    a.lf_off = lf_family(h->corr) * lf_words(h->P);
     return a;
 }
 
@@ -2070,14 +2851,22 @@ template <typename R>
 static hipError_t tri_adjoint_step(zj_tri* h, bool want_dt, hipStream_t s) {
     R** AT = (R**)h->A_T;
     R** Tb = (R**)h->A_Tb;
-    hipError_t e = tri_trace<R>(h->P, h->E, h->U, AT[0], s);
+    const R* geo = (const R*)h->geo;
+    // With the viscous terms every edge's traces are formed after each
+    // recomputed stage: the adjoint's tiles can be smaller than the stage
+    // kernel's, whose own trace writes cover only the edges across its tiles.
+    hipError_t e = h->viscous ? tri_visc_trace<R>(h->P, h->E, h->U, geo, AT[0], s) : tri_trace<R>(h->P, h->E, h->U, AT[0], s);
     if (e != hipSuccess) return e;
     TriArgs<R> f = tri_args<R>(h, MODE_STAGE1);
     f.U_in = h->U; f.U_out = h->A_W1; f.T_in = AT[0]; f.T_out = AT[1]; f.l2_partial = nullptr;
-    if ((e = tri_launch<R>(h->P, MODE_STAGE1, f, s)) != hipSuccess) return e;
+    if ((e = h->viscous ? tri_visc_launch<R>(h->P, MODE_STAGE1, f, s) : tri_launch<R>(h->P, MODE_STAGE1, f, s)) != hipSuccess)
+        return e;
+    if (h->viscous && (e = tri_visc_trace<R>(h->P, h->E, h->A_W1, geo, AT[1], s)) != hipSuccess) return e;
     f = tri_args<R>(h, MODE_STAGE2);
     f.U0 = h->U; f.U_in = h->A_W1; f.U_out = h->A_W2; f.T_in = AT[1]; f.T_out = AT[2]; f.l2_partial = nullptr;
-    if ((e = tri_launch<R>(h->P, MODE_STAGE2, f, s)) != hipSuccess) return e;
+    if ((e = h->viscous ? tri_visc_launch<R>(h->P, MODE_STAGE2, f, s) : tri_launch<R>(h->P, MODE_STAGE2, f, s)) != hipSuccess)
+        return e;
+    if (h->viscous && (e = tri_visc_trace<R>(h->P, h->E, h->A_W2, geo, AT[2], s)) != hipSuccess) return e;
     for (int st = 3; st >= 1; --st) {
         TriVjpArgs<R> a = tvjp_args<R>(h);
         a.U_in = st == 3 ? h->A_W2 : (st == 2 ? h->A_W1 : h->U);
@@ -2088,8 +2877,8 @@ static hipError_t tri_adjoint_step(zj_tri* h, bool want_dt, hipStream_t s) {
         a.Tb_in = h->A_has_tb ? Tb[h->A_cur] : nullptr;
         a.Tb_out = Tb[1 - h->A_cur];
         a.U0b = h->A_U0b;
-        a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->nblk : nullptr;
-        if ((e = tri_vjp<R>(h->P, st, a, s)) != hipSuccess) return e;
+        a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->A_blk : nullptr;
+        if ((e = h->viscous ? tri_visc_vjp<R>(h->P, st, a, s) : tri_vjp<R>(h->P, st, a, s)) != hipSuccess) return e;
         h->A_cur = 1 - h->A_cur;
         h->A_has_tb = true;
     }
@@ -2124,11 +2913,17 @@ extern "C" int zj_tri_adjoint_get_device(zj_tri_handle h, double* d_ubar_aos, vo
     ZJ_THIP(h, hipSetDevice(h->device));
     hipStream_t s = tpick(h, stream);
     const void* tb = h->A_has_tb ? h->A_Tb[h->A_cur] : nullptr;
-    hipError_t e = h->prec == ZJ_PREC_FP64
-                       ? tri_complete<double>(h->P, h->E, h->links, h->A_Ub, (const double*)tb, h->Rres,
-                                              h->xcd_groups, s)
-                       : tri_complete<float>(h->P, h->E, h->links, h->A_Ub, (const float*)tb, h->Rres,
-                                             h->xcd_groups, s);
+    hipError_t e;
+    if (h->viscous)
+        e = h->prec == ZJ_PREC_FP64
+                ? tri_visc_complete<double>(h->P, h->E, h->links, (const double*)h->geo, h->A_Ub, (const double*)tb,
+                                            h->Rres, h->xcd_groups, s)
+                : tri_visc_complete<float>(h->P, h->E, h->links, (const float*)h->geo, h->A_Ub, (const float*)tb,
+                                           h->Rres, h->xcd_groups, s);
+    else
+        e = h->prec == ZJ_PREC_FP64
+                ? tri_complete<double>(h->P, h->E, h->links, h->A_Ub, (const double*)tb, h->Rres, h->xcd_groups, s)
+                : tri_complete<float>(h->P, h->E, h->links, h->A_Ub, (const float*)tb, h->Rres, h->xcd_groups, s);
     ZJ_THIP(h, e);
     ZJ_THIP(h, tri_convert(false, h->Rres, d_ubar_aos, h->E, h->NP, h->perm, s));
     return ZJ_OK;
@@ -2148,7 +2943,6 @@ extern "C" int zj_tri_rk3_step_vjp(zj_tri_handle h, double dt, double* dt_bar, v
     if (!h) return ZJ_ERR_ARG;
     if (!h->has_state) return tfail(h, ZJ_ERR_STATE, "no state set");
     if (!h->A_seeded) return tfail(h, ZJ_ERR_STATE, "no adjoint seed set");
-    if (h->viscous) return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint covers the Euler terms");
     ZJ_THIP(h, hipSetDevice(h->device));
     hipStream_t s = tpick(h, stream);
     if (dt != h->dt_on_device) {
@@ -2158,7 +2952,7 @@ extern "C" int zj_tri_rk3_step_vjp(zj_tri_handle h, double dt, double* dt_bar, v
     const bool want = dt_bar != nullptr;
     ZJ_THIP(h, h->prec == ZJ_PREC_FP64 ? tri_adjoint_step<double>(h, want, s) : tri_adjoint_step<float>(h, want, s));
     if (want) {
-        ZJ_THIP(h, launch_finalize(h->A_dtb, 3 * h->nblk, FIN_SUM, h->d_scalars + 3, s));
+        ZJ_THIP(h, launch_finalize(h->A_dtb, 3 * h->A_blk, FIN_SUM, h->d_scalars + 3, s));
         ZJ_THIP(h, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
         ZJ_THIP(h, hipStreamSynchronize(s));
         *dt_bar = h->h_pinned[3];
@@ -2170,7 +2964,8 @@ template <typename R>
 static hipError_t tri_residual_vjp(zj_tri* h, hipStream_t s) {
     R** AT = (R**)h->A_T;
     R* tb = (R*)h->A_Tb[1 - h->A_cur];  // the pending edge cotangents of the accumulator stay in A_Tb[A_cur]
-    hipError_t e = tri_trace<R>(h->P, h->E, h->U, AT[0], s);
+    const R* geo = (const R*)h->geo;
+    hipError_t e = h->viscous ? tri_visc_trace<R>(h->P, h->E, h->U, geo, AT[0], s) : tri_trace<R>(h->P, h->E, h->U, AT[0], s);
     if (e != hipSuccess) return e;
     TriVjpArgs<R> a = tvjp_args<R>(h);
     a.U_in = h->U;
@@ -2178,6 +2973,10 @@ static hipError_t tri_residual_vjp(zj_tri* h, hipStream_t s) {
     a.Ub_in = h->Rres;
     a.Ub_out = h->Rres;
     a.Tb_out = tb;
+    if (h->viscous) {
+        if ((e = tri_visc_vjp<R>(h->P, MODE_RESIDUAL, a, s)) != hipSuccess) return e;
+        return tri_visc_complete<R>(h->P, h->E, h->links, geo, h->Rres, tb, h->W, h->xcd_groups, s);
+    }
     if ((e = tri_vjp<R>(h->P, MODE_RESIDUAL, a, s)) != hipSuccess) return e;
     return tri_complete<R>(h->P, h->E, h->links, h->Rres, tb, h->W, h->xcd_groups, s);
 }
