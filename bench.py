@@ -239,6 +239,29 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
     e1.record(stream)
     e1.synchronize()
     seq = e0.elapsed_ms(e1) / steps
+    # one step backwards: the batched stage VJPs vs each member's own step VJP
+    # (the cost does not depend on the cotangent's values)
+    for s, (_, U, _) in zip(sols, mem):
+        s.set_state(U)
+        s.adjoint_set(U)
+    k_adj = max(steps // 4, 3)
+    adj = {}
+    for name in ("batched", "sequential"):
+        def back():
+            if name == "batched":
+                batch.rk3_step_vjp(dts, stream.ptr)
+            else:
+                for s, dt in zip(sols, dts):
+                    s.rk3_step_vjp(dt, stream.ptr, want_dt=False)
+        back()
+        stream.synchronize()
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(stream)
+        for _ in range(k_adj):
+            back()
+        e1.record(stream)
+        e1.synchronize()
+        adj[name] = e0.elapsed_ms(e1) / k_adj
     batch.close()
     err = None
     if err_steps > 0 and precision != "fp64":
@@ -281,6 +304,10 @@ def time_batch(n_inst, seed, precision, steps, warmup, stream, hiprt, err_steps=
                               timing="graph-replayed batched step, launch gaps included (grouped: the "
                                      "groups' kernels run side by side on parallel graph branches; single: "
                                      "one launch per stage)"),
+                adjoint=dict(batched_ms_per_step=adj["batched"], sequential_ms_per_step=adj["sequential"],
+                             speedup=adj["sequential"] / adj["batched"], over_forward=adj["batched"] / res[best],
+                             note="one step backwards: each member's stages 1-2 recomputed by its own launches, "
+                                  "then one stage-VJP launch per (order, metric form, scheme) group"),
                 fp32bulk_error_vs_fp64=err)
 
 

@@ -294,6 +294,15 @@ int zj_batch_destroy(zj_batch_handle b);
 const char* zj_batch_last_error(zj_batch_handle b);
 /* dts[i] is member i's step (FRSolver::advance, fr_solver.cpp:305-333, for all members). */
 int zj_batch_rk3_step(zj_batch_handle b, const double* dts, void* stream);
+/* One SSP-RK3 step backwards for every member (no reference counterpart; the
+ * batched form of zj_rk3_step_vjp): each member must hold its state U^n and
+ * an adjoint seed (zj_adjoint_set / zj_adjoint_set_device); each member's
+ * forward stages are recomputed on its own, then every stage VJP runs as one
+ * launch per (order, metric form, scheme) group over all members' tiles.
+ * The cotangents equal the members' own zj_rk3_step_vjp bit for bit.
+ * dt_bars (may be NULL) receives dJ/d(dts[i]) per member; with it the call
+ * waits for the step. */
+int zj_batch_rk3_step_vjp(zj_batch_handle b, const double* dts, double* dt_bars, void* stream);
 int zj_batch_synchronize(zj_batch_handle b);
 /* Workgroups of each launch: 0 = one per work item (element tile);
  * k > 0 = k per CU, each looping over its share of the items (persistent). */

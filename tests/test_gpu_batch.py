@@ -319,3 +319,51 @@ def test_batch_c4_full_size(scheme):
     batch.close()
     for s in a + b:
         s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_batch_step_vjp_equals_members(precision):
+    """Two SSP-RK3 steps backwards for a batch of every scheme (Navier-Stokes
+    Re 100-5000, textbook Euler, reference arithmetic; P1-P5, both metric
+    forms), one stage-VJP launch per group: each member's cotangent, dJ/d(dt)
+    and dJ/dRe equal its own zj_rk3_step_vjp to round-off (the batched kernel
+    is its own compilation of the same stage-VJP body, so the compiler may
+    fuse multiplies and adds differently: bar 1e-13 relative in FP64, 1e-5 in
+    FP32-bulk); the pending cross-tile edge cotangents pass from one batched
+    step to the next."""
+    say(f"case: batched step VJP of {len(SCHEME_MEMBERS)} meshes vs own step VJPs, {precision}")
+    built = [_scheme_member(*m, precision, seed=110 + k) for k, m in enumerate(SCHEME_MEMBERS)]
+    a = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    b = [solver.Solver(t, precision=precision, **kw) for t, _, kw in built]
+    states = []
+    for k, (s1, s2, (t, U, _)) in enumerate(zip(a, b, built)):
+        s1.set_state(U)
+        s2.set_state(U)
+        rng = np.random.default_rng(120 + k)
+        ubar = rng.standard_normal(U.shape)
+        s1.adjoint_set(ubar)
+        s2.adjoint_set(ubar)
+        states.append((U, U * (1.0 + 1e-3 * rng.standard_normal(U.shape))))
+    dts = np.array([0.5 * s.dt_cfl(0.5) for s in a])
+    tol = 1e-13 if precision == "fp64" else 1e-5
+    batch = solver.Batch(a)
+    for step in range(2):
+        for s1, s2, st in zip(a, b, states):
+            s1.set_state(st[1 - step])
+            s2.set_state(st[1 - step])
+        dtb = batch.rk3_step_vjp(dts, want_dt=True)
+        for k, (s2, dt) in enumerate(zip(b, dts)):
+            d2 = s2.rk3_step_vjp(dt)
+            assert abs(dtb[k] - d2) <= tol * abs(d2), (SCHEME_MEMBERS[k], step, dtb[k], d2)
+    worst = 0.0
+    for k, (s1, s2) in enumerate(zip(a, b)):
+        x, y = s1.adjoint_get(), s2.adjoint_get()
+        r = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+        worst = max(worst, r)
+        say(f"  member {k} {SCHEME_MEMBERS[k][:3]}: rel L2 {r:.2e}")
+        assert r <= tol, SCHEME_MEMBERS[k]
+        if SCHEME_MEMBERS[k][2] == "ns":
+            ra, rb = s1.adjoint_get_re(), s2.adjoint_get_re()
+            assert abs(ra - rb) <= tol * abs(rb), (SCHEME_MEMBERS[k], ra, rb)
+    say(f"  worst {worst:.2e}")
+    batch.close()

@@ -78,7 +78,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_set_xcd_groups", "zj_adjoint_set", "zj_adjoint_set_device", "zj_adjoint_get",
            "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
-           "zj_batch_rk3_step", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
+           "zj_batch_rk3_step", "zj_batch_rk3_step_vjp", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
            "zj_batch_set_launch", "zj_batch_autotune", "zj_state_slots", "zj_state_save", "zj_state_load",
            "zj_gmsh_read", "zj_gmsh_error", "zj_gmsh_free", "zj_gmsh_counts", "zj_gmsh_nodes",
            "zj_gmsh_elements", "zj_gmsh_edge_tag", "zj_gmsh_group", "zj_get_tiling",
@@ -142,6 +142,7 @@ def lib() -> C.CDLL:
         L.zj_batch_last_error.argtypes = [H]
         L.zj_batch_last_error.restype = C.c_char_p
         L.zj_batch_rk3_step.argtypes = [H, _dp, C.c_void_p]
+        L.zj_batch_rk3_step_vjp.argtypes = [H, _dp, C.c_void_p, C.c_void_p]
         L.zj_batch_synchronize.argtypes = [H]
         L.zj_batch_set_grid.argtypes = [H, C.c_int]
         L.zj_batch_set_launch.argtypes = [H, C.c_int]

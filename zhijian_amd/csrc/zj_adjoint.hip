@@ -153,7 +153,7 @@ __device__ __forceinline__ void positivity_vjp(const double* q, const double* ob
 
 // ------------------------------------------------------------------ kernel
 template <typename R, int P, int EPB, int MODE, bool AFF, bool TB>
-__global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops) {
+__device__ __forceinline__ void vjp_body(const VjpArgs<R>& a, const OpTables<R>& ops, const int tile) {
     using Y = Lay<R, P>;
     constexpr bool INC = std::is_same<R, float>::value;
     constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     __shared__ double sRed[BLOCK / 64];
 
     const int tid = threadIdx.x;
-    const int e0 = tile_of_block(a.xcd_groups) * EPB;
+    const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma);
     for (int t = tid; t < N1 * N1; t += BLOCK) sD[t] = ops.D[t];
@@ -626,6 +626,21 @@ __global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const Op
     }
 }
 
+template <typename R, int P, int EPB, int MODE, bool AFF, bool TB>
+__global__ __launch_bounds__(BLOCK) void vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops) {
+    vjp_body<R, P, EPB, MODE, AFF, TB>(a, ops, tile_of_block(a.xcd_groups));
+}
+
+// The stage VJP over the tiles of several meshes of one order, metric form
+// and scheme in one launch: one work item (member, tile) per workgroup, each
+// member with its own arguments and operator tables.
+template <typename R, int P, int EPB, int MODE, bool AFF, bool TB>
+__global__ __launch_bounds__(BLOCK) void batched_vjp_kernel(const BatchVjpInst<R>* __restrict__ insts,
+                                                            const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    vjp_body<R, P, EPB, MODE, AFF, TB>(insts[it.x].a, insts[it.x].ops, it.y);
+}
+
 // Completion of a cotangent: U_bar + transposed traces of the cross-tile edge
 // cotangents that the last adjoint launch left for this tile.
 template <typename R, int P, int EPB>
@@ -723,6 +738,35 @@ hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& o
 }
 
 template <typename R>
+hipError_t launch_batched_vjp(int P, bool affine, bool textbook, int mode, const BatchVjpInst<R>* insts,
+                              const int2* items, int n_items, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+#define ZJ_BV_M(PP, AF, TBK)                                                                                    \
+    {                                                                                                           \
+        constexpr int T = adj_tile_max<PP>();                                                                   \
+        switch (mode) {                                                                                         \
+        case MODE_STAGE1: batched_vjp_kernel<R, PP, T, MODE_STAGE1, AF, TBK><<<n_items, BLOCK, 0, s>>>(insts, items); break; \
+        case MODE_STAGE2: batched_vjp_kernel<R, PP, T, MODE_STAGE2, AF, TBK><<<n_items, BLOCK, 0, s>>>(insts, items); break; \
+        case MODE_STAGE3: batched_vjp_kernel<R, PP, T, MODE_STAGE3, AF, TBK><<<n_items, BLOCK, 0, s>>>(insts, items); break; \
+        default: return hipErrorInvalidValue;                                                                   \
+        }                                                                                                       \
+        return hipGetLastError();                                                                               \
+    }
+#define ZJ_BV_P(PP)                                                                                             \
+    case PP:                                                                                                    \
+        if (textbook) {                                                                                         \
+            if (affine) ZJ_BV_M(PP, true, true) else ZJ_BV_M(PP, false, true)                                   \
+        }                                                                                                       \
+        if (affine) ZJ_BV_M(PP, true, false) else ZJ_BV_M(PP, false, false)
+    switch (P) {
+        ZJ_BV_P(1) ZJ_BV_P(2) ZJ_BV_P(3) ZJ_BV_P(4) ZJ_BV_P(5)
+    default: return hipErrorInvalidValue;
+    }
+#undef ZJ_BV_P
+#undef ZJ_BV_M
+}
+
+template <typename R>
 hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& ops, hipStream_t s) {
     if (a.E <= 0) return hipSuccess;
     switch (P) {
@@ -739,6 +783,10 @@ template hipError_t launch_vjp<double>(int, int, const VjpArgs<double>&, const O
                                       bool);
 template hipError_t launch_vjp<float>(int, int, const VjpArgs<float>&, const OpTables<float>&, bool, hipStream_t,
                                      bool);
+template hipError_t launch_batched_vjp<double>(int, bool, bool, int, const BatchVjpInst<double>*, const int2*, int,
+                                              hipStream_t);
+template hipError_t launch_batched_vjp<float>(int, bool, bool, int, const BatchVjpInst<float>*, const int2*, int,
+                                             hipStream_t);
 template hipError_t launch_complete<double>(int, const CompleteArgs<double>&, const OpTables<double>&, hipStream_t);
 template hipError_t launch_complete<float>(int, const CompleteArgs<float>&, const OpTables<float>&, hipStream_t);
 

@@ -38,6 +38,15 @@ struct VjpArgs {
     int riemann;             // 0 Rusanov, 1 Roe, 2 HLLC
 };
 
+// One mesh of a batched VJP launch: its stage-VJP arguments, operator
+// tables and viscous parameters.
+template <typename R>
+struct BatchVjpInst {
+    VjpArgs<R> a;
+    OpTables<R> ops;
+    ViscParams vp;
+};
+
 template <typename R>
 struct CompleteArgs {
     int E;
@@ -56,6 +65,11 @@ hipError_t launch_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<R>& o
                       bool textbook = false);
 template <typename R>
 hipError_t launch_complete(int P, const CompleteArgs<R>& a, const OpTables<R>& ops, hipStream_t s);
+// The same stage VJP over several meshes of one order, metric form and scheme
+// in one launch: work items (member, tile at adjoint_tile(P)), one per workgroup.
+template <typename R>
+hipError_t launch_batched_vjp(int P, bool affine, bool textbook, int mode, const BatchVjpInst<R>* insts,
+                              const int2* items, int n_items, hipStream_t s);
 
 // The BR2 (Navier-Stokes) stage's adjoint (zj_visc.hip): its own tile
 // (visc_adjoint_tile), the forward stages 1 and 2 recomputed at that tile,
@@ -70,5 +84,10 @@ hipError_t launch_visc_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<
 template <typename R>
 hipError_t launch_visc_complete(int P, const CompleteArgs<R>& a, const R* Jinv_e, const OpTables<R>& ops,
                                 hipStream_t s);
+// The BR2 stage VJP over several meshes of one order (work items at
+// visc_adjoint_tile), one per workgroup.
+template <typename R>
+hipError_t launch_batched_visc_vjp(int P, int mode, const BatchVjpInst<R>* insts, const int2* items, int n_items,
+                                   hipStream_t s);
 
 }  // namespace zj

@@ -1508,6 +1508,17 @@ struct zj_batch {
     int n_cu = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // the batched step backwards: per group, work items (position in the
+    // group, tile at the members' adjoint tile); the stage VJPs' instances of
+    // a call ([stage 3, 2, 1][member in group order]) in one device buffer,
+    // filled from pinned staging, which the next call reuses once the copy ran
+    std::vector<int2*> vitems;
+    std::vector<int> vn_items;
+    std::vector<int> voff;          // each group's first instance within a stage's block
+    std::vector<int> vorder;        // members in group order
+    void* d_vinst = nullptr;
+    void* h_vinst = nullptr;
+    hipEvent_t vcopied = nullptr;
     std::string err;
 };
 
@@ -1526,6 +1537,11 @@ static void batch_release(zj_batch* b) {
     for (void* p : b->all_insts)
         if (p) hipFree(p);
     if (b->all_items) hipFree(b->all_items);
+    for (int2* p : b->vitems)
+        if (p) hipFree(p);
+    if (b->d_vinst) hipFree(b->d_vinst);
+    if (b->h_vinst) hipHostFree(b->h_vinst);
+    if (b->vcopied) hipEventDestroy(b->vcopied);
     if (b->d_dt) hipFree(b->d_dt);
     if (b->fork) hipEventDestroy(b->fork);
     if (b->done) hipEventDestroy(b->done);
@@ -1844,6 +1860,140 @@ extern "C" int zj_batch_rk3_step(zj_batch* b, const double* dts, void* stream) {
         h->l2_blocks = h->nblk;
         h->after_batch = b->done;
     }
+    return ZJ_OK;
+}
+
+// Work items of the batched step backwards, at each member's adjoint tile,
+// and the instance buffers (once; the adjoint tile depends only on the
+// member's order and precision).
+template <typename R>
+static int batch_vjp_prepare(zj_batch* b) {
+    if (b->d_vinst) return ZJ_OK;
+    const int n = (int)b->m.size();
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        std::vector<int2> items;
+        b->voff.push_back((int)b->vorder.size());
+        const std::vector<int>& mem = b->group_members[g];
+        for (int k = 0; k < (int)mem.size(); ++k) {
+            b->vorder.push_back(mem[k]);
+            for (int t = 0; t < b->m[mem[k]]->adj_nt; ++t) items.push_back(make_int2(k, t));
+        }
+        int2* d = nullptr;
+        if (!items.empty()) {
+            ZJ_BHIP(b, hipMalloc((void**)&d, items.size() * sizeof(int2)));
+            ZJ_BHIP(b, hipMemcpy(d, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+        }
+        b->vitems.push_back(d);
+        b->vn_items.push_back((int)items.size());
+    }
+    if ((int)b->vorder.size() != n) {
+        b->err = "batch groups do not cover the members";
+        return ZJ_ERR_STATE;
+    }
+    const size_t bytes = (size_t)3 * n * sizeof(BatchVjpInst<R>);
+    ZJ_BHIP(b, hipMalloc(&b->d_vinst, bytes));
+    ZJ_BHIP(b, hipHostMalloc(&b->h_vinst, bytes, hipHostMallocDefault));
+    ZJ_BHIP(b, hipEventCreateWithFlags(&b->vcopied, hipEventDisableTiming));
+    return ZJ_OK;
+}
+
+// One batched step backwards: each member's forward stages 1 and 2 recomputed
+// at its adjoint tile (its own launches), then per stage 3, 2, 1 one VJP launch
+// per group over every member's tiles, each member's arguments as its own
+// zj_rk3_step_vjp would pass them.
+template <typename R>
+static int batch_step_vjp(zj_batch* b, const double* dts, double* dt_bars, hipStream_t s) {
+    int rc = batch_vjp_prepare<R>(b);
+    if (rc) return rc;
+    const int n = (int)b->m.size();
+    const bool want_dt = dt_bars != nullptr;
+    for (int i = 0; i < n; ++i) {
+        zj_solver* h = b->m[i];
+        const OpTables<R>& ops = *(const OpTables<R>*)(std::is_same<R, double>::value ? (const void*)&h->ops64
+                                                                                       : (const void*)&h->ops32);
+        ZJ_BHIP(b, set_dt(h, dts[i], s));
+        if ((rc = refresh_trace(h, s))) {
+            b->err = h->err;
+            return rc;
+        }
+        ZJ_BHIP(b, recompute_stages<R>(h, ops, s, handle_step(h)));
+    }
+    // the instances of the three stages; each member's pending edge
+    // cotangents alternate between its two buffers as its own calls do
+    ZJ_BHIP(b, hipEventSynchronize(b->vcopied));
+    BatchVjpInst<R>* I = (BatchVjpInst<R>*)b->h_vinst;
+    const bool inc = std::is_same<R, float>::value;
+    for (int pos = 0; pos < n; ++pos) {
+        zj_solver* h = b->m[b->vorder[pos]];
+        R** Tb = (R**)h->A_Tb;
+        const StepBufs sb = handle_step(h);
+        for (int st = 3; st >= 1; --st) {
+            BatchVjpInst<R>& x = I[(size_t)(3 - st) * n + pos];
+            VjpArgs<R> a = vjp_args<R>(h);
+            double* W = st == 3 ? sb.W2 : sb.W1;
+            if (st > 1) {
+                a.U0 = sb.U;
+                if (inc) a.dW_in = (const float*)W; else a.U_in = W;
+            } else {
+                a.U_in = sb.U;
+            }
+            a.T_in = (const R*)sb.T[st - 1];
+            a.Ub_in = h->A_Ub;
+            a.Ub_out = h->A_Ub;
+            a.Tb_in = h->A_has_tb ? Tb[h->A_cur] : nullptr;
+            a.Tb_out = Tb[1 - h->A_cur];
+            a.U0b = h->A_U0b;
+            a.dtb_partial = want_dt ? h->A_dtb + (size_t)(3 - st) * h->adj_nt : nullptr;
+            if (h->viscous) a.mub_partial = h->A_mub;
+            x.a = a;
+            x.ops = *(const OpTables<R>*)(std::is_same<R, double>::value ? (const void*)&h->ops64
+                                                                         : (const void*)&h->ops32);
+            x.vp = h->vp;
+            h->A_cur = 1 - h->A_cur;
+            h->A_has_tb = true;
+        }
+    }
+    ZJ_BHIP(b, hipMemcpyAsync(b->d_vinst, b->h_vinst, (size_t)3 * n * sizeof(BatchVjpInst<R>), hipMemcpyHostToDevice,
+                              s));
+    ZJ_BHIP(b, hipEventRecord(b->vcopied, s));
+    const BatchVjpInst<R>* D = (const BatchVjpInst<R>*)b->d_vinst;
+    for (int st = 3; st >= 1; --st) {
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            const zj_batch::Group& G = b->groups[g];
+            const BatchVjpInst<R>* insts = D + (size_t)(3 - st) * n + b->voff[g];
+            if (G.viscous)
+                ZJ_BHIP(b, launch_batched_visc_vjp<R>(G.P, st, insts, b->vitems[g], b->vn_items[g], s));
+            else
+                ZJ_BHIP(b, launch_batched_vjp<R>(G.P, G.affine, G.textbook, st, insts, b->vitems[g], b->vn_items[g], s));
+        }
+    }
+    if (want_dt) {
+        for (int i = 0; i < n; ++i) {
+            zj_solver* h = b->m[i];
+            ZJ_BHIP(b, launch_finalize(h->A_dtb, 3 * h->adj_nt, FIN_SUM, h->d_scalars + 3, s));
+            ZJ_BHIP(b, hipMemcpyAsync(h->h_pinned + 3, h->d_scalars + 3, sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        ZJ_BHIP(b, hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) dt_bars[i] = b->m[i]->h_pinned[3];
+    }
+    return ZJ_OK;
+}
+
+extern "C" int zj_batch_rk3_step_vjp(zj_batch* b, const double* dts, double* dt_bars, void* stream) {
+    if (!b || !dts) return ZJ_ERR_ARG;
+    for (zj_solver* h : b->m) {
+        if (!h->has_state || h->stage_next != 1 || !h->A_seeded) {
+            b->err = "every member needs a state, no partial step and an adjoint seed";
+            return ZJ_ERR_STATE;
+        }
+    }
+    ZJ_BHIP(b, hipSetDevice(b->device));
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    const int rc = b->prec == ZJ_PREC_FP64 ? batch_step_vjp<double>(b, dts, dt_bars, s)
+                                           : batch_step_vjp<float>(b, dts, dt_bars, s);
+    if (rc) return rc;
+    ZJ_BHIP(b, hipEventRecord(b->done, s));
+    for (zj_solver* h : b->m) h->after_batch = b->done;
     return ZJ_OK;
 }
 

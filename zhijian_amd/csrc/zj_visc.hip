@@ -587,9 +587,9 @@ __device__ __forceinline__ void visc_face_point(ViscFacePoint<R, P>& F, const R*
     (void)sU;
 }
 
-template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
-__global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops,
-                                                         const ViscParams vp) {
+template <typename R, int P, int EPB, int MODE, int NTHR>
+__device__ __forceinline__ void visc_vjp_body(const VjpArgs<R>& a, const OpTables<R>& ops, const ViscParams& vp,
+                                              const int tile) {
     using Y = Lay<R, P>;
     constexpr bool INC = std::is_same<R, float>::value;
     constexpr bool HAS_U0 = MODE == MODE_STAGE2 || MODE == MODE_STAGE3;
@@ -609,7 +609,6 @@ __global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, cons
     __shared__ double sRed[BLOCK / 64];
 
     const int tid = threadIdx.x;
-    const int tile = tile_of_block(a.xcd_groups);
     const int e0 = tile * EPB;
     const int64_t PS = (int64_t)a.E * S;
     const R g = R(a.gamma), mu = R(vp.mu), kappa = R(vp.kappa), eta = R(vp.eta);
@@ -1008,6 +1007,22 @@ __global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, cons
     }
 }
 
+template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
+__global__ __launch_bounds__(NTHR) void visc_vjp_kernel(const VjpArgs<R> a, const OpTables<R> ops,
+                                                         const ViscParams vp) {
+    visc_vjp_body<R, P, EPB, MODE, NTHR>(a, ops, vp, tile_of_block(a.xcd_groups));
+}
+
+// The BR2 stage VJP over the tiles of several meshes of one order in one
+// launch: one work item (member, tile) per workgroup, each member with its own
+// arguments, operator tables and viscous parameters.
+template <typename R, int P, int EPB, int MODE, int NTHR = visc_threads<P, EPB>()>
+__global__ __launch_bounds__(NTHR) void batched_visc_vjp_kernel(const BatchVjpInst<R>* __restrict__ insts,
+                                                                 const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    visc_vjp_body<R, P, EPB, MODE, NTHR>(insts[it.x].a, insts[it.x].ops, insts[it.x].vp, it.y);
+}
+
 // Completion of a BR2 cotangent: U_bar plus the transposed traces of U and
 // grad U of the cross-tile edge cotangents the last adjoint launch left.
 template <typename R, int P, int EPB>
@@ -1107,6 +1122,18 @@ hipError_t visc_vjp_p(int mode, const VjpArgs<R>& a, const OpTables<R>& ops, con
 }
 
 template <typename R, int P>
+hipError_t batched_visc_vjp_p(int mode, const BatchVjpInst<R>* insts, const int2* items, int n_items, hipStream_t s) {
+    constexpr int EPB = visc_adj_epb<R, P>(), NTHR = visc_threads<P, EPB>();
+    switch (mode) {
+    case MODE_STAGE1: batched_visc_vjp_kernel<R, P, EPB, MODE_STAGE1><<<n_items, NTHR, 0, s>>>(insts, items); break;
+    case MODE_STAGE2: batched_visc_vjp_kernel<R, P, EPB, MODE_STAGE2><<<n_items, NTHR, 0, s>>>(insts, items); break;
+    case MODE_STAGE3: batched_visc_vjp_kernel<R, P, EPB, MODE_STAGE3><<<n_items, NTHR, 0, s>>>(insts, items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int P>
 hipError_t visc_complete_p(const CompleteArgs<R>& a, const R* Jinv, const OpTables<R>& ops, hipStream_t s) {
     constexpr int EPB = visc_adj_epb<R, P>();
     const int grid = (std::max(a.E, 1) + EPB - 1) / EPB;
@@ -1180,6 +1207,15 @@ hipError_t launch_visc_vjp(int P, int mode, const VjpArgs<R>& a, const OpTables<
 }
 
 template <typename R>
+hipError_t launch_batched_visc_vjp(int P, int mode, const BatchVjpInst<R>* insts, const int2* items, int n_items,
+                                   hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+#define ZJ_C(PP) batched_visc_vjp_p<R, PP>(mode, insts, items, n_items, s)
+    ZJ_VISC_P(ZJ_C)
+#undef ZJ_C
+}
+
+template <typename R>
 hipError_t launch_visc_complete(int P, const CompleteArgs<R>& a, const R* Jinv_e, const OpTables<R>& ops,
                                 hipStream_t s) {
     if (a.E <= 0) return hipSuccess;
@@ -1197,6 +1233,10 @@ template hipError_t launch_visc_vjp<double>(int, int, const VjpArgs<double>&, co
                                             const ViscParams&, hipStream_t);
 template hipError_t launch_visc_vjp<float>(int, int, const VjpArgs<float>&, const OpTables<float>&,
                                            const ViscParams&, hipStream_t);
+template hipError_t launch_batched_visc_vjp<double>(int, int, const BatchVjpInst<double>*, const int2*, int,
+                                                    hipStream_t);
+template hipError_t launch_batched_visc_vjp<float>(int, int, const BatchVjpInst<float>*, const int2*, int,
+                                                   hipStream_t);
 template hipError_t launch_visc_complete<double>(int, const CompleteArgs<double>&, const double*,
                                                  const OpTables<double>&, hipStream_t);
 template hipError_t launch_visc_complete<float>(int, const CompleteArgs<float>&, const float*,

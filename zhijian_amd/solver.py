@@ -385,6 +385,19 @@ class Batch:
             raise RuntimeError("a member of the batch has been closed")
         self._chk(lib().zj_batch_rk3_step(self._h, d, stream))
 
+    def rk3_step_vjp(self, dts, stream=None, want_dt=False):
+        """One SSP-RK3 step backwards for every member (each holding its
+        state U^n and an adjoint seed); returns dJ/d(dts) per member when
+        want_dt, else None."""
+        d = np.ascontiguousarray(dts, np.float64).reshape(-1)
+        if d.size != len(self.members):
+            raise ValueError(f"{d.size} time steps for {len(self.members)} members")
+        if not all(getattr(m, "_h", None) and m._h.value for m in self.members):
+            raise RuntimeError("a member of the batch has been closed")
+        out = np.zeros(d.size) if want_dt else None
+        self._chk(lib().zj_batch_rk3_step_vjp(self._h, d, out.ctypes.data if want_dt else None, stream))
+        return out
+
     def synchronize(self):
         self._chk(lib().zj_batch_synchronize(self._h))
 
