@@ -240,6 +240,14 @@ int zj_adjoint_set_device(zj_handle h, const double* d_ubar_aos, void* stream);
 int zj_adjoint_get(zj_handle h, double* ubar_aos);
 int zj_adjoint_get_device(zj_handle h, double* d_ubar_aos, void* stream);
 int zj_rk3_step_vjp(zj_handle h, double dt, double* dt_bar, void* stream);
+/* Vector-Jacobian product of the CFL step (FRSolver::computeTimeStep,
+ * fr_solver.cpp:784-816, as zj_dt_cfl computes it): adds dt_bar times
+ * d(dt_cfl)/dU at the handle's state to the held adjoint cotangent.  The
+ * minimum's derivative is its minimising point's (the lowest point index
+ * among ties); every max() takes the branch the forward took.  For a
+ * trajectory stepped with dt_n = dt_cfl(U^n): after zj_rk3_step_vjp(dt_n,
+ * &dt_bar) (state U^n set), call zj_dt_cfl_vjp(cfl, dt_bar). */
+int zj_dt_cfl_vjp(zj_handle h, double cfl, double dt_bar, void* stream);
 /* Checkpointed gradient, both passes native (no reference counterpart).
  * zj_forward_checkpointed: n_steps SSP-RK3 steps from the handle's state with
  * dt = dts[n], U^{c*every} saved in state slot c (the slots grow to

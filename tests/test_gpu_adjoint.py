@@ -190,3 +190,79 @@ def test_adjoint_c5_full_size(precision):
     rd = abs(dtb - dfd) / abs(dfd)
     say(f"  dJ/ddt: adjoint {dtb:.12e}  FD {dfd:.12e}  rel {rd:.2e}")
     assert rd < tol
+
+
+def test_dt_cfl_vjp():
+    """The CFL step's VJP (zj_dt_cfl_vjp): one point's four variables carry
+    the derivative of the minimum; against central differences of the
+    oracle's CFL step (fr_solver.cpp:784-816 restated) in three directions."""
+    say("case: CFL step VJP, jittered 6x5 P3")
+    mesh = cases.uniform_quad_mesh(6, 5, 0.0, 1.0, 0.0, 1.0, jitter=0.12, seed=21)
+    s, oc, t = make(mesh, 3)
+    U = tgv_state(t, amp=0.05, seed=22, mach=0.3)
+    o = oc.solver()
+    s.set_state(U)
+    s.adjoint_set(np.zeros_like(U))
+    s.dt_cfl_vjp(0.5, 1.0)
+    g = s.adjoint_get()
+    assert np.count_nonzero(g) == 4
+    for k in range(3):
+        v = direction(U, 23 + k)
+        lhs = (o.dt_cfl(U + EPS * v, 0.5) - o.dt_cfl(U - EPS * v, 0.5)) / (2 * EPS)
+        assert dot_check(lhs, float(g @ v), "dt_cfl") < TOL
+
+
+def test_dt_cfl_vjp_viscous():
+    """With the viscous terms the step is the minimum of
+    cfl / ((2p+1) ws / h + (p+1)^4 nu_k / (rho h^2)); against central
+    differences of the handle's own zj_dt_cfl (pinned by test_viscous_dt_rule)."""
+    say("case: CFL step VJP with the viscous rate, TGV 6x6 P2 Re 20")
+    mesh = cases.tgv_mesh(6)
+    t = solver.build_tables(mesh.xy, mesh.elem_nodes, 2, periodic=(mesh.Lx, mesh.Ly))
+    s = solver.Solver(t, compat="textbook", riemann="roe", corr="hu", positivity=False, viscous=True, Re=20.0)
+    U = tgv_state(t, amp=0.05, seed=27, mach=0.3)
+    s.set_state(U)
+    s.adjoint_set(np.zeros_like(U))
+    s.dt_cfl_vjp(0.5, 2.5)
+    g = s.adjoint_get()
+
+    def dt_of(X):
+        s.set_state(X)
+        return s.dt_cfl(0.5)
+
+    for k in range(2):
+        v = direction(U, 28 + k)
+        lhs = 2.5 * (dt_of(U + EPS * v) - dt_of(U - EPS * v)) / (2 * EPS)
+        assert dot_check(lhs, float(g @ v), "dt_cfl viscous") < TOL
+
+
+def test_cfl_trajectory_gradient():
+    """Three steps with dt_n = dt_cfl(U^n): J = <w, U^3>; the reverse sweep
+    chains each step's dJ/d(dt_n) (zj_rk3_step_vjp) into the CFL step's VJP
+    (zj_dt_cfl_vjp); against a central difference of the oracle's map with
+    the step recomputed from each state."""
+    say("case: gradient through 3 CFL-stepped steps, TGV 6x6 P3")
+    mesh = cases.tgv_mesh(6)
+    s, oc, t = make(mesh, 3)
+    U0 = tgv_state(t, amp=0.02, seed=31, mach=0.3)
+    o = oc.solver()
+    steps = 3
+
+    def run(X):
+        for _ in range(steps):
+            X = o.advance(X, o.dt_cfl(X, 0.5), True)
+        return X
+
+    traj, dts = [U0], []
+    for _ in range(steps):
+        dts.append(o.dt_cfl(traj[-1], 0.5))
+        traj.append(o.advance(traj[-1], dts[-1], True))
+    w = np.random.default_rng(32).standard_normal(U0.shape)
+    v = direction(U0, 33)
+    lhs = float(w @ fd(run, U0, v))
+    s.adjoint_set(w)
+    for n in range(steps - 1, -1, -1):
+        s.set_state(traj[n])
+        dtb = s.rk3_step_vjp(dts[n])
+        s.dt_cfl_vjp(0.5, dtb)
+    assert dot_check(lhs, float(s.adjoint_get() @ v), "3 CFL steps") < TOL

@@ -76,7 +76,7 @@ EXPORTS = ("zj_create", "zj_destroy", "zj_last_error", "zj_version", "zj_set_sta
            "zj_face_normals", "zj_elem_areas", "zj_geometry", "zj_copy_probe", "zj_set_boundary",
            "zj_rk3_combine", "zj_euler_update", "zj_synchronize", "zj_set_geometry_mode",
            "zj_set_xcd_groups", "zj_adjoint_set", "zj_adjoint_set_device", "zj_adjoint_get",
-           "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
+           "zj_adjoint_get_device", "zj_rk3_step_vjp", "zj_dt_cfl_vjp", "zj_residual_vjp", "zj_residual_vjp_device",
            "zj_residual_device", "zj_batch_create", "zj_batch_destroy", "zj_batch_last_error",
            "zj_batch_rk3_step", "zj_batch_rk3_step_vjp", "zj_batch_synchronize", "zj_batch_set_grid", "zj_autotune",
            "zj_batch_set_launch", "zj_batch_autotune", "zj_state_slots", "zj_state_save", "zj_state_load",
@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
         L.zj_adjoint_get.argtypes = [H, _dp]
         L.zj_adjoint_get_device.argtypes = [H, C.c_void_p, C.c_void_p]
         L.zj_rk3_step_vjp.argtypes = [H, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+        L.zj_dt_cfl_vjp.argtypes = [H, C.c_double, C.c_double, C.c_void_p]
         L.zj_residual_vjp.argtypes = [H, _dp, _dp]
         L.zj_residual_vjp_device.argtypes = [H, C.c_void_p, C.c_void_p, C.c_void_p]
         L.zj_residual_device.argtypes = [H, C.c_void_p, C.c_void_p]

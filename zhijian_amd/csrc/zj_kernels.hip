@@ -466,27 +466,91 @@ __global__ void set_many_kernel(double* dst, ScalarChunk c) {
     if (threadIdx.x < c.n) dst[threadIdx.x] = c.v[threadIdx.x];
 }
 
-// FRSolver::computeTimeStep per point (fr_solver.cpp:799-811, euler.hpp:34-41,166-171).
+// FRSolver::computeTimeStep at one point (fr_solver.cpp:799-811,
+// euler.hpp:34-41,166-171); with the viscous terms their rate
+// (p+1)^4 nu_k / h^2 adds to the convective one (visc > 0).
+__device__ __forceinline__ double cfl_dt_point(double rho, double ru, double rv, double rE, double h, double gamma,
+                                               double cfl, double cf, double visc) {
+#pragma clang fp contract(off)
+    const double u = ru / rho, v = rv / rho;
+    const double p = (gamma - 1.0) * rho * (rE / rho - 0.5 * (u * u + v * v));
+    const double ws = sqrt(u * u + v * v) + sqrt(gamma * fmax(p, 1e-10) / fmax(rho, 1e-10));
+    return visc == 0.0 ? cfl * h / (ws * cf) : cfl / (cf * ws / h + visc / (fmax(rho, 1e-10) * h * h));
+}
+
 __global__ __launch_bounds__(BLOCK) void dt_partial_kernel(const double* __restrict__ U,
                                                            const double* __restrict__ hcell,
                                                            int E, int S, double gamma, double cfl,
                                                            double cf, double visc, double* __restrict__ partial) {
-#pragma clang fp contract(off)
     __shared__ double sRed[BLOCK / 64];
     const int64_t n = (int64_t)E * S;
     double m = 1e20;
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        const double rho = U[i], ru = U[n + i], rv = U[2 * n + i], rE = U[3 * n + i];
-        const double u = ru / rho, v = rv / rho;
-        const double p = (gamma - 1.0) * rho * (rE / rho - 0.5 * (u * u + v * v));
-        const double ws = sqrt(u * u + v * v) + sqrt(gamma * fmax(p, 1e-10) / fmax(rho, 1e-10));
-        const double h = hcell[i / S];
-        // with the viscous terms, their rate (p+1)^4 nu_k / h^2 adds to the convective one
-        const double d = visc == 0.0 ? cfl * h / (ws * cf) : cfl / (cf * ws / h + visc / (fmax(rho, 1e-10) * h * h));
+        const double d = cfl_dt_point(U[i], U[n + i], U[2 * n + i], U[3 * n + i], hcell[i / S], gamma, cfl, cf, visc);
         m = d < m ? d : m;
     }
     const double r = block_reduce<OpMin>(m, sRed);
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+// The point whose CFL step is the minimum *dtmin (the lowest flat index among
+// ties): the minimum's derivative is that point's.
+__global__ __launch_bounds__(BLOCK) void dt_argmin_kernel(const double* __restrict__ U,
+                                                          const double* __restrict__ hcell, int E, int S,
+                                                          double gamma, double cfl, double cf, double visc,
+                                                          const double* __restrict__ dtmin,
+                                                          unsigned long long* __restrict__ idx) {
+    const int64_t n = (int64_t)E * S;
+    const double m = *dtmin;
+    unsigned long long best = ~0ull;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const double d = cfl_dt_point(U[i], U[n + i], U[2 * n + i], U[3 * n + i], hcell[i / S], gamma, cfl, cf, visc);
+        if (d == m && (unsigned long long)i < best) best = (unsigned long long)i;
+    }
+    if (best != ~0ull) atomicMin(idx, best);
+}
+
+// Ub += dt_bar * d(dt)/dU at the minimising point *idx: the derivative of
+// cfl_dt_point through u, v, p, the sound speed and (viscous) the density
+// floor, each max() taking the branch the forward took.
+__global__ void dt_cfl_vjp_kernel(const double* __restrict__ U, const double* __restrict__ hcell, int E, int S,
+                                  double gamma, double cfl, double cf, double visc,
+                                  const unsigned long long* __restrict__ idx, double dt_bar, double* __restrict__ Ub) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long i = *idx;
+    const int64_t n = (int64_t)E * S;
+    if (i >= (unsigned long long)n) return;
+    const double rho = U[i], ru = U[n + i], rv = U[2 * n + i], rE = U[3 * n + i], h = hcell[i / S];
+    const double gm = gamma - 1.0;
+    const double u = ru / rho, v = rv / rho, q2 = u * u + v * v, sp = sqrt(q2);
+    const double p = gm * rho * (rE / rho - 0.5 * q2);
+    const double pp = fmax(p, 1e-10), rr = fmax(rho, 1e-10);
+    const double c = sqrt(gamma * pp / rr), ws = sp + c;
+    // speed |u|: d/du = u / |u| (0 at rest)
+    const double su = sp > 0.0 ? u / sp : 0.0, sv = sp > 0.0 ? v / sp : 0.0;
+    const double cp = p > 1e-10 ? gamma / (2.0 * c * rr) : 0.0;     // dc/dp
+    const double cr = rho > 1e-10 ? -c / (2.0 * rr) : 0.0;           // dc/drho through the floor
+    // p = (g-1)(rE - (ru^2 + rv^2) / (2 rho)); u = ru / rho, v = rv / rho
+    const double ws_r = -(su * u + sv * v) / rho + cp * gm * 0.5 * q2 + cr;
+    const double ws_ru = su / rho - cp * gm * u;
+    const double ws_rv = sv / rho - cp * gm * v;
+    const double ws_rE = cp * gm;
+    double dws, drr = 0.0;
+    if (visc == 0.0) {
+        dws = -cfl * h / (ws * ws * cf);
+    } else {
+        const double D = cf * ws / h + visc / (rr * h * h);
+        dws = -cfl * cf / (D * D * h);
+        drr = rho > 1e-10 ? cfl * visc / (D * D * rr * rr * h * h) : 0.0;
+    }
+    Ub[i] += dt_bar * (dws * ws_r + drr);
+    Ub[n + i] += dt_bar * dws * ws_ru;
+    Ub[2 * n + i] += dt_bar * dws * ws_rv;
+    Ub[3 * n + i] += dt_bar * dws * ws_rE;
 }
 
 __global__ __launch_bounds__(BLOCK) void nonfinite_kernel(const double* __restrict__ U, int64_t n,
@@ -732,6 +796,16 @@ hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt,
 hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma, double cfl,
                              double cf, double visc, double* partial, int nblk, hipStream_t s) {
     dt_partial_kernel<<<dim3(nblk), dim3(BLOCK), 0, s>>>(U, hcell, E, S, gamma, cfl, cf, visc, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_cfl_vjp(const double* U, const double* hcell, int E, int S, double gamma, double cfl,
+                             double cf, double visc, const double* dtmin, unsigned long long* idx, double dt_bar,
+                             double* Ub, int nblk, hipStream_t s) {
+    fill_u64_kernel<<<1, 64, 0, s>>>(idx, ~0ull);
+    dt_argmin_kernel<<<dim3((unsigned)std::max(nblk, 1)), dim3(BLOCK), 0, s>>>(U, hcell, E, S, gamma, cfl, cf, visc,
+                                                                               dtmin, idx);
+    dt_cfl_vjp_kernel<<<1, 64, 0, s>>>(U, hcell, E, S, gamma, cfl, cf, visc, idx, dt_bar, Ub);
     return hipGetLastError();
 }
 

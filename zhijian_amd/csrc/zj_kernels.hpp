@@ -214,6 +214,12 @@ hipError_t launch_euler_update(double* U, const double* R, int64_t n, double dt,
 // cfl / (cf*ws/h_e + visc/(rho h_e^2)); partial per block.
 hipError_t launch_dt_partial(const double* U, const double* hcell, int E, int S, double gamma,
                              double cfl, double cf, double visc, double* partial, int nblk, hipStream_t s);
+// Ub += dt_bar * d(dt)/dU, dt the minimum above (*dtmin, already on the
+// device): the derivative at the minimising point (lowest index among ties),
+// found into *idx.
+hipError_t launch_dt_cfl_vjp(const double* U, const double* hcell, int E, int S, double gamma, double cfl,
+                             double cf, double visc, const double* dtmin, unsigned long long* idx, double dt_bar,
+                             double* Ub, int nblk, hipStream_t s);
 hipError_t launch_count_nonfinite(const double* U, int64_t n, unsigned long long* count,
                                   int nblk, hipStream_t s);
 // Sum over the points of w (m_x^2 + m_y^2) / (2 rho) as nblk partials, and its

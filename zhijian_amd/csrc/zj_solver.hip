@@ -93,6 +93,7 @@ struct zj_solver {
     void* A_Tb[2] = {nullptr, nullptr};  // cross-tile edge cotangents, ping-pong
     double* A_dtb = nullptr;       // per-tile dt cotangents of the three stages
     double* A_mub = nullptr;       // viscous terms: per-tile <F_v cotangent, F_v>, summed since the seed
+    unsigned long long* A_idx = nullptr;  // the point that sets the CFL step (zj_dt_cfl_vjp)
     double* slots = nullptr;       // device copies of states (SoA planes), zj_state_slots
     int n_slots = 0;
     // tape of the native reverse sweep: per step of a segment, U^n, U1, U2
@@ -226,7 +227,7 @@ static void release(zj_solver* h) {
     if (h->graph) hipGraphDestroy(h->graph);
     void* bufs[] = {h->perm, h->edge_f_aff, h->edge_f_pt, h->Jpt, h->U, h->W, h->Rres, h->aos, h->Jinv, h->Jinv_e, h->T[0], h->T[1], h->T[2], h->links,
                     h->normals, h->bc_data, h->hcell, h->d_scalars, h->partial, h->partial_dt, h->d_count,
-                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb, h->A_mub, h->slots, h->tape, h->d_dtb_steps};
+                    h->A_W1, h->A_W2, h->A_Ub, h->A_U0b, h->A_Tb[0], h->A_Tb[1], h->A_dtb, h->A_mub, h->A_idx, h->slots, h->tape, h->d_dtb_steps};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -771,6 +772,28 @@ extern "C" int zj_dt_cfl(zj_handle h, double cfl, double* dt) {
     return ZJ_OK;
 }
 
+// The CFL step's vector-Jacobian product: the held adjoint cotangent gains
+// dt_bar * d(dt_cfl)/dU at the held state (the derivative of the minimum is
+// that of its minimising point, the lowest index among ties).
+extern "C" int zj_dt_cfl_vjp(zj_handle h, double cfl, double dt_bar, void* stream) {
+    if (!h) return ZJ_ERR_ARG;
+    if (!h->has_state) return fail(h, ZJ_ERR_STATE, "no state set");
+    if (!h->has_area) return fail(h, ZJ_ERR_STATE, "element areas were not provided");
+    if (h->stage_next != 1) return fail(h, ZJ_ERR_STATE, "a partial step is in progress");
+    if (!h->A_seeded) return fail(h, ZJ_ERR_STATE, "no adjoint seed set");
+    ZJ_HIP(h, hipSetDevice(h->device));
+    hipStream_t s = pick(h, stream);
+    const double p1 = h->P + 1.0;
+    const double visc =
+        h->viscous ? p1 * p1 * p1 * p1 * std::max(h->vp.mu * 4.0 / 3.0, h->vp.kappa * (h->gamma - 1.0)) : 0.0;
+    const double cf = 2.0 * h->P + 1.0;
+    ZJ_HIP(h, launch_dt_partial(h->U, h->hcell, h->E, h->S, h->gamma, cfl, cf, visc, h->partial_dt, h->nred, s));
+    ZJ_HIP(h, launch_finalize(h->partial_dt, h->nred, FIN_MIN, h->d_scalars + 2, s));
+    ZJ_HIP(h, launch_dt_cfl_vjp(h->U, h->hcell, h->E, h->S, h->gamma, cfl, cf, visc, h->d_scalars + 2, h->A_idx,
+                                dt_bar, h->A_Ub, h->nred, s));
+    return ZJ_OK;
+}
+
 extern "C" int zj_residual_l2(zj_handle h, double* l2) {
     if (!h || !l2) return ZJ_ERR_ARG;
     if (h->l2_blocks == 0) return fail(h, ZJ_ERR_STATE, "no residual has been evaluated");
@@ -1017,6 +1040,7 @@ static int ensure_adjoint(zj_solver* h) {
     for (int k = 0; k < 2; ++k) ZJ_HIP(h, hipMalloc(&h->A_Tb[k], ntr * rsz));
     ZJ_HIP(h, hipMalloc((void**)&h->A_dtb, (size_t)3 * h->adj_nt * sizeof(double)));
     if (h->viscous) ZJ_HIP(h, hipMalloc((void**)&h->A_mub, (size_t)h->adj_nt * sizeof(double)));
+    ZJ_HIP(h, hipMalloc((void**)&h->A_idx, sizeof(unsigned long long)));
     h->adj_ready = true;
     return ZJ_OK;
 }

@@ -319,6 +319,11 @@ class Solver:
         self._chk(lib().zj_adjoint_get_re(self._h, C.byref(v)))
         return v.value
 
+    def dt_cfl_vjp(self, cfl, dt_bar, stream=None):
+        """Adds dt_bar * d(dt_cfl)/dU at the held state to the held cotangent
+        (zj_dt_cfl_vjp)."""
+        self._chk(lib().zj_dt_cfl_vjp(self._h, float(cfl), float(dt_bar), stream))
+
     def rk3_step_vjp(self, dt, stream=None, want_dt=True):
         """Maps the held cotangent of U^{n+1} to that of U^n (the state set on
         the handle); returns dJ/d(dt) (synchronising), or None with
