@@ -428,8 +428,8 @@ int zj_tri_lift(int p, int corr, double* lift);
  * (rk_integrator.hpp:33-58); on handles with the viscous terms also the
  * BR2 terms backwards (the viscous flux VJP, navierstokes.hpp:38-99, at the
  * points and face points, the corrected gradient's and face liftings' jumps,
- * the transposed gradient), for FP64 P1-P3 and FP32-bulk P1-P5 (other
- * orders return ZJ_ERR_UNSUPPORTED).  Reverse stepping: zj_tri_adjoint_set(dJ/dU^N);
+ * the transposed gradient), at every order such a handle steps.  Reverse
+ * stepping: zj_tri_adjoint_set(dJ/dU^N);
  * for n = N-1 .. 0: zj_tri_set_state(U^n), zj_tri_rk3_step_vjp(dt_n)
  * (stages recomputed from U^n; dt_bar, may be NULL, receives dJ/d(dt_n));
  * then zj_tri_adjoint_get.  Arrays are [E][Np][4]. */

@@ -21,7 +21,6 @@ from oracle import tri_oracle as T
 from tests.test_gpu_adjoint import TOL, direction, dot_check, fd
 from tests.test_gpu_parity import say
 from zhijian_amd import cases
-from zhijian_amd._lib import ZJError
 from zhijian_amd.tri import TriSolver, sp_coords
 
 pytestmark = pytest.mark.gpu
@@ -132,12 +131,18 @@ def test_tri_viscous_vjp_numbering_independent():
     assert r < 1e-12
 
 
-def test_tri_viscous_vjp_fp64_p5_refused():
-    """FP64 P5 with the viscous terms steps forward, but its adjoint's tile
-    does not fit the LDS: the adjoint calls refuse with an error."""
-    say("case: triangle viscous FP64 P5 adjoint refusal")
-    s, _, m = _pair(4, 5, "roe", seed=5)
+def test_tri_viscous_vjp_fp64_p5():
+    """FP64 P5: the adjoint's 16-element tile takes 91 KiB of LDS (one
+    workgroup per CU); residual and one step against the oracle."""
+    say("case: triangle viscous FP64 P5 residual and step VJP, periodic 6x6 roe/dg Re 100")
+    s, o, m = _pair(6, 5, "roe", seed=5)
     U = _state(m, 5, 160)
     s.set_state(U)
-    with pytest.raises(ZJError, match="FP64 P1-P3"):
-        s.residual_vjp(np.ones_like(U))
+    w = np.random.default_rng(161).standard_normal(U.shape)
+    v = direction(U, 162)
+    assert dot_check(float(w @ fd(o.residual, U, v)), float(s.residual_vjp(w) @ v), "residual") < TOL
+    dt = 5e-4
+    lhs = float(w @ fd(lambda X: o.advance(X, dt), U, v))
+    s.adjoint_set(w)
+    s.rk3_step_vjp(dt, want_dt=False)
+    assert dot_check(lhs, float(s.adjoint_get() @ v), "step") < TOL

@@ -1288,7 +1288,9 @@ template <typename R, int P> struct TriViscVjpTile {
     static constexpr int words = 4 * NFP + 3 * NP + XW;    // per element and variable
     static constexpr int bytes(int tt) { return tt * words * NV * (int)sizeof(R); }
     static constexpr int value = bytes(64) <= 61440 ? 64 : (bytes(32) <= 61440 ? 32 : 16);
-    static constexpr bool fits = bytes(16) <= 61440;
+    // gfx950 gives a workgroup up to 160 KiB of LDS: FP64 P4 and P5 take 69
+    // and 91 KiB at 16 elements (one workgroup per CU)
+    static constexpr bool fits = bytes(16) <= 122880;
 };
 
 template <typename R, int P, int MODE, int TT>
@@ -2135,8 +2137,8 @@ hipError_t tri_complete(int P, int E, const int2* links, const double* Ub, const
     return hipGetLastError();
 }
 
-// The BR2 adjoint's elements per workgroup for an order and precision; 0
-// where its LDS arrays exceed 60 KB at 16 elements (FP64 P4, P5).
+// The BR2 adjoint's elements per workgroup for an order and precision (0
+// where its LDS arrays would exceed 120 KiB at 16 elements: none at P1-P5).
 int tri_visc_vjp_tile(int P, bool fp64) {
     switch (P) {
 #define ZJ_TV(PP)                                                                                           \
@@ -2809,9 +2811,7 @@ static int tensure_adjoint(zj_tri* h) {
     int tt = TILE;
     if (h->viscous) {
         tt = tri_visc_vjp_tile(h->P, h->prec == ZJ_PREC_FP64);
-        if (tt == 0)
-            return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint with the viscous terms covers FP64 P1-P3 "
-                                                "and FP32-bulk P1-P5 (its tile's LDS arrays)");
+        if (tt == 0) return tfail(h, ZJ_ERR_UNSUPPORTED, "the triangle adjoint's LDS arrays exceed the workgroup's");
     }
     ZJ_THIP(h, hipSetDevice(h->device));
     h->A_blk = (h->E + tt - 1) / tt;
