@@ -537,7 +537,9 @@ template <typename R, int P>
 constexpr int visc_adj_epb() {
     using Y = Lay<R, P>;
     constexpr int full = BLOCK / ((P + 1) * (P + 1));
-    constexpr int cap = 65536 / ((6 * Y::UE + 9 * Y::TE) * (int)sizeof(R));  // LDS within 64 KiB
+    // LDS within 120 KiB (gfx950 gives a workgroup up to 160 KiB): every
+    // order's full tile at both precisions
+    constexpr int cap = 122880 / ((6 * Y::UE + 9 * Y::TE) * (int)sizeof(R));
     return cap < 1 ? 1 : (cap < full ? cap : full);
 }
 
