@@ -422,8 +422,9 @@ constexpr int TWV = 12;
 template <typename R, int P> struct TriViscTile {
     static constexpr int NP = (P + 1) * (P + 2) / 2, NFP = 3 * (P + 1);
     static constexpr int bytes(int tt) { return tt * (3 * NFP + 3 * NP) * NV * (int)sizeof(R); }
-    static constexpr int value = bytes(64) <= 61440 ? 64 : (bytes(32) <= 61440 ? 32 : 16);
-    static_assert(bytes(16) <= 61440, "the smallest tile exceeds the LDS");
+    // within 120 KiB of LDS (gfx950 gives a workgroup up to 160 KiB)
+    static constexpr int value = bytes(64) <= 122880 ? 64 : (bytes(32) <= 122880 ? 32 : 16);
+    static_assert(bytes(16) <= 122880, "the smallest tile exceeds the LDS");
 };
 
 template <typename R, int P, int MODE, int TT>
