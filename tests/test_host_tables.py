@@ -198,3 +198,28 @@ def test_batch_step_argument_checks_before_the_library():
     b.members[1]._h = C.c_void_p()
     with pytest.raises(RuntimeError, match="closed"):
         b.rk3_step(np.zeros(2))
+
+
+def test_adjoint_entry_points_reject_null_handles():
+    """The step-backwards entry points added for batches, the CFL step and
+    triangles return ZJ_ERR_ARG on a null handle or null arrays, before any
+    device work (no GPU needed); Batch.rk3_step_vjp checks its dts like
+    rk3_step."""
+    L = _lib.lib()
+    dts = np.zeros(2)
+    assert L.zj_batch_rk3_step_vjp(None, dts, None, None) == 1
+    assert L.zj_dt_cfl_vjp(None, 0.5, 1.0, None) == 1
+    assert L.zj_tri_rk3_step_vjp(None, 1e-3, None, None) == 1
+    assert L.zj_tri_residual_vjp(None, dts, dts) == 1
+
+    class _Member:
+        def __init__(self, v):
+            self._h = C.c_void_p(v)
+    b = solver.Batch.__new__(solver.Batch)
+    b.members = [_Member(1), _Member(2)]
+    b._h = C.c_void_p()
+    with pytest.raises(ValueError, match="2 members"):
+        b.rk3_step_vjp(np.zeros(3))
+    b.members[0]._h = C.c_void_p()
+    with pytest.raises(RuntimeError, match="closed"):
+        b.rk3_step_vjp(np.zeros(2))
