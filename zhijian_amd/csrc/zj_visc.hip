@@ -136,11 +136,18 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     // they are read: with the two extra arrays its LDS caps it at two
     // workgroups per CU and the extra phase measured 2.5% slower at P4.
     constexpr bool PRE = std::is_same<R, float>::value;
+    // With one face point per thread (NF <= NTHR) the face phase B2a reads
+    // all its inputs before a barrier and then writes, so the jumps and the
+    // common fluxes can take the gradient traces' place in LDS (FP32 P3-P5:
+    // 22 instead of 29 KB at P4, seven workgroups per CU instead of five).
+    constexpr bool ALIAS = PRE && NF <= NTHR;
     R* const sTr = sm.Tr();
-    R* const sTGx = PRE ? sm.TGx() : nullptr;
-    R* const sTGy = PRE ? sm.TGy() : nullptr;
-    R* const sJ = sm.J();
-    R* const sFI = sm.FI();
+    R* sTGx = nullptr;
+    R* sTGy = nullptr;
+    if constexpr (PRE) { sTGx = sm.TGx(); sTGy = sm.TGy(); }
+    R* sJ;
+    R* sFI;
+    if constexpr (ALIAS) { sJ = sTGx; sFI = sTGy; } else { sJ = sm.J(); sFI = sm.FI(); }
     R* const sD = sm.D();
     R* const sLm = sm.Lm();
     R* const sLp = sm.Lp();
@@ -213,58 +220,69 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     // ---- B2a: jumps and the common viscous normal flux per (edge, flux point)
     // (kept apart from the Riemann solve so that the two never hold their
     // registers at once)
-    for (int t = tid; t < NF; t += NTHR) {
+    for (int t0 = 0; t0 < NF; t0 += NTHR) {
+        const int t = t0 + tid;
         const int el2 = t / (4 * FP), r = t - el2 * (4 * FP), k = r / FP, f = r - k * FP;
         const int e2 = e0 + el2;
-        if (e2 >= a.E) continue;
-        const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
-        const R nx = a.normals[((int64_t)e2 * 4 + k) * 2], ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
-        const int nl = L.nbr - e0;
-        const bool inside = nl >= 0 && nl < EPB;
-        if (f == 0) sOut[el2 * 4 + k] = !inside;
-        const int oslot = P - f;
-        const TraceLine tl = trace_line<P>(k, f);
-        const TraceLine tn = trace_line<P>(L.nbr_edge, oslot);
-        const R* T = a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
-        // BR2 gradients: own and neighbour, each with its own face's lifting
-        // (this item's element e2 is not the thread's own element)
-        const R* Jo = a.Jinv + (int64_t)e2 * 4;
-        const R* Jn = a.Jinv + (int64_t)L.nbr * 4;
-        const bool oxi = (k & 1) != 0, nxi = (L.nbr_edge & 1) != 0;
-        const R go = eta * ((k == 0 || k == 3) ? gLe : gRe);
-        const R gn = eta * ((L.nbr_edge == 0 || L.nbr_edge == 3) ? gLe : gRe);
-        const R ax = go * (oxi ? Jo[0] : Jo[2]), ay = go * (oxi ? Jo[1] : Jo[3]);
-        const R bx = gn * (nxi ? Jn[0] : Jn[2]), by = gn * (nxi ? Jn[1] : Jn[3]);
-        R qx[NV], qy[NV], Um[NV], Fv[NV], Gv[NV];
+        const bool item = t < NF && e2 < a.E;
+        R qx[NV], qy[NV], Um[NV], jw[NV];
+        R nx = R(0), ny = R(0);
+        if (item) {
+            const EdgeLink L = a.links[(int64_t)e2 * 4 + k];
+            nx = a.normals[((int64_t)e2 * 4 + k) * 2];
+            ny = a.normals[((int64_t)e2 * 4 + k) * 2 + 1];
+            const int nl = L.nbr - e0;
+            const bool inside = nl >= 0 && nl < EPB;
+            if (f == 0) sOut[el2 * 4 + k] = !inside;
+            const int oslot = P - f;
+            const TraceLine tl = trace_line<P>(k, f);
+            const TraceLine tn = trace_line<P>(L.nbr_edge, oslot);
+            const R* T = a.T_in + (((int64_t)L.nbr * 4 + L.nbr_edge) * FP + oslot) * TW;
+            // BR2 gradients: own and neighbour, each with its own face's lifting
+            // (this item's element e2 is not the thread's own element)
+            const R* Jo = a.Jinv + (int64_t)e2 * 4;
+            const R* Jn = a.Jinv + (int64_t)L.nbr * 4;
+            const bool oxi = (k & 1) != 0, nxi = (L.nbr_edge & 1) != 0;
+            const R go = eta * ((k == 0 || k == 3) ? gLe : gRe);
+            const R gn = eta * ((L.nbr_edge == 0 || L.nbr_edge == 3) ? gLe : gRe);
+            const R ax = go * (oxi ? Jo[0] : Jo[2]), ay = go * (oxi ? Jo[1] : Jo[3]);
+            const R bx = gn * (nxi ? Jn[0] : Jn[2]), by = gn * (nxi ? Jn[1] : Jn[3]);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const R own = sTr[Y::tr(el2, k, f, v)];
-            R oth, gxn, gyn;
-            if (inside) {
-                oth = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
-                gxn = PRE ? sTGx[Y::tr(nl, L.nbr_edge, oslot, v)]
-                          : trace_point<R, P>(sGx + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
-                gyn = PRE ? sTGy[Y::tr(nl, L.nbr_edge, oslot, v)]
-                          : trace_point<R, P>(sGy + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
-            } else {
-                oth = T[v];
-                gxn = T[NV + v];
-                gyn = T[2 * NV + v];
+            for (int v = 0; v < NV; ++v) {
+                const R own = sTr[Y::tr(el2, k, f, v)];
+                R oth, gxn, gyn;
+                if (inside) {
+                    oth = sTr[Y::tr(nl, L.nbr_edge, oslot, v)];
+                    gxn = PRE ? sTGx[Y::tr(nl, L.nbr_edge, oslot, v)]
+                              : trace_point<R, P>(sGx + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+                    gyn = PRE ? sTGy[Y::tr(nl, L.nbr_edge, oslot, v)]
+                              : trace_point<R, P>(sGy + nl * Y::UE + v * Y::UV, tn, sLm, sLp);
+                } else {
+                    oth = T[v];
+                    gxn = T[NV + v];
+                    gyn = T[2 * NV + v];
+                }
+                const R gxo = PRE ? sTGx[Y::tr(el2, k, f, v)]
+                                  : trace_point<R, P>(sGx + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+                const R gyo = PRE ? sTGy[Y::tr(el2, k, f, v)]
+                                  : trace_point<R, P>(sGy + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
+                const R uc = R(0.5) * (own + oth);
+                const R jo = uc - own, jn = uc - oth;
+                jw[v] = jo;
+                qx[v] = R(0.5) * ((gxo + ax * jo) + (gxn + bx * jn));
+                qy[v] = R(0.5) * ((gyo + ay * jo) + (gyn + by * jn));
+                Um[v] = uc;
             }
-            const R gxo = PRE ? sTGx[Y::tr(el2, k, f, v)]
-                              : trace_point<R, P>(sGx + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
-            const R gyo = PRE ? sTGy[Y::tr(el2, k, f, v)]
-                              : trace_point<R, P>(sGy + el2 * Y::UE + v * Y::UV, tl, sLm, sLp);
-            const R uc = R(0.5) * (own + oth);
-            const R jo = uc - own, jn = uc - oth;
-            sJ[Y::tr(el2, k, f, v)] = jo;
-            qx[v] = R(0.5) * ((gxo + ax * jo) + (gxn + bx * jn));
-            qy[v] = R(0.5) * ((gyo + ay * jo) + (gyn + by * jn));
-            Um[v] = uc;
         }
-        visc_flux(Um, qx, qy, g, mu, kappa, Fv, Gv);
+        if constexpr (ALIAS) __syncthreads();  // every gradient trace read: the jumps and fluxes take their place
+        if (item) {
+            R Fv[NV], Gv[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) sFI[Y::tr(el2, k, f, v)] = Fv[v] * nx + Gv[v] * ny;  // as the left side sees it
+            for (int v = 0; v < NV; ++v) sJ[Y::tr(el2, k, f, v)] = jw[v];
+            visc_flux(Um, qx, qy, g, mu, kappa, Fv, Gv);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sFI[Y::tr(el2, k, f, v)] = Fv[v] * nx + Gv[v] * ny;  // as the left side sees it
+        }
     }
     __syncthreads();
 
