@@ -139,7 +139,8 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     // With one face point per thread (NF <= NTHR) the face phase B2a reads
     // all its inputs before a barrier and then writes, so the jumps and the
     // common fluxes can take the gradient traces' place in LDS (FP32 P3-P5:
-    // 22 instead of 29 KB at P4, seven workgroups per CU instead of five).
+    // 23 instead of 30 KB at P4, 31 instead of 41 KB at P3, so more
+    // workgroups fit on a CU).
     constexpr bool ALIAS = PRE && NF <= NTHR;
     R* const sTr = sm.Tr();
     R* sTGx = nullptr;
