@@ -66,7 +66,7 @@ template <typename R, int P, int EPB>
 struct ViscSmemDyn {
     using Y = Lay<R, P>;
     static constexpr int N1 = P + 1;
-    static constexpr bool PRE = std::is_same<R, float>::value;
+    static constexpr bool PRE = std::is_same<R, float>::value || EPB * 4 * (P + 1) <= BLOCK;  // as visc_body
     static constexpr int oU = 0, oGx = oU + EPB * Y::UE, oGy = oGx + EPB * Y::UE, oTr = oGy + EPB * Y::UE;
     static constexpr int oTGx = oTr + EPB * Y::TE, oTGy = oTGx + (PRE ? EPB * Y::TE : 0);
     static constexpr int oJ = oTGy + (PRE ? EPB * Y::TE : 0), oFI = oJ + EPB * Y::TE, oD = oFI + EPB * Y::TE;
@@ -131,11 +131,12 @@ __device__ __forceinline__ void visc_body(const StageArgs<R>& a, const OpTables<
     R* const sU = sm.U();
     R* const sGx = sm.Gx();
     R* const sGy = sm.Gy();
-    // FP32: the gradient traces once per face point, in LDS (fewer VALU
-    // instructions, 71 instead of 112 VGPRs at P4).  FP64 recomputes them where
-    // they are read: with the two extra arrays its LDS caps it at two
-    // workgroups per CU and the extra phase measured 2.5% slower at P4.
-    constexpr bool PRE = std::is_same<R, float>::value;
+    // The gradient traces once per face point, in LDS (fewer VALU
+    // instructions and registers: FP32 71 instead of 112 VGPRs at P4, FP64
+    // about 115 instead of 184).  FP64 with more face points than threads
+    // (P1, P2) recomputes them where they are read: there the two extra
+    // arrays would take its LDS past what the registers allow.
+    constexpr bool PRE = std::is_same<R, float>::value || NF <= NTHR;
     // With one face point per thread (NF <= NTHR) the face phase B2a reads
     // all its inputs before a barrier and then writes, so the jumps and the
     // common fluxes can take the gradient traces' place in LDS (FP32 P3-P5:
